@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark of the Flame3D finite-difference RHS + SSP-RK3 path on one MI355X.
+
+A step is one full SSP-RK3 step (three stages: sensor, split/reconstruct in three directions,
+viscous flux, divergence, RK blend, c2Prim, ghost fill) over the whole grid.  Workload: the
+24-degree compression ramp, 512x192x64 cells, FP64, Mach 2.9, Re_delta 6e4, tanh wall clustering
+(first cell 1e-3 delta), laminar boundary-layer inflow with seeded +-1% velocity noise, isothermal
+wall, periodic span.  Inputs (state, metrics, Jacobian) are resident in HBM before timing.
+
+Printed: one JSON line.  `value` is Mcell-updates/s of the FP64 form (interior cells x RK3 steps
+per second / 1e6).  Also reported: the FP32-bulk form on the same grid and its density and
+wall-pressure errors against the FP64 run from the same initial state, the % of HBM peak of the
+whole path (algorithmic bytes per cell-update x rate / 8 TB/s), the dominant kernel's roofline
+from hipEvent timing of its launches inside the timed region, and the CPU oracle timed on the host
+cores on a bounded sample (3 RK3 steps of the same grid).
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic (compulsory) HBM bytes per interior cell-update of a fully fused RK3 step:
+# stage 1 reads U + 9 metrics + J and writes U; stages 2-3 also read Un.
+ALG_BYTES_PER_CELL_UPDATE = {"f64": 560.0, "f32b": 452.0}
+
+
+def kernel_alg_bytes(name, p, es):
+    """Compulsory bytes of one launch of a library kernel (es = bytes of the bulk type)."""
+    nxt, nyt, nzt = p.tot
+    ncell = nxt * nyt * nzt
+    nint = p.Nxp * p.Nyp * p.Nzp
+    nface = (p.Nxp + 1) * p.Nyp * p.Nzp  # per direction, about the same for y/z
+    if name == "flux_split_sw":  # rho,u,v,w,T + 3 metrics in, Fp/Fm out, every cell
+        return ncell * (8 * es + 10 * es)
+    if name == "advect":  # Fp,Fm (10), u,v,w,T (4), 3 metrics, phi in; 5 fluxes out, per face
+        return nface * (10 * es + 4 * es + 3 * es + es + 5 * es)
+    if name == "viscous_flux":  # u,v,w,T + 9 metrics in (+J double), 12 fluxes out
+        return (nxt - 4) * (nyt - 4) * (nzt - 4) * (13 * es + 8 + 12 * es)
+    if name == "div":  # 15 face fluxes + 12 viscous fluxes in, J in, U read+write
+        return nint * (27 * es + 8 + 80)
+    if name == "lin_comb":
+        return nint * 120
+    if name == "c2prim":
+        return nint * (40 + 6 * es)
+    if name == "shock_sensor":
+        return ncell * 2 * es
+    if name in ("fill_ghost", "exchange_ghost"):
+        return (ncell - nint) * 2 * 6 * es
+    if name.startswith("fused_"):
+        return FUSED_BYTES.get(name, lambda p, es: 0)(p, es)
+    return 0
+
+
+FUSED_BYTES = {}
+
+
+def build_case(f3d, cases, torch, nx, ny, nz, seed=2):
+    import numpy as np
+    c = cases.ramp(nx, ny, nz, seed=seed)
+    p = f3d.make_params(nx, ny, nz, **c["params"])
+    ctx = f3d.Flame3D(p)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    m = [torch.zeros(p.cell_shape(), dtype=torch.float64, device="cuda:0") for _ in range(9)]
+    J = torch.zeros(p.cell_shape(), dtype=torch.float64, device="cuda:0")
+    ctx.metrics(d(c["x"]), d(c["y"]), d(c["z"]), m, J)
+    Q = d(c["Q"])
+    inflow = d(c["inflow"])
+    U = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
+    ctx.exchange_ghost(Q, 6)
+    ctx.fillGhost(Q, U, inflow)
+    ctx.prim2c(U, Q)
+    dt_t = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+    ctx.cfl_dt(dt_t, Q, m, J, 0.5)
+    torch.cuda.synchronize()
+    return dict(c=c, p=p, ctx=ctx, m=m, J=J, Q=Q, U=U, inflow=inflow, dt=float(dt_t.item()))
+
+
+def run_steps(torch, ctx, state, prec, steps, warmup, timing_lib=None):
+    """Warm up, then time `steps` RK3 steps between two synchronisations.  Returns ms/step."""
+    U, Un, Q, m, J, inflow, dt = state
+    for _ in range(warmup):
+        ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
+    torch.cuda.synchronize()
+    if timing_lib is not None:
+        timing_lib.f3d_timing_reset()
+        timing_lib.f3d_timing_enable(1)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if timing_lib is not None:
+        timing_lib.f3d_timing_enable(0)
+    return ev0.elapsed_time(ev1) / steps, wall / steps * 1e3
+
+
+def kernel_table(lib):
+    import ctypes
+    buf = ctypes.create_string_buffer(4096)
+    lib.f3d_timing_names(buf, 4096)
+    out = {}
+    for name in filter(None, buf.value.decode().split(",")):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        lib.f3d_timing_query(name.encode(), ctypes.byref(ms), ctypes.byref(n))
+        out[name] = (ms.value, n.value)
+    return out
+
+
+def cpu_baseline(f3d, cases, nx, ny, nz, steps=3):
+    """CPU oracle (C++ FP64 restatement, OpenMP) on the same grid for `steps` RK3 steps."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle, max_threads
+    c = cases.ramp(nx, ny, nz)
+    p = f3d.make_params(nx, ny, nz, **c["params"])
+    o = Oracle(p)
+    m = [np.zeros(p.cell_shape()) for _ in range(9)]
+    J = np.zeros(p.cell_shape())
+    o.metrics(c["x"], c["y"], c["z"], m, J)
+    Q = c["Q"].copy()
+    U = np.zeros(p.cell_shape(5))
+    o.exchange(Q, 6)
+    o.fill(Q, U, c["inflow"])
+    o.prim2c(U, Q)
+    Un = np.zeros_like(U)
+    dt = o.cfl_dt(Q, m, J, 0.5)
+    t0 = time.perf_counter()
+    o.time_step(steps, U, Un, Q, m, J, dt, c["inflow"])
+    el = time.perf_counter() - t0
+    return dict(value=round(nx * ny * nz * steps / el / 1e6, 4), unit="Mcell-updates/s", cores=max_threads(),
+                kind="port", sample=f"oracle FP64, {steps} RK3 steps of the {nx}x{ny}x{nz} ramp ({el:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", default="512x192x64")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f32b", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the ramp's whole subdomain on one MI355X); "
+              "--gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    f3d = importlib.import_module("flame3d-mhd_amd")
+    cases = importlib.import_module("flame3d-mhd_amd.cases")
+    lib = f3d.lib()
+    nx, ny, nz = map(int, args.grid.split("x"))
+    ncells = nx * ny * nz
+
+    st = build_case(f3d, cases, torch, nx, ny, nz)
+    p, ctx, dt = st["p"], st["ctx"], st["dt"]
+    U0 = st["U"].clone()
+    Q0 = st["Q"].clone()
+    # ---- FP64
+    U, Un, Q = st["U"], torch.zeros_like(st["U"]), st["Q"]
+    ms64, wall64 = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], dt), "f64", args.steps,
+                             args.warmup, timing_lib=lib)
+    ktab = kernel_table(lib)
+    ctx.nan_check(U)
+    value64 = ncells / (ms64 * 1e-3) / 1e6
+    rho64 = U[0].clone()
+    # ---- FP32-bulk from the same initial state, same dt and step count
+    out32 = {}
+    if not args.no_f32b:
+        m32 = [a.float().contiguous() for a in st["m"]]
+        U32, Un32, Q32 = U0.clone(), torch.zeros_like(U0), Q0.float().contiguous()
+        in32 = st["inflow"].float().contiguous()
+        ms32, _ = run_steps(torch, ctx, (U32, Un32, Q32, m32, st["J"], in32, dt), "f32b", args.steps, args.warmup)
+        ctx.nan_check(U32)
+        value32 = ncells / (ms32 * 1e-3) / 1e6
+        NG = 4
+        inner = (slice(NG, -NG), slice(NG, -NG), slice(NG, -NG))
+        r64, r32 = U[0][inner].double(), U32[0][inner].double()
+        pw64 = Q[4][NG:-NG, NG, NG:-NG].mean(dim=0)  # p at the first cell above the wall, spanwise mean
+        pw32 = Q32[4][NG:-NG, NG, NG:-NG].double().mean(dim=0)
+        out32 = dict(
+            value=round(value32, 3), ms_per_step=round(ms32, 4),
+            speedup_vs_f64=round(value32 / value64, 3),
+            hbm_frac_path=round(ALG_BYTES_PER_CELL_UPDATE["f32b"] * value32 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+            rho_L2_rel=float(torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)),
+            rho_Linf_rel=float((r32 - r64).abs().max() / r64.abs().max()),
+            pwall_L2_rel=float(torch.linalg.norm(pw32 - pw64) / torch.linalg.norm(pw64)),
+            pwall_Linf_rel=float((pw32 - pw64).abs().max() / pw64.abs().max()),
+            steps_compared=args.steps + args.warmup,
+        )
+    # ---- roofline of the dominant kernel (hipEvent-timed launches inside the FP64 timed region)
+    dom = max(ktab.items(), key=lambda kv: kv[1][0]) if ktab else None
+    roof = None
+    if dom:
+        name, (tot_ms, n) = dom
+        avg_s = tot_ms / n * 1e-3
+        alg = kernel_alg_bytes(name, p, 8)
+        achieved = alg / avg_s / 1e9
+        roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=name,
+                    avg_launch_us=round(avg_s * 1e6, 2), alg_bytes_per_launch=alg,
+                    share_of_step=round(tot_ms / (ms64 * args.steps), 4))
+    kshare = {k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1],
+                      share=round(v[0] / (ms64 * args.steps), 4)) for k, v in sorted(ktab.items())}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(f3d, cases, nx, ny, nz, steps=args.cpu_steps)
+    line = {
+        "metric": "Mcell-updates/s (FP64 RHS + SSP-RK3 step)",
+        "value": round(value64, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms64, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded laminar-BL ramp inflow, generated grid)",
+        "config": {"workload": f"compression ramp 24 deg, {nx}x{ny}x{nz}, FP64, M2.9, Re_delta 6e4, "
+                               f"isothermal wall, periodic span", "cells": ncells, "dt": dt},
+        "hbm_frac_path": round(ALG_BYTES_PER_CELL_UPDATE["f64"] * value64 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp32_bulk": out32 or None,
+        "kernels": kshare,
+        "host_wall_ms_per_step": round(wall64, 4),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,327 @@
+"""Flame3D finite-difference RHS + RK3 stage on MI355X (gfx950): Python host over libf3d_hip.so.
+
+The reference (rygan-code/Flame3D-MHD, Julia + CUDA.jl) launches per-stage kernels from
+`time_step` (solver.jl:191-680).  This module exposes the same entry points with the same names
+and argument order -- `shockSensor`, `fluxSplit_SW`, `advect_xc/yc/zc`, `viscousFlux`, `div`,
+`linComb`, `c2Prim`, `prim2c`, `exchange_ghost`, `fillGhost`, `flowAdvance`, `time_step` -- as
+methods of a `Flame3D` context that carries what the reference keeps in `const` globals
+(run.jl:3-108).  Arrays are torch tensors on the GPU in the reference's column-major layout:
+a Julia array (Nx_tot, Ny_tot, Nz_tot, nvar) is a C-contiguous tensor of shape
+(nvar, Nz_tot, Ny_tot, Nx_tot).  Every call goes through the C-ABI of include/f3d.h; there is no
+CPU fallback: if the library is missing, importing the module fails.
+
+Errors: a non-zero status from the library raises `F3DError` with the library's message (the
+reference raises Julia `error(...)` for bad configuration, solver.jl:143, run.jl:119);
+`nan_check` raises `F3DNaNError` where the reference calls MPI.Abort (solver.jl:481-486).
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int32, c_size_t, c_void_p
+
+__all__ = [
+    "F3DParams", "F3DError", "F3DNaNError", "Flame3D", "Workspace", "lib", "make_params",
+    "BC_PERIODIC", "BC_DIRICHLET", "BC_ZEROGRAD", "BC_WALL_ADIABATIC", "BC_WALL_ISOTHERMAL", "BC_INFLOW",
+    "PREC_F64", "PREC_F32B", "SCHEME_UP7", "SCHEME_WENO7", "SCHEME_WENO5", "SCHEME_NND2",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libf3d_hip.so")
+
+BC_PERIODIC, BC_DIRICHLET, BC_ZEROGRAD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, BC_INFLOW = range(6)
+PREC_F64, PREC_F32B = 0, 1
+SCHEME_UP7, SCHEME_WENO7, SCHEME_WENO5, SCHEME_NND2 = range(4)
+NG = 4
+
+
+class F3DParams(ctypes.Structure):
+    """ctypes mirror of `f3d_params` (include/f3d.h)."""
+    _fields_ = [
+        ("Nxp", c_int32), ("Nyp", c_int32), ("Nzp", c_int32), ("NG", c_int32), ("viscous", c_int32),
+        ("bc", c_int32 * 6), ("pad_", c_int32),
+        ("gamma", c_double), ("Rg", c_double), ("Cp", c_double), ("C_s", c_double), ("T_s", c_double),
+        ("Pr", c_double), ("hybrid_phi1", c_double), ("hybrid_phi2", c_double), ("hybrid_phi3", c_double),
+        ("Linear_phi", c_double), ("weno7_eps", c_double), ("weno5_eps", c_double),
+        ("q_inf", c_double * 6), ("T_wall", c_double),
+    ]
+
+    @property
+    def dims(self):
+        return self.Nxp, self.Nyp, self.Nzp
+
+    @property
+    def tot(self):
+        return self.Nxp + 2 * self.NG, self.Nyp + 2 * self.NG, self.Nzp + 2 * self.NG
+
+    def cell_shape(self, nvar=None):
+        """torch/numpy shape of a Julia (Nx_tot, Ny_tot, Nz_tot[, nvar]) array."""
+        nx, ny, nz = self.tot
+        return (nz, ny, nx) if nvar is None else (nvar, nz, ny, nx)
+
+    def face_shape(self, d):
+        nx, ny, nz = self.Nxp + (d == 0), self.Nyp + (d == 1), self.Nzp + (d == 2)
+        return (5, nz, ny, nx)
+
+    def fv_shape(self):
+        return (4, self.Nzp + self.NG, self.Nyp + self.NG, self.Nxp + self.NG)
+
+    def mask_shape(self, d):
+        return self.face_shape(d)[1:]
+
+
+class F3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"f3d error {code}: {msg}")
+        self.code = code
+
+
+class F3DNaNError(F3DError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libf3d_hip.so.  Raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.f3d_last_error.restype = ctypes.c_char_p
+        L.f3d_params_init.restype = None
+        L.f3d_params_init.argtypes = [POINTER(F3DParams), c_int32, c_int32, c_int32]
+        L.f3d_cell_count.restype = c_size_t
+        L.f3d_workspace_bytes.restype = c_size_t
+        L.f3d_workspace_create.argtypes = [POINTER(F3DParams), c_int32, POINTER(c_void_p)]
+        L.f3d_workspace_destroy.argtypes = [c_void_p]
+        L.f3d_workspace_bytes.argtypes = [c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().f3d_last_error().decode(errors="replace")
+        if rc == -5:
+            raise F3DNaNError(rc, msg)
+        raise F3DError(rc, msg)
+
+
+def make_params(Nxp, Nyp, Nzp, **kw):
+    """Reference defaults (run.jl constants, boundary.jl boundary set), then overrides.
+
+    `bc` may be given as a 6-sequence (xlo, xhi, ylo, yhi, zlo, zhi), `q_inf` as a 6-sequence.
+    """
+    p = F3DParams()
+    lib().f3d_params_init(ctypes.byref(p), Nxp, Nyp, Nzp)
+    for k, v in kw.items():
+        if k in ("bc", "q_inf"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+    if "gamma" in kw or "Rg" in kw:
+        if "Cp" not in kw:
+            p.Cp = p.Rg * p.gamma / (p.gamma - 1)
+    _check(lib().f3d_params_check(ctypes.byref(p)))
+    return p
+
+
+def _ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return c_void_p(t.data_ptr())
+
+
+def _ptrs(ts):
+    arr = (c_void_p * len(ts))()
+    for i, t in enumerate(ts):
+        arr[i] = t.data_ptr() if t is not None else None
+    return arr
+
+
+class Workspace:
+    """Owner of an f3d_workspace (device scratch of the fused path)."""
+
+    def __init__(self, params, precision):
+        self.params = params
+        self.precision = precision
+        self._h = c_void_p()
+        _check(lib().f3d_workspace_create(ctypes.byref(params), precision, ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def nbytes(self):
+        return lib().f3d_workspace_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            _check(lib().f3d_workspace_destroy(self._h))
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Flame3D:
+    """Context holding the run.jl constants (`params`) and a HIP stream.
+
+    Methods carry the reference's names and argument lists.  The numeric form is chosen from the
+    tensors: all-float64 -> `_f64`; float32 bulk arrays with float64 U/Un/J -> `_f32b`.
+    """
+
+    def __init__(self, params, stream=None):
+        self.params = params
+        self._stream = stream
+        self._ws = {}
+
+    # -- plumbing ---------------------------------------------------------------------------
+    @property
+    def stream(self):
+        import torch
+        s = self._stream if self._stream is not None else torch.cuda.current_stream()
+        return c_void_p(s.cuda_stream)
+
+    def _suffix(self, t):
+        import torch
+        if t.dtype == torch.float64:
+            return "f64"
+        if t.dtype == torch.float32:
+            return "f32b"
+        raise TypeError(f"unsupported dtype {t.dtype}")
+
+    def _call(self, name, *args):
+        fn = getattr(lib(), name)
+        _check(fn(ctypes.byref(self.params), *args, self.stream))
+
+    def workspace(self, precision):
+        ws = self._ws.get(precision)
+        if ws is None:
+            ws = self._ws[precision] = Workspace(self.params, precision)
+        return ws
+
+    # -- the reference's per-stage kernels ----------------------------------------------------
+    def shockSensor(self, phi, Q):
+        """schemes.jl:1-24"""
+        self._call(f"f3d_shock_sensor_{self._suffix(Q)}", _ptr(phi), _ptr(Q))
+
+    def schemeMask(self, mask_x, mask_y, mask_z, phi):
+        """Face scheme index of advect_{x,y,z}c (schemes.jl:724-731, 813, 931)."""
+        self._call(f"f3d_scheme_mask_{self._suffix(phi)}", _ptr(mask_x), _ptr(mask_y), _ptr(mask_z), _ptr(phi))
+
+    def fluxSplit_SW(self, Q, Fp, Fm, S, Ax, Ay, Az):
+        """split.jl:2-68"""
+        self._call(f"f3d_flux_split_sw_{self._suffix(Q)}", *map(_ptr, (Q, Fp, Fm, S, Ax, Ay, Az)))
+
+    def _advect(self, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
+        self._call(f"f3d_advect_{'xyz'[d]}c_{self._suffix(Q)}", *map(_ptr, (F, phi, S, Fp, Fm, Q, Ax, Ay, Az)))
+
+    def advect_xc(self, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
+        """schemes.jl:700-1012"""
+        self._advect(0, F, phi, S, Fp, Fm, Q, Ax, Ay, Az)
+
+    def advect_yc(self, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
+        """schemes.jl:1014-1328"""
+        self._advect(1, F, phi, S, Fp, Fm, Q, Ax, Ay, Az)
+
+    def advect_zc(self, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
+        """schemes.jl:1330-1643"""
+        self._advect(2, F, phi, S, Fp, Fm, Q, Ax, Ay, Az)
+
+    def viscousFlux(self, Fv_x, Fv_y, Fv_z, Q, dxidx, dxidy, dxidz, detadx, detady, detadz,
+                    dzetadx, dzetady, dzetadz, J):
+        """viscous.jl:1-134 (laminar)"""
+        self._call(f"f3d_viscous_flux_{self._suffix(Q)}",
+                   *map(_ptr, (Fv_x, Fv_y, Fv_z, Q, dxidx, dxidy, dxidz, detadx, detady, detadz,
+                               dzetadx, dzetady, dzetadz, J)))
+
+    def div(self, U, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, dt, J):
+        """div.jl:1-50"""
+        self._call(f"f3d_div_{self._suffix(Fx)}", *map(_ptr, (U, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z)),
+                   c_double(dt), _ptr(J))
+
+    def linComb(self, U, Un, NV, a, b):
+        """utils.jl:57-70"""
+        self._call("f3d_lin_comb", _ptr(U), _ptr(Un), c_int32(NV), c_double(a), c_double(b))
+
+    def c2Prim(self, U, Q):
+        """utils.jl:2-30"""
+        self._call(f"f3d_c2prim_{self._suffix(Q)}", _ptr(U), _ptr(Q))
+
+    def prim2c(self, U, Q):
+        """utils.jl:33-54"""
+        self._call(f"f3d_prim2c_{self._suffix(Q)}", _ptr(U), _ptr(Q))
+
+    def exchange_ghost(self, Q, NV):
+        """mpi.jl:2-103 with one rank: local periodic copies (no MPI, no host staging)."""
+        self._call(f"f3d_exchange_ghost_{self._suffix(Q)}", _ptr(Q), c_int32(NV))
+
+    def fillGhost(self, Q, U, inflow=None):
+        """boundary.jl:151-156 (fill_x, fill_y, fill_z) with the face kinds of params.bc."""
+        self._call(f"f3d_fill_ghost_{self._suffix(Q)}", _ptr(Q), _ptr(U), _ptr(inflow))
+
+    def flowAdvance(self, U, Q, Fp, Fm, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, s1, s2, s3, dxidx, dxidy, dxidz,
+                    detadx, detady, detadz, dzetadx, dzetady, dzetadz, J, x, y, z, phi):
+        """solver.jl:19, finite-difference branch 133-188."""
+        self._call(f"f3d_flow_advance_{self._suffix(Q)}",
+                   *map(_ptr, (U, Q, Fp, Fm, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, s1, s2, s3, dxidx, dxidy, dxidz,
+                               detadx, detady, detadz, dzetadx, dzetady, dzetadz, J, x, y, z, phi)))
+
+    # -- fused path ---------------------------------------------------------------------------
+    def _prec(self, Q):
+        return PREC_F64 if self._suffix(Q) == "f64" else PREC_F32B
+
+    def rhs(self, R, Q, metrics, J, phi=None):
+        """R = J*(-dF + dFv) on interior cells (the div increment per unit dt)."""
+        prec = self._prec(Q)
+        ws = self.workspace(prec)
+        fn = getattr(lib(), f"f3d_rhs_{self._suffix(Q)}")
+        _check(fn(ctypes.byref(self.params), ws.handle, _ptr(R), _ptr(Q), _ptrs(metrics), _ptr(J), _ptr(phi),
+                  self.stream))
+
+    def rk3_stage(self, KRK, U, Un, Q, metrics, J, dt, inflow=None):
+        """One stage of the loop body solver.jl:416-444."""
+        prec = self._prec(Q)
+        ws = self.workspace(prec)
+        fn = getattr(lib(), f"f3d_rk3_stage_{self._suffix(Q)}")
+        _check(fn(ctypes.byref(self.params), ws.handle, c_int32(KRK), _ptr(U), _ptr(Un), _ptr(Q), _ptrs(metrics),
+                  _ptr(J), c_double(dt), _ptr(inflow), self.stream))
+
+    def time_step(self, nsteps, U, Un, Q, metrics, J, dt, inflow=None):
+        """`nsteps` SSP-RK3 steps (solver.jl:410-444 without I/O, filtering or LTS)."""
+        prec = self._prec(Q)
+        ws = self.workspace(prec)
+        fn = getattr(lib(), f"f3d_time_step_{self._suffix(Q)}")
+        _check(fn(ctypes.byref(self.params), ws.handle, c_int32(nsteps), _ptr(U), _ptr(Un), _ptr(Q),
+                  _ptrs(metrics), _ptr(J), c_double(dt), _ptr(inflow), self.stream))
+
+    def cfl_dt(self, dt_out, Q, metrics, J, cfl):
+        """Global CFL time step into the device scalar dt_out (float64 tensor of one element)."""
+        prec = self._prec(Q)
+        ws = self.workspace(prec)
+        fn = getattr(lib(), f"f3d_cfl_dt_{self._suffix(Q)}")
+        _check(fn(ctypes.byref(self.params), ws.handle, _ptr(dt_out), _ptr(Q), _ptrs(metrics), _ptr(J),
+                  c_double(cfl), self.stream))
+
+    def nan_check(self, U, NV=5):
+        """any(isnan, U) (solver.jl:481-486): raises F3DNaNError when U holds NaN/Inf."""
+        ws = self.workspace(PREC_F64) if PREC_F64 in self._ws else next(iter(self._ws.values()), None)
+        if ws is None:
+            ws = self.workspace(PREC_F64)
+        _check(lib().f3d_nan_check_sync(ctypes.byref(self.params), ws.handle, _ptr(U), c_int32(NV), self.stream))
+
+    def metrics(self, x, y, z, metrics, J, s=(None, None, None)):
+        """FP64 metrics and Jacobian from node coordinates (parse_mesh.jl:139-248)."""
+        _check(lib().f3d_metrics_f64(ctypes.byref(self.params), _ptr(x), _ptr(y), _ptr(z), _ptrs(metrics),
+                                     _ptr(J), _ptrs(list(s)), self.stream))

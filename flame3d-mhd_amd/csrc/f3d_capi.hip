@@ -1,0 +1,374 @@
+// extern "C" entry points of libf3d_hip.so: argument checking, error reporting and dispatch to
+// the launchers.  See include/f3d.h for the contract of every function.
+#include <cstdarg>
+#include <cstring>
+
+#include "f3d_internal.hpp"
+
+namespace f3d {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+const char* last_error() { return g_err; }
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: kernel launch failed: %s", what, hipGetErrorString(e));
+    return (int)e;
+  }
+  return F3D_OK;
+}
+
+KP make_kp(const f3d_params* p) {
+  KP k;
+  std::memset(&k, 0, sizeof(k));
+  k.Nxp = p->Nxp;
+  k.Nyp = p->Nyp;
+  k.Nzp = p->Nzp;
+  k.NG = p->NG;
+  k.Nxt = p->Nxp + 2 * p->NG;
+  k.Nyt = p->Nyp + 2 * p->NG;
+  k.Nzt = p->Nzp + 2 * p->NG;
+  k.viscous = p->viscous;
+  for (int i = 0; i < 6; ++i) k.bc[i] = p->bc[i];
+  k.nc = (long long)k.Nxt * k.Nyt * k.Nzt;
+  k.gamma = p->gamma;
+  k.Rg = p->Rg;
+  k.Cp = p->Cp;
+  k.Cs = p->C_s;
+  k.Ts = p->T_s;
+  k.Pr = p->Pr;
+  k.phi1 = p->hybrid_phi1;
+  k.phi2 = p->hybrid_phi2;
+  k.phi3 = p->hybrid_phi3;
+  k.eps7 = p->weno7_eps;
+  k.eps5 = p->weno5_eps;
+  k.Twall = p->T_wall;
+  // Linear = UP7*Linear_phi + CD6*(1-Linear_phi)  run.jl:75-77
+  const double up7[7] = {-3.0 / 420, 25.0 / 420, -101.0 / 420, 319.0 / 420, 214.0 / 420, -38.0 / 420, 4.0 / 420};
+  const double cd6[7] = {0.0, 1.0 / 60, -2.0 / 15, 37.0 / 60, 37.0 / 60, -2.0 / 15, 1.0 / 60};
+  for (int m = 0; m < 7; ++m) k.lin[m] = up7[m] * p->Linear_phi + cd6[m] * (1.0 - p->Linear_phi);
+  for (int i = 0; i < 6; ++i) k.qinf[i] = p->q_inf[i];
+  return k;
+}
+
+}  // namespace f3d
+
+using namespace f3d;
+
+#define F3D_API extern "C" __attribute__((visibility("default")))
+
+F3D_API int f3d_version(void) { return F3D_VERSION; }
+F3D_API const char* f3d_last_error(void) { return last_error(); }
+
+F3D_API void f3d_params_init(f3d_params* p, int32_t Nxp, int32_t Nyp, int32_t Nzp) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->Nxp = Nxp;
+  p->Nyp = Nyp;
+  p->Nzp = Nzp;
+  p->NG = 4;
+  p->viscous = 1;
+  // boundary set of boundary.jl: inlet Dirichlet, outlet zero-gradient, adiabatic wall below,
+  // Dirichlet farfield above, periodic span (Iperiodic z, run.jl:19)
+  p->bc[0] = F3D_BC_DIRICHLET;
+  p->bc[1] = F3D_BC_ZEROGRAD;
+  p->bc[2] = F3D_BC_WALL_ADIABATIC;
+  p->bc[3] = F3D_BC_DIRICHLET;
+  p->bc[4] = F3D_BC_PERIODIC;
+  p->bc[5] = F3D_BC_PERIODIC;
+  p->gamma = 1.4;
+  p->Rg = 287.0;
+  p->Cp = p->Rg * p->gamma / (p->gamma - 1);
+  p->C_s = 1.458e-6;
+  p->T_s = 110.4;
+  p->Pr = 0.72;
+  p->hybrid_phi1 = 0.05;
+  p->hybrid_phi2 = 1.0;
+  p->hybrid_phi3 = 10.0;
+  p->Linear_phi = 1.0;
+  p->weno7_eps = 1e-20;
+  p->weno5_eps = 1e-16;
+  // freestream of boundary.jl:14-19
+  p->q_inf[0] = 1.0;
+  p->q_inf[1] = 0.5;
+  p->q_inf[2] = 0.0;
+  p->q_inf[3] = 0.0;
+  p->q_inf[4] = 1.0 / 1.4;
+  p->q_inf[5] = p->q_inf[4] / (p->q_inf[0] * p->Rg);
+  p->T_wall = 0.0;
+}
+
+F3D_API int f3d_params_check(const f3d_params* p) {
+  if (!p) {
+    set_error("params: NULL");
+    return F3D_E_NULL;
+  }
+  if (p->NG != 4) {
+    set_error("params: NG=%d, only NG=4 is supported (viscous.jl:119, div.jl:14-39 assume it)", p->NG);
+    return F3D_E_NG;
+  }
+  if (p->Nxp < 4 || p->Nyp < 4 || p->Nzp < 4) {
+    set_error("params: interior dims (%d,%d,%d) must each be >= NG", p->Nxp, p->Nyp, p->Nzp);
+    return F3D_E_DIMS;
+  }
+  const long long nc = (long long)(p->Nxp + 8) * (p->Nyp + 8) * (p->Nzp + 8);
+  if (nc * 6 >= (1LL << 40)) {
+    set_error("params: grid too large");
+    return F3D_E_DIMS;
+  }
+  for (int i = 0; i < 6; ++i) {
+    const int b = p->bc[i];
+    if (b < F3D_BC_PERIODIC || b > F3D_BC_INFLOW) {
+      set_error("params: bc[%d]=%d unknown", i, b);
+      return F3D_E_ARG;
+    }
+    if (b == F3D_BC_INFLOW && i != 0) {
+      set_error("params: F3D_BC_INFLOW is only defined on the x-low face");
+      return F3D_E_ARG;
+    }
+  }
+  for (int d = 0; d < 3; ++d)
+    if ((p->bc[2 * d] == F3D_BC_PERIODIC) != (p->bc[2 * d + 1] == F3D_BC_PERIODIC)) {
+      set_error("params: periodic faces must come in pairs (dimension %d)", d);
+      return F3D_E_ARG;
+    }
+  if (!(p->gamma > 1) || !(p->Rg > 0)) {
+    set_error("params: gamma must be > 1 and Rg > 0");
+    return F3D_E_ARG;
+  }
+  if ((p->bc[2] == F3D_BC_WALL_ISOTHERMAL || p->bc[3] == F3D_BC_WALL_ISOTHERMAL || p->bc[0] == F3D_BC_WALL_ISOTHERMAL ||
+       p->bc[1] == F3D_BC_WALL_ISOTHERMAL || p->bc[4] == F3D_BC_WALL_ISOTHERMAL ||
+       p->bc[5] == F3D_BC_WALL_ISOTHERMAL) &&
+      !(p->T_wall > 0)) {
+    set_error("params: isothermal wall needs T_wall > 0");
+    return F3D_E_ARG;
+  }
+  return F3D_OK;
+}
+
+F3D_API size_t f3d_cell_count(const f3d_params* p) {
+  if (!p) return 0;
+  return (size_t)(p->Nxp + 2 * p->NG) * (p->Nyp + 2 * p->NG) * (p->Nzp + 2 * p->NG);
+}
+
+#define CHECK_P()                          \
+  do {                                     \
+    int rc_ = f3d_params_check(p);         \
+    if (rc_) return rc_;                   \
+  } while (0)
+#define NEED(x)                                               \
+  do {                                                        \
+    if (!(x)) {                                               \
+      set_error("%s: argument %s is NULL", __func__, #x);     \
+      return F3D_E_NULL;                                      \
+    }                                                         \
+  } while (0)
+
+// ---------------------------------------------------------------- sensor / mask / split
+#define SENSOR_IMPL(SUF, T)                                                                      \
+  F3D_API int f3d_shock_sensor_##SUF(const f3d_params* p, T* phi, const T* Q, f3d_stream_t s) { \
+    CHECK_P();                                                                                   \
+    NEED(phi);                                                                                   \
+    NEED(Q);                                                                                     \
+    return launch_shock_sensor<T>(make_kp(p), phi, Q, (hipStream_t)s);                           \
+  }
+SENSOR_IMPL(f64, double)
+SENSOR_IMPL(f32b, float)
+
+#define MASK_IMPL(SUF, T)                                                                                \
+  F3D_API int f3d_scheme_mask_##SUF(const f3d_params* p, uint8_t* mx, uint8_t* my, uint8_t* mz,         \
+                                    const T* phi, f3d_stream_t s) {                                      \
+    CHECK_P();                                                                                           \
+    NEED(phi);                                                                                           \
+    return launch_scheme_mask<T>(make_kp(p), mx, my, mz, phi, (hipStream_t)s);                           \
+  }
+MASK_IMPL(f64, double)
+MASK_IMPL(f32b, float)
+
+#define SPLIT_IMPL(SUF, T)                                                                                   \
+  F3D_API int f3d_flux_split_sw_##SUF(const f3d_params* p, const T* Q, T* Fp, T* Fm, const T* S, const T* Ax, \
+                                      const T* Ay, const T* Az, f3d_stream_t s) {                            \
+    CHECK_P();                                                                                               \
+    NEED(Q);                                                                                                 \
+    NEED(Fp);                                                                                                \
+    NEED(Fm);                                                                                                \
+    NEED(S);                                                                                                 \
+    NEED(Ax);                                                                                                \
+    NEED(Ay);                                                                                                \
+    NEED(Az);                                                                                                \
+    return launch_flux_split<T>(make_kp(p), Q, Fp, Fm, S, Ax, Ay, Az, (hipStream_t)s);                       \
+  }
+SPLIT_IMPL(f64, double)
+SPLIT_IMPL(f32b, float)
+
+#define ADVECT_IMPL(NAME, D, SUF, T)                                                                         \
+  F3D_API int f3d_advect_##NAME##_##SUF(const f3d_params* p, T* F, const T* phi, const T* S, const T* Fp,    \
+                                        const T* Fm, const T* Q, const T* Ax, const T* Ay, const T* Az,      \
+                                        f3d_stream_t s) {                                                    \
+    CHECK_P();                                                                                               \
+    NEED(F);                                                                                                 \
+    NEED(phi);                                                                                               \
+    NEED(S);                                                                                                 \
+    NEED(Fp);                                                                                                \
+    NEED(Fm);                                                                                                \
+    NEED(Q);                                                                                                 \
+    NEED(Ax);                                                                                                \
+    NEED(Ay);                                                                                                \
+    NEED(Az);                                                                                                \
+    return launch_advect<T>(make_kp(p), D, F, phi, S, Fp, Fm, Q, Ax, Ay, Az, (hipStream_t)s);                \
+  }
+ADVECT_IMPL(xc, 0, f64, double)
+ADVECT_IMPL(yc, 1, f64, double)
+ADVECT_IMPL(zc, 2, f64, double)
+ADVECT_IMPL(xc, 0, f32b, float)
+ADVECT_IMPL(yc, 1, f32b, float)
+ADVECT_IMPL(zc, 2, f32b, float)
+
+#define VISC_IMPL(SUF, T)                                                                                      \
+  F3D_API int f3d_viscous_flux_##SUF(const f3d_params* p, T* Fvx, T* Fvy, T* Fvz, const T* Q, const T* m0,     \
+                                     const T* m1, const T* m2, const T* m3, const T* m4, const T* m5,          \
+                                     const T* m6, const T* m7, const T* m8, const double* J, f3d_stream_t s) { \
+    CHECK_P();                                                                                                 \
+    NEED(Fvx);                                                                                                 \
+    NEED(Fvy);                                                                                                 \
+    NEED(Fvz);                                                                                                 \
+    NEED(Q);                                                                                                   \
+    NEED(J);                                                                                                   \
+    MetricPtrs<T> m{{m0, m1, m2, m3, m4, m5, m6, m7, m8}};                                                     \
+    for (int q = 0; q < 9; ++q)                                                                                \
+      if (!m.m[q]) {                                                                                           \
+        set_error("viscous_flux: metric %d is NULL", q);                                                       \
+        return F3D_E_NULL;                                                                                     \
+      }                                                                                                        \
+    return launch_viscous<T>(make_kp(p), Fvx, Fvy, Fvz, Q, m, J, (hipStream_t)s);                              \
+  }
+VISC_IMPL(f64, double)
+VISC_IMPL(f32b, float)
+
+#define DIV_IMPL(SUF, T)                                                                                     \
+  F3D_API int f3d_div_##SUF(const f3d_params* p, double* U, const T* Fx, const T* Fy, const T* Fz,           \
+                            const T* Fvx, const T* Fvy, const T* Fvz, double dt, const double* J,            \
+                            f3d_stream_t s) {                                                                \
+    CHECK_P();                                                                                               \
+    NEED(U);                                                                                                 \
+    NEED(Fx);                                                                                                \
+    NEED(Fy);                                                                                                \
+    NEED(Fz);                                                                                                \
+    NEED(Fvx);                                                                                               \
+    NEED(Fvy);                                                                                               \
+    NEED(Fvz);                                                                                               \
+    NEED(J);                                                                                                 \
+    return launch_div<T>(make_kp(p), U, Fx, Fy, Fz, Fvx, Fvy, Fvz, dt, J, (hipStream_t)s);                   \
+  }
+DIV_IMPL(f64, double)
+DIV_IMPL(f32b, float)
+
+F3D_API int f3d_lin_comb(const f3d_params* p, double* U, const double* Un, int32_t NV, double a, double b,
+                         f3d_stream_t s) {
+  CHECK_P();
+  NEED(U);
+  NEED(Un);
+  if (NV < 1 || NV > 6) {
+    set_error("lin_comb: NV=%d out of range", NV);
+    return F3D_E_ARG;
+  }
+  return launch_lin_comb(make_kp(p), U, Un, NV, a, b, (hipStream_t)s);
+}
+
+#define CONV_IMPL(SUF, T)                                                                           \
+  F3D_API int f3d_c2prim_##SUF(const f3d_params* p, const double* U, T* Q, f3d_stream_t s) {        \
+    CHECK_P();                                                                                      \
+    NEED(U);                                                                                        \
+    NEED(Q);                                                                                        \
+    return launch_c2prim<T>(make_kp(p), U, Q, (hipStream_t)s);                                      \
+  }                                                                                                 \
+  F3D_API int f3d_prim2c_##SUF(const f3d_params* p, double* U, const T* Q, f3d_stream_t s) {        \
+    CHECK_P();                                                                                      \
+    NEED(U);                                                                                        \
+    NEED(Q);                                                                                        \
+    return launch_prim2c<T>(make_kp(p), U, Q, (hipStream_t)s);                                      \
+  }                                                                                                 \
+  F3D_API int f3d_exchange_ghost_##SUF(const f3d_params* p, T* Q, int32_t NV, f3d_stream_t s) {     \
+    CHECK_P();                                                                                      \
+    NEED(Q);                                                                                        \
+    if (NV < 1 || NV > 6) {                                                                         \
+      set_error("exchange_ghost: NV=%d out of range", NV);                                          \
+      return F3D_E_ARG;                                                                             \
+    }                                                                                               \
+    return launch_exchange<T>(make_kp(p), Q, NV, (hipStream_t)s);                                   \
+  }                                                                                                 \
+  F3D_API int f3d_fill_ghost_##SUF(const f3d_params* p, T* Q, double* U, const T* inflow,           \
+                                   f3d_stream_t s) {                                                \
+    CHECK_P();                                                                                      \
+    NEED(Q);                                                                                        \
+    if (p->bc[0] == F3D_BC_INFLOW && !inflow) {                                                     \
+      set_error("fill_ghost: F3D_BC_INFLOW needs an inflow profile");                               \
+      return F3D_E_NULL;                                                                            \
+    }                                                                                               \
+    return launch_fill<T>(make_kp(p), Q, U, inflow, (hipStream_t)s);                                \
+  }
+CONV_IMPL(f64, double)
+CONV_IMPL(f32b, float)
+
+// flowAdvance, finite-difference branch  solver.jl:133-188
+template <typename T>
+static int flow_advance_impl(const f3d_params* p, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, T* Fvx, T* Fvy,
+                             T* Fvz, const T* s1, const T* s2, const T* s3, const MetricPtrs<T>& m,
+                             const double* J, T* phi, hipStream_t s) {
+  KP kp = make_kp(p);
+  const T* S[3] = {s1, s2, s3};
+  T* F[3] = {Fx, Fy, Fz};
+  for (int d = 0; d < 3; ++d) {
+    int rc = launch_flux_split<T>(kp, Q, Fp, Fm, S[d], m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
+    if (rc) return rc;
+    rc = launch_advect<T>(kp, d, F[d], phi, S[d], Fp, Fm, Q, m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
+    if (rc) return rc;
+  }
+  if (p->viscous) return launch_viscous<T>(kp, Fvx, Fvy, Fvz, Q, m, J, s);
+  return F3D_OK;
+}
+
+#define FA_IMPL(SUF, T)                                                                                       \
+  F3D_API int f3d_flow_advance_##SUF(                                                                         \
+      const f3d_params* p, double* U, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, T* Fvx, T* Fvy, T* Fvz,         \
+      const T* s1, const T* s2, const T* s3, const T* m0, const T* m1, const T* m2, const T* m3, const T* m4,  \
+      const T* m5, const T* m6, const T* m7, const T* m8, const double* J, const T* x, const T* y, const T* z, \
+      T* phi, f3d_stream_t s) {                                                                               \
+    (void)U;                                                                                                  \
+    (void)x;                                                                                                  \
+    (void)y;                                                                                                  \
+    (void)z;                                                                                                  \
+    CHECK_P();                                                                                                \
+    NEED(Q);                                                                                                  \
+    NEED(Fp);                                                                                                 \
+    NEED(Fm);                                                                                                 \
+    NEED(Fx);                                                                                                 \
+    NEED(Fy);                                                                                                 \
+    NEED(Fz);                                                                                                 \
+    NEED(s1);                                                                                                 \
+    NEED(s2);                                                                                                 \
+    NEED(s3);                                                                                                 \
+    NEED(phi);                                                                                                \
+    MetricPtrs<T> m{{m0, m1, m2, m3, m4, m5, m6, m7, m8}};                                                    \
+    for (int q = 0; q < 9; ++q)                                                                               \
+      if (!m.m[q]) {                                                                                          \
+        set_error("flow_advance: metric %d is NULL", q);                                                      \
+        return F3D_E_NULL;                                                                                    \
+      }                                                                                                       \
+    if (p->viscous && (!Fvx || !Fvy || !Fvz || !J)) {                                                         \
+      set_error("flow_advance: viscous needs Fv_x, Fv_y, Fv_z and J");                                        \
+      return F3D_E_NULL;                                                                                      \
+    }                                                                                                         \
+    return flow_advance_impl<T>(p, Q, Fp, Fm, Fx, Fy, Fz, Fvx, Fvy, Fvz, s1, s2, s3, m, J, phi,              \
+                                (hipStream_t)s);                                                              \
+  }
+FA_IMPL(f64, double)
+FA_IMPL(f32b, float)

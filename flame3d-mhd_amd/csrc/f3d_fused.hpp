@@ -1,0 +1,38 @@
+// Fused RHS / RK3-stage kernels (the MI355X hot path).
+#pragma once
+#include "f3d_internal.hpp"
+
+namespace f3d {
+
+// Workspace arrays the fused path uses (bulk type T).
+template <typename T>
+struct WsView {
+  T* R;    // interior RHS accumulator [5][Nzp][Nyp][Nxp]
+  T* Fvx;  // viscous flux scratch, reference Fv layout
+  T* Fvy;
+  T* Fvz;
+  T* phi;
+};
+
+}  // namespace f3d
+
+struct f3d_workspace;
+
+namespace f3d {
+template <typename T>
+WsView<T> ws_view(f3d_workspace* ws);
+
+// true when the fused sweeps serve f3d_rhs_* / f3d_rk3_stage_* (F3D_RHS_ENGINE=chain selects the
+// kernel-by-kernel chain instead, for A/B measurement).
+bool use_fused();
+
+// RHS into R (U layout, interior) when R != nullptr.
+template <typename T>
+int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J, T* phi,
+              double* R, int KRK, double dt, double* U, hipStream_t s);
+
+// One RK3 stage up to and including c2Prim of the interior (ghost fill done by the caller).
+template <typename T>
+int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, T* Q, const MetricPtrs<T>& m,
+                const double* J, double dt, hipStream_t s);
+}  // namespace f3d

@@ -1,0 +1,68 @@
+// Host-side internals shared by the translation units of libf3d_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "f3d_device.hpp"
+
+namespace f3d {
+
+template <typename T>
+struct MetricPtrs {
+  const T* m[9];
+};
+
+// Thread-local message of the last failure (f3d_last_error).
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+// Status of the launches just queued (hipGetLastError), as an f3d status code.
+int check_launch(const char* what);
+
+#define F3D_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ::f3d::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                       \
+      return (int)e_;                                                                   \
+    }                                                                                   \
+  } while (0)
+
+KP make_kp(const f3d_params* p);
+
+// RAII bracket of one kernel launch with hipEvents when timing is enabled (f3d_timing_*).
+class TimedScope {
+ public:
+  TimedScope(const char* name, hipStream_t s);
+  ~TimedScope();
+
+ private:
+  int slot_;
+  hipStream_t s_;
+};
+
+// kernel-by-kernel launchers (f3d_kernels.hip)
+template <typename T> int launch_shock_sensor(const KP&, T* phi, const T* Q, hipStream_t);
+template <typename T> int launch_scheme_mask(const KP&, uint8_t*, uint8_t*, uint8_t*, const T* phi, hipStream_t);
+template <typename T>
+int launch_flux_split(const KP&, const T* Q, T* Fp, T* Fm, const T* S, const T* Ax, const T* Ay, const T* Az,
+                      hipStream_t);
+template <typename T>
+int launch_advect(const KP&, int d, T* F, const T* phi, const T* S, const T* Fp, const T* Fm, const T* Q,
+                  const T* Ax, const T* Ay, const T* Az, hipStream_t);
+template <typename T>
+int launch_viscous(const KP&, T* Fvx, T* Fvy, T* Fvz, const T* Q, const MetricPtrs<T>& m, const double* J,
+                   hipStream_t);
+template <typename T>
+int launch_div(const KP&, double* U, const T* Fx, const T* Fy, const T* Fz, const T* Fvx, const T* Fvy,
+               const T* Fvz, double dt, const double* J, hipStream_t);
+int launch_lin_comb(const KP&, double* U, const double* Un, int NV, double a, double b, hipStream_t);
+template <typename T> int launch_c2prim(const KP&, const double* U, T* Q, hipStream_t);
+template <typename T> int launch_prim2c(const KP&, double* U, const T* Q, hipStream_t);
+template <typename T> int launch_exchange(const KP&, T* Q, int NV, hipStream_t);
+template <typename T> int launch_fill(const KP&, T* Q, double* U, const T* inflow, hipStream_t);
+
+}  // namespace f3d

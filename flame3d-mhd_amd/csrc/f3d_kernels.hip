@@ -1,0 +1,445 @@
+// Kernel-by-kernel drop-ins for the reference's per-stage launches (solver.jl:416-444,
+// 133-188): one thread per cell or face, 64x4 blocks along (i, j) so each wave reads 64
+// consecutive i (coalesced rows of the column-major arrays).
+#include "f3d_internal.hpp"
+
+namespace f3d {
+
+struct Range3 {
+  int i0, j0, k0, ni, nj, nk;
+};
+
+#define RANGE_IDX(r)                                       \
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;    \
+  const int jj = blockIdx.y * blockDim.y + threadIdx.y;    \
+  const int kk = blockIdx.z;                               \
+  if (ii >= (r).ni || jj >= (r).nj || kk >= (r).nk) return; \
+  const int i = (r).i0 + ii, j = (r).j0 + jj, k = (r).k0 + kk;
+
+static inline dim3 range_grid(const Range3& r) {
+  return dim3((unsigned)((r.ni + 63) / 64), (unsigned)((r.nj + 3) / 4), (unsigned)r.nk);
+}
+static const dim3 kBlock(64, 4, 1);
+
+// ------------------------------------------------------------------ shockSensor  schemes.jl:1-24
+template <typename T>
+__global__ __launch_bounds__(256) void k_shock_sensor(KP kp, Range3 r, T* __restrict__ phi,
+                                                      const T* __restrict__ Q) {
+  RANGE_IDX(r);
+  const T* P = Q + 4 * kp.nc;
+  const long long o = cidx(kp, i, j, k), sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
+  phi[o] = sensor_cell<T>(P[o - 1], P[o], P[o + 1], P[o - sy], P[o + sy], P[o - sz], P[o + sz]);
+}
+
+// ------------------------------------------------------------------ scheme mask (per face)
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_scheme_mask(KP kp, Range3 r, uint8_t* __restrict__ mask,
+                                                     const T* __restrict__ phi) {
+  RANGE_IDX(r);
+  const long long st = dstride(kp, D), o = cidx(kp, i, j, k);
+  T m = phi[o - 3 * st];
+#pragma unroll
+  for (int s = -2; s <= 2; ++s) m = jmax(m, phi[o + s * st]);
+  mask[ii + (long long)r.ni * (jj + (long long)r.nj * kk)] = (uint8_t)scheme_of<T>(kp, m);
+}
+
+// ------------------------------------------------------------------ fluxSplit_SW  split.jl:2-68
+// SFROM_A: S recomputed as |A| instead of read from an array.
+template <typename T, bool SFROM_A>
+__global__ __launch_bounds__(256) void k_flux_split(KP kp, Range3 r, const T* __restrict__ Q, T* __restrict__ Fp,
+                                                    T* __restrict__ Fm, const T* __restrict__ S,
+                                                    const T* __restrict__ Ax, const T* __restrict__ Ay,
+                                                    const T* __restrict__ Az) {
+  RANGE_IDX(r);
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+  const T A1 = Ax[o], A2 = Ay[o], A3 = Az[o];
+  const T ss = SFROM_A ? sqrt(A1 * A1 + A2 * A2 + A3 * A3) : S[o];
+  T fp[5], fm[5];
+  sw_split<T>(kp, Q[o], Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], A1, A2, A3, ss, fp, fm);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    Fp[o + n * nc] = fp[n];
+    Fm[o + n * nc] = fm[n];
+  }
+}
+
+// ------------------------------------------------------------------ advect_{x,y,z}c
+template <typename T, int D, bool SFROM_A>
+__global__ __launch_bounds__(256) void k_advect(KP kp, Range3 r, T* __restrict__ F, const T* __restrict__ phi,
+                                                const T* __restrict__ S, const T* __restrict__ Fp,
+                                                const T* __restrict__ Fm, const T* __restrict__ Q,
+                                                const T* __restrict__ Ax, const T* __restrict__ Ay,
+                                                const T* __restrict__ Az) {
+  RANGE_IDX(r);
+  const long long st = dstride(kp, D), o = cidx(kp, i, j, k), nc = kp.nc;
+  T m = phi[o - 3 * st];
+#pragma unroll
+  for (int s = -2; s <= 2; ++s) m = jmax(m, phi[o + s * st]);
+  const int sch = scheme_of<T>(kp, m);
+  auto FP = [&](int s, int n) { return Fp[o + s * st + n * nc]; };
+  auto FM = [&](int s, int n) { return Fm[o + s * st + n * nc]; };
+  const long long oL = o - st;
+  const T qL[4] = {Q[oL + nc], Q[oL + 2 * nc], Q[oL + 3 * nc], Q[oL + 5 * nc]};
+  const T qR[4] = {Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc]};
+  const T AL[3] = {Ax[oL], Ay[oL], Az[oL]};
+  const T AR[3] = {Ax[o], Ay[o], Az[o]};
+  const T SL = SFROM_A ? sqrt(AL[0] * AL[0] + AL[1] * AL[1] + AL[2] * AL[2]) : S[oL];
+  const T SR = SFROM_A ? sqrt(AR[0] * AR[0] + AR[1] * AR[1] + AR[2] * AR[2]) : S[o];
+  T out[5];
+  face_flux<T>(kp, sch, FP, FM, qL, qR, AL, AR, SL, SR, out);
+  const long long fo = ii + (long long)r.ni * (jj + (long long)r.nj * kk), fn = (long long)r.ni * r.nj * r.nk;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) F[fo + n * fn] = out[n];
+}
+
+// ------------------------------------------------------------------ viscousFlux  viscous.jl:1-134
+template <typename T>
+__global__ __launch_bounds__(256) void k_viscous(KP kp, Range3 r, T* __restrict__ Fvx, T* __restrict__ Fvy,
+                                                 T* __restrict__ Fvz, const T* __restrict__ Q, MetricPtrs<T> mp,
+                                                 const double* __restrict__ J) {
+  RANGE_IDX(r);
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+  const long long st[3] = {1, kp.Nxt, (long long)kp.Nxt * kp.Nyt};
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  T g[4][3];
+  const int var[4] = {1, 2, 3, 5};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const T* f = Q + var[q] * nc + o;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      g[q][d] = c1 * (f[-2 * st[d]] - f[2 * st[d]]) + c2 * (f[-st[d]] - f[st[d]]);
+  }
+  T m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = mp.m[q][o];
+  T out[3][4];
+  viscous_cell<T>(kp, Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], g, m, (T)J[o], out);
+  const int Nv1 = kp.Nxp + kp.NG, Nv2 = kp.Nyp + kp.NG;
+  const long long vn = (long long)Nv1 * Nv2 * (kp.Nzp + kp.NG);
+  const long long vo = (i - 2) + (long long)Nv1 * ((j - 2) + (long long)Nv2 * (k - 2));
+  T* Fo[3] = {Fvx, Fvy, Fvz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) Fo[d][vo + n * vn] = out[d][n];
+}
+
+// ------------------------------------------------------------------ div  div.jl:1-50
+// Range is over interior cells (0-based i in [0, Nxp)).
+template <typename T>
+__global__ __launch_bounds__(256) void k_div(KP kp, Range3 r, double* __restrict__ U, const T* __restrict__ Fx,
+                                             const T* __restrict__ Fy, const T* __restrict__ Fz,
+                                             const T* __restrict__ Fvx, const T* __restrict__ Fvy,
+                                             const T* __restrict__ Fvz, double dt, const double* __restrict__ J) {
+  RANGE_IDX(r);
+  const int NG = kp.NG, Nxp = kp.Nxp, Nyp = kp.Nyp, Nzp = kp.Nzp;
+  const long long o = cidx(kp, i + NG, j + NG, k + NG), nc = kp.nc;
+  const double Jact = J[o] * dt;
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  const int Nv1 = Nxp + NG, Nv2 = Nyp + NG;
+  const long long vn = (long long)Nv1 * Nv2 * (Nzp + NG);
+  const long long vo = (i + 2) + (long long)Nv1 * ((j + 2) + (long long)Nv2 * (k + 2));
+  const long long vs[3] = {1, Nv1, (long long)Nv1 * Nv2};
+  const T* Fv[3] = {Fvx, Fvy, Fvz};
+  const long long fxn = (long long)(Nxp + 1) * Nyp * Nzp, fyn = (long long)Nxp * (Nyp + 1) * Nzp,
+                  fzn = (long long)Nxp * Nyp * (Nzp + 1);
+  const long long ox = i + (long long)(Nxp + 1) * (j + (long long)Nyp * k);
+  const long long oy = i + (long long)Nxp * (j + (long long)(Nyp + 1) * k);
+  const long long oz = i + (long long)Nxp * (j + (long long)Nyp * k);
+  const long long szf = (long long)Nxp * Nyp;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const T s = Fx[ox + n * fxn] - Fx[ox + 1 + n * fxn] + Fy[oy + n * fyn] - Fy[oy + Nxp + n * fyn] +
+                Fz[oz + n * fzn] - Fz[oz + szf + n * fzn];
+    U[o + n * nc] += (double)s * Jact;
+  }
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    T s = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T* f = Fv[d] + vo + n * vn;
+      s += c1 * (f[-2 * vs[d]] - f[2 * vs[d]]) + c2 * (f[-vs[d]] - f[vs[d]]);
+    }
+    U[o + (n + 1) * nc] += (double)s * Jact;
+  }
+}
+
+// ------------------------------------------------------------------ linComb, c2Prim, prim2c
+__global__ __launch_bounds__(256) void k_lin_comb(KP kp, Range3 r, double* __restrict__ U,
+                                                  const double* __restrict__ Un, int NV, double a, double b) {
+  RANGE_IDX(r);
+  const long long o = cidx(kp, i, j, k);
+  for (int n = 0; n < NV; ++n) U[o + n * kp.nc] = U[o + n * kp.nc] * a + Un[o + n * kp.nc] * b;
+}
+
+template <typename T>
+__device__ __forceinline__ void c2prim_cell(const KP& kp, const double* __restrict__ U, T* __restrict__ Q,
+                                            long long o) {
+  const long long nc = kp.nc;
+  const double eps = 2.220446049250313e-16;  // machine epsilon of double, the type c2Prim runs in
+  const double rho = jmax(U[o], eps);
+  const double ri = 1.0 / rho;
+  const double u = U[o + nc] * ri, v = U[o + 2 * nc] * ri, w = U[o + 3 * nc] * ri;
+  const double ei = jmax(U[o + 4 * nc] - 0.5 * rho * (u * u + v * v + w * w), eps);
+  const double p = (kp.gamma - 1) * ei;
+  const double Tt = p / (rho * kp.Rg);
+  Q[o] = (T)rho;
+  Q[o + nc] = (T)u;
+  Q[o + 2 * nc] = (T)v;
+  Q[o + 3 * nc] = (T)w;
+  Q[o + 4 * nc] = (T)p;
+  Q[o + 5 * nc] = (T)Tt;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_c2prim(KP kp, Range3 r, const double* __restrict__ U, T* __restrict__ Q) {
+  RANGE_IDX(r);
+  c2prim_cell<T>(kp, U, Q, cidx(kp, i, j, k));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prim2c(KP kp, Range3 r, double* __restrict__ U, const T* __restrict__ Q) {
+  RANGE_IDX(r);
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+  const double rho = Q[o], u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc];
+  const double ei = (double)Q[o + 4 * nc] / (kp.gamma - 1);
+  U[o] = rho;
+  U[o + nc] = u * rho;
+  U[o + 2 * nc] = v * rho;
+  U[o + 3 * nc] = w * rho;
+  U[o + 4 * nc] = ei + 0.5 * rho * (u * u + v * v + w * w);
+}
+
+// ------------------------------------------------------------------ ghost cells
+// Periodic exchange of one dimension D (mpi.jl:2-103 with one rank): range covers both NG-thick
+// ghost slabs (2*NG layers along D) and the full extent of the other two dimensions.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_exchange(KP kp, Range3 r, T* __restrict__ Q, int NV) {
+  RANGE_IDX(r);
+  const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+  int idx = D == 0 ? i : (D == 1 ? j : k);
+  // map the packed range [0, 2NG) onto ghost layers [0,NG) and [Np+NG, Np+2NG)
+  const int g = idx >= kp.NG ? idx - kp.NG + Np + kp.NG : idx;
+  const int src = g < kp.NG ? g + Np : g - Np;
+  const int gi = D == 0 ? g : i, gj = D == 1 ? g : j, gk = D == 2 ? g : k;
+  const int si = D == 0 ? src : i, sj = D == 1 ? src : j, sk = D == 2 ? src : k;
+  const long long o = cidx(kp, gi, gj, gk), s = cidx(kp, si, sj, sk);
+  for (int n = 0; n < NV; ++n) Q[o + n * kp.nc] = Q[s + n * kp.nc];
+}
+
+// Ghost fill of dimension D for both faces (fill_x/fill_y/fill_z, boundary.jl:1-148 with the kinds
+// of f3d.h); same packed range as k_exchange.  U may be null.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_fill(KP kp, Range3 r, T* __restrict__ Q, double* __restrict__ U,
+                                              const T* __restrict__ inflow) {
+  RANGE_IDX(r);
+  const int NG = kp.NG, Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+  const int idx = D == 0 ? i : (D == 1 ? j : k);
+  const int side = idx >= NG ? 1 : 0;
+  const int g = side ? idx - NG + Np + NG : idx;
+  const int bc = kp.bc[2 * D + side];
+  const int gi = D == 0 ? g : i, gj = D == 1 ? g : j, gk = D == 2 ? g : k;
+  const long long o = cidx(kp, gi, gj, gk), nc = kp.nc;
+  auto at = [&](int t) { return cidx(kp, D == 0 ? t : i, D == 1 ? t : j, D == 2 ? t : k); };
+  double q[6];
+  if (bc == F3D_BC_PERIODIC || bc == F3D_BC_ZEROGRAD) {
+    const int t = bc == F3D_BC_PERIODIC ? (side ? g - Np : g + Np) : (side ? Np + NG - 1 : NG);
+    const long long s = at(t);
+#pragma unroll
+    for (int n = 0; n < 6; ++n) Q[o + n * nc] = Q[s + n * nc];
+    if (U)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) U[o + n * nc] = U[s + n * nc];
+    return;
+  } else if (bc == F3D_BC_DIRICHLET) {
+#pragma unroll
+    for (int n = 0; n < 6; ++n) q[n] = kp.qinf[n];
+  } else if (bc == F3D_BC_INFLOW) {
+    const long long io = j + (long long)kp.Nyt * k, in = (long long)kp.Nyt * kp.Nzt;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) q[n] = inflow[io + n * in];
+  } else {  // walls: mirror layer (boundary.jl:69), 0-based 2NG-1-g low, 2(Np+NG)-1-g high
+    const int mi = side ? 2 * (Np + NG) - 1 - g : 2 * NG - 1 - g;
+    const long long s = at(mi);
+    q[0] = Q[s];
+    q[1] = -(double)Q[s + nc];
+    q[2] = -(double)Q[s + 2 * nc];
+    q[3] = -(double)Q[s + 3 * nc];
+    q[4] = Q[s + 4 * nc];
+    q[5] = Q[s + 5 * nc];
+    if (bc == F3D_BC_WALL_ISOTHERMAL) {
+      q[5] = 2 * kp.Twall - q[5];
+      q[0] = q[4] / (kp.Rg * q[5]);
+    }
+  }
+  T qs[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    qs[n] = (T)q[n];
+    Q[o + n * nc] = qs[n];
+  }
+  if (U) {
+    const double rho = qs[0], u = qs[1], v = qs[2], w = qs[3], p = qs[4];
+    U[o] = rho;
+    U[o + nc] = rho * u;
+    U[o + 2 * nc] = rho * v;
+    U[o + 3 * nc] = rho * w;
+    U[o + 4 * nc] = p / (kp.gamma - 1) + 0.5 * rho * (u * u + v * v + w * w);
+  }
+}
+
+// ------------------------------------------------------------------ host launchers
+template <typename T>
+int launch_shock_sensor(const KP& kp, T* phi, const T* Q, hipStream_t s) {
+  TimedScope ts_("shock_sensor", s);
+  Range3 r{1, 1, 1, kp.Nxt - 2, kp.Nyt - 2, kp.Nzt - 2};
+  hipLaunchKernelGGL(k_shock_sensor<T>, range_grid(r), kBlock, 0, s, kp, r, phi, Q);
+  return check_launch("shock_sensor");
+}
+
+static Range3 face_range(const KP& kp, int d) {
+  return Range3{kp.NG, kp.NG, kp.NG, kp.Nxp + (d == 0), kp.Nyp + (d == 1), kp.Nzp + (d == 2)};
+}
+
+template <typename T>
+int launch_scheme_mask(const KP& kp, uint8_t* mx, uint8_t* my, uint8_t* mz, const T* phi, hipStream_t s) {
+  TimedScope ts_("scheme_mask", s);
+  if (mx) {
+    Range3 r = face_range(kp, 0);
+    hipLaunchKernelGGL((k_scheme_mask<T, 0>), range_grid(r), kBlock, 0, s, kp, r, mx, phi);
+  }
+  if (my) {
+    Range3 r = face_range(kp, 1);
+    hipLaunchKernelGGL((k_scheme_mask<T, 1>), range_grid(r), kBlock, 0, s, kp, r, my, phi);
+  }
+  if (mz) {
+    Range3 r = face_range(kp, 2);
+    hipLaunchKernelGGL((k_scheme_mask<T, 2>), range_grid(r), kBlock, 0, s, kp, r, mz, phi);
+  }
+  return check_launch("scheme_mask");
+}
+
+template <typename T>
+int launch_flux_split(const KP& kp, const T* Q, T* Fp, T* Fm, const T* S, const T* Ax, const T* Ay, const T* Az,
+                      hipStream_t s) {
+  TimedScope ts_("flux_split_sw", s);
+  Range3 r{0, 0, 0, kp.Nxt, kp.Nyt, kp.Nzt};
+  if (S)
+    hipLaunchKernelGGL((k_flux_split<T, false>), range_grid(r), kBlock, 0, s, kp, r, Q, Fp, Fm, S, Ax, Ay, Az);
+  else
+    hipLaunchKernelGGL((k_flux_split<T, true>), range_grid(r), kBlock, 0, s, kp, r, Q, Fp, Fm, S, Ax, Ay, Az);
+  return check_launch("flux_split_sw");
+}
+
+template <typename T>
+int launch_advect(const KP& kp, int d, T* F, const T* phi, const T* S, const T* Fp, const T* Fm, const T* Q,
+                  const T* Ax, const T* Ay, const T* Az, hipStream_t s) {
+  TimedScope ts_("advect", s);
+  Range3 r = face_range(kp, d);
+#define F3D_ADV(D)                                                                                         \
+  do {                                                                                                     \
+    if (S)                                                                                                 \
+      hipLaunchKernelGGL((k_advect<T, D, false>), range_grid(r), kBlock, 0, s, kp, r, F, phi, S, Fp, Fm, Q, \
+                         Ax, Ay, Az);                                                                      \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_advect<T, D, true>), range_grid(r), kBlock, 0, s, kp, r, F, phi, S, Fp, Fm, Q,  \
+                         Ax, Ay, Az);                                                                      \
+  } while (0)
+  if (d == 0) F3D_ADV(0);
+  else if (d == 1) F3D_ADV(1);
+  else F3D_ADV(2);
+#undef F3D_ADV
+  return check_launch("advect");
+}
+
+template <typename T>
+int launch_viscous(const KP& kp, T* Fvx, T* Fvy, T* Fvz, const T* Q, const MetricPtrs<T>& m, const double* J,
+                   hipStream_t s) {
+  TimedScope ts_("viscous_flux", s);
+  Range3 r{2, 2, 2, kp.Nxt - 4, kp.Nyt - 4, kp.Nzt - 4};
+  hipLaunchKernelGGL(k_viscous<T>, range_grid(r), kBlock, 0, s, kp, r, Fvx, Fvy, Fvz, Q, m, J);
+  return check_launch("viscous_flux");
+}
+
+template <typename T>
+int launch_div(const KP& kp, double* U, const T* Fx, const T* Fy, const T* Fz, const T* Fvx, const T* Fvy,
+               const T* Fvz, double dt, const double* J, hipStream_t s) {
+  TimedScope ts_("div", s);
+  Range3 r{0, 0, 0, kp.Nxp, kp.Nyp, kp.Nzp};
+  hipLaunchKernelGGL(k_div<T>, range_grid(r), kBlock, 0, s, kp, r, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, dt, J);
+  return check_launch("div");
+}
+
+int launch_lin_comb(const KP& kp, double* U, const double* Un, int NV, double a, double b, hipStream_t s) {
+  TimedScope ts_("lin_comb", s);
+  Range3 r{kp.NG, kp.NG, kp.NG, kp.Nxp, kp.Nyp, kp.Nzp};
+  hipLaunchKernelGGL(k_lin_comb, range_grid(r), kBlock, 0, s, kp, r, U, Un, NV, a, b);
+  return check_launch("lin_comb");
+}
+
+template <typename T>
+int launch_c2prim(const KP& kp, const double* U, T* Q, hipStream_t s) {
+  TimedScope ts_("c2prim", s);
+  Range3 r{kp.NG, kp.NG, kp.NG, kp.Nxp, kp.Nyp, kp.Nzp};
+  hipLaunchKernelGGL(k_c2prim<T>, range_grid(r), kBlock, 0, s, kp, r, U, Q);
+  return check_launch("c2prim");
+}
+
+template <typename T>
+int launch_prim2c(const KP& kp, double* U, const T* Q, hipStream_t s) {
+  TimedScope ts_("prim2c", s);
+  Range3 r{0, 0, 0, kp.Nxt, kp.Nyt, kp.Nzt};
+  hipLaunchKernelGGL(k_prim2c<T>, range_grid(r), kBlock, 0, s, kp, r, U, Q);
+  return check_launch("prim2c");
+}
+
+static Range3 slab_range(const KP& kp, int d) {
+  return Range3{0, 0, 0, d == 0 ? 2 * kp.NG : kp.Nxt, d == 1 ? 2 * kp.NG : kp.Nyt, d == 2 ? 2 * kp.NG : kp.Nzt};
+}
+
+template <typename T>
+int launch_exchange(const KP& kp, T* Q, int NV, hipStream_t s) {
+  TimedScope ts_("exchange_ghost", s);
+  for (int d = 0; d < 3; ++d) {
+    if (kp.bc[2 * d] != F3D_BC_PERIODIC || kp.bc[2 * d + 1] != F3D_BC_PERIODIC) continue;
+    Range3 r = slab_range(kp, d);
+    if (d == 0) hipLaunchKernelGGL((k_exchange<T, 0>), range_grid(r), kBlock, 0, s, kp, r, Q, NV);
+    if (d == 1) hipLaunchKernelGGL((k_exchange<T, 1>), range_grid(r), kBlock, 0, s, kp, r, Q, NV);
+    if (d == 2) hipLaunchKernelGGL((k_exchange<T, 2>), range_grid(r), kBlock, 0, s, kp, r, Q, NV);
+  }
+  return check_launch("exchange_ghost");
+}
+
+template <typename T>
+int launch_fill(const KP& kp, T* Q, double* U, const T* inflow, hipStream_t s) {
+  TimedScope ts_("fill_ghost", s);
+  for (int d = 0; d < 3; ++d) {
+    Range3 r = slab_range(kp, d);
+    if (d == 0) hipLaunchKernelGGL((k_fill<T, 0>), range_grid(r), kBlock, 0, s, kp, r, Q, U, inflow);
+    if (d == 1) hipLaunchKernelGGL((k_fill<T, 1>), range_grid(r), kBlock, 0, s, kp, r, Q, U, inflow);
+    if (d == 2) hipLaunchKernelGGL((k_fill<T, 2>), range_grid(r), kBlock, 0, s, kp, r, Q, U, inflow);
+  }
+  return check_launch("fill_ghost");
+}
+
+#define F3D_INST(T)                                                                                              \
+  template int launch_shock_sensor<T>(const KP&, T*, const T*, hipStream_t);                                    \
+  template int launch_scheme_mask<T>(const KP&, uint8_t*, uint8_t*, uint8_t*, const T*, hipStream_t);           \
+  template int launch_flux_split<T>(const KP&, const T*, T*, T*, const T*, const T*, const T*, const T*,        \
+                                    hipStream_t);                                                               \
+  template int launch_advect<T>(const KP&, int, T*, const T*, const T*, const T*, const T*, const T*, const T*, \
+                                const T*, const T*, hipStream_t);                                               \
+  template int launch_viscous<T>(const KP&, T*, T*, T*, const T*, const MetricPtrs<T>&, const double*,          \
+                                 hipStream_t);                                                                  \
+  template int launch_div<T>(const KP&, double*, const T*, const T*, const T*, const T*, const T*, const T*,    \
+                             double, const double*, hipStream_t);                                               \
+  template int launch_c2prim<T>(const KP&, const double*, T*, hipStream_t);                                     \
+  template int launch_prim2c<T>(const KP&, double*, const T*, hipStream_t);                                     \
+  template int launch_exchange<T>(const KP&, T*, int, hipStream_t);                                             \
+  template int launch_fill<T>(const KP&, T*, double*, const T*, hipStream_t);
+F3D_INST(double)
+F3D_INST(float)
+
+}  // namespace f3d

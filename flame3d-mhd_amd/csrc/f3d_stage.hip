@@ -1,0 +1,529 @@
+// Workspace, RHS and RK3 stage entry points, global reductions (CFL dt, NaN flag) and the FP64
+// metric computation.
+#include <cmath>
+#include <cstring>
+
+#include "f3d_internal.hpp"
+#include "f3d_fused.hpp"
+
+using namespace f3d;
+
+struct f3d_workspace {
+  f3d_params p;
+  int prec;
+  size_t bytes;
+  void* base;
+  // bulk-type scratch (element size 8 for F3D_PREC_F64, 4 for F3D_PREC_F32B)
+  void *phi, *Fp, *Fm, *Fx, *Fy, *Fz, *Fvx, *Fvy, *Fvz;
+  void* R;       // interior RHS accumulator [5][Nzp][Nyp][Nxp] of the bulk type
+  double* red;   // reduction partials
+  int* flag;     // NaN flag partials
+  int nred;      // number of partial slots
+};
+
+namespace {
+
+constexpr int kRedBlocks = 2048;
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+bool same_params(const f3d_params* a, const f3d_params* b) { return std::memcmp(a, b, sizeof(f3d_params)) == 0; }
+
+int check_ws(const f3d_params* p, const f3d_workspace* ws, int prec, const char* who) {
+  if (!ws) {
+    set_error("%s: workspace is NULL", who);
+    return F3D_E_NULL;
+  }
+  if (ws->prec != prec) {
+    set_error("%s: workspace precision %d does not match the entry point", who, ws->prec);
+    return F3D_E_WS;
+  }
+  if (p->Nxp != ws->p.Nxp || p->Nyp != ws->p.Nyp || p->Nzp != ws->p.Nzp || p->NG != ws->p.NG) {
+    set_error("%s: workspace was created for another grid", who);
+    return F3D_E_WS;
+  }
+  return F3D_OK;
+}
+
+__global__ __launch_bounds__(256) void k_zero(double* __restrict__ a, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    a[t] = 0.0;
+}
+
+// Zero the interior of a U-shaped array (5 vars).
+__global__ __launch_bounds__(256) void k_zero_interior(KP kp, double* __restrict__ R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxp) return;
+  const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) R[o + n * kp.nc] = 0.0;
+}
+
+// ---------------------------------------------------------------- wave64 reductions
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v |= __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_min(double v) {
+  __shared__ double sm[NT / 64];
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sm[wid] = v;
+  __syncthreads();
+  if (wid == 0) {
+    v = lane < NT / 64 ? sm[lane] : INFINITY;
+    v = wave_min(v);
+  }
+  return v;
+}
+
+// Global CFL dt, stage 1: per-block minima of 1/(J * max_d(|A_d.u| + c|A_d|)) over interior cells.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cfl_partial(KP kp, const T* __restrict__ Q, MetricPtrs<T> mp,
+                                                     const double* __restrict__ J, double* __restrict__ part) {
+  const long long n = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  double best = INFINITY;
+  for (long long t = blockIdx.x * 256LL + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+    const int i = (int)(t % kp.Nxp), j = (int)((t / kp.Nxp) % kp.Nyp), k = (int)(t / ((long long)kp.Nxp * kp.Nyp));
+    const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG);
+    const double u = Q[o + kp.nc], v = Q[o + 2 * kp.nc], w = Q[o + 3 * kp.nc], Tt = Q[o + 5 * kp.nc];
+    const double c = sqrt(kp.gamma * kp.Rg * Tt);
+    double lam = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double a = mp.m[3 * d][o], b = mp.m[3 * d + 1][o], e = mp.m[3 * d + 2][o];
+      lam = fmax(lam, fabs(a * u + b * v + e * w) + c * sqrt(a * a + b * b + e * e));
+    }
+    best = fmin(best, 1.0 / (J[o] * lam));
+  }
+  best = block_min<256>(best);
+  if (threadIdx.x == 0) part[blockIdx.x] = best;
+}
+
+__global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ part, int n, double cfl,
+                                                    double* __restrict__ out) {
+  double v = INFINITY;
+  for (int t = threadIdx.x; t < n; t += 1024) v = fmin(v, part[t]);
+  v = block_min<1024>(v);
+  if (threadIdx.x == 0) out[0] = cfl * v;
+}
+
+__global__ __launch_bounds__(256) void k_nan_partial(const double* __restrict__ U, long long n, int* __restrict__ part) {
+  int bad = 0;
+  for (long long t = blockIdx.x * 256LL + threadIdx.x; t < n; t += (long long)gridDim.x * 256)
+    bad |= !isfinite(U[t]);
+  __shared__ int sm[4];
+  bad = wave_or(bad);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = bad;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0] | sm[1] | sm[2] | sm[3];
+}
+
+__global__ __launch_bounds__(256) void k_nan_final(const int* __restrict__ part, int n, int* __restrict__ out) {
+  int bad = 0;
+  for (int t = threadIdx.x; t < n; t += 256) bad |= part[t];
+  __shared__ int sm[4];
+  bad = wave_or(bad);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = bad;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = sm[0] | sm[1] | sm[2] | sm[3];
+}
+
+// ---------------------------------------------------------------- metrics  parse_mesh.jl:139-248
+__device__ __forceinline__ double dcoord(const double* __restrict__ f, long long o, long long st, int idx, int N) {
+  auto F = [&](int s) { return f[o + s * st]; };
+  if (idx >= 3 && idx <= N - 4)  // CD6 on 0-based [3, N-4]  (1-based [4, N-3])
+    return 1.0 / 60 * (F(3) - F(-3)) - 3.0 / 20 * (F(2) - F(-2)) + 3.0 / 4 * (F(1) - F(-1));
+  if (idx <= 2) return 2 * F(1) - 0.5 * F(2) - 1.5 * F(0);  // CD2_L
+  return -2 * F(-1) + 0.5 * F(-2) + 1.5 * F(0);             // CD2_R
+}
+
+__global__ __launch_bounds__(256) void k_metrics(KP kp, const double* __restrict__ x, const double* __restrict__ y,
+                                                 const double* __restrict__ z, MetricPtrs<double> outm,
+                                                 double* __restrict__ J, MetricPtrs<double> outs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxt || j >= kp.Nyt || k >= kp.Nzt) return;
+  const long long o = cidx(kp, i, j, k);
+  const long long st[3] = {1, kp.Nxt, (long long)kp.Nxt * kp.Nyt};
+  const int idx[3] = {i, j, k}, N[3] = {kp.Nxt, kp.Nyt, kp.Nzt};
+  const double* X[3] = {x, y, z};
+  double d[3][3];  // d[c][dir] = d(x_c)/d(xi_dir)
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int e = 0; e < 3; ++e) d[c][e] = dcoord(X[c], o, st[e], idx[e], N[e]);
+  const double xa = d[0][0], xb = d[0][1], xc = d[0][2], ya = d[1][0], yb = d[1][1], yc = d[1][2];
+  const double za = d[2][0], zb = d[2][1], zc = d[2][2];
+  J[o] = 1 / (xa * (yb * zc - yc * zb) - xb * (ya * zc - yc * za) + xc * (ya * zb - yb * za));
+  double m[9];
+  m[0] = yb * zc - yc * zb;
+  m[1] = xc * zb - xb * zc;
+  m[2] = xb * yc - xc * yb;
+  m[3] = yc * za - ya * zc;
+  m[4] = xa * zc - xc * za;
+  m[5] = xc * ya - xa * yc;
+  m[6] = ya * zb - yb * za;
+  m[7] = xb * za - xa * zb;
+  m[8] = xa * yb - xb * ya;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) ((double*)outm.m[q])[o] = m[q];
+#pragma unroll
+  for (int e = 0; e < 3; ++e)
+    if (outs.m[e]) ((double*)outs.m[e])[o] = sqrt(m[3 * e] * m[3 * e] + m[3 * e + 1] * m[3 * e + 1] + m[3 * e + 2] * m[3 * e + 2]);
+}
+
+// ---------------------------------------------------------------- RHS and stage
+template <typename T>
+int rhs_chain(f3d_workspace* ws, const KP& kp, const T* Q, const MetricPtrs<T>& m, const double* J, T* phi,
+              hipStream_t s) {
+  T* Fp = (T*)ws->Fp;
+  T* Fm = (T*)ws->Fm;
+  T* F[3] = {(T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz};
+  int rc = launch_shock_sensor<T>(kp, phi, Q, s);
+  for (int d = 0; d < 3 && !rc; ++d) {
+    rc = launch_flux_split<T>(kp, Q, Fp, Fm, nullptr, m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
+    if (!rc)
+      rc = launch_advect<T>(kp, d, F[d], phi, nullptr, Fp, Fm, Q, m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
+  }
+  if (!rc && kp.viscous) rc = launch_viscous<T>(kp, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, Q, m, J, s);
+  return rc;
+}
+
+template <typename T>
+int rhs_impl(const f3d_params* p, f3d_workspace* ws, double* R, const T* Q, const T* const* metrics,
+             const double* J, T* phi, hipStream_t s) {
+  KP kp = make_kp(p);
+  MetricPtrs<T> m;
+  for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
+  T* ph = phi ? phi : (T*)ws->phi;
+  int rc;
+  if (use_fused()) {
+    rc = fused_rhs<T>(kp, ws_view<T>(ws), Q, m, J, ph, R, 0, 0.0, nullptr, s);
+  } else {
+    rc = rhs_chain<T>(ws, kp, Q, m, J, ph, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_zero_interior, dim3((kp.Nxp + 255) / 256, kp.Nyp, kp.Nzp), dim3(256), 0, s, kp, R);
+    rc = launch_div<T>(kp, R, (T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, 1.0, J, s);
+  }
+  return rc;
+}
+
+template <typename T>
+int stage_impl(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, T* Q,
+               const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s) {
+  KP kp = make_kp(p);
+  MetricPtrs<T> m;
+  for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
+  int rc = F3D_OK;
+  if (use_fused()) {
+    rc = fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Q, m, J, dt, s);
+  } else {
+    if (KRK == 1) F3D_HIP(hipMemcpyAsync(Un, U, sizeof(double) * 5 * kp.nc, hipMemcpyDeviceToDevice, s));  // solver.jl:418
+    rc = rhs_chain<T>(ws, kp, Q, m, J, (T*)ws->phi, s);
+    if (!rc)
+      rc = launch_div<T>(kp, U, (T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, dt, J, s);
+    if (!rc && KRK == 2) rc = launch_lin_comb(kp, U, Un, 5, 0.25, 0.75, s);
+    if (!rc && KRK == 3) rc = launch_lin_comb(kp, U, Un, 5, 2.0 / 3.0, 1.0 / 3.0, s);
+    if (!rc) rc = launch_c2prim<T>(kp, U, Q, s);
+  }
+  if (!rc) rc = launch_exchange<T>(kp, Q, 6, s);
+  // U ghosts are never read on this path; only Q ghosts are filled by the stage.
+  if (!rc) rc = launch_fill<T>(kp, Q, nullptr, inflow, s);
+  return rc;
+}
+
+template <typename T>
+int check_stage_args(const f3d_params* p, const f3d_workspace* ws, int prec, const void* U, const void* Un,
+                     const void* Q, const T* const* metrics, const double* J, const T* inflow, const char* who) {
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  rc = check_ws(p, ws, prec, who);
+  if (rc) return rc;
+  if (!U || !Un || !Q || !metrics || !J) {
+    set_error("%s: NULL array argument", who);
+    return F3D_E_NULL;
+  }
+  for (int q = 0; q < 9; ++q)
+    if (!metrics[q]) {
+      set_error("%s: metric %d is NULL", who, q);
+      return F3D_E_NULL;
+    }
+  if (p->bc[0] == F3D_BC_INFLOW && !inflow) {
+    set_error("%s: F3D_BC_INFLOW needs an inflow profile", who);
+    return F3D_E_NULL;
+  }
+  return F3D_OK;
+}
+
+}  // namespace
+
+// ================================================================ extern "C"
+#define F3D_API extern "C" __attribute__((visibility("default")))
+
+F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_workspace** out) {
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!out) {
+    set_error("workspace_create: out is NULL");
+    return F3D_E_NULL;
+  }
+  if (precision != F3D_PREC_F64 && precision != F3D_PREC_F32B) {
+    set_error("workspace_create: unknown precision %d", precision);
+    return F3D_E_ARG;
+  }
+  *out = nullptr;
+  const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
+  const size_t nc = f3d_cell_count(p);
+  const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
+  size_t sz[12];
+  sz[0] = nc * es;                               // phi
+  sz[1] = nc * 5 * es;                           // Fp
+  sz[2] = nc * 5 * es;                           // Fm
+  sz[3] = (Nx + 1) * Ny * Nz * 5 * es;           // Fx
+  sz[4] = Nx * (Ny + 1) * Nz * 5 * es;           // Fy
+  sz[5] = Nx * Ny * (Nz + 1) * 5 * es;           // Fz
+  const size_t nv = (Nx + NG) * (Ny + NG) * (Nz + NG) * 4 * es;
+  sz[6] = sz[7] = sz[8] = nv;                    // Fv_x, Fv_y, Fv_z
+  sz[9] = Nx * Ny * Nz * 5 * es;                 // R
+  sz[10] = kRedBlocks * sizeof(double);          // red
+  sz[11] = kRedBlocks * sizeof(int);             // flag
+  size_t total = 0;
+  for (size_t b : sz) total += align_up(b);
+  void* base = nullptr;
+  F3D_HIP(hipMalloc(&base, total));
+  f3d_workspace* ws = new f3d_workspace();
+  ws->p = *p;
+  ws->prec = precision;
+  ws->bytes = total;
+  ws->base = base;
+  char* c = (char*)base;
+  void** slots[12] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+                      &ws->R, (void**)&ws->red, (void**)&ws->flag};
+  for (int i = 0; i < 12; ++i) {
+    *slots[i] = c;
+    c += align_up(sz[i]);
+  }
+  ws->nred = kRedBlocks;
+  // zeroed once: inviscid runs read zero viscous fluxes in div (div.jl:45-48 adds them always)
+  hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, 0, (double*)base, (long long)(total / 8));
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    hipFree(base);
+    delete ws;
+    set_error("workspace_create: zero fill failed: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  *out = ws;
+  return F3D_OK;
+}
+
+F3D_API int f3d_workspace_destroy(f3d_workspace* ws) {
+  if (!ws) return F3D_OK;
+  hipError_t e = hipFree(ws->base);
+  delete ws;
+  if (e != hipSuccess) {
+    set_error("workspace_destroy: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  return F3D_OK;
+}
+
+F3D_API size_t f3d_workspace_bytes(const f3d_workspace* ws) { return ws ? ws->bytes : 0; }
+
+namespace f3d {
+template <typename T>
+WsView<T> ws_view(f3d_workspace* ws) {
+  WsView<T> v;
+  v.R = (T*)ws->R;
+  v.Fvx = (T*)ws->Fvx;
+  v.Fvy = (T*)ws->Fvy;
+  v.Fvz = (T*)ws->Fvz;
+  v.phi = (T*)ws->phi;
+  return v;
+}
+template WsView<double> ws_view<double>(f3d_workspace*);
+template WsView<float> ws_view<float>(f3d_workspace*);
+}  // namespace f3d
+
+F3D_API int f3d_rhs_f64(const f3d_params* p, f3d_workspace* ws, double* R, const double* Q,
+                        const double* const* metrics, const double* J, double* phi, f3d_stream_t s) {
+  int rc = check_stage_args<double>(p, ws, F3D_PREC_F64, R, R, Q, metrics, J, (const double*)1, "rhs_f64");
+  if (rc) return rc;
+  return rhs_impl<double>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s);
+}
+F3D_API int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, const float* Q,
+                         const float* const* metrics, const double* J, float* phi, f3d_stream_t s) {
+  int rc = check_stage_args<float>(p, ws, F3D_PREC_F32B, R, R, Q, metrics, J, (const float*)1, "rhs_f32b");
+  if (rc) return rc;
+  return rhs_impl<float>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s);
+}
+
+F3D_API int f3d_rk3_stage_f64(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U, double* Un,
+                              double* Q, const double* const* metrics, const double* J, double dt,
+                              const double* inflow, f3d_stream_t s) {
+  int rc = check_stage_args<double>(p, ws, F3D_PREC_F64, U, Un, Q, metrics, J, inflow, "rk3_stage_f64");
+  if (rc) return rc;
+  if (KRK < 1 || KRK > 3) {
+    set_error("rk3_stage_f64: KRK=%d not in 1..3", KRK);
+    return F3D_E_ARG;
+  }
+  return stage_impl<double>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
+}
+F3D_API int f3d_rk3_stage_f32b(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U, double* Un,
+                               float* Q, const float* const* metrics, const double* J, double dt,
+                               const float* inflow, f3d_stream_t s) {
+  int rc = check_stage_args<float>(p, ws, F3D_PREC_F32B, U, Un, Q, metrics, J, inflow, "rk3_stage_f32b");
+  if (rc) return rc;
+  if (KRK < 1 || KRK > 3) {
+    set_error("rk3_stage_f32b: KRK=%d not in 1..3", KRK);
+    return F3D_E_ARG;
+  }
+  return stage_impl<float>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
+}
+
+F3D_API int f3d_time_step_f64(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U, double* Un,
+                              double* Q, const double* const* metrics, const double* J, double dt,
+                              const double* inflow, f3d_stream_t s) {
+  int rc = check_stage_args<double>(p, ws, F3D_PREC_F64, U, Un, Q, metrics, J, inflow, "time_step_f64");
+  if (rc) return rc;
+  if (nsteps < 0) {
+    set_error("time_step_f64: nsteps < 0");
+    return F3D_E_ARG;
+  }
+  for (int t = 0; t < nsteps; ++t)
+    for (int KRK = 1; KRK <= 3; ++KRK) {
+      rc = stage_impl<double>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
+      if (rc) return rc;
+    }
+  return F3D_OK;
+}
+F3D_API int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U, double* Un,
+                               float* Q, const float* const* metrics, const double* J, double dt,
+                               const float* inflow, f3d_stream_t s) {
+  int rc = check_stage_args<float>(p, ws, F3D_PREC_F32B, U, Un, Q, metrics, J, inflow, "time_step_f32b");
+  if (rc) return rc;
+  if (nsteps < 0) {
+    set_error("time_step_f32b: nsteps < 0");
+    return F3D_E_ARG;
+  }
+  for (int t = 0; t < nsteps; ++t)
+    for (int KRK = 1; KRK <= 3; ++KRK) {
+      rc = stage_impl<float>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
+      if (rc) return rc;
+    }
+  return F3D_OK;
+}
+
+template <typename T>
+static int cfl_impl(const f3d_params* p, f3d_workspace* ws, int prec, double* dt_out, const T* Q,
+                    const T* const* metrics, const double* J, double cfl, hipStream_t s, const char* who) {
+  int rc = check_stage_args<T>(p, ws, prec, dt_out, dt_out, Q, metrics, J, (const T*)1, who);
+  if (rc) return rc;
+  if (!(cfl > 0)) {
+    set_error("%s: cfl must be > 0", who);
+    return F3D_E_ARG;
+  }
+  KP kp = make_kp(p);
+  MetricPtrs<T> m;
+  for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
+  const long long n = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  const int nb = (int)std::min<long long>(ws->nred, (n + 255) / 256);
+  hipLaunchKernelGGL(k_cfl_partial<T>, dim3(nb), dim3(256), 0, s, kp, Q, m, J, ws->red);
+  hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(1024), 0, s, ws->red, nb, cfl, dt_out);
+  return check_launch(who);
+}
+
+F3D_API int f3d_cfl_dt_f64(const f3d_params* p, f3d_workspace* ws, double* dt_out, const double* Q,
+                           const double* const* metrics, const double* J, double cfl, f3d_stream_t s) {
+  return cfl_impl<double>(p, ws, F3D_PREC_F64, dt_out, Q, metrics, J, cfl, (hipStream_t)s, "cfl_dt_f64");
+}
+F3D_API int f3d_cfl_dt_f32b(const f3d_params* p, f3d_workspace* ws, double* dt_out, const float* Q,
+                            const float* const* metrics, const double* J, double cfl, f3d_stream_t s) {
+  return cfl_impl<float>(p, ws, F3D_PREC_F32B, dt_out, Q, metrics, J, cfl, (hipStream_t)s, "cfl_dt_f32b");
+}
+
+F3D_API int f3d_nan_check(const f3d_params* p, f3d_workspace* ws, int32_t* flag_out, const double* U, int32_t NV,
+                          f3d_stream_t s) {
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!ws || !flag_out || !U) {
+    set_error("nan_check: NULL argument");
+    return F3D_E_NULL;
+  }
+  if (NV < 1 || NV > 6) {
+    set_error("nan_check: NV=%d out of range", NV);
+    return F3D_E_ARG;
+  }
+  const long long n = (long long)f3d_cell_count(p) * NV;
+  const int nb = (int)std::min<long long>(ws->nred, (n + 255) / 256);
+  hipLaunchKernelGGL(k_nan_partial, dim3(nb), dim3(256), 0, (hipStream_t)s, U, n, ws->flag);
+  hipLaunchKernelGGL(k_nan_final, dim3(1), dim3(256), 0, (hipStream_t)s, ws->flag, nb, (int*)flag_out);
+  return check_launch("nan_check");
+}
+
+F3D_API int f3d_nan_check_sync(const f3d_params* p, f3d_workspace* ws, const double* U, int32_t NV,
+                               f3d_stream_t s) {
+  if (!ws) {
+    set_error("nan_check_sync: workspace is NULL");
+    return F3D_E_NULL;
+  }
+  // the flag lands in the last partial slot; the partials of this call use at most nred-1 slots
+  int32_t* dflag = ws->flag + ws->nred - 1;
+  const long long n = (long long)f3d_cell_count(p) * (NV > 0 ? NV : 1);
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!U) {
+    set_error("nan_check_sync: U is NULL");
+    return F3D_E_NULL;
+  }
+  if (NV < 1 || NV > 6) {
+    set_error("nan_check_sync: NV=%d out of range", NV);
+    return F3D_E_ARG;
+  }
+  const int nb = (int)std::min<long long>(ws->nred - 1, (n + 255) / 256);
+  hipLaunchKernelGGL(k_nan_partial, dim3(nb), dim3(256), 0, (hipStream_t)s, U, n, ws->flag);
+  hipLaunchKernelGGL(k_nan_final, dim3(1), dim3(256), 0, (hipStream_t)s, ws->flag, nb, dflag);
+  rc = check_launch("nan_check_sync");
+  if (rc) return rc;
+  int32_t h = 0;
+  F3D_HIP(hipMemcpyAsync(&h, dflag, sizeof(int32_t), hipMemcpyDeviceToHost, (hipStream_t)s));
+  F3D_HIP(hipStreamSynchronize((hipStream_t)s));
+  if (h) {
+    set_error("NaN or Inf detected in U (solver.jl:481-486)");
+    return F3D_E_NAN;
+  }
+  return F3D_OK;
+}
+
+F3D_API int f3d_metrics_f64(const f3d_params* p, const double* x, const double* y, const double* z,
+                            double* const* metrics, double* J, double* const* s_out, f3d_stream_t s) {
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!x || !y || !z || !metrics || !J) {
+    set_error("metrics_f64: NULL argument");
+    return F3D_E_NULL;
+  }
+  MetricPtrs<double> m, so;
+  for (int q = 0; q < 9; ++q) {
+    if (!metrics[q]) {
+      set_error("metrics_f64: metric %d is NULL", q);
+      return F3D_E_NULL;
+    }
+    m.m[q] = metrics[q];
+  }
+  for (int q = 0; q < 9; ++q) so.m[q] = nullptr;
+  if (s_out)
+    for (int e = 0; e < 3; ++e) so.m[e] = s_out[e];
+  KP kp = make_kp(p);
+  hipLaunchKernelGGL(k_metrics, dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4, kp.Nzt), dim3(64, 4, 1), 0,
+                     (hipStream_t)s, kp, x, y, z, m, J, so);
+  return check_launch("metrics_f64");
+}

@@ -1,0 +1,278 @@
+/*
+ * f3d.h - C-ABI of the MI355X (gfx950) Flame3D RHS + RK3 stage library (libf3d_hip.so).
+ *
+ * Replaces the finite-difference right-hand-side path of Flame3D.jl (rygan-code/Flame3D-MHD):
+ * shock sensor, Steger-Warming flux splitting, hybrid UP7/WENO7/WENO5/NND characteristic
+ * reconstruction, laminar viscous flux on J^-1-scaled curvilinear metrics, conservative flux
+ * divergence, SSP-RK3 blending, conserved<->primitive conversion and ghost-cell filling.
+ *
+ * Array layout (identical to the reference's CuArrays, solver.jl:294-308): Julia column-major,
+ * element (i,j,k,n) (0-based here) at offset  i + Nxt*(j + Nyt*(k + Nzt*n)),
+ * Nxt = Nxp+2*NG etc.  Cell arrays carry NG=4 ghost layers per side.
+ *   Q   [Nxt,Nyt,Nzt,6]  primitive (rho,u,v,w,p,T)          U, Un [Nxt,Nyt,Nzt,5] conserved
+ *   phi, s1..s3, 9 metrics, J, x,y,z  [Nxt,Nyt,Nzt]
+ *   Fp, Fm [Nxt,Nyt,Nzt,5]   Fx [Nxp+1,Nyp,Nzp,5]  Fy [Nxp,Nyp+1,Nzp,5]  Fz [Nxp,Nyp,Nzp+1,5]
+ *   Fv_x, Fv_y, Fv_z [Nxp+NG,Nyp+NG,Nzp+NG,4]
+ *   mask_x [Nxp+1,Nyp,Nzp], mask_y [Nxp,Nyp+1,Nzp], mask_z [Nxp,Nyp,Nzp+1]  (uint8)
+ *
+ * Two numeric forms, selected by the suffix:
+ *   _f64  : every array is double.
+ *   _f32b : "FP32-bulk": U, Un, J, dt and the RK3 accumulation are double; Q, phi, Fp, Fm, Fx..Fz,
+ *           Fv_*, s1..s3 and the 9 metrics are float, and splitting, reconstruction and the viscous
+ *           flux are evaluated in float with float accumulation.
+ *
+ * Every pointer argument is a device pointer unless stated otherwise.  Every call is asynchronous
+ * on `stream` (NULL = the null stream); the caller synchronises.  Return value: 0 on success,
+ * F3D_E_* (<0) for an invalid argument, or the positive hipError_t of a failed HIP call.
+ * f3d_last_error() returns a thread-local message for the last failure.  Nothing here throws.
+ * The library never frees caller memory; hot calls never allocate (workspaces are created once).
+ */
+#ifndef F3D_H
+#define F3D_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* f3d_stream_t; /* == hipStream_t */
+
+#define F3D_VERSION 10000
+
+/* status codes */
+#define F3D_OK 0
+#define F3D_E_NULL (-1)     /* required pointer is NULL */
+#define F3D_E_DIMS (-2)     /* bad Nxp/Nyp/Nzp */
+#define F3D_E_NG (-3)       /* NG != 4 */
+#define F3D_E_ARG (-4)      /* other invalid argument (BC kind, NV, KRK, precision ...) */
+#define F3D_E_NAN (-5)      /* NaN/Inf detected by f3d_nan_check_* (solver.jl:481-486) */
+#define F3D_E_WS (-6)       /* workspace does not match params / precision */
+
+/* boundary-condition kinds, one per domain face (xlo,xhi,ylo,yhi,zlo,zhi) */
+#define F3D_BC_PERIODIC 0      /* single-rank exchange_ghost semantics (mpi.jl:2-103) + fill_z (boundary.jl:123-148) */
+#define F3D_BC_DIRICHLET 1     /* freestream q_inf in every ghost layer (boundary.jl:12-38, 98-118) */
+#define F3D_BC_ZEROGRAD 2      /* copy of the last interior layer (boundary.jl:42-51) */
+#define F3D_BC_WALL_ADIABATIC 3 /* mirror, velocity negated, rho/p/T copied (boundary.jl:67-93) */
+#define F3D_BC_WALL_ISOTHERMAL 4 /* mirror, velocity negated, p copied, T = 2*T_wall - T_mirror, rho = p/(Rg T) (no reference counterpart) */
+#define F3D_BC_INFLOW 5        /* Dirichlet from a caller profile inflow[j,k,6] (xlo only; no reference counterpart) */
+
+#define F3D_PREC_F64 0
+#define F3D_PREC_F32B 1
+
+/* Scheme selected per face by the shock-sensor mask (schemes.jl:731, 813, 931, 1000). */
+#define F3D_SCHEME_UP7 0
+#define F3D_SCHEME_WENO7 1
+#define F3D_SCHEME_WENO5 2
+#define F3D_SCHEME_NND2 3
+
+/* Runtime form of the reference's compile-time constants (run.jl:8-13, 57-77, 80-86). */
+typedef struct f3d_params {
+  int32_t Nxp, Nyp, Nzp;   /* interior cells (run.jl:84-86) */
+  int32_t NG;              /* ghost layers; must be 4 (run.jl:80; viscous.jl:119 and div.jl:14-39 assume it) */
+  int32_t viscous;         /* run.jl:61 */
+  int32_t bc[6];           /* F3D_BC_* for xlo,xhi,ylo,yhi,zlo,zhi */
+  int32_t pad_;
+  double gamma, Rg, Cp, C_s, T_s, Pr;          /* run.jl:8-13 */
+  double hybrid_phi1, hybrid_phi2, hybrid_phi3; /* run.jl:70-72 */
+  double Linear_phi;                            /* run.jl:73: Linear = UP7*Linear_phi + CD6*(1-Linear_phi) */
+  double weno7_eps, weno5_eps;                  /* schemes.jl:712-713 */
+  double q_inf[6];   /* Dirichlet state (rho,u,v,w,p,T); reference values boundary.jl:14-19 */
+  double T_wall;     /* F3D_BC_WALL_ISOTHERMAL wall temperature */
+} f3d_params;
+
+int f3d_version(void);
+const char* f3d_last_error(void);
+
+/* Fill *p with the reference's run.jl constants and boundary set (inlet Dirichlet, outlet
+ * zero-gradient, adiabatic wall below, Dirichlet farfield above, periodic span). Host-only. */
+void f3d_params_init(f3d_params* p, int32_t Nxp, int32_t Nyp, int32_t Nzp);
+/* Validate params; 0 or F3D_E_*. Host-only, no GPU call. */
+int f3d_params_check(const f3d_params* p);
+/* Sizes in elements of the arrays above (n_cell = Nxt*Nyt*Nzt, etc.). Host-only. */
+size_t f3d_cell_count(const f3d_params* p);
+
+/* ---------------- kernel-by-kernel drop-ins (the reference's per-stage launches) ---------- */
+
+/* shockSensor(phi, Q)  schemes.jl:1-24; writes phi on [2, N_tot-1]^3 (1-based) */
+int f3d_shock_sensor_f64(const f3d_params* p, double* phi, const double* Q, f3d_stream_t s);
+int f3d_shock_sensor_f32b(const f3d_params* p, float* phi, const float* Q, f3d_stream_t s);
+
+/* Face scheme mask, the branch advect_{x,y,z}c takes on every face (schemes.jl:724-731, 813, 931,
+ * 1000; y 1038-1045; z 1352-1359).  No reference counterpart as a kernel: it is the integer the
+ * reference computes implicitly, materialised for bit-exact checking. */
+int f3d_scheme_mask_f64(const f3d_params* p, uint8_t* mask_x, uint8_t* mask_y, uint8_t* mask_z,
+                        const double* phi, f3d_stream_t s);
+int f3d_scheme_mask_f32b(const f3d_params* p, uint8_t* mask_x, uint8_t* mask_y, uint8_t* mask_z,
+                         const float* phi, f3d_stream_t s);
+
+/* fluxSplit_SW(Q, Fp, Fm, S, Ax, Ay, Az)  split.jl:2-68 */
+int f3d_flux_split_sw_f64(const f3d_params* p, const double* Q, double* Fp, double* Fm,
+                          const double* S, const double* Ax, const double* Ay, const double* Az,
+                          f3d_stream_t s);
+int f3d_flux_split_sw_f32b(const f3d_params* p, const float* Q, float* Fp, float* Fm,
+                           const float* S, const float* Ax, const float* Ay, const float* Az,
+                           f3d_stream_t s);
+
+/* advect_xc / advect_yc / advect_zc(F, phi, S, Fp, Fm, Q, Ax, Ay, Az)  schemes.jl:700, 1014, 1330 */
+int f3d_advect_xc_f64(const f3d_params* p, double* F, const double* phi, const double* S,
+                      const double* Fp, const double* Fm, const double* Q, const double* Ax,
+                      const double* Ay, const double* Az, f3d_stream_t s);
+int f3d_advect_yc_f64(const f3d_params* p, double* F, const double* phi, const double* S,
+                      const double* Fp, const double* Fm, const double* Q, const double* Ax,
+                      const double* Ay, const double* Az, f3d_stream_t s);
+int f3d_advect_zc_f64(const f3d_params* p, double* F, const double* phi, const double* S,
+                      const double* Fp, const double* Fm, const double* Q, const double* Ax,
+                      const double* Ay, const double* Az, f3d_stream_t s);
+int f3d_advect_xc_f32b(const f3d_params* p, float* F, const float* phi, const float* S,
+                       const float* Fp, const float* Fm, const float* Q, const float* Ax,
+                       const float* Ay, const float* Az, f3d_stream_t s);
+int f3d_advect_yc_f32b(const f3d_params* p, float* F, const float* phi, const float* S,
+                       const float* Fp, const float* Fm, const float* Q, const float* Ax,
+                       const float* Ay, const float* Az, f3d_stream_t s);
+int f3d_advect_zc_f32b(const f3d_params* p, float* F, const float* phi, const float* S,
+                       const float* Fp, const float* Fm, const float* Q, const float* Ax,
+                       const float* Ay, const float* Az, f3d_stream_t s);
+
+/* viscousFlux(Fv_x, Fv_y, Fv_z, Q, dxi_dx..dzeta_dz, J)  viscous.jl:1-134 (laminar) */
+int f3d_viscous_flux_f64(const f3d_params* p, double* Fv_x, double* Fv_y, double* Fv_z,
+                         const double* Q, const double* dxidx, const double* dxidy,
+                         const double* dxidz, const double* detadx, const double* detady,
+                         const double* detadz, const double* dzetadx, const double* dzetady,
+                         const double* dzetadz, const double* J, f3d_stream_t s);
+int f3d_viscous_flux_f32b(const f3d_params* p, float* Fv_x, float* Fv_y, float* Fv_z,
+                          const float* Q, const float* dxidx, const float* dxidy,
+                          const float* dxidz, const float* detadx, const float* detady,
+                          const float* detadz, const float* dzetadx, const float* dzetady,
+                          const float* dzetadz, const double* J, f3d_stream_t s);
+
+/* div(U, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, dt, J)  div.jl:1-50 : U += J*dt*(-dF + dFv) on the interior */
+int f3d_div_f64(const f3d_params* p, double* U, const double* Fx, const double* Fy,
+                const double* Fz, const double* Fv_x, const double* Fv_y, const double* Fv_z,
+                double dt, const double* J, f3d_stream_t s);
+int f3d_div_f32b(const f3d_params* p, double* U, const float* Fx, const float* Fy,
+                 const float* Fz, const float* Fv_x, const float* Fv_y, const float* Fv_z,
+                 double dt, const double* J, f3d_stream_t s);
+
+/* linComb(U, Un, NV, a, b)  utils.jl:57-70 : U = a*U + b*Un on the interior (both forms: double) */
+int f3d_lin_comb(const f3d_params* p, double* U, const double* Un, int32_t NV, double a,
+                 double b, f3d_stream_t s);
+
+/* c2Prim(U, Q)  utils.jl:2-30 (interior)   prim2c(U, Q)  utils.jl:33-54 (whole array) */
+int f3d_c2prim_f64(const f3d_params* p, const double* U, double* Q, f3d_stream_t s);
+int f3d_c2prim_f32b(const f3d_params* p, const double* U, float* Q, f3d_stream_t s);
+int f3d_prim2c_f64(const f3d_params* p, double* U, const double* Q, f3d_stream_t s);
+int f3d_prim2c_f32b(const f3d_params* p, double* U, const float* Q, f3d_stream_t s);
+
+/* exchange_ghost(Q, NV, comm_cart, ...)  mpi.jl:2-103 with one rank: for every periodic dimension
+ * (x, then y, then z) the NG-thick ghost slabs of Q[..., 1:NV] are copied from the opposite
+ * interior slabs, over the full extent of the other two dimensions.  Host staging and MPI are gone. */
+int f3d_exchange_ghost_f64(const f3d_params* p, double* Q, int32_t NV, f3d_stream_t s);
+int f3d_exchange_ghost_f32b(const f3d_params* p, float* Q, int32_t NV, f3d_stream_t s);
+
+/* fillGhost(Q, U, rankx, ranky)  boundary.jl:151-156 : fill_x, fill_y, fill_z in that order with
+ * the face kinds of p->bc.  `inflow` [Nyt,Nzt,6] (device) is read only by F3D_BC_INFLOW, else NULL.
+ * U may be NULL (ghost U is not read on the finite-difference path). */
+int f3d_fill_ghost_f64(const f3d_params* p, double* Q, double* U, const double* inflow,
+                       f3d_stream_t s);
+int f3d_fill_ghost_f32b(const f3d_params* p, float* Q, double* U, const float* inflow,
+                        f3d_stream_t s);
+
+/* flowAdvance(U, Q, Fp, Fm, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, s1, s2, s3, 9 metrics, J, x, y, z, phi)
+ * solver.jl:19, finite-difference branch solver.jl:133-188: split/advect x, y, z then viscousFlux.
+ * x, y, z are accepted and unused (finite-volume only), as in the reference. */
+int f3d_flow_advance_f64(const f3d_params* p, double* U, double* Q, double* Fp, double* Fm,
+                         double* Fx, double* Fy, double* Fz, double* Fv_x, double* Fv_y,
+                         double* Fv_z, const double* s1, const double* s2, const double* s3,
+                         const double* dxidx, const double* dxidy, const double* dxidz,
+                         const double* detadx, const double* detady, const double* detadz,
+                         const double* dzetadx, const double* dzetady, const double* dzetadz,
+                         const double* J, const double* x, const double* y, const double* z,
+                         double* phi, f3d_stream_t s);
+int f3d_flow_advance_f32b(const f3d_params* p, double* U, float* Q, float* Fp, float* Fm,
+                          float* Fx, float* Fy, float* Fz, float* Fv_x, float* Fv_y,
+                          float* Fv_z, const float* s1, const float* s2, const float* s3,
+                          const float* dxidx, const float* dxidy, const float* dxidz,
+                          const float* detadx, const float* detady, const float* detadz,
+                          const float* dzetadx, const float* dzetady, const float* dzetadz,
+                          const double* J, const float* x, const float* y, const float* z,
+                          float* phi, f3d_stream_t s);
+
+/* ---------------- fused path (the MI355X hot path) ------------------------------------------ */
+
+/* Opaque persistent workspace (RHS accumulator, phi, viscous-flux scratch, reduction partials),
+ * allocated once for one params/precision pair.  Not thread-safe; re-entrant across workspaces. */
+typedef struct f3d_workspace f3d_workspace;
+int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_workspace** ws);
+int f3d_workspace_destroy(f3d_workspace* ws);
+/* Bytes of device memory held by a workspace. Host-only. */
+size_t f3d_workspace_bytes(const f3d_workspace* ws);
+
+/* Metric layout for the fused entries: metrics[9] = device pointers in the reference's order
+ * (dxi_dx, dxi_dy, dxi_dz, deta_dx, deta_dy, deta_dz, dzeta_dx, dzeta_dy, dzeta_dz), each
+ * [Nxt,Nyt,Nzt]; s1..s3 are not needed (recomputed as |grad xi_d| on the fly). */
+
+/* RHS: R = J * ( -(Fx[i+1]-Fx[i]) - ... + dFv_x/dxi + dFv_y/deta + dFv_z/dzeta ) on interior cells,
+ * from primitive Q with ghosts already filled; R has U's layout (ghost entries untouched).
+ * Equals the reference's div increment per unit dt (div.jl:40-48) after flowAdvance.
+ * `phi` (optional, may be NULL) receives the shock sensor; `mask_*` (optional) the face schemes. */
+int f3d_rhs_f64(const f3d_params* p, f3d_workspace* ws, double* R, const double* Q,
+                const double* const* metrics, const double* J, double* phi, f3d_stream_t s);
+int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, const float* Q,
+                 const float* const* metrics, const double* J, float* phi, f3d_stream_t s);
+
+/* One SSP-RK3 stage KRK in {1,2,3} of the loop body solver.jl:416-444: (KRK==1: Un <- U), sensor,
+ * RHS, U += dt*R, (KRK==2: U = U/4 + 3Un/4, KRK==3: U = 2U/3 + Un/3), c2Prim, ghost fill of Q. */
+int f3d_rk3_stage_f64(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U,
+                      double* Un, double* Q, const double* const* metrics, const double* J,
+                      double dt, const double* inflow, f3d_stream_t s);
+int f3d_rk3_stage_f32b(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U,
+                       double* Un, float* Q, const float* const* metrics, const double* J,
+                       double dt, const float* inflow, f3d_stream_t s);
+
+/* `nsteps` full RK3 steps with a fixed dt (time_step's loop, solver.jl:410-444, without I/O). */
+int f3d_time_step_f64(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
+                      double* Un, double* Q, const double* const* metrics, const double* J,
+                      double dt, const double* inflow, f3d_stream_t s);
+int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
+                       double* Un, float* Q, const float* const* metrics, const double* J,
+                       double dt, const float* inflow, f3d_stream_t s);
+
+/* Global CFL time step (no reference counterpart as a reduction; per-cell formula of compute_dt,
+ * utils.jl:73-100, with |contravariant velocity| + c):
+ *   dt = cfl * min over interior cells of  1 / ( J * max_d (|A_d . u| + c*|A_d|) ),  c = sqrt(gamma Rg T).
+ * Result written to *dt_out (device double). */
+int f3d_cfl_dt_f64(const f3d_params* p, f3d_workspace* ws, double* dt_out, const double* Q,
+                   const double* const* metrics, const double* J, double cfl, f3d_stream_t s);
+int f3d_cfl_dt_f32b(const f3d_params* p, f3d_workspace* ws, double* dt_out, const float* Q,
+                    const float* const* metrics, const double* J, double cfl, f3d_stream_t s);
+
+/* any(isnan, U) of solver.jl:481 over the whole U array (NaN or Inf); writes 0/1 to *flag_out
+ * (device int32).  Synchronous variant below returns F3D_E_NAN when set. */
+int f3d_nan_check(const f3d_params* p, f3d_workspace* ws, int32_t* flag_out, const double* U,
+                  int32_t NV, f3d_stream_t s);
+int f3d_nan_check_sync(const f3d_params* p, f3d_workspace* ws, const double* U, int32_t NV,
+                       f3d_stream_t s);
+
+/* Metrics from node coordinates (parse_mesh.jl:139-248), always FP64: CD6 interior, one-sided
+ * second order at the first/last three indices, J = 1/det, nine J^-1-scaled metrics in the order
+ * of `metrics`; s_out[3] (optional, may hold NULLs) receive |grad xi_d| (solver.jl:286-288). */
+int f3d_metrics_f64(const f3d_params* p, const double* x, const double* y, const double* z,
+                    double* const* metrics, double* J, double* const* s_out, f3d_stream_t s);
+
+/* ---------------- profiling: hipEvent timing of the library's own kernel launches -------------
+ * No reference counterpart (the reference prints wall-clock time only, solver.jl:472-479).
+ * When enabled, every kernel the library launches is bracketed by a pair of hipEvents recorded on
+ * the launching stream; queries synchronise on those events.  Process-global, not thread-safe. */
+int f3d_timing_enable(int32_t on);
+int f3d_timing_reset(void);
+/* Sum of elapsed ms and number of launches of the kernels whose name equals `name`. */
+int f3d_timing_query(const char* name, double* total_ms, int64_t* launches);
+/* Comma-separated distinct kernel names timed since the last reset (truncated to len-1 chars). */
+int f3d_timing_names(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* F3D_H */

@@ -1,0 +1,251 @@
+// C entry points of the CPU oracle (liboracle_f3d.so), loaded through ctypes by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  TEST INFRASTRUCTURE ONLY: the
+// product path (flame3d-mhd_amd/, libf3d_hip.so) never links, loads or calls this file.
+//
+// What pins each oracle function, apart from GPU parity (tests/test_oracle_*.py):
+//   c2prim / prim2c (utils.jl:2-54)   round trip; eps floors; hand-evaluated cell.
+//   shock_sensor (schemes.jl:1-24)    constant p -> 0; closed form on a planted jump; brute numpy.
+//   scheme_mask (schemes.jl:724-1009) isolated jump -> the known band pattern at distance <= 3.
+//   flux_split_sw (split.jl:2-68)     Fp+Fm = Euler flux projected on A (exact algebra);
+//                                     Fp has no negative eigen-part (supersonic: Fm = 0).
+//   advect_c UP7 (schemes.jl:731-754) constant fluxes -> F = Fp+Fm; 7th-order convergence.
+//   advect_c WENO7/WENO5/NND          constant state -> exact; equal smoothness -> UP7 / the
+//                                     5th-order linear scheme; L*R = I; minmod truth table;
+//                                     independent numpy restatement on small random cases.
+//   viscous_flux + div (viscous.jl, div.jl) linear fields -> zero viscous RHS; quadratic fields ->
+//                                     closed form; periodic telescoping sum(dU/J) = 0.
+//   exchange_ghost / fill_ghost       index identities on a labelled array (value = index).
+//   rk3 (solver.jl:416-444)           du/dt = lambda*u amplification 1+z+z^2/2+z^3/6.
+//   cfl_dt                            uniform flow closed form.
+//   metrics (parse_mesh.jl:139-248)   Cartesian spacing h -> dxi/dx = h^2, J = h^-3; affine exact.
+// No reference test, fixture or golden vector exists for this path (the reference has no tests);
+// the CUDA.jl fastmath Float32 outputs of the reference itself are unpinned (no Julia here).
+#include "f3d_oracle.hpp"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using namespace f3d_oracle;
+
+#define OR_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+template <typename T>
+struct M9 {
+  const T* m[9];
+  M9(const void* const* p) {
+    for (int i = 0; i < 9; ++i) m[i] = (const T*)p[i];
+  }
+};
+}  // namespace
+
+OR_API int oracle_version(void) { return F3D_VERSION; }
+
+OR_API void oracle_c2prim_f64(const f3d_params* p, const double* U, double* Q) { c2prim<double>(p, U, Q); }
+OR_API void oracle_c2prim_f32b(const f3d_params* p, const double* U, float* Q) { c2prim<float>(p, U, Q); }
+OR_API void oracle_prim2c_f64(const f3d_params* p, double* U, const double* Q) { prim2c<double>(p, U, Q); }
+OR_API void oracle_prim2c_f32b(const f3d_params* p, double* U, const float* Q) { prim2c<float>(p, U, Q); }
+OR_API void oracle_lin_comb(const f3d_params* p, double* U, const double* Un, int NV, double a, double b) {
+  lin_comb(p, U, Un, NV, a, b);
+}
+OR_API void oracle_shock_sensor_f64(const f3d_params* p, double* phi, const double* Q) { shock_sensor<double>(p, phi, Q); }
+OR_API void oracle_shock_sensor_f32b(const f3d_params* p, float* phi, const float* Q) { shock_sensor<float>(p, phi, Q); }
+OR_API void oracle_scheme_mask_f64(const f3d_params* p, uint8_t* mx, uint8_t* my, uint8_t* mz, const double* phi) {
+  scheme_mask<double>(p, mx, my, mz, phi);
+}
+OR_API void oracle_scheme_mask_f32b(const f3d_params* p, uint8_t* mx, uint8_t* my, uint8_t* mz, const float* phi) {
+  scheme_mask<float>(p, mx, my, mz, phi);
+}
+OR_API void oracle_flux_split_sw_f64(const f3d_params* p, const double* Q, double* Fp, double* Fm, const double* S,
+                                     const double* Ax, const double* Ay, const double* Az) {
+  flux_split_sw<double>(p, Q, Fp, Fm, S, Ax, Ay, Az);
+}
+OR_API void oracle_flux_split_sw_f32b(const f3d_params* p, const float* Q, float* Fp, float* Fm, const float* S,
+                                      const float* Ax, const float* Ay, const float* Az) {
+  flux_split_sw<float>(p, Q, Fp, Fm, S, Ax, Ay, Az);
+}
+OR_API void oracle_advect_c_f64(const f3d_params* p, int d, double* F, const double* phi, const double* S,
+                                const double* Fp, const double* Fm, const double* Q, const double* Ax,
+                                const double* Ay, const double* Az) {
+  advect_c<double>(p, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az);
+}
+OR_API void oracle_advect_c_f32b(const f3d_params* p, int d, float* F, const float* phi, const float* S,
+                                 const float* Fp, const float* Fm, const float* Q, const float* Ax,
+                                 const float* Ay, const float* Az) {
+  advect_c<float>(p, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az);
+}
+OR_API void oracle_viscous_flux_f64(const f3d_params* p, double* Fvx, double* Fvy, double* Fvz, const double* Q,
+                                    const void* const* m, const double* J) {
+  viscous_flux<double>(p, Fvx, Fvy, Fvz, Q, M9<double>(m).m, J);
+}
+OR_API void oracle_viscous_flux_f32b(const f3d_params* p, float* Fvx, float* Fvy, float* Fvz, const float* Q,
+                                     const void* const* m, const double* J) {
+  viscous_flux<float>(p, Fvx, Fvy, Fvz, Q, M9<float>(m).m, J);
+}
+OR_API void oracle_div_f64(const f3d_params* p, double* U, const double* Fx, const double* Fy, const double* Fz,
+                           const double* Fvx, const double* Fvy, const double* Fvz, double dt, const double* J) {
+  div<double>(p, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, dt, J);
+}
+OR_API void oracle_div_f32b(const f3d_params* p, double* U, const float* Fx, const float* Fy, const float* Fz,
+                            const float* Fvx, const float* Fvy, const float* Fvz, double dt, const double* J) {
+  div<float>(p, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, dt, J);
+}
+OR_API void oracle_exchange_ghost_f64(const f3d_params* p, double* Q, int NV) { exchange_ghost<double>(p, Q, NV); }
+OR_API void oracle_exchange_ghost_f32b(const f3d_params* p, float* Q, int NV) { exchange_ghost<float>(p, Q, NV); }
+OR_API void oracle_fill_ghost_f64(const f3d_params* p, double* Q, double* U, const double* inflow) {
+  fill_ghost<double>(p, Q, U, inflow);
+}
+OR_API void oracle_fill_ghost_f32b(const f3d_params* p, float* Q, double* U, const float* inflow) {
+  fill_ghost<float>(p, Q, U, inflow);
+}
+OR_API void oracle_metric_norms_f64(const f3d_params* p, const void* const* m, double* s1, double* s2, double* s3) {
+  metric_norms<double>(p, M9<double>(m).m, s1, s2, s3);
+}
+OR_API void oracle_metric_norms_f32b(const f3d_params* p, const void* const* m, float* s1, float* s2, float* s3) {
+  metric_norms<float>(p, M9<float>(m).m, s1, s2, s3);
+}
+
+namespace {
+// RHS per unit dt exactly as the reference produces it: sensor, flowAdvance, then div into a
+// zeroed U with dt = 1 (div.jl:40-48).
+template <typename T>
+void rhs_ref(const f3d_params* p, double* R, const T* Q, const void* const* mv, const double* J, T* phi_out,
+             uint8_t* mx, uint8_t* my, uint8_t* mz) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  Scratch<T> sc(p);
+  M9<T> m(mv);
+  std::vector<T> s1(nc), s2(nc), s3(nc);
+  metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
+  std::vector<T> Qc(Q, Q + nc * 6);
+  shock_sensor<T>(p, sc.phi.data(), Qc.data());
+  flow_advance<T>(p, Qc.data(), sc.Fp.data(), sc.Fm.data(), sc.Fx.data(), sc.Fy.data(), sc.Fz.data(),
+                  sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), s1.data(), s2.data(), s3.data(), m.m, J,
+                  sc.phi.data());
+  std::fill(R, R + nc * 5, 0.0);
+  div<T>(p, R, sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), 1.0, J);
+  if (phi_out) std::copy(sc.phi.begin(), sc.phi.end(), phi_out);
+  if (mx || my || mz) scheme_mask<T>(p, mx, my, mz, sc.phi.data());
+}
+
+template <typename T>
+void time_step_ref(const f3d_params* p, int nsteps, double* U, double* Un, T* Q, const void* const* mv,
+                   const double* J, double dt, const T* inflow) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  Scratch<T> sc(p);
+  M9<T> m(mv);
+  std::vector<T> s1(nc), s2(nc), s3(nc);
+  metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
+  for (int t = 0; t < nsteps; ++t)
+    for (int KRK = 1; KRK <= 3; ++KRK)
+      rk3_stage<T>(p, sc, KRK, U, Un, Q, s1.data(), s2.data(), s3.data(), m.m, J, dt, inflow);
+}
+}  // namespace
+
+OR_API void oracle_rhs_f64(const f3d_params* p, double* R, const double* Q, const void* const* m, const double* J,
+                           double* phi, uint8_t* mx, uint8_t* my, uint8_t* mz) {
+  rhs_ref<double>(p, R, Q, m, J, phi, mx, my, mz);
+}
+OR_API void oracle_rhs_f32b(const f3d_params* p, double* R, const float* Q, const void* const* m, const double* J,
+                            float* phi, uint8_t* mx, uint8_t* my, uint8_t* mz) {
+  rhs_ref<float>(p, R, Q, m, J, phi, mx, my, mz);
+}
+OR_API void oracle_time_step_f64(const f3d_params* p, int nsteps, double* U, double* Un, double* Q,
+                                 const void* const* m, const double* J, double dt, const double* inflow) {
+  time_step_ref<double>(p, nsteps, U, Un, Q, m, J, dt, inflow);
+}
+OR_API void oracle_time_step_f32b(const f3d_params* p, int nsteps, double* U, double* Un, float* Q,
+                                  const void* const* m, const double* J, double dt, const float* inflow) {
+  time_step_ref<float>(p, nsteps, U, Un, Q, m, J, dt, inflow);
+}
+OR_API double oracle_cfl_dt_f64(const f3d_params* p, const double* Q, const void* const* m, const double* J,
+                                double cfl) {
+  return cfl_dt<double>(p, Q, M9<double>(m).m, J, cfl);
+}
+OR_API double oracle_cfl_dt_f32b(const f3d_params* p, const float* Q, const void* const* m, const double* J,
+                                 double cfl) {
+  return cfl_dt<float>(p, Q, M9<float>(m).m, J, cfl);
+}
+OR_API void oracle_metrics_f64(const f3d_params* p, const double* x, const double* y, const double* z,
+                               void* const* m, double* J) {
+  double* mm[9];
+  for (int i = 0; i < 9; ++i) mm[i] = (double*)m[i];
+  metrics(p, x, y, z, mm, J);
+}
+namespace {
+// Magnitude of the terms the RHS sums (div.jl:40-48 with every term replaced by its absolute
+// value): A = J*(|Fx_i|+|Fx_i+1|+|Fy_j|+...) + J*sum_d(|c1|(|Fv_-2|+|Fv_+2|) + |c2|(|Fv_-1|+|Fv_+1|)).
+// Parity of a sum with cancellation is judged relative to this scale.
+template <typename T>
+void rhs_absterms(const f3d_params* p, double* A, const T* Q, const void* const* mv, const double* J) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  Scratch<T> sc(p);
+  M9<T> m(mv);
+  std::vector<T> s1(nc), s2(nc), s3(nc);
+  metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
+  std::vector<T> Qc(Q, Q + nc * 6);
+  shock_sensor<T>(p, sc.phi.data(), Qc.data());
+  flow_advance<T>(p, Qc.data(), sc.Fp.data(), sc.Fm.data(), sc.Fx.data(), sc.Fy.data(), sc.Fz.data(),
+                  sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), s1.data(), s2.data(), s3.data(), m.m, J,
+                  sc.phi.data());
+  std::fill(A, A + nc * 5, 0.0);
+  const long NG = g.NG;
+  const double c1 = 1.0 / 12, c2 = 2.0 / 3;
+  for (long k = 1; k <= g.Nzp; ++k)
+    for (long j = 1; j <= g.Nyp; ++j)
+      for (long i = 1; i <= g.Nxp; ++i) {
+        const double Jc = std::fabs(J[g.cell.at(i + NG, j + NG, k + NG)]);
+        for (long n = 1; n <= 5; ++n) {
+          double s = std::fabs((double)sc.Fx[g.fx.at(i, j, k, n)]) + std::fabs((double)sc.Fx[g.fx.at(i + 1, j, k, n)]) +
+                     std::fabs((double)sc.Fy[g.fy.at(i, j, k, n)]) + std::fabs((double)sc.Fy[g.fy.at(i, j + 1, k, n)]) +
+                     std::fabs((double)sc.Fz[g.fz.at(i, j, k, n)]) + std::fabs((double)sc.Fz[g.fz.at(i, j, k + 1, n)]);
+          if (n >= 2 && p->viscous) {
+            const int v = (int)n - 1;
+            auto a = [&](const std::vector<T>& F, long ii, long jj, long kk) { return std::fabs((double)F[g.fv.at(ii, jj, kk, v)]); };
+            s += c1 * (a(sc.Fvx, i, j + 2, k + 2) + a(sc.Fvx, i + 4, j + 2, k + 2)) + c2 * (a(sc.Fvx, i + 1, j + 2, k + 2) + a(sc.Fvx, i + 3, j + 2, k + 2));
+            s += c1 * (a(sc.Fvy, i + 2, j, k + 2) + a(sc.Fvy, i + 2, j + 4, k + 2)) + c2 * (a(sc.Fvy, i + 2, j + 1, k + 2) + a(sc.Fvy, i + 2, j + 3, k + 2));
+            s += c1 * (a(sc.Fvz, i + 2, j + 2, k) + a(sc.Fvz, i + 2, j + 2, k + 4)) + c2 * (a(sc.Fvz, i + 2, j + 2, k + 1) + a(sc.Fvz, i + 2, j + 2, k + 3));
+          }
+          A[g.cell.at(i + NG, j + NG, k + NG, n)] = Jc * s;
+        }
+      }
+}
+
+// advect_c evaluated in long double from double inputs (conditioning reference for the tests).
+void advect_ld(const f3d_params* p, int d, const double* phi, const double* S, const double* Fp, const double* Fm,
+               const double* Q, const double* Ax, const double* Ay, const double* Az, double* F) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  std::vector<long double> phiL(phi, phi + nc), SL(S, S + nc), FpL(Fp, Fp + 5 * nc), FmL(Fm, Fm + 5 * nc),
+      QL(Q, Q + 6 * nc), AxL(Ax, Ax + nc), AyL(Ay, Ay + nc), AzL(Az, Az + nc);
+  size_t nf = (size_t)(g.Nxp + (d == 0)) * (g.Nyp + (d == 1)) * (g.Nzp + (d == 2)) * 5;
+  std::vector<long double> FL(nf);
+  advect_c<long double>(p, d, FL.data(), phiL.data(), SL.data(), FpL.data(), FmL.data(), QL.data(), AxL.data(),
+                        AyL.data(), AzL.data());
+  for (size_t i = 0; i < nf; ++i) F[i] = (double)FL[i];
+}
+}  // namespace
+
+OR_API void oracle_rhs_absterms_f64(const f3d_params* p, double* A, const double* Q, const void* const* m,
+                                    const double* J) {
+  rhs_absterms<double>(p, A, Q, m, J);
+}
+OR_API void oracle_rhs_absterms_f32b(const f3d_params* p, double* A, const float* Q, const void* const* m,
+                                     const double* J) {
+  rhs_absterms<float>(p, A, Q, m, J);
+}
+OR_API void oracle_advect_c_ld(const f3d_params* p, int d, double* F, const double* phi, const double* S,
+                               const double* Fp, const double* Fm, const double* Q, const double* Ax,
+                               const double* Ay, const double* Az) {
+  advect_ld(p, d, phi, S, Fp, Fm, Q, Ax, Ay, Az, F);
+}
+
+OR_API int oracle_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,796 @@
+// CPU oracle for the Flame3D finite-difference RHS + RK3 stage path.
+//
+// TEST INFRASTRUCTURE ONLY.  This is a plain restatement, on the CPU, of the reference kernel
+// bodies of rygan-code/Flame3D-MHD (Julia + CUDA.jl), written from reading those files.  It is the
+// checker for the HIP library: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// may load it, and never as the thing measured or shipped.
+//
+// Conventions: 1-based (i,j,k,n) exactly as in the Julia source so that each loop can be read
+// against the cited lines; arrays are column-major with i fastest.  T is the "bulk" type (double
+// for the FP64 oracle, float for the FP32-bulk form); conserved U, Un and J are always double.
+//
+// Pinning status of each function (what checks it besides GPU parity) is listed in the header of
+// oracle/f3d_oracle.cpp.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "../include/f3d.h"
+
+namespace f3d_oracle {
+
+// Extents of one 4-D column-major array (n1,n2,n3,nv), 1-based accessors.
+struct Ext {
+  long n1, n2, n3;
+  size_t at(long i, long j, long k) const {
+    return (size_t)(i - 1) + (size_t)n1 * ((size_t)(j - 1) + (size_t)n2 * (size_t)(k - 1));
+  }
+  size_t at(long i, long j, long k, long n) const {
+    return at(i, j, k) + (size_t)n1 * n2 * n3 * (size_t)(n - 1);
+  }
+};
+
+struct Grid {
+  long Nxp, Nyp, Nzp, NG, Nxt, Nyt, Nzt;
+  Ext cell, fx, fy, fz, fv;
+  explicit Grid(const f3d_params* p)
+      : Nxp(p->Nxp), Nyp(p->Nyp), Nzp(p->Nzp), NG(p->NG), Nxt(p->Nxp + 2 * p->NG),
+        Nyt(p->Nyp + 2 * p->NG), Nzt(p->Nzp + 2 * p->NG) {
+    cell = {Nxt, Nyt, Nzt};
+    fx = {Nxp + 1, Nyp, Nzp};
+    fy = {Nxp, Nyp + 1, Nzp};
+    fz = {Nxp, Nyp, Nzp + 1};
+    fv = {Nxp + NG, Nyp + NG, Nzp + NG};
+  }
+};
+
+// Julia's max for floats propagates NaN (Base.max); C's std::max does not.
+template <typename T>
+inline T jmax(T a, T b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return (b > a) ? b : a;
+}
+
+// Linear = UP7*Linear_phi + CD6*(1-Linear_phi)   run.jl:75-77
+template <typename T>
+inline void linear_coeffs(const f3d_params* p, T c[7]) {
+  const double up7[7] = {-3.0 / 420, 25.0 / 420, -101.0 / 420, 319.0 / 420,
+                         214.0 / 420, -38.0 / 420, 4.0 / 420};
+  const double cd6[7] = {0.0, 1.0 / 60, -2.0 / 15, 37.0 / 60, 37.0 / 60, -2.0 / 15, 1.0 / 60};
+  for (int m = 0; m < 7; ++m)
+    c[m] = (T)(up7[m] * p->Linear_phi + cd6[m] * (1.0 - p->Linear_phi));
+}
+
+// ---------------------------------------------------------------- utils.jl
+// c2Prim  utils.jl:2-30 (interior [NG+1, Np+NG]^3).  The eps floors are the machine epsilon of the
+// type the conversion runs in (the reference uses eps(Float32) in its Float32 build).
+template <typename T>
+void c2prim(const f3d_params* p, const double* U, T* Q) {
+  Grid g(p);
+  const double gam = p->gamma, Rg = p->Rg, eps = std::numeric_limits<double>::epsilon();
+#pragma omp parallel for collapse(2)
+  for (long k = g.NG + 1; k <= g.Nzp + g.NG; ++k)
+    for (long j = g.NG + 1; j <= g.Nyp + g.NG; ++j)
+      for (long i = g.NG + 1; i <= g.Nxp + g.NG; ++i) {
+        double rho = jmax(U[g.cell.at(i, j, k, 1)], eps);
+        double rinv = 1 / rho;
+        double u = U[g.cell.at(i, j, k, 2)] * rinv;
+        double v = U[g.cell.at(i, j, k, 3)] * rinv;
+        double w = U[g.cell.at(i, j, k, 4)] * rinv;
+        double ei = jmax(U[g.cell.at(i, j, k, 5)] - 0.5 * rho * (u * u + v * v + w * w), eps);
+        double pr = (gam - 1) * ei;
+        double Tt = pr / (rho * Rg);
+        Q[g.cell.at(i, j, k, 1)] = (T)rho;
+        Q[g.cell.at(i, j, k, 2)] = (T)u;
+        Q[g.cell.at(i, j, k, 3)] = (T)v;
+        Q[g.cell.at(i, j, k, 4)] = (T)w;
+        Q[g.cell.at(i, j, k, 5)] = (T)pr;
+        Q[g.cell.at(i, j, k, 6)] = (T)Tt;
+      }
+}
+
+// prim2c  utils.jl:33-54 (whole array incl. ghosts)
+template <typename T>
+void prim2c(const f3d_params* p, double* U, const T* Q) {
+  Grid g(p);
+  const double gam = p->gamma;
+#pragma omp parallel for collapse(2)
+  for (long k = 1; k <= g.Nzt; ++k)
+    for (long j = 1; j <= g.Nyt; ++j)
+      for (long i = 1; i <= g.Nxt; ++i) {
+        double rho = Q[g.cell.at(i, j, k, 1)], u = Q[g.cell.at(i, j, k, 2)];
+        double v = Q[g.cell.at(i, j, k, 3)], w = Q[g.cell.at(i, j, k, 4)];
+        double ei = (double)Q[g.cell.at(i, j, k, 5)] / (gam - 1);
+        U[g.cell.at(i, j, k, 1)] = rho;
+        U[g.cell.at(i, j, k, 2)] = u * rho;
+        U[g.cell.at(i, j, k, 3)] = v * rho;
+        U[g.cell.at(i, j, k, 4)] = w * rho;
+        U[g.cell.at(i, j, k, 5)] = ei + 0.5 * rho * (u * u + v * v + w * w);
+      }
+}
+
+// linComb  utils.jl:57-70
+inline void lin_comb(const f3d_params* p, double* U, const double* Un, int NV, double a, double b) {
+  Grid g(p);
+  for (long n = 1; n <= NV; ++n)
+#pragma omp parallel for collapse(2)
+    for (long k = g.NG + 1; k <= g.Nzp + g.NG; ++k)
+      for (long j = g.NG + 1; j <= g.Nyp + g.NG; ++j)
+        for (long i = g.NG + 1; i <= g.Nxp + g.NG; ++i) {
+          size_t o = g.cell.at(i, j, k, n);
+          U[o] = U[o] * a + Un[o] * b;
+        }
+}
+
+// ---------------------------------------------------------------- schemes.jl
+// shockSensor  schemes.jl:1-24, domain [2, N_tot-1]^3
+template <typename T>
+void shock_sensor(const f3d_params* p, T* phi, const T* Q) {
+  Grid g(p);
+#pragma omp parallel for collapse(2)
+  for (long k = 2; k <= g.Nzt - 1; ++k)
+    for (long j = 2; j <= g.Nyt - 1; ++j)
+      for (long i = 2; i <= g.Nxt - 1; ++i) {
+        T Px1 = Q[g.cell.at(i - 1, j, k, 5)], Px2 = Q[g.cell.at(i, j, k, 5)];
+        T Px3 = Q[g.cell.at(i + 1, j, k, 5)];
+        T Py1 = Q[g.cell.at(i, j - 1, k, 5)], Py2 = Px2, Py3 = Q[g.cell.at(i, j + 1, k, 5)];
+        T Pz1 = Q[g.cell.at(i, j, k - 1, 5)], Pz2 = Px2, Pz3 = Q[g.cell.at(i, j, k + 1, 5)];
+        T px = std::fabs(-Px1 + 2 * Px2 - Px3) / (Px1 + 2 * Px2 + Px3);
+        T py = std::fabs(-Py1 + 2 * Py2 - Py3) / (Py1 + 2 * Py2 + Py3);
+        T pz = std::fabs(-Pz1 + 2 * Pz2 - Pz3) / (Pz1 + 2 * Pz2 + Pz3);
+        phi[g.cell.at(i, j, k)] = px + py + pz;
+      }
+}
+
+// minmod  schemes.jl:26-28
+template <typename T>
+inline T minmod(T a, T b) {
+  return (a * b > 0) ? ((std::fabs(a) > std::fabs(b)) ? b : a) : T(0);
+}
+
+// Direction helper: face loop bounds and stencil offsets of advect_{x,y,z}c.
+struct Dir {
+  int d;  // 0,1,2
+  long di, dj, dk;
+};
+inline Dir make_dir(int d) {
+  return Dir{d, d == 0 ? 1L : 0L, d == 1 ? 1L : 0L, d == 2 ? 1L : 0L};
+}
+
+// Scheme chosen on one face: max of phi over the 6 cells i-3..i+2 (schemes.jl:724-729), then the
+// thresholds of schemes.jl:731, 813, 931.
+template <typename T>
+inline int face_scheme(const f3d_params* p, const Grid& g, const T* phi, long i, long j, long k,
+                       const Dir& D, T* phix_out) {
+  T m = phi[g.cell.at(i - 3 * D.di, j - 3 * D.dj, k - 3 * D.dk)];
+  for (int o = -2; o <= 2; ++o) m = jmax(m, phi[g.cell.at(i + o * D.di, j + o * D.dj, k + o * D.dk)]);
+  if (phix_out) *phix_out = m;
+  if (m < (T)p->hybrid_phi1) return F3D_SCHEME_UP7;
+  if (m < (T)p->hybrid_phi2) return F3D_SCHEME_WENO7;
+  if (m < (T)p->hybrid_phi3) return F3D_SCHEME_WENO5;
+  return F3D_SCHEME_NND2;
+}
+
+template <typename T>
+void scheme_mask(const f3d_params* p, uint8_t* mx, uint8_t* my, uint8_t* mz, const T* phi) {
+  Grid g(p);
+  uint8_t* out[3] = {mx, my, mz};
+  const Ext* fe[3] = {&g.fx, &g.fy, &g.fz};
+  for (int d = 0; d < 3; ++d) {
+    if (!out[d]) continue;
+    Dir D = make_dir(d);
+#pragma omp parallel for collapse(2)
+    for (long k = g.NG + 1; k <= g.Nzp + g.NG + D.dk; ++k)
+      for (long j = g.NG + 1; j <= g.Nyp + g.NG + D.dj; ++j)
+        for (long i = g.NG + 1; i <= g.Nxp + g.NG + D.di; ++i)
+          out[d][fe[d]->at(i - g.NG, j - g.NG, k - g.NG)] =
+              (uint8_t)face_scheme(p, g, phi, i, j, k, D, (T*)nullptr);
+  }
+}
+
+// WENO7 nonlinear weights (schemes.jl:871-879).  The reference evaluates them in Float64 (its
+// epsilon is a Float64 literal); for T=double that is the literal formula.  For T=float the same
+// weights are formed range-safely: every (eps+Is*s0)^2 is divided by the smallest one before
+// inverting, which leaves the normalised weights unchanged and avoids 1/(1e-20)^2 overflowing.
+template <typename T>
+struct Weno7Weights;
+template <>
+struct Weno7Weights<double> {
+  static double combine(double eps, double s0, const double Is[4], const double q[4]) {
+    double a1 = 1 / ((eps + Is[0] * s0) * (eps + Is[0] * s0));
+    double a2 = 12 / ((eps + Is[1] * s0) * (eps + Is[1] * s0));
+    double a3 = 18 / ((eps + Is[2] * s0) * (eps + Is[2] * s0));
+    double a4 = 4 / ((eps + Is[3] * s0) * (eps + Is[3] * s0));
+    double invsum = 1 / (a1 + a2 + a3 + a4);
+    return invsum * (a1 * q[0] + a2 * q[1] + a3 * q[2] + a4 * q[3]);
+  }
+};
+template <>
+struct Weno7Weights<long double> {
+  static long double combine(long double eps, long double s0, const long double Is[4], const long double q[4]) {
+    const long double d[4] = {1, 12, 18, 4};
+    long double a[4], sum = 0, acc = 0;
+    for (int m = 0; m < 4; ++m) {
+      const long double b = eps + Is[m] * s0;
+      a[m] = d[m] / (b * b);
+      sum += a[m];
+      acc += a[m] * q[m];
+    }
+    return acc / sum;
+  }
+};
+template <>
+struct Weno7Weights<float> {
+  static float combine(float eps, float s0, const float Is[4], const float q[4]) {
+    float b[4];
+    for (int m = 0; m < 4; ++m) b[m] = eps + Is[m] * s0;
+    float bmin = std::min(std::min(b[0], b[1]), std::min(b[2], b[3]));
+    const float d[4] = {1.f, 12.f, 18.f, 4.f};
+    float a[4], sum = 0.f, acc = 0.f;
+    for (int m = 0; m < 4; ++m) {
+      float r = bmin / b[m];
+      a[m] = d[m] * (r * r);
+      sum += a[m];
+      acc += a[m] * q[m];
+    }
+    return acc / sum;
+  }
+};
+
+// One WENO7 reconstruction of a 7-point characteristic stencil V[0..6]  (schemes.jl:847-879)
+template <typename T>
+inline T weno7(const T* V, T s0, T eps) {
+  const T V1 = V[0], V2 = V[1], V3 = V[2], V4 = V[3], V5 = V[4], V6 = V[5], V7 = V[6];
+  T q[4], Is[4];
+  q[0] = -3 * V1 + 13 * V2 - 23 * V3 + 25 * V4;
+  q[1] = V2 - 5 * V3 + 13 * V4 + 3 * V5;
+  q[2] = -V3 + 7 * V4 + 7 * V5 - V6;
+  q[3] = 3 * V4 + 13 * V5 - 5 * V6 + V7;
+  Is[0] = V1 * (547 * V1 - 3882 * V2 + 4642 * V3 - 1854 * V4) +
+          V2 * (7043 * V2 - 17246 * V3 + 7042 * V4) + V3 * (11003 * V3 - 9402 * V4) +
+          V4 * (2107 * V4);
+  Is[1] = V2 * (267 * V2 - 1642 * V3 + 1602 * V4 - 494 * V5) +
+          V3 * (2843 * V3 - 5966 * V4 + 1922 * V5) + V4 * (3443 * V4 - 2522 * V5) +
+          V5 * (547 * V5);
+  Is[2] = V3 * (547 * V3 - 2522 * V4 + 1922 * V5 - 494 * V6) +
+          V4 * (3443 * V4 - 5966 * V5 + 1602 * V6) + V5 * (2843 * V5 - 1642 * V6) +
+          V6 * (267 * V6);
+  Is[3] = V4 * (2107 * V4 - 9402 * V5 + 7042 * V6 - 1854 * V7) +
+          V5 * (11003 * V5 - 17246 * V6 + 4642 * V7) + V6 * (7043 * V6 - 3882 * V7) +
+          V7 * (547 * V7);
+  return Weno7Weights<T>::combine(eps, s0, Is, q);
+}
+
+// One WENO5 reconstruction of a 5-point stencil  (schemes.jl:958-972)
+template <typename T>
+inline T weno5(const T* V, T s0, T eps) {
+  const T V1 = V[0], V2 = V[1], V3 = V[2], V4 = V[3], V5 = V[4];
+  const T a1 = V1 - 2 * V2 + V3, b1 = V1 - 4 * V2 + 3 * V3;
+  const T a2 = V2 - 2 * V3 + V4, b2 = V2 - V4;
+  const T a3 = V3 - 2 * V4 + V5, b3 = 3 * V3 - 4 * V4 + V5;
+  T s11 = 13 * (a1 * a1) + 3 * (b1 * b1);
+  T s22 = 13 * (a2 * a2) + 3 * (b2 * b2);
+  T s33 = 13 * (a3 * a3) + 3 * (b3 * b3);
+  s11 = 1 / ((eps + s11 * s0) * (eps + s11 * s0));
+  s22 = 6 / ((eps + s22 * s0) * (eps + s22 * s0));
+  s33 = 3 / ((eps + s33 * s0) * (eps + s33 * s0));
+  T invsum = 1 / (s11 + s22 + s33);
+  T v1 = 2 * V1 - 7 * V2 + 11 * V3;
+  T v2 = -V2 + 5 * V3 + 2 * V4;
+  T v3 = 2 * V3 + 5 * V4 - V5;
+  return invsum * (s11 * v1 + s22 * v2 + s33 * v3);
+}
+
+// advect_xc / advect_yc / advect_zc  schemes.jl:700-1012 / 1014-1328 / 1330-1643 (index
+// permutations of one another).  F has the face extents of direction d.
+template <typename T>
+void advect_c(const f3d_params* p, int d, T* F, const T* phi, const T* S, const T* Fp,
+              const T* Fm, const T* Q, const T* Ax, const T* Ay, const T* Az) {
+  Grid g(p);
+  Dir D = make_dir(d);
+  const Ext& fe = d == 0 ? g.fx : (d == 1 ? g.fy : g.fz);
+  T c[7];
+  linear_coeffs<T>(p, c);
+  const T tmp1 = T(1) / T(12), tmp2 = T(1) / T(6);
+  const T eps7 = (T)p->weno7_eps, eps5 = (T)p->weno5_eps;
+  const T gam = (T)p->gamma, Rg = (T)p->Rg;
+#pragma omp parallel for collapse(2)
+  for (long k = g.NG + 1; k <= g.Nzp + g.NG + D.dk; ++k)
+    for (long j = g.NG + 1; j <= g.Nyp + g.NG + D.dj; ++j)
+      for (long i = g.NG + 1; i <= g.Nxp + g.NG + D.di; ++i) {
+        auto C = [&](long o) { return g.cell.at(i + o * D.di, j + o * D.dj, k + o * D.dk); };
+        auto out = [&](long n) { return fe.at(i - g.NG, j - g.NG, k - g.NG, n); };
+        const size_t ncell = (size_t)g.Nxt * g.Nyt * g.Nzt;
+        T phix;
+        int sch = face_scheme(p, g, phi, i, j, k, D, &phix);
+        if (sch == F3D_SCHEME_UP7) {  // schemes.jl:731-754
+          for (long n = 1; n <= 5; ++n) {
+            const size_t nn = ncell * (n - 1);
+            T V1 = Fp[C(-4) + nn], V2 = Fp[C(-3) + nn], V3 = Fp[C(-2) + nn], V4 = Fp[C(-1) + nn];
+            T V5 = Fp[C(0) + nn], V6 = Fp[C(1) + nn], V7 = Fp[C(2) + nn];
+            T fp = c[0] * V1 + c[1] * V2 + c[2] * V3 + c[3] * V4 + c[4] * V5 + c[5] * V6 + c[6] * V7;
+            V1 = Fm[C(3) + nn]; V2 = Fm[C(2) + nn]; V3 = Fm[C(1) + nn]; V4 = Fm[C(0) + nn];
+            V5 = Fm[C(-1) + nn]; V6 = Fm[C(-2) + nn]; V7 = Fm[C(-3) + nn];
+            T fm = c[0] * V1 + c[1] * V2 + c[2] * V3 + c[3] * V4 + c[4] * V5 + c[5] * V6 + c[6] * V7;
+            F[out(n)] = fp + fm;
+          }
+          continue;
+        }
+        // characteristic setup  schemes.jl:756-811
+        T s0 = 2 / (S[C(-1)] + S[C(0)]);
+        T u = T(0.5) * (Q[C(0) + ncell * 1] + Q[C(-1) + ncell * 1]);
+        T v = T(0.5) * (Q[C(0) + ncell * 2] + Q[C(-1) + ncell * 2]);
+        T w = T(0.5) * (Q[C(0) + ncell * 3] + Q[C(-1) + ncell * 3]);
+        T Tt = T(0.5) * (Q[C(0) + ncell * 5] + Q[C(-1) + ncell * 5]);
+        T A1 = (Ax[C(0)] + Ax[C(-1)]) * T(0.5);
+        T A2 = (Ay[C(0)] + Ay[C(-1)]) * T(0.5);
+        T A3 = (Az[C(0)] + Az[C(-1)]) * T(0.5);
+        T nx = A1 * s0, ny = A2 * s0, nz = A3 * s0;
+        T lx, ly, lz, ss;
+        if (std::fabs(nz) <= std::fabs(ny)) {
+          ss = 1 / std::sqrt(nx * nx + ny * ny);
+          lx = -ny * ss; ly = nx * ss; lz = 0;
+        } else {
+          ss = 1 / std::sqrt(nx * nx + nz * nz);
+          lx = -nz * ss; ly = 0; lz = nx * ss;
+        }
+        T mx = ny * lz - nz * ly, my = nz * lx - nx * lz, mz = nx * ly - ny * lx;
+        T qn = u * nx + v * ny + w * nz;
+        T ql = u * lx + v * ly + w * lz;
+        T qm = u * mx + v * my + w * mz;
+        T q2 = T(0.5) * (u * u + v * v + w * w);
+        T cc = std::sqrt(gam * Rg * Tt);
+        T c2 = 1 / (2 * cc);
+        T K = (gam - 1) / (cc * cc);
+        T K2 = K * T(0.5);
+        T H = 1 / K + q2;
+        T L[5][5] = {{K2 * q2 + qn * c2, -(K2 * u + nx * c2), -(K2 * v + ny * c2), -(K2 * w + nz * c2), K2},
+                     {1 - K * q2, K * u, K * v, K * w, -K},
+                     {K2 * q2 - qn * c2, -(K2 * u - nx * c2), -(K2 * v - ny * c2), -(K2 * w - nz * c2), K2},
+                     {-ql, lx, ly, lz, 0},
+                     {-qm, mx, my, mz, 0}};
+        T R[5][5] = {{1, 1, 1, 0, 0},
+                     {u - cc * nx, u, u + cc * nx, lx, mx},
+                     {v - cc * ny, v, v + cc * ny, ly, my},
+                     {w - cc * nz, w, w + cc * nz, lz, mz},
+                     {H - qn * cc, q2, H + qn * cc, ql, qm}};
+        T flux[5];
+        if (sch == F3D_SCHEME_WENO7) {  // schemes.jl:813-930
+          for (int n = 0; n < 5; ++n) {
+            T Fpc[7] = {0, 0, 0, 0, 0, 0, 0}, Fmc[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int m = 0; m < 5; ++m) {
+              T ll = L[n][m];
+              const size_t mm = ncell * m;
+              for (int s = 0; s < 7; ++s) {
+                Fpc[s] += Fp[C(s - 4) + mm] * ll;  // Fp[i-4 .. i+2]
+                Fmc[s] += Fm[C(3 - s) + mm] * ll;  // Fm[i+3 .. i-3]
+              }
+            }
+            T fp = weno7(Fpc, s0, eps7);
+            T fm = weno7(Fmc, s0, eps7);
+            flux[n] = (fp + fm) * tmp1;
+          }
+        } else if (sch == F3D_SCHEME_WENO5) {  // schemes.jl:931-999
+          for (int n = 0; n < 5; ++n) {
+            T Fpc[5] = {0, 0, 0, 0, 0}, Fmc[5] = {0, 0, 0, 0, 0};
+            for (int m = 0; m < 5; ++m) {
+              T ll = L[n][m];
+              const size_t mm = ncell * m;
+              for (int s = 0; s < 5; ++s) {
+                Fpc[s] += Fp[C(s - 3) + mm] * ll;  // Fp[i-3 .. i+1]
+                Fmc[s] += Fm[C(2 - s) + mm] * ll;  // Fm[i+2 .. i-2]
+              }
+            }
+            T fp = weno5(Fpc, s0, eps5);
+            T fm = weno5(Fmc, s0, eps5);
+            flux[n] = (fp + fm) * tmp2;
+          }
+        } else {  // NND2, schemes.jl:1000-1009 (component-wise; L and R are built but unused)
+          for (long n = 1; n <= 5; ++n) {
+            const size_t nn = ncell * (n - 1);
+            T fp = Fp[C(-1) + nn] + T(0.5) * minmod(Fp[C(0) + nn] - Fp[C(-1) + nn], Fp[C(-1) + nn] - Fp[C(-2) + nn]);
+            T fm = Fm[C(0) + nn] - T(0.5) * minmod(Fm[C(1) + nn] - Fm[C(0) + nn], Fm[C(0) + nn] - Fm[C(-1) + nn]);
+            F[out(n)] = fp + fm;
+          }
+          continue;
+        }
+        for (int n = 0; n < 5; ++n)  // F = R * flux  schemes.jl:926-930
+          F[out(n + 1)] = flux[0] * R[n][0] + flux[1] * R[n][1] + flux[2] * R[n][2] +
+                          flux[3] * R[n][3] + flux[4] * R[n][4];
+      }
+}
+
+// ---------------------------------------------------------------- split.jl
+// fluxSplit_SW  split.jl:2-68, every cell [1, N_tot]^3
+template <typename T>
+void flux_split_sw(const f3d_params* p, const T* Q, T* Fp, T* Fm, const T* S, const T* Ax,
+                   const T* Ay, const T* Az) {
+  Grid g(p);
+  const T gam = (T)p->gamma, Rg = (T)p->Rg;
+#pragma omp parallel for collapse(2)
+  for (long k = 1; k <= g.Nzt; ++k)
+    for (long j = 1; j <= g.Nyt; ++j)
+      for (long i = 1; i <= g.Nxt; ++i) {
+        T rho = Q[g.cell.at(i, j, k, 1)], u = Q[g.cell.at(i, j, k, 2)];
+        T v = Q[g.cell.at(i, j, k, 3)], w = Q[g.cell.at(i, j, k, 4)];
+        T Tt = Q[g.cell.at(i, j, k, 6)];
+        T A1 = Ax[g.cell.at(i, j, k)], A2 = Ay[g.cell.at(i, j, k)], A3 = Az[g.cell.at(i, j, k)];
+        T ss = S[g.cell.at(i, j, k)];
+        T c = std::sqrt(gam * Rg * Tt);
+        T E1 = A1 * u + A2 * v + A3 * w;
+        T E2 = E1 - c * ss;
+        T E3 = E1 + c * ss;
+        ss = 1 / ss;
+        A1 *= ss; A2 *= ss; A3 *= ss;
+        T E1P = (E1 + std::fabs(E1)) * T(0.5);
+        T E2P = (E2 + std::fabs(E2)) * T(0.5);
+        T E3P = (E3 + std::fabs(E3)) * T(0.5);
+        T E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+        T uc1 = u - c * A1, uc2 = u + c * A1;
+        T vc1 = v - c * A2, vc2 = v + c * A2;
+        T wc1 = w - c * A3, wc2 = w + c * A3;
+        T vvc1 = (uc1 * uc1 + vc1 * vc1 + wc1 * wc1) * T(0.5);
+        T vvc2 = (uc2 * uc2 + vc2 * vc2 + wc2 * wc2) * T(0.5);
+        T vv = (gam - 1) * (u * u + v * v + w * w);
+        T W2 = (3 - gam) / (2 * (gam - 1)) * (c * c);
+        T tmp1 = rho / (2 * gam);
+        T tmp2 = 2 * (gam - 1);
+        T* out[2] = {Fp, Fm};
+        T e1[2] = {E1P, E1M}, e2[2] = {E2P, E2M}, e3[2] = {E3P, E3M};
+        for (int s = 0; s < 2; ++s) {
+          out[s][g.cell.at(i, j, k, 1)] = tmp1 * (tmp2 * e1[s] + e2[s] + e3[s]);
+          out[s][g.cell.at(i, j, k, 2)] = tmp1 * (tmp2 * e1[s] * u + e2[s] * uc1 + e3[s] * uc2);
+          out[s][g.cell.at(i, j, k, 3)] = tmp1 * (tmp2 * e1[s] * v + e2[s] * vc1 + e3[s] * vc2);
+          out[s][g.cell.at(i, j, k, 4)] = tmp1 * (tmp2 * e1[s] * w + e2[s] * wc1 + e3[s] * wc2);
+          out[s][g.cell.at(i, j, k, 5)] =
+              tmp1 * (e1[s] * vv + e2[s] * vvc1 + e3[s] * vvc2 + W2 * (e2[s] + e3[s]));
+        }
+      }
+}
+
+// ---------------------------------------------------------------- viscous.jl
+// viscousFlux  viscous.jl:1-65, 108-133 (laminar; the LES branches 66-106 are off), domain
+// [3, N_tot-2]^3, stored at (i-2, j-2, k-2).  J is double in both forms (cast to T here).
+template <typename T>
+void viscous_flux(const f3d_params* p, T* Fvx, T* Fvy, T* Fvz, const T* Q, const T* const m[9],
+                  const double* J) {
+  Grid g(p);
+  const T Cs = (T)p->C_s, Ts = (T)p->T_s, Cp = (T)p->Cp, Pr = (T)p->Pr;
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3), c23 = T(2) / T(3);
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+#pragma omp parallel for collapse(2)
+  for (long k = 3; k <= g.Nzt - 2; ++k)
+    for (long j = 3; j <= g.Nyt - 2; ++j)
+      for (long i = 3; i <= g.Nxt - 2; ++i) {
+        const size_t o = g.cell.at(i, j, k);
+        T xx = m[0][o], xy = m[1][o], xz = m[2][o];
+        T ex = m[3][o], ey = m[4][o], ez = m[5][o];
+        T zx = m[6][o], zy = m[7][o], zz = m[8][o];
+        T Jac = (T)J[o];
+        T Tt = Q[o + 5 * nc];
+        T mu = Cs * Tt * std::sqrt(Tt) / (Tt + Ts);
+        T lam = Cp * mu / Pr;
+        auto D = [&](long di, long dj, long dk, int n) {
+          const size_t nn = nc * (n - 1);
+          return c1 * (Q[g.cell.at(i - 2 * di, j - 2 * dj, k - 2 * dk) + nn] -
+                       Q[g.cell.at(i + 2 * di, j + 2 * dj, k + 2 * dk) + nn]) +
+                 c2 * (Q[g.cell.at(i - di, j - dj, k - dk) + nn] -
+                       Q[g.cell.at(i + di, j + dj, k + dk) + nn]);
+        };
+        T uxi = D(1, 0, 0, 2), vxi = D(1, 0, 0, 3), wxi = D(1, 0, 0, 4), Txi = D(1, 0, 0, 6);
+        T ueta = D(0, 1, 0, 2), veta = D(0, 1, 0, 3), weta = D(0, 1, 0, 4), Teta = D(0, 1, 0, 6);
+        T uzeta = D(0, 0, 1, 2), vzeta = D(0, 0, 1, 3), wzeta = D(0, 0, 1, 4), Tzeta = D(0, 0, 1, 6);
+        T u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc];
+        T dudx = (uxi * xx + ueta * ex + uzeta * zx) * Jac;
+        T dudy = (uxi * xy + ueta * ey + uzeta * zy) * Jac;
+        T dudz = (uxi * xz + ueta * ez + uzeta * zz) * Jac;
+        T dvdx = (vxi * xx + veta * ex + vzeta * zx) * Jac;
+        T dvdy = (vxi * xy + veta * ey + vzeta * zy) * Jac;
+        T dvdz = (vxi * xz + veta * ez + vzeta * zz) * Jac;
+        T dwdx = (wxi * xx + weta * ex + wzeta * zx) * Jac;
+        T dwdy = (wxi * xy + weta * ey + wzeta * zy) * Jac;
+        T dwdz = (wxi * xz + weta * ez + wzeta * zz) * Jac;
+        T dTdx = (Txi * xx + Teta * ex + Tzeta * zx) * Jac;
+        T dTdy = (Txi * xy + Teta * ey + Tzeta * zy) * Jac;
+        T dTdz = (Txi * xz + Teta * ez + Tzeta * zz) * Jac;
+        T dv = dudx + dvdy + dwdz;
+        T t11 = mu * (2 * dudx - c23 * dv);
+        T t12 = mu * (dudy + dvdx);
+        T t13 = mu * (dudz + dwdx);
+        T t22 = mu * (2 * dvdy - c23 * dv);
+        T t23 = mu * (dwdy + dvdz);
+        T t33 = mu * (2 * dwdz - c23 * dv);
+        T E1 = u * t11 + v * t12 + w * t13 + lam * dTdx;
+        T E2 = u * t12 + v * t22 + w * t23 + lam * dTdy;
+        T E3 = u * t13 + v * t23 + w * t33 + lam * dTdz;
+        T* Fo[3] = {Fvx, Fvy, Fvz};
+        T mr[3][3] = {{xx, xy, xz}, {ex, ey, ez}, {zx, zy, zz}};
+        for (int d = 0; d < 3; ++d) {
+          Fo[d][g.fv.at(i - 2, j - 2, k - 2, 1)] = mr[d][0] * t11 + mr[d][1] * t12 + mr[d][2] * t13;
+          Fo[d][g.fv.at(i - 2, j - 2, k - 2, 2)] = mr[d][0] * t12 + mr[d][1] * t22 + mr[d][2] * t23;
+          Fo[d][g.fv.at(i - 2, j - 2, k - 2, 3)] = mr[d][0] * t13 + mr[d][1] * t23 + mr[d][2] * t33;
+          Fo[d][g.fv.at(i - 2, j - 2, k - 2, 4)] = mr[d][0] * E1 + mr[d][1] * E2 + mr[d][2] * E3;
+        }
+      }
+}
+
+// ---------------------------------------------------------------- div.jl
+// div  div.jl:1-50 : interior (i,j,k) in [1, Np]^3, U index +NG.  The flux differences accumulate
+// in T; the conservative update is in double.
+template <typename T>
+void div(const f3d_params* p, double* U, const T* Fx, const T* Fy, const T* Fz, const T* Fvx,
+         const T* Fvy, const T* Fvz, double dt, const double* J) {
+  Grid g(p);
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  const long NG = g.NG;
+#pragma omp parallel for collapse(2)
+  for (long k = 1; k <= g.Nzp; ++k)
+    for (long j = 1; j <= g.Nyp; ++j)
+      for (long i = 1; i <= g.Nxp; ++i) {
+        double Jact = J[g.cell.at(i + NG, j + NG, k + NG)] * dt;
+        T dV[3][4];
+        for (int n = 1; n <= 4; ++n) {
+          dV[0][n - 1] = c1 * (Fvx[g.fv.at(i, j + 2, k + 2, n)] - Fvx[g.fv.at(i + 4, j + 2, k + 2, n)]) +
+                         c2 * (Fvx[g.fv.at(i + 1, j + 2, k + 2, n)] - Fvx[g.fv.at(i + 3, j + 2, k + 2, n)]);
+          dV[1][n - 1] = c1 * (Fvy[g.fv.at(i + 2, j, k + 2, n)] - Fvy[g.fv.at(i + 2, j + 4, k + 2, n)]) +
+                         c2 * (Fvy[g.fv.at(i + 2, j + 1, k + 2, n)] - Fvy[g.fv.at(i + 2, j + 3, k + 2, n)]);
+          dV[2][n - 1] = c1 * (Fvz[g.fv.at(i + 2, j + 2, k, n)] - Fvz[g.fv.at(i + 2, j + 2, k + 4, n)]) +
+                         c2 * (Fvz[g.fv.at(i + 2, j + 2, k + 1, n)] - Fvz[g.fv.at(i + 2, j + 2, k + 3, n)]);
+        }
+        for (long n = 1; n <= 5; ++n) {
+          T s = Fx[g.fx.at(i, j, k, n)] - Fx[g.fx.at(i + 1, j, k, n)] + Fy[g.fy.at(i, j, k, n)] -
+                Fy[g.fy.at(i, j + 1, k, n)] + Fz[g.fz.at(i, j, k, n)] - Fz[g.fz.at(i, j, k + 1, n)];
+          U[g.cell.at(i + NG, j + NG, k + NG, n)] += (double)s * Jact;
+        }
+        for (int n = 2; n <= 5; ++n) {
+          T s = dV[0][n - 2] + dV[1][n - 2] + dV[2][n - 2];
+          U[g.cell.at(i + NG, j + NG, k + NG, n)] += (double)s * Jact;
+        }
+      }
+}
+
+// ---------------------------------------------------------------- mpi.jl / boundary.jl
+// exchange_ghost with a single rank  mpi.jl:2-103 (+ pack/unpack 105-283): for each periodic
+// dimension in the order x, y, z: low ghosts <- Q[Np+g], high ghosts <- Q[NG+g], over the full
+// extent (ghosts included) of the other two dimensions.
+template <typename T>
+void exchange_ghost(const f3d_params* p, T* Q, int NV) {
+  Grid g(p);
+  const long Np[3] = {g.Nxp, g.Nyp, g.Nzp};
+  for (int d = 0; d < 3; ++d) {
+    if (p->bc[2 * d] != F3D_BC_PERIODIC || p->bc[2 * d + 1] != F3D_BC_PERIODIC) continue;
+    Dir D = make_dir(d);
+    for (long n = 1; n <= NV; ++n)
+#pragma omp parallel for collapse(2)
+      for (long k = 1; k <= g.Nzt; ++k)
+        for (long j = 1; j <= g.Nyt; ++j)
+          for (long i = 1; i <= g.Nxt; ++i) {
+            long idx = d == 0 ? i : (d == 1 ? j : k);
+            long src;
+            if (idx <= g.NG) src = idx + Np[d];
+            else if (idx > Np[d] + g.NG) src = idx - Np[d];
+            else continue;
+            long si = i + (src - idx) * D.di, sj = j + (src - idx) * D.dj, sk = k + (src - idx) * D.dk;
+            Q[g.cell.at(i, j, k, n)] = Q[g.cell.at(si, sj, sk, n)];
+          }
+  }
+}
+
+// Ghost fill  boundary.jl:1-156 generalised to per-face kinds; applied in the order fill_x,
+// fill_y, fill_z (boundary.jl:153-155), each over the full extent of the other two dimensions.
+template <typename T>
+void fill_ghost(const f3d_params* p, T* Q, double* U, const T* inflow) {
+  Grid g(p);
+  const long Np[3] = {g.Nxp, g.Nyp, g.Nzp};
+  const double gam = p->gamma, Rg = p->Rg;
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  for (int d = 0; d < 3; ++d) {
+    for (int side = 0; side < 2; ++side) {
+      const int bc = p->bc[2 * d + side];
+#pragma omp parallel for collapse(2)
+      for (long k = 1; k <= g.Nzt; ++k)
+        for (long j = 1; j <= g.Nyt; ++j)
+          for (long i = 1; i <= g.Nxt; ++i) {
+            const long idx = d == 0 ? i : (d == 1 ? j : k);
+            const bool ghost = side == 0 ? (idx <= g.NG) : (idx > Np[d] + g.NG);
+            if (!ghost) continue;
+            auto shifted = [&](long to) {
+              return g.cell.at(d == 0 ? to : i, d == 1 ? to : j, d == 2 ? to : k);
+            };
+            const size_t o = g.cell.at(i, j, k);
+            double q[6];
+            if (bc == F3D_BC_PERIODIC) {  // fill_z  boundary.jl:132-146
+              const size_t s = shifted(side == 0 ? idx + Np[d] : idx - Np[d]);
+              for (int n = 0; n < 6; ++n) Q[o + n * nc] = Q[s + n * nc];
+              if (U)
+                for (int n = 0; n < 5; ++n) U[o + n * nc] = U[s + n * nc];
+              continue;
+            } else if (bc == F3D_BC_ZEROGRAD) {  // boundary.jl:42-51
+              const size_t s = shifted(side == 0 ? g.NG + 1 : Np[d] + g.NG);
+              for (int n = 0; n < 6; ++n) Q[o + n * nc] = Q[s + n * nc];
+              if (U)
+                for (int n = 0; n < 5; ++n) U[o + n * nc] = U[s + n * nc];
+              continue;
+            } else if (bc == F3D_BC_DIRICHLET) {  // boundary.jl:12-38, 98-118
+              for (int n = 0; n < 6; ++n) q[n] = p->q_inf[n];
+            } else if (bc == F3D_BC_INFLOW) {
+              for (int n = 0; n < 6; ++n)
+                q[n] = inflow[(size_t)(j - 1) + (size_t)g.Nyt * ((size_t)(k - 1) + (size_t)g.Nzt * n)];
+            } else {  // walls: mirror index 2NG+1-idx (boundary.jl:69), or its high-side image
+              const long mi = side == 0 ? 2 * g.NG + 1 - idx : 2 * (Np[d] + g.NG) + 1 - idx;
+              const size_t s = shifted(mi);
+              q[0] = Q[s];
+              q[1] = -(double)Q[s + nc];
+              q[2] = -(double)Q[s + 2 * nc];
+              q[3] = -(double)Q[s + 3 * nc];
+              q[4] = Q[s + 4 * nc];
+              q[5] = Q[s + 5 * nc];
+              if (bc == F3D_BC_WALL_ISOTHERMAL) {
+                q[5] = 2 * p->T_wall - q[5];
+                q[0] = q[4] / (Rg * q[5]);
+              }
+            }
+            for (int n = 0; n < 6; ++n) Q[o + n * nc] = (T)q[n];
+            if (U) {
+              // U from the stored (rounded) Q, as boundary.jl:82-92 reads Q back
+              double rho = (double)Q[o], u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc];
+              double pr = Q[o + 4 * nc];
+              U[o] = rho;
+              U[o + nc] = rho * u;
+              U[o + 2 * nc] = rho * v;
+              U[o + 3 * nc] = rho * w;
+              U[o + 4 * nc] = pr / (gam - 1) + 0.5 * rho * (u * u + v * v + w * w);
+            }
+          }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- solver.jl
+// flowAdvance, finite-difference branch  solver.jl:133-188 (SW splitting, characteristic advect)
+template <typename T>
+void flow_advance(const f3d_params* p, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, T* Fvx, T* Fvy,
+                  T* Fvz, const T* s1, const T* s2, const T* s3, const T* const m[9],
+                  const double* J, T* phi) {
+  flux_split_sw<T>(p, Q, Fp, Fm, s1, m[0], m[1], m[2]);
+  advect_c<T>(p, 0, Fx, phi, s1, Fp, Fm, Q, m[0], m[1], m[2]);
+  flux_split_sw<T>(p, Q, Fp, Fm, s2, m[3], m[4], m[5]);
+  advect_c<T>(p, 1, Fy, phi, s2, Fp, Fm, Q, m[3], m[4], m[5]);
+  flux_split_sw<T>(p, Q, Fp, Fm, s3, m[6], m[7], m[8]);
+  advect_c<T>(p, 2, Fz, phi, s3, Fp, Fm, Q, m[6], m[7], m[8]);
+  if (p->viscous) viscous_flux<T>(p, Fvx, Fvy, Fvz, Q, m, J);
+}
+
+// Scratch arrays of the reference's time_step (solver.jl:294-308), sized for one params.
+template <typename T>
+struct Scratch {
+  std::vector<T> phi, Fp, Fm, Fx, Fy, Fz, Fvx, Fvy, Fvz;
+  explicit Scratch(const f3d_params* p) {
+    Grid g(p);
+    size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+    phi.assign(nc, 0);
+    Fp.assign(nc * 5, 0);
+    Fm.assign(nc * 5, 0);
+    Fx.assign((size_t)(g.Nxp + 1) * g.Nyp * g.Nzp * 5, 0);
+    Fy.assign((size_t)g.Nxp * (g.Nyp + 1) * g.Nzp * 5, 0);
+    Fz.assign((size_t)g.Nxp * g.Nyp * (g.Nzp + 1) * 5, 0);
+    size_t nv = (size_t)(g.Nxp + g.NG) * (g.Nyp + g.NG) * (g.Nzp + g.NG) * 4;
+    Fvx.assign(nv, 0);
+    Fvy.assign(nv, 0);
+    Fvz.assign(nv, 0);
+  }
+};
+
+// |grad xi_d| per cell  solver.jl:286-288
+template <typename T>
+void metric_norms(const f3d_params* p, const T* const m[9], T* s1, T* s2, T* s3) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  T* s[3] = {s1, s2, s3};
+  for (int d = 0; d < 3; ++d)
+    for (size_t o = 0; o < nc; ++o) {
+      T a = m[3 * d][o], b = m[3 * d + 1][o], c = m[3 * d + 2][o];
+      s[d][o] = std::sqrt(a * a + b * b + c * c);
+    }
+}
+
+// One RK3 stage of the time loop body  solver.jl:416-444 (LTS, filtering and I/O off).
+template <typename T>
+void rk3_stage(const f3d_params* p, Scratch<T>& sc, int KRK, double* U, double* Un, T* Q,
+               const T* s1, const T* s2, const T* s3, const T* const m[9], const double* J,
+               double dt, const T* inflow) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  if (KRK == 1) std::memcpy(Un, U, nc * 5 * sizeof(double));  // solver.jl:418
+  shock_sensor<T>(p, sc.phi.data(), Q);                        // solver.jl:424
+  flow_advance<T>(p, Q, sc.Fp.data(), sc.Fm.data(), sc.Fx.data(), sc.Fy.data(), sc.Fz.data(),
+                  sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), s1, s2, s3, m, J, sc.phi.data());
+  div<T>(p, U, sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(),
+         sc.Fvz.data(), dt, J);                                // solver.jl:429
+  if (KRK == 2) lin_comb(p, U, Un, 5, 0.25, 0.75);             // solver.jl:433
+  if (KRK == 3) lin_comb(p, U, Un, 5, 2.0 / 3.0, 1.0 / 3.0);   // solver.jl:435
+  c2prim<T>(p, U, Q);                                          // solver.jl:438
+  exchange_ghost<T>(p, Q, 6);                                  // solver.jl:439
+  fill_ghost<T>(p, Q, U, inflow);                              // solver.jl:443
+}
+
+// Global CFL dt (see f3d_cfl_dt_*): cfl * min over interior of 1/(J * max_d(|A_d.u| + c|A_d|)).
+template <typename T>
+double cfl_dt(const f3d_params* p, const T* Q, const T* const m[9], const double* J, double cfl) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  double best = std::numeric_limits<double>::infinity();
+#pragma omp parallel for collapse(2) reduction(min : best)
+  for (long k = g.NG + 1; k <= g.Nzp + g.NG; ++k)
+    for (long j = g.NG + 1; j <= g.Nyp + g.NG; ++j)
+      for (long i = g.NG + 1; i <= g.Nxp + g.NG; ++i) {
+        size_t o = g.cell.at(i, j, k);
+        double u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc], Tt = Q[o + 5 * nc];
+        double c = std::sqrt(p->gamma * p->Rg * Tt);
+        double lam = 0;
+        for (int d = 0; d < 3; ++d) {
+          double a = m[3 * d][o], b = m[3 * d + 1][o], e = m[3 * d + 2][o];
+          double s = std::sqrt(a * a + b * b + e * e);
+          lam = std::max(lam, std::fabs(a * u + b * v + e * w) + c * s);
+        }
+        best = std::min(best, 1.0 / (J[o] * lam));
+      }
+  return cfl * best;
+}
+
+// Metrics from coordinates  parse_mesh.jl:139-248 (FP64 throughout).
+inline void metrics(const f3d_params* p, const double* x, const double* y, const double* z,
+                    double* const m[9], double* J) {
+  Grid g(p);
+  const long N[3] = {g.Nxt, g.Nyt, g.Nzt};
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  std::vector<double> d(9 * nc);  // dx/dxi, dx/deta, dx/dzeta, dy/..., dz/...
+  const double* X[3] = {x, y, z};
+  for (int c = 0; c < 3; ++c)
+    for (int dir = 0; dir < 3; ++dir) {
+      Dir D = make_dir(dir);
+      double* out = &d[(size_t)(3 * c + dir) * nc];
+#pragma omp parallel for collapse(2)
+      for (long k = 1; k <= g.Nzt; ++k)
+        for (long j = 1; j <= g.Nyt; ++j)
+          for (long i = 1; i <= g.Nxt; ++i) {
+            long idx = dir == 0 ? i : (dir == 1 ? j : k);
+            auto f = [&](long o) {  // value at idx+o along dir
+              return X[c][g.cell.at(i + o * D.di, j + o * D.dj, k + o * D.dk)];
+            };
+            double r;
+            if (idx >= 4 && idx <= N[dir] - 3)  // CD6  parse_mesh.jl:139-142
+              r = 1.0 / 60 * (f(3) - f(-3)) - 3.0 / 20 * (f(2) - f(-2)) + 3.0 / 4 * (f(1) - f(-1));
+            else if (idx <= 3)  // CD2_L on f[i..i+2]  parse_mesh.jl:144-147
+              r = 2 * f(1) - 0.5 * f(2) - 1.5 * f(0);
+            else  // CD2_R on f[i-2..i]  parse_mesh.jl:149-152
+              r = -2 * f(-1) + 0.5 * f(-2) + 1.5 * f(0);
+            out[g.cell.at(i, j, k)] = r;
+          }
+    }
+  const double *xa = &d[0], *xb = &d[nc], *xc = &d[2 * nc];
+  const double *ya = &d[3 * nc], *yb = &d[4 * nc], *yc = &d[5 * nc];
+  const double *za = &d[6 * nc], *zb = &d[7 * nc], *zc = &d[8 * nc];
+#pragma omp parallel for
+  for (size_t o = 0; o < nc; ++o) {
+    J[o] = 1 / (xa[o] * (yb[o] * zc[o] - yc[o] * zb[o]) - xb[o] * (ya[o] * zc[o] - yc[o] * za[o]) +
+                xc[o] * (ya[o] * zb[o] - yb[o] * za[o]));  // parse_mesh.jl:237
+    m[0][o] = yb[o] * zc[o] - yc[o] * zb[o];  // parse_mesh.jl:240-248
+    m[1][o] = xc[o] * zb[o] - xb[o] * zc[o];
+    m[2][o] = xb[o] * yc[o] - xc[o] * yb[o];
+    m[3][o] = yc[o] * za[o] - ya[o] * zc[o];
+    m[4][o] = xa[o] * zc[o] - xc[o] * za[o];
+    m[5][o] = xc[o] * ya[o] - xa[o] * yc[o];
+    m[6][o] = ya[o] * zb[o] - yb[o] * za[o];
+    m[7][o] = xb[o] * za[o] - xa[o] * zb[o];
+    m[8][o] = xa[o] * yb[o] - xb[o] * ya[o];
+  }
+}
+
+}  // namespace f3d_oracle

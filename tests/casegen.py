@@ -1,0 +1,127 @@
+"""Seeded small cases for parity tests: inputs prepared on the CPU with the oracle (ghost fill,
+metrics), identical bytes then handed to the oracle and to the GPU library."""
+import importlib
+
+import numpy as np
+
+from oracle_lib import Oracle
+
+f3d = importlib.import_module("flame3d-mhd_amd")
+cases = importlib.import_module("flame3d-mhd_amd.cases")
+
+
+class Case:
+    """params + Q (ghosts filled) + 9 metrics + J (+ s1..s3, inflow) in one numeric form."""
+
+    def __init__(self, params, Q, metrics, J, inflow=None, name=""):
+        self.params = params
+        self.Q = Q
+        self.metrics = metrics
+        self.J = J
+        self.inflow = inflow
+        self.name = name
+        bulk = Q.dtype
+        o = Oracle(params)
+        self.s = [np.zeros_like(J, dtype=bulk) for _ in range(3)]
+        o.norms(self.metrics, *self.s)
+
+    @property
+    def bulk(self):
+        return self.Q.dtype
+
+    def U(self):
+        U = np.zeros(self.params.cell_shape(5))
+        Oracle(self.params).prim2c(U, self.Q)
+        return U
+
+
+def _fill(params, Q, inflow):
+    o = Oracle(params)
+    U = np.zeros(params.cell_shape(5))
+    o.exchange(Q, 6)
+    o.fill(Q, U, inflow)
+
+
+def plant_spikes(Q, spikes):
+    """Multiply p and rho (T unchanged) of single interior cells: (i, j, k, factor), 0-based interior
+    indices.  A factor-20 spike gives phi ~ 2.7 (WENO5, or NND when hybrid_phi3 < 2.7) and a factor-3
+    spike phi ~ 1.5 (WENO5) at the faces within three cells."""
+    for (i, j, k, f) in spikes:
+        Q[0, k + 4, j + 4, i + 4] *= f
+        Q[4, k + 4, j + 4, i + 4] *= f
+    return Q
+
+
+SPIKES16 = [(3, 4, 5, 20.0), (11, 12, 10, 3.0), (8, 2, 13, 20.0), (13, 9, 3, 3.0)]
+
+
+def tgv_case(N=16, dtype=np.float64, noise=1e-3, jump=0.6, seed=1, viscous=1, spikes=SPIKES16, **pkw):
+    c = cases.tgv(N=N, noise=noise, jump=jump, seed=seed)
+    kw = {k: v for k, v in c["params"].items() if k != "viscous"}
+    kw.update(pkw)
+    p = f3d.make_params(N, N, N, viscous=viscous, **kw)
+    Q = plant_spikes(c["Q"].copy(), spikes or [])
+    Q = np.ascontiguousarray(Q.astype(dtype))
+    _fill(p, Q, None)
+    m = [np.ascontiguousarray(a.astype(dtype)) for a in c["metrics"]]
+    return Case(p, Q, m, np.ascontiguousarray(c["J"]), None, name=f"tgv{N}")
+
+
+def ramp_case(Nx=24, Ny=16, Nz=8, sweep=0.0, dtype=np.float64, seed=2, viscous=1, noise=0.01, jump=0.0,
+              spikes=(), **pkw):
+    c = cases.ramp(Nx, Ny, Nz, sweep=sweep, seed=seed, noise=noise)
+    kw = dict(c["params"])
+    kw["viscous"] = viscous
+    kw.update(pkw)
+    p = f3d.make_params(Nx, Ny, Nz, **kw)
+    o = Oracle(p)
+    m64 = [np.zeros(p.cell_shape()) for _ in range(9)]
+    J = np.zeros(p.cell_shape())
+    o.metrics(c["x"], c["y"], c["z"], m64, J)
+    Q = c["Q"].copy()
+    if jump:
+        # a planted pressure jump (shock-like) across the middle of the domain in x
+        sl = slice(4 + Nx // 2, None)
+        Q[4, :, :, sl] *= 1 + jump
+        Q[0, :, :, sl] *= 1 + jump
+    Q = plant_spikes(Q, spikes)
+    Q = np.ascontiguousarray(Q.astype(dtype))
+    inflow = np.ascontiguousarray(c["inflow"].astype(dtype))
+    _fill(p, Q, inflow)
+    m = [np.ascontiguousarray(a.astype(dtype)) for a in m64]
+    return Case(p, Q, m, J, inflow, name=f"ramp{Nx}x{Ny}x{Nz}s{sweep:g}")
+
+
+def scaled_err(a, b, scale):
+    """max over variables n of max|a_n - b_n| / scale[n] (scale: per-variable magnitude of the
+    terms that produced b, e.g. from term_scales)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return max(float(np.max(np.abs(a[n] - b[n]))) / (scale[n] if scale[n] > 0 else 1.0) for n in range(a.shape[0]))
+
+
+def face_scales(F3):
+    """Per-variable scale of face fluxes: max |F_n| over the faces of all three directions."""
+    return [max(float(np.max(np.abs(F[n]))) for F in F3) for n in range(5)]
+
+
+def rhs_scales(case):
+    """Per-variable scale of the RHS: max over cells of J * sum |terms| (oracle_rhs_absterms)."""
+    from oracle_lib import Oracle
+    A = np.zeros(case.params.cell_shape(5))
+    Oracle(case.params).rhs_absterms(A, case.Q, case.metrics, case.J)
+    return [float(np.max(A[n])) for n in range(5)]
+
+
+def relerr(a, b, axis_vars=True):
+    """max|a-b| / max|b| per leading variable (the normwise relative error of the parity bar)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if axis_vars and a.ndim == 4:
+        out = []
+        for n in range(a.shape[0]):
+            den = np.max(np.abs(b[n]))
+            out.append(np.max(np.abs(a[n] - b[n])) / (den if den > 0 else 1.0))
+        return max(out)
+    den = np.max(np.abs(b))
+    return np.max(np.abs(a - b)) / (den if den > 0 else 1.0)

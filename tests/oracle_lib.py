@@ -1,0 +1,123 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle_f3d.so).  Test infrastructure only.
+
+Arrays are numpy, C-order with the reversed (Julia column-major) shape used by the package.
+"""
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_void_p
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle_f3d.so")
+
+_lib = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    return ORACLE_SO
+
+
+def olib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        L = ctypes.CDLL(ORACLE_SO)
+        L.oracle_cfl_dt_f64.restype = c_double
+        L.oracle_cfl_dt_f32b.restype = c_double
+        _lib = L
+    return _lib
+
+
+def P(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return c_void_p(a.ctypes.data)
+
+
+def PP(arrs):
+    arr = (c_void_p * len(arrs))()
+    for i, a in enumerate(arrs):
+        assert a.flags["C_CONTIGUOUS"]
+        arr[i] = a.ctypes.data
+    return arr
+
+
+def suf(a):
+    return "f64" if a.dtype == np.float64 else "f32b"
+
+
+class Oracle:
+    """Oracle calls bound to one params struct (a flame3d F3DParams instance)."""
+
+    def __init__(self, params):
+        self.p = params
+        self.L = olib()
+
+    def _c(self, name, *args):
+        getattr(self.L, name)(ctypes.byref(self.p), *args)
+
+    def shock_sensor(self, phi, Q):
+        self._c(f"oracle_shock_sensor_{suf(Q)}", P(phi), P(Q))
+
+    def scheme_mask(self, mx, my, mz, phi):
+        self._c(f"oracle_scheme_mask_{suf(phi)}", P(mx), P(my), P(mz), P(phi))
+
+    def flux_split(self, Q, Fp, Fm, S, Ax, Ay, Az):
+        self._c(f"oracle_flux_split_sw_{suf(Q)}", P(Q), P(Fp), P(Fm), P(S), P(Ax), P(Ay), P(Az))
+
+    def advect(self, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
+        self._c(f"oracle_advect_c_{suf(Q)}", c_int(d), P(F), P(phi), P(S), P(Fp), P(Fm), P(Q), P(Ax), P(Ay), P(Az))
+
+    def viscous(self, Fvx, Fvy, Fvz, Q, m, J):
+        self._c(f"oracle_viscous_flux_{suf(Q)}", P(Fvx), P(Fvy), P(Fvz), P(Q), PP(m), P(J))
+
+    def div(self, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, dt, J):
+        self._c(f"oracle_div_{suf(Fx)}", P(U), P(Fx), P(Fy), P(Fz), P(Fvx), P(Fvy), P(Fvz), c_double(dt), P(J))
+
+    def lin_comb(self, U, Un, NV, a, b):
+        self._c("oracle_lin_comb", P(U), P(Un), c_int(NV), c_double(a), c_double(b))
+
+    def c2prim(self, U, Q):
+        self._c(f"oracle_c2prim_{suf(Q)}", P(U), P(Q))
+
+    def prim2c(self, U, Q):
+        self._c(f"oracle_prim2c_{suf(Q)}", P(U), P(Q))
+
+    def exchange(self, Q, NV):
+        self._c(f"oracle_exchange_ghost_{suf(Q)}", P(Q), c_int(NV))
+
+    def fill(self, Q, U, inflow=None):
+        self._c(f"oracle_fill_ghost_{suf(Q)}", P(Q), P(U), P(inflow))
+
+    def norms(self, m, s1, s2, s3):
+        self._c(f"oracle_metric_norms_{suf(s1)}", PP(m), P(s1), P(s2), P(s3))
+
+    def rhs(self, R, Q, m, J, phi=None, masks=(None, None, None)):
+        self._c(f"oracle_rhs_{suf(Q)}", P(R), P(Q), PP(m), P(J), P(phi), *map(P, masks))
+
+    def rhs_absterms(self, A, Q, m, J):
+        """J * sum of |terms| of the RHS per cell and variable (scale of the parity bar)."""
+        self._c(f"oracle_rhs_absterms_{suf(Q)}", P(A), P(Q), PP(m), P(J))
+
+    def advect_ld(self, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
+        """advect_c in long double from float64 inputs (rounding-floor reference)."""
+        self._c("oracle_advect_c_ld", c_int(d), P(F), P(phi), P(S), P(Fp), P(Fm), P(Q), P(Ax), P(Ay), P(Az))
+
+    def time_step(self, nsteps, U, Un, Q, m, J, dt, inflow=None):
+        self._c(f"oracle_time_step_{suf(Q)}", c_int(nsteps), P(U), P(Un), P(Q), PP(m), P(J), c_double(dt), P(inflow))
+
+    def cfl_dt(self, Q, m, J, cfl):
+        return getattr(self.L, f"oracle_cfl_dt_{suf(Q)}")(ctypes.byref(self.p), P(Q), PP(m), P(J), c_double(cfl))
+
+    def metrics(self, x, y, z, m, J):
+        self._c("oracle_metrics_f64", P(x), P(y), P(z), PP(m), P(J))
+
+
+def max_threads():
+    return olib().oracle_max_threads()
