@@ -57,7 +57,23 @@ def kernel_alg_bytes(name, p, es):
     return 0
 
 
-FUSED_BYTES = {}
+def _nint(p):
+    return p.Nxp * p.Nyp * p.Nzp
+
+
+# Compulsory bytes per launch of the fused kernels (es = bulk element size; U, Un, J are double).
+FUSED_BYTES = {
+    # p in, phi out on every cell
+    "fused_sensor": lambda p, es: p.tot[0] * p.tot[1] * p.tot[2] * 2 * es,
+    # rho,u,v,w,T + 3 metrics + phi in, 5 RHS values out, per interior cell
+    "fused_sweep_x": lambda p, es: _nint(p) * (9 * es + 5 * es),
+    # as x, but the RHS is read and written
+    "fused_sweep_y": lambda p, es: _nint(p) * (9 * es + 10 * es),
+    "fused_sweep_z": lambda p, es: _nint(p) * (9 * es + 10 * es),
+    # RHS 5 + viscous fluxes 12 + J in; U, Un in; U and Q out (stage-2/3 form)
+    "fused_update": lambda p, es: _nint(p) * (5 * es + 12 * es + 8 + 40 + 40 + 40 + 6 * es),
+    "fused_heavy": lambda p, es: 0,
+}
 
 
 def build_case(f3d, cases, torch, nx, ny, nz, seed=2):
@@ -152,6 +168,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32b", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the ramp's whole subdomain on one MI355X); "
@@ -160,6 +178,14 @@ def main():
     import numpy as np
     import torch
     assert torch.cuda.is_available(), "bench needs a GPU"
+    if args.copy_probe:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")  # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copies": 5}))
+        return
     f3d = importlib.import_module("flame3d-mhd_amd")
     cases = importlib.import_module("flame3d-mhd_amd.cases")
     lib = f3d.lib()

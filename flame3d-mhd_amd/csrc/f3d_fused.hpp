@@ -7,11 +7,14 @@ namespace f3d {
 // Workspace arrays the fused path uses (bulk type T).
 template <typename T>
 struct WsView {
-  T* R;    // interior RHS accumulator [5][Nzp][Nyp][Nxp]
-  T* Fvx;  // viscous flux scratch, reference Fv layout
+  T* R;                      // interior RHS accumulator [5][Nzp][Nyp][Nxp]
+  T* Fvx;                    // viscous flux scratch, reference Fv layout [4][Nzp+4][Nyp+4][Nxp+4]
   T* Fvy;
   T* Fvz;
-  T* phi;
+  T* phi;                    // shock sensor [Nzt][Nyt][Nxt]
+  unsigned long long* heavy; // compacted non-UP7 faces: (right cell << 4) | (scheme << 2) | dir
+  unsigned int* hcount;      // number of entries in `heavy`
+  unsigned long long hcap;   // capacity of `heavy` (total face count of the three directions)
 };
 
 }  // namespace f3d
@@ -26,10 +29,10 @@ WsView<T> ws_view(f3d_workspace* ws);
 // kernel-by-kernel chain instead, for A/B measurement).
 bool use_fused();
 
-// RHS into R (U layout, interior) when R != nullptr.
+// RHS into R (U layout, interior, double): R = J*(-dF + dFv).
 template <typename T>
 int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J, T* phi,
-              double* R, int KRK, double dt, double* U, hipStream_t s);
+              double* R, hipStream_t s);
 
 // One RK3 stage up to and including c2Prim of the interior (ghost fill done by the caller).
 template <typename T>

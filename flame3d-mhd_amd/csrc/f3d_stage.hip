@@ -19,6 +19,9 @@ struct f3d_workspace {
   double* red;   // reduction partials
   int* flag;     // NaN flag partials
   int nred;      // number of partial slots
+  unsigned long long* heavy;  // compacted non-UP7 face list (fused path)
+  unsigned int* hcount;
+  unsigned long long hcap;
 };
 
 namespace {
@@ -206,7 +209,7 @@ int rhs_impl(const f3d_params* p, f3d_workspace* ws, double* R, const T* Q, cons
   T* ph = phi ? phi : (T*)ws->phi;
   int rc;
   if (use_fused()) {
-    rc = fused_rhs<T>(kp, ws_view<T>(ws), Q, m, J, ph, R, 0, 0.0, nullptr, s);
+    rc = fused_rhs<T>(kp, ws_view<T>(ws), Q, m, J, ph, R, s);
   } else {
     rc = rhs_chain<T>(ws, kp, Q, m, J, ph, s);
     if (rc) return rc;
@@ -283,7 +286,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
   const size_t nc = f3d_cell_count(p);
   const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
-  size_t sz[12];
+  size_t sz[14];
   sz[0] = nc * es;                               // phi
   sz[1] = nc * 5 * es;                           // Fp
   sz[2] = nc * 5 * es;                           // Fm
@@ -295,6 +298,9 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[9] = Nx * Ny * Nz * 5 * es;                 // R
   sz[10] = kRedBlocks * sizeof(double);          // red
   sz[11] = kRedBlocks * sizeof(int);             // flag
+  const size_t nfaces = (Nx + 1) * Ny * Nz + Nx * (Ny + 1) * Nz + Nx * Ny * (Nz + 1);
+  sz[12] = nfaces * sizeof(unsigned long long);  // heavy-face list
+  sz[13] = 256;                                  // heavy-face counter
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -305,13 +311,14 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   ws->bytes = total;
   ws->base = base;
   char* c = (char*)base;
-  void** slots[12] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
-                      &ws->R, (void**)&ws->red, (void**)&ws->flag};
-  for (int i = 0; i < 12; ++i) {
+  void** slots[14] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+                      &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount};
+  for (int i = 0; i < 14; ++i) {
     *slots[i] = c;
     c += align_up(sz[i]);
   }
   ws->nred = kRedBlocks;
+  ws->hcap = nfaces;
   // zeroed once: inviscid runs read zero viscous fluxes in div (div.jl:45-48 adds them always)
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, 0, (double*)base, (long long)(total / 8));
   hipError_t e = hipDeviceSynchronize();
@@ -347,6 +354,9 @@ WsView<T> ws_view(f3d_workspace* ws) {
   v.Fvy = (T*)ws->Fvy;
   v.Fvz = (T*)ws->Fvz;
   v.phi = (T*)ws->phi;
+  v.heavy = ws->heavy;
+  v.hcount = ws->hcount;
+  v.hcap = ws->hcap;
   return v;
 }
 template WsView<double> ws_view<double>(f3d_workspace*);
