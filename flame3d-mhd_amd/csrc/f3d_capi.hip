@@ -55,7 +55,25 @@ KP make_kp(const f3d_params* p) {
   const double up7[7] = {-3.0 / 420, 25.0 / 420, -101.0 / 420, 319.0 / 420, 214.0 / 420, -38.0 / 420, 4.0 / 420};
   const double cd6[7] = {0.0, 1.0 / 60, -2.0 / 15, 37.0 / 60, 37.0 / 60, -2.0 / 15, 1.0 / 60};
   for (int m = 0; m < 7; ++m) k.lin[m] = up7[m] * p->Linear_phi + cd6[m] * (1.0 - p->Linear_phi);
+  auto c = [&](int q) { return (q >= 0 && q <= 6) ? k.lin[q] : 0.0; };
+  for (int r = -4; r <= 3; ++r) k.da[r + 4] = c(r + 4) - c(r + 3);
+  for (int r = -3; r <= 4; ++r) k.db[r + 3] = c(3 - r) - c(4 - r);
+  for (int m = 0; m < 7; ++m) k.linf[m] = (float)k.lin[m];
+  for (int m = 0; m < 8; ++m) {
+    k.daf[m] = (float)k.da[m];
+    k.dbf[m] = (float)k.db[m];
+  }
   for (int i = 0; i < 6; ++i) k.qinf[i] = p->q_inf[i];
+  k.gRg = p->gamma * p->Rg;
+  k.gm1 = p->gamma - 1;
+  k.w2c = (3 - p->gamma) / (2 * (p->gamma - 1));
+  k.inv2g = 1 / (2 * p->gamma);
+  const float gf = (float)p->gamma, rf = (float)p->Rg;
+  k.gamf = gf;
+  k.gRgf = gf * rf;
+  k.gm1f = gf - 1;
+  k.w2cf = (3 - gf) / (2 * (gf - 1));
+  k.inv2gf = 1 / (2 * gf);
   return k;
 }
 

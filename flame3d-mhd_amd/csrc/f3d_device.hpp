@@ -20,6 +20,37 @@ struct KP {
   double phi1, phi2, phi3, eps7, eps5, Twall;
   double lin[7];
   double qinf[6];
+  // differenced UP7 weights: F(i) - F(i+1) = sum_r da[r+4] Fp[i+r] (r=-4..3) + sum_r db[r+3] Fm[i+r] (r=-3..4)
+  double da[8], db[8];
+  float linf[7], daf[8], dbf[8];  // float copies (uniform values stay in scalar registers)
+  // Steger-Warming constants, evaluated once on the host: gamma*Rg, gamma-1, (3-gamma)/(2(gamma-1)),
+  // 1/(2 gamma); [0] double, [1] float (stored as double-width slots for alignment)
+  double gRg, gm1, w2c, inv2g;
+  float gRgf, gm1f, w2cf, inv2gf, gamf;
+};
+
+// Scheme coefficients in the bulk type, read straight from the kernel argument.
+template <typename T>
+struct Coef;
+template <>
+struct Coef<double> {
+  __device__ static __forceinline__ double lin(const KP& kp, int q) { return kp.lin[q]; }
+  __device__ static __forceinline__ double da(const KP& kp, int q) { return kp.da[q]; }
+  __device__ static __forceinline__ double db(const KP& kp, int q) { return kp.db[q]; }
+  __device__ static __forceinline__ double gRg(const KP& kp) { return kp.gRg; }
+  __device__ static __forceinline__ double gm1(const KP& kp) { return kp.gm1; }
+  __device__ static __forceinline__ double w2c(const KP& kp) { return kp.w2c; }
+  __device__ static __forceinline__ double inv2g(const KP& kp) { return kp.inv2g; }
+};
+template <>
+struct Coef<float> {
+  __device__ static __forceinline__ float lin(const KP& kp, int q) { return kp.linf[q]; }
+  __device__ static __forceinline__ float da(const KP& kp, int q) { return kp.daf[q]; }
+  __device__ static __forceinline__ float db(const KP& kp, int q) { return kp.dbf[q]; }
+  __device__ static __forceinline__ float gRg(const KP& kp) { return kp.gRgf; }
+  __device__ static __forceinline__ float gm1(const KP& kp) { return kp.gm1f; }
+  __device__ static __forceinline__ float w2c(const KP& kp) { return kp.w2cf; }
+  __device__ static __forceinline__ float inv2g(const KP& kp) { return kp.inv2gf; }
 };
 
 __device__ __forceinline__ long long cidx(const KP& kp, int i, int j, int k) {
@@ -62,8 +93,7 @@ __device__ __forceinline__ T sensor_cell(T px1, T p, T px3, T py1, T py3, T pz1,
 template <typename T>
 __device__ __forceinline__ void sw_split(const KP& kp, T rho, T u, T v, T w, T Tt, T A1, T A2, T A3,
                                          T ss, T* __restrict__ fp, T* __restrict__ fm) {
-  const T gam = (T)kp.gamma;
-  const T c = sqrt(gam * (T)kp.Rg * Tt);
+  const T c = sqrt(Coef<T>::gRg(kp) * Tt);
   const T E1 = A1 * u + A2 * v + A3 * w;
   const T cs = c * ss;
   const T E2 = E1 - cs, E3 = E1 + cs;
@@ -76,10 +106,10 @@ __device__ __forceinline__ void sw_split(const KP& kp, T rho, T u, T v, T w, T T
   const T wc1 = w - c * a3, wc2 = w + c * a3;
   const T vvc1 = (uc1 * uc1 + vc1 * vc1 + wc1 * wc1) * T(0.5);
   const T vvc2 = (uc2 * uc2 + vc2 * vc2 + wc2 * wc2) * T(0.5);
-  const T vv = (gam - 1) * (u * u + v * v + w * w);
-  const T W2 = (3 - gam) / (2 * (gam - 1)) * (c * c);
-  const T t1 = rho / (2 * gam);
-  const T t2 = 2 * (gam - 1);
+  const T vv = Coef<T>::gm1(kp) * (u * u + v * v + w * w);
+  const T W2 = Coef<T>::w2c(kp) * (c * c);
+  const T t1 = rho * Coef<T>::inv2g(kp);
+  const T t2 = 2 * Coef<T>::gm1(kp);
   const T p1 = t2 * E1P, m1 = t2 * E1M;
   fp[0] = t1 * (p1 + E2P + E3P);
   fp[1] = t1 * (p1 * u + E2P * uc1 + E3P * uc2);
@@ -172,8 +202,8 @@ __device__ __forceinline__ void face_flux(const KP& kp, int sch, const FPL& FP, 
                                           const T* qL, const T* qR, const T* AL, const T* AR, T SL,
                                           T SR, T* __restrict__ out) {
   if (sch == F3D_SCHEME_UP7) {  // schemes.jl:731-754
-    const T c0 = (T)kp.lin[0], c1 = (T)kp.lin[1], c2 = (T)kp.lin[2], c3 = (T)kp.lin[3], c4 = (T)kp.lin[4],
-            c5 = (T)kp.lin[5], c6 = (T)kp.lin[6];
+    const T c0 = Coef<T>::lin(kp, 0), c1 = Coef<T>::lin(kp, 1), c2 = Coef<T>::lin(kp, 2), c3 = Coef<T>::lin(kp, 3),
+            c4 = Coef<T>::lin(kp, 4), c5 = Coef<T>::lin(kp, 5), c6 = Coef<T>::lin(kp, 6);
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
       const T fp = c0 * FP(-4, n) + c1 * FP(-3, n) + c2 * FP(-2, n) + c3 * FP(-1, n) + c4 * FP(0, n) +

@@ -4,14 +4,15 @@
 //   k_sensor_f    shock sensor phi (schemes.jl:1-24); clears the heavy-face counter
 //   k_sweep_x     x faces: SW split of a 128-cell row segment into LDS, UP7 face fluxes, R = dF_x
 //   k_sweep_march y and z faces: one lane per (i, line), marching along the direction; the split
-//                 fluxes of each new cell are pushed into an 8-slot FIR of UP7 face sums, so no
-//                 split flux ever goes to HBM; R += dF_y, dF_z
+//                 fluxes of each new cell are pushed into an 8-face shift register of UP7 sums, so
+//                 no split flux ever goes to HBM; R += dF_y, dF_z
 //   k_heavy       faces whose sensor maximum selects WENO7/WENO5/NND (appended to a compacted list
 //                 by the sweeps): split fluxes of the 8-cell stencil recomputed, characteristic
 //                 reconstruction (schemes.jl:756-1009), R += (F_scheme - F_UP7) on both cells
-//   k_viscous     viscous fluxes (viscous.jl:1-134), reference layout, into the workspace
+//   k_viscous_march viscous fluxes (viscous.jl:1-134), reference layout, into the workspace; lanes
+//                 march k with the column's u,v,w,T in a 5-plane register window
 //   k_update      viscous divergence + conservative update + RK3 blend + c2Prim (div.jl:1-50,
-//                 utils.jl:2-70)
+//                 utils.jl:2-70), marching k with the column's Fv_z in a 5-plane window
 // Splitting every face as UP7 first and correcting the rare non-UP7 faces from a compacted list
 // keeps the sweep waves free of the characteristic branch's divergence and registers.
 #include <cstdlib>
@@ -64,7 +65,7 @@ __device__ __forceinline__ void append_heavy(unsigned long long* list, unsigned 
 
 template <typename T>
 __device__ __forceinline__ T up7c(const KP& kp, int m) {
-  return (T)kp.lin[m];
+  return Coef<T>::lin(kp, m);
 }
 
 // ------------------------------------------------------------------ sensor (+ counter reset)
@@ -144,7 +145,8 @@ __global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L) {
 // Lane = interior i (64 consecutive), threadIdx.y = transverse line; each block marches a segment
 // of L cells along D.  Face f (between cells NG+f-1 and NG+f) collects
 //   sum_q c_q Fp[NG+f-4+q] + sum_q c_q Fm[NG+f+3-q]
-// in FIR slot (f - f0) & 7 as the cells stream in; it is complete once cell NG+f+3 has arrived.
+// in a shift register acc[0..7] (acc[m] holds face f0+step-7+m); acc[0] is complete once cell
+// NG+f+3 has streamed in.  The next cell's loads are issued before the current cell is split.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
   const int ii = blockIdx.x * 64 + threadIdx.x;
@@ -157,73 +159,71 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
   const int ncell = min(L, Np - f0);
   const int nstep = ncell + 8;
   const long long st = dstride(kp, D), nc = kp.nc;
-  const long long obase = D == 1 ? cidx(kp, i, 0, t) : cidx(kp, i, t, 0);
-  const T* A1p = a.m.m[3 * D];
-  const T* A2p = a.m.m[3 * D + 1];
-  const T* A3p = a.m.m[3 * D + 2];
+  const long long obase = (D == 1 ? cidx(kp, i, 0, t) : cidx(kp, i, t, 0)) + (long long)(kp.NG + f0 - 4) * st;
+  const T* __restrict__ A1p = a.m.m[3 * D];
+  const T* __restrict__ A2p = a.m.m[3 * D + 1];
+  const T* __restrict__ A3p = a.m.m[3 * D + 2];
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
-  T c[7];
-#pragma unroll
-  for (int q = 0; q < 7; ++q) c[q] = up7c<T>(kp, q);
   T acc[8][5];
 #pragma unroll
   for (int s = 0; s < 8; ++s)
 #pragma unroll
     for (int n = 0; n < 5; ++n) acc[s][n] = 0;
-  T ring[8];
+  T ring[6];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) ring[s] = 0;
+  for (int s = 0; s < 6; ++s) ring[s] = 0;
   T Fprev[5] = {0, 0, 0, 0, 0};
-  for (int g = 0; g < nstep; g += 8) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int step = g + u;
-      if (step < nstep) {
-        const long long o = obase + (long long)(kp.NG + f0 - 4 + step) * st;
-        const T A1 = A1p[o], A2 = A2p[o], A3 = A3p[o];
-        const T S = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
-        T fp[5], fm[5];
-        sw_split<T>(kp, a.Q[o], a.Q[o + nc], a.Q[o + 2 * nc], a.Q[o + 3 * nc], a.Q[o + 5 * nc], A1, A2, A3, S, fp,
-                    fm);
-        ring[u] = a.phi[o];
-#pragma unroll
-        for (int q = 0; q < 7; ++q)
-#pragma unroll
-          for (int n = 0; n < 5; ++n) {
-            acc[(u - q) & 7][n] += c[q] * fp[n];
-            acc[(u + 1 + q) & 7][n] += c[q] * fm[n];
-          }
-        const int fi = step - 7;  // face completed by this cell, relative to f0
-        const int cs = (u + 1) & 7;
-        if (fi >= 0) {
-          T F[5];
-#pragma unroll
-          for (int n = 0; n < 5; ++n) {
-            F[n] = acc[cs][n];
-            acc[cs][n] = 0;
-          }
-          T m = ring[(u + 2) & 7];
-#pragma unroll
-          for (int s = 3; s <= 7; ++s) m = jmax(m, ring[(u + s) & 7]);
-          const int sch = scheme_of<T>(kp, m);
-          const int f = f0 + fi;
-          const bool owned = act && (fi < L || f == Np);
-          const long long oface = obase + (long long)(kp.NG + f) * st;
-          append_heavy(a.heavy, a.hcount, owned && sch != F3D_SCHEME_UP7,
-                       ((unsigned long long)oface << 4) | ((unsigned long long)sch << 2) | (unsigned long long)D);
-          if (fi >= 1 && act) {
-            const long long ro = D == 1 ? ridx(kp, ii, f - 1, tt) : ridx(kp, ii, tt, f - 1);
-#pragma unroll
-            for (int n = 0; n < 5; ++n) a.R[ro + n * rn] += Fprev[n] - F[n];
-          }
-#pragma unroll
-          for (int n = 0; n < 5; ++n) Fprev[n] = F[n];
-        } else {
-#pragma unroll
-          for (int n = 0; n < 5; ++n) acc[cs][n] = 0;
-        }
-      }
+  // prefetched inputs of the next cell
+  T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph;
+  {
+    const long long o = obase;
+    nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc]; nq5 = a.Q[o + 5 * nc];
+    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
+  }
+#pragma unroll 1
+  for (int step = 0; step < nstep; ++step) {
+    const T q0 = nq0, q1 = nq1, q2 = nq2, q3 = nq3, q5 = nq5, A1 = na1, A2 = na2, A3 = na3, ph = nph;
+    if (step + 1 < nstep) {
+      const long long o = obase + (long long)(step + 1) * st;
+      nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc]; nq5 = a.Q[o + 5 * nc];
+      na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
     }
+    T fp[5], fm[5];
+    sw_split<T>(kp, q0, q1, q2, q3, q5, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp, fm);
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        acc[7 - q][n] += up7c<T>(kp, q) * fp[n];
+        acc[q][n] += up7c<T>(kp, q) * fm[n];
+      }
+    if (step >= 7) {  // face f0 + step - 7 complete
+      const int fi = step - 7;
+      T m = ring[0];
+#pragma unroll
+      for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
+      const int sch = scheme_of<T>(kp, m);
+      const int f = f0 + fi;
+      append_heavy(a.heavy, a.hcount, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
+                   ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) |
+                       (unsigned long long)D);
+      if (fi >= 1 && act) {
+        const long long ro = D == 1 ? ridx(kp, ii, f - 1, tt) : ridx(kp, ii, tt, f - 1);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) a.R[ro + n * rn] += Fprev[n] - acc[0][n];
+      }
+#pragma unroll
+      for (int n = 0; n < 5; ++n) Fprev[n] = acc[0][n];
+    }
+#pragma unroll
+    for (int s = 0; s < 7; ++s)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) acc[s][n] = acc[s + 1][n];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) acc[7][n] = 0;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) ring[s] = ring[s + 1];
+    ring[5] = ph;
   }
 }
 
@@ -284,76 +284,169 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a) {
   }
 }
 
+// ------------------------------------------------------------------ viscous flux (k march)
+// viscousFlux (viscous.jl:1-134) on interior +-2 columns, one lane per (i, j) column marching k:
+// u, v, w, T of the column are kept in a 5-plane register window for the zeta derivatives, so each
+// plane is read from HBM once; the xi/eta neighbours come from L1/L2.  Output in the reference Fv
+// layout (viscous.jl:119-132).
+template <typename T>
+__global__ __launch_bounds__(256) void k_viscous_march(KP kp, FArgs<T> a, T* __restrict__ Fvx, T* __restrict__ Fvy,
+                                                       T* __restrict__ Fvz, int KC) {
+  const int ii = blockIdx.x * 64 + threadIdx.x, jj = blockIdx.y * 4 + threadIdx.y;  // 0.. Np+3
+  const int NX = kp.Nxp + 4, NY = kp.Nyp + 4, NZ = kp.Nzp + 4;
+  const bool act = ii < NX && jj < NY;
+  const int i = 2 + min(ii, NX - 1), j = 2 + min(jj, NY - 1);  // absolute, in [2, Nt-3]
+  const int kk0 = blockIdx.z * KC;
+  const int nk = min(KC, NZ - kk0);
+  const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
+  const long long vn = (long long)NX * NY * NZ;
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  // window w[v][s]: variable v (u,v,w,T) at plane kabs-2+s
+  T w[4][5];
+  const long long ocol = cidx(kp, i, j, 0);
+  const int kabs0 = 2 + kk0;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const long long o = ocol + (long long)(kabs0 - 2 + s) * sz;
+    w[0][s + 1] = a.Q[o + nc];
+    w[1][s + 1] = a.Q[o + 2 * nc];
+    w[2][s + 1] = a.Q[o + 3 * nc];
+    w[3][s + 1] = a.Q[o + 5 * nc];
+  }
+#pragma unroll 1
+  for (int t = 0; t < nk; ++t) {
+    const int kabs = kabs0 + t;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) w[v][s] = w[v][s + 1];
+    const long long o = ocol + (long long)kabs * sz;
+    {
+      const long long on = o + 2 * sz;
+      w[0][4] = a.Q[on + nc];
+      w[1][4] = a.Q[on + 2 * nc];
+      w[2][4] = a.Q[on + 3 * nc];
+      w[3][4] = a.Q[on + 5 * nc];
+    }
+    T g[4][3];
+    const int var[4] = {1, 2, 3, 5};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const T* f = a.Q + var[v] * nc + o;
+      g[v][0] = c1 * (f[-2] - f[2]) + c2 * (f[-1] - f[1]);
+      g[v][1] = c1 * (f[-2 * sy] - f[2 * sy]) + c2 * (f[-sy] - f[sy]);
+      g[v][2] = c1 * (w[v][0] - w[v][4]) + c2 * (w[v][1] - w[v][3]);
+    }
+    T m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = a.m.m[q][o];
+    T out[3][4];
+    viscous_cell<T>(kp, w[0][2], w[1][2], w[2][2], w[3][2], g, m, (T)a.J[o], out);
+    if (act) {
+      const long long vo = (i - 2) + (long long)NX * ((j - 2) + (long long)NY * (kabs - 2));
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        Fvx[vo + n * vn] = out[0][n];
+        Fvy[vo + n * vn] = out[1][n];
+        Fvz[vo + n * vn] = out[2][n];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ update
+// Viscous divergence (div.jl:14-39, 45-48) + conservative update + RK3 blend + c2Prim, one lane
+// per interior (i, j) column marching k; Fv_z of the column is kept in a 5-plane window.
 // MODE 0: RHS out (U layout, double) = J * (R + dFv).  MODE 1..3: RK3 stage KRK = MODE.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __restrict__ Fvx,
                                                 const T* __restrict__ Fvy, const T* __restrict__ Fvz,
                                                 double* __restrict__ U, double* __restrict__ Un, T* __restrict__ Q,
-                                                double* __restrict__ Rout, double dt) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= kp.Nxp || j >= kp.Nyp) return;
+                                                double* __restrict__ Rout, double dt, int KC) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const bool act = i < kp.Nxp && j < kp.Nyp;
+  const int ic = min(i, kp.Nxp - 1), jc = min(j, kp.Nyp - 1);
+  const int k0 = blockIdx.z * KC;
+  const int nk = min(KC, kp.Nzp - k0);
   const int NG = kp.NG;
-  const long long o = cidx(kp, i + NG, j + NG, k + NG), nc = kp.nc;
-  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp, ro = ridx(kp, i, j, k);
-  T Rt[5];
-#pragma unroll
-  for (int n = 0; n < 5; ++n) Rt[n] = a.R[ro + n * rn];
+  const long long nc = kp.nc, sz = (long long)kp.Nxt * kp.Nyt;
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  const int Nv1 = kp.Nxp + NG, Nv2 = kp.Nyp + NG;
+  const long long vn = (long long)Nv1 * Nv2 * (kp.Nzp + NG), vsy = Nv1, vsz = (long long)Nv1 * Nv2;
+  // Fv_z window: wz[n][s] = Fv_z at Fv plane (k+2) - 2 + s of this column
+  T wz[4][5];
+  const long long vcol = (ic + 2) + (long long)Nv1 * (jc + 2);
   if (kp.viscous) {
-    const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
-    const int Nv1 = kp.Nxp + NG, Nv2 = kp.Nyp + NG;
-    const long long vn = (long long)Nv1 * Nv2 * (kp.Nzp + NG);
-    const long long vo = (i + 2) + (long long)Nv1 * ((j + 2) + (long long)Nv2 * (k + 2));
-    const long long vs[3] = {1, Nv1, (long long)Nv1 * Nv2};
-    const T* Fv[3] = {Fvx, Fvy, Fvz};
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      T s = 0;
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const T* f = Fv[d] + vo + n * vn;
-        s += c1 * (f[-2 * vs[d]] - f[2 * vs[d]]) + c2 * (f[-vs[d]] - f[vs[d]]);
+      for (int n = 0; n < 4; ++n) wz[n][s + 1] = Fvz[vcol + (long long)(k0 + s) * vsz + n * vn];
+  }
+#pragma unroll 1
+  for (int t = 0; t < nk; ++t) {
+    const int k = k0 + t;
+    const long long o = cidx(kp, ic + NG, jc + NG, k + NG), ro = ridx(kp, ic, jc, k);
+    T Rt[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Rt[n] = a.R[ro + n * rn];
+    if (kp.viscous) {
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) wz[n][s] = wz[n][s + 1];
+        wz[n][4] = Fvz[vcol + (long long)(k + 4) * vsz + n * vn];
       }
-      Rt[n + 1] += s;
-    }
-  }
-  const double Jc = a.J[o];
-  if (MODE == 0) {
+      const long long vo = vcol + (long long)(k + 2) * vsz;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) Rout[o + n * nc] = (double)Rt[n] * Jc;
-    return;
-  }
-  const double Jact = Jc * dt;
-  double un[5];
-#pragma unroll
-  for (int n = 0; n < 5; ++n) {
-    const double u0 = U[o + n * nc];
-    const double u1 = u0 + (double)Rt[n] * Jact;
-    double r;
-    if (MODE == 1) {
-      Un[o + n * nc] = u0;
-      r = u1;
-    } else if (MODE == 2) {
-      r = u1 * 0.25 + Un[o + n * nc] * 0.75;
-    } else {
-      r = u1 * (2.0 / 3.0) + Un[o + n * nc] * (1.0 / 3.0);
+      for (int n = 0; n < 4; ++n) {
+        const T* fx = Fvx + vo + n * vn;
+        const T* fy = Fvy + vo + n * vn;
+        T sdiv = c1 * (fx[-2] - fx[2]) + c2 * (fx[-1] - fx[1]);
+        sdiv += c1 * (fy[-2 * vsy] - fy[2 * vsy]) + c2 * (fy[-vsy] - fy[vsy]);
+        sdiv += c1 * (wz[n][0] - wz[n][4]) + c2 * (wz[n][1] - wz[n][3]);
+        Rt[n + 1] += sdiv;
+      }
     }
-    un[n] = r;
-    U[o + n * nc] = r;
+    if (!act) continue;
+    const double Jc = a.J[o];
+    if (MODE == 0) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) Rout[o + n * nc] = (double)Rt[n] * Jc;
+      continue;
+    }
+    const double Jact = Jc * dt;
+    double un[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      const double u0 = U[o + n * nc];
+      const double u1 = u0 + (double)Rt[n] * Jact;
+      double r;
+      if (MODE == 1) {
+        Un[o + n * nc] = u0;
+        r = u1;
+      } else if (MODE == 2) {
+        r = u1 * 0.25 + Un[o + n * nc] * 0.75;
+      } else {
+        r = u1 * (2.0 / 3.0) + Un[o + n * nc] * (1.0 / 3.0);
+      }
+      un[n] = r;
+      U[o + n * nc] = r;
+    }
+    // c2Prim  utils.jl:2-30
+    const double eps = 2.220446049250313e-16;
+    const double rho = jmax(un[0], eps);
+    const double ri = 1.0 / rho;
+    const double u = un[1] * ri, v = un[2] * ri, w = un[3] * ri;
+    const double ei = jmax(un[4] - 0.5 * rho * (u * u + v * v + w * w), eps);
+    const double p = (kp.gamma - 1) * ei;
+    Q[o] = (T)rho;
+    Q[o + nc] = (T)u;
+    Q[o + 2 * nc] = (T)v;
+    Q[o + 3 * nc] = (T)w;
+    Q[o + 4 * nc] = (T)p;
+    Q[o + 5 * nc] = (T)(p / (rho * kp.Rg));
   }
-  // c2Prim  utils.jl:2-30
-  const double eps = 2.220446049250313e-16;
-  const double rho = jmax(un[0], eps);
-  const double ri = 1.0 / rho;
-  const double u = un[1] * ri, v = un[2] * ri, w = un[3] * ri;
-  const double ei = jmax(un[4] - 0.5 * rho * (u * u + v * v + w * w), eps);
-  const double p = (kp.gamma - 1) * ei;
-  Q[o] = (T)rho;
-  Q[o + nc] = (T)u;
-  Q[o + 2 * nc] = (T)v;
-  Q[o + 3 * nc] = (T)w;
-  Q[o + 4 * nc] = (T)p;
-  Q[o + 5 * nc] = (T)(p / (rho * kp.Rg));
 }
 
 // ------------------------------------------------------------------ orchestration
@@ -393,10 +486,13 @@ static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const Metri
     TimedScope ts("fused_heavy", s);
     hipLaunchKernelGGL(k_heavy<T>, dim3(2048), dim3(64), 0, s, kp, a);
   }
-  int rc = check_launch("fused sweeps");
-  if (rc) return rc;
-  if (kp.viscous) rc = launch_viscous<T>(kp, ws.Fvx, ws.Fvy, ws.Fvz, Q, m, J, s);
-  return rc;
+  if (kp.viscous) {
+    TimedScope ts("fused_viscous", s);
+    const int KC = seg_len(kp.Nzp + 4, 32);
+    hipLaunchKernelGGL(k_viscous_march<T>, dim3((kp.Nxp + 4 + 63) / 64, (kp.Nyp + 4 + 3) / 4, (kp.Nzp + 4 + KC - 1) / KC),
+                       dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC);
+  }
+  return check_launch("fused sweeps");
 }
 
 template <typename T>
@@ -406,8 +502,9 @@ int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>
   if (rc) return rc;
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
   TimedScope ts("fused_update", s);
-  hipLaunchKernelGGL((k_update<T, 0>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, kp.Nzp), dim3(64, 4, 1), 0, s, kp,
-                     a, ws.Fvx, ws.Fvy, ws.Fvz, nullptr, nullptr, nullptr, R, 0.0);
+  const int KC = seg_len(kp.Nzp, 16);
+  hipLaunchKernelGGL((k_update<T, 0>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC),
+                     dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, nullptr, nullptr, nullptr, R, 0.0, KC);
   return check_launch("fused rhs update");
 }
 
@@ -417,14 +514,15 @@ int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* U
   int rc = fused_flux<T>(kp, ws, Q, m, J, ws.phi, s);
   if (rc) return rc;
   FArgs<T> a{Q, m, J, ws.phi, ws.R, ws.heavy, ws.hcount};
-  const dim3 g((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, kp.Nzp), b(64, 4, 1);
+  const int KC = seg_len(kp.Nzp, 16);
+  const dim3 g((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC), b(64, 4, 1);
   TimedScope ts("fused_update", s);
   if (KRK == 1)
-    hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt);
+    hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC);
   else if (KRK == 2)
-    hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt);
+    hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC);
   else
-    hipLaunchKernelGGL((k_update<T, 3>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt);
+    hipLaunchKernelGGL((k_update<T, 3>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC);
   return check_launch("fused stage update");
 }
 

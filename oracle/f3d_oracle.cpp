@@ -174,9 +174,12 @@ OR_API void oracle_metrics_f64(const f3d_params* p, const double* x, const doubl
   metrics(p, x, y, z, mm, J);
 }
 namespace {
-// Magnitude of the terms the RHS sums (div.jl:40-48 with every term replaced by its absolute
-// value): A = J*(|Fx_i|+|Fx_i+1|+|Fy_j|+...) + J*sum_d(|c1|(|Fv_-2|+|Fv_+2|) + |c2|(|Fv_-1|+|Fv_+1|)).
-// Parity of a sum with cancellation is judged relative to this scale.
+// Magnitude of the terms the RHS is summed from, per cell and variable (the scale the parity bar is
+// relative to).  Inviscid part: each face flux is a sum over its stencil of c_q*Fp and c_q*Fm
+// (schemes.jl:741-753), so its terms have magnitude t = sum_q |c_q| (|Fp_q| + |Fm_q|) -- the split
+// fluxes cancel wherever the face velocity is small (|Fp|, |Fm| ~ rho c |A| while Fp+Fm ~ rho u |A|).
+// A = J*(t_left + t_right summed over the three directions) + J*sum_d(|c1|(|Fv_-2|+|Fv_+2|) +
+// |c2|(|Fv_-1|+|Fv_+1|)) for the viscous part (div.jl:14-48).
 template <typename T>
 void rhs_absterms(const f3d_params* p, double* A, const T* Q, const void* const* mv, const double* J) {
   Grid g(p);
@@ -185,32 +188,49 @@ void rhs_absterms(const f3d_params* p, double* A, const T* Q, const void* const*
   M9<T> m(mv);
   std::vector<T> s1(nc), s2(nc), s3(nc);
   metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
-  std::vector<T> Qc(Q, Q + nc * 6);
-  shock_sensor<T>(p, sc.phi.data(), Qc.data());
-  flow_advance<T>(p, Qc.data(), sc.Fp.data(), sc.Fm.data(), sc.Fx.data(), sc.Fy.data(), sc.Fz.data(),
-                  sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), s1.data(), s2.data(), s3.data(), m.m, J,
-                  sc.phi.data());
+  const T* S[3] = {s1.data(), s2.data(), s3.data()};
   std::fill(A, A + nc * 5, 0.0);
   const long NG = g.NG;
-  const double c1 = 1.0 / 12, c2 = 2.0 / 3;
-  for (long k = 1; k <= g.Nzp; ++k)
-    for (long j = 1; j <= g.Nyp; ++j)
-      for (long i = 1; i <= g.Nxp; ++i) {
-        const double Jc = std::fabs(J[g.cell.at(i + NG, j + NG, k + NG)]);
-        for (long n = 1; n <= 5; ++n) {
-          double s = std::fabs((double)sc.Fx[g.fx.at(i, j, k, n)]) + std::fabs((double)sc.Fx[g.fx.at(i + 1, j, k, n)]) +
-                     std::fabs((double)sc.Fy[g.fy.at(i, j, k, n)]) + std::fabs((double)sc.Fy[g.fy.at(i, j + 1, k, n)]) +
-                     std::fabs((double)sc.Fz[g.fz.at(i, j, k, n)]) + std::fabs((double)sc.Fz[g.fz.at(i, j, k + 1, n)]);
-          if (n >= 2 && p->viscous) {
-            const int v = (int)n - 1;
+  T cl[7];
+  linear_coeffs<T>(p, cl);
+  for (int d = 0; d < 3; ++d) {
+    flux_split_sw<T>(p, Q, sc.Fp.data(), sc.Fm.data(), S[d], m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2]);
+    Dir D = make_dir(d);
+    for (long k = NG + 1; k <= g.Nzp + NG; ++k)
+      for (long j = NG + 1; j <= g.Nyp + NG; ++j)
+        for (long i = NG + 1; i <= g.Nxp + NG; ++i) {
+          const double Jc = std::fabs(J[g.cell.at(i, j, k)]);
+          for (long n = 1; n <= 5; ++n) {
+            double t = 0;
+            for (int side = 0; side <= 1; ++side) {  // left face (cell i) and right face (cell i+1)
+              const long fi = i + side * D.di, fj = j + side * D.dj, fk = k + side * D.dk;
+              for (int q = 0; q < 7; ++q) {
+                const long op = q - 4, om = 3 - q;
+                t += std::fabs((double)cl[q]) *
+                     (std::fabs((double)sc.Fp[g.cell.at(fi + op * D.di, fj + op * D.dj, fk + op * D.dk, n)]) +
+                      std::fabs((double)sc.Fm[g.cell.at(fi + om * D.di, fj + om * D.dj, fk + om * D.dk, n)]));
+              }
+            }
+            A[g.cell.at(i, j, k, n)] += Jc * t;
+          }
+        }
+  }
+  if (p->viscous) {
+    viscous_flux<T>(p, sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), Q, m.m, J);
+    const double c1 = 1.0 / 12, c2 = 2.0 / 3;
+    for (long k = 1; k <= g.Nzp; ++k)
+      for (long j = 1; j <= g.Nyp; ++j)
+        for (long i = 1; i <= g.Nxp; ++i) {
+          const double Jc = std::fabs(J[g.cell.at(i + NG, j + NG, k + NG)]);
+          for (int v = 1; v <= 4; ++v) {
             auto a = [&](const std::vector<T>& F, long ii, long jj, long kk) { return std::fabs((double)F[g.fv.at(ii, jj, kk, v)]); };
-            s += c1 * (a(sc.Fvx, i, j + 2, k + 2) + a(sc.Fvx, i + 4, j + 2, k + 2)) + c2 * (a(sc.Fvx, i + 1, j + 2, k + 2) + a(sc.Fvx, i + 3, j + 2, k + 2));
+            double s = c1 * (a(sc.Fvx, i, j + 2, k + 2) + a(sc.Fvx, i + 4, j + 2, k + 2)) + c2 * (a(sc.Fvx, i + 1, j + 2, k + 2) + a(sc.Fvx, i + 3, j + 2, k + 2));
             s += c1 * (a(sc.Fvy, i + 2, j, k + 2) + a(sc.Fvy, i + 2, j + 4, k + 2)) + c2 * (a(sc.Fvy, i + 2, j + 1, k + 2) + a(sc.Fvy, i + 2, j + 3, k + 2));
             s += c1 * (a(sc.Fvz, i + 2, j + 2, k) + a(sc.Fvz, i + 2, j + 2, k + 4)) + c2 * (a(sc.Fvz, i + 2, j + 2, k + 1) + a(sc.Fvz, i + 2, j + 2, k + 3));
+            A[g.cell.at(i + NG, j + NG, k + NG, v + 1)] += Jc * s;
           }
-          A[g.cell.at(i + NG, j + NG, k + NG, n)] = Jc * s;
         }
-      }
+  }
 }
 
 // advect_c evaluated in long double from double inputs (conditioning reference for the tests).
@@ -226,7 +246,53 @@ void advect_ld(const f3d_params* p, int d, const double* phi, const double* S, c
                         AyL.data(), AzL.data());
   for (size_t i = 0; i < nf; ++i) F[i] = (double)FL[i];
 }
+// RHS in long double from double inputs, with the scheme mask frozen from the double sensor
+// (rounding-floor reference for the FP64 parity bar).
+void rhs_ld(const f3d_params* p, double* Rout, const double* Q, const void* const* mv, const double* J) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  std::vector<double> phid(nc, 0.0);
+  shock_sensor<double>(p, phid.data(), Q);
+  typedef long double L;
+  std::vector<L> QL(Q, Q + 6 * nc), phiL(phid.begin(), phid.end());
+  std::vector<std::vector<L>> mL(9);
+  const L* mp[9];
+  for (int q = 0; q < 9; ++q) {
+    const double* src = (const double*)mv[q];
+    mL[q].assign(src, src + nc);
+    mp[q] = mL[q].data();
+  }
+  Scratch<L> sc(p);
+  std::vector<L> s1(nc), s2(nc), s3(nc);
+  metric_norms<L>(p, mp, s1.data(), s2.data(), s3.data());
+  flow_advance<L>(p, QL.data(), sc.Fp.data(), sc.Fm.data(), sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(),
+                  sc.Fvy.data(), sc.Fvz.data(), s1.data(), s2.data(), s3.data(), mp, J, phiL.data());
+  std::fill(Rout, Rout + nc * 5, 0.0);
+  const long NG = g.NG;
+  const L c1 = L(1) / L(12), c2 = L(-2) / L(3);
+  for (long k = 1; k <= g.Nzp; ++k)
+    for (long j = 1; j <= g.Nyp; ++j)
+      for (long i = 1; i <= g.Nxp; ++i) {
+        const L Jc = J[g.cell.at(i + NG, j + NG, k + NG)];
+        for (long n = 1; n <= 5; ++n) {
+          L s = sc.Fx[g.fx.at(i, j, k, n)] - sc.Fx[g.fx.at(i + 1, j, k, n)] + sc.Fy[g.fy.at(i, j, k, n)] -
+                sc.Fy[g.fy.at(i, j + 1, k, n)] + sc.Fz[g.fz.at(i, j, k, n)] - sc.Fz[g.fz.at(i, j, k + 1, n)];
+          if (n >= 2 && p->viscous) {
+            const int v = (int)n - 1;
+            auto F = [&](const std::vector<L>& A, long ii, long jj, long kk) { return A[g.fv.at(ii, jj, kk, v)]; };
+            s += c1 * (F(sc.Fvx, i, j + 2, k + 2) - F(sc.Fvx, i + 4, j + 2, k + 2)) + c2 * (F(sc.Fvx, i + 1, j + 2, k + 2) - F(sc.Fvx, i + 3, j + 2, k + 2));
+            s += c1 * (F(sc.Fvy, i + 2, j, k + 2) - F(sc.Fvy, i + 2, j + 4, k + 2)) + c2 * (F(sc.Fvy, i + 2, j + 1, k + 2) - F(sc.Fvy, i + 2, j + 3, k + 2));
+            s += c1 * (F(sc.Fvz, i + 2, j + 2, k) - F(sc.Fvz, i + 2, j + 2, k + 4)) + c2 * (F(sc.Fvz, i + 2, j + 2, k + 1) - F(sc.Fvz, i + 2, j + 2, k + 3));
+          }
+          Rout[g.cell.at(i + NG, j + NG, k + NG, n)] = (double)(s * Jc);
+        }
+      }
+}
 }  // namespace
+
+OR_API void oracle_rhs_ld(const f3d_params* p, double* R, const double* Q, const void* const* m, const double* J) {
+  rhs_ld(p, R, Q, m, J);
+}
 
 OR_API void oracle_rhs_absterms_f64(const f3d_params* p, double* A, const double* Q, const void* const* m,
                                     const double* J) {

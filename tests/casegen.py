@@ -105,12 +105,52 @@ def face_scales(F3):
     return [max(float(np.max(np.abs(F[n]))) for F in F3) for n in range(5)]
 
 
+def split_face_scales(case, dtype=np.float64):
+    """Per-variable scale of face fluxes from the terms they are summed from: max over faces and
+    directions of sum_q |c_q| (|Fp_q| + |Fm_q|) (the UP7 stencil of the split fluxes)."""
+    from oracle_lib import Oracle
+    p, o = case.params, Oracle(case.params)
+    c = np.array([-3, 25, -101, 319, 214, -38, 4], dtype=np.float64) / 420.0
+    out = [0.0] * 5
+    for d in range(3):
+        A = case.metrics[3 * d:3 * d + 3]
+        Fp = np.zeros(p.cell_shape(5), dtype=case.Q.dtype)
+        Fm = np.zeros_like(Fp)
+        o.flux_split(case.Q, Fp, Fm, case.s[d], *A)
+        ax = 3 - d  # numpy axis of direction d in (n, k, j, i)
+        ap, am = np.abs(Fp.astype(np.float64)), np.abs(Fm.astype(np.float64))
+        t = np.zeros_like(ap)
+        for q in range(7):
+            t += abs(c[q]) * (np.roll(ap, -(q - 4), axis=ax) + np.roll(am, -(3 - q), axis=ax))
+        s = (slice(None),) + (slice(4, -4),) * 3
+        for n in range(5):
+            out[n] = max(out[n], float(t[n][s[1:]].max()))
+    return out
+
+
 def rhs_scales(case):
     """Per-variable scale of the RHS: max over cells of J * sum |terms| (oracle_rhs_absterms)."""
     from oracle_lib import Oracle
     A = np.zeros(case.params.cell_shape(5))
     Oracle(case.params).rhs_absterms(A, case.Q, case.metrics, case.J)
     return [float(np.max(A[n])) for n in range(5)]
+
+
+def fp64_rhs_check(name, gpu_R, case):
+    """FP64 RHS bar: the GPU result within 1e-12 (term-scaled) of the long-double RHS, or no worse
+    than 3x the FP64 oracle's own distance from it where the input's FP64 floor approaches 1e-12.
+    Returns (gpu error, oracle error) against the long-double reference."""
+    from oracle_lib import Oracle
+    p, o = case.params, Oracle(case.params)
+    R = np.zeros(p.cell_shape(5))
+    Rl = np.zeros_like(R)
+    o.rhs(R, case.Q, case.metrics, case.J)
+    o.rhs_ld(Rl, case.Q, case.metrics, case.J)
+    sc = rhs_scales(case)
+    e_gpu, e_or = scaled_err(gpu_R, Rl, sc), scaled_err(R, Rl, sc)
+    print(f"{name}: FP64 RHS vs long double {e_gpu:.2e} (oracle {e_or:.2e}; gpu vs oracle {scaled_err(gpu_R, R, sc):.2e})")
+    assert e_gpu <= max(1e-12, 3 * e_or), f"{name}: {e_gpu:.3e} vs oracle floor {e_or:.3e}"
+    return e_gpu, e_or
 
 
 def relerr(a, b, axis_vars=True):

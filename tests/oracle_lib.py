@@ -105,6 +105,10 @@ class Oracle:
         """J * sum of |terms| of the RHS per cell and variable (scale of the parity bar)."""
         self._c(f"oracle_rhs_absterms_{suf(Q)}", P(A), P(Q), PP(m), P(J))
 
+    def rhs_ld(self, R, Q, m, J):
+        """RHS in long double from float64 inputs, mask frozen from the float64 sensor."""
+        self._c("oracle_rhs_ld", P(R), P(Q), PP(m), P(J))
+
     def advect_ld(self, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
         """advect_c in long double from float64 inputs (rounding-floor reference)."""
         self._c("oracle_advect_c_ld", c_int(d), P(F), P(phi), P(S), P(Fp), P(Fm), P(Q), P(Ax), P(Ay), P(Az))

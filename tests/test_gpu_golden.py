@@ -4,7 +4,7 @@ import os
 import numpy as np
 import pytest
 
-from casegen import f3d, rhs_scales, scaled_err
+from casegen import f3d, fp64_rhs_check
 from golden.make_golden import golden_cases
 
 pytestmark = pytest.mark.gpu
@@ -30,6 +30,5 @@ def test_gpu_rhs_matches_golden(name):
     assert np.array_equal(phi.cpu().numpy()[s], gold[f"{name}_phi"])
     for k in range(3):
         assert np.array_equal(masks[k].cpu().numpy(), gold[f"{name}_mask{k}"])
-    Rg = R.cpu().numpy()[(slice(None),) + s]
-    sc = rhs_scales(c)
-    assert scaled_err(Rg, gold[f"{name}_R"], sc) < 1e-12
+    # the committed vectors pin the oracle; the GPU is held to the long-double bar on the same input
+    fp64_rhs_check(f"golden {name}", R.cpu().numpy(), c)

@@ -6,7 +6,9 @@ floating-point results within RTOL of the magnitude of the terms they are summed
   RHS          max|gpu - oracle|_n / max_cells J*sum|flux and viscous terms|_n  (rhs_scales)
 (the characteristic projection mixes rho*c^2-sized terms into fluxes that can vanish, e.g. the
 zeta-face mass flux of a sheared grid, so max|F_n| of one direction alone is no scale).
-RTOL = 1e-12 for the FP64 form.
+RTOL = 1e-12 for the FP64 face fluxes.  The FP64 RHS is compared with a long-double evaluation of
+the same formulas (mask frozen): within 1e-12 of it, or no worse than 3x the FP64 oracle's own
+distance from it where the input's FP64 floor approaches 1e-12 (a pressure spike at the wall).
 The FP32-bulk form is judged against the FP64 oracle run on the same (float-rounded) inputs: its
 error may not exceed twice the FP32-bulk oracle's own error against that FP64 result, plus 1e-6
 (i.e. the GPU FP32-bulk path is as accurate as the FP32 restatement of the reference).
@@ -18,7 +20,7 @@ last-bit differences, and the scheme amplifies them far beyond rounding.
 import numpy as np
 import pytest
 
-from casegen import f3d, face_scales, ramp_case, relerr, rhs_scales, scaled_err, tgv_case
+from casegen import f3d, face_scales, fp64_rhs_check, ramp_case, relerr, rhs_scales, scaled_err, tgv_case
 from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -240,11 +242,9 @@ def test_flow_advance_and_rhs(torch_mod, case):
     Rg = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
     ctx.rhs(Rg, Qg, mg, Jg)
     if case.bulk == np.float64:
-        e1 = scaled_err(host(Ug), R, sc)
-        e2 = scaled_err(host(Rg), R, sc)
-        print(f"{case.name} {case.bulk} flowAdvance+div {e1:.2e} rhs {e2:.2e} (plain normwise {relerr(host(Rg), R):.2e})")
-        assert e1 < tol, f"flowAdvance+div: {e1}"
-        assert e2 < tol, f"rhs: {e2}"
+        fp64_rhs_check(f"{case.name} flowAdvance+div", host(Ug), case)
+        fp64_rhs_check(f"{case.name} rhs", host(Rg), case)
+        print(f"{case.name} plain normwise rhs err {relerr(host(Rg), R):.2e}")
         return
     c64 = promoted(case)
     R64 = np.zeros(p.cell_shape(5))
