@@ -19,7 +19,8 @@ import os
 from ctypes import POINTER, c_double, c_int32, c_size_t, c_void_p
 
 __all__ = [
-    "F3DParams", "F3DError", "F3DNaNError", "Flame3D", "Workspace", "lib", "make_params",
+    "F3DParams", "F3DError", "F3DNaNError", "Flame3D", "Workspace", "AdjointWorkspace", "RHSFunction", "lib",
+    "make_params",
     "BC_PERIODIC", "BC_DIRICHLET", "BC_ZEROGRAD", "BC_WALL_ADIABATIC", "BC_WALL_ISOTHERMAL", "BC_INFLOW",
     "PREC_F64", "PREC_F32B", "SCHEME_UP7", "SCHEME_WENO7", "SCHEME_WENO5", "SCHEME_NND2",
 ]
@@ -96,6 +97,12 @@ def lib():
         L.f3d_workspace_create.argtypes = [POINTER(F3DParams), c_int32, POINTER(c_void_p)]
         L.f3d_workspace_destroy.argtypes = [c_void_p]
         L.f3d_workspace_bytes.argtypes = [c_void_p]
+        L.f3d_adjoint_workspace_create.argtypes = [POINTER(F3DParams), POINTER(c_void_p)]
+        L.f3d_adjoint_workspace_destroy.argtypes = [c_void_p]
+        L.f3d_adjoint_workspace_bytes.argtypes = [c_void_p]
+        L.f3d_adjoint_workspace_bytes.restype = c_size_t
+        L.f3d_adjoint_ckpt_bytes.argtypes = [POINTER(F3DParams), c_int32, c_int32]
+        L.f3d_adjoint_ckpt_bytes.restype = c_size_t
         _lib = L
     return _lib
 
@@ -174,6 +181,35 @@ class Workspace:
             pass
 
 
+class AdjointWorkspace:
+    """Owner of an f3d_adj_workspace (device scratch of the FP64 adjoint, with its own forward
+    workspace for stage recomputation)."""
+
+    def __init__(self, params):
+        self.params = params
+        self._h = c_void_p()
+        _check(lib().f3d_adjoint_workspace_create(ctypes.byref(params), ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def nbytes(self):
+        return lib().f3d_adjoint_workspace_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            _check(lib().f3d_adjoint_workspace_destroy(self._h))
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Flame3D:
     """Context holding the run.jl constants (`params`) and a HIP stream.
 
@@ -185,6 +221,7 @@ class Flame3D:
         self.params = params
         self._stream = stream
         self._ws = {}
+        self._aws = None
 
     # -- plumbing ---------------------------------------------------------------------------
     @property
@@ -325,3 +362,85 @@ class Flame3D:
         """FP64 metrics and Jacobian from node coordinates (parse_mesh.jl:139-248)."""
         _check(lib().f3d_metrics_f64(ctypes.byref(self.params), _ptr(x), _ptr(y), _ptr(z), _ptrs(metrics),
                                      _ptr(J), _ptrs(list(s)), self.stream))
+
+    # -- adjoint (FP64, scheme mask frozen) ------------------------------------------------------
+    def adjoint_workspace(self):
+        if self._aws is None:
+            self._aws = AdjointWorkspace(self.params)
+        return self._aws
+
+    def rhs_vjp(self, lamQ, lamR, Q, metrics, J):
+        """lamQ (all cells, 6 vars) = (dR/dQ)^T lamR for R of `rhs`."""
+        aws = self.adjoint_workspace()
+        _check(lib().f3d_rhs_vjp_f64(ctypes.byref(self.params), aws.handle, _ptr(lamR), _ptr(Q), _ptrs(metrics),
+                                     _ptr(J), _ptr(lamQ), self.stream))
+
+    def ghost_vjp(self, lamQ, Q, lam_inflow=None, lam_bc=None):
+        """Reverse of exchange_ghost + fillGhost: ghost adjoints folded into their sources."""
+        self._call("f3d_ghost_vjp_f64", _ptr(lamQ), _ptr(Q), _ptr(lam_inflow), _ptr(lam_bc))
+
+    def c2Prim_vjp(self, lamU, lamQ, U):
+        self._call("f3d_c2prim_vjp_f64", _ptr(lamU), _ptr(lamQ), _ptr(U))
+
+    def rk3_stage_vjp(self, KRK, lam_out, lam_s, lamUn, Us, Qs, metrics, J, dt, lam_inflow=None, lam_bc=None):
+        """Reverse of `rk3_stage` KRK at its input state (Us, Qs)."""
+        aws = self.adjoint_workspace()
+        _check(lib().f3d_rk3_stage_vjp_f64(ctypes.byref(self.params), aws.handle, c_int32(KRK), _ptr(lam_out),
+                                           _ptr(lam_s), _ptr(lamUn), _ptr(Us), _ptr(Qs), _ptrs(metrics), _ptr(J),
+                                           c_double(dt), _ptr(lam_inflow), _ptr(lam_bc), self.stream))
+
+    def adjoint(self, nsteps, every, U0, metrics, J, dt, W, obj, lamU0, inflow=None, lam_inflow=None, lam_bc=None,
+                ckpt=None):
+        """Checkpointed multi-step adjoint of sum_n dt <W, Q^{n+1}> (see f3d_adjoint_f64).
+        `ckpt` is a float64 device tensor of at least adjoint_ckpt_bytes() bytes (allocated if None)."""
+        import torch
+        need = lib().f3d_adjoint_ckpt_bytes(ctypes.byref(self.params), nsteps, every)
+        if ckpt is None:
+            ckpt = torch.empty(need // 8, dtype=torch.float64, device=U0.device)
+        aws = self.adjoint_workspace()
+        _check(lib().f3d_adjoint_f64(ctypes.byref(self.params), aws.handle, c_int32(nsteps), c_int32(every),
+                                     _ptr(U0), _ptrs(metrics), _ptr(J), c_double(dt), _ptr(inflow), _ptr(W),
+                                     _ptr(ckpt), c_size_t(ckpt.numel() * ckpt.element_size()), _ptr(obj),
+                                     _ptr(lamU0), _ptr(lam_inflow), _ptr(lam_bc), self.stream))
+        return ckpt
+
+    def adjoint_ckpt_bytes(self, nsteps, every):
+        return lib().f3d_adjoint_ckpt_bytes(ctypes.byref(self.params), nsteps, every)
+
+
+def _rhs_function():
+    import torch
+
+    class _RHS(torch.autograd.Function):
+        """R = rhs(Q) (FP64) with its hand-written VJP as the backward (mask frozen)."""
+
+        @staticmethod
+        def forward(ctx, Q, f3d, metrics, J):
+            R = torch.zeros(f3d.params.cell_shape(5), dtype=torch.float64, device=Q.device)
+            f3d.rhs(R, Q, metrics, J)
+            ctx.f3d, ctx.metrics = f3d, metrics
+            ctx.save_for_backward(Q, J)
+            return R
+
+        @staticmethod
+        def backward(ctx, gR):
+            Q, J = ctx.saved_tensors
+            lamQ = torch.empty_like(Q)
+            ctx.f3d.rhs_vjp(lamQ, gR.contiguous(), Q, ctx.metrics, J)
+            return lamQ, None, None, None
+
+    return _RHS
+
+
+class RHSFunction:
+    """`RHSFunction.apply(Q, f3d, metrics, J)`: torch.autograd entry of the FP64 RHS."""
+
+    @staticmethod
+    def apply(Q, f3d, metrics, J):
+        global _RHS_FN
+        if _RHS_FN is None:
+            _RHS_FN = _rhs_function()
+        return _RHS_FN.apply(Q, f3d, metrics, J)
+
+
+_RHS_FN = None

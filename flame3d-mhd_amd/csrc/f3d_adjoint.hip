@@ -1,0 +1,833 @@
+// Adjoint of the FP64 RHS / RK3 stage (hand-written reverse mode, scheme mask frozen) and the
+// checkpointed multi-step adjoint.
+//
+// RHS VJP, lambda (interior, U layout) -> lamQ = (dR/dQ)^T lambda over every cell of Q:
+//   k_adj_lambda     Lam = scale * J * lambda on interior cells (ghost entries stay zero)
+//   k_adj_heavy_list faces whose 6-cell sensor maximum selects WENO7/WENO5/NND, compacted
+//   k_adj_inviscid   every face taken as UP7: the transposed difference-of-UP7 stencil gathers
+//                    dFp, dFm per cell and direction from Lam, then the reverse of the
+//                    Steger-Warming split gives (rho,u,v,w,T) adjoints; lamQ written for all cells
+//   k_adj_heavy      per listed face: (scheme - UP7) reverse through the characteristic WENO /
+//                    NND reconstruction, the frame and the eight splits, atomically added
+//   k_adj_visc1      per cell of the interior+-2 box: transposed viscous divergence -> adjoint of
+//                    the cell's viscous fluxes -> adjoint of its 12 derivatives and (u,v,w,T)
+//   k_adj_visc2      transposed 4th-order derivative stencils gather those into lamQ
+// Stage VJP: RHS VJP with scale a*dt, reverse of the ghost procedure (fill z,y,x then exchange
+// z,y,x; ghost adjoints fold into their sources, Dirichlet / inflow / isothermal-wall values
+// into parameter gradients) and k_adj_finalize (reverse of c2Prim + the RK3 blend).
+#include <cstring>
+
+#include "f3d_adjoint.hpp"
+#include "f3d_fused.hpp"
+#include "f3d_internal.hpp"
+
+using namespace f3d;
+
+struct f3d_adj_workspace {
+  f3d_params p;
+  size_t bytes;
+  void* base;
+  f3d_workspace* fws;  // FP64 forward workspace: sensor, heavy-face list, stage recomputation
+  double* Lam;         // scale * J * lambda (5 vars, cell layout, ghosts zero)
+  double* lamQ;        // adjoint of Q (6 vars, cell layout)
+  double* G;           // viscous partial adjoints (16 vars, cell layout, interior+-2 box)
+  double* Us[3];       // stage inputs U^n, U^1, U^2 of one step (5 vars each)
+  double* Qs[3];       // their primitive states with ghosts (6 vars each)
+  double* Un;          // Un of the step being recomputed
+  double* lam;         // adjoint of the state (5 vars)
+  double* lamUn;       // adjoint of Un (5 vars)
+  double* Ucur;        // forward sweep state (5 vars)
+  double* Qcur;        // forward sweep primitive state (6 vars)
+  double* red;         // reduction partials
+  int nred;
+};
+
+namespace {
+
+constexpr int kObjBlocks = 1024;
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+__device__ __forceinline__ bool interior(int a, int Np, int NG) { return a >= NG && a < NG + Np; }
+
+// ------------------------------------------------------------------ small kernels
+__global__ __launch_bounds__(256) void k_adj_zero(double* __restrict__ a, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    a[t] = 0.0;
+}
+
+__global__ void k_adj_zero_count(unsigned int* c) { *c = 0; }
+
+__global__ __launch_bounds__(256) void k_adj_lambda(KP kp, const double* __restrict__ lam, const double* __restrict__ J,
+                                                    double scale, double* __restrict__ Lam) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxp || j >= kp.Nyp) return;
+  const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG);
+  const double s = scale * J[o];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) Lam[o + n * kp.nc] = s * lam[o + n * kp.nc];
+}
+
+// Faces of direction D whose sensor maximum over cells c-3..c+2 (c = right cell) is not UP7.
+template <int D>
+__global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __restrict__ phi,
+                                                        unsigned long long* __restrict__ list,
+                                                        unsigned int* __restrict__ count) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  const int nx = kp.Nxp + (D == 0), ny = kp.Nyp + (D == 1);
+  const bool act = i < nx && j < ny;
+  bool heavy = false;
+  unsigned long long entry = 0;
+  if (act) {
+    const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), st = dstride(kp, D);
+    double m = phi[o - 3 * st];
+#pragma unroll
+    for (int s = -2; s <= 2; ++s) m = jmax(m, phi[o + s * st]);
+    const int sch = scheme_of<double>(kp, m);
+    heavy = sch != F3D_SCHEME_UP7;
+    entry = ((unsigned long long)o << 4) | ((unsigned long long)sch << 2) | (unsigned long long)D;
+  }
+  append_heavy(list, count, heavy, entry);
+}
+
+// ------------------------------------------------------------------ inviscid, dense part
+// lamQ(rho,u,v,w,T) of every cell from the UP7 form of every face: for direction d the RHS is
+//   R[c] = J_c * (sum_r da[r+4] Fp[c+r] + sum_r db[r+3] Fm[c+r]),
+// so dFp[e] = sum_{r=-3..4} da[4-r] Lam[e+r] and dFm[e] = sum_{r=-4..3} db[3-r] Lam[e+r].
+__global__ __launch_bounds__(256) void k_adj_inviscid(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
+                                                      const double* __restrict__ Lam, double* __restrict__ lamQ) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxt || j >= kp.Nyt) return;
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+  const int NG = kp.NG;
+  const bool inx = interior(i, kp.Nxp, NG), iny = interior(j, kp.Nyp, NG), inz = interior(k, kp.Nzp, NG);
+  double acc[5] = {0, 0, 0, 0, 0};
+  const bool slab[3] = {iny && inz, inx && inz, inx && iny};
+  if (slab[0] || slab[1] || slab[2]) {
+    const double rho = Q[o], u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc], Tt = Q[o + 5 * nc];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!slab[d]) continue;
+      const int a = d == 0 ? i : (d == 1 ? j : k);
+      const int Nt = d == 0 ? kp.Nxt : (d == 1 ? kp.Nyt : kp.Nzt);
+      const long long st = dstride(kp, d);
+      double gp[5] = {0, 0, 0, 0, 0}, gm[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int r = -4; r <= 4; ++r) {
+        if (a + r < 0 || a + r >= Nt) continue;
+        const double ca = r >= -3 ? kp.da[4 - r] : 0.0;
+        const double cb = r <= 3 ? kp.db[3 - r] : 0.0;
+        const long long oo = o + r * st;
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          const double L = Lam[oo + n * nc];
+          gp[n] += ca * L;
+          gm[n] += cb * L;
+        }
+      }
+      const double A1 = mp.m[3 * d][o], A2 = mp.m[3 * d + 1][o], A3 = mp.m[3 * d + 2][o];
+      const double S = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
+      double g[5];
+      sw_split_vjp(kp, rho, u, v, w, Tt, A1, A2, A3, S, gp, gm, g);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) acc[n] += g[n];
+    }
+  }
+  lamQ[o] = acc[0];
+  lamQ[o + nc] = acc[1];
+  lamQ[o + 2 * nc] = acc[2];
+  lamQ[o + 3 * nc] = acc[3];
+  lamQ[o + 4 * nc] = 0.0;
+  lamQ[o + 5 * nc] = acc[4];
+}
+
+// ------------------------------------------------------------------ inviscid, listed faces
+__global__ __launch_bounds__(64) void k_adj_heavy(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
+                                                  const double* __restrict__ Lam,
+                                                  const unsigned long long* __restrict__ list,
+                                                  const unsigned int* __restrict__ count, double* __restrict__ lamQ) {
+  const unsigned int n = *count;
+  const long long nc = kp.nc;
+  for (unsigned int e = blockIdx.x * 64 + threadIdx.x; e < n; e += gridDim.x * 64) {
+    const unsigned long long v = list[e];
+    const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
+    const long long o = (long long)(v >> 4), st = dstride(kp, D);
+    double gF[5];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      gF[q] = Lam[o + q * nc] - Lam[o - st + q * nc];  // face is the left face of o, the right one of o-st
+      any |= gF[q] != 0.0;
+    }
+    if (!any) continue;
+    const double* A1p = mp.m[3 * D];
+    const double* A2p = mp.m[3 * D + 1];
+    const double* A3p = mp.m[3 * D + 2];
+    double fp[8][5], fm[8][5], gfp[8][5], gfm[8][5];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const long long oc = o + (s - 4) * st;
+      const double A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
+      sw_split<double>(kp, Q[oc], Q[oc + nc], Q[oc + 2 * nc], Q[oc + 3 * nc], Q[oc + 5 * nc], A1, A2, A3,
+                       sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp[s], fm[s]);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) gfp[s][q] = gfm[s][q] = 0.0;
+    }
+    const long long oL = o - st;
+    const double qL[4] = {Q[oL + nc], Q[oL + 2 * nc], Q[oL + 3 * nc], Q[oL + 5 * nc]};
+    const double qR[4] = {Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc]};
+    const double AL[3] = {A1p[oL], A2p[oL], A3p[oL]};
+    const double AR[3] = {A1p[o], A2p[o], A3p[o]};
+    const double SL = sqrt(AL[0] * AL[0] + AL[1] * AL[1] + AL[2] * AL[2]);
+    const double SR = sqrt(AR[0] * AR[0] + AR[1] * AR[1] + AR[2] * AR[2]);
+    double gqL[4] = {0, 0, 0, 0}, gqR[4] = {0, 0, 0, 0};
+    face_flux_vjp(kp, sch, fp, fm, qL, qR, AL, AR, SL, SR, gF, gfp, gfm, gqL, gqR);
+    // the dense kernel already took this face as UP7
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        gfp[q][m] -= kp.lin[q] * gF[m];
+        gfm[7 - q][m] -= kp.lin[q] * gF[m];
+      }
+#pragma unroll 1
+    for (int s = 0; s < 8; ++s) {
+      const long long oc = o + (s - 4) * st;
+      const double A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
+      double g[5];
+      sw_split_vjp(kp, Q[oc], Q[oc + nc], Q[oc + 2 * nc], Q[oc + 3 * nc], Q[oc + 5 * nc], A1, A2, A3,
+                   sqrt(A1 * A1 + A2 * A2 + A3 * A3), gfp[s], gfm[s], g);
+      atomicAdd(&lamQ[oc], g[0]);
+      atomicAdd(&lamQ[oc + nc], g[1]);
+      atomicAdd(&lamQ[oc + 2 * nc], g[2]);
+      atomicAdd(&lamQ[oc + 3 * nc], g[3]);
+      atomicAdd(&lamQ[oc + 5 * nc], g[4]);
+    }
+    const int var[4] = {1, 2, 3, 5};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      atomicAdd(&lamQ[oL + var[q] * nc], gqL[q]);
+      atomicAdd(&lamQ[o + var[q] * nc], gqR[q]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ viscous
+__device__ __forceinline__ bool in_box(const KP& kp, int i, int j, int k) {
+  return i >= 2 && i <= kp.Nxt - 3 && j >= 2 && j <= kp.Nyt - 3 && k >= 2 && k <= kp.Nzt - 3;
+}
+
+// Per cell of the interior+-2 box: the adjoint of its viscous fluxes (from the transposed
+// 4th-order divergence of Lam) taken back through viscous_cell.  G[3v+d] = adjoint of the
+// d-derivative of (u,v,w,T)[v]; G[12+v] = adjoint of (u,v,w,T)[v] at the cell.
+__global__ __launch_bounds__(256) void k_adj_visc1(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
+                                                   const double* __restrict__ J, const double* __restrict__ Lam,
+                                                   double* __restrict__ G) {
+  const int i = 2 + blockIdx.x * 64 + threadIdx.x, j = 2 + blockIdx.y * 4 + threadIdx.y, k = 2 + blockIdx.z;
+  if (i > kp.Nxt - 3 || j > kp.Nyt - 3) return;
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+  const int NG = kp.NG;
+  const bool inx = interior(i, kp.Nxp, NG), iny = interior(j, kp.Nyp, NG), inz = interior(k, kp.Nzp, NG);
+  const bool slab[3] = {iny && inz, inx && inz, inx && iny};
+  if (!(slab[0] || slab[1] || slab[2])) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) G[o + q * nc] = 0.0;
+    return;
+  }
+  const double c1 = 1.0 / 12.0, c2 = -2.0 / 3.0;
+  double go[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const long long st = dstride(kp, d);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      if (!slab[d]) {
+        go[d][n] = 0.0;
+        continue;
+      }
+      const double* L = Lam + (n + 1) * nc + o;
+      go[d][n] = c1 * (L[2 * st] - L[-2 * st]) + c2 * (L[st] - L[-st]);
+    }
+  }
+  const int var[4] = {1, 2, 3, 5};
+  double g[4][3];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double* f = Q + var[v] * nc + o;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long long st = dstride(kp, d);
+      g[v][d] = c1 * (f[-2 * st] - f[2 * st]) + c2 * (f[-st] - f[st]);
+    }
+  }
+  double m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = mp.m[q][o];
+  double gg[4][3], gval[4];
+  viscous_cell_vjp(kp, Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], g, m, J[o], go, gg, gval);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) G[o + (3 * v + d) * nc] = gg[v][d];
+    G[o + (12 + v) * nc] = gval[v];
+  }
+}
+
+// lamQ(u,v,w,T) += own G[12+v] + transposed derivative stencils of G[3v+d] (box cells only).
+__global__ __launch_bounds__(256) void k_adj_visc2(KP kp, const double* __restrict__ G, double* __restrict__ lamQ) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxt || j >= kp.Nyt) return;
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+  const double c1 = 1.0 / 12.0, c2 = -2.0 / 3.0;
+  const int var[4] = {1, 2, 3, 5};
+  const bool self = in_box(kp, i, j, k);
+  double acc[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) acc[v] = self ? G[o + (12 + v) * nc] : 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const long long st = dstride(kp, d);
+#pragma unroll
+    for (int r = -2; r <= 2; ++r) {
+      if (r == 0) continue;
+      const int ii = i + (d == 0 ? r : 0), jj = j + (d == 1 ? r : 0), kk = k + (d == 2 ? r : 0);
+      if (!in_box(kp, ii, jj, kk)) continue;
+      // d(g[c])/d(f[a]) for c = a + r:  r=+2 -> c1, r=-2 -> -c1, r=+1 -> c2, r=-1 -> -c2
+      const double cf = r == 2 ? c1 : (r == -2 ? -c1 : (r == 1 ? c2 : -c2));
+      const long long oo = o + r * st;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] += cf * G[oo + (3 * v + d) * nc];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) lamQ[o + var[v] * nc] += acc[v];
+}
+
+// ------------------------------------------------------------------ ghost procedure, reverse
+// Reverse of one side of one dimension's ghost fill (boundary.jl:1-156 with the kinds of f3d.h):
+// each lane owns one line of NG ghost cells, folds their adjoints into the source cells and
+// clears them.  lam_bc[0..5] += adjoint of q_inf, lam_bc[6] += adjoint of T_wall.
+template <int D>
+__global__ __launch_bounds__(256) void k_adj_ghost(KP kp, int side, int bc, double* __restrict__ lamQ,
+                                                   const double* __restrict__ Q, double* __restrict__ lam_inflow,
+                                                   double* __restrict__ lam_bc) {
+  const int a = blockIdx.x * 64 + threadIdx.x, b = blockIdx.y * 4 + threadIdx.y;
+  // the other two dimensions, full extents
+  const int na = D == 0 ? kp.Nyt : kp.Nxt, nb = D == 2 ? kp.Nyt : kp.Nzt;
+  const bool act = a < na && b < nb;
+  const int NG = kp.NG, Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+  const long long nc = kp.nc;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (act) {
+    auto at = [&](int t) {
+      return D == 0 ? cidx(kp, t, a, b) : (D == 1 ? cidx(kp, a, t, b) : cidx(kp, a, b, t));
+    };
+    for (int l = 0; l < NG; ++l) {
+      const int g = side ? Np + NG + l : l;
+      const long long o = at(g);
+      double lg[6];
+#pragma unroll
+      for (int n = 0; n < 6; ++n) {
+        lg[n] = lamQ[o + n * nc];
+        lamQ[o + n * nc] = 0.0;
+      }
+      if (bc == F3D_BC_PERIODIC || bc == F3D_BC_ZEROGRAD) {
+        const int t = bc == F3D_BC_PERIODIC ? (side ? g - Np : g + Np) : (side ? Np + NG - 1 : NG);
+        const long long s = at(t);
+#pragma unroll
+        for (int n = 0; n < 6; ++n) lamQ[s + n * nc] += lg[n];
+      } else if (bc == F3D_BC_DIRICHLET) {
+#pragma unroll
+        for (int n = 0; n < 6; ++n) acc[n] += lg[n];
+      } else if (bc == F3D_BC_INFLOW) {
+        if (lam_inflow) {
+          const long long io = a + (long long)kp.Nyt * b, in = (long long)kp.Nyt * kp.Nzt;  // D == 0: (j, k)
+#pragma unroll
+          for (int n = 0; n < 6; ++n) lam_inflow[io + n * in] += lg[n];
+        }
+      } else {
+        const int mi = side ? 2 * (Np + NG) - 1 - g : 2 * NG - 1 - g;
+        const long long s = at(mi);
+        double gr = lg[0], gpv = lg[4], gT = lg[5];
+        if (bc == F3D_BC_WALL_ISOTHERMAL) {
+          // ghost: T = 2 Tw - T_s, rho = p_s / (Rg T)
+          const double q0 = Q[o], q5 = Q[o + 5 * nc];
+          gpv += lg[0] / (kp.Rg * q5);
+          const double gq5 = lg[5] - lg[0] * q0 / q5;
+          gT = -gq5;
+          acc[6] += 2 * gq5;
+          gr = 0.0;
+        }
+        lamQ[s] += gr;
+        lamQ[s + nc] -= lg[1];
+        lamQ[s + 2 * nc] -= lg[2];
+        lamQ[s + 3 * nc] -= lg[3];
+        lamQ[s + 4 * nc] += gpv;
+        lamQ[s + 5 * nc] += gT;
+      }
+    }
+  }
+  if (bc == F3D_BC_DIRICHLET || bc == F3D_BC_WALL_ISOTHERMAL) {
+    __shared__ double sm[7][4];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+#pragma unroll
+    for (int n = 0; n < 7; ++n) {
+      double v = acc[n];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      if ((tid & 63) == 0) sm[n][tid >> 6] = v;
+    }
+    __syncthreads();
+    if (tid < 7 && lam_bc) {
+      const double v = sm[tid][0] + sm[tid][1] + sm[tid][2] + sm[tid][3];
+      if (v != 0.0) atomicAdd(&lam_bc[tid], v);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ c2Prim + RK3 blend, reverse
+// lam_s = a * lam_out + c2Prim^T(lamQ) at U_s (+ lamUn for KRK 1);  lamUn = b*lam_out (KRK 3),
+// += b*lam_out (KRK 2).  lam_s may alias lam_out.
+template <int KRK>
+__global__ __launch_bounds__(256) void k_adj_finalize(KP kp, const double* __restrict__ lamQ, const double* Us,
+                                                      const double* lam_out, double* lam_s, double* lamUn) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxp || j >= kp.Nyp) return;
+  const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), nc = kp.nc;
+  const double a = KRK == 1 ? 1.0 : (KRK == 2 ? 0.25 : 2.0 / 3.0);
+  const double b = KRK == 1 ? 0.0 : (KRK == 2 ? 0.75 : 1.0 / 3.0);
+  double U[5], gQ[6], gU[5], lo[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    U[n] = Us[o + n * nc];
+    lo[n] = lam_out[o + n * nc];
+  }
+#pragma unroll
+  for (int n = 0; n < 6; ++n) gQ[n] = lamQ[o + n * nc];
+  c2prim_vjp(kp, U, gQ, gU);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    double r = a * lo[n] + gU[n];
+    if (KRK == 1) r += lamUn[o + n * nc];
+    if (KRK == 2) lamUn[o + n * nc] += b * lo[n];
+    if (KRK == 3) lamUn[o + n * nc] = b * lo[n];
+    lam_s[o + n * nc] = r;
+  }
+}
+
+// c2Prim^T of one cell into a U-shaped adjoint: lam += scale * c2Prim^T(W) at U (objective seed).
+__global__ __launch_bounds__(256) void k_adj_seed(KP kp, const double* __restrict__ U, const double* __restrict__ W,
+                                                  double scale, double* __restrict__ lam) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxp || j >= kp.Nyp) return;
+  const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), nc = kp.nc;
+  double Uc[5], gQ[6], gU[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) Uc[n] = U[o + n * nc];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) gQ[n] = scale * W[o + n * nc];
+  c2prim_vjp(kp, Uc, gQ, gU);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) lam[o + n * nc] += gU[n];
+}
+
+// c2Prim^T over the interior (f3d_c2prim_vjp_f64): gU = c2Prim^T(lamQ) at U, overwriting gU.
+__global__ __launch_bounds__(256) void k_adj_c2prim(KP kp, const double* __restrict__ U,
+                                                    const double* __restrict__ lamQ, double* __restrict__ gUo) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxp || j >= kp.Nyp) return;
+  const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), nc = kp.nc;
+  double Uc[5], gQ[6], gU[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) Uc[n] = U[o + n * nc];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) gQ[n] = lamQ[o + n * nc];
+  c2prim_vjp(kp, Uc, gQ, gU);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) gUo[o + n * nc] = gU[n];
+}
+
+// Objective term sum over interior cells of W . Q (6 vars): per-block partials, then one sum.
+__global__ __launch_bounds__(256) void k_obj_partial(KP kp, const double* __restrict__ Q, const double* __restrict__ W,
+                                                     double* __restrict__ part) {
+  const long long n = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  double s = 0.0;
+  for (long long t = blockIdx.x * 256LL + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+    const int i = (int)(t % kp.Nxp), j = (int)((t / kp.Nxp) % kp.Nyp), k = (int)(t / ((long long)kp.Nxp * kp.Nyp));
+    const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s += W[o + q * kp.nc] * Q[o + q * kp.nc];
+  }
+  __shared__ double sm[4];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+
+__global__ __launch_bounds__(256) void k_obj_final(const double* __restrict__ part, int n, double scale,
+                                                   double* __restrict__ out) {
+  double s = 0.0;
+  for (int t = threadIdx.x; t < n; t += 256) s += part[t];
+  __shared__ double sm[4];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = scale * (sm[0] + sm[1] + sm[2] + sm[3]);
+}
+
+// ------------------------------------------------------------------ host orchestration
+dim3 grid_interior(const KP& kp) { return dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, kp.Nzp); }
+dim3 grid_all(const KP& kp) { return dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4, kp.Nzt); }
+const dim3 kB(64, 4, 1);
+
+int check_aws(const f3d_params* p, const f3d_adj_workspace* aws, const char* who) {
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!aws) {
+    set_error("%s: adjoint workspace is NULL", who);
+    return F3D_E_NULL;
+  }
+  if (p->Nxp != aws->p.Nxp || p->Nyp != aws->p.Nyp || p->Nzp != aws->p.Nzp || p->NG != aws->p.NG) {
+    set_error("%s: adjoint workspace was created for another grid", who);
+    return F3D_E_WS;
+  }
+  return F3D_OK;
+}
+
+int check_metrics(const double* const* metrics, const double* J, const char* who) {
+  if (!metrics || !J) {
+    set_error("%s: NULL metric / Jacobian argument", who);
+    return F3D_E_NULL;
+  }
+  for (int q = 0; q < 9; ++q)
+    if (!metrics[q]) {
+      set_error("%s: metric %d is NULL", who, q);
+      return F3D_E_NULL;
+    }
+  return F3D_OK;
+}
+
+// lamQ = (dR/dQ)^T (scale * lam) over all cells of Q (RHS per unit dt, R = J * sum of differences).
+int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam, double scale, const double* Q,
+                 const double* const* metrics, const double* J, double* lamQ, hipStream_t s) {
+  const KP kp = make_kp(p);
+  MetricPtrs<double> mp;
+  for (int q = 0; q < 9; ++q) mp.m[q] = metrics[q];
+  const WsView<double> fv = ws_view<double>(aws->fws);
+  int rc = launch_shock_sensor<double>(kp, fv.phi, Q, s);
+  if (rc) return rc;
+  {
+    TimedScope ts("adj_heavy_list", s);
+    hipLaunchKernelGGL(k_adj_zero_count, dim3(1), dim3(1), 0, s, fv.hcount);
+    hipLaunchKernelGGL(k_adj_heavy_list<0>, dim3((kp.Nxp + 1 + 63) / 64, (kp.Nyp + 3) / 4, kp.Nzp), kB, 0, s, kp,
+                       fv.phi, fv.heavy, fv.hcount);
+    hipLaunchKernelGGL(k_adj_heavy_list<1>, dim3((kp.Nxp + 63) / 64, (kp.Nyp + 1 + 3) / 4, kp.Nzp), kB, 0, s, kp,
+                       fv.phi, fv.heavy, fv.hcount);
+    hipLaunchKernelGGL(k_adj_heavy_list<2>, dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, kp.Nzp + 1), kB, 0, s, kp,
+                       fv.phi, fv.heavy, fv.hcount);
+  }
+  {
+    TimedScope ts("adj_lambda", s);
+    hipLaunchKernelGGL(k_adj_lambda, grid_interior(kp), kB, 0, s, kp, lam, J, scale, aws->Lam);
+  }
+  {
+    TimedScope ts("adj_inviscid", s);
+    hipLaunchKernelGGL(k_adj_inviscid, grid_all(kp), kB, 0, s, kp, Q, mp, aws->Lam, lamQ);
+  }
+  {
+    TimedScope ts("adj_heavy", s);
+    hipLaunchKernelGGL(k_adj_heavy, dim3(1024), dim3(64), 0, s, kp, Q, mp, aws->Lam, fv.heavy, fv.hcount, lamQ);
+  }
+  if (kp.viscous) {
+    {
+      TimedScope ts("adj_visc1", s);
+      hipLaunchKernelGGL(k_adj_visc1, dim3((kp.Nxt - 4 + 63) / 64, (kp.Nyt - 4 + 3) / 4, kp.Nzt - 4), kB, 0, s, kp,
+                         Q, mp, J, aws->Lam, aws->G);
+    }
+    {
+      TimedScope ts("adj_visc2", s);
+      hipLaunchKernelGGL(k_adj_visc2, grid_all(kp), kB, 0, s, kp, aws->G, lamQ);
+    }
+  }
+  return check_launch("rhs vjp");
+}
+
+// Reverse of the stage ghost procedure (exchange_ghost then fill_ghost, solver.jl:439-443).
+int ghost_vjp_impl(const f3d_params* p, double* lamQ, const double* Q, double* lam_inflow, double* lam_bc,
+                   hipStream_t s) {
+  const KP kp = make_kp(p);
+  TimedScope ts("adj_ghost", s);
+  auto launch = [&](int d, int side, int bc) {
+    if (d == 0)
+      hipLaunchKernelGGL(k_adj_ghost<0>, dim3((kp.Nyt + 63) / 64, (kp.Nzt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
+                         lam_inflow, lam_bc);
+    else if (d == 1)
+      hipLaunchKernelGGL(k_adj_ghost<1>, dim3((kp.Nxt + 63) / 64, (kp.Nzt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
+                         lam_inflow, lam_bc);
+    else
+      hipLaunchKernelGGL(k_adj_ghost<2>, dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
+                         lam_inflow, lam_bc);
+  };
+  for (int d = 2; d >= 0; --d)
+    for (int side = 1; side >= 0; --side) launch(d, side, p->bc[2 * d + side]);
+  for (int d = 2; d >= 0; --d)
+    if (p->bc[2 * d] == F3D_BC_PERIODIC && p->bc[2 * d + 1] == F3D_BC_PERIODIC)
+      for (int side = 1; side >= 0; --side) launch(d, side, F3D_BC_PERIODIC);
+  return check_launch("ghost vjp");
+}
+
+int stage_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, int KRK, const double* lam_out, double* lam_s,
+                   double* lamUn, const double* Us, const double* Qs, const double* const* metrics, const double* J,
+                   double dt, double* lam_inflow, double* lam_bc, hipStream_t s) {
+  const KP kp = make_kp(p);
+  const double a = KRK == 1 ? 1.0 : (KRK == 2 ? 0.25 : 2.0 / 3.0);
+  int rc = rhs_vjp_impl(p, aws, lam_out, a * dt, Qs, metrics, J, aws->lamQ, s);
+  if (rc) return rc;
+  rc = ghost_vjp_impl(p, aws->lamQ, Qs, lam_inflow, lam_bc, s);
+  if (rc) return rc;
+  TimedScope ts("adj_finalize", s);
+  if (KRK == 1)
+    hipLaunchKernelGGL(k_adj_finalize<1>, grid_interior(kp), kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
+  else if (KRK == 2)
+    hipLaunchKernelGGL(k_adj_finalize<2>, grid_interior(kp), kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
+  else
+    hipLaunchKernelGGL(k_adj_finalize<3>, grid_interior(kp), kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
+  return check_launch("stage vjp finalize");
+}
+
+size_t ckpt_slots(int nsteps, int every) { return (size_t)((nsteps + every - 1) / every) + (size_t)(every - 1) + 1; }
+
+}  // namespace
+
+// ================================================================ extern "C"
+#define F3D_API extern "C" __attribute__((visibility("default")))
+
+F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace** out) {
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!out) {
+    set_error("adjoint_workspace_create: out is NULL");
+    return F3D_E_NULL;
+  }
+  *out = nullptr;
+  f3d_workspace* fws = nullptr;
+  rc = f3d_workspace_create(p, F3D_PREC_F64, &fws);
+  if (rc) return rc;
+  const size_t nc = f3d_cell_count(p), v = nc * sizeof(double);
+  // Lam 5, lamQ 6, G 16, Us 3x5, Qs 3x6, Un 5, lam 5, lamUn 5, Ucur 5, Qcur 6, red
+  const size_t nvars = 5 + 6 + 16 + 15 + 18 + 5 + 5 + 5 + 5 + 6;
+  const size_t total = nvars * align_up(v) + align_up(kObjBlocks * sizeof(double));
+  void* base = nullptr;
+  hipError_t e = hipMalloc(&base, total);
+  if (e != hipSuccess) {
+    f3d_workspace_destroy(fws);
+    set_error("adjoint_workspace_create: hipMalloc of %zu bytes failed: %s", total, hipGetErrorString(e));
+    return (int)e;
+  }
+  f3d_adj_workspace* a = new f3d_adj_workspace();
+  a->p = *p;
+  a->fws = fws;
+  a->base = base;
+  a->bytes = total + f3d_workspace_bytes(fws);
+  char* c = (char*)base;
+  auto take = [&](int nv) {
+    double* r = (double*)c;
+    c += nv * align_up(v);
+    return r;
+  };
+  a->Lam = take(5);
+  a->lamQ = take(6);
+  a->G = take(16);
+  for (int q = 0; q < 3; ++q) a->Us[q] = take(5);
+  for (int q = 0; q < 3; ++q) a->Qs[q] = take(6);
+  a->Un = take(5);
+  a->lam = take(5);
+  a->lamUn = take(5);
+  a->Ucur = take(5);
+  a->Qcur = take(6);
+  a->red = (double*)c;
+  a->nred = kObjBlocks;
+  // zeroed once: ghost entries of Lam are never written and must read as zero
+  hipLaunchKernelGGL(k_adj_zero, dim3(1024), dim3(256), 0, 0, (double*)base, (long long)(total / 8));
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    hipFree(base);
+    f3d_workspace_destroy(fws);
+    delete a;
+    set_error("adjoint_workspace_create: zero fill failed: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  *out = a;
+  return F3D_OK;
+}
+
+F3D_API int f3d_adjoint_workspace_destroy(f3d_adj_workspace* a) {
+  if (!a) return F3D_OK;
+  hipError_t e = hipFree(a->base);
+  int rc = f3d_workspace_destroy(a->fws);
+  delete a;
+  if (e != hipSuccess) {
+    set_error("adjoint_workspace_destroy: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  return rc;
+}
+
+F3D_API size_t f3d_adjoint_workspace_bytes(const f3d_adj_workspace* a) { return a ? a->bytes : 0; }
+
+F3D_API int f3d_rhs_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, const double* lamR, const double* Q,
+                            const double* const* metrics, const double* J, double* lamQ, f3d_stream_t s) {
+  int rc = check_aws(p, aws, "rhs_vjp");
+  if (!rc) rc = check_metrics(metrics, J, "rhs_vjp");
+  if (rc) return rc;
+  if (!lamR || !Q || !lamQ) {
+    set_error("rhs_vjp: NULL array argument");
+    return F3D_E_NULL;
+  }
+  return rhs_vjp_impl(p, aws, lamR, 1.0, Q, metrics, J, lamQ, s);
+}
+
+F3D_API int f3d_ghost_vjp_f64(const f3d_params* p, double* lamQ, const double* Q, double* lam_inflow, double* lam_bc,
+                              f3d_stream_t s) {
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!lamQ || !Q) {
+    set_error("ghost_vjp: NULL array argument");
+    return F3D_E_NULL;
+  }
+  return ghost_vjp_impl(p, lamQ, Q, lam_inflow, lam_bc, s);
+}
+
+F3D_API int f3d_c2prim_vjp_f64(const f3d_params* p, double* lamU, const double* lamQ, const double* U,
+                               f3d_stream_t s) {
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!lamU || !lamQ || !U) {
+    set_error("c2prim_vjp: NULL array argument");
+    return F3D_E_NULL;
+  }
+  const KP kp = make_kp(p);
+  hipLaunchKernelGGL(k_adj_c2prim, grid_interior(kp), kB, 0, s, kp, U, lamQ, lamU);
+  return check_launch("c2prim vjp");
+}
+
+F3D_API int f3d_rk3_stage_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t KRK, const double* lam_out,
+                                  double* lam_s, double* lamUn, const double* Us, const double* Qs,
+                                  const double* const* metrics, const double* J, double dt, double* lam_inflow,
+                                  double* lam_bc, f3d_stream_t s) {
+  int rc = check_aws(p, aws, "rk3_stage_vjp");
+  if (!rc) rc = check_metrics(metrics, J, "rk3_stage_vjp");
+  if (rc) return rc;
+  if (KRK < 1 || KRK > 3) {
+    set_error("rk3_stage_vjp: KRK must be 1, 2 or 3 (got %d)", KRK);
+    return F3D_E_ARG;
+  }
+  if (!lam_out || !lam_s || !lamUn || !Us || !Qs) {
+    set_error("rk3_stage_vjp: NULL array argument");
+    return F3D_E_NULL;
+  }
+  return stage_vjp_impl(p, aws, KRK, lam_out, lam_s, lamUn, Us, Qs, metrics, J, dt, lam_inflow, lam_bc, s);
+}
+
+F3D_API size_t f3d_adjoint_ckpt_bytes(const f3d_params* p, int32_t nsteps, int32_t every) {
+  if (!p || nsteps < 1 || every < 1) return 0;
+  return ckpt_slots(nsteps, every) * 5 * f3d_cell_count(p) * sizeof(double);
+}
+
+F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t nsteps, int32_t every,
+                            const double* U0, const double* const* metrics, const double* J, double dt,
+                            const double* inflow, const double* W, double* ckpt, size_t ckpt_bytes, double* obj,
+                            double* lamU0, double* lam_inflow, double* lam_bc, f3d_stream_t s) {
+  int rc = check_aws(p, aws, "adjoint");
+  if (!rc) rc = check_metrics(metrics, J, "adjoint");
+  if (rc) return rc;
+  if (nsteps < 1 || every < 1) {
+    set_error("adjoint: nsteps (%d) and every (%d) must be >= 1", nsteps, every);
+    return F3D_E_ARG;
+  }
+  if (!U0 || !W || !ckpt || !obj || !lamU0) {
+    set_error("adjoint: NULL array argument");
+    return F3D_E_NULL;
+  }
+  if (p->bc[0] == F3D_BC_INFLOW && !inflow) {
+    set_error("adjoint: F3D_BC_INFLOW needs an inflow profile");
+    return F3D_E_NULL;
+  }
+  const size_t need = f3d_adjoint_ckpt_bytes(p, nsteps, every);
+  if (ckpt_bytes < need) {
+    set_error("adjoint: checkpoint buffer of %zu bytes, %zu needed", ckpt_bytes, need);
+    return F3D_E_ARG;
+  }
+  const KP kp = make_kp(p);
+  const size_t nc = kp.nc, vb = 5 * nc * sizeof(double), qb = 6 * nc * sizeof(double);
+  const int nck = (nsteps + every - 1) / every;
+  auto slot = [&](size_t q) { return ckpt + q * 5 * nc; };
+  double* seg0 = slot(nck);                 // every-1 recomputed states of a segment
+  double* Ufin = slot(nck + every - 1);     // U^nsteps
+  auto seg = [&](int q) { return q == 0 ? nullptr : seg0 + (size_t)(q - 1) * 5 * nc; };
+  auto obj_term = [&](const double* Qn, double* outp) {
+    hipLaunchKernelGGL(k_obj_partial, dim3(aws->nred), dim3(256), 0, s, kp, Qn, W, aws->red);
+    hipLaunchKernelGGL(k_obj_final, dim3(1), dim3(256), 0, s, aws->red, aws->nred, dt, outp);
+  };
+  auto prim = [&](const double* Usrc, double* Qd) {
+    int r = f3d_c2prim_f64(p, Usrc, Qd, s);
+    if (!r) r = f3d_exchange_ghost_f64(p, Qd, 6, s);
+    if (!r) r = f3d_fill_ghost_f64(p, Qd, nullptr, inflow, s);
+    return r;
+  };
+  auto step = [&](double* U, double* Q) {
+    int r = F3D_OK;
+    for (int KRK = 1; KRK <= 3 && !r; ++KRK)
+      r = f3d_rk3_stage_f64(p, aws->fws, KRK, U, aws->Un, Q, metrics, J, dt, inflow, s);
+    return r;
+  };
+  // ---- forward sweep: checkpoints every `every` steps, objective per step
+  F3D_HIP(hipMemcpyAsync(aws->Ucur, U0, vb, hipMemcpyDeviceToDevice, s));
+  rc = prim(aws->Ucur, aws->Qcur);
+  for (int n = 0; n < nsteps && !rc; ++n) {
+    if (n % every == 0) F3D_HIP(hipMemcpyAsync(slot(n / every), aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
+    rc = step(aws->Ucur, aws->Qcur);
+    if (!rc) obj_term(aws->Qcur, obj + n);
+  }
+  if (rc) return rc;
+  F3D_HIP(hipMemcpyAsync(Ufin, aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
+  // ---- reverse sweep
+  hipLaunchKernelGGL(k_adj_zero, dim3(1024), dim3(256), 0, s, aws->lam, (long long)(5 * nc));
+  for (int c = nck - 1; c >= 0 && !rc; --c) {
+    const int n0 = c * every, n1 = (n0 + every < nsteps) ? n0 + every : nsteps;
+    // recompute U^{n0+1} .. U^{n1-1} of the segment
+    if (n1 - n0 > 1) {
+      F3D_HIP(hipMemcpyAsync(aws->Ucur, slot(c), vb, hipMemcpyDeviceToDevice, s));
+      rc = prim(aws->Ucur, aws->Qcur);
+      for (int n = n0; n < n1 - 1 && !rc; ++n) {
+        rc = step(aws->Ucur, aws->Qcur);
+        if (!rc) F3D_HIP(hipMemcpyAsync(seg(n + 1 - n0), aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
+      }
+    }
+    for (int n = n1 - 1; n >= n0 && !rc; --n) {
+      const double* Unext = (n + 1 == nsteps) ? Ufin : (n + 1 == n1 ? slot(c + 1) : seg(n + 1 - n0));
+      const double* Ustart = (n == n0) ? slot(c) : seg(n - n0);
+      {
+        TimedScope ts("adj_seed", s);
+        hipLaunchKernelGGL(k_adj_seed, grid_interior(kp), kB, 0, s, kp, Unext, W, dt, aws->lam);
+      }
+      // stage states of step n: (U^n, Q^n), (U^1, Q^1), (U^2, Q^2)
+      F3D_HIP(hipMemcpyAsync(aws->Us[0], Ustart, vb, hipMemcpyDeviceToDevice, s));
+      rc = prim(aws->Us[0], aws->Qs[0]);
+      for (int q = 1; q <= 2 && !rc; ++q) {
+        F3D_HIP(hipMemcpyAsync(aws->Us[q], aws->Us[q - 1], vb, hipMemcpyDeviceToDevice, s));
+        F3D_HIP(hipMemcpyAsync(aws->Qs[q], aws->Qs[q - 1], qb, hipMemcpyDeviceToDevice, s));
+        rc = f3d_rk3_stage_f64(p, aws->fws, q, aws->Us[q], aws->Un, aws->Qs[q], metrics, J, dt, inflow, s);
+      }
+      for (int KRK = 3; KRK >= 1 && !rc; --KRK)
+        rc = stage_vjp_impl(p, aws, KRK, aws->lam, aws->lam, aws->lamUn, aws->Us[KRK - 1], aws->Qs[KRK - 1], metrics,
+                            J, dt, lam_inflow, lam_bc, s);
+    }
+  }
+  if (rc) return rc;
+  F3D_HIP(hipMemcpyAsync(lamU0, aws->lam, vb, hipMemcpyDeviceToDevice, s));
+  return check_launch("adjoint");
+}

@@ -8,6 +8,9 @@
 
 #include "../../include/f3d.h"
 
+// Point-wise math is host+device so that the local derivative pieces can be checked on the CPU.
+#define F3D_HD __host__ __device__ __forceinline__
+
 namespace f3d {
 
 // Kernel-side copy of f3d_params (passed by value as a kernel argument).
@@ -34,43 +37,43 @@ template <typename T>
 struct Coef;
 template <>
 struct Coef<double> {
-  __device__ static __forceinline__ double lin(const KP& kp, int q) { return kp.lin[q]; }
-  __device__ static __forceinline__ double da(const KP& kp, int q) { return kp.da[q]; }
-  __device__ static __forceinline__ double db(const KP& kp, int q) { return kp.db[q]; }
-  __device__ static __forceinline__ double gRg(const KP& kp) { return kp.gRg; }
-  __device__ static __forceinline__ double gm1(const KP& kp) { return kp.gm1; }
-  __device__ static __forceinline__ double w2c(const KP& kp) { return kp.w2c; }
-  __device__ static __forceinline__ double inv2g(const KP& kp) { return kp.inv2g; }
+  static F3D_HD double lin(const KP& kp, int q) { return kp.lin[q]; }
+  static F3D_HD double da(const KP& kp, int q) { return kp.da[q]; }
+  static F3D_HD double db(const KP& kp, int q) { return kp.db[q]; }
+  static F3D_HD double gRg(const KP& kp) { return kp.gRg; }
+  static F3D_HD double gm1(const KP& kp) { return kp.gm1; }
+  static F3D_HD double w2c(const KP& kp) { return kp.w2c; }
+  static F3D_HD double inv2g(const KP& kp) { return kp.inv2g; }
 };
 template <>
 struct Coef<float> {
-  __device__ static __forceinline__ float lin(const KP& kp, int q) { return kp.linf[q]; }
-  __device__ static __forceinline__ float da(const KP& kp, int q) { return kp.daf[q]; }
-  __device__ static __forceinline__ float db(const KP& kp, int q) { return kp.dbf[q]; }
-  __device__ static __forceinline__ float gRg(const KP& kp) { return kp.gRgf; }
-  __device__ static __forceinline__ float gm1(const KP& kp) { return kp.gm1f; }
-  __device__ static __forceinline__ float w2c(const KP& kp) { return kp.w2cf; }
-  __device__ static __forceinline__ float inv2g(const KP& kp) { return kp.inv2gf; }
+  static F3D_HD float lin(const KP& kp, int q) { return kp.linf[q]; }
+  static F3D_HD float da(const KP& kp, int q) { return kp.daf[q]; }
+  static F3D_HD float db(const KP& kp, int q) { return kp.dbf[q]; }
+  static F3D_HD float gRg(const KP& kp) { return kp.gRgf; }
+  static F3D_HD float gm1(const KP& kp) { return kp.gm1f; }
+  static F3D_HD float w2c(const KP& kp) { return kp.w2cf; }
+  static F3D_HD float inv2g(const KP& kp) { return kp.inv2gf; }
 };
 
-__device__ __forceinline__ long long cidx(const KP& kp, int i, int j, int k) {
+F3D_HD long long cidx(const KP& kp, int i, int j, int k) {
   return (long long)i + (long long)kp.Nxt * ((long long)j + (long long)kp.Nyt * k);
 }
 
 // Stride of one cell step along direction d (0:i, 1:j, 2:k).
-__device__ __forceinline__ long long dstride(const KP& kp, int d) {
+F3D_HD long long dstride(const KP& kp, int d) {
   return d == 0 ? 1LL : (d == 1 ? (long long)kp.Nxt : (long long)kp.Nxt * kp.Nyt);
 }
 
 // NaN-propagating max (Julia Base.max semantics on floats; schemes.jl:724-729 uses it).
 template <typename T>
-__device__ __forceinline__ T jmax(T a, T b) {
+F3D_HD T jmax(T a, T b) {
   return (a != a) ? a : ((b != b) ? b : ((b > a) ? b : a));
 }
 
 // Scheme index from the 6-cell sensor maximum (schemes.jl:731, 813, 931, else NND2 at 1000).
 template <typename T>
-__device__ __forceinline__ int scheme_of(const KP& kp, T phix) {
+F3D_HD int scheme_of(const KP& kp, T phix) {
   return phix < (T)kp.phi1 ? F3D_SCHEME_UP7
                            : (phix < (T)kp.phi2 ? F3D_SCHEME_WENO7
                                                 : (phix < (T)kp.phi3 ? F3D_SCHEME_WENO5 : F3D_SCHEME_NND2));
@@ -79,7 +82,7 @@ __device__ __forceinline__ int scheme_of(const KP& kp, T phix) {
 // Jameson-type pressure sensor of one cell (schemes.jl:19-22), contraction off so that the mask
 // derived from it is reproducible bit for bit against a non-contracted CPU evaluation.
 template <typename T>
-__device__ __forceinline__ T sensor_cell(T px1, T p, T px3, T py1, T py3, T pz1, T pz3) {
+F3D_HD T sensor_cell(T px1, T p, T px3, T py1, T py3, T pz1, T pz3) {
 #pragma clang fp contract(off)
   T a = fabs(-px1 + 2 * p - px3) / (px1 + 2 * p + px3);
   T b = fabs(-py1 + 2 * p - py3) / (py1 + 2 * p + py3);
@@ -91,7 +94,7 @@ __device__ __forceinline__ T sensor_cell(T px1, T p, T px3, T py1, T py3, T pz1,
 // split.jl:2-68 for one cell: Fp, Fm (5 each) from rho,u,v,w,T and the metric row A (J^-1
 // scaled) with S = |A|.
 template <typename T>
-__device__ __forceinline__ void sw_split(const KP& kp, T rho, T u, T v, T w, T Tt, T A1, T A2, T A3,
+F3D_HD void sw_split(const KP& kp, T rho, T u, T v, T w, T Tt, T A1, T A2, T A3,
                                          T ss, T* __restrict__ fp, T* __restrict__ fm) {
   const T c = sqrt(Coef<T>::gRg(kp) * Tt);
   const T E1 = A1 * u + A2 * v + A3 * w;
@@ -131,7 +134,7 @@ template <typename T>
 struct W7;
 template <>
 struct W7<double> {
-  __device__ static __forceinline__ double mix(double eps, double s0, double I1, double I2, double I3,
+  static F3D_HD double mix(double eps, double s0, double I1, double I2, double I3,
                                                double I4, double q1, double q2, double q3, double q4) {
     const double b1 = eps + I1 * s0, b2 = eps + I2 * s0, b3 = eps + I3 * s0, b4 = eps + I4 * s0;
     const double a1 = 1.0 / (b1 * b1), a2 = 12.0 / (b2 * b2), a3 = 18.0 / (b3 * b3), a4 = 4.0 / (b4 * b4);
@@ -141,7 +144,7 @@ struct W7<double> {
 };
 template <>
 struct W7<float> {
-  __device__ static __forceinline__ float mix(float eps, float s0, float I1, float I2, float I3, float I4,
+  static F3D_HD float mix(float eps, float s0, float I1, float I2, float I3, float I4,
                                               float q1, float q2, float q3, float q4) {
     const float b1 = eps + I1 * s0, b2 = eps + I2 * s0, b3 = eps + I3 * s0, b4 = eps + I4 * s0;
     const float bm = fminf(fminf(b1, b2), fminf(b3, b4));
@@ -153,7 +156,7 @@ struct W7<float> {
 
 // WENO7 of a 7-point stencil (schemes.jl:847-879), unscaled (the caller multiplies by 1/12).
 template <typename T>
-__device__ __forceinline__ T weno7(const T* V, T s0, T eps) {
+F3D_HD T weno7(const T* V, T s0, T eps) {
   const T V1 = V[0], V2 = V[1], V3 = V[2], V4 = V[3], V5 = V[4], V6 = V[5], V7 = V[6];
   const T q1 = -3 * V1 + 13 * V2 - 23 * V3 + 25 * V4;
   const T q2 = V2 - 5 * V3 + 13 * V4 + 3 * V5;
@@ -172,7 +175,7 @@ __device__ __forceinline__ T weno7(const T* V, T s0, T eps) {
 
 // WENO5 of a 5-point stencil (schemes.jl:958-972), unscaled (the caller multiplies by 1/6).
 template <typename T>
-__device__ __forceinline__ T weno5(const T* V, T s0, T eps) {
+F3D_HD T weno5(const T* V, T s0, T eps) {
   const T V1 = V[0], V2 = V[1], V3 = V[2], V4 = V[3], V5 = V[4];
   const T a1 = V1 - 2 * V2 + V3, c1 = V1 - 4 * V2 + 3 * V3;
   const T a2 = V2 - 2 * V3 + V4, c2 = V2 - V4;
@@ -189,7 +192,7 @@ __device__ __forceinline__ T weno5(const T* V, T s0, T eps) {
 }
 
 template <typename T>
-__device__ __forceinline__ T minmod(T a, T b) {
+F3D_HD T minmod(T a, T b) {
   return (a * b > 0) ? ((fabs(a) > fabs(b)) ? b : a) : T(0);
 }
 
@@ -198,7 +201,7 @@ __device__ __forceinline__ T minmod(T a, T b) {
 //   FP(s,m), FM(s,m): split fluxes of variable m at stencil offset s in [-4, 3] from cell c.
 //   qL/qR: (u,v,w,T) of cells c-1 / c; AL/AR: metric row; SL/SR: |A|.
 template <typename T, class FPL, class FML>
-__device__ __forceinline__ void face_flux(const KP& kp, int sch, const FPL& FP, const FML& FM,
+F3D_HD void face_flux(const KP& kp, int sch, const FPL& FP, const FML& FM,
                                           const T* qL, const T* qR, const T* AL, const T* AR, T SL,
                                           T SR, T* __restrict__ out) {
   if (sch == F3D_SCHEME_UP7) {  // schemes.jl:731-754
@@ -293,7 +296,7 @@ __device__ __forceinline__ void face_flux(const KP& kp, int sch, const FPL& FP, 
 //   g[v][d]: 4th-order central derivative of (u,v,w,T)[v] along xi/eta/zeta (not yet divided).
 //   m[9]: metric row-major (xi_x, xi_y, xi_z, eta_x, ...); out[d][4].
 template <typename T>
-__device__ __forceinline__ void viscous_cell(const KP& kp, T u, T v, T w, T Tt, const T g[4][3],
+F3D_HD void viscous_cell(const KP& kp, T u, T v, T w, T Tt, const T g[4][3],
                                              const T m[9], T Jac, T out[3][4]) {
   const T mu = (T)kp.Cs * Tt * sqrt(Tt) / (Tt + (T)kp.Ts);
   const T lam = (T)kp.Cp * mu / (T)kp.Pr;

@@ -47,22 +47,6 @@ __device__ __forceinline__ long long ridx(const KP& kp, int i, int j, int k) {
   return (long long)i + (long long)kp.Nxp * ((long long)j + (long long)kp.Nyp * k);
 }
 
-// Wave-aggregated append of heavy faces.
-__device__ __forceinline__ void append_heavy(unsigned long long* list, unsigned int* count, bool pred,
-                                             unsigned long long entry) {
-  const unsigned long long mask = __ballot(pred);
-  if (mask == 0) return;
-  const int lane = __lane_id();
-  unsigned int base = 0;
-  const int leader = __ffsll((long long)mask) - 1;
-  if (lane == leader) base = atomicAdd(count, (unsigned int)__popcll(mask));
-  base = __shfl(base, leader, 64);
-  if (pred) {
-    const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
-    list[base + __popcll(below)] = entry;
-  }
-}
-
 template <typename T>
 __device__ __forceinline__ T up7c(const KP& kp, int m) {
   return Coef<T>::lin(kp, m);

@@ -17,6 +17,22 @@ struct WsView {
   unsigned long long hcap;   // capacity of `heavy` (total face count of the three directions)
 };
 
+// Wave-aggregated append to the compacted heavy-face list: one atomic per wave.
+__device__ __forceinline__ void append_heavy(unsigned long long* list, unsigned int* count, bool pred,
+                                             unsigned long long entry) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  unsigned int base = 0;
+  const int leader = __ffsll((long long)mask) - 1;
+  if (lane == leader) base = atomicAdd(count, (unsigned int)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (pred) {
+    const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
+    list[base + __popcll(below)] = entry;
+  }
+}
+
 }  // namespace f3d
 
 struct f3d_workspace;

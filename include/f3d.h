@@ -261,6 +261,58 @@ int f3d_nan_check_sync(const f3d_params* p, f3d_workspace* ws, const double* U, 
 int f3d_metrics_f64(const f3d_params* p, const double* x, const double* y, const double* z,
                     double* const* metrics, double* J, double* const* s_out, f3d_stream_t s);
 
+/* ---------------- adjoint (reverse mode, FP64; no reference counterpart) ---------------------
+ * Hand-written vector-Jacobian products of the path above with the scheme mask frozen at the
+ * primal state (the sensor, face schemes, minmod selections and tangent-vector branch are not
+ * differentiated; |x| uses the subgradient sign(0) = 0).  They back a ChainRulesCore.rrule of
+ * the RHS on the Julia side.  The adjoint workspace holds its own FP64 forward workspace. */
+typedef struct f3d_adj_workspace f3d_adj_workspace;
+int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace** aws);
+int f3d_adjoint_workspace_destroy(f3d_adj_workspace* aws);
+size_t f3d_adjoint_workspace_bytes(const f3d_adj_workspace* aws); /* host-only */
+
+/* lamQ = (dR/dQ)^T lamR for R of f3d_rhs_f64 (interior, U layout).  lamQ [Nxt,Nyt,Nzt,6] is
+ * written for every cell, ghosts included (ghost adjoints are folded by f3d_ghost_vjp_f64); its
+ * p entries are zero (the RHS reads rho,u,v,w,T). */
+int f3d_rhs_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, const double* lamR, const double* Q,
+                    const double* const* metrics, const double* J, double* lamQ, f3d_stream_t s);
+
+/* Reverse of the stage ghost procedure (exchange_ghost then fillGhost, solver.jl:439-443) on the
+ * filled Q: ghost adjoints of lamQ are added into their source cells and cleared; Dirichlet,
+ * inflow and isothermal-wall ghost values add to the parameter adjoints
+ *   lam_inflow [Nyt,Nzt,6] (may be NULL), lam_bc[7] = (q_inf[0..5], T_wall) (may be NULL). */
+int f3d_ghost_vjp_f64(const f3d_params* p, double* lamQ, const double* Q, double* lam_inflow,
+                      double* lam_bc, f3d_stream_t s);
+
+/* lamU = c2Prim^T lamQ at U on interior cells (utils.jl:2-30; eps floors pass nothing). */
+int f3d_c2prim_vjp_f64(const f3d_params* p, double* lamU, const double* lamQ, const double* U,
+                       f3d_stream_t s);
+
+/* Reverse of f3d_rk3_stage_f64 stage KRK taken at its input (Us, Qs filled): from the adjoint of
+ * the stage output lam_out, lam_s = adjoint of Us (may alias lam_out);  lamUn (adjoint of Un) is
+ * set to lam_out/3 by KRK 3, incremented by 3 lam_out/4 by KRK 2 and added into lam_s by KRK 1
+ * (Un = U^n), so calling KRK 3, 2, 1 in turn reverses one RK3 step.  lam_inflow/lam_bc accumulate. */
+int f3d_rk3_stage_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t KRK,
+                          const double* lam_out, double* lam_s, double* lamUn, const double* Us,
+                          const double* Qs, const double* const* metrics, const double* J,
+                          double dt, double* lam_inflow, double* lam_bc, f3d_stream_t s);
+
+/* Device bytes of the checkpoint buffer f3d_adjoint_f64 needs for `nsteps` with a checkpoint
+ * every `every` steps: (ceil(nsteps/every) + every) conserved states.  Host-only. */
+size_t f3d_adjoint_ckpt_bytes(const f3d_params* p, int32_t nsteps, int32_t every);
+
+/* Checkpointed multi-step adjoint.  Forward: Q0 = ghosts(c2Prim(U0)), `nsteps` RK3 steps of
+ * f3d_rk3_stage_f64 with a fixed dt, obj[n] = dt * sum over interior cells of W . Q^{n+1}
+ * (W [Nxt,Nyt,Nzt,6], e.g. wall-face area on p of the first cell row above a wall).  Reverse:
+ * U^n checkpointed every `every` steps, states inside a segment recomputed, the three stage
+ * states of each step recomputed from U^n, stage VJPs 3,2,1.  Outputs: lamU0 = d(sum obj)/dU0
+ * (interior; ghosts 0), lam_inflow / lam_bc accumulate d(sum obj)/d(inflow, q_inf, T_wall). */
+int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t nsteps, int32_t every,
+                    const double* U0, const double* const* metrics, const double* J, double dt,
+                    const double* inflow, const double* W, double* ckpt, size_t ckpt_bytes,
+                    double* obj, double* lamU0, double* lam_inflow, double* lam_bc,
+                    f3d_stream_t s);
+
 /* ---------------- profiling: hipEvent timing of the library's own kernel launches -------------
  * No reference counterpart (the reference prints wall-clock time only, solver.jl:472-479).
  * When enabled, every kernel the library launches is bracketed by a pair of hipEvents recorded on

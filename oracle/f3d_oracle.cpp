@@ -288,7 +288,198 @@ void rhs_ld(const f3d_params* p, double* Rout, const double* Q, const void* cons
         }
       }
 }
+// Forward-mode (dual-number) pieces of the FP64 path with the scheme mask frozen at the primal
+// state: the CPU forward-mode reference the adjoint (VJP) is checked against.
+struct DualMetrics {
+  std::vector<std::vector<Dual>> a;
+  const Dual* mp[9];
+  std::vector<Dual> s1, s2, s3;
+  DualMetrics(const f3d_params* p, const void* const* mv) : a(9) {
+    Grid g(p);
+    size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+    for (int q = 0; q < 9; ++q) {
+      const double* src = (const double*)mv[q];
+      a[q].assign(src, src + nc);
+      mp[q] = a[q].data();
+    }
+    s1.resize(nc); s2.resize(nc); s3.resize(nc);
+    metric_norms<Dual>(p, mp, s1.data(), s2.data(), s3.data());
+  }
+};
+
+// R = J * (sum_d dF_d + viscous divergence) on the interior (U layout, ghosts 0), in Dual; the
+// sensor is evaluated on the value part only.
+void rhs_dual(const f3d_params* p, const DualMetrics& dm, Dual* R, const Dual* QD, const double* J) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  std::vector<double> qv(6 * nc), phid(nc, 0.0);
+  for (size_t i = 0; i < 6 * nc; ++i) qv[i] = QD[i].v;
+  shock_sensor<double>(p, phid.data(), qv.data());
+  std::vector<Dual> phiD(phid.begin(), phid.end());
+  std::vector<Dual> Qc(QD, QD + 6 * nc);
+  Scratch<Dual> sc(p);
+  flow_advance<Dual>(p, Qc.data(), sc.Fp.data(), sc.Fm.data(), sc.Fx.data(), sc.Fy.data(), sc.Fz.data(),
+                     sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), dm.s1.data(), dm.s2.data(), dm.s3.data(), dm.mp, J,
+                     phiD.data());
+  std::fill(R, R + nc * 5, Dual(0.0));
+  const long NG = g.NG;
+  const double c1 = 1.0 / 12, c2 = -2.0 / 3;
+  for (long k = 1; k <= g.Nzp; ++k)
+    for (long j = 1; j <= g.Nyp; ++j)
+      for (long i = 1; i <= g.Nxp; ++i) {
+        const double Jc = J[g.cell.at(i + NG, j + NG, k + NG)];
+        for (long n = 1; n <= 5; ++n) {
+          Dual s = sc.Fx[g.fx.at(i, j, k, n)] - sc.Fx[g.fx.at(i + 1, j, k, n)] + sc.Fy[g.fy.at(i, j, k, n)] -
+                   sc.Fy[g.fy.at(i, j + 1, k, n)] + sc.Fz[g.fz.at(i, j, k, n)] - sc.Fz[g.fz.at(i, j, k + 1, n)];
+          if (n >= 2 && p->viscous) {
+            const int v = (int)n - 1;
+            auto F = [&](const std::vector<Dual>& A, long ii, long jj, long kk) { return A[g.fv.at(ii, jj, kk, v)]; };
+            s += c1 * (F(sc.Fvx, i, j + 2, k + 2) - F(sc.Fvx, i + 4, j + 2, k + 2)) +
+                 c2 * (F(sc.Fvx, i + 1, j + 2, k + 2) - F(sc.Fvx, i + 3, j + 2, k + 2));
+            s += c1 * (F(sc.Fvy, i + 2, j, k + 2) - F(sc.Fvy, i + 2, j + 4, k + 2)) +
+                 c2 * (F(sc.Fvy, i + 2, j + 1, k + 2) - F(sc.Fvy, i + 2, j + 3, k + 2));
+            s += c1 * (F(sc.Fvz, i + 2, j + 2, k) - F(sc.Fvz, i + 2, j + 2, k + 4)) +
+                 c2 * (F(sc.Fvz, i + 2, j + 2, k + 1) - F(sc.Fvz, i + 2, j + 2, k + 3));
+          }
+          R[g.cell.at(i + NG, j + NG, k + NG, n)] = s * Jc;
+        }
+      }
+}
+
+std::vector<Dual> duals(const double* v, const double* d, size_t n) {
+  std::vector<Dual> out(n);
+  for (size_t i = 0; i < n; ++i) out[i] = Dual(v[i], d ? d[i] : 0.0);
+  return out;
+}
+
+void split(const std::vector<Dual>& a, double* v, double* d) {
+  for (size_t i = 0; i < a.size(); ++i) {
+    if (v) v[i] = a[i].v;
+    if (d) d[i] = a[i].d;
+  }
+}
+
+// Ghost procedure of an RK stage (exchange, then fill; solver.jl:439-443) in Dual.
+void ghosts_dual(const f3d_params* p, Dual* QD, const Dual* inflowD, const Dual* qinfD, const Dual* TwD) {
+  exchange_ghost<Dual>(p, QD, 6);
+  fill_ghost<Dual, Dual>(p, QD, nullptr, inflowD, qinfD, TwD);
+}
+
+// Jacobian-vector product of the RHS with respect to Q (all cells, ghosts included), mask frozen.
+void rhs_jvp(const f3d_params* p, double* R, double* dR, const double* Q, const double* dQ, const void* const* mv,
+             const double* J) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  DualMetrics dm(p, mv);
+  std::vector<Dual> QD = duals(Q, dQ, 6 * nc), RD(5 * nc);
+  rhs_dual(p, dm, RD.data(), QD.data(), J);
+  split(RD, R, dR);
+}
+
+// nsteps RK3 steps of the FP64 path (solver.jl:416-444) in Dual, from Q0 = ghosts(c2Prim(U0)),
+// with the objective sum_n dt <W, Q^{n+1}> over interior cells and all 6 primitive variables.
+// Tangents: dU (interior), dinflow (inflow layout), dqinf[6], dTw; any may be null (zero).
+void time_step_jvp(const f3d_params* p, int nsteps, double* U, double* dU, const double* inflow,
+                   const double* dinflow, const double* dqinf, const double* dTw, const void* const* mv,
+                   const double* J, double dt, const double* W, double* obj, double* dobj) {
+  Grid g(p);
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  DualMetrics dm(p, mv);
+  std::vector<Dual> UD = duals(U, dU, 5 * nc), UnD(5 * nc), QD(6 * nc), RD(5 * nc);
+  const size_t nin = (size_t)g.Nyt * g.Nzt * 6;
+  std::vector<Dual> inflowD;
+  if (inflow) inflowD = duals(inflow, dinflow, nin);
+  Dual qinfD[6], TwD(p->T_wall, dTw ? *dTw : 0.0);
+  for (int n = 0; n < 6; ++n) qinfD[n] = Dual(p->q_inf[n], dqinf ? dqinf[n] : 0.0);
+  const Dual* inP = inflow ? inflowD.data() : nullptr;
+  c2prim<Dual, Dual>(p, UD.data(), QD.data());
+  ghosts_dual(p, QD.data(), inP, qinfD, &TwD);
+  Dual J_obj(0.0);
+  const long NG = g.NG;
+  for (int t = 0; t < nsteps; ++t) {
+    UnD = UD;
+    for (int KRK = 1; KRK <= 3; ++KRK) {
+      rhs_dual(p, dm, RD.data(), QD.data(), J);
+      const double a = KRK == 1 ? 1.0 : (KRK == 2 ? 0.25 : 2.0 / 3.0);
+      const double b = KRK == 1 ? 0.0 : (KRK == 2 ? 0.75 : 1.0 / 3.0);
+      for (long n = 1; n <= 5; ++n)
+        for (long k = NG + 1; k <= g.Nzp + NG; ++k)
+          for (long j = NG + 1; j <= g.Nyp + NG; ++j)
+            for (long i = NG + 1; i <= g.Nxp + NG; ++i) {
+              const size_t o = g.cell.at(i, j, k, n);
+              const Dual u1 = UD[o] + RD[o] * dt;
+              UD[o] = KRK == 1 ? u1 : u1 * a + UnD[o] * b;
+            }
+      c2prim<Dual, Dual>(p, UD.data(), QD.data());
+      ghosts_dual(p, QD.data(), inP, qinfD, &TwD);
+    }
+    if (W) {
+      Dual acc(0.0);
+      for (long n = 1; n <= 6; ++n)
+        for (long k = NG + 1; k <= g.Nzp + NG; ++k)
+          for (long j = NG + 1; j <= g.Nyp + NG; ++j)
+            for (long i = NG + 1; i <= g.Nxp + NG; ++i) {
+              const size_t o = g.cell.at(i, j, k, n);
+              if (W[o] != 0.0) acc += QD[o] * W[o];
+            }
+      J_obj += acc * dt;
+    }
+  }
+  split(UD, U, dU);
+  if (obj) *obj = J_obj.v;
+  if (dobj) *dobj = J_obj.d;
+}
 }  // namespace
+
+OR_API void oracle_rhs_jvp_f64(const f3d_params* p, double* R, double* dR, const double* Q, const double* dQ,
+                               const void* const* m, const double* J) {
+  rhs_jvp(p, R, dR, Q, dQ, m, J);
+}
+
+OR_API void oracle_time_step_jvp_f64(const f3d_params* p, int nsteps, double* U, double* dU, const double* inflow,
+                                     const double* dinflow, const double* dqinf, const double* dTw,
+                                     const void* const* m, const double* J, double dt, const double* W, double* obj,
+                                     double* dobj) {
+  time_step_jvp(p, nsteps, U, dU, inflow, dinflow, dqinf, dTw, m, J, dt, W, obj, dobj);
+}
+
+// Forward-mode c2Prim (interior) and stage ghost procedure (exchange + fill), FP64.
+OR_API void oracle_c2prim_jvp_f64(const f3d_params* p, const double* U, const double* dU, double* Q, double* dQ) {
+  Grid g(p);
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  std::vector<Dual> UD = duals(U, dU, 5 * nc), QD = duals(Q, dQ, 6 * nc);
+  c2prim<Dual, Dual>(p, UD.data(), QD.data());
+  split(QD, Q, dQ);
+}
+
+OR_API void oracle_ghosts_jvp_f64(const f3d_params* p, double* Q, double* dQ, const double* inflow,
+                                  const double* dinflow, const double* dqinf, const double* dTw) {
+  Grid g(p);
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  std::vector<Dual> QD = duals(Q, dQ, 6 * nc), inflowD;
+  if (inflow) inflowD = duals(inflow, dinflow, (size_t)g.Nyt * g.Nzt * 6);
+  Dual qinfD[6], TwD(p->T_wall, dTw ? *dTw : 0.0);
+  for (int n = 0; n < 6; ++n) qinfD[n] = Dual(p->q_inf[n], dqinf ? dqinf[n] : 0.0);
+  ghosts_dual(p, QD.data(), inflow ? inflowD.data() : nullptr, qinfD, &TwD);
+  split(QD, Q, dQ);
+}
+
+// advect_c in Dual: face fluxes and their tangents from tangents of Fp, Fm and Q (mask from phi).
+OR_API void oracle_advect_c_jvp_f64(const f3d_params* p, int d, double* F, double* dF, const double* phi,
+                                    const double* S, const double* Fp, const double* dFp, const double* Fm,
+                                    const double* dFm, const double* Q, const double* dQ, const double* Ax,
+                                    const double* Ay, const double* Az) {
+  Grid g(p);
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  const Ext& fe = d == 0 ? g.fx : (d == 1 ? g.fy : g.fz);
+  const size_t nf = (size_t)fe.n1 * fe.n2 * fe.n3 * 5;
+  std::vector<Dual> FpD = duals(Fp, dFp, 5 * nc), FmD = duals(Fm, dFm, 5 * nc), QD = duals(Q, dQ, 6 * nc);
+  std::vector<Dual> phiD = duals(phi, nullptr, nc), SD = duals(S, nullptr, nc), AxD = duals(Ax, nullptr, nc),
+                    AyD = duals(Ay, nullptr, nc), AzD = duals(Az, nullptr, nc), FD(nf);
+  advect_c<Dual>(p, d, FD.data(), phiD.data(), SD.data(), FpD.data(), FmD.data(), QD.data(), AxD.data(), AyD.data(),
+                 AzD.data());
+  split(FD, F, dF);
+}
 
 OR_API void oracle_rhs_ld(const f3d_params* p, double* R, const double* Q, const void* const* m, const double* J) {
   rhs_ld(p, R, Q, m, J);

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../include/f3d.h"
+#include "f3d_dual.hpp"
 
 namespace f3d_oracle {
 
@@ -70,22 +71,22 @@ inline void linear_coeffs(const f3d_params* p, T c[7]) {
 // ---------------------------------------------------------------- utils.jl
 // c2Prim  utils.jl:2-30 (interior [NG+1, Np+NG]^3).  The eps floors are the machine epsilon of the
 // type the conversion runs in (the reference uses eps(Float32) in its Float32 build).
-template <typename T>
-void c2prim(const f3d_params* p, const double* U, T* Q) {
+template <typename T, typename TU = double>
+void c2prim(const f3d_params* p, const TU* U, T* Q) {
   Grid g(p);
   const double gam = p->gamma, Rg = p->Rg, eps = std::numeric_limits<double>::epsilon();
 #pragma omp parallel for collapse(2)
   for (long k = g.NG + 1; k <= g.Nzp + g.NG; ++k)
     for (long j = g.NG + 1; j <= g.Nyp + g.NG; ++j)
       for (long i = g.NG + 1; i <= g.Nxp + g.NG; ++i) {
-        double rho = jmax(U[g.cell.at(i, j, k, 1)], eps);
-        double rinv = 1 / rho;
-        double u = U[g.cell.at(i, j, k, 2)] * rinv;
-        double v = U[g.cell.at(i, j, k, 3)] * rinv;
-        double w = U[g.cell.at(i, j, k, 4)] * rinv;
-        double ei = jmax(U[g.cell.at(i, j, k, 5)] - 0.5 * rho * (u * u + v * v + w * w), eps);
-        double pr = (gam - 1) * ei;
-        double Tt = pr / (rho * Rg);
+        TU rho = jmax(U[g.cell.at(i, j, k, 1)], TU(eps));
+        TU rinv = 1 / rho;
+        TU u = U[g.cell.at(i, j, k, 2)] * rinv;
+        TU v = U[g.cell.at(i, j, k, 3)] * rinv;
+        TU w = U[g.cell.at(i, j, k, 4)] * rinv;
+        TU ei = jmax(U[g.cell.at(i, j, k, 5)] - 0.5 * rho * (u * u + v * v + w * w), TU(eps));
+        TU pr = (gam - 1) * ei;
+        TU Tt = pr / (rho * Rg);
         Q[g.cell.at(i, j, k, 1)] = (T)rho;
         Q[g.cell.at(i, j, k, 2)] = (T)u;
         Q[g.cell.at(i, j, k, 3)] = (T)v;
@@ -226,6 +227,20 @@ struct Weno7Weights<long double> {
   }
 };
 template <>
+struct Weno7Weights<Dual> {  // literal form, as for double
+  static Dual combine(Dual eps, Dual s0, const Dual Is[4], const Dual q[4]) {
+    const double d[4] = {1, 12, 18, 4};
+    Dual a[4], sum = 0, acc = 0;
+    for (int m = 0; m < 4; ++m) {
+      const Dual b = eps + Is[m] * s0;
+      a[m] = d[m] / (b * b);
+      sum += a[m];
+      acc += a[m] * q[m];
+    }
+    return acc / sum;
+  }
+};
+template <>
 struct Weno7Weights<float> {
   static float combine(float eps, float s0, const float Is[4], const float q[4]) {
     float b[4];
@@ -267,6 +282,42 @@ inline T weno7(const T* V, T s0, T eps) {
   return Weno7Weights<T>::combine(eps, s0, Is, q);
 }
 
+// Forward mode of weno7: the tangent of a smoothness indicator below the rounding floor of its own
+// evaluation (Is < 1e-12 x the quadratic form on |V| with |coefficients|) is dropped, so a stencil
+// flat to the last bit keeps its weight frozen (with eps = 1e-20 its weight is 1/noise^2 and the
+// derivative a ratio of noise terms).  The adjoint uses the same convention.
+template <>
+inline Dual weno7<Dual>(const Dual* V, Dual s0, Dual eps) {
+  const Dual V1 = V[0], V2 = V[1], V3 = V[2], V4 = V[3], V5 = V[4], V6 = V[5], V7 = V[6];
+  Dual q[4], Is[4];
+  q[0] = -3 * V1 + 13 * V2 - 23 * V3 + 25 * V4;
+  q[1] = V2 - 5 * V3 + 13 * V4 + 3 * V5;
+  q[2] = -V3 + 7 * V4 + 7 * V5 - V6;
+  q[3] = 3 * V4 + 13 * V5 - 5 * V6 + V7;
+  Is[0] = V1 * (547 * V1 - 3882 * V2 + 4642 * V3 - 1854 * V4) + V2 * (7043 * V2 - 17246 * V3 + 7042 * V4) +
+          V3 * (11003 * V3 - 9402 * V4) + V4 * (2107 * V4);
+  Is[1] = V2 * (267 * V2 - 1642 * V3 + 1602 * V4 - 494 * V5) + V3 * (2843 * V3 - 5966 * V4 + 1922 * V5) +
+          V4 * (3443 * V4 - 2522 * V5) + V5 * (547 * V5);
+  Is[2] = V3 * (547 * V3 - 2522 * V4 + 1922 * V5 - 494 * V6) + V4 * (3443 * V4 - 5966 * V5 + 1602 * V6) +
+          V5 * (2843 * V5 - 1642 * V6) + V6 * (267 * V6);
+  Is[3] = V4 * (2107 * V4 - 9402 * V5 + 7042 * V6 - 1854 * V7) + V5 * (11003 * V5 - 17246 * V6 + 4642 * V7) +
+          V6 * (7043 * V6 - 3882 * V7) + V7 * (547 * V7);
+  const double a1 = std::fabs(V1.v), a2 = std::fabs(V2.v), a3 = std::fabs(V3.v), a4 = std::fabs(V4.v),
+               a5 = std::fabs(V5.v), a6 = std::fabs(V6.v), a7 = std::fabs(V7.v);
+  const double Ia[4] = {
+      a1 * (547 * a1 + 3882 * a2 + 4642 * a3 + 1854 * a4) + a2 * (7043 * a2 + 17246 * a3 + 7042 * a4) +
+          a3 * (11003 * a3 + 9402 * a4) + a4 * (2107 * a4),
+      a2 * (267 * a2 + 1642 * a3 + 1602 * a4 + 494 * a5) + a3 * (2843 * a3 + 5966 * a4 + 1922 * a5) +
+          a4 * (3443 * a4 + 2522 * a5) + a5 * (547 * a5),
+      a3 * (547 * a3 + 2522 * a4 + 1922 * a5 + 494 * a6) + a4 * (3443 * a4 + 5966 * a5 + 1602 * a6) +
+          a5 * (2843 * a5 + 1642 * a6) + a6 * (267 * a6),
+      a4 * (2107 * a4 + 9402 * a5 + 7042 * a6 + 1854 * a7) + a5 * (11003 * a5 + 17246 * a6 + 4642 * a7) +
+          a6 * (7043 * a6 + 3882 * a7) + a7 * (547 * a7)};
+  for (int m = 0; m < 4; ++m)
+    if (Is[m].v < 1e-12 * Ia[m]) Is[m].d = 0.0;
+  return Weno7Weights<Dual>::combine(eps, s0, Is, q);
+}
+
 // One WENO5 reconstruction of a 5-point stencil  (schemes.jl:958-972)
 template <typename T>
 inline T weno5(const T* V, T s0, T eps) {
@@ -284,6 +335,33 @@ inline T weno5(const T* V, T s0, T eps) {
   T v1 = 2 * V1 - 7 * V2 + 11 * V3;
   T v2 = -V2 + 5 * V3 + 2 * V4;
   T v3 = 2 * V3 + 5 * V4 - V5;
+  return invsum * (s11 * v1 + s22 * v2 + s33 * v3);
+}
+
+// Forward mode of weno5 with the rounding-floor convention of weno7<Dual>: Is = 13 a^2 + 3 c^2 below
+// 1e-24 x (13 A^2 + 3 C^2) (A, C the |V|-sums of the differences a, c) passes no tangent.
+template <>
+inline Dual weno5<Dual>(const Dual* V, Dual s0, Dual eps) {
+  const Dual V1 = V[0], V2 = V[1], V3 = V[2], V4 = V[3], V5 = V[4];
+  const Dual a1 = V1 - 2 * V2 + V3, b1 = V1 - 4 * V2 + 3 * V3;
+  const Dual a2 = V2 - 2 * V3 + V4, b2 = V2 - V4;
+  const Dual a3 = V3 - 2 * V4 + V5, b3 = 3 * V3 - 4 * V4 + V5;
+  Dual s11 = 13 * (a1 * a1) + 3 * (b1 * b1);
+  Dual s22 = 13 * (a2 * a2) + 3 * (b2 * b2);
+  Dual s33 = 13 * (a3 * a3) + 3 * (b3 * b3);
+  const double x1 = std::fabs(V1.v), x2 = std::fabs(V2.v), x3 = std::fabs(V3.v), x4 = std::fabs(V4.v),
+               x5 = std::fabs(V5.v);
+  auto fl = [](double A, double C) { return 1e-24 * (13 * A * A + 3 * C * C); };
+  if (s11.v < fl(x1 + 2 * x2 + x3, x1 + 4 * x2 + 3 * x3)) s11.d = 0.0;
+  if (s22.v < fl(x2 + 2 * x3 + x4, x2 + x4)) s22.d = 0.0;
+  if (s33.v < fl(x3 + 2 * x4 + x5, 3 * x3 + 4 * x4 + x5)) s33.d = 0.0;
+  s11 = 1 / ((eps + s11 * s0) * (eps + s11 * s0));
+  s22 = 6 / ((eps + s22 * s0) * (eps + s22 * s0));
+  s33 = 3 / ((eps + s33 * s0) * (eps + s33 * s0));
+  Dual invsum = 1 / (s11 + s22 + s33);
+  Dual v1 = 2 * V1 - 7 * V2 + 11 * V3;
+  Dual v2 = -V2 + 5 * V3 + 2 * V4;
+  Dual v3 = 2 * V3 + 5 * V4 - V5;
   return invsum * (s11 * v1 + s22 * v2 + s33 * v3);
 }
 
@@ -584,8 +662,11 @@ void exchange_ghost(const f3d_params* p, T* Q, int NV) {
 
 // Ghost fill  boundary.jl:1-156 generalised to per-face kinds; applied in the order fill_x,
 // fill_y, fill_z (boundary.jl:153-155), each over the full extent of the other two dimensions.
-template <typename T>
-void fill_ghost(const f3d_params* p, T* Q, double* U, const T* inflow) {
+// P is the type the ghost values are formed in (double, or Dual for the forward-mode reference);
+// qinf / Tw override p->q_inf / p->T_wall when given.
+template <typename T, typename P = double>
+void fill_ghost(const f3d_params* p, T* Q, double* U, const T* inflow, const P* qinf = nullptr,
+                const P* Tw = nullptr) {
   Grid g(p);
   const long Np[3] = {g.Nxp, g.Nyp, g.Nzp};
   const double gam = p->gamma, Rg = p->Rg;
@@ -604,7 +685,7 @@ void fill_ghost(const f3d_params* p, T* Q, double* U, const T* inflow) {
               return g.cell.at(d == 0 ? to : i, d == 1 ? to : j, d == 2 ? to : k);
             };
             const size_t o = g.cell.at(i, j, k);
-            double q[6];
+            P q[6];
             if (bc == F3D_BC_PERIODIC) {  // fill_z  boundary.jl:132-146
               const size_t s = shifted(side == 0 ? idx + Np[d] : idx - Np[d]);
               for (int n = 0; n < 6; ++n) Q[o + n * nc] = Q[s + n * nc];
@@ -618,29 +699,29 @@ void fill_ghost(const f3d_params* p, T* Q, double* U, const T* inflow) {
                 for (int n = 0; n < 5; ++n) U[o + n * nc] = U[s + n * nc];
               continue;
             } else if (bc == F3D_BC_DIRICHLET) {  // boundary.jl:12-38, 98-118
-              for (int n = 0; n < 6; ++n) q[n] = p->q_inf[n];
+              for (int n = 0; n < 6; ++n) q[n] = qinf ? qinf[n] : P(p->q_inf[n]);
             } else if (bc == F3D_BC_INFLOW) {
               for (int n = 0; n < 6; ++n)
-                q[n] = inflow[(size_t)(j - 1) + (size_t)g.Nyt * ((size_t)(k - 1) + (size_t)g.Nzt * n)];
+                q[n] = (P)inflow[(size_t)(j - 1) + (size_t)g.Nyt * ((size_t)(k - 1) + (size_t)g.Nzt * n)];
             } else {  // walls: mirror index 2NG+1-idx (boundary.jl:69), or its high-side image
               const long mi = side == 0 ? 2 * g.NG + 1 - idx : 2 * (Np[d] + g.NG) + 1 - idx;
               const size_t s = shifted(mi);
-              q[0] = Q[s];
-              q[1] = -(double)Q[s + nc];
-              q[2] = -(double)Q[s + 2 * nc];
-              q[3] = -(double)Q[s + 3 * nc];
-              q[4] = Q[s + 4 * nc];
-              q[5] = Q[s + 5 * nc];
+              q[0] = (P)Q[s];
+              q[1] = -(P)Q[s + nc];
+              q[2] = -(P)Q[s + 2 * nc];
+              q[3] = -(P)Q[s + 3 * nc];
+              q[4] = (P)Q[s + 4 * nc];
+              q[5] = (P)Q[s + 5 * nc];
               if (bc == F3D_BC_WALL_ISOTHERMAL) {
-                q[5] = 2 * p->T_wall - q[5];
+                q[5] = 2 * (Tw ? *Tw : P(p->T_wall)) - q[5];
                 q[0] = q[4] / (Rg * q[5]);
               }
             }
             for (int n = 0; n < 6; ++n) Q[o + n * nc] = (T)q[n];
             if (U) {
               // U from the stored (rounded) Q, as boundary.jl:82-92 reads Q back
-              double rho = (double)Q[o], u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc];
-              double pr = Q[o + 4 * nc];
+              double rho = (double)Q[o], u = (double)Q[o + nc], v = (double)Q[o + 2 * nc], w = (double)Q[o + 3 * nc];
+              double pr = (double)Q[o + 4 * nc];
               U[o] = rho;
               U[o + nc] = rho * u;
               U[o + 2 * nc] = rho * v;

@@ -68,7 +68,7 @@ def tgv_case(N=16, dtype=np.float64, noise=1e-3, jump=0.6, seed=1, viscous=1, sp
 
 
 def ramp_case(Nx=24, Ny=16, Nz=8, sweep=0.0, dtype=np.float64, seed=2, viscous=1, noise=0.01, jump=0.0,
-              spikes=(), **pkw):
+              spikes=(), interior_noise=0.0, **pkw):
     c = cases.ramp(Nx, Ny, Nz, sweep=sweep, seed=seed, noise=noise)
     kw = dict(c["params"])
     kw["viscous"] = viscous
@@ -85,6 +85,15 @@ def ramp_case(Nx=24, Ny=16, Nz=8, sweep=0.0, dtype=np.float64, seed=2, viscous=1
         Q[4, :, :, sl] *= 1 + jump
         Q[0, :, :, sl] *= 1 + jump
     Q = plant_spikes(Q, spikes)
+    if interior_noise:
+        # seeded relative noise on rho, u, v, w, p (no stencil flat to the last bit)
+        r = np.random.default_rng(seed + 100)
+        u_ref = float(np.abs(Q[1]).max())
+        for n in (0, 4):
+            Q[n] *= 1 + interior_noise * (2 * r.random(Q[n].shape) - 1)
+        for n in (1, 2, 3):
+            Q[n] += interior_noise * u_ref * (2 * r.random(Q[n].shape) - 1)
+        Q[5] = Q[4] / (Q[0] * p.Rg)
     Q = np.ascontiguousarray(Q.astype(dtype))
     inflow = np.ascontiguousarray(c["inflow"].astype(dtype))
     _fill(p, Q, inflow)

@@ -105,6 +105,29 @@ class Oracle:
         """J * sum of |terms| of the RHS per cell and variable (scale of the parity bar)."""
         self._c(f"oracle_rhs_absterms_{suf(Q)}", P(A), P(Q), PP(m), P(J))
 
+    def rhs_jvp(self, R, dR, Q, dQ, m, J):
+        """Forward-mode JVP of the FP64 RHS w.r.t. Q (all cells), mask frozen at Q."""
+        self._c("oracle_rhs_jvp_f64", P(R), P(dR), P(Q), P(dQ), PP(m), P(J))
+
+    def time_step_jvp(self, nsteps, U, dU, m, J, dt, inflow=None, dinflow=None, dqinf=None, dTw=None, W=None):
+        """Forward-mode derivative of nsteps FP64 RK3 steps from Q0 = ghosts(c2Prim(U0)), mask frozen;
+        U/dU are advanced in place.  Returns (objective, d objective) for sum_n dt <W, Q^{n+1}>."""
+        obj = ctypes.c_double(0.0)
+        dobj = ctypes.c_double(0.0)
+        dq = None if dqinf is None else np.ascontiguousarray(dqinf, dtype=np.float64)
+        dt_w = None if dTw is None else np.array([dTw], dtype=np.float64)
+        self._c("oracle_time_step_jvp_f64", int(nsteps), P(U), P(dU), P(inflow), P(dinflow), P(dq), P(dt_w),
+                PP(m), P(J), ctypes.c_double(dt), P(W), ctypes.byref(obj), ctypes.byref(dobj))
+        return obj.value, dobj.value
+
+    def c2prim_jvp(self, U, dU, Q, dQ):
+        self._c("oracle_c2prim_jvp_f64", P(U), P(dU), P(Q), P(dQ))
+
+    def ghosts_jvp(self, Q, dQ, inflow=None, dinflow=None, dqinf=None, dTw=None):
+        dq = None if dqinf is None else np.ascontiguousarray(dqinf, dtype=np.float64)
+        dt_w = None if dTw is None else np.array([dTw], dtype=np.float64)
+        self._c("oracle_ghosts_jvp_f64", P(Q), P(dQ), P(inflow), P(dinflow), P(dq), P(dt_w))
+
     def rhs_ld(self, R, Q, m, J):
         """RHS in long double from float64 inputs, mask frozen from the float64 sensor."""
         self._c("oracle_rhs_ld", P(R), P(Q), PP(m), P(J))

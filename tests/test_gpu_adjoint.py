@@ -1,0 +1,288 @@
+"""GPU adjoint (hand-written VJP, mask frozen) against the CPU forward-mode oracle and central FD.
+
+The oracle has no reverse mode: it instantiates its own restatement of the reference with
+dual numbers (oracle/f3d_dual.hpp), giving J v for any direction v.  The VJP is pinned by
+  * the dot-product identity  <lam, J v> = <J^T lam, v>  for random lam and v, and restricted to
+    each variable and to the interior / ghost cells separately, so a wrong entry cannot hide;
+  * sampled entries: (J^T lam)_j against <lam, J e_j> from one JVP per sampled j;
+  * the RK3 stage and the checkpointed multi-step adjoint against the oracle's forward-mode
+    derivative of the same steps (state, inflow profile, q_inf and T_wall directions), against
+    GPU central finite differences of the objective, and checkpoint interval 1 against 2.
+Bars (relative to the sum of |terms| of the dot product): 1e-12 where every face is UP7; 1e-9
+where WENO faces occur.  The WENO7 weight derivative (q_m - W) d(alpha_m)/S is O(1) however
+flat the stencil is (the weights depend on ratios of smoothness indicators), so its rounding
+error is that of Is itself, eps * (sum of |terms|) / Is: ~1e-10 for data with 1e-3 relative
+variation.  Stencils flat to the last bit (a uniform freestream next to a jump) carry no
+weight derivative at all, by the convention of f3d_adjoint.hpp (their computed weight
+derivative is a ratio of rounding noises), so such cases are checked with all faces UP7, and
+WENO cases carry seeded 1e-3 interior variation.  The multi-step gradient: within 1e-7 of
+forward mode (the two trajectories differ by rounding amplified by the scheme, see
+test_gpu_parity) and within 1e-4 of central FD (truncation and mask flips); checkpoint
+interval 2 against 1 within 1e-8 (the forward heavy-face correction adds with floating-point
+atomics, so recomputed stage states differ from the first pass in the last bits).
+"""
+import numpy as np
+import pytest
+
+from casegen import f3d, ramp_case, tgv_case
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+UP7 = dict(hybrid_phi1=1e30, hybrid_phi2=2e30, hybrid_phi3=3e30)
+NOISE = 1e-3
+CASES = [
+    ("tgv16", lambda: tgv_case(16), 1e-9),
+    ("tgv16_nnd", lambda: tgv_case(16, hybrid_phi3=2.5), 1e-9),
+    ("tgv12_inviscid", lambda: tgv_case(12, viscous=0), 1e-9),
+    ("ramp", lambda: ramp_case(24, 16, 8, jump=1.0, spikes=[(5, 3, 2, 20.0), (15, 8, 5, 3.0)], hybrid_phi3=2.5,
+                               interior_noise=NOISE), 1e-9),
+    ("swept", lambda: ramp_case(20, 12, 10, sweep=30.0, jump=0.5, interior_noise=NOISE), 1e-9),
+    ("ramp_adiabatic", lambda: ramp_case(16, 12, 8, jump=0.5, bc=[5, 2, 3, 1, 0, 0], interior_noise=NOISE), 1e-9),
+    ("ramp_flat_up7", lambda: ramp_case(24, 16, 8, jump=1.0, **UP7), 1e-12),
+]
+IDS = [c[0] for c in CASES]
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+INT = (slice(None),) + (slice(4, -4),) * 3
+
+
+def interior_only(a):
+    out = np.zeros_like(a)
+    out[INT] = a[INT]
+    return out
+
+
+def rand_lambda(case, rng):
+    """Random interior adjoint, each variable scaled by 1/max|R_n| so that all five contribute."""
+    p, o = case.params, Oracle(case.params)
+    R = np.zeros(p.cell_shape(5))
+    o.rhs(R, case.Q, case.metrics, case.J)
+    lam = rng.standard_normal(R.shape)
+    for n in range(5):
+        lam[n] /= max(np.abs(R[n][INT[1:]]).max(), 1e-300)
+    return interior_only(lam)
+
+
+def rand_dQ(case, rng, rel=1e-3):
+    dQ = rng.standard_normal(case.Q.shape) * np.abs(case.Q) * rel
+    return dQ
+
+
+def gpu_rhs_vjp(torch, case, lam):
+    p = case.params
+    ctx = f3d.Flame3D(p)
+    lamQ = torch.zeros(p.cell_shape(6), dtype=torch.float64, device="cuda:0")
+    ctx.rhs_vjp(lamQ, dev(torch, lam), dev(torch, case.Q), [dev(torch, m) for m in case.metrics], dev(torch, case.J))
+    return host(lamQ)
+
+
+def cpu_rhs_jvp(case, dQ):
+    p, o = case.params, Oracle(case.params)
+    R = np.zeros(p.cell_shape(5))
+    dR = np.zeros_like(R)
+    o.rhs_jvp(R, dR, case.Q, dQ, case.metrics, case.J)
+    return dR
+
+
+def dot_check(name, a_terms, b_terms, tol):
+    a, b = float(np.sum(a_terms)), float(np.sum(b_terms))
+    scale = max(float(np.sum(np.abs(a_terms))), float(np.sum(np.abs(b_terms))), 1e-300)
+    err = abs(a - b) / scale
+    print(f"{name}: <lam,Jv> {a:.6e}  <J^T lam,v> {b:.6e}  rel {err:.2e}")
+    assert err <= tol, f"{name}: {err:.3e} > {tol}"
+    return err
+
+
+@pytest.mark.parametrize("mk,tol", [(c[1], c[2]) for c in CASES], ids=IDS)
+def test_rhs_vjp_dot_products(torch_mod, mk, tol):
+    case = mk()
+    rng = np.random.default_rng(7)
+    lam = rand_lambda(case, rng)
+    lamQ = gpu_rhs_vjp(torch_mod, case, lam)
+    assert np.all(np.isfinite(lamQ))
+    assert np.all(lamQ[4] == 0.0), "the RHS does not read p"
+    for trial in range(2):
+        dQ = rand_dQ(case, rng)
+        dR = cpu_rhs_jvp(case, dQ)
+        dot_check(f"{case.name} random {trial}", lam * dR, lamQ * dQ, tol)
+    # each variable and region on its own
+    ghost = np.ones(case.Q.shape[1:], dtype=bool)
+    ghost[INT[1:]] = False
+    for n in (0, 1, 2, 3, 5):
+        for region, mask in (("interior", ~ghost), ("ghost", ghost)):
+            dQ = np.zeros_like(case.Q)
+            dQ[n][mask] = rand_dQ(case, rng)[n][mask]
+            dR = cpu_rhs_jvp(case, dQ)
+            dot_check(f"{case.name} var{n} {region}", lam * dR, lamQ * dQ, tol)
+
+
+@pytest.mark.parametrize("mk", [CASES[0][1], CASES[3][1]], ids=[IDS[0], IDS[3]])
+def test_rhs_vjp_sampled_entries(torch_mod, mk):
+    case = mk()
+    rng = np.random.default_rng(11)
+    lam = rand_lambda(case, rng)
+    lamQ = gpu_rhs_vjp(torch_mod, case, lam)
+    flat = np.abs(lamQ).reshape(-1)
+    # the largest entries plus random ones (ghosts included)
+    idx = list(np.argsort(flat)[-12:]) + list(rng.choice(flat.size, 12, replace=False))
+    for j in idx:
+        e = np.zeros(case.Q.size)
+        e[j] = 1.0
+        col = cpu_rhs_jvp(case, e.reshape(case.Q.shape))
+        ref = float(np.sum(lam * col))
+        scale = max(float(np.sum(np.abs(lam * col))), 1e-300)
+        got = float(lamQ.reshape(-1)[j])
+        assert abs(got - ref) <= 1e-8 * scale + 1e-300, (case.name, np.unravel_index(j, case.Q.shape), got, ref)
+
+
+def test_rhs_autograd_function(torch_mod):
+    torch = torch_mod
+    case = CASES[3][1]()
+    p = case.params
+    ctx = f3d.Flame3D(p)
+    Q = dev(torch, case.Q).requires_grad_(True)
+    m = [dev(torch, a) for a in case.metrics]
+    J = dev(torch, case.J)
+    R = f3d.RHSFunction.apply(Q, ctx, m, J)
+    lam = dev(torch, rand_lambda(case, np.random.default_rng(3)))
+    (R * lam).sum().backward()
+    ref = gpu_rhs_vjp(torch, case, host(lam))
+    # equal up to the order of the heavy-face atomics
+    assert np.max(np.abs(host(Q.grad) - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def _stage_case(name):
+    if name == "ramp":
+        return ramp_case(16, 12, 8, jump=1.0, spikes=[(5, 3, 2, 20.0)], hybrid_phi3=2.5, interior_noise=1e-3)
+    return tgv_case(12)
+
+
+@pytest.mark.parametrize("KRK", [1, 2, 3])
+@pytest.mark.parametrize("name", ["ramp", "tgv"])
+def test_stage_vjp(torch_mod, name, KRK):
+    torch = torch_mod
+    case = _stage_case(name)
+    p, o = case.params, Oracle(case.params)
+    rng = np.random.default_rng(20 + KRK)
+    Us = case.U()
+    dt = o.cfl_dt(case.Q, case.metrics, case.J, 0.3)
+    a = {1: 1.0, 2: 0.25, 3: 2.0 / 3.0}[KRK]
+    b = {1: 0.0, 2: 0.75, 3: 1.0 / 3.0}[KRK]
+    # forward-mode stage derivative from oracle pieces
+    dU = interior_only(rng.standard_normal(Us.shape) * np.abs(Us) * 1e-3)
+    dUn = interior_only(rng.standard_normal(Us.shape) * np.abs(Us) * 1e-3) if KRK > 1 else np.zeros_like(Us)
+    dinfl = None if case.inflow is None else rng.standard_normal(case.inflow.shape) * np.abs(case.inflow) * 1e-3
+    dqinf = rng.standard_normal(6) * np.abs(np.array(list(p.q_inf))) * 1e-3
+    dTw = 0.3
+    Qc = case.Q.copy()
+    dQ = np.zeros_like(Qc)
+    o.c2prim_jvp(Us, dU, Qc, dQ)
+    o.ghosts_jvp(Qc, dQ, case.inflow, dinfl, dqinf, dTw)
+    dR = cpu_rhs_jvp(case, dQ)
+    dUout = interior_only(a * (dU + dt * dR) + b * dUn)
+    # reverse on the GPU
+    lam_out = interior_only(rng.standard_normal(Us.shape) / np.maximum(np.abs(Us), 1e-30))
+    ctx = f3d.Flame3D(p)
+    t_lam = dev(torch, lam_out)
+    t_lams = torch.zeros_like(t_lam)
+    t_lamUn = torch.zeros_like(t_lam)
+    t_lin = None if case.inflow is None else torch.zeros(case.inflow.shape, dtype=torch.float64, device="cuda:0")
+    t_bc = torch.zeros(7, dtype=torch.float64, device="cuda:0")
+    ctx.rk3_stage_vjp(KRK, t_lam, t_lams, t_lamUn, dev(torch, Us), dev(torch, case.Q),
+                      [dev(torch, m) for m in case.metrics], dev(torch, case.J), dt, t_lin, t_bc)
+    lam_s, lamUn, lam_bc = host(t_lams), host(t_lamUn), host(t_bc)
+    if KRK == 1:
+        assert not lamUn.any()
+    else:
+        np.testing.assert_allclose(lamUn[INT], b * lam_out[INT], rtol=0, atol=0)
+    rhs_terms = [np.ravel(lam_s * dU), np.ravel(lamUn * dUn), np.ravel(lam_bc[:6] * dqinf), [lam_bc[6] * dTw]]
+    if dinfl is not None:
+        rhs_terms.append(np.ravel(host(t_lin) * dinfl))
+    dot_check(f"{case.name} stage {KRK}", np.ravel(lam_out * dUout), np.concatenate(rhs_terms), 1e-9)
+
+
+def _wall_weights(case):
+    """Objective weights: wall pressure x wall-face area |grad eta|/J on the first cell row."""
+    W = np.zeros(case.params.cell_shape(6))
+    if case.params.bc[2] in (f3d.BC_WALL_ISOTHERMAL, f3d.BC_WALL_ADIABATIC):
+        W[4, 4:-4, 4, 4:-4] = case.s[1][4:-4, 4, 4:-4]
+    else:
+        W[4][INT[1:]] = case.s[1][INT[1:]]
+    return W
+
+
+@pytest.mark.parametrize("name", ["ramp", "tgv"])
+def test_multistep_adjoint(torch_mod, name):
+    torch = torch_mod
+    case = _stage_case(name)
+    p, o = case.params, Oracle(case.params)
+    nsteps = 3
+    U0 = case.U()
+    dt = o.cfl_dt(case.Q, case.metrics, case.J, 0.3)
+    W = _wall_weights(case)
+    ctx = f3d.Flame3D(p)
+    m = [dev(torch, a) for a in case.metrics]
+    J = dev(torch, case.J)
+    inflow = None if case.inflow is None else dev(torch, case.inflow)
+
+    def run(U0_np, every, inflow_t=inflow, params=None):
+        c = ctx if params is None else f3d.Flame3D(params)
+        obj = torch.zeros(nsteps, dtype=torch.float64, device="cuda:0")
+        lamU0 = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
+        lin = None if case.inflow is None else torch.zeros(case.inflow.shape, dtype=torch.float64, device="cuda:0")
+        lbc = torch.zeros(7, dtype=torch.float64, device="cuda:0")
+        c.adjoint(nsteps, every, dev(torch, U0_np), m, J, dt, dev(torch, W), obj, lamU0, inflow_t, lin, lbc)
+        return host(obj), host(lamU0), (None if lin is None else host(lin)), host(lbc)
+
+    obj1, g1, gin1, gbc1 = run(U0, 1)
+    obj2, g2, gin2, gbc2 = run(U0, 2)
+    np.testing.assert_allclose(obj2, obj1, rtol=1e-12)
+    for x, y in ((g2, g1), (gbc2, gbc1)) + (((gin2, gin1),) if gin1 is not None else ()):
+        assert np.max(np.abs(x - y)) <= 1e-8 * max(np.max(np.abs(y)), 1e-300)
+    ghost = np.ones(U0.shape[1:], dtype=bool)
+    ghost[INT[1:]] = False
+    assert not g1[:, ghost].any(), "ghost entries of the state gradient stay zero"
+    # forward mode through the same steps
+    rng = np.random.default_rng(5)
+    dU = interior_only(rng.standard_normal(U0.shape) * np.abs(U0) * 1e-3)
+    dinfl = None if case.inflow is None else rng.standard_normal(case.inflow.shape) * np.abs(case.inflow) * 1e-3
+    dqinf = rng.standard_normal(6) * np.abs(np.array(list(p.q_inf))) * 1e-3
+    dTw = 0.5 if p.bc[2] == f3d.BC_WALL_ISOTHERMAL else 0.0
+    Uw, dUw = U0.copy(), dU.copy()
+    objv, dobj = o.time_step_jvp(nsteps, Uw, dUw, case.metrics, case.J, dt, inflow=case.inflow, dinflow=dinfl,
+                                 dqinf=dqinf, dTw=dTw, W=W)
+    rel_obj = abs(obj1.sum() - objv) / abs(objv)
+    adj = float(np.sum(g1 * dU) + np.sum(gbc1[:6] * dqinf) + gbc1[6] * dTw
+                + (0.0 if dinfl is None else np.sum(gin1 * dinfl)))
+    rel = abs(adj - dobj) / abs(dobj)
+    print(f"{case.name}: objective {obj1.sum():.10e} (oracle {objv:.10e}, rel {rel_obj:.1e}); "
+          f"directional derivative adjoint {adj:.10e} forward-mode {dobj:.10e} rel {rel:.2e}")
+    assert rel_obj <= 1e-10
+    assert rel <= 1e-7
+    # central finite differences of the GPU objective along the state direction
+    eps = 1e-4
+    op, _, _, _ = run(U0 + eps * dU, 1)
+    om, _, _, _ = run(U0 - eps * dU, 1)
+    fd = (op.sum() - om.sum()) / (2 * eps)
+    adj_u = float(np.sum(g1 * dU))
+    rel_fd = abs(fd - adj_u) / max(abs(adj_u), 1e-300)
+    print(f"{case.name}: state direction FD {fd:.10e} adjoint {adj_u:.10e} rel {rel_fd:.2e}")
+    assert rel_fd <= 1e-4
