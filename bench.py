@@ -73,7 +73,15 @@ FUSED_BYTES = {
     # RHS 5 + viscous fluxes 12 + J in; U, Un in; U and Q out (stage-2/3 form)
     "fused_update": lambda p, es: _nint(p) * (5 * es + 12 * es + 8 + 40 + 40 + 40 + 6 * es),
     "fused_heavy": lambda p, es: 0,
+    # u,v,w,T + 9 metrics + J in, 12 viscous fluxes out, per cell of the interior+-2 box
+    "fused_viscous": lambda p, es: (p.Nxp + 4) * (p.Nyp + 4) * (p.Nzp + 4) * (13 * es + 8 + 12 * es),
 }
+
+
+# HBM bytes per launch actually moved, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
+# default workload (ramp 512x192x64, FP64), each scaled by what the counter reads for a 1 GiB device
+# copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); average over the three stage variants of the kernel.
+MEASURED_TRAFFIC = {("fused_update", "512x192x64", "f64"): (1839.9e6 * 2 + 1513.4e6 + 553.6e6 * 2 + 805.3e6) / 3}
 
 
 def build_case(f3d, cases, torch, nx, ny, nz, seed=2):
@@ -159,6 +167,122 @@ def cpu_baseline(f3d, cases, nx, ny, nz, steps=3):
                 kind="port", sample=f"oracle FP64, {steps} RK3 steps of the {nx}x{ny}x{nz} ramp ({el:.1f} s)")
 
 
+def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True):
+    """Inflow-parameter adjoint of the ramp: theta = (M_inf, T_wall, 16 Fourier amplitudes),
+    objective = sum over `nsteps` FP64 RK3 steps of dt * sum_wall p * |grad eta|/J (wall pressure
+    x wall-face area on the first cell row), checkpoint every RK step.  Times one forward step and
+    the adjoint (forward sweep + reverse with recomputation), chains the boundary-data adjoints to
+    d objective / d theta with complex-step derivatives of the boundary data, and checks three
+    components against GPU central finite differences."""
+    import numpy as np
+    NG = 4
+    c = cases.ramp(nx, ny, nz)
+    st = c["state"]
+    ygh = c["y"][:, :, 0:NG].mean(axis=2)
+    th = cases.theta0()
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+    def setup(theta):
+        inflow, qinf, Tw = cases.ramp_theta(ygh, st, theta)
+        kw = dict(c["params"])
+        kw.update(q_inf=list(qinf), T_wall=float(Tw))
+        return f3d.make_params(nx, ny, nz, **kw), d(inflow)
+
+    p, inflow = setup(th)
+    ctx = f3d.Flame3D(p)
+    m = [torch.zeros(p.cell_shape(), dtype=torch.float64, device="cuda:0") for _ in range(9)]
+    J = torch.zeros(p.cell_shape(), dtype=torch.float64, device="cuda:0")
+    ctx.metrics(d(c["x"]), d(c["y"]), d(c["z"]), m, J)
+    Q0 = d(c["Q"])
+    U0 = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
+    ctx.exchange_ghost(Q0, 6)
+    ctx.fillGhost(Q0, U0, inflow)
+    ctx.prim2c(U0, Q0)
+    dt_t = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+    ctx.cfl_dt(dt_t, Q0, m, J, 0.5)
+    dt = float(dt_t.item())
+    W = torch.zeros(p.cell_shape(6), dtype=torch.float64, device="cuda:0")
+    area = torch.sqrt(m[3] ** 2 + m[4] ** 2 + m[5] ** 2)
+    W[4, NG:-NG, NG, NG:-NG] = area[NG:-NG, NG, NG:-NG]
+
+    def objective(params, inflow_t):
+        cx = f3d.Flame3D(params)
+        U = U0.clone()
+        Un = torch.zeros_like(U)
+        Q = torch.zeros(params.cell_shape(6), dtype=torch.float64, device="cuda:0")
+        cx.c2Prim(U, Q)
+        cx.exchange_ghost(Q, 6)
+        cx.fillGhost(Q, None, inflow_t)
+        obj = torch.zeros((), dtype=torch.float64, device="cuda:0")
+        for _ in range(nsteps):
+            cx.time_step(1, U, Un, Q, m, J, dt, inflow_t)
+            obj += dt * (W * Q).sum()
+        return float(obj.item())
+
+    # forward step time on this grid
+    U, Un, Q = U0.clone(), torch.zeros_like(U0), torch.zeros(p.cell_shape(6), dtype=torch.float64, device="cuda:0")
+    ctx.c2Prim(U, Q)
+    ctx.exchange_ghost(Q, 6)
+    ctx.fillGhost(Q, None, inflow)
+    ms_fwd, _ = run_steps(torch, ctx, (U, Un, Q, m, J, inflow, dt), "f64", nsteps, 2)
+    # adjoint (one short call first: workspace creation)
+    obj = torch.zeros(nsteps, dtype=torch.float64, device="cuda:0")
+    lamU0 = torch.zeros_like(U0)
+    lam_in = torch.zeros_like(inflow)
+    lam_bc = torch.zeros(7, dtype=torch.float64, device="cuda:0")
+    ckpt = torch.empty(ctx.adjoint_ckpt_bytes(nsteps, 1) // 8, dtype=torch.float64, device="cuda:0")
+    ctx.adjoint(1, 1, U0, m, J, dt, W, obj, lamU0, inflow, lam_in, lam_bc, ckpt=ckpt)
+    lam_in.zero_()
+    lam_bc.zero_()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    ctx.adjoint(nsteps, 1, U0, m, J, dt, W, obj, lamU0, inflow, lam_in, lam_bc, ckpt=ckpt)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms_adj = ev0.elapsed_time(ev1)
+    # per-kernel time of the reverse sweep (separate run, hipEvent-bracketed launches)
+    lib.f3d_timing_reset()
+    lib.f3d_timing_enable(1)
+    obj2 = torch.zeros_like(obj)
+    ctx.adjoint(nsteps, 1, U0, m, J, dt, W, obj2, torch.zeros_like(U0), inflow, torch.zeros_like(inflow),
+                torch.zeros_like(lam_bc), ckpt=ckpt)
+    torch.cuda.synchronize()
+    lib.f3d_timing_enable(0)
+    ktab = {k: v for k, v in kernel_table(lib).items() if k.startswith("adj_")}
+    # chain to theta
+    jac = cases.ramp_theta_jacobian(ygh, st, th)
+    li, lb = lam_in.cpu().numpy(), lam_bc.cpu().numpy()
+    grad = np.array([float(np.sum(li * dinf) + np.sum(lb[:6] * dq) + lb[6] * dTw) for dinf, dq, dTw in jac])
+    J0 = float(obj.sum().item())
+    fd_out = []
+    if fd:
+        # repeatability of the objective itself (heavy-face corrections add with atomics)
+        j_a, j_b = objective(*setup(th)), objective(*setup(th))
+        for k, name in ((0, "M_inf"), (1, "T_wall"), (2, "a_1")):
+            h = 1e-4 * abs(th[k]) if k < 2 else 1e-2  # amplitudes enter almost linearly
+            tp, tm = th.copy(), th.copy()
+            tp[k] += h
+            tm[k] -= h
+            jp = objective(*setup(tp))
+            jm = objective(*setup(tm))
+            fdv = (jp - jm) / (2 * h)
+            fd_out.append(dict(param=name, adjoint=grad[k], central_fd=fdv, step=h,
+                               rel_diff=abs(fdv - grad[k]) / max(abs(fdv), 1e-300),
+                               fd_noise_rel=abs(j_a - j_b) / max(abs(jp - jm), 1e-300)))
+    rev = (ms_adj - nsteps * ms_fwd) / nsteps
+    return dict(
+        config=f"ramp {nx}x{ny}x{nz} FP64, {nsteps} RK3 steps, checkpoint every step; theta = (M_inf, T_wall, "
+               f"16 inflow Fourier amplitudes); objective = sum_n dt * wall pressure x wall-face area",
+        ms_forward_step=round(ms_fwd, 4), ms_adjoint_total=round(ms_adj, 3),
+        ms_reverse_step=round(rev, 4), adjoint_over_forward_step=round(rev / ms_fwd, 3),
+        objective=J0, objective_check_rel=abs(J0 - float(obj2.sum().item())) / abs(J0),
+        grad_theta=[float(g) for g in grad], fd_check=fd_out,
+        kernels={k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1], total_ms=round(v[0], 3))
+                 for k, v in sorted(ktab.items())},
+    )
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +292,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32b", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-grid", default="256x128x64")
+    ap.add_argument("--adjoint-steps", type=int, default=50)
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint leg (profiling)")
+    ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the adjoint")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -191,6 +320,11 @@ def main():
     lib = f3d.lib()
     nx, ny, nz = map(int, args.grid.split("x"))
     ncells = nx * ny * nz
+    if args.adjoint_only:
+        ax, ay, az = map(int, args.adjoint_grid.split("x"))
+        print(json.dumps({"adjoint": adjoint_bench(f3d, cases, torch, lib, ax, ay, az, args.adjoint_steps,
+                                                   fd=not args.no_fd)}))
+        return
 
     st = build_case(f3d, cases, torch, nx, ny, nz)
     p, ctx, dt = st["p"], st["ctx"], st["dt"]
@@ -236,12 +370,17 @@ def main():
         avg_s = tot_ms / n * 1e-3
         alg = kernel_alg_bytes(name, p, 8)
         achieved = alg / avg_s / 1e9
+        traffic = MEASURED_TRAFFIC.get((name, args.grid, "f64"))
         roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=name,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
                     avg_launch_us=round(avg_s * 1e6, 2), alg_bytes_per_launch=alg,
                     share_of_step=round(tot_ms / (ms64 * args.steps), 4))
     kshare = {k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1],
                       share=round(v[0] / (ms64 * args.steps), 4)) for k, v in sorted(ktab.items())}
+    adj = None
+    if not args.no_adjoint:
+        ax, ay, az = map(int, args.adjoint_grid.split("x"))
+        adj = adjoint_bench(f3d, cases, torch, lib, ax, ay, az, args.adjoint_steps, fd=not args.no_fd)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(f3d, cases, nx, ny, nz, steps=args.cpu_steps)
@@ -264,6 +403,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp32_bulk": out32 or None,
+        "adjoint": adj,
         "kernels": kshare,
         "host_wall_ms_per_step": round(wall64, 4),
     }

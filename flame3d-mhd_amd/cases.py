@@ -195,3 +195,57 @@ def inflow_profile(ygh, st, noise=0.01, seed=2, mach=None, T_wall=None, fourier=
             v = v + a * st["u"] * np.sin(2 * math.pi * m * kz / max(Nzt - 2 * NG, 1)) * (u / st["u"])
     p = np.full_like(u, st["p"])
     return _prim_from(rho, u, v, np.zeros_like(u), p)
+
+
+# ---------------------------------------------------------------- inflow-parameter adjoint case
+N_FOURIER = 16
+
+
+def theta0(seed=4, amp=2e-3):
+    """Inflow parameters theta = (M_inf, T_wall, a_1..a_16) of the adjoint case: the ramp's
+    freestream Mach number and wall temperature, and 16 seeded amplitudes of a spanwise Fourier
+    perturbation of the inflow v (relative to u_inf)."""
+    st = ramp_state()
+    rng = np.random.default_rng(seed)
+    return np.concatenate([[st["mach"], st["Tw"]], amp * (2 * rng.random(N_FOURIER) - 1)])
+
+
+def ramp_theta(ygh, st, theta, noise=0.01, seed=2):
+    """Boundary data of the ramp as a function of theta (complex-step safe: complex theta gives
+    complex outputs).  The freestream thermodynamic state (rho, p, T) stays fixed; M_inf scales
+    the inflow and farfield velocity, T_wall sets the isothermal wall and the Crocco-Busemann
+    inflow temperature, a_m add a_m u_inf sin(2 pi m k/Nz) (u/u_inf) to the inflow v.
+    Returns (inflow [6, Nz_tot, Ny_tot], q_inf [6], T_wall)."""
+    mach, Tw, amps = theta[0], theta[1], theta[2:]
+    e = np.clip(np.maximum(ygh, 0.0) / st["delta"], 0.0, 1.0)
+    f = 2 * e - 2 * e ** 3 + e ** 4
+    Tinf = st["T"]
+    Tr = Tinf * (1 + math.sqrt(PR) * (GAMMA - 1) / 2 * mach ** 2)
+    T = Tw + (Tr - Tw) * f + (Tinf - Tr) * f ** 2
+    uinf = st["u"] * mach / st["mach"]
+    u = uinf * f
+    rho = st["p"] / (RG * T)
+    Nzt, Nyt = ygh.shape
+    rng = np.random.default_rng(seed)
+    v = np.zeros(u.shape, dtype=u.dtype)
+    if noise:
+        u = u * (1 + noise * (2 * rng.random(u.shape) - 1))
+        v = v + noise * st["u"] * (2 * rng.random(u.shape) - 1)
+    kz = np.arange(Nzt)[:, None] - NG
+    for m, a in enumerate(amps, start=1):
+        v = v + a * uinf * np.sin(2 * math.pi * m * kz / max(Nzt - 2 * NG, 1)) * (u / uinf)
+    p = np.full(u.shape, st["p"], dtype=u.dtype)
+    inflow = np.stack([rho, u, v, np.zeros(u.shape, dtype=u.dtype), p, p / (rho * RG)])
+    q_inf = np.array([st["rho"], uinf, 0.0, 0.0, st["p"], st["T"]], dtype=np.asarray(uinf).dtype)
+    return inflow, q_inf, Tw
+
+
+def ramp_theta_jacobian(ygh, st, theta, noise=0.01, seed=2, h=1e-30):
+    """d(inflow, q_inf, T_wall)/d theta_k for every k by complex steps (exact to rounding)."""
+    out = []
+    for k in range(len(theta)):
+        th = np.array(theta, dtype=np.complex128)
+        th[k] += 1j * h
+        inflow, q_inf, Tw = ramp_theta(ygh, st, th, noise=noise, seed=seed)
+        out.append((inflow.imag / h, q_inf.imag / h, float(np.imag(Tw)) / h))
+    return out

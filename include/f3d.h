@@ -264,8 +264,10 @@ int f3d_metrics_f64(const f3d_params* p, const double* x, const double* y, const
 /* ---------------- adjoint (reverse mode, FP64; no reference counterpart) ---------------------
  * Hand-written vector-Jacobian products of the path above with the scheme mask frozen at the
  * primal state (the sensor, face schemes, minmod selections and tangent-vector branch are not
- * differentiated; |x| uses the subgradient sign(0) = 0).  They back a ChainRulesCore.rrule of
- * the RHS on the Julia side.  The adjoint workspace holds its own FP64 forward workspace. */
+ * differentiated; |x| uses the subgradient sign(0) = 0; a WENO smoothness indicator below the
+ * rounding floor of its own evaluation -- a stencil flat to the last bit -- passes no weight
+ * derivative).  They back a ChainRulesCore.rrule of the RHS on the Julia side.  The adjoint
+ * workspace holds its own FP64 forward workspace. */
 typedef struct f3d_adj_workspace f3d_adj_workspace;
 int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace** aws);
 int f3d_adjoint_workspace_destroy(f3d_adj_workspace* aws);

@@ -101,3 +101,20 @@ def test_shapes_follow_julia_column_major():
     assert p.face_shape(0) == (5, 5, 7, 11) and p.face_shape(2) == (5, 6, 7, 10)
     assert p.fv_shape() == (4, 9, 11, 14)
     assert p.mask_shape(1) == (5, 8, 10)
+
+
+def test_adjoint_entry_points_validate_before_any_launch():
+    p = f3d.make_params(16, 12, 8)
+    L = f3d.lib()
+    # NULL adjoint workspace and NULL arrays are errors, not crashes (no GPU call is made)
+    assert L.f3d_rhs_vjp_f64(ctypes.byref(p), None, None, None, None, None, None, None) == -1
+    assert L.f3d_rk3_stage_vjp_f64(ctypes.byref(p), None, 1, None, None, None, None, None, None, None,
+                                   ctypes.c_double(1.0), None, None, None) == -1
+    assert L.f3d_ghost_vjp_f64(ctypes.byref(p), None, None, None, None, None) == -1
+    assert L.f3d_c2prim_vjp_f64(ctypes.byref(p), None, None, None, None) == -1
+    assert b"NULL" in L.f3d_last_error()
+    # checkpoint buffer size: (ceil(n/every) + every) states of 5 variables
+    nc = L.f3d_cell_count(ctypes.byref(p))
+    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 1) == (50 + 1) * 5 * nc * 8
+    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 10) == (5 + 10) * 5 * nc * 8
+    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 0, 1) == 0
