@@ -105,15 +105,30 @@ def build_case(f3d, cases, torch, nx, ny, nz, seed=2):
     return dict(c=c, p=p, ctx=ctx, m=m, J=J, Q=Q, U=U, inflow=inflow, dt=float(dt_t.item()))
 
 
-def run_steps(torch, ctx, state, prec, steps, warmup, timing_lib=None):
-    """Warm up, then time `steps` RK3 steps between two synchronisations.  Returns ms/step."""
+def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3):
+    """Warm up; run `table_steps` steps with every library launch bracketed by hipEvents (per-kernel
+    table); then time `steps` RK3 steps between two synchronisations with events around the
+    launches of the dominant kernel only (its live average launch time).  Returns
+    (ms/step, host wall ms/step, kernel table, (dominant name, its total ms, its launches))."""
     U, Un, Q, m, J, inflow, dt = state
     for _ in range(warmup):
         ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
     torch.cuda.synchronize()
-    if timing_lib is not None:
-        timing_lib.f3d_timing_reset()
-        timing_lib.f3d_timing_enable(1)
+    table, dom = {}, None
+    if lib is not None and table_steps:
+        lib.f3d_timing_only(None)
+        lib.f3d_timing_reset()
+        lib.f3d_timing_enable(1)
+        for _ in range(table_steps):
+            ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
+        torch.cuda.synchronize()
+        lib.f3d_timing_enable(0)
+        table = kernel_table(lib)
+        if table:
+            dom = max(table.items(), key=lambda kv: kv[1][0])[0]
+            lib.f3d_timing_only(dom.encode())
+            lib.f3d_timing_reset()
+            lib.f3d_timing_enable(1)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -124,9 +139,12 @@ def run_steps(torch, ctx, state, prec, steps, warmup, timing_lib=None):
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    if timing_lib is not None:
-        timing_lib.f3d_timing_enable(0)
-    return ev0.elapsed_time(ev1) / steps, wall / steps * 1e3
+    live = None
+    if dom is not None:
+        lib.f3d_timing_enable(0)
+        live = (dom,) + kernel_table(lib)[dom]
+        lib.f3d_timing_only(None)
+    return ev0.elapsed_time(ev1) / steps, wall / steps * 1e3, table, live
 
 
 def kernel_table(lib):
@@ -224,7 +242,7 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True):
     ctx.c2Prim(U, Q)
     ctx.exchange_ghost(Q, 6)
     ctx.fillGhost(Q, None, inflow)
-    ms_fwd, _ = run_steps(torch, ctx, (U, Un, Q, m, J, inflow, dt), "f64", nsteps, 2)
+    ms_fwd, _, _, _ = run_steps(torch, ctx, (U, Un, Q, m, J, inflow, dt), nsteps, 2)
     # adjoint (one short call first: workspace creation)
     obj = torch.zeros(nsteps, dtype=torch.float64, device="cuda:0")
     lamU0 = torch.zeros_like(U0)
@@ -332,9 +350,8 @@ def main():
     Q0 = st["Q"].clone()
     # ---- FP64
     U, Un, Q = st["U"], torch.zeros_like(st["U"]), st["Q"]
-    ms64, wall64 = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], dt), "f64", args.steps,
-                             args.warmup, timing_lib=lib)
-    ktab = kernel_table(lib)
+    ms64, wall64, ktab, live = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], dt), args.steps,
+                                         args.warmup, lib=lib)
     ctx.nan_check(U)
     value64 = ncells / (ms64 * 1e-3) / 1e6
     rho64 = U[0].clone()
@@ -344,7 +361,8 @@ def main():
         m32 = [a.float().contiguous() for a in st["m"]]
         U32, Un32, Q32 = U0.clone(), torch.zeros_like(U0), Q0.float().contiguous()
         in32 = st["inflow"].float().contiguous()
-        ms32, _ = run_steps(torch, ctx, (U32, Un32, Q32, m32, st["J"], in32, dt), "f32b", args.steps, args.warmup)
+        ms32, _, ktab32, _ = run_steps(torch, ctx, (U32, Un32, Q32, m32, st["J"], in32, dt), args.steps, args.warmup,
+                                       lib=lib)
         ctx.nan_check(U32)
         value32 = ncells / (ms32 * 1e-3) / 1e6
         NG = 4
@@ -360,13 +378,14 @@ def main():
             rho_Linf_rel=float((r32 - r64).abs().max() / r64.abs().max()),
             pwall_L2_rel=float(torch.linalg.norm(pw32 - pw64) / torch.linalg.norm(pw64)),
             pwall_Linf_rel=float((pw32 - pw64).abs().max() / pw64.abs().max()),
-            steps_compared=args.steps + args.warmup,
+            steps_compared=args.steps + args.warmup + 3,
+            kernels={k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1],
+                             share=round(v[0] / (ms32 * 3), 4)) for k, v in sorted(ktab32.items())},
         )
     # ---- roofline of the dominant kernel (hipEvent-timed launches inside the FP64 timed region)
-    dom = max(ktab.items(), key=lambda kv: kv[1][0]) if ktab else None
     roof = None
-    if dom:
-        name, (tot_ms, n) = dom
+    if live:
+        name, tot_ms, n = live
         avg_s = tot_ms / n * 1e-3
         alg = kernel_alg_bytes(name, p, 8)
         achieved = alg / avg_s / 1e9
@@ -376,7 +395,7 @@ def main():
                     avg_launch_us=round(avg_s * 1e6, 2), alg_bytes_per_launch=alg,
                     share_of_step=round(tot_ms / (ms64 * args.steps), 4))
     kshare = {k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1],
-                      share=round(v[0] / (ms64 * args.steps), 4)) for k, v in sorted(ktab.items())}
+                      share=round(v[0] / (ms64 * 3), 4)) for k, v in sorted(ktab.items())}
     adj = None
     if not args.no_adjoint:
         ax, ay, az = map(int, args.adjoint_grid.split("x"))

@@ -22,6 +22,15 @@
 
 namespace f3d {
 
+int xcd_remap() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("F3D_XCD_REMAP");
+    v = (e && std::strcmp(e, "0") == 0) ? 0 : 1;
+  }
+  return v;
+}
+
 bool use_fused() {
   static int v = -1;
   if (v < 0) {
@@ -275,12 +284,13 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a) {
 // layout (viscous.jl:119-132).
 template <typename T>
 __global__ __launch_bounds__(256) void k_viscous_march(KP kp, FArgs<T> a, T* __restrict__ Fvx, T* __restrict__ Fvy,
-                                                       T* __restrict__ Fvz, int KC) {
-  const int ii = blockIdx.x * 64 + threadIdx.x, jj = blockIdx.y * 4 + threadIdx.y;  // 0.. Np+3
+                                                       T* __restrict__ Fvz, int KC, int remap) {
+  const Blk B = xcd_block(remap);
+  const int ii = B.x * 64 + threadIdx.x, jj = B.y * 4 + threadIdx.y;  // 0.. Np+3
   const int NX = kp.Nxp + 4, NY = kp.Nyp + 4, NZ = kp.Nzp + 4;
   const bool act = ii < NX && jj < NY;
   const int i = 2 + min(ii, NX - 1), j = 2 + min(jj, NY - 1);  // absolute, in [2, Nt-3]
-  const int kk0 = blockIdx.z * KC;
+  const int kk0 = B.z * KC;
   const int nk = min(KC, NZ - kk0);
   const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
   const long long vn = (long long)NX * NY * NZ;
@@ -346,11 +356,12 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __restrict__ Fvx,
                                                 const T* __restrict__ Fvy, const T* __restrict__ Fvz,
                                                 double* __restrict__ U, double* __restrict__ Un, T* __restrict__ Q,
-                                                double* __restrict__ Rout, double dt, int KC) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+                                                double* __restrict__ Rout, double dt, int KC, int remap) {
+  const Blk B = xcd_block(remap);
+  const int i = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y;
   const bool act = i < kp.Nxp && j < kp.Nyp;
   const int ic = min(i, kp.Nxp - 1), jc = min(j, kp.Nyp - 1);
-  const int k0 = blockIdx.z * KC;
+  const int k0 = B.z * KC;
   const int nk = min(KC, kp.Nzp - k0);
   const int NG = kp.NG;
   const long long nc = kp.nc, sz = (long long)kp.Nxt * kp.Nyt;
@@ -434,6 +445,20 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
 }
 
 // ------------------------------------------------------------------ orchestration
+// Segment length of the y/z marches; F3D_MARCH_SEG_Y / F3D_MARCH_SEG_Z override the defaults.
+static int march_seg(int D, int es) {
+  static int v[2][2] = {{-1, -1}, {-1, -1}};
+  int& r = v[D - 1][es == 8];
+  if (r < 0) {
+    const char* e = std::getenv(D == 1 ? "F3D_MARCH_SEG_Y" : "F3D_MARCH_SEG_Z");
+    // defaults measured on the 512x192x64 ramp (one MI355X): FP64 y 48, z 16; FP32 y 24, z 32
+    const int def = es == 8 ? (D == 1 ? 48 : 16) : (D == 1 ? 24 : 32);
+    r = e ? std::atoi(e) : def;
+    if (r < 8) r = def;
+  }
+  return r;
+}
+
 static int seg_len(int Np, int max_len) {
   const int nseg = (Np + max_len - 1) / max_len;
   return (Np + nseg - 1) / nseg;
@@ -456,13 +481,13 @@ static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const Metri
   }
   {
     TimedScope ts("fused_sweep_y", s);
-    const int L = seg_len(kp.Nyp, 32);
+    const int L = seg_len(kp.Nyp, march_seg(1, (int)sizeof(T)));
     hipLaunchKernelGGL((k_sweep_march<T, 1>), dim3((kp.Nxp + 63) / 64, (kp.Nzp + 3) / 4, (kp.Nyp + L - 1) / L),
                        dim3(64, 4, 1), 0, s, kp, a, L);
   }
   {
     TimedScope ts("fused_sweep_z", s);
-    const int L = seg_len(kp.Nzp, 32);
+    const int L = seg_len(kp.Nzp, march_seg(2, (int)sizeof(T)));
     hipLaunchKernelGGL((k_sweep_march<T, 2>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + L - 1) / L),
                        dim3(64, 4, 1), 0, s, kp, a, L);
   }
@@ -474,7 +499,7 @@ static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const Metri
     TimedScope ts("fused_viscous", s);
     const int KC = seg_len(kp.Nzp + 4, 32);
     hipLaunchKernelGGL(k_viscous_march<T>, dim3((kp.Nxp + 4 + 63) / 64, (kp.Nyp + 4 + 3) / 4, (kp.Nzp + 4 + KC - 1) / KC),
-                       dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC);
+                       dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC, xcd_remap());
   }
   return check_launch("fused sweeps");
 }
@@ -488,7 +513,8 @@ int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>
   TimedScope ts("fused_update", s);
   const int KC = seg_len(kp.Nzp, 16);
   hipLaunchKernelGGL((k_update<T, 0>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC),
-                     dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, nullptr, nullptr, nullptr, R, 0.0, KC);
+                     dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, nullptr, nullptr, nullptr, R, 0.0, KC,
+                     xcd_remap());
   return check_launch("fused rhs update");
 }
 
@@ -501,12 +527,13 @@ int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* U
   const int KC = seg_len(kp.Nzp, 16);
   const dim3 g((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC), b(64, 4, 1);
   TimedScope ts("fused_update", s);
+  const int rm = xcd_remap();
   if (KRK == 1)
-    hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC);
+    hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC, rm);
   else if (KRK == 2)
-    hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC);
+    hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC, rm);
   else
-    hipLaunchKernelGGL((k_update<T, 3>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC);
+    hipLaunchKernelGGL((k_update<T, 3>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC, rm);
   return check_launch("fused stage update");
 }
 

@@ -17,6 +17,25 @@ struct WsView {
   unsigned long long hcap;   // capacity of `heavy` (total face count of the three directions)
 };
 
+// Block coordinates with an XCD-aware order.  Workgroups are dispatched round-robin over the 8 XCDs
+// (XCD = linear workgroup id mod 8) and each XCD has its own L2; giving every XCD one contiguous
+// range of the grid (x fastest) keeps the neighbouring blocks whose halo rows a block re-reads in
+// the same L2.  remap = 0 returns the plain blockIdx.
+struct Blk {
+  int x, y, z;
+};
+__device__ __forceinline__ Blk xcd_block(int remap) {
+  if (!remap) return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
+  const unsigned gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
+  const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned xcd = id & 7u, r = id >> 3, q = nb >> 3, rem = nb & 7u;
+  const unsigned L = xcd * q + (xcd < rem ? xcd : rem) + r;
+  return Blk{(int)(L % gx), (int)((L / gx) % gy), (int)(L / (gx * gy))};
+}
+
+// 1 unless F3D_XCD_REMAP=0 (A/B measurement of the block order).
+int xcd_remap();
+
 // Wave-aggregated append to the compacted heavy-face list: one atomic per wave.
 __device__ __forceinline__ void append_heavy(unsigned long long* list, unsigned int* count, bool pred,
                                              unsigned long long entry) {

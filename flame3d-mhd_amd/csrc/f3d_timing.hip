@@ -13,6 +13,7 @@ struct Rec {
   hipStream_t s;
 };
 bool g_on = false;
+std::string g_only;        // when non-empty, only launches of this name are bracketed
 std::vector<Rec> g_recs;   // recorded brackets since the last reset
 std::vector<Rec> g_pool;   // event pairs available for reuse
 size_t g_used = 0;         // brackets in use (prefix of g_recs)
@@ -20,6 +21,7 @@ size_t g_used = 0;         // brackets in use (prefix of g_recs)
 
 TimedScope::TimedScope(const char* name, hipStream_t s) : slot_(-1), s_(s) {
   if (!g_on) return;
+  if (!g_only.empty() && g_only != name) return;
   if (g_used == g_recs.size()) {
     Rec r{name, nullptr, nullptr, s};
     if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
@@ -42,6 +44,11 @@ using namespace f3d;
 
 F3D_API int f3d_timing_enable(int32_t on) {
   g_on = on != 0;
+  return F3D_OK;
+}
+
+F3D_API int f3d_timing_only(const char* name) {
+  g_only = name ? name : "";
   return F3D_OK;
 }
 

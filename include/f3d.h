@@ -320,6 +320,9 @@ int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t nsteps,
  * When enabled, every kernel the library launches is bracketed by a pair of hipEvents recorded on
  * the launching stream; queries synchronise on those events.  Process-global, not thread-safe. */
 int f3d_timing_enable(int32_t on);
+/* Restrict the bracketing to the launches named `name` (NULL: every launch), so that a timed
+ * region can carry the events of one kernel only. */
+int f3d_timing_only(const char* name);
 int f3d_timing_reset(void);
 /* Sum of elapsed ms and number of launches of the kernels whose name equals `name`. */
 int f3d_timing_query(const char* name, double* total_ms, int64_t* launches);
