@@ -70,8 +70,9 @@ FUSED_BYTES = {
     # as x, but the RHS is read and written
     "fused_sweep_y": lambda p, es: _nint(p) * (9 * es + 10 * es),
     "fused_sweep_z": lambda p, es: _nint(p) * (9 * es + 10 * es),
-    # RHS 5 + viscous fluxes 12 + J in; U, Un in; U and Q out (stage-2/3 form)
-    "fused_update": lambda p, es: _nint(p) * (5 * es + 12 * es + 8 + 40 + 40 + 40 + 6 * es),
+    # viscous divergence fused into the update march: RHS 5 + u,v,w,T 4 + 9 metrics + J in, U and Un in,
+    # U and Q out (stage-2/3 form), per interior cell
+    "fused_update": lambda p, es: _nint(p) * (5 * es + 4 * es + 9 * es + 8 + 40 + 40 + 40 + 6 * es),
     "fused_heavy": lambda p, es: 0,
     # u,v,w,T + 9 metrics + J in, 12 viscous fluxes out, per cell of the interior+-2 box
     "fused_viscous": lambda p, es: (p.Nxp + 4) * (p.Nyp + 4) * (p.Nzp + 4) * (13 * es + 8 + 12 * es),
@@ -81,7 +82,7 @@ FUSED_BYTES = {
 # HBM bytes per launch actually moved, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # default workload (ramp 512x192x64, FP64), each scaled by what the counter reads for a 1 GiB device
 # copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); average over the three stage variants of the kernel.
-MEASURED_TRAFFIC = {("fused_update", "512x192x64", "f64"): (1839.9e6 * 2 + 1513.4e6 + 553.6e6 * 2 + 805.3e6) / 3}
+MEASURED_TRAFFIC = {}
 
 
 def build_case(f3d, cases, torch, nx, ny, nz, seed=2):

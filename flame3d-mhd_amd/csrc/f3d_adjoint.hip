@@ -762,7 +762,7 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
     return F3D_E_ARG;
   }
   const KP kp = make_kp(p);
-  const size_t nc = kp.nc, vb = 5 * nc * sizeof(double), qb = 6 * nc * sizeof(double);
+  const size_t nc = kp.nc, vb = 5 * nc * sizeof(double);
   const int nck = (nsteps + every - 1) / every;
   auto slot = [&](size_t q) { return ckpt + q * 5 * nc; };
   double* seg0 = slot(nck);                 // every-1 recomputed states of a segment
@@ -779,10 +779,7 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
     return r;
   };
   auto step = [&](double* U, double* Q) {
-    int r = F3D_OK;
-    for (int KRK = 1; KRK <= 3 && !r; ++KRK)
-      r = f3d_rk3_stage_f64(p, aws->fws, KRK, U, aws->Un, Q, metrics, J, dt, inflow, s);
-    return r;
+    return steps_io<double>(p, aws->fws, 1, U, aws->Un, Q, metrics, J, dt, inflow, s);
   };
   // ---- forward sweep: checkpoints every `every` steps, objective per step
   F3D_HIP(hipMemcpyAsync(aws->Ucur, U0, vb, hipMemcpyDeviceToDevice, s));
@@ -819,8 +816,8 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
       rc = prim(aws->Us[0], aws->Qs[0]);
       for (int q = 1; q <= 2 && !rc; ++q) {
         F3D_HIP(hipMemcpyAsync(aws->Us[q], aws->Us[q - 1], vb, hipMemcpyDeviceToDevice, s));
-        F3D_HIP(hipMemcpyAsync(aws->Qs[q], aws->Qs[q - 1], qb, hipMemcpyDeviceToDevice, s));
-        rc = f3d_rk3_stage_f64(p, aws->fws, q, aws->Us[q], aws->Un, aws->Qs[q], metrics, J, dt, inflow, s);
+        rc = stage_io<double>(p, aws->fws, q, aws->Us[q], aws->Un, aws->Qs[q - 1], aws->Qs[q], metrics, J, dt, inflow,
+                              s);
       }
       for (int KRK = 3; KRK >= 1 && !rc; --KRK)
         rc = stage_vjp_impl(p, aws, KRK, aws->lam, aws->lam, aws->lamUn, aws->Us[KRK - 1], aws->Qs[KRK - 1], metrics,

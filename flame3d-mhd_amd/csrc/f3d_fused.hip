@@ -31,6 +31,17 @@ int xcd_remap() {
   return v;
 }
 
+// 1 unless F3D_VISC_FUSED=0: the viscous divergence computed inside the update march (k_visc_update)
+// instead of k_viscous_march + k_update (A/B measurement).
+static int visc_fused() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("F3D_VISC_FUSED");
+    v = (e && std::strcmp(e, "0") == 0) ? 0 : 1;
+  }
+  return v;
+}
+
 bool use_fused() {
   static int v = -1;
   if (v < 0) {
@@ -444,6 +455,310 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
   }
 }
 
+// ------------------------------------------------------------------ viscous divergence + update
+// k_update with the viscous divergence computed in place: the viscous fluxes never go to HBM.
+// Block = 64 x 4 lanes: 4 interior rows of a 60-column tile plus 2 halo lanes each side, marching
+// k.  Each iteration, every lane evaluates the velocity/temperature gradient and viscous_cell at
+// plane p (2 ahead of the updated plane k) for its own cell and for one y-halo cell (rows -2, -1
+// for lanes of rows 0, 1; rows 4, 5 for rows 2, 3; Fv_y only).  Fv_x of the interior rows and
+// Fv_y of the interior columns are exchanged through LDS for the x/y divergence of plane p, kept
+// two iterations in a register ring; Fv_z enters a 5-plane register window for the z divergence
+// of plane k; u,v,w,T of the own column sit in a 5-plane window for the zeta derivatives (the halo
+// cell's come through L2).  Every wave updates cells, so the update's HBM streams keep all waves.
+// Summation order of the divergence as k_update.
+constexpr int VX = 64, VXI = 60, VYI = 4;
+
+template <typename T>
+__device__ __forceinline__ void visc_eval(const KP& kp, const FArgs<T>& a, long long o, const T* wz, int wzs,
+                                          long long sy, long long sz, const T (*wcol)[5], T out[3][4]) {
+  // wcol != nullptr: zeta derivatives from the register window; else from memory around o
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  const long long nc = kp.nc;
+  const int var[4] = {1, 2, 3, 5};
+  T g[4][3], val[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const T* f = a.Q + var[v] * nc + o;
+    g[v][0] = c1 * (f[-2] - f[2]) + c2 * (f[-1] - f[1]);
+    g[v][1] = c1 * (f[-2 * sy] - f[2 * sy]) + c2 * (f[-sy] - f[sy]);
+    if (wcol) {
+      g[v][2] = c1 * (wcol[v][0] - wcol[v][4]) + c2 * (wcol[v][1] - wcol[v][3]);
+      val[v] = wcol[v][2];
+    } else {
+      g[v][2] = c1 * (f[-2 * sz] - f[2 * sz]) + c2 * (f[-sz] - f[sz]);
+      val[v] = f[0];
+    }
+  }
+  T m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = a.m.m[q][o];
+  viscous_cell<T>(kp, val[0], val[1], val[2], val[3], g, m, (T)a.J[o], out);
+}
+
+// Conservative update + RK3 blend + c2Prim of one interior cell (utils.jl:2-70, div.jl:40-48);
+// MODE 0 writes the RHS instead.
+template <typename T, int MODE>
+__device__ __forceinline__ void update_cell(const KP& kp, const T* Rt, long long oc, double* __restrict__ U,
+                                            double* __restrict__ Un, T* __restrict__ Q, double* __restrict__ Rout,
+                                            double Jc, double dt) {
+  const long long nc = kp.nc;
+  if (MODE == 0) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Rout[oc + n * nc] = (double)Rt[n] * Jc;
+    return;
+  }
+  const double Jact = Jc * dt;
+  double un[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const double u0 = U[oc + n * nc];
+    const double u1 = u0 + (double)Rt[n] * Jact;
+    double r;
+    if (MODE == 1) {
+      Un[oc + n * nc] = u0;
+      r = u1;
+    } else if (MODE == 2) {
+      r = u1 * 0.25 + Un[oc + n * nc] * 0.75;
+    } else {
+      r = u1 * (2.0 / 3.0) + Un[oc + n * nc] * (1.0 / 3.0);
+    }
+    un[n] = r;
+    U[oc + n * nc] = r;
+  }
+  const double eps = 2.220446049250313e-16;
+  const double rho = jmax(un[0], eps);
+  const double ri = 1.0 / rho;
+  const double u = un[1] * ri, v = un[2] * ri, ww = un[3] * ri;
+  const double ei = jmax(un[4] - 0.5 * rho * (u * u + v * v + ww * ww), eps);
+  const double pr = (kp.gamma - 1) * ei;
+  Q[oc] = (T)rho;
+  Q[oc + nc] = (T)u;
+  Q[oc + 2 * nc] = (T)v;
+  Q[oc + 3 * nc] = (T)ww;
+  Q[oc + 4 * nc] = (T)pr;
+  Q[oc + 5 * nc] = (T)(pr / (rho * kp.Rg));
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_visc_update(KP kp, FArgs<T> a, double* __restrict__ U,
+                                                     double* __restrict__ Un, T* __restrict__ Q,
+                                                     double* __restrict__ Rout, double dt, int KC, int remap) {
+  __shared__ T sFx[4][VYI][VX];      // Fv_x of the interior rows, all 64 lanes
+  __shared__ T sFy[4][VYI + 4][VXI];  // Fv_y of rows -2 .. 5, interior columns
+  const Blk B = xcd_block(remap);
+  const int lane = threadIdx.x, row = threadIdx.y;
+  const int NG = kp.NG;
+  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * VYI + row;  // absolute cell of this lane
+  const int yh = row < 2 ? y - 2 : y + 2;                               // its y-halo cell
+  const int k0 = NG + B.z * KC;
+  const int nk = min(KC, NG + kp.Nzp - k0);
+  const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;  // last cell of the +-2 box
+  const bool xin = lane >= 2 && lane < 2 + VXI && x < NG + kp.Nxp;
+  const bool yin = y < NG + kp.Nyp;
+  const bool needFx = yin && x <= xmax;
+  const bool needFy = xin && y <= ymax;
+  const bool needH = xin && yh <= ymax;
+  const bool inner = xin && yin;
+  const int xc = min(x, xmax), yc = min(y, ymax), yhc = min(yh, ymax);  // in-bounds loads for idle lanes
+  const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  const long long col = cidx(kp, xc, yc, 0), colh = cidx(kp, xc, yhc, 0);
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  const int var[4] = {1, 2, 3, 5};
+  const int hrow = row < 2 ? row : row + 4;  // sFy row index of the halo cell
+  T w[4][5];   // u,v,w,T of the own column, planes p-2 .. p+2
+  T fz[4][5];  // Fv_z of the own column, planes p-4 .. p
+  T xy[3][4];  // x+y viscous divergence of planes p-2, p-1, p
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) w[v][q] = fz[v][q] = T(0);
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) xy[q][n] = T(0);
+#pragma unroll
+  for (int q = 1; q < 5; ++q) {
+    const long long o = col + (long long)(k0 - 5 + q) * sz;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) w[v][q] = a.Q[o + var[v] * nc];
+  }
+#pragma unroll 1
+  for (int s = -4; s < nk; ++s) {
+    const int p = k0 + s + 2;
+    {
+      const long long o = col + (long long)(p + 2) * sz;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[v][q] = w[v][q + 1];
+        w[v][4] = a.Q[o + var[v] * nc];
+      }
+    }
+    {
+      T out[3][4];
+      if (needFx || needFy) {
+        visc_eval<T>(kp, a, col + (long long)p * sz, nullptr, 0, sy, sz, w, out);
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int n = 0; n < 4; ++n) out[d][n] = T(0);
+      }
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) fz[n][q] = fz[n][q + 1];
+        fz[n][4] = out[2][n];
+        if (needFx) sFx[n][row][lane] = out[0][n];
+        if (needFy) sFy[n][row + 2][lane - 2] = out[1][n];
+      }
+    }
+    if (needH) {
+      T out[3][4];
+      visc_eval<T>(kp, a, colh + (long long)p * sz, nullptr, 0, sy, sz, nullptr, out);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) sFy[n][hrow][lane - 2] = out[1][n];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      xy[0][n] = xy[1][n];
+      xy[1][n] = xy[2][n];
+    }
+    if (inner) {
+      const int l = lane - 2, r = row + 2;
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        T d = c1 * (sFx[n][row][lane - 2] - sFx[n][row][lane + 2]) + c2 * (sFx[n][row][lane - 1] - sFx[n][row][lane + 1]);
+        d += c1 * (sFy[n][r - 2][l] - sFy[n][r + 2][l]) + c2 * (sFy[n][r - 1][l] - sFy[n][r + 1][l]);
+        xy[2][n] = d;
+      }
+    }
+    __syncthreads();
+    if (s < 0 || !inner) continue;
+    // ---- update of plane k = p - 2
+    const int k = k0 + s;
+    const long long oc = col + (long long)k * sz, ro = ridx(kp, x - NG, y - NG, k - NG);
+    T Rt[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Rt[n] = a.R[ro + n * rn];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      T sdiv = xy[0][n];
+      sdiv += c1 * (fz[n][0] - fz[n][4]) + c2 * (fz[n][1] - fz[n][3]);
+      Rt[n + 1] += sdiv;
+    }
+    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt);
+  }
+}
+
+// Same computation with the y-halo rows as lanes of their own: 64 x 8 lanes over a 60 x 4 tile
+// (rows 0, 1, 6, 7 are halo rows, Fv_y only), each lane with its own u,v,w,T window.  Used for the
+// FP32-bulk form, whose smaller registers leave room for two 8-wave blocks per CU; the FP64 form
+// uses k_visc_update (halo evaluated by the interior lanes, every wave updating).
+constexpr int VY8 = 8;
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(512) void k_visc_update8(KP kp, FArgs<T> a, double* __restrict__ U,
+                                                      double* __restrict__ Un, T* __restrict__ Q,
+                                                      double* __restrict__ Rout, double dt, int KC, int remap) {
+  __shared__ T sFx[4][VYI][VX];
+  __shared__ T sFy[4][VY8][VXI];
+  const Blk B = xcd_block(remap);
+  const int lane = threadIdx.x, row = threadIdx.y;
+  const int NG = kp.NG;
+  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * VYI - 2 + row;
+  const int k0 = NG + B.z * KC;
+  const int nk = min(KC, NG + kp.Nzp - k0);
+  const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;
+  const bool xin = lane >= 2 && lane < 2 + VXI && x < NG + kp.Nxp;
+  const bool yin = row >= 2 && row < 2 + VYI && y < NG + kp.Nyp;
+  const bool needFx = yin && x <= xmax;
+  const bool needFy = xin && y <= ymax;
+  const bool inner = xin && yin;
+  const int xc = min(x, xmax), yc = min(y, ymax);
+  const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  const long long col = cidx(kp, xc, yc, 0);
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  const int var[4] = {1, 2, 3, 5};
+  T w[4][5], fz[4][5], xy[3][4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) w[v][q] = fz[v][q] = T(0);
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) xy[q][n] = T(0);
+#pragma unroll
+  for (int q = 1; q < 5; ++q) {
+    const long long o = col + (long long)(k0 - 5 + q) * sz;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) w[v][q] = a.Q[o + var[v] * nc];
+  }
+#pragma unroll 1
+  for (int s = -4; s < nk; ++s) {
+    const int p = k0 + s + 2;
+    {
+      const long long o = col + (long long)(p + 2) * sz;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[v][q] = w[v][q + 1];
+        w[v][4] = a.Q[o + var[v] * nc];
+      }
+    }
+    T out[3][4];
+    if (needFx || needFy) {
+      visc_eval<T>(kp, a, col + (long long)p * sz, nullptr, 0, sy, sz, w, out);
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) out[d][n] = T(0);
+    }
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) fz[n][q] = fz[n][q + 1];
+      fz[n][4] = out[2][n];
+      if (needFx) sFx[n][row - 2][lane] = out[0][n];
+      if (needFy) sFy[n][row][lane - 2] = out[1][n];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      xy[0][n] = xy[1][n];
+      xy[1][n] = xy[2][n];
+    }
+    if (inner) {
+      const int r = row - 2, l = lane - 2;
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        T d = c1 * (sFx[n][r][lane - 2] - sFx[n][r][lane + 2]) + c2 * (sFx[n][r][lane - 1] - sFx[n][r][lane + 1]);
+        d += c1 * (sFy[n][row - 2][l] - sFy[n][row + 2][l]) + c2 * (sFy[n][row - 1][l] - sFy[n][row + 1][l]);
+        xy[2][n] = d;
+      }
+    }
+    __syncthreads();
+    if (s < 0 || !inner) continue;
+    const int k = k0 + s;
+    const long long oc = col + (long long)k * sz, ro = ridx(kp, x - NG, y - NG, k - NG);
+    T Rt[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Rt[n] = a.R[ro + n * rn];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      T sdiv = xy[0][n];
+      sdiv += c1 * (fz[n][0] - fz[n][4]) + c2 * (fz[n][1] - fz[n][3]);
+      Rt[n + 1] += sdiv;
+    }
+    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt);
+  }
+}
+
 // ------------------------------------------------------------------ orchestration
 // Segment length of the y/z marches; F3D_MARCH_SEG_Y / F3D_MARCH_SEG_Z override the defaults.
 static int march_seg(int D, int es) {
@@ -462,6 +777,22 @@ static int march_seg(int D, int es) {
 static int seg_len(int Np, int max_len) {
   const int nseg = (Np + max_len - 1) / max_len;
   return (Np + nseg - 1) / nseg;
+}
+
+static dim3 visc_grid(const KP& kp, int KC) {
+  return dim3((kp.Nxp + VXI - 1) / VXI, (kp.Nyp + VYI - 1) / VYI, (kp.Nzp + KC - 1) / KC);
+}
+
+// The viscous-update march: 64x8 lanes with halo rows in FP32-bulk, 64x4 lanes in FP64.
+template <typename T, int MODE>
+static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt,
+                               int KC, int rm, hipStream_t s) {
+  if (sizeof(T) == 4)
+    hipLaunchKernelGGL((k_visc_update8<T, MODE>), visc_grid(kp, KC), dim3(VX, VY8, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                       KC, rm);
+  else
+    hipLaunchKernelGGL((k_visc_update<T, MODE>), visc_grid(kp, KC), dim3(VX, VYI, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                       KC, rm);
 }
 
 template <typename T>
@@ -495,7 +826,7 @@ static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const Metri
     TimedScope ts("fused_heavy", s);
     hipLaunchKernelGGL(k_heavy<T>, dim3(2048), dim3(64), 0, s, kp, a);
   }
-  if (kp.viscous) {
+  if (kp.viscous && !visc_fused()) {
     TimedScope ts("fused_viscous", s);
     const int KC = seg_len(kp.Nzp + 4, 32);
     hipLaunchKernelGGL(k_viscous_march<T>, dim3((kp.Nxp + 4 + 63) / 64, (kp.Nyp + 4 + 3) / 4, (kp.Nzp + 4 + KC - 1) / KC),
@@ -511,6 +842,11 @@ int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>
   if (rc) return rc;
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
   TimedScope ts("fused_update", s);
+  if (kp.viscous && visc_fused()) {
+    const int KC = seg_len(kp.Nzp, 16);
+    launch_visc_update<T, 0>(kp, a, nullptr, nullptr, nullptr, R, 0.0, KC, xcd_remap(), s);
+    return check_launch("fused rhs update");
+  }
   const int KC = seg_len(kp.Nzp, 16);
   hipLaunchKernelGGL((k_update<T, 0>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC),
                      dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, nullptr, nullptr, nullptr, R, 0.0, KC,
@@ -518,16 +854,33 @@ int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>
   return check_launch("fused rhs update");
 }
 
+bool stage_needs_separate_q(const KP& kp) { return use_fused() && kp.viscous && visc_fused(); }
+
 template <typename T>
-int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, T* Q, const MetricPtrs<T>& m,
-                const double* J, double dt, hipStream_t s) {
-  int rc = fused_flux<T>(kp, ws, Q, m, J, ws.phi, s);
+int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
+                const MetricPtrs<T>& m, const double* J, double dt, hipStream_t s) {
+  if (Qin == Qout && stage_needs_separate_q(kp)) {
+    set_error("fused stage: the output state must not alias the input state");
+    return F3D_E_ARG;
+  }
+  int rc = fused_flux<T>(kp, ws, Qin, m, J, ws.phi, s);
   if (rc) return rc;
-  FArgs<T> a{Q, m, J, ws.phi, ws.R, ws.heavy, ws.hcount};
+  FArgs<T> a{Qin, m, J, ws.phi, ws.R, ws.heavy, ws.hcount};
+  T* Q = Qout;
   const int KC = seg_len(kp.Nzp, 16);
   const dim3 g((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC), b(64, 4, 1);
   TimedScope ts("fused_update", s);
   const int rm = xcd_remap();
+  if (kp.viscous && visc_fused()) {
+    const int KV = seg_len(kp.Nzp, 16);
+    if (KRK == 1)
+      launch_visc_update<T, 1>(kp, a, U, Un, Q, nullptr, dt, KV, rm, s);
+    else if (KRK == 2)
+      launch_visc_update<T, 2>(kp, a, U, Un, Q, nullptr, dt, KV, rm, s);
+    else
+      launch_visc_update<T, 3>(kp, a, U, Un, Q, nullptr, dt, KV, rm, s);
+    return check_launch("fused stage update");
+  }
   if (KRK == 1)
     hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC, rm);
   else if (KRK == 2)
@@ -541,8 +894,8 @@ template int fused_rhs<double>(const KP&, const WsView<double>&, const double*, 
                                const double*, double*, double*, hipStream_t);
 template int fused_rhs<float>(const KP&, const WsView<float>&, const float*, const MetricPtrs<float>&, const double*,
                               float*, double*, hipStream_t);
-template int fused_stage<double>(const KP&, const WsView<double>&, int, double*, double*, double*,
+template int fused_stage<double>(const KP&, const WsView<double>&, int, double*, double*, const double*, double*,
                                  const MetricPtrs<double>&, const double*, double, hipStream_t);
-template int fused_stage<float>(const KP&, const WsView<float>&, int, double*, double*, float*,
+template int fused_stage<float>(const KP&, const WsView<float>&, int, double*, double*, const float*, float*,
                                 const MetricPtrs<float>&, const double*, double, hipStream_t);
 }  // namespace f3d

@@ -69,8 +69,20 @@ template <typename T>
 int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J, T* phi,
               double* R, hipStream_t s);
 
-// One RK3 stage up to and including c2Prim of the interior (ghost fill done by the caller).
+// One RK3 stage up to and including c2Prim of the interior (ghost fill done by the caller): reads the
+// primitive state Qin, writes the new one into Qout.  Qout must differ from Qin when
+// stage_needs_separate_q(kp) (the viscous divergence is then computed in the update march, which
+// still reads Qin of neighbouring columns while other blocks write Qout).
 template <typename T>
-int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, T* Q, const MetricPtrs<T>& m,
-                const double* J, double dt, hipStream_t s);
+int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
+                const MetricPtrs<T>& m, const double* J, double dt, hipStream_t s);
+bool stage_needs_separate_q(const KP& kp);
+
+// Stage / multi-step drivers of f3d_stage.hip (with the ghost procedure of the output state).
+template <typename T>
+int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
+             const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s);
+template <typename T>
+int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, double* Un, T* Q,
+             const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s);
 }  // namespace f3d

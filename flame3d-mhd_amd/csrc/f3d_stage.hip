@@ -22,6 +22,7 @@ struct f3d_workspace {
   unsigned long long* heavy;  // compacted non-UP7 face list (fused path)
   unsigned int* hcount;
   unsigned long long hcap;
+  void *Qa, *Qb;  // stage outputs of the fused path (primitive state, 6 vars, bulk type)
 };
 
 namespace {
@@ -219,29 +220,79 @@ int rhs_impl(const f3d_params* p, f3d_workspace* ws, double* R, const T* Q, cons
   return rc;
 }
 
+}  // namespace
+
+namespace f3d {
 template <typename T>
-int stage_impl(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, T* Q,
-               const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s) {
+int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
+             const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s) {
   KP kp = make_kp(p);
   MetricPtrs<T> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
   int rc = F3D_OK;
   if (use_fused()) {
-    rc = fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Q, m, J, dt, s);
+    rc = fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, dt, s);
   } else {
+    if (Qout != Qin) F3D_HIP(hipMemcpyAsync(Qout, Qin, sizeof(T) * 6 * kp.nc, hipMemcpyDeviceToDevice, s));
     if (KRK == 1) F3D_HIP(hipMemcpyAsync(Un, U, sizeof(double) * 5 * kp.nc, hipMemcpyDeviceToDevice, s));  // solver.jl:418
-    rc = rhs_chain<T>(ws, kp, Q, m, J, (T*)ws->phi, s);
+    rc = rhs_chain<T>(ws, kp, Qout, m, J, (T*)ws->phi, s);
     if (!rc)
       rc = launch_div<T>(kp, U, (T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, dt, J, s);
     if (!rc && KRK == 2) rc = launch_lin_comb(kp, U, Un, 5, 0.25, 0.75, s);
     if (!rc && KRK == 3) rc = launch_lin_comb(kp, U, Un, 5, 2.0 / 3.0, 1.0 / 3.0, s);
-    if (!rc) rc = launch_c2prim<T>(kp, U, Q, s);
+    if (!rc) rc = launch_c2prim<T>(kp, U, Qout, s);
   }
-  if (!rc) rc = launch_exchange<T>(kp, Q, 6, s);
+  if (!rc) rc = launch_exchange<T>(kp, Qout, 6, s);
   // U ghosts are never read on this path; only Q ghosts are filled by the stage.
-  if (!rc) rc = launch_fill<T>(kp, Q, nullptr, inflow, s);
+  if (!rc) rc = launch_fill<T>(kp, Qout, nullptr, inflow, s);
   return rc;
 }
+
+template <typename T>
+int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, double* Un, T* Q,
+             const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s) {
+  const KP kp = make_kp(p);
+  if (!stage_needs_separate_q(kp)) {
+    for (int t = 0; t < nsteps; ++t)
+      for (int KRK = 1; KRK <= 3; ++KRK) {
+        const int rc = stage_io<T>(p, ws, KRK, U, Un, Q, Q, metrics, J, dt, inflow, s);
+        if (rc) return rc;
+      }
+    return F3D_OK;
+  }
+  // stage outputs rotate Q -> Qa -> Qb -> Q: a whole step needs no copy
+  T* Qa = (T*)ws->Qa;
+  T* Qb = (T*)ws->Qb;
+  for (int t = 0; t < nsteps; ++t) {
+    int rc = stage_io<T>(p, ws, 1, U, Un, Q, Qa, metrics, J, dt, inflow, s);
+    if (!rc) rc = stage_io<T>(p, ws, 2, U, Un, Qa, Qb, metrics, J, dt, inflow, s);
+    if (!rc) rc = stage_io<T>(p, ws, 3, U, Un, Qb, Q, metrics, J, dt, inflow, s);
+    if (rc) return rc;
+  }
+  return F3D_OK;
+}
+
+template <typename T>
+int stage_inplace(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, T* Q,
+                  const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s) {
+  const KP kp = make_kp(p);
+  if (!stage_needs_separate_q(kp)) return stage_io<T>(p, ws, KRK, U, Un, Q, Q, metrics, J, dt, inflow, s);
+  T* Qa = (T*)ws->Qa;
+  const int rc = stage_io<T>(p, ws, KRK, U, Un, Q, Qa, metrics, J, dt, inflow, s);
+  if (rc) return rc;
+  F3D_HIP(hipMemcpyAsync(Q, Qa, sizeof(T) * 6 * kp.nc, hipMemcpyDeviceToDevice, s));
+  return F3D_OK;
+}
+
+template int stage_io<double>(const f3d_params*, f3d_workspace*, int, double*, double*, const double*, double*,
+                              const double* const*, const double*, double, const double*, hipStream_t);
+template int stage_io<float>(const f3d_params*, f3d_workspace*, int, double*, double*, const float*, float*,
+                             const float* const*, const double*, double, const float*, hipStream_t);
+template int steps_io<double>(const f3d_params*, f3d_workspace*, int, double*, double*, double*,
+                              const double* const*, const double*, double, const double*, hipStream_t);
+}  // namespace f3d
+
+namespace {
 
 template <typename T>
 int check_stage_args(const f3d_params* p, const f3d_workspace* ws, int prec, const void* U, const void* Un,
@@ -286,7 +337,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
   const size_t nc = f3d_cell_count(p);
   const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
-  size_t sz[14];
+  size_t sz[16];
   sz[0] = nc * es;                               // phi
   sz[1] = nc * 5 * es;                           // Fp
   sz[2] = nc * 5 * es;                           // Fm
@@ -301,6 +352,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t nfaces = (Nx + 1) * Ny * Nz + Nx * (Ny + 1) * Nz + Nx * Ny * (Nz + 1);
   sz[12] = nfaces * sizeof(unsigned long long);  // heavy-face list
   sz[13] = 256;                                  // heavy-face counter
+  sz[14] = sz[15] = nc * 6 * es;                 // Qa, Qb
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -311,9 +363,10 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   ws->bytes = total;
   ws->base = base;
   char* c = (char*)base;
-  void** slots[14] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
-                      &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount};
-  for (int i = 0; i < 14; ++i) {
+  void** slots[16] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+                      &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount,
+                      &ws->Qa, &ws->Qb};
+  for (int i = 0; i < 16; ++i) {
     *slots[i] = c;
     c += align_up(sz[i]);
   }
@@ -385,7 +438,7 @@ F3D_API int f3d_rk3_stage_f64(const f3d_params* p, f3d_workspace* ws, int32_t KR
     set_error("rk3_stage_f64: KRK=%d not in 1..3", KRK);
     return F3D_E_ARG;
   }
-  return stage_impl<double>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
+  return stage_inplace<double>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
 }
 F3D_API int f3d_rk3_stage_f32b(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U, double* Un,
                                float* Q, const float* const* metrics, const double* J, double dt,
@@ -396,7 +449,7 @@ F3D_API int f3d_rk3_stage_f32b(const f3d_params* p, f3d_workspace* ws, int32_t K
     set_error("rk3_stage_f32b: KRK=%d not in 1..3", KRK);
     return F3D_E_ARG;
   }
-  return stage_impl<float>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
+  return stage_inplace<float>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
 }
 
 F3D_API int f3d_time_step_f64(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U, double* Un,
@@ -408,12 +461,7 @@ F3D_API int f3d_time_step_f64(const f3d_params* p, f3d_workspace* ws, int32_t ns
     set_error("time_step_f64: nsteps < 0");
     return F3D_E_ARG;
   }
-  for (int t = 0; t < nsteps; ++t)
-    for (int KRK = 1; KRK <= 3; ++KRK) {
-      rc = stage_impl<double>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
-      if (rc) return rc;
-    }
-  return F3D_OK;
+  return steps_io<double>(p, ws, nsteps, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
 }
 F3D_API int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U, double* Un,
                                float* Q, const float* const* metrics, const double* J, double dt,
@@ -424,12 +472,7 @@ F3D_API int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t n
     set_error("time_step_f32b: nsteps < 0");
     return F3D_E_ARG;
   }
-  for (int t = 0; t < nsteps; ++t)
-    for (int KRK = 1; KRK <= 3; ++KRK) {
-      rc = stage_impl<float>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
-      if (rc) return rc;
-    }
-  return F3D_OK;
+  return steps_io<float>(p, ws, nsteps, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s);
 }
 
 template <typename T>
