@@ -457,10 +457,10 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
 
 // ------------------------------------------------------------------ viscous divergence + update
 // k_update with the viscous divergence computed in place: the viscous fluxes never go to HBM.
-// Block = 64 x 4 lanes: 4 interior rows of a 60-column tile plus 2 halo lanes each side, marching
-// k.  Each iteration, every lane evaluates the velocity/temperature gradient and viscous_cell at
-// plane p (2 ahead of the updated plane k) for its own cell and for one y-halo cell (rows -2, -1
-// for lanes of rows 0, 1; rows 4, 5 for rows 2, 3; Fv_y only).  Fv_x of the interior rows and
+// Block = 64 x ROWS lanes: ROWS interior rows of a 60-column tile plus 2 halo lanes each side,
+// marching k.  Each iteration, every lane evaluates the velocity/temperature gradient and
+// viscous_cell at plane p (2 ahead of the updated plane k) for its own cell; the lanes of the
+// first and last two rows also evaluate one y-halo cell (rows -2, -1 and ROWS, ROWS+1; Fv_y only).  Fv_x of the interior rows and
 // Fv_y of the interior columns are exchanged through LDS for the x/y divergence of plane p, kept
 // two iterations in a register ring; Fv_z enters a 5-plane register window for the z divergence
 // of plane k; u,v,w,T of the own column sit in a 5-plane window for the zeta derivatives (the halo
@@ -539,17 +539,18 @@ __device__ __forceinline__ void update_cell(const KP& kp, const T* Rt, long long
   Q[oc + 5 * nc] = (T)(pr / (rho * kp.Rg));
 }
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_visc_update(KP kp, FArgs<T> a, double* __restrict__ U,
-                                                     double* __restrict__ Un, T* __restrict__ Q,
-                                                     double* __restrict__ Rout, double dt, int KC, int remap) {
-  __shared__ T sFx[4][VYI][VX];      // Fv_x of the interior rows, all 64 lanes
-  __shared__ T sFy[4][VYI + 4][VXI];  // Fv_y of rows -2 .. 5, interior columns
+template <typename T, int MODE, int ROWS>
+__global__ __launch_bounds__(VX * ROWS) void k_visc_update(KP kp, FArgs<T> a, double* __restrict__ U,
+                                                           double* __restrict__ Un, T* __restrict__ Q,
+                                                           double* __restrict__ Rout, double dt, int KC, int remap) {
+  __shared__ T sFx[4][ROWS][VX];       // Fv_x of the interior rows, all 64 lanes
+  __shared__ T sFy[4][ROWS + 4][VXI];  // Fv_y of rows -2 .. ROWS+1, interior columns
   const Blk B = xcd_block(remap);
   const int lane = threadIdx.x, row = threadIdx.y;
   const int NG = kp.NG;
-  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * VYI + row;  // absolute cell of this lane
-  const int yh = row < 2 ? y - 2 : y + 2;                               // its y-halo cell
+  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * ROWS + row;  // absolute cell of this lane
+  const bool hasH = row < 2 || row >= ROWS - 2;                          // lanes that also evaluate a y-halo cell
+  const int yh = row < 2 ? y - 2 : y + 2;
   const int k0 = NG + B.z * KC;
   const int nk = min(KC, NG + kp.Nzp - k0);
   const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;  // last cell of the +-2 box
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(256) void k_visc_update(KP kp, FArgs<T> a, double* 
   const bool yin = y < NG + kp.Nyp;
   const bool needFx = yin && x <= xmax;
   const bool needFy = xin && y <= ymax;
-  const bool needH = xin && yh <= ymax;
+  const bool needH = hasH && xin && yh <= ymax;
   const bool inner = xin && yin;
   const int xc = min(x, xmax), yc = min(y, ymax), yhc = min(yh, ymax);  // in-bounds loads for idle lanes
   const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
@@ -779,20 +780,22 @@ static int seg_len(int Np, int max_len) {
   return (Np + nseg - 1) / nseg;
 }
 
-static dim3 visc_grid(const KP& kp, int KC) {
-  return dim3((kp.Nxp + VXI - 1) / VXI, (kp.Nyp + VYI - 1) / VYI, (kp.Nzp + KC - 1) / KC);
+static dim3 visc_grid(const KP& kp, int KC, int rows) {
+  return dim3((kp.Nxp + VXI - 1) / VXI, (kp.Nyp + rows - 1) / rows, (kp.Nzp + KC - 1) / KC);
 }
 
-// The viscous-update march: 64x8 lanes with halo rows in FP32-bulk, 64x4 lanes in FP64.
+// The viscous-update march, per precision the layout measured fastest on the 512x192x64 ramp:
+// FP32-bulk k_visc_update8 (halo rows as lanes, two 8-wave blocks per CU), FP64 k_visc_update with
+// 4 interior rows (237 VGPRs: two 4-wave blocks per CU; taller blocks stall more waves per barrier).
 template <typename T, int MODE>
 static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt,
                                int KC, int rm, hipStream_t s) {
   if (sizeof(T) == 4)
-    hipLaunchKernelGGL((k_visc_update8<T, MODE>), visc_grid(kp, KC), dim3(VX, VY8, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                       KC, rm);
+    hipLaunchKernelGGL((k_visc_update8<T, MODE>), visc_grid(kp, KC, VYI), dim3(VX, VY8, 1), 0, s, kp, a, U, Un, Q,
+                       Rout, dt, KC, rm);
   else
-    hipLaunchKernelGGL((k_visc_update<T, MODE>), visc_grid(kp, KC), dim3(VX, VYI, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                       KC, rm);
+    hipLaunchKernelGGL((k_visc_update<T, MODE, VYI>), visc_grid(kp, KC, VYI), dim3(VX, VYI, 1), 0, s, kp, a, U, Un, Q,
+                       Rout, dt, KC, rm);
 }
 
 template <typename T>
