@@ -780,6 +780,21 @@ static int seg_len(int Np, int max_len) {
   return (Np + nseg - 1) / nseg;
 }
 
+// Planes per segment of the viscous-update march: 16 where that gives >= ~1000 blocks, shorter on
+// small grids (each segment re-evaluates 4 planes of viscous fluxes); F3D_VISC_KC overrides.
+static int visc_kc(const KP& kp, int rows) {
+  static int env = -2;
+  if (env == -2) {
+    const char* e = std::getenv("F3D_VISC_KC");
+    env = e ? std::atoi(e) : -1;
+  }
+  if (env >= 4) return seg_len(kp.Nzp, env);
+  const long long cols = (long long)((kp.Nxp + VXI - 1) / VXI) * ((kp.Nyp + rows - 1) / rows);
+  int kc = 16;
+  while (kc > 8 && cols * ((kp.Nzp + kc - 1) / kc) < 1000) kc -= 4;
+  return seg_len(kp.Nzp, kc);
+}
+
 static dim3 visc_grid(const KP& kp, int KC, int rows) {
   return dim3((kp.Nxp + VXI - 1) / VXI, (kp.Nyp + rows - 1) / rows, (kp.Nzp + KC - 1) / KC);
 }
@@ -846,7 +861,7 @@ int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
   TimedScope ts("fused_update", s);
   if (kp.viscous && visc_fused()) {
-    const int KC = seg_len(kp.Nzp, 16);
+    const int KC = visc_kc(kp, VYI);
     launch_visc_update<T, 0>(kp, a, nullptr, nullptr, nullptr, R, 0.0, KC, xcd_remap(), s);
     return check_launch("fused rhs update");
   }
@@ -875,7 +890,7 @@ int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* U
   TimedScope ts("fused_update", s);
   const int rm = xcd_remap();
   if (kp.viscous && visc_fused()) {
-    const int KV = seg_len(kp.Nzp, 16);
+    const int KV = visc_kc(kp, VYI);
     if (KRK == 1)
       launch_visc_update<T, 1>(kp, a, U, Un, Q, nullptr, dt, KV, rm, s);
     else if (KRK == 2)
