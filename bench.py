@@ -85,9 +85,9 @@ FUSED_BYTES = {
 MEASURED_TRAFFIC = {("fused_update", "512x192x64", "f64"): (3223.7e6 + 3216.1e6 + 2961.3e6 + 565.7e6 * 2 + 824.2e6) / 3}
 
 
-def build_case(f3d, cases, torch, nx, ny, nz, seed=2):
+def build_case(f3d, cases, torch, nx, ny, nz, seed=2, sweep=0.0):
     import numpy as np
-    c = cases.ramp(nx, ny, nz, seed=seed)
+    c = cases.ramp(nx, ny, nz, seed=seed, sweep=sweep)
     p = f3d.make_params(nx, ny, nz, **c["params"])
     ctx = f3d.Flame3D(p)
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
@@ -148,6 +148,11 @@ def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3):
     return ev0.elapsed_time(ev1) / steps, wall / steps * 1e3, table, live
 
 
+def progress(msg):
+    """One line per finished leg on stderr (long runs stay visibly alive)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def kernel_table(lib):
     import ctypes
     buf = ctypes.create_string_buffer(4096)
@@ -186,7 +191,7 @@ def cpu_baseline(f3d, cases, nx, ny, nz, steps=3):
                 kind="port", sample=f"oracle FP64, {steps} RK3 steps of the {nx}x{ny}x{nz} ramp ({el:.1f} s)")
 
 
-def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True):
+def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True, every=1, sweep=0.0):
     """Inflow-parameter adjoint of the ramp: theta = (M_inf, T_wall, 16 Fourier amplitudes),
     objective = sum over `nsteps` FP64 RK3 steps of dt * sum_wall p * |grad eta|/J (wall pressure
     x wall-face area on the first cell row), checkpoint every RK step.  Times one forward step and
@@ -195,7 +200,7 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True):
     components against GPU central finite differences."""
     import numpy as np
     NG = 4
-    c = cases.ramp(nx, ny, nz)
+    c = cases.ramp(nx, ny, nz, sweep=sweep)
     st = c["state"]
     ygh = c["y"][:, :, 0:NG].mean(axis=2)
     th = cases.theta0()
@@ -249,14 +254,14 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True):
     lamU0 = torch.zeros_like(U0)
     lam_in = torch.zeros_like(inflow)
     lam_bc = torch.zeros(7, dtype=torch.float64, device="cuda:0")
-    ckpt = torch.empty(ctx.adjoint_ckpt_bytes(nsteps, 1) // 8, dtype=torch.float64, device="cuda:0")
+    ckpt = torch.empty(ctx.adjoint_ckpt_bytes(nsteps, every) // 8, dtype=torch.float64, device="cuda:0")
     ctx.adjoint(1, 1, U0, m, J, dt, W, obj, lamU0, inflow, lam_in, lam_bc, ckpt=ckpt)
     lam_in.zero_()
     lam_bc.zero_()
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record()
-    ctx.adjoint(nsteps, 1, U0, m, J, dt, W, obj, lamU0, inflow, lam_in, lam_bc, ckpt=ckpt)
+    ctx.adjoint(nsteps, every, U0, m, J, dt, W, obj, lamU0, inflow, lam_in, lam_bc, ckpt=ckpt)
     ev1.record()
     torch.cuda.synchronize()
     ms_adj = ev0.elapsed_time(ev1)
@@ -264,7 +269,7 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True):
     lib.f3d_timing_reset()
     lib.f3d_timing_enable(1)
     obj2 = torch.zeros_like(obj)
-    ctx.adjoint(nsteps, 1, U0, m, J, dt, W, obj2, torch.zeros_like(U0), inflow, torch.zeros_like(inflow),
+    ctx.adjoint(nsteps, every, U0, m, J, dt, W, obj2, torch.zeros_like(U0), inflow, torch.zeros_like(inflow),
                 torch.zeros_like(lam_bc), ckpt=ckpt)
     torch.cuda.synchronize()
     lib.f3d_timing_enable(0)
@@ -291,8 +296,10 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True):
                                fd_noise_rel=abs(j_a - j_b) / max(abs(jp - jm), 1e-300)))
     rev = (ms_adj - nsteps * ms_fwd) / nsteps
     return dict(
-        config=f"ramp {nx}x{ny}x{nz} FP64, {nsteps} RK3 steps, checkpoint every step; theta = (M_inf, T_wall, "
-               f"16 inflow Fourier amplitudes); objective = sum_n dt * wall pressure x wall-face area",
+        config=f"ramp {nx}x{ny}x{nz}{f' swept {sweep:g} deg' if sweep else ''} FP64, {nsteps} RK3 steps, checkpoint "
+               f"every {every} step(s) (states inside a segment recomputed); theta = (M_inf, T_wall, 16 inflow "
+               f"Fourier amplitudes); objective = sum_n dt * wall pressure x wall-face area",
+        checkpoint_bytes=int(ckpt.numel() * 8),
         ms_forward_step=round(ms_fwd, 4), ms_adjoint_total=round(ms_adj, 3),
         ms_reverse_step=round(rev, 4), adjoint_over_forward_step=round(rev / ms_fwd, 3),
         objective=J0, objective_check_rel=abs(J0 - float(obj2.sum().item())) / abs(J0),
@@ -316,6 +323,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint leg (profiling)")
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference check of the adjoint")
+    ap.add_argument("--sweep", type=float, default=0.0, help="sweep angle (deg) of the ramp grid")
+    ap.add_argument("--adjoint-every", type=int, default=1, help="checkpoint interval of the adjoint (RK3 steps)")
+    ap.add_argument("--adjoint-sweep", type=float, default=0.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -342,10 +352,12 @@ def main():
     if args.adjoint_only:
         ax, ay, az = map(int, args.adjoint_grid.split("x"))
         print(json.dumps({"adjoint": adjoint_bench(f3d, cases, torch, lib, ax, ay, az, args.adjoint_steps,
-                                                   fd=not args.no_fd)}))
+                                                   fd=not args.no_fd, every=args.adjoint_every,
+                                                   sweep=args.adjoint_sweep)}))
         return
 
-    st = build_case(f3d, cases, torch, nx, ny, nz)
+    st = build_case(f3d, cases, torch, nx, ny, nz, sweep=args.sweep)
+    progress(f"case {nx}x{ny}x{nz} built")
     p, ctx, dt = st["p"], st["ctx"], st["dt"]
     U0 = st["U"].clone()
     Q0 = st["Q"].clone()
@@ -355,6 +367,7 @@ def main():
                                          args.warmup, lib=lib)
     ctx.nan_check(U)
     value64 = ncells / (ms64 * 1e-3) / 1e6
+    progress(f"FP64 {args.steps} steps: {ms64:.3f} ms/step")
     rho64 = U[0].clone()
     # ---- FP32-bulk from the same initial state, same dt and step count
     out32 = {}
@@ -366,6 +379,7 @@ def main():
                                        lib=lib)
         ctx.nan_check(U32)
         value32 = ncells / (ms32 * 1e-3) / 1e6
+        progress(f"FP32-bulk {args.steps} steps: {ms32:.3f} ms/step")
         NG = 4
         inner = (slice(NG, -NG), slice(NG, -NG), slice(NG, -NG))
         r64, r32 = U[0][inner].double(), U32[0][inner].double()
@@ -400,7 +414,10 @@ def main():
     adj = None
     if not args.no_adjoint:
         ax, ay, az = map(int, args.adjoint_grid.split("x"))
-        adj = adjoint_bench(f3d, cases, torch, lib, ax, ay, az, args.adjoint_steps, fd=not args.no_fd)
+        adj = adjoint_bench(f3d, cases, torch, lib, ax, ay, az, args.adjoint_steps, fd=not args.no_fd,
+                            every=args.adjoint_every, sweep=args.adjoint_sweep)
+    if adj:
+        progress(f"adjoint: {adj['ms_reverse_step']:.3f} ms per reverse step")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(f3d, cases, nx, ny, nz, steps=args.cpu_steps)
@@ -417,8 +434,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded laminar-BL ramp inflow, generated grid)",
-        "config": {"workload": f"compression ramp 24 deg, {nx}x{ny}x{nz}, FP64, M2.9, Re_delta 6e4, "
-                               f"isothermal wall, periodic span", "cells": ncells, "dt": dt},
+        "config": {"workload": f"compression ramp 24 deg{f', swept {args.sweep:g} deg' if args.sweep else ''}, "
+                               f"{nx}x{ny}x{nz}, FP64, M2.9, Re_delta 6e4, isothermal wall, periodic span",
+                   "cells": ncells, "dt": dt},
         "hbm_frac_path": round(ALG_BYTES_PER_CELL_UPDATE["f64"] * value64 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
         "roofline": roof,
         "cpu_baseline": cpu,
