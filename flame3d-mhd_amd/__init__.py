@@ -1,9 +1,10 @@
-"""Flame3D finite-difference RHS + RK3 stage on MI355X (gfx950): Python host over libf3d_hip.so.
+"""Flame3D RHS + RK3 stage on MI355X (gfx950): Python host over libf3d_hip.so.
 
 The reference (rygan-code/Flame3D-MHD, Julia + CUDA.jl) launches per-stage kernels from
 `time_step` (solver.jl:191-680).  This module exposes the same entry points with the same names
 and argument order -- `shockSensor`, `fluxSplit_SW`, `advect_xc/yc/zc`, `viscousFlux`, `div`,
-`linComb`, `c2Prim`, `prim2c`, `exchange_ghost`, `fillGhost`, `flowAdvance`, `time_step` -- as
+`linComb`, `c2Prim`, `prim2c`, `exchange_ghost`, `fillGhost`, `flowAdvance`, `time_step`, and the
+finite-volume branch's `Eigen_reconstruct_{i,j,k}` / `Conser_reconstruct_{i,j,k}` (FVM.jl) -- as
 methods of a `Flame3D` context that carries what the reference keeps in `const` globals
 (run.jl:3-108).  Arrays are torch tensors on the GPU in the reference's column-major layout:
 a Julia array (Nx_tot, Ny_tot, Nz_tot, nvar) is a C-contiguous tensor of shape
@@ -23,6 +24,7 @@ __all__ = [
     "make_params",
     "BC_PERIODIC", "BC_DIRICHLET", "BC_ZEROGRAD", "BC_WALL_ADIABATIC", "BC_WALL_ISOTHERMAL", "BC_INFLOW",
     "PREC_F64", "PREC_F32B", "SCHEME_UP7", "SCHEME_WENO7", "SCHEME_WENO5", "SCHEME_NND2",
+    "FDM", "FVM_EIGEN", "FVM_CONSER",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -31,6 +33,7 @@ LIB_PATH = os.path.join(_HERE, "libf3d_hip.so")
 BC_PERIODIC, BC_DIRICHLET, BC_ZEROGRAD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, BC_INFLOW = range(6)
 PREC_F64, PREC_F32B = 0, 1
 SCHEME_UP7, SCHEME_WENO7, SCHEME_WENO5, SCHEME_NND2 = range(4)
+FDM, FVM_EIGEN, FVM_CONSER = range(3)
 NG = 4
 
 
@@ -38,7 +41,7 @@ class F3DParams(ctypes.Structure):
     """ctypes mirror of `f3d_params` (include/f3d.h)."""
     _fields_ = [
         ("Nxp", c_int32), ("Nyp", c_int32), ("Nzp", c_int32), ("NG", c_int32), ("viscous", c_int32),
-        ("bc", c_int32 * 6), ("pad_", c_int32),
+        ("bc", c_int32 * 6), ("fvm", c_int32),
         ("gamma", c_double), ("Rg", c_double), ("Cp", c_double), ("C_s", c_double), ("T_s", c_double),
         ("Pr", c_double), ("hybrid_phi1", c_double), ("hybrid_phi2", c_double), ("hybrid_phi3", c_double),
         ("Linear_phi", c_double), ("weno7_eps", c_double), ("weno5_eps", c_double),
@@ -308,9 +311,38 @@ class Flame3D:
         """boundary.jl:151-156 (fill_x, fill_y, fill_z) with the face kinds of params.bc."""
         self._call(f"f3d_fill_ghost_{self._suffix(Q)}", _ptr(Q), _ptr(U), _ptr(inflow))
 
+    # -- finite-volume branch (FVM.jl) ---------------------------------------------------------
+    def _fvm(self, name, Q, U, phi, S, F, x, y, z):
+        self._call(f"f3d_{name}_{self._suffix(Q)}", *map(_ptr, (Q, U, phi, S, F, x, y, z)))
+
+    def Eigen_reconstruct_i(self, Q, U, phi, S, Fx, x, y, z):
+        """FVM.jl:6-238 (x, y, z: float64 mesh points)."""
+        self._fvm("eigen_reconstruct_i", Q, U, phi, S, Fx, x, y, z)
+
+    def Eigen_reconstruct_j(self, Q, U, phi, S, Fy, x, y, z):
+        """FVM.jl:240-453"""
+        self._fvm("eigen_reconstruct_j", Q, U, phi, S, Fy, x, y, z)
+
+    def Eigen_reconstruct_k(self, Q, U, phi, S, Fz, x, y, z):
+        """FVM.jl:455-670"""
+        self._fvm("eigen_reconstruct_k", Q, U, phi, S, Fz, x, y, z)
+
+    def Conser_reconstruct_i(self, Q, U, phi, S, Fx, x, y, z):
+        """FVM.jl:672-865"""
+        self._fvm("conser_reconstruct_i", Q, U, phi, S, Fx, x, y, z)
+
+    def Conser_reconstruct_j(self, Q, U, phi, S, Fy, x, y, z):
+        """FVM.jl:867-1044"""
+        self._fvm("conser_reconstruct_j", Q, U, phi, S, Fy, x, y, z)
+
+    def Conser_reconstruct_k(self, Q, U, phi, S, Fz, x, y, z):
+        """FVM.jl:1046-1222"""
+        self._fvm("conser_reconstruct_k", Q, U, phi, S, Fz, x, y, z)
+
     def flowAdvance(self, U, Q, Fp, Fm, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, s1, s2, s3, dxidx, dxidy, dxidz,
                     detadx, detady, detadz, dzetadx, dzetady, dzetadz, J, x, y, z, phi):
-        """solver.jl:19, finite-difference branch 133-188."""
+        """solver.jl:19: finite-volume branch (21-31) when params.fvm, else finite-difference (133-188);
+        x, y, z are float64 mesh points in both numeric forms."""
         self._call(f"f3d_flow_advance_{self._suffix(Q)}",
                    *map(_ptr, (U, Q, Fp, Fm, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, s1, s2, s3, dxidx, dxidy, dxidz,
                                detadx, detady, detadz, dzetadx, dzetady, dzetadz, J, x, y, z, phi)))
@@ -342,6 +374,25 @@ class Flame3D:
         fn = getattr(lib(), f"f3d_time_step_{self._suffix(Q)}")
         _check(fn(ctypes.byref(self.params), ws.handle, c_int32(nsteps), _ptr(U), _ptr(Un), _ptr(Q),
                   _ptrs(metrics), _ptr(J), c_double(dt), _ptr(inflow), self.stream))
+
+    def rhs_fvm(self, R, Q, U, metrics, J, coords, phi=None):
+        """Finite-volume RHS (params.fvm = FVM_EIGEN / FVM_CONSER); U with ghosts, coords = (x, y, z)."""
+        ws = self.workspace(self._prec(Q))
+        fn = getattr(lib(), f"f3d_rhs_fvm_{self._suffix(Q)}")
+        _check(fn(ctypes.byref(self.params), ws.handle, _ptr(R), _ptr(Q), _ptr(U), _ptrs(metrics), _ptr(J),
+                  _ptrs(coords), _ptr(phi), self.stream))
+
+    def rk3_stage_fvm(self, KRK, U, Un, Q, metrics, J, coords, dt, inflow=None):
+        ws = self.workspace(self._prec(Q))
+        fn = getattr(lib(), f"f3d_rk3_stage_fvm_{self._suffix(Q)}")
+        _check(fn(ctypes.byref(self.params), ws.handle, c_int32(KRK), _ptr(U), _ptr(Un), _ptr(Q), _ptrs(metrics),
+                  _ptr(J), _ptrs(coords), c_double(dt), _ptr(inflow), self.stream))
+
+    def time_step_fvm(self, nsteps, U, Un, Q, metrics, J, coords, dt, inflow=None):
+        ws = self.workspace(self._prec(Q))
+        fn = getattr(lib(), f"f3d_time_step_fvm_{self._suffix(Q)}")
+        _check(fn(ctypes.byref(self.params), ws.handle, c_int32(nsteps), _ptr(U), _ptr(Un), _ptr(Q),
+                  _ptrs(metrics), _ptr(J), _ptrs(coords), c_double(dt), _ptr(inflow), self.stream))
 
     def cfl_dt(self, dt_out, Q, metrics, J, cfl):
         """Global CFL time step into the device scalar dt_out (float64 tensor of one element)."""

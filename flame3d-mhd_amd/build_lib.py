@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libf3d_hip.so")
-SOURCES = ["f3d_capi.hip", "f3d_kernels.hip", "f3d_stage.hip", "f3d_fused.hip", "f3d_adjoint.hip", "f3d_timing.hip"]
+SOURCES = ["f3d_capi.hip", "f3d_kernels.hip", "f3d_stage.hip", "f3d_fused.hip", "f3d_fvm.hip", "f3d_adjoint.hip",
+           "f3d_timing.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

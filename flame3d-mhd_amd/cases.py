@@ -52,7 +52,7 @@ def tgv(N=64, mach=0.1, Re=1600.0, T0=300.0, noise=0.0, jump=0.0, seed=1):
     U0 = mach*c0, L = 1 m, rho0 = Re*mu(T0)/(U0 L).  Optional seeded relative noise on every
     primitive and a planted pressure jump (factor 1+jump on the half-box i >= N/2) so that every
     scheme branch of the hybrid reconstruction fires.  Returns a dict with params kwargs, Q (with
-    ghosts left zero: fill them with the ghost fill), metrics (9) and J.
+    ghosts left zero: fill them with the ghost fill), metrics (9), J and the mesh points x, y, z.
     """
     h = 2 * math.pi / N
     Nt = N + 2 * NG
@@ -91,6 +91,7 @@ def tgv(N=64, mach=0.1, Re=1600.0, T0=300.0, noise=0.0, jump=0.0, seed=1):
     return dict(
         params=dict(bc=[0] * 6, viscous=1), dims=(N, N, N), Q=Q, metrics=metrics, J=J, h=h, U0=U0,
         rho0=rho0, p0=p0, T0=T0, dt_hint=dt_cfl,
+        x=np.ascontiguousarray(x), y=np.ascontiguousarray(y), z=np.ascontiguousarray(z),
     )
 
 

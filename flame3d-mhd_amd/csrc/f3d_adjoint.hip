@@ -483,8 +483,17 @@ dim3 grid_interior(const KP& kp) { return dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) 
 dim3 grid_all(const KP& kp) { return dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4, kp.Nzt); }
 const dim3 kB(64, 4, 1);
 
+int check_fdm(const f3d_params* p, const char* who) {
+  if (p->fvm != F3D_FDM) {
+    set_error("%s: the adjoint covers the finite-difference branch only (params fvm = %d)", who, p->fvm);
+    return F3D_E_ARG;
+  }
+  return F3D_OK;
+}
+
 int check_aws(const f3d_params* p, const f3d_adj_workspace* aws, const char* who) {
   int rc = f3d_params_check(p);
+  if (!rc) rc = check_fdm(p, who);
   if (rc) return rc;
   if (!aws) {
     set_error("%s: adjoint workspace is NULL", who);
@@ -607,6 +616,7 @@ size_t ckpt_slots(int nsteps, int every) { return (size_t)((nsteps + every - 1) 
 
 F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace** out) {
   int rc = f3d_params_check(p);
+  if (!rc) rc = check_fdm(p, "adjoint_workspace_create");
   if (rc) return rc;
   if (!out) {
     set_error("adjoint_workspace_create: out is NULL");

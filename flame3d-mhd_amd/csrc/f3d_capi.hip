@@ -37,6 +37,7 @@ KP make_kp(const f3d_params* p) {
   k.Nyt = p->Nyp + 2 * p->NG;
   k.Nzt = p->Nzp + 2 * p->NG;
   k.viscous = p->viscous;
+  k.fvm = p->fvm;
   for (int i = 0; i < 6; ++i) k.bc[i] = p->bc[i];
   k.nc = (long long)k.Nxt * k.Nyt * k.Nzt;
   k.gamma = p->gamma;
@@ -152,6 +153,10 @@ F3D_API int f3d_params_check(const f3d_params* p) {
       set_error("params: F3D_BC_INFLOW is only defined on the x-low face");
       return F3D_E_ARG;
     }
+  }
+  if (p->fvm < F3D_FDM || p->fvm > F3D_FVM_CONSER) {
+    set_error("params: fvm=%d unknown (F3D_FDM, F3D_FVM_EIGEN or F3D_FVM_CONSER)", p->fvm);
+    return F3D_E_ARG;
   }
   for (int d = 0; d < 3; ++d)
     if ((p->bc[2 * d] == F3D_BC_PERIODIC) != (p->bc[2 * d + 1] == F3D_BC_PERIODIC)) {
@@ -336,57 +341,98 @@ F3D_API int f3d_lin_comb(const f3d_params* p, double* U, const double* Un, int32
 CONV_IMPL(f64, double)
 CONV_IMPL(f32b, float)
 
-// flowAdvance, finite-difference branch  solver.jl:133-188
+// flowAdvance  solver.jl:19-188: finite-volume branch (solver.jl:21-31) or finite-difference branch
+// (solver.jl:133-184), then viscousFlux (solver.jl:185-187).
 template <typename T>
-static int flow_advance_impl(const f3d_params* p, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, T* Fvx, T* Fvy,
-                             T* Fvz, const T* s1, const T* s2, const T* s3, const MetricPtrs<T>& m,
-                             const double* J, T* phi, hipStream_t s) {
+static int flow_advance_impl(const f3d_params* p, const double* U, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, T* Fvx,
+                             T* Fvy, T* Fvz, const T* s1, const T* s2, const T* s3, const MetricPtrs<T>& m,
+                             const double* J, const double* x, const double* y, const double* z, T* phi,
+                             hipStream_t s) {
   KP kp = make_kp(p);
   const T* S[3] = {s1, s2, s3};
   T* F[3] = {Fx, Fy, Fz};
   for (int d = 0; d < 3; ++d) {
-    int rc = launch_flux_split<T>(kp, Q, Fp, Fm, S[d], m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
-    if (rc) return rc;
-    rc = launch_advect<T>(kp, d, F[d], phi, S[d], Fp, Fm, Q, m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
+    int rc;
+    if (kp.fvm) {
+      rc = launch_fvm_face<T>(kp, d, Q, U, phi, S[d], nullptr, nullptr, nullptr, F[d], x, y, z, s);
+    } else {
+      rc = launch_flux_split<T>(kp, Q, Fp, Fm, S[d], m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
+      if (!rc)
+        rc = launch_advect<T>(kp, d, F[d], phi, S[d], Fp, Fm, Q, m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
+    }
     if (rc) return rc;
   }
   if (p->viscous) return launch_viscous<T>(kp, Fvx, Fvy, Fvz, Q, m, J, s);
   return F3D_OK;
 }
 
-#define FA_IMPL(SUF, T)                                                                                       \
-  F3D_API int f3d_flow_advance_##SUF(                                                                         \
-      const f3d_params* p, double* U, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, T* Fvx, T* Fvy, T* Fvz,         \
-      const T* s1, const T* s2, const T* s3, const T* m0, const T* m1, const T* m2, const T* m3, const T* m4,  \
-      const T* m5, const T* m6, const T* m7, const T* m8, const double* J, const T* x, const T* y, const T* z, \
-      T* phi, f3d_stream_t s) {                                                                               \
-    (void)U;                                                                                                  \
-    (void)x;                                                                                                  \
-    (void)y;                                                                                                  \
-    (void)z;                                                                                                  \
-    CHECK_P();                                                                                                \
-    NEED(Q);                                                                                                  \
-    NEED(Fp);                                                                                                 \
-    NEED(Fm);                                                                                                 \
-    NEED(Fx);                                                                                                 \
-    NEED(Fy);                                                                                                 \
-    NEED(Fz);                                                                                                 \
-    NEED(s1);                                                                                                 \
-    NEED(s2);                                                                                                 \
-    NEED(s3);                                                                                                 \
-    NEED(phi);                                                                                                \
-    MetricPtrs<T> m{{m0, m1, m2, m3, m4, m5, m6, m7, m8}};                                                    \
-    for (int q = 0; q < 9; ++q)                                                                               \
-      if (!m.m[q]) {                                                                                          \
-        set_error("flow_advance: metric %d is NULL", q);                                                      \
-        return F3D_E_NULL;                                                                                    \
-      }                                                                                                       \
-    if (p->viscous && (!Fvx || !Fvy || !Fvz || !J)) {                                                         \
-      set_error("flow_advance: viscous needs Fv_x, Fv_y, Fv_z and J");                                        \
-      return F3D_E_NULL;                                                                                      \
-    }                                                                                                         \
-    return flow_advance_impl<T>(p, Q, Fp, Fm, Fx, Fy, Fz, Fvx, Fvy, Fvz, s1, s2, s3, m, J, phi,              \
-                                (hipStream_t)s);                                                              \
+#define FA_IMPL(SUF, T)                                                                                        \
+  F3D_API int f3d_flow_advance_##SUF(                                                                          \
+      const f3d_params* p, double* U, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, T* Fvx, T* Fvy, T* Fvz,          \
+      const T* s1, const T* s2, const T* s3, const T* m0, const T* m1, const T* m2, const T* m3, const T* m4,   \
+      const T* m5, const T* m6, const T* m7, const T* m8, const double* J, const double* x, const double* y,    \
+      const double* z, T* phi, f3d_stream_t s) {                                                               \
+    CHECK_P();                                                                                                 \
+    NEED(Q);                                                                                                   \
+    NEED(Fx);                                                                                                  \
+    NEED(Fy);                                                                                                  \
+    NEED(Fz);                                                                                                  \
+    NEED(s1);                                                                                                  \
+    NEED(s2);                                                                                                  \
+    NEED(s3);                                                                                                  \
+    NEED(phi);                                                                                                 \
+    if (p->fvm) {                                                                                              \
+      NEED(U);                                                                                                 \
+      NEED(x);                                                                                                 \
+      NEED(y);                                                                                                 \
+      NEED(z);                                                                                                 \
+    } else {                                                                                                   \
+      NEED(Fp);                                                                                                \
+      NEED(Fm);                                                                                                \
+    }                                                                                                          \
+    MetricPtrs<T> m{{m0, m1, m2, m3, m4, m5, m6, m7, m8}};                                                     \
+    for (int q = 0; q < 9; ++q)                                                                                \
+      if (!m.m[q] && (!p->fvm || p->viscous)) {                                                                \
+        set_error("flow_advance: metric %d is NULL", q);                                                       \
+        return F3D_E_NULL;                                                                                     \
+      }                                                                                                        \
+    if (p->viscous && (!Fvx || !Fvy || !Fvz || !J)) {                                                          \
+      set_error("flow_advance: viscous needs Fv_x, Fv_y, Fv_z and J");                                         \
+      return F3D_E_NULL;                                                                                       \
+    }                                                                                                          \
+    return flow_advance_impl<T>(p, U, Q, Fp, Fm, Fx, Fy, Fz, Fvx, Fvy, Fvz, s1, s2, s3, m, J, x, y, z, phi,    \
+                                (hipStream_t)s);                                                               \
   }
 FA_IMPL(f64, double)
 FA_IMPL(f32b, float)
+
+// Eigen_reconstruct_{i,j,k} / Conser_reconstruct_{i,j,k}  FVM.jl:6-670 / 672-1222.  The variant is
+// the entry point's, whatever p->fvm says.
+#define FVM_IMPL(NAME, SUF, T, D, EIG)                                                                         \
+  F3D_API int f3d_##NAME##_##SUF(const f3d_params* p, const T* Q, const double* U, const T* phi, const T* S,   \
+                                 T* F, const double* x, const double* y, const double* z, f3d_stream_t s) {   \
+    CHECK_P();                                                                                                 \
+    NEED(Q);                                                                                                   \
+    NEED(U);                                                                                                   \
+    NEED(phi);                                                                                                 \
+    NEED(S);                                                                                                   \
+    NEED(F);                                                                                                   \
+    NEED(x);                                                                                                   \
+    NEED(y);                                                                                                   \
+    NEED(z);                                                                                                   \
+    KP kp = make_kp(p);                                                                                        \
+    kp.fvm = EIG ? F3D_FVM_EIGEN : F3D_FVM_CONSER;                                                             \
+    return launch_fvm_face<T>(kp, D, Q, U, phi, S, nullptr, nullptr, nullptr, F, x, y, z, (hipStream_t)s);     \
+  }
+FVM_IMPL(eigen_reconstruct_i, f64, double, 0, 1)
+FVM_IMPL(eigen_reconstruct_j, f64, double, 1, 1)
+FVM_IMPL(eigen_reconstruct_k, f64, double, 2, 1)
+FVM_IMPL(conser_reconstruct_i, f64, double, 0, 0)
+FVM_IMPL(conser_reconstruct_j, f64, double, 1, 0)
+FVM_IMPL(conser_reconstruct_k, f64, double, 2, 0)
+FVM_IMPL(eigen_reconstruct_i, f32b, float, 0, 1)
+FVM_IMPL(eigen_reconstruct_j, f32b, float, 1, 1)
+FVM_IMPL(eigen_reconstruct_k, f32b, float, 2, 1)
+FVM_IMPL(conser_reconstruct_i, f32b, float, 0, 0)
+FVM_IMPL(conser_reconstruct_j, f32b, float, 1, 0)
+FVM_IMPL(conser_reconstruct_k, f32b, float, 2, 0)

@@ -18,6 +18,7 @@ struct KP {
   int Nxp, Nyp, Nzp, NG;
   int Nxt, Nyt, Nzt, viscous;
   int bc[6];
+  int fvm;       // F3D_FDM / F3D_FVM_EIGEN / F3D_FVM_CONSER
   long long nc;  // cells per variable, Nxt*Nyt*Nzt
   double gamma, Rg, Cp, Cs, Ts, Pr;
   double phi1, phi2, phi3, eps7, eps5, Twall;
@@ -129,7 +130,7 @@ F3D_HD void sw_split(const KP& kp, T rho, T u, T v, T w, T Tt, T A1, T A2, T A3,
 // ------------------------------------------------------------------ WENO kernels
 // Nonlinear WENO7 weights (schemes.jl:871-879).  Double: the literal form of the reference, whose
 // weights are Float64.  Float: the same normalised weights formed from ratios to the smallest
-// (eps + Is*s0) so that 1/(1e-20)^2 cannot overflow float.
+// |eps + Is*s0| so that 1/(1e-20)^2 cannot overflow float.
 template <typename T>
 struct W7;
 template <>
@@ -142,19 +143,42 @@ struct W7<double> {
     return inv * (a1 * q1 + a2 * q2 + a3 * q3 + a4 * q4);
   }
 };
+// Ratio bm/|b| of the range-safe float weights, bm the smallest |b| of the stencil (so the ratio lies
+// in [0, 1] even where rounding makes a smoothness indicator slightly negative); a vanishing |b|, whose
+// weight is unbounded, gets 1 and every other stencil 0.
+F3D_HD float wratio(float bm, float b) {
+  const float ab = fabsf(b);
+  return ab > 0.f ? bm / ab : 1.f;
+}
+
 template <>
 struct W7<float> {
   static F3D_HD float mix(float eps, float s0, float I1, float I2, float I3, float I4,
                                               float q1, float q2, float q3, float q4) {
     const float b1 = eps + I1 * s0, b2 = eps + I2 * s0, b3 = eps + I3 * s0, b4 = eps + I4 * s0;
-    const float bm = fminf(fminf(b1, b2), fminf(b3, b4));
-    const float r1 = bm / b1, r2 = bm / b2, r3 = bm / b3, r4 = bm / b4;
+    const float bm = fminf(fminf(fabsf(b1), fabsf(b2)), fminf(fabsf(b3), fabsf(b4)));
+    const float r1 = wratio(bm, b1), r2 = wratio(bm, b2), r3 = wratio(bm, b3), r4 = wratio(bm, b4);
     const float a1 = r1 * r1, a2 = 12.f * (r2 * r2), a3 = 18.f * (r3 * r3), a4 = 4.f * (r4 * r4);
     return (a1 * q1 + a2 * q2 + a3 * q3 + a4 * q4) / (a1 + a2 + a3 + a4);
   }
 };
 
+// WENO7 smoothness indicators (schemes.jl:848-869) in the float form: the same quadratic forms
+// regrouped in the first differences a_k = V_{k+1} - V_k (each indicator is invariant to adding a
+// constant to V; identical polynomials in rational arithmetic).  In float the V form loses smooth
+// stencils' indicators to the cancellation of O(17246 V^2) terms and the weights follow rounding noise.
+template <typename T>
+F3D_HD void weno7_is_diff(const T* V, T& I1, T& I2, T& I3, T& I4) {
+  const T a1 = V[1] - V[0], a2 = V[2] - V[1], a3 = V[3] - V[2], a4 = V[4] - V[3], a5 = V[5] - V[4],
+          a6 = V[6] - V[5];
+  I1 = a1 * (547 * a1 - 2788 * a2 + 1854 * a3) + a2 * (3708 * a2 - 5188 * a3) + a3 * (2107 * a3);
+  I2 = a2 * (267 * a2 - 1108 * a3 + 494 * a4) + a3 * (1468 * a3 - 1428 * a4) + a4 * (547 * a4);
+  I3 = a3 * (547 * a3 - 1428 * a4 + 494 * a5) + a4 * (1468 * a4 - 1108 * a5) + a5 * (267 * a5);
+  I4 = a4 * (2107 * a4 - 5188 * a5 + 1854 * a6) + a5 * (3708 * a5 - 2788 * a6) + a6 * (547 * a6);
+}
+
 // WENO7 of a 7-point stencil (schemes.jl:847-879), unscaled (the caller multiplies by 1/12).
+// FP64: the reference's indicators in V; float: weno7_is_diff.
 template <typename T>
 F3D_HD T weno7(const T* V, T s0, T eps) {
   const T V1 = V[0], V2 = V[1], V3 = V[2], V4 = V[3], V5 = V[4], V6 = V[5], V7 = V[6];
@@ -162,6 +186,11 @@ F3D_HD T weno7(const T* V, T s0, T eps) {
   const T q2 = V2 - 5 * V3 + 13 * V4 + 3 * V5;
   const T q3 = -V3 + 7 * V4 + 7 * V5 - V6;
   const T q4 = 3 * V4 + 13 * V5 - 5 * V6 + V7;
+  if (sizeof(T) == 4) {
+    T J1, J2, J3, J4;
+    weno7_is_diff(V, J1, J2, J3, J4);
+    return W7<T>::mix(eps, s0, J1, J2, J3, J4, q1, q2, q3, q4);
+  }
   const T I1 = V1 * (547 * V1 - 3882 * V2 + 4642 * V3 - 1854 * V4) + V2 * (7043 * V2 - 17246 * V3 + 7042 * V4) +
                V3 * (11003 * V3 - 9402 * V4) + V4 * (2107 * V4);
   const T I2 = V2 * (267 * V2 - 1642 * V3 + 1602 * V4 - 494 * V5) + V3 * (2843 * V3 - 5966 * V4 + 1922 * V5) +

@@ -814,14 +814,18 @@ static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, doubl
 }
 
 template <typename T>
+void fused_sensor(const KP& kp, const WsView<T>& ws, const T* Q, T* phi, hipStream_t s) {
+  TimedScope ts("fused_sensor", s);
+  hipLaunchKernelGGL(k_sensor_f<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, kp.Nzt - 2), dim3(64, 4, 1), 0,
+                     s, kp, phi, Q, ws.hcount);
+}
+
+// Inviscid increments of the finite-difference branch into ws.R: x, y, z sweeps and the heavy-face
+// corrections (after fused_sensor).
+template <typename T>
 static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J,
                       T* phi, hipStream_t s) {
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
-  {
-    TimedScope ts("fused_sensor", s);
-    hipLaunchKernelGGL(k_sensor_f<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, kp.Nzt - 2), dim3(64, 4, 1),
-                       0, s, kp, phi, Q, ws.hcount);
-  }
   {
     TimedScope ts("fused_sweep_x", s);
     const int L = seg_len(kp.Nxp, XW - 8);
@@ -844,32 +848,58 @@ static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const Metri
     TimedScope ts("fused_heavy", s);
     hipLaunchKernelGGL(k_heavy<T>, dim3(2048), dim3(64), 0, s, kp, a);
   }
+  return check_launch("fused sweeps");
+}
+
+template <typename T>
+int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double* Un, const T* Qin, T* Qout,
+                 const MetricPtrs<T>& m, const double* J, const T* phi, double* Rout, double dt, hipStream_t s) {
+  if (mode != 0 && Qin == Qout && stage_needs_separate_q(kp)) {
+    set_error("fused stage: the output state must not alias the input state");
+    return F3D_E_ARG;
+  }
+  FArgs<T> a{Qin, m, J, phi, ws.R, ws.heavy, ws.hcount};
+  const int rm = xcd_remap();
   if (kp.viscous && !visc_fused()) {
     TimedScope ts("fused_viscous", s);
     const int KC = seg_len(kp.Nzp + 4, 32);
     hipLaunchKernelGGL(k_viscous_march<T>, dim3((kp.Nxp + 4 + 63) / 64, (kp.Nyp + 4 + 3) / 4, (kp.Nzp + 4 + KC - 1) / KC),
-                       dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC, xcd_remap());
+                       dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC, rm);
   }
-  return check_launch("fused sweeps");
+  TimedScope ts("fused_update", s);
+  if (kp.viscous && visc_fused()) {
+    const int KV = visc_kc(kp, VYI);
+    if (mode == 0)
+      launch_visc_update<T, 0>(kp, a, nullptr, nullptr, nullptr, Rout, 0.0, KV, rm, s);
+    else if (mode == 1)
+      launch_visc_update<T, 1>(kp, a, U, Un, Qout, nullptr, dt, KV, rm, s);
+    else if (mode == 2)
+      launch_visc_update<T, 2>(kp, a, U, Un, Qout, nullptr, dt, KV, rm, s);
+    else
+      launch_visc_update<T, 3>(kp, a, U, Un, Qout, nullptr, dt, KV, rm, s);
+    return check_launch("fused update");
+  }
+  const int KC = seg_len(kp.Nzp, 16);
+  const dim3 g((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC), b(64, 4, 1);
+  if (mode == 0)
+    hipLaunchKernelGGL((k_update<T, 0>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, nullptr, nullptr, nullptr, Rout,
+                       0.0, KC, rm);
+  else if (mode == 1)
+    hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Qout, nullptr, dt, KC, rm);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Qout, nullptr, dt, KC, rm);
+  else
+    hipLaunchKernelGGL((k_update<T, 3>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Qout, nullptr, dt, KC, rm);
+  return check_launch("fused update");
 }
 
 template <typename T>
 int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J, T* phi,
               double* R, hipStream_t s) {
+  fused_sensor<T>(kp, ws, Q, phi, s);
   int rc = fused_flux<T>(kp, ws, Q, m, J, phi, s);
   if (rc) return rc;
-  FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
-  TimedScope ts("fused_update", s);
-  if (kp.viscous && visc_fused()) {
-    const int KC = visc_kc(kp, VYI);
-    launch_visc_update<T, 0>(kp, a, nullptr, nullptr, nullptr, R, 0.0, KC, xcd_remap(), s);
-    return check_launch("fused rhs update");
-  }
-  const int KC = seg_len(kp.Nzp, 16);
-  hipLaunchKernelGGL((k_update<T, 0>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC),
-                     dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, nullptr, nullptr, nullptr, R, 0.0, KC,
-                     xcd_remap());
-  return check_launch("fused rhs update");
+  return fused_update<T>(kp, ws, 0, nullptr, nullptr, Q, nullptr, m, J, phi, R, 0.0, s);
 }
 
 bool stage_needs_separate_q(const KP& kp) { return use_fused() && kp.viscous && visc_fused(); }
@@ -881,31 +911,10 @@ int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* U
     set_error("fused stage: the output state must not alias the input state");
     return F3D_E_ARG;
   }
+  fused_sensor<T>(kp, ws, Qin, ws.phi, s);
   int rc = fused_flux<T>(kp, ws, Qin, m, J, ws.phi, s);
   if (rc) return rc;
-  FArgs<T> a{Qin, m, J, ws.phi, ws.R, ws.heavy, ws.hcount};
-  T* Q = Qout;
-  const int KC = seg_len(kp.Nzp, 16);
-  const dim3 g((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + KC - 1) / KC), b(64, 4, 1);
-  TimedScope ts("fused_update", s);
-  const int rm = xcd_remap();
-  if (kp.viscous && visc_fused()) {
-    const int KV = visc_kc(kp, VYI);
-    if (KRK == 1)
-      launch_visc_update<T, 1>(kp, a, U, Un, Q, nullptr, dt, KV, rm, s);
-    else if (KRK == 2)
-      launch_visc_update<T, 2>(kp, a, U, Un, Q, nullptr, dt, KV, rm, s);
-    else
-      launch_visc_update<T, 3>(kp, a, U, Un, Q, nullptr, dt, KV, rm, s);
-    return check_launch("fused stage update");
-  }
-  if (KRK == 1)
-    hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC, rm);
-  else if (KRK == 2)
-    hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC, rm);
-  else
-    hipLaunchKernelGGL((k_update<T, 3>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, U, Un, Q, nullptr, dt, KC, rm);
-  return check_launch("fused stage update");
+  return fused_update<T>(kp, ws, KRK, U, Un, Qin, Qout, m, J, ws.phi, nullptr, dt, s);
 }
 
 template int fused_rhs<double>(const KP&, const WsView<double>&, const double*, const MetricPtrs<double>&,
@@ -916,4 +925,11 @@ template int fused_stage<double>(const KP&, const WsView<double>&, int, double*,
                                  const MetricPtrs<double>&, const double*, double, hipStream_t);
 template int fused_stage<float>(const KP&, const WsView<float>&, int, double*, double*, const float*, float*,
                                 const MetricPtrs<float>&, const double*, double, hipStream_t);
+template void fused_sensor<double>(const KP&, const WsView<double>&, const double*, double*, hipStream_t);
+template void fused_sensor<float>(const KP&, const WsView<float>&, const float*, float*, hipStream_t);
+template int fused_update<double>(const KP&, const WsView<double>&, int, double*, double*, const double*, double*,
+                                  const MetricPtrs<double>&, const double*, const double*, double*, double,
+                                  hipStream_t);
+template int fused_update<float>(const KP&, const WsView<float>&, int, double*, double*, const float*, float*,
+                                 const MetricPtrs<float>&, const double*, const float*, double*, double, hipStream_t);
 }  // namespace f3d

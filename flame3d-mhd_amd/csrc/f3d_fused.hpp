@@ -78,11 +78,34 @@ int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* U
                 const MetricPtrs<T>& m, const double* J, double dt, hipStream_t s);
 bool stage_needs_separate_q(const KP& kp);
 
+// Building blocks shared by the two inviscid branches.  fused_sensor: shock sensor into phi (and the
+// heavy-face counter cleared).  fused_update: viscous divergence + conservative update from the
+// inviscid increments in ws.R; mode 0 writes the RHS into Rout, mode KRK = 1..3 performs the stage
+// update (+ RK3 blend + c2Prim of the interior into Qout).
+template <typename T>
+void fused_sensor(const KP& kp, const WsView<T>& ws, const T* Q, T* phi, hipStream_t s);
+template <typename T>
+int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double* Un, const T* Qin, T* Qout,
+                 const MetricPtrs<T>& m, const double* J, const T* phi, double* Rout, double dt, hipStream_t s);
+
+// Finite-volume branch (f3d_fvm.hip): RHS and stage as fused_rhs / fused_stage, reconstructing the
+// conserved state U (ghost cells included) with the face geometry of the mesh points coords[3].
+template <typename T>
+int fvm_fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
+                  const double* J, const double* const* coords, T* phi, double* R, hipStream_t s);
+template <typename T>
+int fvm_fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
+                    const MetricPtrs<T>& m, const double* J, const double* const* coords, double dt, hipStream_t s);
+
 // Stage / multi-step drivers of f3d_stage.hip (with the ghost procedure of the output state).
+// coords == nullptr: finite-difference branch; otherwise finite-volume (p->fvm) with these mesh points,
+// and the ghost procedure refills U's ghost cells as well.
 template <typename T>
 int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
-             const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s);
+             const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
+             const double* const* coords = nullptr);
 template <typename T>
 int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, double* Un, T* Q,
-             const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s);
+             const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
+             const double* const* coords = nullptr);
 }  // namespace f3d

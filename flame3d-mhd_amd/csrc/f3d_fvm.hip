@@ -1,0 +1,388 @@
+// Finite-volume branch of flowAdvance (solver.jl:21-31; FVM.jl) for gfx950.
+//
+// Kernel-by-kernel (Eigen_/Conser_reconstruct_{i,j,k}, flowAdvance):
+//   k_fvm_face       one lane per face: 8-cell stencil of U, sensor window, geometry from the mesh
+//                    points, states (UP7 / WENO7 / WENO5 / minmod, characteristic or component-wise),
+//                    HLLC, times the face area, into the reference's face array.
+// Fused RHS / RK3 stage (the same split as the finite-difference fused path):
+//   fused_sensor     shock sensor (f3d_fused.hip), heavy-face counter cleared
+//   k_fvm_sweep_x    x faces of a 120-cell row segment staged in LDS (U, sensor, per-cell halves of
+//                    the face diagonals): UP7 states + HLLC for the faces the sensor leaves linear,
+//                    R = dF_x; other faces contribute zero and go to the compacted heavy list
+//   k_fvm_march      y and z faces: lanes march along the direction with the last 8 cells' U in a
+//                    register window and the previous cell's diagonal halves; R += dF_y, dF_z
+//   k_fvm_heavy      the listed faces: full characteristic / component-wise reconstruction + HLLC,
+//                    added to the two cells with atomics
+//   fused_update     viscous divergence + update + RK3 blend + c2Prim (f3d_fused.hip)
+#include <cstdlib>
+#include <cstring>
+
+#include "f3d_fused.hpp"
+#include "f3d_fvm.hpp"
+
+namespace f3d {
+
+namespace {
+
+constexpr int kXW = 128;  // lanes per x row segment (cells incl. the 4+4 halo)
+
+template <typename T>
+struct FvmArgs {
+  const double* U;
+  const double* x;
+  const double* y;
+  const double* z;
+  const T* phi;
+  T* R;
+  unsigned long long* heavy;
+  unsigned int* hcount;
+};
+
+__device__ __forceinline__ long long rix(const KP& kp, int i, int j, int k) {
+  // interior RHS layout [5][Nzp][Nyp][Nxp], 0-based interior indices
+  return (long long)i + (long long)kp.Nxp * ((long long)j + (long long)kp.Nyp * k);
+}
+
+template <typename T>
+__device__ __forceinline__ T row_norm(const T* __restrict__ A1, const T* __restrict__ A2, const T* __restrict__ A3,
+                                      long long o) {
+  const T a = A1[o], b = A2[o], c = A3[o];
+  return sqrt(a * a + b * b + c * c);
+}
+
+// ------------------------------------------------------------------ one lane per face
+template <typename T, int D, bool EIGEN, bool SFROM_A>
+__global__ __launch_bounds__(256) void k_fvm_face(KP kp, int ni, int nj, int nk, const T* __restrict__ Q,
+                                                  const double* __restrict__ U, const T* __restrict__ phi,
+                                                  const T* __restrict__ S, const T* __restrict__ Ax,
+                                                  const T* __restrict__ Ay, const T* __restrict__ Az,
+                                                  T* __restrict__ F, const double* __restrict__ x,
+                                                  const double* __restrict__ y, const double* __restrict__ z) {
+  const int ii = blockIdx.x * 64 + threadIdx.x, jj = blockIdx.y * 4 + threadIdx.y, kk = blockIdx.z;
+  if (ii >= ni || jj >= nj || kk >= nk) return;
+  // face (ii, jj, kk) of direction D lies between cells (left) NG+idx-1 and NG+idx along D
+  const int i = kp.NG + ii - (D == 0), j = kp.NG + jj - (D == 1), k = kp.NG + kk - (D == 2);
+  const FvmDir dd = fvm_dir(kp, D);
+  const long long o = cidx(kp, i, j, k);
+  const int sch = fvm_scheme<T>(kp, phi, o, dd.st);
+  T Sc = 0, Sc1 = 0;
+  if (sch != F3D_SCHEME_UP7) {
+    Sc = SFROM_A ? row_norm(Ax, Ay, Az, o) : S[o];
+    Sc1 = SFROM_A ? row_norm(Ax, Ay, Az, o + dd.st) : S[o + dd.st];
+  }
+  T out[5];
+  fvm_face_flux<T, EIGEN>(kp, sch, Q, U, x, y, z, o, dd, Sc, Sc1, out);
+  const long long fo = ii + (long long)ni * (jj + (long long)nj * kk), fn = (long long)ni * nj * nk;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) F[fo + n * fn] = out[n];
+}
+
+// ------------------------------------------------------------------ fused x sweep (LDS rows)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L) {
+  __shared__ T sU[5][2][kXW];
+  __shared__ T sphi[2][kXW];
+  __shared__ double sP[6][2][kXW];
+  __shared__ T sF[5][2][kXW];
+  const int l = threadIdx.x, r = threadIdx.y;
+  const int jj = blockIdx.y * 2 + r, kk = blockIdx.z;
+  const bool rowok = jj < kp.Nyp;
+  const int j = kp.NG + (rowok ? jj : kp.Nyp - 1), k = kp.NG + kk;
+  const int f0 = blockIdx.x * L;
+  const int ncell = min(L, kp.Nxp - f0);
+  const int nface = ncell + 1;
+  const int xc = kp.NG + f0 - 4 + l;  // cell this lane stages
+  const long long nc = kp.nc;
+  const FvmDir dd = fvm_dir(kp, 0);
+  if (l < nface + 7) {
+    const long long o = cidx(kp, xc, j, k);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) sU[n][r][l] = (T)a.U[o + n * nc];
+    sphi[r][l] = a.phi[o];
+    if (l >= 3 && l < nface + 4) {  // left or right cell of a face of this segment
+      double P[6];
+      fvm_partials(a.x, a.y, a.z, o, dd.sa, dd.sb, P);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) sP[q][r][l] = P[q];
+    }
+  }
+  __syncthreads();
+  // lane l handles the face between cells l-1 and l of the segment (face index f)
+  const bool fact = l >= 4 && l < 4 + nface;
+  const int f = f0 + l - 4;
+  int sch = F3D_SCHEME_UP7;
+  if (fact) {
+    T m = sphi[r][l - 3];
+#pragma unroll
+    for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
+    sch = scheme_of<T>(kp, m);
+    T Fl[5] = {0, 0, 0, 0, 0};
+    if (sch == F3D_SCHEME_UP7) {
+      T W[8][5];
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) W[s][n] = sU[n][r][l - 4 + s];
+      T UL[5], UR[5];
+      fvm_up7_states<T>(kp, W, UL, UR);
+      double Pc[6], Pn[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        Pc[q] = sP[q][r][l - 1];
+        Pn[q] = sP[q][r][l];
+      }
+      T nv[3], A;
+      fvm_geom<T>(Pc, Pn, dd.flip, nv, &A);
+      fvm_hllc<T>(kp, UL, UR, nv[0], nv[1], nv[2], Fl);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) Fl[n] *= A;
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) sF[n][r][l] = Fl[n];
+  }
+  const bool owned = rowok && fact && (f < f0 + L || f == kp.Nxp);
+  append_heavy(a.heavy, a.hcount, owned && sch != F3D_SCHEME_UP7,
+               ((unsigned long long)cidx(kp, kp.NG + f, j, k) << 4) | ((unsigned long long)sch << 2) | 0ULL);
+  __syncthreads();
+  if (rowok && l >= 4 && l < 4 + ncell) {
+    const long long ro = rix(kp, f, jj, kk), rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) a.R[ro + n * rn] = sF[n][r][l] - sF[n][r][l + 1];
+  }
+}
+
+// ------------------------------------------------------------------ fused y / z march
+// Lane = interior i (64 consecutive), threadIdx.y = transverse line; a block marches a segment of
+// L cells along D.  W[s] holds U of the last 8 cells streamed; once 8 are in, W[0..7] is the stencil
+// c-3..c+4 of the face between W[3] and W[4] (face f = f0 + step - 7).  The diagonal halves of W[4]
+// are formed as it enters position 4 and kept for the next face, so each cell's mesh neighbourhood
+// is read once per march.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
+  const int ii = blockIdx.x * 64 + threadIdx.x;
+  const int tt = blockIdx.y * 4 + threadIdx.y;
+  const int Nt2 = D == 1 ? kp.Nzp : kp.Nyp;
+  const int Np = D == 1 ? kp.Nyp : kp.Nzp;
+  const bool act = ii < kp.Nxp && tt < Nt2;
+  const int i = kp.NG + min(ii, kp.Nxp - 1), t = kp.NG + min(tt, Nt2 - 1);
+  const int f0 = blockIdx.z * L;
+  const int ncell = min(L, Np - f0);
+  const int nstep = ncell + 8;
+  const FvmDir dd = fvm_dir(kp, D);
+  const long long st = dd.st, nc = kp.nc;
+  const long long obase = (D == 1 ? cidx(kp, i, 0, t) : cidx(kp, i, t, 0)) + (long long)(kp.NG + f0 - 4) * st;
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  T W[8][5];
+  T ph[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    ph[s] = 0;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) W[s][n] = 0;
+  }
+  double Pp[6] = {0, 0, 0, 0, 0, 0};
+  T Fprev[5] = {0, 0, 0, 0, 0};
+  double nu[5];
+  T nph;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) nu[n] = a.U[obase + n * nc];
+  nph = a.phi[obase];
+#pragma unroll 1
+  for (int step = 0; step < nstep; ++step) {
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+      ph[s] = ph[s + 1];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) W[s][n] = W[s + 1][n];
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) W[7][n] = (T)nu[n];
+    ph[7] = nph;
+    if (step + 1 < nstep) {
+      const long long o = obase + (long long)(step + 1) * st;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) nu[n] = a.U[o + n * nc];
+      nph = a.phi[o];
+    }
+    if (step >= 6) {
+      double Pn[6];
+      fvm_partials(a.x, a.y, a.z, obase + (long long)(step - 3) * st, dd.sa, dd.sb, Pn);
+      if (step >= 7) {  // face f0 + step - 7 complete
+        const int fi = step - 7;
+        const int f = f0 + fi;
+        T m = ph[1];
+#pragma unroll
+        for (int s = 2; s <= 6; ++s) m = jmax(m, ph[s]);
+        const int sch = scheme_of<T>(kp, m);
+        T Fc[5] = {0, 0, 0, 0, 0};
+        if (sch == F3D_SCHEME_UP7) {
+          T UL[5], UR[5];
+          fvm_up7_states<T>(kp, W, UL, UR);
+          T nv[3], A;
+          fvm_geom<T>(Pp, Pn, dd.flip, nv, &A);
+          fvm_hllc<T>(kp, UL, UR, nv[0], nv[1], nv[2], Fc);
+#pragma unroll
+          for (int n = 0; n < 5; ++n) Fc[n] *= A;
+        }
+        append_heavy(a.heavy, a.hcount, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
+                     ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) |
+                         ((unsigned long long)sch << 2) | (unsigned long long)D);
+        if (fi >= 1 && act) {
+          const long long ro = D == 1 ? rix(kp, ii, f - 1, tt) : rix(kp, ii, tt, f - 1);
+#pragma unroll
+          for (int n = 0; n < 5; ++n) a.R[ro + n * rn] += Fprev[n] - Fc[n];
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) Fprev[n] = Fc[n];
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Pp[q] = Pn[q];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ heavy faces
+template <typename T, bool EIGEN>
+__global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* __restrict__ Q, MetricPtrs<T> m) {
+  const unsigned int n = *a.hcount;
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  for (unsigned int e = blockIdx.x * 64 + threadIdx.x; e < n; e += gridDim.x * 64) {
+    const unsigned long long v = a.heavy[e];
+    const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
+    const long long o = (long long)(v >> 4);  // right cell of the face
+    const FvmDir dd = fvm_dir(kp, D);
+    const long long oc = o - dd.st;
+    const T Sc = row_norm(m.m[3 * D], m.m[3 * D + 1], m.m[3 * D + 2], oc);
+    const T Sc1 = row_norm(m.m[3 * D], m.m[3 * D + 1], m.m[3 * D + 2], o);
+    T F[5];
+    fvm_face_flux<T, EIGEN>(kp, sch, Q, a.U, a.x, a.y, a.z, oc, dd, Sc, Sc1, F);
+    const int ci = (int)(o % kp.Nxt), cj = (int)((o / kp.Nxt) % kp.Nyt), ck = (int)(o / ((long long)kp.Nxt * kp.Nyt));
+    const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
+    const int f = D == 0 ? I : (D == 1 ? Jj : K);
+    const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+    const long long rr = rix(kp, I, Jj, K);
+    const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      if (f <= Np - 1) atomicAdd(&a.R[rr + q * rn], F[q]);  // left face of cell f
+      if (f >= 1) atomicAdd(&a.R[rl + q * rn], -F[q]);      // right face of cell f-1
+    }
+  }
+}
+
+int seg_len(int Np, int max_len) {
+  const int nseg = (Np + max_len - 1) / max_len;
+  return (Np + nseg - 1) / nseg;
+}
+
+// Segment length of the y / z marches; F3D_FVM_SEG_Y / F3D_FVM_SEG_Z override.
+int fvm_seg(int D) {
+  static int v[2] = {-1, -1};
+  int& r = v[D - 1];
+  if (r < 0) {
+    const char* e = std::getenv(D == 1 ? "F3D_FVM_SEG_Y" : "F3D_FVM_SEG_Z");
+    const int def = D == 1 ? 32 : 16;
+    r = e ? std::atoi(e) : def;
+    if (r < 8) r = def;
+  }
+  return r;
+}
+
+template <typename T>
+int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
+             const double* const* coords, const T* phi, hipStream_t s) {
+  FvmArgs<T> a{U, coords[0], coords[1], coords[2], phi, ws.R, ws.heavy, ws.hcount};
+  {
+    TimedScope ts("fvm_sweep_x", s);
+    const int L = seg_len(kp.Nxp, kXW - 8);
+    hipLaunchKernelGGL(k_fvm_sweep_x<T>, dim3((kp.Nxp + L - 1) / L, (kp.Nyp + 1) / 2, kp.Nzp), dim3(kXW, 2, 1), 0, s,
+                       kp, a, L);
+  }
+  {
+    TimedScope ts("fvm_sweep_y", s);
+    const int L = seg_len(kp.Nyp, fvm_seg(1));
+    hipLaunchKernelGGL((k_fvm_march<T, 1>), dim3((kp.Nxp + 63) / 64, (kp.Nzp + 3) / 4, (kp.Nyp + L - 1) / L),
+                       dim3(64, 4, 1), 0, s, kp, a, L);
+  }
+  {
+    TimedScope ts("fvm_sweep_z", s);
+    const int L = seg_len(kp.Nzp, fvm_seg(2));
+    hipLaunchKernelGGL((k_fvm_march<T, 2>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + L - 1) / L),
+                       dim3(64, 4, 1), 0, s, kp, a, L);
+  }
+  {
+    TimedScope ts("fvm_heavy", s);
+    if (kp.fvm == F3D_FVM_EIGEN)
+      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(2048), dim3(64), 0, s, kp, a, Q, m);
+    else
+      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(2048), dim3(64), 0, s, kp, a, Q, m);
+  }
+  return check_launch("fvm sweeps");
+}
+
+}  // namespace
+
+template <typename T>
+int launch_fvm_face(const KP& kp, int d, const T* Q, const double* U, const T* phi, const T* S, const T* Ax,
+                    const T* Ay, const T* Az, T* F, const double* x, const double* y, const double* z, hipStream_t s) {
+  TimedScope ts_("fvm_reconstruct", s);
+  const int ni = kp.Nxp + (d == 0), nj = kp.Nyp + (d == 1), nk = kp.Nzp + (d == 2);
+  const dim3 g((ni + 63) / 64, (nj + 3) / 4, nk), b(64, 4, 1);
+  const bool eig = kp.fvm == F3D_FVM_EIGEN;
+#define F3D_FVMF(D, E)                                                                                          \
+  do {                                                                                                          \
+    if (S)                                                                                                      \
+      hipLaunchKernelGGL((k_fvm_face<T, D, E, false>), g, b, 0, s, kp, ni, nj, nk, Q, U, phi, S, Ax, Ay, Az, F, \
+                         x, y, z);                                                                              \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_fvm_face<T, D, E, true>), g, b, 0, s, kp, ni, nj, nk, Q, U, phi, S, Ax, Ay, Az, F,  \
+                         x, y, z);                                                                              \
+  } while (0)
+  if (d == 0) {
+    if (eig) F3D_FVMF(0, true);
+    else F3D_FVMF(0, false);
+  } else if (d == 1) {
+    if (eig) F3D_FVMF(1, true);
+    else F3D_FVMF(1, false);
+  } else {
+    if (eig) F3D_FVMF(2, true);
+    else F3D_FVMF(2, false);
+  }
+#undef F3D_FVMF
+  return check_launch("fvm_reconstruct");
+}
+
+template <typename T>
+int fvm_fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
+                  const double* J, const double* const* coords, T* phi, double* R, hipStream_t s) {
+  fused_sensor<T>(kp, ws, Q, phi, s);
+  int rc = fvm_flux<T>(kp, ws, Q, U, m, coords, phi, s);
+  if (rc) return rc;
+  return fused_update<T>(kp, ws, 0, nullptr, nullptr, Q, nullptr, m, J, phi, R, 0.0, s);
+}
+
+template <typename T>
+int fvm_fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
+                    const MetricPtrs<T>& m, const double* J, const double* const* coords, double dt, hipStream_t s) {
+  if (Qin == Qout && stage_needs_separate_q(kp)) {
+    set_error("fused stage: the output state must not alias the input state");
+    return F3D_E_ARG;
+  }
+  fused_sensor<T>(kp, ws, Qin, ws.phi, s);
+  int rc = fvm_flux<T>(kp, ws, Qin, U, m, coords, ws.phi, s);
+  if (rc) return rc;
+  return fused_update<T>(kp, ws, KRK, U, Un, Qin, Qout, m, J, ws.phi, nullptr, dt, s);
+}
+
+#define F3D_FVM_INST(T)                                                                                           \
+  template int launch_fvm_face<T>(const KP&, int, const T*, const double*, const T*, const T*, const T*,         \
+                                  const T*, const T*, T*, const double*, const double*, const double*,            \
+                                  hipStream_t);                                                                   \
+  template int fvm_fused_rhs<T>(const KP&, const WsView<T>&, const T*, const double*, const MetricPtrs<T>&,       \
+                                const double*, const double* const*, T*, double*, hipStream_t);                   \
+  template int fvm_fused_stage<T>(const KP&, const WsView<T>&, int, double*, double*, const T*, T*,              \
+                                  const MetricPtrs<T>&, const double*, const double* const*, double, hipStream_t);
+F3D_FVM_INST(double)
+F3D_FVM_INST(float)
+
+}  // namespace f3d

@@ -65,4 +65,10 @@ template <typename T> int launch_prim2c(const KP&, double* U, const T* Q, hipStr
 template <typename T> int launch_exchange(const KP&, T* Q, int NV, hipStream_t);
 template <typename T> int launch_fill(const KP&, T* Q, double* U, const T* inflow, hipStream_t);
 
+// finite-volume face fluxes of direction d (f3d_fvm.hip), variant kp.fvm; S == nullptr: |grad xi_d|
+// from the metric row A (Ax, Ay, Az)
+template <typename T>
+int launch_fvm_face(const KP&, int d, const T* Q, const double* U, const T* phi, const T* S, const T* Ax,
+                    const T* Ay, const T* Az, T* F, const double* x, const double* y, const double* z, hipStream_t);
+
 }  // namespace f3d

@@ -67,13 +67,22 @@ typedef struct ihipStream_t* f3d_stream_t; /* == hipStream_t */
 #define F3D_SCHEME_WENO5 2
 #define F3D_SCHEME_NND2 3
 
+/* Inviscid discretisation (run.jl:64-65 finite_volume / eigen_reconstruction; solver.jl:21-31).
+ * F3D_FDM: Steger-Warming flux-vector splitting with hybrid characteristic reconstruction of the
+ * split fluxes (schemes.jl, split.jl).  F3D_FVM_*: reconstruction of the conserved state U to both
+ * sides of each face, HLLC flux times the face area from the node coordinates (FVM.jl), with the
+ * same sensor thresholds (the non-UP7 branches use WENO epsilons 1e-10 / 1e-8, FVM.jl:91). */
+#define F3D_FDM 0
+#define F3D_FVM_EIGEN 1   /* characteristic (Roe-frame) reconstruction, FVM.jl:6-670 */
+#define F3D_FVM_CONSER 2  /* component-wise reconstruction of U, FVM.jl:672-1222 */
+
 /* Runtime form of the reference's compile-time constants (run.jl:8-13, 57-77, 80-86). */
 typedef struct f3d_params {
   int32_t Nxp, Nyp, Nzp;   /* interior cells (run.jl:84-86) */
   int32_t NG;              /* ghost layers; must be 4 (run.jl:80; viscous.jl:119 and div.jl:14-39 assume it) */
   int32_t viscous;         /* run.jl:61 */
   int32_t bc[6];           /* F3D_BC_* for xlo,xhi,ylo,yhi,zlo,zhi */
-  int32_t pad_;
+  int32_t fvm;             /* F3D_FDM / F3D_FVM_EIGEN / F3D_FVM_CONSER (run.jl:64-65) */
   double gamma, Rg, Cp, C_s, T_s, Pr;          /* run.jl:8-13 */
   double hybrid_phi1, hybrid_phi2, hybrid_phi3; /* run.jl:70-72 */
   double Linear_phi;                            /* run.jl:73: Linear = UP7*Linear_phi + CD6*(1-Linear_phi) */
@@ -179,9 +188,59 @@ int f3d_fill_ghost_f64(const f3d_params* p, double* Q, double* U, const double* 
 int f3d_fill_ghost_f32b(const f3d_params* p, float* Q, double* U, const float* inflow,
                         f3d_stream_t s);
 
+/* ---------------- finite-volume branch (FVM.jl; selected by p->fvm, solver.jl:21-31) ----------
+ * One entry per reference kernel: face fluxes F (Fx [Nxp+1,Nyp,Nzp,5], Fy, Fz as for advect_*) =
+ * HLLC flux (FVM.jl:1224-1358) of the states reconstructed to both sides of the face, times the
+ * face area.  U: conserved state with its ghost cells filled (fillGhost fills U as well as Q,
+ * boundary.jl:1-156); Q: primitive state (the Roe-averaged frame of the eigen variant); phi: shock
+ * sensor; S: |grad xi_d| of the direction (s1, s2 or s3).  x, y, z: the mesh points [Nxt,Nyt,Nzt]
+ * the metrics are computed from, double in both numeric forms: the face normal and area are formed
+ * in FP64 and rounded to the bulk type (the near-wall spacing of a stretched mesh is below float's
+ * resolution of the coordinates themselves).
+ *   eigen:  characteristic reconstruction in the Roe frame of cells i, i+1   FVM.jl:6-670
+ *   conser: component-wise reconstruction of U                               FVM.jl:672-1222 */
+int f3d_eigen_reconstruct_i_f64(const f3d_params* p, const double* Q, const double* U,
+                                const double* phi, const double* S, double* Fx, const double* x,
+                                const double* y, const double* z, f3d_stream_t s);
+int f3d_eigen_reconstruct_j_f64(const f3d_params* p, const double* Q, const double* U,
+                                const double* phi, const double* S, double* Fy, const double* x,
+                                const double* y, const double* z, f3d_stream_t s);
+int f3d_eigen_reconstruct_k_f64(const f3d_params* p, const double* Q, const double* U,
+                                const double* phi, const double* S, double* Fz, const double* x,
+                                const double* y, const double* z, f3d_stream_t s);
+int f3d_conser_reconstruct_i_f64(const f3d_params* p, const double* Q, const double* U,
+                                 const double* phi, const double* S, double* Fx, const double* x,
+                                 const double* y, const double* z, f3d_stream_t s);
+int f3d_conser_reconstruct_j_f64(const f3d_params* p, const double* Q, const double* U,
+                                 const double* phi, const double* S, double* Fy, const double* x,
+                                 const double* y, const double* z, f3d_stream_t s);
+int f3d_conser_reconstruct_k_f64(const f3d_params* p, const double* Q, const double* U,
+                                 const double* phi, const double* S, double* Fz, const double* x,
+                                 const double* y, const double* z, f3d_stream_t s);
+int f3d_eigen_reconstruct_i_f32b(const f3d_params* p, const float* Q, const double* U,
+                                 const float* phi, const float* S, float* Fx, const double* x,
+                                 const double* y, const double* z, f3d_stream_t s);
+int f3d_eigen_reconstruct_j_f32b(const f3d_params* p, const float* Q, const double* U,
+                                 const float* phi, const float* S, float* Fy, const double* x,
+                                 const double* y, const double* z, f3d_stream_t s);
+int f3d_eigen_reconstruct_k_f32b(const f3d_params* p, const float* Q, const double* U,
+                                 const float* phi, const float* S, float* Fz, const double* x,
+                                 const double* y, const double* z, f3d_stream_t s);
+int f3d_conser_reconstruct_i_f32b(const f3d_params* p, const float* Q, const double* U,
+                                  const float* phi, const float* S, float* Fx, const double* x,
+                                  const double* y, const double* z, f3d_stream_t s);
+int f3d_conser_reconstruct_j_f32b(const f3d_params* p, const float* Q, const double* U,
+                                  const float* phi, const float* S, float* Fy, const double* x,
+                                  const double* y, const double* z, f3d_stream_t s);
+int f3d_conser_reconstruct_k_f32b(const f3d_params* p, const float* Q, const double* U,
+                                  const float* phi, const float* S, float* Fz, const double* x,
+                                  const double* y, const double* z, f3d_stream_t s);
+
 /* flowAdvance(U, Q, Fp, Fm, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, s1, s2, s3, 9 metrics, J, x, y, z, phi)
- * solver.jl:19, finite-difference branch solver.jl:133-188: split/advect x, y, z then viscousFlux.
- * x, y, z are accepted and unused (finite-volume only), as in the reference. */
+ * solver.jl:19.  p->fvm == F3D_FDM: the finite-difference branch solver.jl:133-188 (split/advect
+ * x, y, z; x, y, z unused); otherwise the finite-volume branch solver.jl:21-31 (eigen or conser
+ * reconstruct i, j, k; Fp, Fm unused).  Then viscousFlux when p->viscous.  x, y, z are double in
+ * both forms (see the finite-volume entries). */
 int f3d_flow_advance_f64(const f3d_params* p, double* U, double* Q, double* Fp, double* Fm,
                          double* Fx, double* Fy, double* Fz, double* Fv_x, double* Fv_y,
                          double* Fv_z, const double* s1, const double* s2, const double* s3,
@@ -196,7 +255,7 @@ int f3d_flow_advance_f32b(const f3d_params* p, double* U, float* Q, float* Fp, f
                           const float* dxidx, const float* dxidy, const float* dxidz,
                           const float* detadx, const float* detady, const float* detadz,
                           const float* dzetadx, const float* dzetady, const float* dzetadz,
-                          const double* J, const float* x, const float* y, const float* z,
+                          const double* J, const double* x, const double* y, const double* z,
                           float* phi, f3d_stream_t s);
 
 /* ---------------- fused path (the MI355X hot path) ------------------------------------------ */
@@ -239,6 +298,34 @@ int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, d
                        double* Un, float* Q, const float* const* metrics, const double* J,
                        double dt, const float* inflow, f3d_stream_t s);
 
+/* Finite-volume form of the three entries above (p->fvm = F3D_FVM_EIGEN or F3D_FVM_CONSER; the
+ * entries without _fvm reject finite-volume params and these reject F3D_FDM).  Extra arguments: the
+ * conserved state U (its ghost cells are read by the reconstruction; the stage drivers refill them
+ * together with Q's, boundary.jl:1-156) and coords[3] = device pointers x, y, z (double, see the
+ * finite-volume entries).  f3d_rhs_fvm_* reads U and does not modify it. */
+int f3d_rhs_fvm_f64(const f3d_params* p, f3d_workspace* ws, double* R, const double* Q,
+                    const double* U, const double* const* metrics, const double* J,
+                    const double* const* coords, double* phi, f3d_stream_t s);
+int f3d_rhs_fvm_f32b(const f3d_params* p, f3d_workspace* ws, double* R, const float* Q,
+                     const double* U, const float* const* metrics, const double* J,
+                     const double* const* coords, float* phi, f3d_stream_t s);
+int f3d_rk3_stage_fvm_f64(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U,
+                          double* Un, double* Q, const double* const* metrics, const double* J,
+                          const double* const* coords, double dt, const double* inflow,
+                          f3d_stream_t s);
+int f3d_rk3_stage_fvm_f32b(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U,
+                           double* Un, float* Q, const float* const* metrics, const double* J,
+                           const double* const* coords, double dt, const float* inflow,
+                           f3d_stream_t s);
+int f3d_time_step_fvm_f64(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
+                          double* Un, double* Q, const double* const* metrics, const double* J,
+                          const double* const* coords, double dt, const double* inflow,
+                          f3d_stream_t s);
+int f3d_time_step_fvm_f32b(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
+                           double* Un, float* Q, const float* const* metrics, const double* J,
+                           const double* const* coords, double dt, const float* inflow,
+                           f3d_stream_t s);
+
 /* Global CFL time step (no reference counterpart as a reduction; per-cell formula of compute_dt,
  * utils.jl:73-100, with |contravariant velocity| + c):
  *   dt = cfl * min over interior cells of  1 / ( J * max_d (|A_d . u| + c*|A_d|) ),  c = sqrt(gamma Rg T).
@@ -267,7 +354,9 @@ int f3d_metrics_f64(const f3d_params* p, const double* x, const double* y, const
  * differentiated; |x| uses the subgradient sign(0) = 0; a WENO smoothness indicator below the
  * rounding floor of its own evaluation -- a stencil flat to the last bit -- passes no weight
  * derivative).  They back a ChainRulesCore.rrule of the RHS on the Julia side.  The adjoint
- * workspace holds its own FP64 forward workspace. */
+ * workspace holds its own FP64 forward workspace.  Finite-difference branch only: the workspace,
+ * RHS/stage VJPs and the multi-step driver reject p->fvm != F3D_FDM (the ghost and c2Prim VJPs
+ * are the same for both branches). */
 typedef struct f3d_adj_workspace f3d_adj_workspace;
 int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace** aws);
 int f3d_adjoint_workspace_destroy(f3d_adj_workspace* aws);

@@ -18,9 +18,18 @@
 //   rk3 (solver.jl:416-444)           du/dt = lambda*u amplification 1+z+z^2/2+z^3/6.
 //   cfl_dt                            uniform flow closed form.
 //   metrics (parse_mesh.jl:139-248)   Cartesian spacing h -> dxi/dx = h^2, J = h^-3; affine exact.
+//   fvm geometry (FVM.jl:16-31)       affine mesh -> n*Area = the J^-1-scaled metric row of the
+//                                     direction (all three, incl. the j-face orientation flip).
+//   hllc_flux (FVM.jl:1224-1358)      UL = UR -> the Euler flux; F(UL,UR,n) = -F(UR,UL,-n);
+//                                     rotation covariance; an isolated contact is resolved exactly.
+//   fvm reconstruction (FVM.jl:33-226, 707-846) cell averages of polynomials -> the exact face
+//                                     value (UP7 degree 6, WENO7 3, WENO5 2, minmod-left 1), eigen
+//                                     and conser; the right minmod state equals the right cell on
+//                                     monotone data (as written in the reference).
 // No reference test, fixture or golden vector exists for this path (the reference has no tests);
 // the CUDA.jl fastmath Float32 outputs of the reference itself are unpinned (no Julia here).
 #include "f3d_oracle.hpp"
+#include "f3d_oracle_fvm.hpp"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -174,6 +183,28 @@ OR_API void oracle_metrics_f64(const f3d_params* p, const double* x, const doubl
   metrics(p, x, y, z, mm, J);
 }
 namespace {
+// Viscous part of rhs_absterms: J * sum_d (|c1|(|Fv_-2|+|Fv_+2|) + |c2|(|Fv_-1|+|Fv_+1|)) (div.jl:14-48).
+template <typename T>
+void viscous_absterms(const f3d_params* p, double* A, const T* Q, const T* const m[9], const double* J) {
+  Grid g(p);
+  const long NG = g.NG;
+  Scratch<T> sc(p);
+  viscous_flux<T>(p, sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), Q, m, J);
+  const double c1 = 1.0 / 12, c2 = 2.0 / 3;
+  for (long k = 1; k <= g.Nzp; ++k)
+    for (long j = 1; j <= g.Nyp; ++j)
+      for (long i = 1; i <= g.Nxp; ++i) {
+        const double Jc = std::fabs(J[g.cell.at(i + NG, j + NG, k + NG)]);
+        for (int v = 1; v <= 4; ++v) {
+          auto a = [&](const std::vector<T>& F, long ii, long jj, long kk) { return std::fabs((double)F[g.fv.at(ii, jj, kk, v)]); };
+          double s = c1 * (a(sc.Fvx, i, j + 2, k + 2) + a(sc.Fvx, i + 4, j + 2, k + 2)) + c2 * (a(sc.Fvx, i + 1, j + 2, k + 2) + a(sc.Fvx, i + 3, j + 2, k + 2));
+          s += c1 * (a(sc.Fvy, i + 2, j, k + 2) + a(sc.Fvy, i + 2, j + 4, k + 2)) + c2 * (a(sc.Fvy, i + 2, j + 1, k + 2) + a(sc.Fvy, i + 2, j + 3, k + 2));
+          s += c1 * (a(sc.Fvz, i + 2, j + 2, k) + a(sc.Fvz, i + 2, j + 2, k + 4)) + c2 * (a(sc.Fvz, i + 2, j + 2, k + 1) + a(sc.Fvz, i + 2, j + 2, k + 3));
+          A[g.cell.at(i + NG, j + NG, k + NG, v + 1)] += Jc * s;
+        }
+      }
+}
+
 // Magnitude of the terms the RHS is summed from, per cell and variable (the scale the parity bar is
 // relative to).  Inviscid part: each face flux is a sum over its stencil of c_q*Fp and c_q*Fm
 // (schemes.jl:741-753), so its terms have magnitude t = sum_q |c_q| (|Fp_q| + |Fm_q|) -- the split
@@ -215,22 +246,7 @@ void rhs_absterms(const f3d_params* p, double* A, const T* Q, const void* const*
           }
         }
   }
-  if (p->viscous) {
-    viscous_flux<T>(p, sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), Q, m.m, J);
-    const double c1 = 1.0 / 12, c2 = 2.0 / 3;
-    for (long k = 1; k <= g.Nzp; ++k)
-      for (long j = 1; j <= g.Nyp; ++j)
-        for (long i = 1; i <= g.Nxp; ++i) {
-          const double Jc = std::fabs(J[g.cell.at(i + NG, j + NG, k + NG)]);
-          for (int v = 1; v <= 4; ++v) {
-            auto a = [&](const std::vector<T>& F, long ii, long jj, long kk) { return std::fabs((double)F[g.fv.at(ii, jj, kk, v)]); };
-            double s = c1 * (a(sc.Fvx, i, j + 2, k + 2) + a(sc.Fvx, i + 4, j + 2, k + 2)) + c2 * (a(sc.Fvx, i + 1, j + 2, k + 2) + a(sc.Fvx, i + 3, j + 2, k + 2));
-            s += c1 * (a(sc.Fvy, i + 2, j, k + 2) + a(sc.Fvy, i + 2, j + 4, k + 2)) + c2 * (a(sc.Fvy, i + 2, j + 1, k + 2) + a(sc.Fvy, i + 2, j + 3, k + 2));
-            s += c1 * (a(sc.Fvz, i + 2, j + 2, k) + a(sc.Fvz, i + 2, j + 2, k + 4)) + c2 * (a(sc.Fvz, i + 2, j + 2, k + 1) + a(sc.Fvz, i + 2, j + 2, k + 3));
-            A[g.cell.at(i + NG, j + NG, k + NG, v + 1)] += Jc * s;
-          }
-        }
-  }
+  if (p->viscous) viscous_absterms<T>(p, A, Q, m.m, J);
 }
 
 // advect_c evaluated in long double from double inputs (conditioning reference for the tests).
@@ -497,6 +513,230 @@ OR_API void oracle_advect_c_ld(const f3d_params* p, int d, double* F, const doub
                                const double* Fp, const double* Fm, const double* Q, const double* Ax,
                                const double* Ay, const double* Az) {
   advect_ld(p, d, phi, S, Fp, Fm, Q, Ax, Ay, Az, F);
+}
+
+// ---------------------------------------------------------------- finite-volume branch (FVM.jl)
+namespace {
+struct Coords {
+  const double* x;
+  const double* y;
+  const double* z;
+  explicit Coords(const void* const* c) : x((const double*)c[0]), y((const double*)c[1]), z((const double*)c[2]) {}
+};
+
+template <typename T>
+void rhs_fvm_ref(const f3d_params* p, double* R, const T* Q, const double* U, const void* const* mv, const double* J,
+                 const void* const* cv, T* phi_out) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  Scratch<T> sc(p);
+  M9<T> m(mv);
+  Coords c(cv);
+  std::vector<T> s1(nc), s2(nc), s3(nc);
+  metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
+  shock_sensor<T>(p, sc.phi.data(), Q);
+  flow_advance_fvm<T>(p, U, Q, sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(),
+                      s1.data(), s2.data(), s3.data(), m.m, J, c.x, c.y, c.z, sc.phi.data());
+  std::fill(R, R + nc * 5, 0.0);
+  div<T>(p, R, sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), 1.0, J);
+  if (phi_out) std::copy(sc.phi.begin(), sc.phi.end(), phi_out);
+}
+
+template <typename T>
+void time_step_fvm_ref(const f3d_params* p, int nsteps, double* U, double* Un, T* Q, const void* const* mv,
+                       const double* J, const void* const* cv, double dt, const T* inflow) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  Scratch<T> sc(p);
+  M9<T> m(mv);
+  Coords c(cv);
+  std::vector<T> s1(nc), s2(nc), s3(nc);
+  metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
+  for (int t = 0; t < nsteps; ++t)
+    for (int KRK = 1; KRK <= 3; ++KRK)
+      rk3_stage_fvm<T>(p, sc, KRK, U, Un, Q, s1.data(), s2.data(), s3.data(), m.m, J, c.x, c.y, c.z, dt, inflow);
+}
+
+// Finite-volume RHS in long double from double inputs, the scheme mask taken from the double sensor
+// (rounding-floor reference of the FP64 parity bar, as rhs_ld).
+void rhs_fvm_ld(const f3d_params* p, double* Rout, const double* Q, const double* U, const void* const* mv,
+                const double* J, const void* const* cv) {
+  typedef long double L;
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  std::vector<double> phid(nc, 0.0);
+  shock_sensor<double>(p, phid.data(), Q);
+  std::vector<L> QL(Q, Q + 6 * nc), phiL(phid.begin(), phid.end());
+  std::vector<std::vector<L>> mL(9);
+  const L* mp[9];
+  for (int q = 0; q < 9; ++q) {
+    const double* src = (const double*)mv[q];
+    mL[q].assign(src, src + nc);
+    mp[q] = mL[q].data();
+  }
+  Coords c(cv);
+  Scratch<L> sc(p);
+  std::vector<L> s1(nc), s2(nc), s3(nc);
+  metric_norms<L>(p, mp, s1.data(), s2.data(), s3.data());
+  flow_advance_fvm<L>(p, U, QL.data(), sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(),
+                      sc.Fvz.data(), s1.data(), s2.data(), s3.data(), mp, J, c.x, c.y, c.z, phiL.data());
+  std::fill(Rout, Rout + nc * 5, 0.0);
+  const long NG = g.NG;
+  const L c1 = L(1) / L(12), c2 = L(-2) / L(3);
+  for (long k = 1; k <= g.Nzp; ++k)
+    for (long j = 1; j <= g.Nyp; ++j)
+      for (long i = 1; i <= g.Nxp; ++i) {
+        const L Jc = J[g.cell.at(i + NG, j + NG, k + NG)];
+        for (long n = 1; n <= 5; ++n) {
+          L s = sc.Fx[g.fx.at(i, j, k, n)] - sc.Fx[g.fx.at(i + 1, j, k, n)] + sc.Fy[g.fy.at(i, j, k, n)] -
+                sc.Fy[g.fy.at(i, j + 1, k, n)] + sc.Fz[g.fz.at(i, j, k, n)] - sc.Fz[g.fz.at(i, j, k + 1, n)];
+          if (n >= 2 && p->viscous) {
+            const int v = (int)n - 1;
+            auto F = [&](const std::vector<L>& A, long ii, long jj, long kk) { return A[g.fv.at(ii, jj, kk, v)]; };
+            s += c1 * (F(sc.Fvx, i, j + 2, k + 2) - F(sc.Fvx, i + 4, j + 2, k + 2)) + c2 * (F(sc.Fvx, i + 1, j + 2, k + 2) - F(sc.Fvx, i + 3, j + 2, k + 2));
+            s += c1 * (F(sc.Fvy, i + 2, j, k + 2) - F(sc.Fvy, i + 2, j + 4, k + 2)) + c2 * (F(sc.Fvy, i + 2, j + 1, k + 2) - F(sc.Fvy, i + 2, j + 3, k + 2));
+            s += c1 * (F(sc.Fvz, i + 2, j + 2, k) - F(sc.Fvz, i + 2, j + 2, k + 4)) + c2 * (F(sc.Fvz, i + 2, j + 2, k + 1) - F(sc.Fvz, i + 2, j + 2, k + 3));
+          }
+          Rout[g.cell.at(i + NG, j + NG, k + NG, n)] = (double)(s * Jc);
+        }
+      }
+}
+
+// Magnitude of the terms of the finite-volume RHS per cell and variable: J * sum_d (|F_d| on the
+// cell's two faces) for the inviscid part (each face flux enters one cell difference), plus the
+// viscous terms of rhs_absterms.
+template <typename T>
+void rhs_absterms_fvm(const f3d_params* p, double* A, const T* Q, const double* U, const void* const* mv,
+                      const double* J, const void* const* cv) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  Scratch<T> sc(p);
+  M9<T> m(mv);
+  Coords c(cv);
+  std::vector<T> s1(nc), s2(nc), s3(nc);
+  metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
+  shock_sensor<T>(p, sc.phi.data(), Q);
+  const bool eigen = p->fvm == F3D_FVM_EIGEN;
+  fvm_reconstruct<T>(p, 0, eigen, Q, U, sc.phi.data(), s1.data(), sc.Fx.data(), c.x, c.y, c.z);
+  fvm_reconstruct<T>(p, 1, eigen, Q, U, sc.phi.data(), s2.data(), sc.Fy.data(), c.x, c.y, c.z);
+  fvm_reconstruct<T>(p, 2, eigen, Q, U, sc.phi.data(), s3.data(), sc.Fz.data(), c.x, c.y, c.z);
+  std::fill(A, A + nc * 5, 0.0);
+  const long NG = g.NG;
+  for (long k = 1; k <= g.Nzp; ++k)
+    for (long j = 1; j <= g.Nyp; ++j)
+      for (long i = 1; i <= g.Nxp; ++i) {
+        const double Jc = std::fabs(J[g.cell.at(i + NG, j + NG, k + NG)]);
+        for (long n = 1; n <= 5; ++n) {
+          auto a = [](T v) { return std::fabs((double)v); };
+          const double t = a(sc.Fx[g.fx.at(i, j, k, n)]) + a(sc.Fx[g.fx.at(i + 1, j, k, n)]) +
+                           a(sc.Fy[g.fy.at(i, j, k, n)]) + a(sc.Fy[g.fy.at(i, j + 1, k, n)]) +
+                           a(sc.Fz[g.fz.at(i, j, k, n)]) + a(sc.Fz[g.fz.at(i, j, k + 1, n)]);
+          A[g.cell.at(i + NG, j + NG, k + NG, n)] = Jc * t;
+        }
+      }
+  if (p->viscous) viscous_absterms<T>(p, A, Q, m.m, J);
+}
+}  // namespace
+
+OR_API void oracle_fvm_reconstruct_f64(const f3d_params* p, int d, int eigen, const double* Q, const double* U,
+                                       const double* phi, const double* S, double* F, const double* x,
+                                       const double* y, const double* z, double* UL, double* UR, double* geo) {
+  fvm_reconstruct<double>(p, d, eigen != 0, Q, U, phi, S, F, x, y, z, UL, UR, geo);
+}
+// Face fluxes in long double from double inputs (rounding-floor reference of the FP64 face bar).
+OR_API void oracle_fvm_reconstruct_ld(const f3d_params* p, int d, int eigen, const double* Q, const double* U,
+                                      const double* phi, const double* S, double* F, const double* x,
+                                      const double* y, const double* z) {
+  Grid g(p);
+  size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  std::vector<long double> QL(Q, Q + 6 * nc), phiL(phi, phi + nc), SL(S, S + nc);
+  size_t nf = (size_t)(g.Nxp + (d == 0)) * (g.Nyp + (d == 1)) * (g.Nzp + (d == 2)) * 5;
+  std::vector<long double> FL(nf);
+  fvm_reconstruct<long double>(p, d, eigen != 0, QL.data(), U, phiL.data(), SL.data(), FL.data(), x, y, z);
+  for (size_t i = 0; i < nf; ++i) F[i] = (double)FL[i];
+}
+OR_API void oracle_fvm_reconstruct_f32b(const f3d_params* p, int d, int eigen, const float* Q, const double* U,
+                                        const float* phi, const float* S, float* F, const double* x,
+                                        const double* y, const double* z, float* UL, float* UR, float* geo) {
+  fvm_reconstruct<float>(p, d, eigen != 0, Q, U, phi, S, F, x, y, z, UL, UR, geo);
+}
+// Face geometry only (nx, ny, nz, Area in the face layout of direction d).
+OR_API void oracle_fvm_geometry_f64(const f3d_params* p, int d, const double* x, const double* y, const double* z,
+                                    double* geo) {
+  Grid g(p);
+  const Dir D = make_dir(d);
+  const Ext& fe = d == 0 ? g.fx : (d == 1 ? g.fy : g.fz);
+  const double* const X[3] = {x, y, z};
+  for (long k = g.NG + 1 - D.dk; k <= g.Nzp + g.NG; ++k)
+    for (long j = g.NG + 1 - D.dj; j <= g.Nyp + g.NG; ++j)
+      for (long i = g.NG + 1 - D.di; i <= g.Nxp + g.NG; ++i) {
+        double n[3], A;
+        fvm_face_geometry<double>(g, d, X, i, j, k, n, &A);
+        const long fi = i - g.NG + D.di, fj = j - g.NG + D.dj, fk = k - g.NG + D.dk;
+        for (int m = 0; m < 3; ++m) geo[fe.at(fi, fj, fk, m + 1)] = n[m];
+        geo[fe.at(fi, fj, fk, 4)] = A;
+      }
+}
+// HLLC flux of nf independent faces: UL, UR [5][nf], n [3][nf], F [5][nf] (struct of arrays).
+OR_API void oracle_hllc_f64(const f3d_params* p, long nf, const double* UL, const double* UR, const double* n,
+                            double* F) {
+  for (long f = 0; f < nf; ++f) {
+    double l[5], r[5], o[5];
+    for (int q = 0; q < 5; ++q) {
+      l[q] = UL[q * nf + f];
+      r[q] = UR[q * nf + f];
+    }
+    hllc_flux<double>(p, l, r, n[f], n[nf + f], n[2 * nf + f], o);
+    for (int q = 0; q < 5; ++q) F[q * nf + f] = o[q];
+  }
+}
+OR_API void oracle_rhs_fvm_f64(const f3d_params* p, double* R, const double* Q, const double* U, const void* const* m,
+                               const double* J, const void* const* coords, double* phi) {
+  rhs_fvm_ref<double>(p, R, Q, U, m, J, coords, phi);
+}
+OR_API void oracle_rhs_fvm_f32b(const f3d_params* p, double* R, const float* Q, const double* U, const void* const* m,
+                                const double* J, const void* const* coords, float* phi) {
+  rhs_fvm_ref<float>(p, R, Q, U, m, J, coords, phi);
+}
+OR_API void oracle_rhs_fvm_ld(const f3d_params* p, double* R, const double* Q, const double* U, const void* const* m,
+                              const double* J, const void* const* coords) {
+  rhs_fvm_ld(p, R, Q, U, m, J, coords);
+}
+OR_API void oracle_rhs_absterms_fvm_f64(const f3d_params* p, double* A, const double* Q, const double* U,
+                                        const void* const* m, const double* J, const void* const* coords) {
+  rhs_absterms_fvm<double>(p, A, Q, U, m, J, coords);
+}
+OR_API void oracle_rhs_absterms_fvm_f32b(const f3d_params* p, double* A, const float* Q, const double* U,
+                                         const void* const* m, const double* J, const void* const* coords) {
+  rhs_absterms_fvm<float>(p, A, Q, U, m, J, coords);
+}
+OR_API void oracle_time_step_fvm_f64(const f3d_params* p, int nsteps, double* U, double* Un, double* Q,
+                                     const void* const* m, const double* J, const void* const* coords, double dt,
+                                     const double* inflow) {
+  time_step_fvm_ref<double>(p, nsteps, U, Un, Q, m, J, coords, dt, inflow);
+}
+OR_API void oracle_time_step_fvm_f32b(const f3d_params* p, int nsteps, double* U, double* Un, float* Q,
+                                      const void* const* m, const double* J, const void* const* coords, double dt,
+                                      const float* inflow) {
+  time_step_fvm_ref<float>(p, nsteps, U, Un, Q, m, J, coords, dt, inflow);
+}
+
+// WENO7 smoothness indicators of one 7-point stencil: form 0 the reference's quadratic forms in V
+// (schemes.jl:848-869, FVM.jl:127-135), form 1 the first-difference forms of the float path.
+OR_API void oracle_weno7_is_f64(int form, const double* V, double* Is) {
+  if (form == 1) {
+    weno7_is_diff<double>(V, Is);
+    return;
+  }
+  const double V1 = V[0], V2 = V[1], V3 = V[2], V4 = V[3], V5 = V[4], V6 = V[5], V7 = V[6];
+  Is[0] = V1 * (547 * V1 - 3882 * V2 + 4642 * V3 - 1854 * V4) + V2 * (7043 * V2 - 17246 * V3 + 7042 * V4) +
+          V3 * (11003 * V3 - 9402 * V4) + V4 * (2107 * V4);
+  Is[1] = V2 * (267 * V2 - 1642 * V3 + 1602 * V4 - 494 * V5) + V3 * (2843 * V3 - 5966 * V4 + 1922 * V5) +
+          V4 * (3443 * V4 - 2522 * V5) + V5 * (547 * V5);
+  Is[2] = V3 * (547 * V3 - 2522 * V4 + 1922 * V5 - 494 * V6) + V4 * (3443 * V4 - 5966 * V5 + 1602 * V6) +
+          V5 * (2843 * V5 - 1642 * V6) + V6 * (267 * V6);
+  Is[3] = V4 * (2107 * V4 - 9402 * V5 + 7042 * V6 - 1854 * V7) + V5 * (11003 * V5 - 17246 * V6 + 4642 * V7) +
+          V6 * (7043 * V6 - 3882 * V7) + V7 * (547 * V7);
 }
 
 OR_API int oracle_max_threads(void) {

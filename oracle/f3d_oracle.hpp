@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "../include/f3d.h"
@@ -195,6 +196,14 @@ void scheme_mask(const f3d_params* p, uint8_t* mx, uint8_t* my, uint8_t* mz, con
   }
 }
 
+// Ratio bmin/|b| of the range-safe float weights (bmin the smallest |b| of the stencil, so the
+// ratio lies in [0, 1] even where rounding makes a smoothness indicator slightly negative); a
+// vanishing |b|, whose weight is unbounded, gets 1 and every other stencil 0.
+inline float wratio(float bmin, float b) {
+  const float ab = std::fabs(b);
+  return ab > 0.f ? bmin / ab : 1.f;
+}
+
 // WENO7 nonlinear weights (schemes.jl:871-879).  The reference evaluates them in Float64 (its
 // epsilon is a Float64 literal); for T=double that is the literal formula.  For T=float the same
 // weights are formed range-safely: every (eps+Is*s0)^2 is divided by the smallest one before
@@ -245,11 +254,11 @@ struct Weno7Weights<float> {
   static float combine(float eps, float s0, const float Is[4], const float q[4]) {
     float b[4];
     for (int m = 0; m < 4; ++m) b[m] = eps + Is[m] * s0;
-    float bmin = std::min(std::min(b[0], b[1]), std::min(b[2], b[3]));
+    float bmin = std::min(std::min(std::fabs(b[0]), std::fabs(b[1])), std::min(std::fabs(b[2]), std::fabs(b[3])));
     const float d[4] = {1.f, 12.f, 18.f, 4.f};
     float a[4], sum = 0.f, acc = 0.f;
     for (int m = 0; m < 4; ++m) {
-      float r = bmin / b[m];
+      float r = wratio(bmin, b[m]);
       a[m] = d[m] * (r * r);
       sum += a[m];
       acc += a[m] * q[m];
@@ -257,6 +266,21 @@ struct Weno7Weights<float> {
     return acc / sum;
   }
 };
+
+// The four WENO7 smoothness indicators as quadratic forms of the first differences
+// a_k = V_{k+1} - V_k: the reference's quadratic forms in V (schemes.jl:848-869) regrouped (each is
+// invariant to adding a constant to V), the same polynomials exactly in rational arithmetic.  The
+// float form evaluates them this way: on smooth data the V form loses the indicator to the
+// cancellation of O(V^2) terms (float's 6e-8 x 17246 V^2), which sets the weights by rounding noise.
+template <typename T>
+inline void weno7_is_diff(const T* V, T Is[4]) {
+  const T a1 = V[1] - V[0], a2 = V[2] - V[1], a3 = V[3] - V[2], a4 = V[4] - V[3], a5 = V[5] - V[4],
+          a6 = V[6] - V[5];
+  Is[0] = a1 * (T(547) * a1 - T(2788) * a2 + T(1854) * a3) + a2 * (T(3708) * a2 - T(5188) * a3) + a3 * (T(2107) * a3);
+  Is[1] = a2 * (T(267) * a2 - T(1108) * a3 + T(494) * a4) + a3 * (T(1468) * a3 - T(1428) * a4) + a4 * (T(547) * a4);
+  Is[2] = a3 * (T(547) * a3 - T(1428) * a4 + T(494) * a5) + a4 * (T(1468) * a4 - T(1108) * a5) + a5 * (T(267) * a5);
+  Is[3] = a4 * (T(2107) * a4 - T(5188) * a5 + T(1854) * a6) + a5 * (T(3708) * a5 - T(2788) * a6) + a6 * (T(547) * a6);
+}
 
 // One WENO7 reconstruction of a 7-point characteristic stencil V[0..6]  (schemes.jl:847-879)
 template <typename T>
@@ -267,6 +291,10 @@ inline T weno7(const T* V, T s0, T eps) {
   q[1] = V2 - 5 * V3 + 13 * V4 + 3 * V5;
   q[2] = -V3 + 7 * V4 + 7 * V5 - V6;
   q[3] = 3 * V4 + 13 * V5 - 5 * V6 + V7;
+  if (std::is_same<T, float>::value) {
+    weno7_is_diff(V, Is);
+    return Weno7Weights<T>::combine(eps, s0, Is, q);
+  }
   Is[0] = V1 * (547 * V1 - 3882 * V2 + 4642 * V3 - 1854 * V4) +
           V2 * (7043 * V2 - 17246 * V3 + 7042 * V4) + V3 * (11003 * V3 - 9402 * V4) +
           V4 * (2107 * V4);

@@ -13,8 +13,9 @@ cases = importlib.import_module("flame3d-mhd_amd.cases")
 class Case:
     """params + Q (ghosts filled) + 9 metrics + J (+ s1..s3, inflow) in one numeric form."""
 
-    def __init__(self, params, Q, metrics, J, inflow=None, name=""):
+    def __init__(self, params, Q, metrics, J, inflow=None, name="", coords=None):
         self.params = params
+        self.coords = coords  # mesh points x, y, z (float64, with ghosts) for the finite-volume branch
         self.Q = Q
         self.metrics = metrics
         self.J = J
@@ -64,7 +65,8 @@ def tgv_case(N=16, dtype=np.float64, noise=1e-3, jump=0.6, seed=1, viscous=1, sp
     Q = np.ascontiguousarray(Q.astype(dtype))
     _fill(p, Q, None)
     m = [np.ascontiguousarray(a.astype(dtype)) for a in c["metrics"]]
-    return Case(p, Q, m, np.ascontiguousarray(c["J"]), None, name=f"tgv{N}")
+    coords = [np.ascontiguousarray(c[a]) for a in ("x", "y", "z")]
+    return Case(p, Q, m, np.ascontiguousarray(c["J"]), None, name=f"tgv{N}", coords=coords)
 
 
 def ramp_case(Nx=24, Ny=16, Nz=8, sweep=0.0, dtype=np.float64, seed=2, viscous=1, noise=0.01, jump=0.0,
@@ -98,7 +100,8 @@ def ramp_case(Nx=24, Ny=16, Nz=8, sweep=0.0, dtype=np.float64, seed=2, viscous=1
     inflow = np.ascontiguousarray(c["inflow"].astype(dtype))
     _fill(p, Q, inflow)
     m = [np.ascontiguousarray(a.astype(dtype)) for a in m64]
-    return Case(p, Q, m, J, inflow, name=f"ramp{Nx}x{Ny}x{Nz}s{sweep:g}")
+    coords = [np.ascontiguousarray(c[a]) for a in ("x", "y", "z")]
+    return Case(p, Q, m, J, inflow, name=f"ramp{Nx}x{Ny}x{Nz}s{sweep:g}", coords=coords)
 
 
 def scaled_err(a, b, scale):

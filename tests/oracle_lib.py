@@ -145,6 +145,38 @@ class Oracle:
     def metrics(self, x, y, z, m, J):
         self._c("oracle_metrics_f64", P(x), P(y), P(z), PP(m), P(J))
 
+    # -- finite-volume branch (FVM.jl) --------------------------------------------------------
+    def fvm_reconstruct(self, d, eigen, Q, U, phi, S, F, x, y, z, UL=None, UR=None, geo=None):
+        """Eigen_/Conser_reconstruct of direction d: F = HLLC * Area; optional UL/UR states and
+        geo (nx, ny, nz, Area) per face."""
+        self._c(f"oracle_fvm_reconstruct_{suf(Q)}", c_int(d), c_int(1 if eigen else 0), P(Q), P(U), P(phi), P(S),
+                P(F), P(x), P(y), P(z), P(UL), P(UR), P(geo))
+
+    def fvm_reconstruct_ld(self, d, eigen, Q, U, phi, S, F, x, y, z):
+        """Face fluxes in long double from float64 inputs (rounding-floor reference)."""
+        self._c("oracle_fvm_reconstruct_ld", c_int(d), c_int(1 if eigen else 0), P(Q), P(U), P(phi), P(S), P(F),
+                P(x), P(y), P(z))
+
+    def fvm_geometry(self, d, x, y, z, geo):
+        self._c("oracle_fvm_geometry_f64", c_int(d), P(x), P(y), P(z), P(geo))
+
+    def hllc(self, UL, UR, n, F):
+        """HLLC of independent faces: UL, UR, F (5, nf); n (3, nf); float64."""
+        nf = UL.shape[1]
+        self._c("oracle_hllc_f64", ctypes.c_long(nf), P(UL), P(UR), P(n), P(F))
+
+    def rhs_fvm(self, R, Q, U, m, J, coords, phi=None):
+        self._c(f"oracle_rhs_fvm_{suf(Q)}", P(R), P(Q), P(U), PP(m), P(J), PP(coords), P(phi))
+
+    def rhs_fvm_ld(self, R, Q, U, m, J, coords):
+        self._c("oracle_rhs_fvm_ld", P(R), P(Q), P(U), PP(m), P(J), PP(coords))
+
+    def rhs_absterms_fvm(self, A, Q, U, m, J, coords):
+        self._c(f"oracle_rhs_absterms_fvm_{suf(Q)}", P(A), P(Q), P(U), PP(m), P(J), PP(coords))
+
+    def time_step_fvm(self, nsteps, U, Un, Q, m, J, coords, dt, inflow=None):
+        self._c(f"oracle_time_step_fvm_{suf(Q)}", c_int(nsteps), P(U), P(Un), P(Q), PP(m), P(J), PP(coords),
+                c_double(dt), P(inflow))
 
 def max_threads():
     return olib().oracle_max_threads()

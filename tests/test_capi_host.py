@@ -118,3 +118,24 @@ def test_adjoint_entry_points_validate_before_any_launch():
     assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 1) == (50 + 1) * 5 * nc * 8
     assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 10) == (5 + 10) * 5 * nc * 8
     assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 0, 1) == 0
+
+
+def test_fvm_selector_and_entry_validation():
+    L = f3d.lib()
+    assert f3d.make_params(16, 12, 8).fvm == f3d.FDM  # finite-difference branch unless asked
+    p = f3d.make_params(16, 12, 8, fvm=f3d.FVM_EIGEN)
+    q = f3d.F3DParams()
+    L.f3d_params_init(ctypes.byref(q), 16, 16, 16)
+    q.fvm = 3
+    assert L.f3d_params_check(ctypes.byref(q)) == -4 and b"fvm" in L.f3d_last_error()
+    for name in ("eigen_reconstruct_i", "eigen_reconstruct_k", "conser_reconstruct_j"):
+        for suf in ("f64", "f32b"):
+            fn = getattr(L, f"f3d_{name}_{suf}")
+            assert fn(ctypes.byref(p), None, None, None, None, None, None, None, None, None) == -1
+    assert L.f3d_rhs_fvm_f64(ctypes.byref(p), None, None, None, None, None, None, None, None, None) == -1
+    assert L.f3d_time_step_fvm_f32b(ctypes.byref(p), None, 1, None, None, None, None, None, None,
+                                    ctypes.c_double(1.0), None, None) == -1
+    # the adjoint differentiates the finite-difference branch only: rejected before any allocation
+    h = ctypes.c_void_p()
+    assert L.f3d_adjoint_workspace_create(ctypes.byref(p), ctypes.byref(h)) == -4
+    assert b"finite-difference" in L.f3d_last_error()
