@@ -8,7 +8,8 @@ viscous flux, divergence, RK blend, c2Prim, ghost fill) over the whole grid.  Wo
 wall, periodic span.  Inputs (state, metrics, Jacobian) are resident in HBM before timing.
 
 Printed: one JSON line.  `value` is Mcell-updates/s of the FP64 form (interior cells x RK3 steps
-per second / 1e6).  Also reported: the FP32-bulk form on the same grid and its density and
+per second / 1e6).  Also reported: the finite-volume branch (FVM.jl, the reference run's default
+scheme) on the same grid and state in both numeric forms, the FP32-bulk form on the same grid and its density and
 wall-pressure errors against the FP64 run from the same initial state, the % of HBM peak of the
 whole path (algorithmic bytes per cell-update x rate / 8 TB/s), the dominant kernel's roofline
 from hipEvent timing of its launches inside the timed region, and the CPU oracle timed on the host
@@ -76,7 +77,14 @@ FUSED_BYTES = {
     "fused_heavy": lambda p, es: 0,
     # u,v,w,T + 9 metrics + J in, 12 viscous fluxes out, per cell of the interior+-2 box
     "fused_viscous": lambda p, es: (p.Nxp + 4) * (p.Nyp + 4) * (p.Nzp + 4) * (13 * es + 8 + 12 * es),
+    # finite-volume sweeps: U (double) + mesh points (double) + phi in, RHS out (x) / in and out (y, z)
+    "fvm_sweep_x": lambda p, es: _nint(p) * (40 + 24 + es + 5 * es),
+    "fvm_sweep_y": lambda p, es: _nint(p) * (40 + 24 + es + 10 * es),
+    "fvm_sweep_z": lambda p, es: _nint(p) * (40 + 24 + es + 10 * es),
+    "fvm_heavy": lambda p, es: 0,
 }
+# Finite-volume step: per stage U + mesh points + 9 metrics + J read, U written; stages 2-3 read Un.
+ALG_BYTES_FVM = {"f64": 3 * (40 + 24 + 72 + 8 + 40) + 80.0, "f32b": 3 * (40 + 24 + 36 + 8 + 40) + 80.0}
 
 
 # HBM bytes per launch actually moved, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
@@ -106,14 +114,19 @@ def build_case(f3d, cases, torch, nx, ny, nz, seed=2, sweep=0.0):
     return dict(c=c, p=p, ctx=ctx, m=m, J=J, Q=Q, U=U, inflow=inflow, dt=float(dt_t.item()))
 
 
-def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3):
+def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3, coords=None):
     """Warm up; run `table_steps` steps with every library launch bracketed by hipEvents (per-kernel
     table); then time `steps` RK3 steps between two synchronisations with events around the
-    launches of the dominant kernel only (its live average launch time).  Returns
-    (ms/step, host wall ms/step, kernel table, (dominant name, its total ms, its launches))."""
+    launches of the dominant kernel only (its live average launch time).  `coords` (mesh points):
+    the finite-volume branch.  Returns (ms/step, host wall ms/step, kernel table,
+    (dominant name, its total ms, its launches))."""
     U, Un, Q, m, J, inflow, dt = state
+    if coords is None:
+        step = lambda: ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
+    else:
+        step = lambda: ctx.time_step_fvm(1, U, Un, Q, m, J, coords, dt, inflow)
     for _ in range(warmup):
-        ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
+        step()
     torch.cuda.synchronize()
     table, dom = {}, None
     if lib is not None and table_steps:
@@ -121,7 +134,7 @@ def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3):
         lib.f3d_timing_reset()
         lib.f3d_timing_enable(1)
         for _ in range(table_steps):
-            ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
+            step()
         torch.cuda.synchronize()
         lib.f3d_timing_enable(0)
         table = kernel_table(lib)
@@ -136,7 +149,7 @@ def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3):
     t0 = time.perf_counter()
     ev0.record()
     for _ in range(steps):
-        ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
+        step()
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -163,6 +176,47 @@ def kernel_table(lib):
         n = ctypes.c_int64()
         lib.f3d_timing_query(name.encode(), ctypes.byref(ms), ctypes.byref(n))
         out[name] = (ms.value, n.value)
+    return out
+
+
+def fvm_bench(f3d, torch, lib, st, U0, Q0, steps, warmup, f32b=True):
+    """The finite-volume branch (characteristic reconstruction + HLLC, run.jl's default
+    finite_volume = eigen_reconstruction = true) on the same grid, initial state and dt: FP64 and
+    FP32-bulk rates, the FP32-bulk density error against the FP64 run, and per-kernel tables."""
+    c, dt = st["c"], st["dt"]
+    nx, ny, nz = st["p"].Nxp, st["p"].Nyp, st["p"].Nzp
+    p = f3d.make_params(nx, ny, nz, **c["params"], fvm=f3d.FVM_EIGEN)
+    ctx = f3d.Flame3D(p)
+    X = [torch.from_numpy(c[a]).to("cuda:0") for a in ("x", "y", "z")]
+    ncells = nx * ny * nz
+    U, Un, Q = U0.clone(), torch.zeros_like(U0), Q0.clone()
+    ms64, _, ktab, _ = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], dt), steps, warmup, lib=lib,
+                                 coords=X)
+    ctx.nan_check(U)
+    v64 = ncells / (ms64 * 1e-3) / 1e6
+    progress(f"FVM FP64 {steps} steps: {ms64:.3f} ms/step")
+    tab = lambda kt, ms: {k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1], share=round(v[0] / (ms * 3), 4))
+                          for k, v in sorted(kt.items())}
+    out = dict(scheme="finite volume, characteristic (Roe-frame) WENO7/WENO5/minmod + HLLC (FVM.jl)",
+               value=round(v64, 3), unit="Mcell-updates/s", ms_per_step=round(ms64, 4),
+               hbm_frac_path=round(ALG_BYTES_FVM["f64"] * v64 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+               kernels=tab(ktab, ms64))
+    if f32b:
+        m32 = [a.float().contiguous() for a in st["m"]]
+        U32, Un32, Q32 = U0.clone(), torch.zeros_like(U0), Q0.float().contiguous()
+        ms32, _, ktab32, _ = run_steps(torch, ctx, (U32, Un32, Q32, m32, st["J"], st["inflow"].float().contiguous(),
+                                                    dt), steps, warmup, lib=lib, coords=X)
+        ctx.nan_check(U32)
+        v32 = ncells / (ms32 * 1e-3) / 1e6
+        progress(f"FVM FP32-bulk {steps} steps: {ms32:.3f} ms/step")
+        NG = 4
+        inner = (slice(NG, -NG),) * 3
+        r64, r32 = U[0][inner], U32[0][inner]
+        out["fp32_bulk"] = dict(value=round(v32, 3), ms_per_step=round(ms32, 4), speedup_vs_f64=round(v32 / v64, 3),
+                                hbm_frac_path=round(ALG_BYTES_FVM["f32b"] * v32 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+                                rho_L2_rel=float(torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)),
+                                rho_Linf_rel=float((r32 - r64).abs().max() / r64.abs().max()),
+                                steps_compared=steps + warmup + 3, kernels=tab(ktab32, ms32))
     return out
 
 
@@ -326,6 +380,8 @@ def main():
     ap.add_argument("--sweep", type=float, default=0.0, help="sweep angle (deg) of the ramp grid")
     ap.add_argument("--adjoint-every", type=int, default=1, help="checkpoint interval of the adjoint (RK3 steps)")
     ap.add_argument("--adjoint-sweep", type=float, default=0.0)
+    ap.add_argument("--no-fvm", action="store_true", help="skip the finite-volume leg")
+    ap.add_argument("--fvm-only", action="store_true", help="only the finite-volume leg (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -361,6 +417,10 @@ def main():
     p, ctx, dt = st["p"], st["ctx"], st["dt"]
     U0 = st["U"].clone()
     Q0 = st["Q"].clone()
+    if args.fvm_only:
+        print(json.dumps({"finite_volume": fvm_bench(f3d, torch, lib, st, U0, Q0, args.steps, args.warmup,
+                                                     f32b=not args.no_f32b)}))
+        return
     # ---- FP64
     U, Un, Q = st["U"], torch.zeros_like(st["U"]), st["Q"]
     ms64, wall64, ktab, live = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], dt), args.steps,
@@ -411,6 +471,9 @@ def main():
                     share_of_step=round(tot_ms / (ms64 * args.steps), 4))
     kshare = {k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1],
                       share=round(v[0] / (ms64 * 3), 4)) for k, v in sorted(ktab.items())}
+    fvm = None
+    if not args.no_fvm:
+        fvm = fvm_bench(f3d, torch, lib, st, U0, Q0, args.steps, args.warmup, f32b=not args.no_f32b)
     adj = None
     if not args.no_adjoint:
         ax, ay, az = map(int, args.adjoint_grid.split("x"))
@@ -441,6 +504,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp32_bulk": out32 or None,
+        "finite_volume": fvm,
         "adjoint": adj,
         "kernels": kshare,
         "host_wall_ms_per_step": round(wall64, 4),
