@@ -77,14 +77,15 @@ FUSED_BYTES = {
     "fused_heavy": lambda p, es: 0,
     # u,v,w,T + 9 metrics + J in, 12 viscous fluxes out, per cell of the interior+-2 box
     "fused_viscous": lambda p, es: (p.Nxp + 4) * (p.Nyp + 4) * (p.Nzp + 4) * (13 * es + 8 + 12 * es),
-    # finite-volume sweeps: U (double) + mesh points (double) + phi in, RHS out (x) / in and out (y, z)
-    "fvm_sweep_x": lambda p, es: _nint(p) * (40 + 24 + es + 5 * es),
-    "fvm_sweep_y": lambda p, es: _nint(p) * (40 + 24 + es + 10 * es),
-    "fvm_sweep_z": lambda p, es: _nint(p) * (40 + 24 + es + 10 * es),
+    # finite-volume sweeps: U (double) + face geometry (4) + phi in, RHS out (x) / in and out (y, z)
+    "fvm_sweep_x": lambda p, es: _nint(p) * (40 + 4 * es + es + 5 * es),
+    "fvm_sweep_y": lambda p, es: _nint(p) * (40 + 4 * es + es + 10 * es),
+    "fvm_sweep_z": lambda p, es: _nint(p) * (40 + 4 * es + es + 10 * es),
     "fvm_heavy": lambda p, es: 0,
 }
-# Finite-volume step: per stage U + mesh points + 9 metrics + J read, U written; stages 2-3 read Un.
-ALG_BYTES_FVM = {"f64": 3 * (40 + 24 + 72 + 8 + 40) + 80.0, "f32b": 3 * (40 + 24 + 36 + 8 + 40) + 80.0}
+# Finite-volume step: per stage U + face geometry (3 directions x 4) + 9 metrics + J read, U written;
+# stages 2-3 read Un.
+ALG_BYTES_FVM = {"f64": 3 * (40 + 96 + 72 + 8 + 40) + 80.0, "f32b": 3 * (40 + 48 + 36 + 8 + 40) + 80.0}
 
 
 # HBM bytes per launch actually moved, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
@@ -114,17 +115,17 @@ def build_case(f3d, cases, torch, nx, ny, nz, seed=2, sweep=0.0):
     return dict(c=c, p=p, ctx=ctx, m=m, J=J, Q=Q, U=U, inflow=inflow, dt=float(dt_t.item()))
 
 
-def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3, coords=None):
+def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3, geo=None):
     """Warm up; run `table_steps` steps with every library launch bracketed by hipEvents (per-kernel
     table); then time `steps` RK3 steps between two synchronisations with events around the
-    launches of the dominant kernel only (its live average launch time).  `coords` (mesh points):
-    the finite-volume branch.  Returns (ms/step, host wall ms/step, kernel table,
+    launches of the dominant kernel only (its live average launch time).  `geo` (face geometry
+    arrays): the finite-volume branch.  Returns (ms/step, host wall ms/step, kernel table,
     (dominant name, its total ms, its launches))."""
     U, Un, Q, m, J, inflow, dt = state
-    if coords is None:
+    if geo is None:
         step = lambda: ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
     else:
-        step = lambda: ctx.time_step_fvm(1, U, Un, Q, m, J, coords, dt, inflow)
+        step = lambda: ctx.time_step_fvm(1, U, Un, Q, m, J, geo, dt, inflow)
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
@@ -188,10 +189,14 @@ def fvm_bench(f3d, torch, lib, st, U0, Q0, steps, warmup, f32b=True):
     p = f3d.make_params(nx, ny, nz, **c["params"], fvm=f3d.FVM_EIGEN)
     ctx = f3d.Flame3D(p)
     X = [torch.from_numpy(c[a]).to("cuda:0") for a in ("x", "y", "z")]
+    geo = lambda dt_: [torch.zeros((4,) + p.face_shape(d)[1:], dtype=dt_, device="cuda:0") for d in range(3)]
+    G64, G32 = geo(torch.float64), geo(torch.float32)
+    ctx.fvm_geometry(*X, G64)
+    ctx.fvm_geometry(*X, G32)
     ncells = nx * ny * nz
     U, Un, Q = U0.clone(), torch.zeros_like(U0), Q0.clone()
     ms64, _, ktab, _ = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], dt), steps, warmup, lib=lib,
-                                 coords=X)
+                                 geo=G64)
     ctx.nan_check(U)
     v64 = ncells / (ms64 * 1e-3) / 1e6
     progress(f"FVM FP64 {steps} steps: {ms64:.3f} ms/step")
@@ -205,7 +210,7 @@ def fvm_bench(f3d, torch, lib, st, U0, Q0, steps, warmup, f32b=True):
         m32 = [a.float().contiguous() for a in st["m"]]
         U32, Un32, Q32 = U0.clone(), torch.zeros_like(U0), Q0.float().contiguous()
         ms32, _, ktab32, _ = run_steps(torch, ctx, (U32, Un32, Q32, m32, st["J"], st["inflow"].float().contiguous(),
-                                                    dt), steps, warmup, lib=lib, coords=X)
+                                                    dt), steps, warmup, lib=lib, geo=G32)
         ctx.nan_check(U32)
         v32 = ncells / (ms32 * 1e-3) / 1e6
         progress(f"FVM FP32-bulk {steps} steps: {ms32:.3f} ms/step")

@@ -141,6 +141,11 @@ def make_params(Nxp, Nyp, Nzp, **kw):
     return p
 
 
+def _torch():
+    import torch
+    return torch
+
+
 def _ptr(t):
     """Device pointer of a tensor (None -> NULL)."""
     if t is None:
@@ -375,24 +380,30 @@ class Flame3D:
         _check(fn(ctypes.byref(self.params), ws.handle, c_int32(nsteps), _ptr(U), _ptr(Un), _ptr(Q),
                   _ptrs(metrics), _ptr(J), c_double(dt), _ptr(inflow), self.stream))
 
-    def rhs_fvm(self, R, Q, U, metrics, J, coords, phi=None):
-        """Finite-volume RHS (params.fvm = FVM_EIGEN / FVM_CONSER); U with ghosts, coords = (x, y, z)."""
+    def fvm_geometry(self, x, y, z, geo):
+        """Face geometry arrays geo[d] (params.face_shape(d) with 4 components: nx, ny, nz, Area) from
+        the float64 mesh points, in the dtype of geo (float64 or float32)."""
+        sfx = "f64" if geo[0].dtype == _torch().float64 else "f32b"
+        self._call(f"f3d_fvm_geometry_{sfx}", _ptr(x), _ptr(y), _ptr(z), _ptrs(geo))
+
+    def rhs_fvm(self, R, Q, U, metrics, J, geo, phi=None):
+        """Finite-volume RHS (params.fvm = FVM_EIGEN / FVM_CONSER); U with ghosts, geo from fvm_geometry."""
         ws = self.workspace(self._prec(Q))
         fn = getattr(lib(), f"f3d_rhs_fvm_{self._suffix(Q)}")
         _check(fn(ctypes.byref(self.params), ws.handle, _ptr(R), _ptr(Q), _ptr(U), _ptrs(metrics), _ptr(J),
-                  _ptrs(coords), _ptr(phi), self.stream))
+                  _ptrs(geo), _ptr(phi), self.stream))
 
-    def rk3_stage_fvm(self, KRK, U, Un, Q, metrics, J, coords, dt, inflow=None):
+    def rk3_stage_fvm(self, KRK, U, Un, Q, metrics, J, geo, dt, inflow=None):
         ws = self.workspace(self._prec(Q))
         fn = getattr(lib(), f"f3d_rk3_stage_fvm_{self._suffix(Q)}")
         _check(fn(ctypes.byref(self.params), ws.handle, c_int32(KRK), _ptr(U), _ptr(Un), _ptr(Q), _ptrs(metrics),
-                  _ptr(J), _ptrs(coords), c_double(dt), _ptr(inflow), self.stream))
+                  _ptr(J), _ptrs(geo), c_double(dt), _ptr(inflow), self.stream))
 
-    def time_step_fvm(self, nsteps, U, Un, Q, metrics, J, coords, dt, inflow=None):
+    def time_step_fvm(self, nsteps, U, Un, Q, metrics, J, geo, dt, inflow=None):
         ws = self.workspace(self._prec(Q))
         fn = getattr(lib(), f"f3d_time_step_fvm_{self._suffix(Q)}")
         _check(fn(ctypes.byref(self.params), ws.handle, c_int32(nsteps), _ptr(U), _ptr(Un), _ptr(Q),
-                  _ptrs(metrics), _ptr(J), _ptrs(coords), c_double(dt), _ptr(inflow), self.stream))
+                  _ptrs(metrics), _ptr(J), _ptrs(geo), c_double(dt), _ptr(inflow), self.stream))
 
     def cfl_dt(self, dt_out, Q, metrics, J, cfl):
         """Global CFL time step into the device scalar dt_out (float64 tensor of one element)."""

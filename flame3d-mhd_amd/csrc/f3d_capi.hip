@@ -354,7 +354,7 @@ static int flow_advance_impl(const f3d_params* p, const double* U, T* Q, T* Fp, 
   for (int d = 0; d < 3; ++d) {
     int rc;
     if (kp.fvm) {
-      rc = launch_fvm_face<T>(kp, d, Q, U, phi, S[d], nullptr, nullptr, nullptr, F[d], x, y, z, s);
+      rc = launch_fvm_face<T>(kp, d, Q, U, phi, S[d], nullptr, nullptr, nullptr, F[d], x, y, z, nullptr, s);
     } else {
       rc = launch_flux_split<T>(kp, Q, Fp, Fm, S[d], m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], s);
       if (!rc)
@@ -422,8 +422,28 @@ FA_IMPL(f32b, float)
     NEED(z);                                                                                                   \
     KP kp = make_kp(p);                                                                                        \
     kp.fvm = EIG ? F3D_FVM_EIGEN : F3D_FVM_CONSER;                                                             \
-    return launch_fvm_face<T>(kp, D, Q, U, phi, S, nullptr, nullptr, nullptr, F, x, y, z, (hipStream_t)s);     \
+    return launch_fvm_face<T>(kp, D, Q, U, phi, S, nullptr, nullptr, nullptr, F, x, y, z, nullptr,             \
+                              (hipStream_t)s);                                                                 \
   }
+// Face geometry arrays of the fused finite-volume entries (FVM.jl:16-31 per face, once per mesh).
+#define GEO_IMPL(SUF, T)                                                                                     \
+  F3D_API int f3d_fvm_geometry_##SUF(const f3d_params* p, const double* x, const double* y, const double* z, \
+                                     T* const* geo, f3d_stream_t s) {                                        \
+    CHECK_P();                                                                                               \
+    NEED(x);                                                                                                 \
+    NEED(y);                                                                                                 \
+    NEED(z);                                                                                                 \
+    NEED(geo);                                                                                               \
+    for (int d = 0; d < 3; ++d)                                                                              \
+      if (!geo[d]) {                                                                                         \
+        set_error("fvm_geometry: geo[%d] is NULL", d);                                                       \
+        return F3D_E_NULL;                                                                                   \
+      }                                                                                                      \
+    return launch_fvm_geometry<T>(make_kp(p), x, y, z, geo, (hipStream_t)s);                                 \
+  }
+GEO_IMPL(f64, double)
+GEO_IMPL(f32b, float)
+
 FVM_IMPL(eigen_reconstruct_i, f64, double, 0, 1)
 FVM_IMPL(eigen_reconstruct_j, f64, double, 1, 1)
 FVM_IMPL(eigen_reconstruct_k, f64, double, 2, 1)

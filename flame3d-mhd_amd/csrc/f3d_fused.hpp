@@ -89,23 +89,23 @@ int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double*
                  const MetricPtrs<T>& m, const double* J, const T* phi, double* Rout, double dt, hipStream_t s);
 
 // Finite-volume branch (f3d_fvm.hip): RHS and stage as fused_rhs / fused_stage, reconstructing the
-// conserved state U (ghost cells included) with the face geometry of the mesh points coords[3].
+// conserved state U (ghost cells included); geo[3]: face geometry arrays (f3d_fvm_geometry_*).
 template <typename T>
 int fvm_fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
-                  const double* J, const double* const* coords, T* phi, double* R, hipStream_t s);
+                  const double* J, const T* const* geo, T* phi, double* R, hipStream_t s);
 template <typename T>
 int fvm_fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
-                    const MetricPtrs<T>& m, const double* J, const double* const* coords, double dt, hipStream_t s);
+                    const MetricPtrs<T>& m, const double* J, const T* const* geo, double dt, hipStream_t s);
 
 // Stage / multi-step drivers of f3d_stage.hip (with the ghost procedure of the output state).
-// coords == nullptr: finite-difference branch; otherwise finite-volume (p->fvm) with these mesh points,
-// and the ghost procedure refills U's ghost cells as well.
+// geo == nullptr: finite-difference branch; otherwise finite-volume (p->fvm) with these face geometry
+// arrays, and the ghost procedure refills U's ghost cells as well.
 template <typename T>
 int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
-             const double* const* coords = nullptr);
+             const T* const* geo = nullptr);
 template <typename T>
 int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, double* Un, T* Q,
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
-             const double* const* coords = nullptr);
+             const T* const* geo = nullptr);
 }  // namespace f3d

@@ -1,16 +1,20 @@
 // Finite-volume branch of flowAdvance (solver.jl:21-31; FVM.jl) for gfx950.
 //
 // Kernel-by-kernel (Eigen_/Conser_reconstruct_{i,j,k}, flowAdvance):
-//   k_fvm_face       one lane per face: 8-cell stencil of U, sensor window, geometry from the mesh
-//                    points, states (UP7 / WENO7 / WENO5 / minmod, characteristic or component-wise),
-//                    HLLC, times the face area, into the reference's face array.
+//   k_fvm_face       one lane per face: 8-cell stencil of U, sensor window, geometry (from the mesh
+//                    points, or from the face geometry arrays), states (UP7 / WENO7 / WENO5 / minmod,
+//                    characteristic or component-wise), HLLC, times the face area, into the
+//                    reference's face array.
+//   k_fvm_geometry   face unit normals and areas of one direction from the mesh points, once per mesh
+//                    (f3d_fvm_geometry_*): the fused path reads 4 values per face instead of the 24
+//                    mesh points around it.
 // Fused RHS / RK3 stage (the same split as the finite-difference fused path):
 //   fused_sensor     shock sensor (f3d_fused.hip), heavy-face counter cleared
-//   k_fvm_sweep_x    x faces of a 120-cell row segment staged in LDS (U, sensor, per-cell halves of
-//                    the face diagonals): UP7 states + HLLC for the faces the sensor leaves linear,
-//                    R = dF_x; other faces contribute zero and go to the compacted heavy list
+//   k_fvm_sweep_x    x faces of a 120-cell row segment staged in LDS (U, sensor): UP7 states + HLLC
+//                    for the faces the sensor leaves linear, R = dF_x; other faces contribute zero and
+//                    go to the compacted heavy list
 //   k_fvm_march      y and z faces: lanes march along the direction with the last 8 cells' U in a
-//                    register window and the previous cell's diagonal halves; R += dF_y, dF_z
+//                    register window, next cell and next face geometry prefetched; R += dF_y, dF_z
 //   k_fvm_heavy      the listed faces: full characteristic / component-wise reconstruction + HLLC,
 //                    added to the two cells with atomics
 //   fused_update     viscous divergence + update + RK3 blend + c2Prim (f3d_fused.hip)
@@ -29,9 +33,7 @@ constexpr int kXW = 128;  // lanes per x row segment (cells incl. the 4+4 halo)
 template <typename T>
 struct FvmArgs {
   const double* U;
-  const double* x;
-  const double* y;
-  const double* z;
+  const T* geo[3];  // face geometry (nx, ny, nz, Area) per direction, face-array layout
   const T* phi;
   T* R;
   unsigned long long* heavy;
@@ -57,7 +59,8 @@ __global__ __launch_bounds__(256) void k_fvm_face(KP kp, int ni, int nj, int nk,
                                                   const T* __restrict__ S, const T* __restrict__ Ax,
                                                   const T* __restrict__ Ay, const T* __restrict__ Az,
                                                   T* __restrict__ F, const double* __restrict__ x,
-                                                  const double* __restrict__ y, const double* __restrict__ z) {
+                                                  const double* __restrict__ y, const double* __restrict__ z,
+                                                  const T* __restrict__ geo) {
   const int ii = blockIdx.x * 64 + threadIdx.x, jj = blockIdx.y * 4 + threadIdx.y, kk = blockIdx.z;
   if (ii >= ni || jj >= nj || kk >= nk) return;
   // face (ii, jj, kk) of direction D lies between cells (left) NG+idx-1 and NG+idx along D
@@ -71,10 +74,36 @@ __global__ __launch_bounds__(256) void k_fvm_face(KP kp, int ni, int nj, int nk,
     Sc1 = SFROM_A ? row_norm(Ax, Ay, Az, o + dd.st) : S[o + dd.st];
   }
   T out[5];
-  fvm_face_flux<T, EIGEN>(kp, sch, Q, U, x, y, z, o, dd, Sc, Sc1, out);
   const long long fo = ii + (long long)ni * (jj + (long long)nj * kk), fn = (long long)ni * nj * nk;
+  if (geo) {
+    const T n[3] = {geo[fo], geo[fo + fn], geo[fo + 2 * fn]};
+    fvm_face_flux_n<T, EIGEN>(kp, sch, Q, U, o, dd.st, n, geo[fo + 3 * fn], Sc, Sc1, out);
+  } else {
+    fvm_face_flux<T, EIGEN>(kp, sch, Q, U, x, y, z, o, dd, Sc, Sc1, out);
+  }
 #pragma unroll
   for (int n = 0; n < 5; ++n) F[fo + n * fn] = out[n];
+}
+
+// ------------------------------------------------------------------ face geometry
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_fvm_geometry(KP kp, const double* __restrict__ x, const double* __restrict__ y,
+                                                      const double* __restrict__ z, T* __restrict__ geo) {
+  const int ni = kp.Nxp + (D == 0), nj = kp.Nyp + (D == 1), nk = kp.Nzp + (D == 2);
+  const int ii = blockIdx.x * 64 + threadIdx.x, jj = blockIdx.y * 4 + threadIdx.y, kk = blockIdx.z;
+  if (ii >= ni || jj >= nj || kk >= nk) return;
+  const FvmDir dd = fvm_dir(kp, D);
+  const long long o = cidx(kp, kp.NG + ii - (D == 0), kp.NG + jj - (D == 1), kp.NG + kk - (D == 2));
+  double Pc[6], Pn[6];
+  fvm_partials(x, y, z, o, dd.sa, dd.sb, Pc);
+  fvm_partials(x, y, z, o + dd.st, dd.sa, dd.sb, Pn);
+  T n[3], A;
+  fvm_geom<T>(Pc, Pn, dd.flip, n, &A);
+  const long long fo = ii + (long long)ni * (jj + (long long)nj * kk), fn = (long long)ni * nj * nk;
+  geo[fo] = n[0];
+  geo[fo + fn] = n[1];
+  geo[fo + 2 * fn] = n[2];
+  geo[fo + 3 * fn] = A;
 }
 
 // ------------------------------------------------------------------ fused x sweep (LDS rows)
@@ -82,7 +111,6 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L) {
   __shared__ T sU[5][2][kXW];
   __shared__ T sphi[2][kXW];
-  __shared__ double sP[6][2][kXW];
   __shared__ T sF[5][2][kXW];
   const int l = threadIdx.x, r = threadIdx.y;
   const int jj = blockIdx.y * 2 + r, kk = blockIdx.z;
@@ -93,23 +121,23 @@ __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L)
   const int nface = ncell + 1;
   const int xc = kp.NG + f0 - 4 + l;  // cell this lane stages
   const long long nc = kp.nc;
-  const FvmDir dd = fvm_dir(kp, 0);
+  // lane l handles the face between cells l-1 and l of the segment (face index f); its geometry is
+  // requested before the row is staged
+  const bool fact = l >= 4 && l < 4 + nface;
+  const int f = f0 + l - 4;
+  T gn[4] = {0, 0, 0, 0};
+  if (fact) {
+    const long long go = fidx(kp, 0, f, rowok ? jj : kp.Nyp - 1, kk), gs = fcount(kp, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) gn[q] = a.geo[0][go + q * gs];
+  }
   if (l < nface + 7) {
     const long long o = cidx(kp, xc, j, k);
 #pragma unroll
     for (int n = 0; n < 5; ++n) sU[n][r][l] = (T)a.U[o + n * nc];
     sphi[r][l] = a.phi[o];
-    if (l >= 3 && l < nface + 4) {  // left or right cell of a face of this segment
-      double P[6];
-      fvm_partials(a.x, a.y, a.z, o, dd.sa, dd.sb, P);
-#pragma unroll
-      for (int q = 0; q < 6; ++q) sP[q][r][l] = P[q];
-    }
   }
   __syncthreads();
-  // lane l handles the face between cells l-1 and l of the segment (face index f)
-  const bool fact = l >= 4 && l < 4 + nface;
-  const int f = f0 + l - 4;
   int sch = F3D_SCHEME_UP7;
   if (fact) {
     T m = sphi[r][l - 3];
@@ -125,17 +153,9 @@ __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L)
         for (int n = 0; n < 5; ++n) W[s][n] = sU[n][r][l - 4 + s];
       T UL[5], UR[5];
       fvm_up7_states<T>(kp, W, UL, UR);
-      double Pc[6], Pn[6];
+      fvm_hllc<T>(kp, UL, UR, gn[0], gn[1], gn[2], Fl);
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        Pc[q] = sP[q][r][l - 1];
-        Pn[q] = sP[q][r][l];
-      }
-      T nv[3], A;
-      fvm_geom<T>(Pc, Pn, dd.flip, nv, &A);
-      fvm_hllc<T>(kp, UL, UR, nv[0], nv[1], nv[2], Fl);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) Fl[n] *= A;
+      for (int n = 0; n < 5; ++n) Fl[n] *= gn[3];
     }
 #pragma unroll
     for (int n = 0; n < 5; ++n) sF[n][r][l] = Fl[n];
@@ -154,9 +174,8 @@ __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L)
 // ------------------------------------------------------------------ fused y / z march
 // Lane = interior i (64 consecutive), threadIdx.y = transverse line; a block marches a segment of
 // L cells along D.  W[s] holds U of the last 8 cells streamed; once 8 are in, W[0..7] is the stencil
-// c-3..c+4 of the face between W[3] and W[4] (face f = f0 + step - 7).  The diagonal halves of W[4]
-// are formed as it enters position 4 and kept for the next face, so each cell's mesh neighbourhood
-// is read once per march.
+// c-3..c+4 of the face between W[3] and W[4] (face f = f0 + step - 7).  The next cell's U and the
+// next face's geometry are requested a step ahead.
 template <typename T, int D>
 __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
   const int ii = blockIdx.x * 64 + threadIdx.x;
@@ -168,10 +187,13 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
   const int f0 = blockIdx.z * L;
   const int ncell = min(L, Np - f0);
   const int nstep = ncell + 8;
-  const FvmDir dd = fvm_dir(kp, D);
-  const long long st = dd.st, nc = kp.nc;
+  const long long st = dstride(kp, D), nc = kp.nc;
   const long long obase = (D == 1 ? cidx(kp, i, 0, t) : cidx(kp, i, t, 0)) + (long long)(kp.NG + f0 - 4) * st;
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  const int ic = i - kp.NG, tc = t - kp.NG;
+  const long long gs = fcount(kp, D);
+  const long long gstride = D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp;  // next face along D
+  const long long gbase = D == 1 ? fidx(kp, 1, ic, f0, tc) : fidx(kp, 2, ic, tc, f0);
   T W[8][5];
   T ph[8];
 #pragma unroll
@@ -180,8 +202,8 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
 #pragma unroll
     for (int n = 0; n < 5; ++n) W[s][n] = 0;
   }
-  double Pp[6] = {0, 0, 0, 0, 0, 0};
   T Fprev[5] = {0, 0, 0, 0, 0};
+  T gn[4] = {0, 0, 0, 0};  // geometry of the next face to complete
   double nu[5];
   T nph;
 #pragma unroll
@@ -205,8 +227,12 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
       nph = a.phi[o];
     }
     if (step >= 6) {
-      double Pn[6];
-      fvm_partials(a.x, a.y, a.z, obase + (long long)(step - 3) * st, dd.sa, dd.sb, Pn);
+      const T g0 = gn[0], g1 = gn[1], g2 = gn[2], g3 = gn[3];  // face f0 + step - 7 (step >= 7)
+      if (step - 6 <= ncell) {  // request face f0 + step - 6
+        const long long go = gbase + (long long)(step - 6) * gstride;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) gn[q] = a.geo[D][go + q * gs];
+      }
       if (step >= 7) {  // face f0 + step - 7 complete
         const int fi = step - 7;
         const int f = f0 + fi;
@@ -218,11 +244,9 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
         if (sch == F3D_SCHEME_UP7) {
           T UL[5], UR[5];
           fvm_up7_states<T>(kp, W, UL, UR);
-          T nv[3], A;
-          fvm_geom<T>(Pp, Pn, dd.flip, nv, &A);
-          fvm_hllc<T>(kp, UL, UR, nv[0], nv[1], nv[2], Fc);
+          fvm_hllc<T>(kp, UL, UR, g0, g1, g2, Fc);
 #pragma unroll
-          for (int n = 0; n < 5; ++n) Fc[n] *= A;
+          for (int n = 0; n < 5; ++n) Fc[n] *= g3;
         }
         append_heavy(a.heavy, a.hcount, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
                      ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) |
@@ -235,8 +259,6 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
 #pragma unroll
         for (int n = 0; n < 5; ++n) Fprev[n] = Fc[n];
       }
-#pragma unroll
-      for (int q = 0; q < 6; ++q) Pp[q] = Pn[q];
     }
   }
 }
@@ -254,10 +276,13 @@ __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* 
     const long long oc = o - dd.st;
     const T Sc = row_norm(m.m[3 * D], m.m[3 * D + 1], m.m[3 * D + 2], oc);
     const T Sc1 = row_norm(m.m[3 * D], m.m[3 * D + 1], m.m[3 * D + 2], o);
-    T F[5];
-    fvm_face_flux<T, EIGEN>(kp, sch, Q, a.U, a.x, a.y, a.z, oc, dd, Sc, Sc1, F);
     const int ci = (int)(o % kp.Nxt), cj = (int)((o / kp.Nxt) % kp.Nyt), ck = (int)(o / ((long long)kp.Nxt * kp.Nyt));
     const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
+    const long long go = fidx(kp, D, I, Jj, K), gs = fcount(kp, D);
+    const T* g = a.geo[D];
+    const T n3[3] = {g[go], g[go + gs], g[go + 2 * gs]};
+    T F[5];
+    fvm_face_flux_n<T, EIGEN>(kp, sch, Q, a.U, oc, dd.st, n3, g[go + 3 * gs], Sc, Sc1, F);
     const int f = D == 0 ? I : (D == 1 ? Jj : K);
     const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
     const long long rr = rix(kp, I, Jj, K);
@@ -290,8 +315,8 @@ int fvm_seg(int D) {
 
 template <typename T>
 int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
-             const double* const* coords, const T* phi, hipStream_t s) {
-  FvmArgs<T> a{U, coords[0], coords[1], coords[2], phi, ws.R, ws.heavy, ws.hcount};
+             const T* const* geo, const T* phi, hipStream_t s) {
+  FvmArgs<T> a{U, {geo[0], geo[1], geo[2]}, phi, ws.R, ws.heavy, ws.hcount};
   {
     TimedScope ts("fvm_sweep_x", s);
     const int L = seg_len(kp.Nxp, kXW - 8);
@@ -324,7 +349,8 @@ int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, con
 
 template <typename T>
 int launch_fvm_face(const KP& kp, int d, const T* Q, const double* U, const T* phi, const T* S, const T* Ax,
-                    const T* Ay, const T* Az, T* F, const double* x, const double* y, const double* z, hipStream_t s) {
+                    const T* Ay, const T* Az, T* F, const double* x, const double* y, const double* z, const T* geo,
+                    hipStream_t s) {
   TimedScope ts_("fvm_reconstruct", s);
   const int ni = kp.Nxp + (d == 0), nj = kp.Nyp + (d == 1), nk = kp.Nzp + (d == 2);
   const dim3 g((ni + 63) / 64, (nj + 3) / 4, nk), b(64, 4, 1);
@@ -333,10 +359,10 @@ int launch_fvm_face(const KP& kp, int d, const T* Q, const double* U, const T* p
   do {                                                                                                          \
     if (S)                                                                                                      \
       hipLaunchKernelGGL((k_fvm_face<T, D, E, false>), g, b, 0, s, kp, ni, nj, nk, Q, U, phi, S, Ax, Ay, Az, F, \
-                         x, y, z);                                                                              \
+                         x, y, z, geo);                                                                         \
     else                                                                                                        \
       hipLaunchKernelGGL((k_fvm_face<T, D, E, true>), g, b, 0, s, kp, ni, nj, nk, Q, U, phi, S, Ax, Ay, Az, F,  \
-                         x, y, z);                                                                              \
+                         x, y, z, geo);                                                                         \
   } while (0)
   if (d == 0) {
     if (eig) F3D_FVMF(0, true);
@@ -353,35 +379,50 @@ int launch_fvm_face(const KP& kp, int d, const T* Q, const double* U, const T* p
 }
 
 template <typename T>
+int launch_fvm_geometry(const KP& kp, const double* x, const double* y, const double* z, T* const* geo, hipStream_t s) {
+  TimedScope ts_("fvm_geometry", s);
+  for (int d = 0; d < 3; ++d) {
+    const int ni = kp.Nxp + (d == 0), nj = kp.Nyp + (d == 1), nk = kp.Nzp + (d == 2);
+    const dim3 g((ni + 63) / 64, (nj + 3) / 4, nk), b(64, 4, 1);
+    if (d == 0) hipLaunchKernelGGL((k_fvm_geometry<T, 0>), g, b, 0, s, kp, x, y, z, geo[0]);
+    if (d == 1) hipLaunchKernelGGL((k_fvm_geometry<T, 1>), g, b, 0, s, kp, x, y, z, geo[1]);
+    if (d == 2) hipLaunchKernelGGL((k_fvm_geometry<T, 2>), g, b, 0, s, kp, x, y, z, geo[2]);
+  }
+  return check_launch("fvm_geometry");
+}
+
+template <typename T>
 int fvm_fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
-                  const double* J, const double* const* coords, T* phi, double* R, hipStream_t s) {
+                  const double* J, const T* const* geo, T* phi, double* R, hipStream_t s) {
   fused_sensor<T>(kp, ws, Q, phi, s);
-  int rc = fvm_flux<T>(kp, ws, Q, U, m, coords, phi, s);
+  int rc = fvm_flux<T>(kp, ws, Q, U, m, geo, phi, s);
   if (rc) return rc;
   return fused_update<T>(kp, ws, 0, nullptr, nullptr, Q, nullptr, m, J, phi, R, 0.0, s);
 }
 
 template <typename T>
 int fvm_fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
-                    const MetricPtrs<T>& m, const double* J, const double* const* coords, double dt, hipStream_t s) {
+                    const MetricPtrs<T>& m, const double* J, const T* const* geo, double dt, hipStream_t s) {
   if (Qin == Qout && stage_needs_separate_q(kp)) {
     set_error("fused stage: the output state must not alias the input state");
     return F3D_E_ARG;
   }
   fused_sensor<T>(kp, ws, Qin, ws.phi, s);
-  int rc = fvm_flux<T>(kp, ws, Qin, U, m, coords, ws.phi, s);
+  int rc = fvm_flux<T>(kp, ws, Qin, U, m, geo, ws.phi, s);
   if (rc) return rc;
   return fused_update<T>(kp, ws, KRK, U, Un, Qin, Qout, m, J, ws.phi, nullptr, dt, s);
 }
 
 #define F3D_FVM_INST(T)                                                                                           \
   template int launch_fvm_face<T>(const KP&, int, const T*, const double*, const T*, const T*, const T*,         \
-                                  const T*, const T*, T*, const double*, const double*, const double*,            \
+                                  const T*, const T*, T*, const double*, const double*, const double*, const T*,  \
                                   hipStream_t);                                                                   \
+  template int launch_fvm_geometry<T>(const KP&, const double*, const double*, const double*, T* const*,         \
+                                      hipStream_t);                                                               \
   template int fvm_fused_rhs<T>(const KP&, const WsView<T>&, const T*, const double*, const MetricPtrs<T>&,       \
-                                const double*, const double* const*, T*, double*, hipStream_t);                   \
+                                const double*, const T* const*, T*, double*, hipStream_t);                        \
   template int fvm_fused_stage<T>(const KP&, const WsView<T>&, int, double*, double*, const T*, T*,              \
-                                  const MetricPtrs<T>&, const double*, const double* const*, double, hipStream_t);
+                                  const MetricPtrs<T>&, const double*, const T* const*, double, hipStream_t);
 F3D_FVM_INST(double)
 F3D_FVM_INST(float)
 

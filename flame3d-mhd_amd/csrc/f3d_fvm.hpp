@@ -309,19 +309,23 @@ __device__ __forceinline__ int fvm_scheme(const KP& kp, const T* __restrict__ ph
   return scheme_of<T>(kp, m);
 }
 
-// Complete flux (times area) through the face between cells oc and oc+st, for scheme sch:
-// stencil loads, geometry, states, HLLC.  Sc, Sc1: |grad xi_d| of the two cells.
+// Face-array index of the face (I, J, K) of direction d (extents Nxp+[d==0], Nyp+[d==1], Nzp+[d==2],
+// the layout of Fx / Fy / Fz and of the face geometry arrays), and the number of faces.
+F3D_HD long long fidx(const KP& kp, int d, int I, int J, int K) {
+  const long long ni = kp.Nxp + (d == 0), nj = kp.Nyp + (d == 1);
+  return (long long)I + ni * ((long long)J + nj * (long long)K);
+}
+F3D_HD long long fcount(const KP& kp, int d) {
+  return (long long)(kp.Nxp + (d == 0)) * (kp.Nyp + (d == 1)) * (kp.Nzp + (d == 2));
+}
+
+// Flux (times area) through the face between cells oc and oc+st of unit normal n and area A, for
+// scheme sch: stencil loads, states, HLLC.  Sc, Sc1: |grad xi_d| of the two cells.
 template <typename T, bool EIGEN>
-__device__ __forceinline__ void fvm_face_flux(const KP& kp, int sch, const T* __restrict__ Q,
-                                              const double* __restrict__ U, const double* __restrict__ x,
-                                              const double* __restrict__ y, const double* __restrict__ z,
-                                              long long oc, const FvmDir& dd, T Sc, T Sc1, T* __restrict__ F) {
-  const long long nc = kp.nc, st = dd.st;
-  double Pc[6], Pn[6];
-  fvm_partials(x, y, z, oc, dd.sa, dd.sb, Pc);
-  fvm_partials(x, y, z, oc + st, dd.sa, dd.sb, Pn);
-  T n[3], A;
-  fvm_geom<T>(Pc, Pn, dd.flip, n, &A);
+__device__ __forceinline__ void fvm_face_flux_n(const KP& kp, int sch, const T* __restrict__ Q,
+                                                const double* __restrict__ U, long long oc, long long st,
+                                                const T n[3], T A, T Sc, T Sc1, T* __restrict__ F) {
+  const long long nc = kp.nc;
   T W[8][5];
 #pragma unroll
   for (int s = 0; s < 8; ++s)
@@ -342,6 +346,20 @@ __device__ __forceinline__ void fvm_face_flux(const KP& kp, int sch, const T* __
   fvm_hllc<T>(kp, UL, UR, n[0], n[1], n[2], F);
 #pragma unroll
   for (int m = 0; m < 5; ++m) F[m] *= A;
+}
+
+// As fvm_face_flux_n with the geometry formed from the mesh points.
+template <typename T, bool EIGEN>
+__device__ __forceinline__ void fvm_face_flux(const KP& kp, int sch, const T* __restrict__ Q,
+                                              const double* __restrict__ U, const double* __restrict__ x,
+                                              const double* __restrict__ y, const double* __restrict__ z,
+                                              long long oc, const FvmDir& dd, T Sc, T Sc1, T* __restrict__ F) {
+  double Pc[6], Pn[6];
+  fvm_partials(x, y, z, oc, dd.sa, dd.sb, Pc);
+  fvm_partials(x, y, z, oc + dd.st, dd.sa, dd.sb, Pn);
+  T n[3], A;
+  fvm_geom<T>(Pc, Pn, dd.flip, n, &A);
+  fvm_face_flux_n<T, EIGEN>(kp, sch, Q, U, oc, dd.st, n, A, Sc, Sc1, F);
 }
 
 }  // namespace f3d

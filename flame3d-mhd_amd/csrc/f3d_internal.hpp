@@ -66,9 +66,14 @@ template <typename T> int launch_exchange(const KP&, T* Q, int NV, hipStream_t);
 template <typename T> int launch_fill(const KP&, T* Q, double* U, const T* inflow, hipStream_t);
 
 // finite-volume face fluxes of direction d (f3d_fvm.hip), variant kp.fvm; S == nullptr: |grad xi_d|
-// from the metric row A (Ax, Ay, Az)
+// from the metric row A (Ax, Ay, Az); geo == nullptr: face geometry from the mesh points x, y, z,
+// else from the face geometry array of direction d
 template <typename T>
 int launch_fvm_face(const KP&, int d, const T* Q, const double* U, const T* phi, const T* S, const T* Ax,
-                    const T* Ay, const T* Az, T* F, const double* x, const double* y, const double* z, hipStream_t);
+                    const T* Ay, const T* Az, T* F, const double* x, const double* y, const double* z, const T* geo,
+                    hipStream_t);
+// face geometry arrays (nx, ny, nz, Area) of the three directions
+template <typename T>
+int launch_fvm_geometry(const KP&, const double* x, const double* y, const double* z, T* const* geo, hipStream_t);
 
 }  // namespace f3d

@@ -204,33 +204,32 @@ int rhs_chain(f3d_workspace* ws, const KP& kp, const T* Q, const MetricPtrs<T>& 
 // Kernel-by-kernel finite-volume branch (solver.jl:21-31) into the workspace face arrays.
 template <typename T>
 int rhs_chain_fvm(f3d_workspace* ws, const KP& kp, const T* Q, const double* U, const MetricPtrs<T>& m,
-                  const double* J, const double* const* coords, T* phi, hipStream_t s) {
+                  const double* J, const T* const* geo, T* phi, hipStream_t s) {
   T* F[3] = {(T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz};
   int rc = launch_shock_sensor<T>(kp, phi, Q, s);
   for (int d = 0; d < 3 && !rc; ++d)
-    rc = launch_fvm_face<T>(kp, d, Q, U, phi, nullptr, m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], F[d], coords[0],
-                            coords[1], coords[2], s);
+    rc = launch_fvm_face<T>(kp, d, Q, U, phi, nullptr, m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2], F[d], nullptr,
+                            nullptr, nullptr, geo[d], s);
   if (!rc && kp.viscous) rc = launch_viscous<T>(kp, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, Q, m, J, s);
   return rc;
 }
 
-// U, coords: finite-volume branch when coords != nullptr.
+// U, geo: finite-volume branch when geo != nullptr.
 template <typename T>
 int rhs_impl(const f3d_params* p, f3d_workspace* ws, double* R, const T* Q, const T* const* metrics,
-             const double* J, T* phi, hipStream_t s, const double* U = nullptr,
-             const double* const* coords = nullptr) {
+             const double* J, T* phi, hipStream_t s, const double* U = nullptr, const T* const* geo = nullptr) {
   KP kp = make_kp(p);
   MetricPtrs<T> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
   T* ph = phi ? phi : (T*)ws->phi;
   int rc;
   if (use_fused()) {
-    if (coords)
-      rc = fvm_fused_rhs<T>(kp, ws_view<T>(ws), Q, U, m, J, coords, ph, R, s);
+    if (geo)
+      rc = fvm_fused_rhs<T>(kp, ws_view<T>(ws), Q, U, m, J, geo, ph, R, s);
     else
       rc = fused_rhs<T>(kp, ws_view<T>(ws), Q, m, J, ph, R, s);
   } else {
-    rc = coords ? rhs_chain_fvm<T>(ws, kp, Q, U, m, J, coords, ph, s) : rhs_chain<T>(ws, kp, Q, m, J, ph, s);
+    rc = geo ? rhs_chain_fvm<T>(ws, kp, Q, U, m, J, geo, ph, s) : rhs_chain<T>(ws, kp, Q, m, J, ph, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_zero_interior, dim3((kp.Nxp + 255) / 256, kp.Nyp, kp.Nzp), dim3(256), 0, s, kp, R);
     rc = launch_div<T>(kp, R, (T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, 1.0, J, s);
@@ -244,18 +243,18 @@ namespace f3d {
 template <typename T>
 int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
-             const double* const* coords) {
+             const T* const* geo) {
   KP kp = make_kp(p);
   MetricPtrs<T> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
   int rc = F3D_OK;
   if (use_fused()) {
-    rc = coords ? fvm_fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, coords, dt, s)
+    rc = geo ? fvm_fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, geo, dt, s)
                 : fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, dt, s);
   } else {
     if (Qout != Qin) F3D_HIP(hipMemcpyAsync(Qout, Qin, sizeof(T) * 6 * kp.nc, hipMemcpyDeviceToDevice, s));
     if (KRK == 1) F3D_HIP(hipMemcpyAsync(Un, U, sizeof(double) * 5 * kp.nc, hipMemcpyDeviceToDevice, s));  // solver.jl:418
-    rc = coords ? rhs_chain_fvm<T>(ws, kp, Qout, U, m, J, coords, (T*)ws->phi, s)
+    rc = geo ? rhs_chain_fvm<T>(ws, kp, Qout, U, m, J, geo, (T*)ws->phi, s)
                 : rhs_chain<T>(ws, kp, Qout, m, J, (T*)ws->phi, s);
     if (!rc)
       rc = launch_div<T>(kp, U, (T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, dt, J, s);
@@ -265,19 +264,19 @@ int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double*
   }
   if (!rc) rc = launch_exchange<T>(kp, Qout, 6, s);
   // U ghosts are read only by the finite-volume reconstruction; fillGhost refills them from Q's there
-  if (!rc) rc = launch_fill<T>(kp, Qout, coords ? U : nullptr, inflow, s);
+  if (!rc) rc = launch_fill<T>(kp, Qout, geo ? U : nullptr, inflow, s);
   return rc;
 }
 
 template <typename T>
 int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, double* Un, T* Q,
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
-             const double* const* coords) {
+             const T* const* geo) {
   const KP kp = make_kp(p);
   if (!stage_needs_separate_q(kp)) {
     for (int t = 0; t < nsteps; ++t)
       for (int KRK = 1; KRK <= 3; ++KRK) {
-        const int rc = stage_io<T>(p, ws, KRK, U, Un, Q, Q, metrics, J, dt, inflow, s, coords);
+        const int rc = stage_io<T>(p, ws, KRK, U, Un, Q, Q, metrics, J, dt, inflow, s, geo);
         if (rc) return rc;
       }
     return F3D_OK;
@@ -286,9 +285,9 @@ int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, doub
   T* Qa = (T*)ws->Qa;
   T* Qb = (T*)ws->Qb;
   for (int t = 0; t < nsteps; ++t) {
-    int rc = stage_io<T>(p, ws, 1, U, Un, Q, Qa, metrics, J, dt, inflow, s, coords);
-    if (!rc) rc = stage_io<T>(p, ws, 2, U, Un, Qa, Qb, metrics, J, dt, inflow, s, coords);
-    if (!rc) rc = stage_io<T>(p, ws, 3, U, Un, Qb, Q, metrics, J, dt, inflow, s, coords);
+    int rc = stage_io<T>(p, ws, 1, U, Un, Q, Qa, metrics, J, dt, inflow, s, geo);
+    if (!rc) rc = stage_io<T>(p, ws, 2, U, Un, Qa, Qb, metrics, J, dt, inflow, s, geo);
+    if (!rc) rc = stage_io<T>(p, ws, 3, U, Un, Qb, Q, metrics, J, dt, inflow, s, geo);
     if (rc) return rc;
   }
   return F3D_OK;
@@ -297,11 +296,11 @@ int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, doub
 template <typename T>
 int stage_inplace(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, T* Q,
                   const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
-                  const double* const* coords = nullptr) {
+                  const T* const* geo = nullptr) {
   const KP kp = make_kp(p);
-  if (!stage_needs_separate_q(kp)) return stage_io<T>(p, ws, KRK, U, Un, Q, Q, metrics, J, dt, inflow, s, coords);
+  if (!stage_needs_separate_q(kp)) return stage_io<T>(p, ws, KRK, U, Un, Q, Q, metrics, J, dt, inflow, s, geo);
   T* Qa = (T*)ws->Qa;
-  const int rc = stage_io<T>(p, ws, KRK, U, Un, Q, Qa, metrics, J, dt, inflow, s, coords);
+  const int rc = stage_io<T>(p, ws, KRK, U, Un, Q, Qa, metrics, J, dt, inflow, s, geo);
   if (rc) return rc;
   F3D_HIP(hipMemcpyAsync(Q, Qa, sizeof(T) * 6 * kp.nc, hipMemcpyDeviceToDevice, s));
   return F3D_OK;
@@ -312,7 +311,7 @@ template int stage_io<double>(const f3d_params*, f3d_workspace*, int, double*, d
                               const double* const*);
 template int stage_io<float>(const f3d_params*, f3d_workspace*, int, double*, double*, const float*, float*,
                              const float* const*, const double*, double, const float*, hipStream_t,
-                             const double* const*);
+                             const float* const*);
 template int steps_io<double>(const f3d_params*, f3d_workspace*, int, double*, double*, double*,
                               const double* const*, const double*, double, const double*, hipStream_t,
                               const double* const*);
@@ -345,7 +344,8 @@ int check_stage_args(const f3d_params* p, const f3d_workspace* ws, int prec, con
 
 // The entries without _fvm serve the finite-difference branch, the _fvm entries the finite-volume
 // one; each rejects the other's params.
-int check_branch(const f3d_params* p, bool fvm_entry, const double* U, const double* const* coords, const char* who) {
+template <typename T>
+int check_branch(const f3d_params* p, bool fvm_entry, const double* U, const T* const* geo, const char* who) {
   if (fvm_entry && p->fvm == F3D_FDM) {
     set_error("%s: params select the finite-difference branch (fvm = F3D_FDM); use the entry without _fvm", who);
     return F3D_E_ARG;
@@ -355,8 +355,8 @@ int check_branch(const f3d_params* p, bool fvm_entry, const double* U, const dou
     return F3D_E_ARG;
   }
   if (fvm_entry) {
-    if (!U || !coords || !coords[0] || !coords[1] || !coords[2]) {
-      set_error("%s: the finite-volume branch needs U and coords x, y, z", who);
+    if (!U || !geo || !geo[0] || !geo[1] || !geo[2]) {
+      set_error("%s: the finite-volume branch needs U and the face geometry arrays geo[3]", who);
       return F3D_E_NULL;
     }
   }
@@ -466,7 +466,7 @@ F3D_API int f3d_rhs_f64(const f3d_params* p, f3d_workspace* ws, double* R, const
                         const double* const* metrics, const double* J, double* phi, f3d_stream_t s) {
   int rc = check_stage_args<double>(p, ws, F3D_PREC_F64, R, R, Q, metrics, J, (const double*)1, "rhs_f64");
   if (rc) return rc;
-  rc = check_branch(p, false, nullptr, nullptr, "rhs_f64");
+  rc = check_branch<double>(p, false, nullptr, nullptr, "rhs_f64");
   if (rc) return rc;
   return rhs_impl<double>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s);
 }
@@ -474,7 +474,7 @@ F3D_API int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, cons
                          const float* const* metrics, const double* J, float* phi, f3d_stream_t s) {
   int rc = check_stage_args<float>(p, ws, F3D_PREC_F32B, R, R, Q, metrics, J, (const float*)1, "rhs_f32b");
   if (rc) return rc;
-  rc = check_branch(p, false, nullptr, nullptr, "rhs_f32b");
+  rc = check_branch<float>(p, false, nullptr, nullptr, "rhs_f32b");
   if (rc) return rc;
   return rhs_impl<float>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s);
 }
@@ -484,7 +484,7 @@ F3D_API int f3d_rk3_stage_f64(const f3d_params* p, f3d_workspace* ws, int32_t KR
                               const double* inflow, f3d_stream_t s) {
   int rc = check_stage_args<double>(p, ws, F3D_PREC_F64, U, Un, Q, metrics, J, inflow, "rk3_stage_f64");
   if (rc) return rc;
-  rc = check_branch(p, false, nullptr, nullptr, "rk3_stage_f64");
+  rc = check_branch<double>(p, false, nullptr, nullptr, "rk3_stage_f64");
   if (rc) return rc;
   if (KRK < 1 || KRK > 3) {
     set_error("rk3_stage_f64: KRK=%d not in 1..3", KRK);
@@ -497,7 +497,7 @@ F3D_API int f3d_rk3_stage_f32b(const f3d_params* p, f3d_workspace* ws, int32_t K
                                const float* inflow, f3d_stream_t s) {
   int rc = check_stage_args<float>(p, ws, F3D_PREC_F32B, U, Un, Q, metrics, J, inflow, "rk3_stage_f32b");
   if (rc) return rc;
-  rc = check_branch(p, false, nullptr, nullptr, "rk3_stage_f32b");
+  rc = check_branch<float>(p, false, nullptr, nullptr, "rk3_stage_f32b");
   if (rc) return rc;
   if (KRK < 1 || KRK > 3) {
     set_error("rk3_stage_f32b: KRK=%d not in 1..3", KRK);
@@ -511,7 +511,7 @@ F3D_API int f3d_time_step_f64(const f3d_params* p, f3d_workspace* ws, int32_t ns
                               const double* inflow, f3d_stream_t s) {
   int rc = check_stage_args<double>(p, ws, F3D_PREC_F64, U, Un, Q, metrics, J, inflow, "time_step_f64");
   if (rc) return rc;
-  rc = check_branch(p, false, nullptr, nullptr, "time_step_f64");
+  rc = check_branch<double>(p, false, nullptr, nullptr, "time_step_f64");
   if (rc) return rc;
   if (nsteps < 0) {
     set_error("time_step_f64: nsteps < 0");
@@ -524,7 +524,7 @@ F3D_API int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t n
                                const float* inflow, f3d_stream_t s) {
   int rc = check_stage_args<float>(p, ws, F3D_PREC_F32B, U, Un, Q, metrics, J, inflow, "time_step_f32b");
   if (rc) return rc;
-  rc = check_branch(p, false, nullptr, nullptr, "time_step_f32b");
+  rc = check_branch<float>(p, false, nullptr, nullptr, "time_step_f32b");
   if (rc) return rc;
   if (nsteps < 0) {
     set_error("time_step_f32b: nsteps < 0");
@@ -534,46 +534,46 @@ F3D_API int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t n
 }
 
 F3D_API int f3d_rhs_fvm_f64(const f3d_params* p, f3d_workspace* ws, double* R, const double* Q, const double* U,
-                            const double* const* metrics, const double* J, const double* const* coords, double* phi,
+                            const double* const* metrics, const double* J, const double* const* geo, double* phi,
                             f3d_stream_t s) {
   int rc = check_stage_args<double>(p, ws, F3D_PREC_F64, R, R, Q, metrics, J, (const double*)1, "rhs_fvm_f64");
-  if (!rc) rc = check_branch(p, true, U, coords, "rhs_fvm_f64");
+  if (!rc) rc = check_branch<double>(p, true, U, geo, "rhs_fvm_f64");
   if (rc) return rc;
-  return rhs_impl<double>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s, U, coords);
+  return rhs_impl<double>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s, U, geo);
 }
 F3D_API int f3d_rhs_fvm_f32b(const f3d_params* p, f3d_workspace* ws, double* R, const float* Q, const double* U,
-                             const float* const* metrics, const double* J, const double* const* coords, float* phi,
+                             const float* const* metrics, const double* J, const float* const* geo, float* phi,
                              f3d_stream_t s) {
   int rc = check_stage_args<float>(p, ws, F3D_PREC_F32B, R, R, Q, metrics, J, (const float*)1, "rhs_fvm_f32b");
-  if (!rc) rc = check_branch(p, true, U, coords, "rhs_fvm_f32b");
+  if (!rc) rc = check_branch<float>(p, true, U, geo, "rhs_fvm_f32b");
   if (rc) return rc;
-  return rhs_impl<float>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s, U, coords);
+  return rhs_impl<float>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s, U, geo);
 }
 
 #define F3D_STAGE_FVM(SUF, T, PREC)                                                                              \
   F3D_API int f3d_rk3_stage_fvm_##SUF(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U, double* Un, \
-                                      T* Q, const T* const* metrics, const double* J, const double* const* coords, \
+                                      T* Q, const T* const* metrics, const double* J, const T* const* geo,        \
                                       double dt, const T* inflow, f3d_stream_t s) {                              \
     int rc = check_stage_args<T>(p, ws, PREC, U, Un, Q, metrics, J, inflow, "rk3_stage_fvm_" #SUF);              \
-    if (!rc) rc = check_branch(p, true, U, coords, "rk3_stage_fvm_" #SUF);                                       \
+    if (!rc) rc = check_branch<T>(p, true, U, geo, "rk3_stage_fvm_" #SUF);                                       \
     if (rc) return rc;                                                                                           \
     if (KRK < 1 || KRK > 3) {                                                                                    \
       set_error("rk3_stage_fvm_" #SUF ": KRK=%d not in 1..3", KRK);                                              \
       return F3D_E_ARG;                                                                                          \
     }                                                                                                            \
-    return stage_inplace<T>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s, coords);               \
+    return stage_inplace<T>(p, ws, KRK, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s, geo);                  \
   }                                                                                                              \
   F3D_API int f3d_time_step_fvm_##SUF(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,          \
                                       double* Un, T* Q, const T* const* metrics, const double* J,                 \
-                                      const double* const* coords, double dt, const T* inflow, f3d_stream_t s) {  \
+                                      const T* const* geo, double dt, const T* inflow, f3d_stream_t s) {         \
     int rc = check_stage_args<T>(p, ws, PREC, U, Un, Q, metrics, J, inflow, "time_step_fvm_" #SUF);              \
-    if (!rc) rc = check_branch(p, true, U, coords, "time_step_fvm_" #SUF);                                       \
+    if (!rc) rc = check_branch<T>(p, true, U, geo, "time_step_fvm_" #SUF);                                       \
     if (rc) return rc;                                                                                           \
     if (nsteps < 0) {                                                                                            \
       set_error("time_step_fvm_" #SUF ": nsteps < 0");                                                           \
       return F3D_E_ARG;                                                                                          \
     }                                                                                                            \
-    return steps_io<T>(p, ws, nsteps, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s, coords);                 \
+    return steps_io<T>(p, ws, nsteps, U, Un, Q, metrics, J, dt, inflow, (hipStream_t)s, geo);                    \
   }
 F3D_STAGE_FVM(f64, double, F3D_PREC_F64)
 F3D_STAGE_FVM(f32b, float, F3D_PREC_F32B)

@@ -298,32 +298,42 @@ int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, d
                        double* Un, float* Q, const float* const* metrics, const double* J,
                        double dt, const float* inflow, f3d_stream_t s);
 
+/* Face geometry of the finite-volume branch, once per mesh (as f3d_metrics_f64 for the metrics):
+ * geo[d] (d = 0, 1, 2) = [face extents of Fx / Fy / Fz, 4] holding (nx, ny, nz, Area) of every
+ * face, n the unit normal pointing along +d (FVM.jl:16-31, 250-264, 465-479), formed in FP64 from the
+ * mesh points and stored in the bulk type.  The fused entries below read it instead of forming the
+ * geometry from the 24 mesh points around each face on every stage. */
+int f3d_fvm_geometry_f64(const f3d_params* p, const double* x, const double* y, const double* z,
+                         double* const* geo, f3d_stream_t s);
+int f3d_fvm_geometry_f32b(const f3d_params* p, const double* x, const double* y, const double* z,
+                          float* const* geo, f3d_stream_t s);
+
 /* Finite-volume form of the three entries above (p->fvm = F3D_FVM_EIGEN or F3D_FVM_CONSER; the
  * entries without _fvm reject finite-volume params and these reject F3D_FDM).  Extra arguments: the
  * conserved state U (its ghost cells are read by the reconstruction; the stage drivers refill them
- * together with Q's, boundary.jl:1-156) and coords[3] = device pointers x, y, z (double, see the
- * finite-volume entries).  f3d_rhs_fvm_* reads U and does not modify it. */
+ * together with Q's, boundary.jl:1-156) and geo[3] from f3d_fvm_geometry_*.  f3d_rhs_fvm_* reads U
+ * and does not modify it. */
 int f3d_rhs_fvm_f64(const f3d_params* p, f3d_workspace* ws, double* R, const double* Q,
                     const double* U, const double* const* metrics, const double* J,
-                    const double* const* coords, double* phi, f3d_stream_t s);
+                    const double* const* geo, double* phi, f3d_stream_t s);
 int f3d_rhs_fvm_f32b(const f3d_params* p, f3d_workspace* ws, double* R, const float* Q,
                      const double* U, const float* const* metrics, const double* J,
-                     const double* const* coords, float* phi, f3d_stream_t s);
+                     const float* const* geo, float* phi, f3d_stream_t s);
 int f3d_rk3_stage_fvm_f64(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U,
                           double* Un, double* Q, const double* const* metrics, const double* J,
-                          const double* const* coords, double dt, const double* inflow,
+                          const double* const* geo, double dt, const double* inflow,
                           f3d_stream_t s);
 int f3d_rk3_stage_fvm_f32b(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U,
                            double* Un, float* Q, const float* const* metrics, const double* J,
-                           const double* const* coords, double dt, const float* inflow,
+                           const float* const* geo, double dt, const float* inflow,
                            f3d_stream_t s);
 int f3d_time_step_fvm_f64(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
                           double* Un, double* Q, const double* const* metrics, const double* J,
-                          const double* const* coords, double dt, const double* inflow,
+                          const double* const* geo, double dt, const double* inflow,
                           f3d_stream_t s);
 int f3d_time_step_fvm_f32b(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
                            double* Un, float* Q, const float* const* metrics, const double* J,
-                           const double* const* coords, double dt, const float* inflow,
+                           const float* const* geo, double dt, const float* inflow,
                            f3d_stream_t s);
 
 /* Global CFL time step (no reference counterpart as a reduction; per-cell formula of compute_dt,

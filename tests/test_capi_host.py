@@ -133,6 +133,7 @@ def test_fvm_selector_and_entry_validation():
             fn = getattr(L, f"f3d_{name}_{suf}")
             assert fn(ctypes.byref(p), None, None, None, None, None, None, None, None, None) == -1
     assert L.f3d_rhs_fvm_f64(ctypes.byref(p), None, None, None, None, None, None, None, None, None) == -1
+    assert L.f3d_fvm_geometry_f32b(ctypes.byref(p), None, None, None, None, None) == -1
     assert L.f3d_time_step_fvm_f32b(ctypes.byref(p), None, 1, None, None, None, None, None, None,
                                     ctypes.c_double(1.0), None, None) == -1
     # the adjoint differentiates the finite-difference branch only: rejected before any allocation

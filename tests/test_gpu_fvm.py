@@ -85,6 +85,15 @@ def check_f32b(name, gpu, o32, o64, scale):
     assert e_gpu <= 2 * e_floor + 1e-6, f"{name}: {e_gpu:.3e} > 2*{e_floor:.3e}+1e-6"
 
 
+def geo_dev(torch, ctx, case):
+    """Face geometry arrays of the case's mesh, computed on the GPU (f3d_fvm_geometry_*)."""
+    p = case.params
+    tdt = torch.float64 if case.bulk == np.float64 else torch.float32
+    geo = [torch.zeros((4,) + p.face_shape(d)[1:], dtype=tdt, device="cuda:0") for d in range(3)]
+    ctx.fvm_geometry(*[dev(torch, a) for a in case.coords], geo)
+    return geo
+
+
 def fvm_rhs_scales(case, U):
     A = np.zeros(case.params.cell_shape(5))
     Oracle(case.params).rhs_absterms_fvm(A, case.Q, U, case.metrics, case.J, case.coords)
@@ -151,6 +160,23 @@ def test_reconstruct_faces(torch_mod, case):
         check_f32b(f"{case.name} {name} d={d}", Fgs[d], Fo[d], F64, sc)
 
 
+def test_face_geometry(torch_mod, case):
+    """f3d_fvm_geometry_* against the oracle's face geometry (FVM.jl:16-31 order of summation there,
+    differences first on the GPU; on the ramp's stretched mesh the reference order loses ~1e-13 to
+    the cancellation of nearly equal coordinates): FP64 within 1e-12 (normals, and areas relative to
+    the largest); FP32-bulk: the FP64 values rounded to float."""
+    torch = torch_mod
+    p, o, ctx = case.params, Oracle(case.params), f3d.Flame3D(case.params)
+    geo = geo_dev(torch, ctx, case)
+    for d in range(3):
+        g = np.zeros((4,) + p.face_shape(d)[1:])
+        o.fvm_geometry(d, *case.coords, g)
+        h = host(geo[d]).astype(np.float64)
+        tol = 1e-12 if case.bulk == np.float64 else 1e-7
+        assert np.abs(h[:3] - g[:3]).max() < tol, f"normals d={d}"
+        assert np.abs(h[3] - g[3]).max() < tol * np.abs(g[3]).max(), f"areas d={d}"
+
+
 def test_flow_advance_and_rhs_fvm(torch_mod, case):
     """flowAdvance's finite-volume branch + div, and the fused f3d_rhs_fvm, against the oracle RHS."""
     torch = torch_mod
@@ -171,7 +197,7 @@ def test_flow_advance_and_rhs_fvm(torch_mod, case):
     Rk = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
     ctx.div(Rk, *F, *Fv, 1.0, Jg)
     Rg = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
-    ctx.rhs_fvm(Rg, Qg, Ug0, mg, Jg, Xg)
+    ctx.rhs_fvm(Rg, Qg, Ug0, mg, Jg, geo_dev(torch, ctx, case))
     assert np.array_equal(host(Ug0), U), "rhs_fvm / flowAdvance modified U"
     if case.bulk == np.float64:
         fp64_fvm_rhs_check(f"{case.name} fvm={p.fvm} flowAdvance+div", host(Rk), case, U)
@@ -210,9 +236,8 @@ def test_time_step_fvm(torch_mod, case, nsteps):
     Unp = np.zeros_like(U)
     o.time_step_fvm(nsteps, Up, Unp, Qp, m, case.J, case.coords, dt, case.inflow)
     Ug, Ung, Qg = dev(torch, U), dev(torch, Un), dev(torch, Q)
-    ctx.time_step_fvm(nsteps, Ug, Ung, Qg, [dev(torch, a) for a in m], dev(torch, case.J),
-                      [dev(torch, a) for a in case.coords], dt,
-                      None if case.inflow is None else dev(torch, case.inflow))
+    ctx.time_step_fvm(nsteps, Ug, Ung, Qg, [dev(torch, a) for a in m], dev(torch, case.J), geo_dev(torch, ctx, case),
+                      dt, None if case.inflow is None else dev(torch, case.inflow))
     floor = 1e-12 if case.bulk == np.float64 else 1e-6
     sens_u = relerr(Up, Uo)
     sens_q = relerr(Qp, Qo)
@@ -232,7 +257,7 @@ def test_stage_entry_matches_time_step(torch_mod):
     U = case.U()
     dt = 0.2 * o.cfl_dt(case.Q, case.metrics, case.J, 1.0)
     mg = [dev(torch, a) for a in case.metrics]
-    Jg, Xg, ig = dev(torch, case.J), [dev(torch, a) for a in case.coords], dev(torch, case.inflow)
+    Jg, Xg, ig = dev(torch, case.J), geo_dev(torch, ctx, case), dev(torch, case.inflow)
     Ua, Una, Qa = dev(torch, U), dev(torch, np.zeros_like(U)), dev(torch, case.Q)
     for KRK in (1, 2, 3):
         ctx.rk3_stage_fvm(KRK, Ua, Una, Qa, mg, Jg, Xg, dt, ig)
@@ -256,4 +281,4 @@ def test_branch_selection_is_enforced(torch_mod):
         f3d.Flame3D(fv.params).rhs(R, dev(torch, fv.Q), mg, dev(torch, fv.J))
     with pytest.raises(f3d.F3DError, match="finite-difference"):
         f3d.Flame3D(fd.params).rhs_fvm(R, dev(torch, fd.Q), dev(torch, fd.U()), mg, dev(torch, fd.J),
-                                       [dev(torch, a) for a in fd.coords])
+                                       geo_dev(torch, f3d.Flame3D(fv.params), fv))
