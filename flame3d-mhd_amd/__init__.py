@@ -2,7 +2,8 @@
 
 The reference (rygan-code/Flame3D-MHD, Julia + CUDA.jl) launches per-stage kernels from
 `time_step` (solver.jl:191-680).  This module exposes the same entry points with the same names
-and argument order -- `shockSensor`, `fluxSplit_SW`, `advect_xc/yc/zc`, `viscousFlux`, `div`,
+and argument order -- `shockSensor`, `fluxSplit_SW/LF/VL/AUSM`, `advect_xc/yc/zc`, the
+component-wise `advect_x/y/z`, `viscousFlux`, `div`,
 `linComb`, `c2Prim`, `prim2c`, `exchange_ghost`, `fillGhost`, `flowAdvance`, `time_step`, and the
 finite-volume branch's `Eigen_reconstruct_{i,j,k}` / `Conser_reconstruct_{i,j,k}` (FVM.jl) -- as
 methods of a `Flame3D` context that carries what the reference keeps in `const` globals
@@ -24,7 +25,8 @@ __all__ = [
     "make_params",
     "BC_PERIODIC", "BC_DIRICHLET", "BC_ZEROGRAD", "BC_WALL_ADIABATIC", "BC_WALL_ISOTHERMAL", "BC_INFLOW",
     "PREC_F64", "PREC_F32B", "SCHEME_UP7", "SCHEME_WENO7", "SCHEME_WENO5", "SCHEME_NND2",
-    "FDM", "FVM_EIGEN", "FVM_CONSER",
+    "FDM", "FVM_EIGEN", "FVM_CONSER", "SPLIT_SW", "SPLIT_LF", "SPLIT_VL", "SPLIT_AUSM",
+    "LES_NONE", "LES_SMAG", "LES_WALE",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -34,6 +36,8 @@ BC_PERIODIC, BC_DIRICHLET, BC_ZEROGRAD, BC_WALL_ADIABATIC, BC_WALL_ISOTHERMAL, B
 PREC_F64, PREC_F32B = 0, 1
 SCHEME_UP7, SCHEME_WENO7, SCHEME_WENO5, SCHEME_NND2 = range(4)
 FDM, FVM_EIGEN, FVM_CONSER = range(3)
+SPLIT_SW, SPLIT_LF, SPLIT_VL, SPLIT_AUSM = range(4)
+LES_NONE, LES_SMAG, LES_WALE = range(3)
 NG = 4
 
 
@@ -46,6 +50,9 @@ class F3DParams(ctypes.Structure):
         ("Pr", c_double), ("hybrid_phi1", c_double), ("hybrid_phi2", c_double), ("hybrid_phi3", c_double),
         ("Linear_phi", c_double), ("weno7_eps", c_double), ("weno5_eps", c_double),
         ("q_inf", c_double * 6), ("T_wall", c_double),
+        ("split", c_int32), ("character", c_int32), ("les", c_int32), ("lts", c_int32),
+        ("filtering", c_int32), ("filtering_nonlinear", c_int32), ("filtering_interval", c_int32), ("pad_", c_int32),
+        ("lts_cfl", c_double), ("filtering_rth", c_double), ("filtering_s0", c_double),
     ]
 
     @property
@@ -268,6 +275,33 @@ class Flame3D:
     def fluxSplit_SW(self, Q, Fp, Fm, S, Ax, Ay, Az):
         """split.jl:2-68"""
         self._call(f"f3d_flux_split_sw_{self._suffix(Q)}", *map(_ptr, (Q, Fp, Fm, S, Ax, Ay, Az)))
+
+    def fluxSplit_LF(self, Q, Fp, Fm, S, Ax, Ay, Az):
+        """split.jl:70-108 (local Lax-Friedrichs)"""
+        self._call(f"f3d_flux_split_lf_{self._suffix(Q)}", *map(_ptr, (Q, Fp, Fm, S, Ax, Ay, Az)))
+
+    def fluxSplit_VL(self, Q, Fp, Fm, S, Ax, Ay, Az):
+        """split.jl:110-183 (van Leer)"""
+        self._call(f"f3d_flux_split_vl_{self._suffix(Q)}", *map(_ptr, (Q, Fp, Fm, S, Ax, Ay, Az)))
+
+    def fluxSplit_AUSM(self, Q, Fp, Fm, S, Ax, Ay, Az):
+        """split.jl:185-255 (AUSM)"""
+        self._call(f"f3d_flux_split_ausm_{self._suffix(Q)}", *map(_ptr, (Q, Fp, Fm, S, Ax, Ay, Az)))
+
+    def _advect_comp(self, d, F, phi, S, Fp, Fm, NV):
+        self._call(f"f3d_advect_{'xyz'[d]}_{self._suffix(Fp)}", *map(_ptr, (F, phi, S, Fp, Fm)), c_int32(NV))
+
+    def advect_x(self, F, phi, S, Fp, Fm, NV=5):
+        """schemes.jl:86-289 (component-wise)"""
+        self._advect_comp(0, F, phi, S, Fp, Fm, NV)
+
+    def advect_y(self, F, phi, S, Fp, Fm, NV=5):
+        """schemes.jl:291-494 (component-wise)"""
+        self._advect_comp(1, F, phi, S, Fp, Fm, NV)
+
+    def advect_z(self, F, phi, S, Fp, Fm, NV=5):
+        """schemes.jl:496-698 (component-wise)"""
+        self._advect_comp(2, F, phi, S, Fp, Fm, NV)
 
     def _advect(self, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
         self._call(f"f3d_advect_{'xyz'[d]}c_{self._suffix(Q)}", *map(_ptr, (F, phi, S, Fp, Fm, Q, Ax, Ay, Az)))

@@ -488,6 +488,12 @@ int check_fdm(const f3d_params* p, const char* who) {
     set_error("%s: the adjoint covers the finite-difference branch only (params fvm = %d)", who, p->fvm);
     return F3D_E_ARG;
   }
+  if (p->split != F3D_SPLIT_SW || p->character != 1 || p->les != F3D_LES_NONE || p->lts) {
+    set_error("%s: the adjoint covers the Steger-Warming, characteristic, constant-viscosity, global-dt "
+              "path only (params split = %d, character = %d, les = %d, lts = %d)",
+              who, p->split, p->character, p->les, p->lts);
+    return F3D_E_ARG;
+  }
   return F3D_OK;
 }
 

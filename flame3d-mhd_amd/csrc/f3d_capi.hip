@@ -38,6 +38,10 @@ KP make_kp(const f3d_params* p) {
   k.Nzt = p->Nzp + 2 * p->NG;
   k.viscous = p->viscous;
   k.fvm = p->fvm;
+  k.split = p->split;
+  k.character = p->character;
+  k.les = p->les;
+  k.lts = p->lts;
   for (int i = 0; i < 6; ++i) k.bc[i] = p->bc[i];
   k.nc = (long long)k.Nxt * k.Nyt * k.Nzt;
   k.gamma = p->gamma;
@@ -115,6 +119,17 @@ F3D_API void f3d_params_init(f3d_params* p, int32_t Nxp, int32_t Nyp, int32_t Nz
   p->Linear_phi = 1.0;
   p->weno7_eps = 1e-20;
   p->weno5_eps = 1e-16;
+  // flag-selected alternatives, run.jl:4-5, 21-22, 50-54, 68-69
+  p->split = F3D_SPLIT_SW;
+  p->character = 1;
+  p->les = F3D_LES_NONE;
+  p->lts = 0;
+  p->lts_cfl = 0.5;
+  p->filtering = 0;
+  p->filtering_nonlinear = 0;
+  p->filtering_interval = 100;
+  p->filtering_rth = 1e-5;
+  p->filtering_s0 = 1.0;
   // freestream of boundary.jl:14-19
   p->q_inf[0] = 1.0;
   p->q_inf[1] = 0.5;
@@ -153,6 +168,27 @@ F3D_API int f3d_params_check(const f3d_params* p) {
       set_error("params: F3D_BC_INFLOW is only defined on the x-low face");
       return F3D_E_ARG;
     }
+  }
+  if (p->split < F3D_SPLIT_SW || p->split > F3D_SPLIT_AUSM) {
+    set_error("params: split=%d unknown (F3D_SPLIT_SW, _LF, _VL, _AUSM)", p->split);
+    return F3D_E_ARG;
+  }
+  if ((p->character != 0 && p->character != 1) || (p->lts != 0 && p->lts != 1) ||
+      (p->filtering != 0 && p->filtering != 1) || (p->filtering_nonlinear != 0 && p->filtering_nonlinear != 1)) {
+    set_error("params: character, lts, filtering and filtering_nonlinear are 0 or 1");
+    return F3D_E_ARG;
+  }
+  if (p->les < F3D_LES_NONE || p->les > F3D_LES_WALE) {
+    set_error("params: les=%d unknown (F3D_LES_NONE, _SMAG, _WALE)", p->les);
+    return F3D_E_ARG;
+  }
+  if (p->lts && !(p->lts_cfl > 0)) {
+    set_error("params: local time stepping needs lts_cfl > 0");
+    return F3D_E_ARG;
+  }
+  if (p->filtering && (p->filtering_interval < 1 || !(p->filtering_s0 >= 0))) {
+    set_error("params: filtering needs filtering_interval >= 1 and filtering_s0 >= 0");
+    return F3D_E_ARG;
   }
   if (p->fvm < F3D_FDM || p->fvm > F3D_FVM_CONSER) {
     set_error("params: fvm=%d unknown (F3D_FDM, F3D_FVM_EIGEN or F3D_FVM_CONSER)", p->fvm);
@@ -216,21 +252,30 @@ SENSOR_IMPL(f32b, float)
 MASK_IMPL(f64, double)
 MASK_IMPL(f32b, float)
 
-#define SPLIT_IMPL(SUF, T)                                                                                   \
-  F3D_API int f3d_flux_split_sw_##SUF(const f3d_params* p, const T* Q, T* Fp, T* Fm, const T* S, const T* Ax, \
-                                      const T* Ay, const T* Az, f3d_stream_t s) {                            \
-    CHECK_P();                                                                                               \
-    NEED(Q);                                                                                                 \
-    NEED(Fp);                                                                                                \
-    NEED(Fm);                                                                                                \
-    NEED(S);                                                                                                 \
-    NEED(Ax);                                                                                                \
-    NEED(Ay);                                                                                                \
-    NEED(Az);                                                                                                \
-    return launch_flux_split<T>(make_kp(p), Q, Fp, Fm, S, Ax, Ay, Az, (hipStream_t)s);                       \
+// fluxSplit_SW / _LF / _VL / _AUSM  split.jl:2 / 70 / 110 / 185 (the method is the entry point's)
+#define SPLIT_IMPL(NAME, METHOD, SUF, T)                                                                       \
+  F3D_API int f3d_flux_split_##NAME##_##SUF(const f3d_params* p, const T* Q, T* Fp, T* Fm, const T* S,         \
+                                            const T* Ax, const T* Ay, const T* Az, f3d_stream_t s) {           \
+    CHECK_P();                                                                                                 \
+    NEED(Q);                                                                                                   \
+    NEED(Fp);                                                                                                  \
+    NEED(Fm);                                                                                                  \
+    NEED(S);                                                                                                   \
+    NEED(Ax);                                                                                                  \
+    NEED(Ay);                                                                                                  \
+    NEED(Az);                                                                                                  \
+    KP kp = make_kp(p);                                                                                        \
+    kp.split = METHOD;                                                                                         \
+    return launch_flux_split<T>(kp, Q, Fp, Fm, S, Ax, Ay, Az, (hipStream_t)s);                                 \
   }
-SPLIT_IMPL(f64, double)
-SPLIT_IMPL(f32b, float)
+SPLIT_IMPL(sw, F3D_SPLIT_SW, f64, double)
+SPLIT_IMPL(sw, F3D_SPLIT_SW, f32b, float)
+SPLIT_IMPL(lf, F3D_SPLIT_LF, f64, double)
+SPLIT_IMPL(lf, F3D_SPLIT_LF, f32b, float)
+SPLIT_IMPL(vl, F3D_SPLIT_VL, f64, double)
+SPLIT_IMPL(vl, F3D_SPLIT_VL, f32b, float)
+SPLIT_IMPL(ausm, F3D_SPLIT_AUSM, f64, double)
+SPLIT_IMPL(ausm, F3D_SPLIT_AUSM, f32b, float)
 
 #define ADVECT_IMPL(NAME, D, SUF, T)                                                                         \
   F3D_API int f3d_advect_##NAME##_##SUF(const f3d_params* p, T* F, const T* phi, const T* S, const T* Fp,    \
@@ -246,7 +291,9 @@ SPLIT_IMPL(f32b, float)
     NEED(Ax);                                                                                                \
     NEED(Ay);                                                                                                \
     NEED(Az);                                                                                                \
-    return launch_advect<T>(make_kp(p), D, F, phi, S, Fp, Fm, Q, Ax, Ay, Az, (hipStream_t)s);                \
+    KP kp = make_kp(p);                                                                                      \
+    kp.character = 1;                                                                                        \
+    return launch_advect<T>(kp, D, F, phi, S, Fp, Fm, Q, Ax, Ay, Az, (hipStream_t)s);                        \
   }
 ADVECT_IMPL(xc, 0, f64, double)
 ADVECT_IMPL(yc, 1, f64, double)
@@ -254,6 +301,32 @@ ADVECT_IMPL(zc, 2, f64, double)
 ADVECT_IMPL(xc, 0, f32b, float)
 ADVECT_IMPL(yc, 1, f32b, float)
 ADVECT_IMPL(zc, 2, f32b, float)
+
+// advect_x / _y / _z  schemes.jl:86 / 291 / 496: component-wise, the first NV split-flux components
+#define CADVECT_IMPL(NAME, D, SUF, T)                                                                         \
+  F3D_API int f3d_advect_##NAME##_##SUF(const f3d_params* p, T* F, const T* phi, const T* S, const T* Fp,     \
+                                        const T* Fm, int32_t NV, f3d_stream_t s) {                            \
+    CHECK_P();                                                                                                \
+    NEED(F);                                                                                                  \
+    NEED(phi);                                                                                                \
+    NEED(S);                                                                                                  \
+    NEED(Fp);                                                                                                 \
+    NEED(Fm);                                                                                                 \
+    if (NV < 1 || NV > 5) {                                                                                   \
+      set_error("advect_" #NAME ": NV=%d out of 1..5", NV);                                                   \
+      return F3D_E_ARG;                                                                                       \
+    }                                                                                                         \
+    KP kp = make_kp(p);                                                                                       \
+    kp.character = 0;                                                                                         \
+    return launch_advect<T>(kp, D, F, phi, S, Fp, Fm, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, \
+                            (const T*)nullptr, (hipStream_t)s, NV);                                           \
+  }
+CADVECT_IMPL(x, 0, f64, double)
+CADVECT_IMPL(y, 1, f64, double)
+CADVECT_IMPL(z, 2, f64, double)
+CADVECT_IMPL(x, 0, f32b, float)
+CADVECT_IMPL(y, 1, f32b, float)
+CADVECT_IMPL(z, 2, f32b, float)
 
 #define VISC_IMPL(SUF, T)                                                                                      \
   F3D_API int f3d_viscous_flux_##SUF(const f3d_params* p, T* Fvx, T* Fvy, T* Fvz, const T* Q, const T* m0,     \

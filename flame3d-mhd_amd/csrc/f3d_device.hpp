@@ -19,6 +19,7 @@ struct KP {
   int Nxt, Nyt, Nzt, viscous;
   int bc[6];
   int fvm;       // F3D_FDM / F3D_FVM_EIGEN / F3D_FVM_CONSER
+  int split, character, les, lts;  // F3D_SPLIT_*, characteristic flag, F3D_LES_*, local time stepping
   long long nc;  // cells per variable, Nxt*Nyt*Nzt
   double gamma, Rg, Cp, Cs, Ts, Pr;
   double phi1, phi2, phi3, eps7, eps5, Twall;
@@ -127,6 +128,90 @@ F3D_HD void sw_split(const KP& kp, T rho, T u, T v, T w, T Tt, T A1, T A2, T A3,
   fm[4] = t1 * (E1M * vv + E2M * vvc1 + E3M * vvc2 + W2 * (E2M + E3M));
 }
 
+// The three other splittings of split.jl (LF 70-108, van Leer 110-183, AUSM 185-255) for one cell from
+// rho, u, v, w, p and the metric row A with S = |A|; c = sqrt(gamma p / rho).
+template <typename T>
+F3D_HD void other_split(const KP& kp, int method, T rho, T u, T v, T w, T pr, T A1, T A2, T A3, T ss,
+                        T* __restrict__ fp, T* __restrict__ fm) {
+  const T gam = (T)kp.gamma;
+  const T c = sqrt(gam * pr / rho);
+  T un = A1 * u + A2 * v + A3 * w;
+  const T q2 = u * u + v * v + w * w;
+  if (method == F3D_SPLIT_LF) {
+    const T l0 = fabs(un) + c * ss;
+    const T E0 = pr / (gam - 1) + T(0.5) * rho * q2;
+    const T f[5] = {rho * un, rho * u * un + A1 * pr, rho * v * un + A2 * pr, rho * w * un + A3 * pr, (E0 + pr) * un};
+    const T U[5] = {rho, rho * u, rho * v, rho * w, E0};
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      fp[q] = T(0.5) * (f[q] + l0 * U[q]);
+      fm[q] = T(0.5) * (f[q] - l0 * U[q]);
+    }
+    return;
+  }
+  const T M = un / (c * ss);
+  const T E0 = pr / (gam - 1) + T(0.5) * rho * q2;
+  const T h = gam * pr / (rho * (gam - 1)) + T(0.5) * q2;
+  if (M <= T(-1) || M >= T(1)) {
+    const T f[5] = {rho * un, rho * u * un + A1 * pr, rho * v * un + A2 * pr, rho * w * un + A3 * pr,
+                    method == F3D_SPLIT_VL ? (E0 + pr) * un : rho * h * un};
+    const bool right = M >= T(1);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      fp[q] = right ? f[q] : T(0);
+      fm[q] = right ? T(0) : f[q];
+    }
+    return;
+  }
+  const T f1p = T(0.25) * ((M + 1) * (M + 1)) * rho * c * ss;
+  const T f1m = -T(0.25) * ((M - 1) * (M - 1)) * rho * c * ss;
+  if (method == F3D_SPLIT_VL) {
+    un = un / ss;
+    const T t1 = (-un + 2 * c) / (gam * ss), t2 = (-un - 2 * c) / (gam * ss);
+    const T t3 = T(1) / (2 * (gam * gam - 1));
+    const T t4 = T(0.5) * (q2 - un * un);
+    const T ep = (gam - 1) * un + 2 * c, em = (gam - 1) * un - 2 * c;
+    fp[0] = f1p;
+    fp[1] = f1p * (A1 * t1 + u);
+    fp[2] = f1p * (A2 * t1 + v);
+    fp[3] = f1p * (A3 * t1 + w);
+    fp[4] = f1p * (ep * ep * t3 + t4);
+    fm[0] = f1m;
+    fm[1] = f1m * (A1 * t2 + u);
+    fm[2] = f1m * (A2 * t2 + v);
+    fm[3] = f1m * (A3 * t2 + w);
+    fm[4] = f1m * (em * em * t3 + t4);
+    return;
+  }
+  const T Pp = pr * (T(0.25) * ((M + 1) * (M + 1)) * (2 - M));
+  const T Pm = pr * (T(0.25) * ((M - 1) * (M - 1)) * (2 + M));
+  fp[0] = f1p;
+  fp[1] = f1p * u + A1 * Pp;
+  fp[2] = f1p * v + A2 * Pp;
+  fp[3] = f1p * w + A3 * Pp;
+  fp[4] = f1p * h;
+  fm[0] = f1m;
+  fm[1] = f1m * u + A1 * Pm;
+  fm[2] = f1m * v + A2 * Pm;
+  fm[3] = f1m * w + A3 * Pm;
+  fm[4] = f1m * h;
+}
+
+// Split of one cell by the method SPLIT (F3D_SPLIT_*): Steger-Warming reads T (Q index 5), the others p
+// (Q index 4); `qe` is that value.
+template <typename T, int SPLIT>
+F3D_HD void split_cell(const KP& kp, T rho, T u, T v, T w, T qe, T A1, T A2, T A3, T ss, T* __restrict__ fp,
+                       T* __restrict__ fm) {
+  if (SPLIT == F3D_SPLIT_SW)
+    sw_split<T>(kp, rho, u, v, w, qe, A1, A2, A3, ss, fp, fm);
+  else
+    other_split<T>(kp, SPLIT, rho, u, v, w, qe, A1, A2, A3, ss, fp, fm);
+}
+template <int SPLIT>
+F3D_HD constexpr int split_qvar() {
+  return SPLIT == F3D_SPLIT_SW ? 5 : 4;
+}
+
 // ------------------------------------------------------------------ WENO kernels
 // Nonlinear WENO7 weights (schemes.jl:871-879).  Double: the literal form of the reference, whose
 // weights are Float64.  Float: the same normalised weights formed from ratios to the smallest
@@ -229,7 +314,9 @@ F3D_HD T minmod(T a, T b) {
 // Flux through the face between stencil cells c-1 and c (advect_xc, schemes.jl:731-1009).
 //   FP(s,m), FM(s,m): split fluxes of variable m at stencil offset s in [-4, 3] from cell c.
 //   qL/qR: (u,v,w,T) of cells c-1 / c; AL/AR: metric row; SL/SR: |A|.
-template <typename T, class FPL, class FML>
+// CHAR = false: the component-wise advect_* (schemes.jl:86-698): the same schemes on each split-flux
+// component, no characteristic projection (qL, qR, AL, AR unused).
+template <typename T, bool CHAR = true, class FPL, class FML>
 F3D_HD void face_flux(const KP& kp, int sch, const FPL& FP, const FML& FM,
                                           const T* qL, const T* qR, const T* AL, const T* AR, T SL,
                                           T SR, T* __restrict__ out) {
@@ -252,6 +339,35 @@ F3D_HD void face_flux(const KP& kp, int sch, const FPL& FP, const FML& FM,
       const T fp = FP(-1, n) + T(0.5) * minmod(FP(0, n) - FP(-1, n), FP(-1, n) - FP(-2, n));
       const T fm = FM(0, n) - T(0.5) * minmod(FM(1, n) - FM(0, n), FM(0, n) - FM(-1, n));
       out[n] = fp + fm;
+    }
+    return;
+  }
+  if (!CHAR) {  // component-wise WENO7 / WENO5  schemes.jl:144-276
+    const T s0 = T(2) / (SL + SR);
+    if (sch == F3D_SCHEME_WENO7) {
+      const T eps = (T)kp.eps7;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        T vp[7], vm[7];
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+          vp[s] = FP(s - 4, n);
+          vm[s] = FM(3 - s, n);
+        }
+        out[n] = (weno7(vp, s0, eps) + weno7(vm, s0, eps)) * (T(1) / T(12));
+      }
+    } else {
+      const T eps = (T)kp.eps5;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        T vp[5], vm[5];
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+          vp[s] = FP(s - 3, n);
+          vm[s] = FM(2 - s, n);
+        }
+        out[n] = (weno5(vp, s0, eps) + weno5(vm, s0, eps)) * (T(1) / T(6));
+      }
     }
     return;
   }

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, co
 // ------------------------------------------------------------------ x sweep (LDS row segments)
 constexpr int XW = 128;  // lanes per row segment (cells incl. the 4+4 halo)
 
-template <typename T>
+template <typename T, int SPLIT>
 __global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L) {
   __shared__ T sFp[5][2][XW];
   __shared__ T sFm[5][2][XW];
@@ -107,7 +107,8 @@ __global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L) {
     const T A1 = a.m.m[0][o], A2 = a.m.m[1][o], A3 = a.m.m[2][o];
     const T S = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
     T fp[5], fm[5];
-    sw_split<T>(kp, a.Q[o], a.Q[o + nc], a.Q[o + 2 * nc], a.Q[o + 3 * nc], a.Q[o + 5 * nc], A1, A2, A3, S, fp, fm);
+    split_cell<T, SPLIT>(kp, a.Q[o], a.Q[o + nc], a.Q[o + 2 * nc], a.Q[o + 3 * nc], a.Q[o + split_qvar<SPLIT>() * nc],
+                         A1, A2, A3, S, fp, fm);
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
       sFp[n][r][l] = fp[n];
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L) {
 //   sum_q c_q Fp[NG+f-4+q] + sum_q c_q Fm[NG+f+3-q]
 // in a shift register acc[0..7] (acc[m] holds face f0+step-7+m); acc[0] is complete once cell
 // NG+f+3 has streamed in.  The next cell's loads are issued before the current cell is split.
-template <typename T, int D>
+template <typename T, int D, int SPLIT>
 __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
   const int ii = blockIdx.x * 64 + threadIdx.x;
   const int tt = blockIdx.y * 4 + threadIdx.y;
@@ -181,7 +182,8 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
   T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph;
   {
     const long long o = obase;
-    nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc]; nq5 = a.Q[o + 5 * nc];
+    nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
+    nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
     na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
   }
 #pragma unroll 1
@@ -189,11 +191,12 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
     const T q0 = nq0, q1 = nq1, q2 = nq2, q3 = nq3, q5 = nq5, A1 = na1, A2 = na2, A3 = na3, ph = nph;
     if (step + 1 < nstep) {
       const long long o = obase + (long long)(step + 1) * st;
-      nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc]; nq5 = a.Q[o + 5 * nc];
+      nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
+      nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
       na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
     }
     T fp[5], fm[5];
-    sw_split<T>(kp, q0, q1, q2, q3, q5, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp, fm);
+    split_cell<T, SPLIT>(kp, q0, q1, q2, q3, q5, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp, fm);
 #pragma unroll
     for (int q = 0; q < 7; ++q)
 #pragma unroll
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
 }
 
 // ------------------------------------------------------------------ heavy faces
-template <typename T>
+template <typename T, int SPLIT, bool CHAR>
 __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a) {
   const unsigned int n = *a.hcount;
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp, nc = kp.nc;
@@ -249,7 +252,8 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a) {
     for (int s = 0; s < 8; ++s) {
       const long long oc = o + (s - 4) * st;
       const T A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
-      sw_split<T>(kp, a.Q[oc], a.Q[oc + nc], a.Q[oc + 2 * nc], a.Q[oc + 3 * nc], a.Q[oc + 5 * nc], A1, A2, A3,
+      split_cell<T, SPLIT>(kp, a.Q[oc], a.Q[oc + nc], a.Q[oc + 2 * nc], a.Q[oc + 3 * nc],
+                           a.Q[oc + split_qvar<SPLIT>() * nc], A1, A2, A3,
                   sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp[s], fm[s]);
     }
     auto FP = [&](int s, int q) { return fp[s + 4][q]; };
@@ -262,7 +266,7 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a) {
     const T SL = sqrt(AL[0] * AL[0] + AL[1] * AL[1] + AL[2] * AL[2]);
     const T SR = sqrt(AR[0] * AR[0] + AR[1] * AR[1] + AR[2] * AR[2]);
     T Fs[5], Fu[5];
-    face_flux<T>(kp, sch, FP, FM, qL, qR, AL, AR, SL, SR, Fs);
+    face_flux<T, CHAR>(kp, sch, FP, FM, qL, qR, AL, AR, SL, SR, Fs);
 #pragma unroll
     for (int q = 0; q < 5; ++q) {  // the UP7 flux the sweep accumulated, same summation order
       T acc = 0;
@@ -821,34 +825,48 @@ void fused_sensor(const KP& kp, const WsView<T>& ws, const T* Q, T* phi, hipStre
 }
 
 // Inviscid increments of the finite-difference branch into ws.R: x, y, z sweeps and the heavy-face
-// corrections (after fused_sensor).
-template <typename T>
-static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J,
-                      T* phi, hipStream_t s) {
+// corrections (after fused_sensor), for the splitting SPLIT and the reconstruction kp.character.
+template <typename T, int SPLIT>
+static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J,
+                            T* phi, hipStream_t s) {
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
   {
     TimedScope ts("fused_sweep_x", s);
     const int L = seg_len(kp.Nxp, XW - 8);
-    hipLaunchKernelGGL(k_sweep_x<T>, dim3((kp.Nxp + L - 1) / L, (kp.Nyp + 1) / 2, kp.Nzp), dim3(XW, 2, 1), 0, s, kp,
-                       a, L);
+    hipLaunchKernelGGL((k_sweep_x<T, SPLIT>), dim3((kp.Nxp + L - 1) / L, (kp.Nyp + 1) / 2, kp.Nzp), dim3(XW, 2, 1), 0,
+                       s, kp, a, L);
   }
   {
     TimedScope ts("fused_sweep_y", s);
     const int L = seg_len(kp.Nyp, march_seg(1, (int)sizeof(T)));
-    hipLaunchKernelGGL((k_sweep_march<T, 1>), dim3((kp.Nxp + 63) / 64, (kp.Nzp + 3) / 4, (kp.Nyp + L - 1) / L),
+    hipLaunchKernelGGL((k_sweep_march<T, 1, SPLIT>), dim3((kp.Nxp + 63) / 64, (kp.Nzp + 3) / 4, (kp.Nyp + L - 1) / L),
                        dim3(64, 4, 1), 0, s, kp, a, L);
   }
   {
     TimedScope ts("fused_sweep_z", s);
     const int L = seg_len(kp.Nzp, march_seg(2, (int)sizeof(T)));
-    hipLaunchKernelGGL((k_sweep_march<T, 2>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + L - 1) / L),
+    hipLaunchKernelGGL((k_sweep_march<T, 2, SPLIT>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + L - 1) / L),
                        dim3(64, 4, 1), 0, s, kp, a, L);
   }
   {
     TimedScope ts("fused_heavy", s);
-    hipLaunchKernelGGL(k_heavy<T>, dim3(2048), dim3(64), 0, s, kp, a);
+    if (kp.character)
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(2048), dim3(64), 0, s, kp, a);
+    else
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(2048), dim3(64), 0, s, kp, a);
   }
   return check_launch("fused sweeps");
+}
+
+template <typename T>
+static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J,
+                      T* phi, hipStream_t s) {
+  switch (kp.split) {
+    case F3D_SPLIT_LF: return fused_flux_split<T, F3D_SPLIT_LF>(kp, ws, Q, m, J, phi, s);
+    case F3D_SPLIT_VL: return fused_flux_split<T, F3D_SPLIT_VL>(kp, ws, Q, m, J, phi, s);
+    case F3D_SPLIT_AUSM: return fused_flux_split<T, F3D_SPLIT_AUSM>(kp, ws, Q, m, J, phi, s);
+    default: return fused_flux_split<T, F3D_SPLIT_SW>(kp, ws, Q, m, J, phi, s);
+  }
 }
 
 template <typename T>

@@ -50,9 +50,10 @@ template <typename T> int launch_scheme_mask(const KP&, uint8_t*, uint8_t*, uint
 template <typename T>
 int launch_flux_split(const KP&, const T* Q, T* Fp, T* Fm, const T* S, const T* Ax, const T* Ay, const T* Az,
                       hipStream_t);
+// split method kp.split; advect: characteristic when kp.character, else component-wise on NV fluxes
 template <typename T>
 int launch_advect(const KP&, int d, T* F, const T* phi, const T* S, const T* Fp, const T* Fm, const T* Q,
-                  const T* Ax, const T* Ay, const T* Az, hipStream_t);
+                  const T* Ax, const T* Ay, const T* Az, hipStream_t, int NV = 5);
 template <typename T>
 int launch_viscous(const KP&, T* Fvx, T* Fvy, T* Fvz, const T* Q, const MetricPtrs<T>& m, const double* J,
                    hipStream_t);

@@ -43,9 +43,9 @@ __global__ __launch_bounds__(256) void k_scheme_mask(KP kp, Range3 r, uint8_t* _
   mask[ii + (long long)r.ni * (jj + (long long)r.nj * kk)] = (uint8_t)scheme_of<T>(kp, m);
 }
 
-// ------------------------------------------------------------------ fluxSplit_SW  split.jl:2-68
-// SFROM_A: S recomputed as |A| instead of read from an array.
-template <typename T, bool SFROM_A>
+// ------------------------------------------------------------------ fluxSplit_SW / LF / VL / AUSM  split.jl
+// SFROM_A: S recomputed as |A| instead of read from an array; SPLIT: F3D_SPLIT_*.
+template <typename T, bool SFROM_A, int SPLIT>
 __global__ __launch_bounds__(256) void k_flux_split(KP kp, Range3 r, const T* __restrict__ Q, T* __restrict__ Fp,
                                                     T* __restrict__ Fm, const T* __restrict__ S,
                                                     const T* __restrict__ Ax, const T* __restrict__ Ay,
@@ -55,7 +55,8 @@ __global__ __launch_bounds__(256) void k_flux_split(KP kp, Range3 r, const T* __
   const T A1 = Ax[o], A2 = Ay[o], A3 = Az[o];
   const T ss = SFROM_A ? sqrt(A1 * A1 + A2 * A2 + A3 * A3) : S[o];
   T fp[5], fm[5];
-  sw_split<T>(kp, Q[o], Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], A1, A2, A3, ss, fp, fm);
+  split_cell<T, SPLIT>(kp, Q[o], Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + split_qvar<SPLIT>() * nc], A1, A2, A3,
+                       ss, fp, fm);
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
     Fp[o + n * nc] = fp[n];
@@ -63,13 +64,15 @@ __global__ __launch_bounds__(256) void k_flux_split(KP kp, Range3 r, const T* __
   }
 }
 
-// ------------------------------------------------------------------ advect_{x,y,z}c
-template <typename T, int D, bool SFROM_A>
+// ------------------------------------------------------------------ advect_{x,y,z}c / advect_{x,y,z}
+// CHAR: characteristic advect_*c (schemes.jl:700-1643); otherwise the component-wise advect_*
+// (schemes.jl:86-698), which needs neither Q nor the metric rows (S only) and writes the first NV fluxes.
+template <typename T, int D, bool SFROM_A, bool CHAR>
 __global__ __launch_bounds__(256) void k_advect(KP kp, Range3 r, T* __restrict__ F, const T* __restrict__ phi,
                                                 const T* __restrict__ S, const T* __restrict__ Fp,
                                                 const T* __restrict__ Fm, const T* __restrict__ Q,
                                                 const T* __restrict__ Ax, const T* __restrict__ Ay,
-                                                const T* __restrict__ Az) {
+                                                const T* __restrict__ Az, int NV) {
   RANGE_IDX(r);
   const long long st = dstride(kp, D), o = cidx(kp, i, j, k), nc = kp.nc;
   T m = phi[o - 3 * st];
@@ -79,17 +82,25 @@ __global__ __launch_bounds__(256) void k_advect(KP kp, Range3 r, T* __restrict__
   auto FP = [&](int s, int n) { return Fp[o + s * st + n * nc]; };
   auto FM = [&](int s, int n) { return Fm[o + s * st + n * nc]; };
   const long long oL = o - st;
-  const T qL[4] = {Q[oL + nc], Q[oL + 2 * nc], Q[oL + 3 * nc], Q[oL + 5 * nc]};
-  const T qR[4] = {Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc]};
-  const T AL[3] = {Ax[oL], Ay[oL], Az[oL]};
-  const T AR[3] = {Ax[o], Ay[o], Az[o]};
-  const T SL = SFROM_A ? sqrt(AL[0] * AL[0] + AL[1] * AL[1] + AL[2] * AL[2]) : S[oL];
-  const T SR = SFROM_A ? sqrt(AR[0] * AR[0] + AR[1] * AR[1] + AR[2] * AR[2]) : S[o];
   T out[5];
-  face_flux<T>(kp, sch, FP, FM, qL, qR, AL, AR, SL, SR, out);
+  if (CHAR) {
+    const T qL[4] = {Q[oL + nc], Q[oL + 2 * nc], Q[oL + 3 * nc], Q[oL + 5 * nc]};
+    const T qR[4] = {Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc]};
+    const T AL[3] = {Ax[oL], Ay[oL], Az[oL]};
+    const T AR[3] = {Ax[o], Ay[o], Az[o]};
+    const T SL = SFROM_A ? sqrt(AL[0] * AL[0] + AL[1] * AL[1] + AL[2] * AL[2]) : S[oL];
+    const T SR = SFROM_A ? sqrt(AR[0] * AR[0] + AR[1] * AR[1] + AR[2] * AR[2]) : S[o];
+    face_flux<T, true>(kp, sch, FP, FM, qL, qR, AL, AR, SL, SR, out);
+  } else {
+    const T SL = SFROM_A ? sqrt(Ax[oL] * Ax[oL] + Ay[oL] * Ay[oL] + Az[oL] * Az[oL]) : S[oL];
+    const T SR = SFROM_A ? sqrt(Ax[o] * Ax[o] + Ay[o] * Ay[o] + Az[o] * Az[o]) : S[o];
+    face_flux<T, false>(kp, sch, FP, FM, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+                        SL, SR, out);
+  }
   const long long fo = ii + (long long)r.ni * (jj + (long long)r.nj * kk), fn = (long long)r.ni * r.nj * r.nk;
 #pragma unroll
-  for (int n = 0; n < 5; ++n) F[fo + n * fn] = out[n];
+  for (int n = 0; n < 5; ++n)
+    if (n < NV) F[fo + n * fn] = out[n];
 }
 
 // ------------------------------------------------------------------ viscousFlux  viscous.jl:1-134
@@ -324,32 +335,49 @@ int launch_scheme_mask(const KP& kp, uint8_t* mx, uint8_t* my, uint8_t* mz, cons
 template <typename T>
 int launch_flux_split(const KP& kp, const T* Q, T* Fp, T* Fm, const T* S, const T* Ax, const T* Ay, const T* Az,
                       hipStream_t s) {
-  TimedScope ts_("flux_split_sw", s);
+  static const char* names[4] = {"flux_split_sw", "flux_split_lf", "flux_split_vl", "flux_split_ausm"};
+  TimedScope ts_(names[kp.split & 3], s);
   Range3 r{0, 0, 0, kp.Nxt, kp.Nyt, kp.Nzt};
-  if (S)
-    hipLaunchKernelGGL((k_flux_split<T, false>), range_grid(r), kBlock, 0, s, kp, r, Q, Fp, Fm, S, Ax, Ay, Az);
-  else
-    hipLaunchKernelGGL((k_flux_split<T, true>), range_grid(r), kBlock, 0, s, kp, r, Q, Fp, Fm, S, Ax, Ay, Az);
-  return check_launch("flux_split_sw");
+#define F3D_SPL(M)                                                                                                 \
+  do {                                                                                                             \
+    if (S)                                                                                                         \
+      hipLaunchKernelGGL((k_flux_split<T, false, M>), range_grid(r), kBlock, 0, s, kp, r, Q, Fp, Fm, S, Ax, Ay, Az); \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_flux_split<T, true, M>), range_grid(r), kBlock, 0, s, kp, r, Q, Fp, Fm, S, Ax, Ay, Az);  \
+  } while (0)
+  switch (kp.split) {
+    case F3D_SPLIT_LF: F3D_SPL(F3D_SPLIT_LF); break;
+    case F3D_SPLIT_VL: F3D_SPL(F3D_SPLIT_VL); break;
+    case F3D_SPLIT_AUSM: F3D_SPL(F3D_SPLIT_AUSM); break;
+    default: F3D_SPL(F3D_SPLIT_SW); break;
+  }
+#undef F3D_SPL
+  return check_launch(names[kp.split & 3]);
 }
 
 template <typename T>
 int launch_advect(const KP& kp, int d, T* F, const T* phi, const T* S, const T* Fp, const T* Fm, const T* Q,
-                  const T* Ax, const T* Ay, const T* Az, hipStream_t s) {
-  TimedScope ts_("advect", s);
+                  const T* Ax, const T* Ay, const T* Az, hipStream_t s, int NV) {
+  TimedScope ts_(kp.character ? "advect" : "advect_component", s);
   Range3 r = face_range(kp, d);
-#define F3D_ADV(D)                                                                                         \
-  do {                                                                                                     \
-    if (S)                                                                                                 \
-      hipLaunchKernelGGL((k_advect<T, D, false>), range_grid(r), kBlock, 0, s, kp, r, F, phi, S, Fp, Fm, Q, \
-                         Ax, Ay, Az);                                                                      \
-    else                                                                                                   \
-      hipLaunchKernelGGL((k_advect<T, D, true>), range_grid(r), kBlock, 0, s, kp, r, F, phi, S, Fp, Fm, Q,  \
-                         Ax, Ay, Az);                                                                      \
+#define F3D_ADV(D, C)                                                                                         \
+  do {                                                                                                        \
+    if (S)                                                                                                    \
+      hipLaunchKernelGGL((k_advect<T, D, false, C>), range_grid(r), kBlock, 0, s, kp, r, F, phi, S, Fp, Fm, Q, \
+                         Ax, Ay, Az, NV);                                                                     \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_advect<T, D, true, C>), range_grid(r), kBlock, 0, s, kp, r, F, phi, S, Fp, Fm, Q,  \
+                         Ax, Ay, Az, NV);                                                                     \
   } while (0)
-  if (d == 0) F3D_ADV(0);
-  else if (d == 1) F3D_ADV(1);
-  else F3D_ADV(2);
+  if (kp.character) {
+    if (d == 0) F3D_ADV(0, true);
+    else if (d == 1) F3D_ADV(1, true);
+    else F3D_ADV(2, true);
+  } else {
+    if (d == 0) F3D_ADV(0, false);
+    else if (d == 1) F3D_ADV(1, false);
+    else F3D_ADV(2, false);
+  }
 #undef F3D_ADV
   return check_launch("advect");
 }
@@ -430,7 +458,7 @@ int launch_fill(const KP& kp, T* Q, double* U, const T* inflow, hipStream_t s) {
   template int launch_flux_split<T>(const KP&, const T*, T*, T*, const T*, const T*, const T*, const T*,        \
                                     hipStream_t);                                                               \
   template int launch_advect<T>(const KP&, int, T*, const T*, const T*, const T*, const T*, const T*, const T*, \
-                                const T*, const T*, hipStream_t);                                               \
+                                const T*, const T*, hipStream_t, int);                                          \
   template int launch_viscous<T>(const KP&, T*, T*, T*, const T*, const MetricPtrs<T>&, const double*,          \
                                  hipStream_t);                                                                  \
   template int launch_div<T>(const KP&, double*, const T*, const T*, const T*, const T*, const T*, const T*,    \

@@ -354,6 +354,11 @@ int check_branch(const f3d_params* p, bool fvm_entry, const double* U, const T* 
     set_error("%s: params select the finite-volume branch (fvm = %d); use the _fvm entry", who, p->fvm);
     return F3D_E_ARG;
   }
+  if (p->les != F3D_LES_NONE || p->lts) {
+    set_error("%s: LES (les = %d) and local time stepping (lts = %d) are not available in this entry", who,
+              p->les, p->lts);
+    return F3D_E_ARG;
+  }
   if (fvm_entry) {
     if (!U || !geo || !geo[0] || !geo[1] || !geo[2]) {
       set_error("%s: the finite-volume branch needs U and the face geometry arrays geo[3]", who);

@@ -72,6 +72,17 @@ typedef struct ihipStream_t* f3d_stream_t; /* == hipStream_t */
  * split fluxes (schemes.jl, split.jl).  F3D_FVM_*: reconstruction of the conserved state U to both
  * sides of each face, HLLC flux times the face area from the node coordinates (FVM.jl), with the
  * same sensor thresholds (the non-UP7 branches use WENO epsilons 1e-10 / 1e-8, FVM.jl:91). */
+/* Flux-vector splitting of the finite-difference branch (split.jl; solver.jl:134-144). */
+#define F3D_SPLIT_SW 0    /* Steger-Warming, split.jl:2-68 */
+#define F3D_SPLIT_LF 1    /* local Lax-Friedrichs, split.jl:70-108 */
+#define F3D_SPLIT_VL 2    /* van Leer, split.jl:110-183 */
+#define F3D_SPLIT_AUSM 3  /* AUSM, split.jl:185-255 */
+
+/* Subgrid model of the viscous flux (viscous.jl:66-106). */
+#define F3D_LES_NONE 0
+#define F3D_LES_SMAG 1
+#define F3D_LES_WALE 2
+
 #define F3D_FDM 0
 #define F3D_FVM_EIGEN 1   /* characteristic (Roe-frame) reconstruction, FVM.jl:6-670 */
 #define F3D_FVM_CONSER 2  /* component-wise reconstruction of U, FVM.jl:672-1222 */
@@ -89,6 +100,15 @@ typedef struct f3d_params {
   double weno7_eps, weno5_eps;                  /* schemes.jl:712-713 */
   double q_inf[6];   /* Dirichlet state (rho,u,v,w,p,T); reference values boundary.jl:14-19 */
   double T_wall;     /* F3D_BC_WALL_ISOTHERMAL wall temperature */
+  /* flag-selected alternatives of run.jl (f3d_params_init sets the reference's values) */
+  int32_t split;     /* F3D_SPLIT_* (run.jl:69 splitMethod) */
+  int32_t character; /* 1: characteristic reconstruction advect_*c, 0: component-wise advect_* (run.jl:68) */
+  int32_t les;       /* F3D_LES_NONE / F3D_LES_SMAG / F3D_LES_WALE (run.jl:4-5) */
+  int32_t lts;       /* local time stepping (run.jl:21) */
+  int32_t filtering, filtering_nonlinear, filtering_interval; /* run.jl:50-52 */
+  int32_t pad_;
+  double lts_cfl;                    /* run.jl:22 */
+  double filtering_rth, filtering_s0; /* run.jl:53-54 */
 } f3d_params;
 
 int f3d_version(void);
@@ -123,6 +143,41 @@ int f3d_flux_split_sw_f64(const f3d_params* p, const double* Q, double* Fp, doub
 int f3d_flux_split_sw_f32b(const f3d_params* p, const float* Q, float* Fp, float* Fm,
                            const float* S, const float* Ax, const float* Ay, const float* Az,
                            f3d_stream_t s);
+/* fluxSplit_LF / fluxSplit_VL / fluxSplit_AUSM(Q, Fp, Fm, S, Ax, Ay, Az)  split.jl:70 / 110 / 185:
+ * local Lax-Friedrichs, van Leer and AUSM splittings, every cell, from rho, u, v, w, p. */
+int f3d_flux_split_lf_f64(const f3d_params* p, const double* Q, double* Fp, double* Fm,
+                          const double* S, const double* Ax, const double* Ay, const double* Az,
+                          f3d_stream_t s);
+int f3d_flux_split_lf_f32b(const f3d_params* p, const float* Q, float* Fp, float* Fm,
+                           const float* S, const float* Ax, const float* Ay, const float* Az,
+                           f3d_stream_t s);
+int f3d_flux_split_vl_f64(const f3d_params* p, const double* Q, double* Fp, double* Fm,
+                          const double* S, const double* Ax, const double* Ay, const double* Az,
+                          f3d_stream_t s);
+int f3d_flux_split_vl_f32b(const f3d_params* p, const float* Q, float* Fp, float* Fm,
+                           const float* S, const float* Ax, const float* Ay, const float* Az,
+                           f3d_stream_t s);
+int f3d_flux_split_ausm_f64(const f3d_params* p, const double* Q, double* Fp, double* Fm,
+                            const double* S, const double* Ax, const double* Ay, const double* Az,
+                            f3d_stream_t s);
+int f3d_flux_split_ausm_f32b(const f3d_params* p, const float* Q, float* Fp, float* Fm,
+                             const float* S, const float* Ax, const float* Ay, const float* Az,
+                             f3d_stream_t s);
+
+/* advect_x / advect_y / advect_z(F, phi, S, Fp, Fm, NV)  schemes.jl:86, 291, 496: the component-wise
+ * reconstruction (no characteristic projection) of the first NV (1..5) split-flux components. */
+int f3d_advect_x_f64(const f3d_params* p, double* F, const double* phi, const double* S,
+                     const double* Fp, const double* Fm, int32_t NV, f3d_stream_t s);
+int f3d_advect_y_f64(const f3d_params* p, double* F, const double* phi, const double* S,
+                     const double* Fp, const double* Fm, int32_t NV, f3d_stream_t s);
+int f3d_advect_z_f64(const f3d_params* p, double* F, const double* phi, const double* S,
+                     const double* Fp, const double* Fm, int32_t NV, f3d_stream_t s);
+int f3d_advect_x_f32b(const f3d_params* p, float* F, const float* phi, const float* S,
+                      const float* Fp, const float* Fm, int32_t NV, f3d_stream_t s);
+int f3d_advect_y_f32b(const f3d_params* p, float* F, const float* phi, const float* S,
+                      const float* Fp, const float* Fm, int32_t NV, f3d_stream_t s);
+int f3d_advect_z_f32b(const f3d_params* p, float* F, const float* phi, const float* S,
+                      const float* Fp, const float* Fm, int32_t NV, f3d_stream_t s);
 
 /* advect_xc / advect_yc / advect_zc(F, phi, S, Fp, Fm, Q, Ax, Ay, Az)  schemes.jl:700, 1014, 1330 */
 int f3d_advect_xc_f64(const f3d_params* p, double* F, const double* phi, const double* S,
@@ -237,8 +292,8 @@ int f3d_conser_reconstruct_k_f32b(const f3d_params* p, const float* Q, const dou
                                   const double* y, const double* z, f3d_stream_t s);
 
 /* flowAdvance(U, Q, Fp, Fm, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, s1, s2, s3, 9 metrics, J, x, y, z, phi)
- * solver.jl:19.  p->fvm == F3D_FDM: the finite-difference branch solver.jl:133-188 (split/advect
- * x, y, z; x, y, z unused); otherwise the finite-volume branch solver.jl:21-31 (eigen or conser
+ * solver.jl:19.  p->fvm == F3D_FDM: the finite-difference branch solver.jl:133-188 (per direction the
+ * splitting p->split, then advect_*c when p->character, else advect_*; x, y, z unused); otherwise the finite-volume branch solver.jl:21-31 (eigen or conser
  * reconstruct i, j, k; Fp, Fm unused).  Then viscousFlux when p->viscous.  x, y, z are double in
  * both forms (see the finite-volume entries). */
 int f3d_flow_advance_f64(const f3d_params* p, double* U, double* Q, double* Fp, double* Fm,

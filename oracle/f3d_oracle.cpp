@@ -83,6 +83,28 @@ OR_API void oracle_advect_c_f32b(const f3d_params* p, int d, float* F, const flo
                                  const float* Ay, const float* Az) {
   advect_c<float>(p, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az);
 }
+OR_API void oracle_flux_split_f64(const f3d_params* p, int method, const double* Q, double* Fp, double* Fm,
+                                  const double* S, const double* Ax, const double* Ay, const double* Az) {
+  if (method == F3D_SPLIT_SW)
+    flux_split_sw<double>(p, Q, Fp, Fm, S, Ax, Ay, Az);
+  else
+    flux_split_other<double>(p, method, Q, Fp, Fm, S, Ax, Ay, Az);
+}
+OR_API void oracle_flux_split_f32b(const f3d_params* p, int method, const float* Q, float* Fp, float* Fm,
+                                   const float* S, const float* Ax, const float* Ay, const float* Az) {
+  if (method == F3D_SPLIT_SW)
+    flux_split_sw<float>(p, Q, Fp, Fm, S, Ax, Ay, Az);
+  else
+    flux_split_other<float>(p, method, Q, Fp, Fm, S, Ax, Ay, Az);
+}
+OR_API void oracle_advect_f64(const f3d_params* p, int d, double* F, const double* phi, const double* S,
+                              const double* Fp, const double* Fm, int NV) {
+  advect_comp<double>(p, d, F, phi, S, Fp, Fm, NV);
+}
+OR_API void oracle_advect_f32b(const f3d_params* p, int d, float* F, const float* phi, const float* S,
+                               const float* Fp, const float* Fm, int NV) {
+  advect_comp<float>(p, d, F, phi, S, Fp, Fm, NV);
+}
 OR_API void oracle_viscous_flux_f64(const f3d_params* p, double* Fvx, double* Fvy, double* Fvz, const double* Q,
                                     const void* const* m, const double* J) {
   viscous_flux<double>(p, Fvx, Fvy, Fvz, Q, M9<double>(m).m, J);
@@ -225,7 +247,11 @@ void rhs_absterms(const f3d_params* p, double* A, const T* Q, const void* const*
   T cl[7];
   linear_coeffs<T>(p, cl);
   for (int d = 0; d < 3; ++d) {
-    flux_split_sw<T>(p, Q, sc.Fp.data(), sc.Fm.data(), S[d], m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2]);
+    if (p->split == F3D_SPLIT_SW)
+      flux_split_sw<T>(p, Q, sc.Fp.data(), sc.Fm.data(), S[d], m.m[3 * d], m.m[3 * d + 1], m.m[3 * d + 2]);
+    else
+      flux_split_other<T>(p, p->split, Q, sc.Fp.data(), sc.Fm.data(), S[d], m.m[3 * d], m.m[3 * d + 1],
+                          m.m[3 * d + 2]);
     Dir D = make_dir(d);
     for (long k = NG + 1; k <= g.Nzp + NG; ++k)
       for (long j = NG + 1; j <= g.Nyp + NG; ++j)

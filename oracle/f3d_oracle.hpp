@@ -560,6 +560,152 @@ void flux_split_sw(const f3d_params* p, const T* Q, T* Fp, T* Fm, const T* S, co
       }
 }
 
+// The three other splittings of split.jl, every cell [1, N_tot]^3.  Each reads rho, u, v, w and p (not T)
+// and the metric row A with S = |A|; sound speed c = sqrt(gamma p / rho).
+// fluxSplit_LF  split.jl:70-108: F± = (F(A) ± lambda0 U)/2, lambda0 = |A.u| + c S.
+// fluxSplit_VL  split.jl:110-183: van Leer mass-flux splitting in M = A.u/(c S).
+// fluxSplit_AUSM  split.jl:185-255: AUSM mass-flux and pressure splitting in M.
+template <typename T>
+void flux_split_other(const f3d_params* p, int method, const T* Q, T* Fp, T* Fm, const T* S, const T* Ax,
+                      const T* Ay, const T* Az) {
+  Grid g(p);
+  const T gam = (T)p->gamma;
+#pragma omp parallel for collapse(2)
+  for (long k = 1; k <= g.Nzt; ++k)
+    for (long j = 1; j <= g.Nyt; ++j)
+      for (long i = 1; i <= g.Nxt; ++i) {
+        const T rho = Q[g.cell.at(i, j, k, 1)], u = Q[g.cell.at(i, j, k, 2)];
+        const T v = Q[g.cell.at(i, j, k, 3)], w = Q[g.cell.at(i, j, k, 4)];
+        const T pr = Q[g.cell.at(i, j, k, 5)];
+        const T A1 = Ax[g.cell.at(i, j, k)], A2 = Ay[g.cell.at(i, j, k)], A3 = Az[g.cell.at(i, j, k)];
+        const T ss = S[g.cell.at(i, j, k)];
+        const T c = std::sqrt(gam * pr / rho);
+        T un = A1 * u + A2 * v + A3 * w;
+        T fp[5], fm[5];
+        if (method == F3D_SPLIT_LF) {  // split.jl:92-107
+          const T l0 = std::fabs(un) + c * ss;
+          const T E0 = pr / (gam - 1) + T(0.5) * rho * (u * u + v * v + w * w);
+          fp[0] = T(0.5) * (rho * un + l0 * rho);
+          fp[1] = T(0.5) * (rho * u * un + A1 * pr + l0 * rho * u);
+          fp[2] = T(0.5) * (rho * v * un + A2 * pr + l0 * rho * v);
+          fp[3] = T(0.5) * (rho * w * un + A3 * pr + l0 * rho * w);
+          fp[4] = T(0.5) * ((E0 + pr) * un + l0 * E0);
+          fm[0] = T(0.5) * (rho * un - l0 * rho);
+          fm[1] = T(0.5) * (rho * u * un + A1 * pr - l0 * rho * u);
+          fm[2] = T(0.5) * (rho * v * un + A2 * pr - l0 * rho * v);
+          fm[3] = T(0.5) * (rho * w * un + A3 * pr - l0 * rho * w);
+          fm[4] = T(0.5) * ((E0 + pr) * un - l0 * E0);
+        } else {
+          const T M = un / (c * ss);
+          // full flux on the upwind side when |M| >= 1 (split.jl:136-159, 211-234)
+          const T E0 = pr / (gam - 1) + T(0.5) * rho * (u * u + v * v + w * w);
+          const T h = gam * pr / (rho * (gam - 1)) + T(0.5) * (u * u + v * v + w * w);
+          const T full[5] = {rho * un, rho * u * un + A1 * pr, rho * v * un + A2 * pr, rho * w * un + A3 * pr,
+                             method == F3D_SPLIT_VL ? (E0 + pr) * un : rho * h * un};
+          if (M <= -1) {
+            for (int q = 0; q < 5; ++q) {
+              fp[q] = 0;
+              fm[q] = full[q];
+            }
+          } else if (M >= 1) {
+            for (int q = 0; q < 5; ++q) {
+              fm[q] = 0;
+              fp[q] = full[q];
+            }
+          } else if (method == F3D_SPLIT_VL) {  // split.jl:161-181
+            un = un / ss;
+            const T f1p = rho * c * T(0.25) * ((M + 1) * (M + 1)) * ss;
+            const T f1m = -rho * c * T(0.25) * ((M - 1) * (M - 1)) * ss;
+            const T t1 = (-un + 2 * c) / (gam * ss);
+            const T t2 = (-un - 2 * c) / (gam * ss);
+            const T t3 = 1 / (2 * (gam * gam - 1));
+            const T t4 = T(0.5) * (u * u + v * v + w * w - un * un);
+            fp[0] = f1p;
+            fp[1] = f1p * (A1 * t1 + u);
+            fp[2] = f1p * (A2 * t1 + v);
+            fp[3] = f1p * (A3 * t1 + w);
+            fp[4] = f1p * (((gam - 1) * un + 2 * c) * ((gam - 1) * un + 2 * c) * t3 + t4);
+            fm[0] = f1m;
+            fm[1] = f1m * (A1 * t2 + u);
+            fm[2] = f1m * (A2 * t2 + v);
+            fm[3] = f1m * (A3 * t2 + w);
+            fm[4] = f1m * (((gam - 1) * un - 2 * c) * ((gam - 1) * un - 2 * c) * t3 + t4);
+          } else {  // AUSM, split.jl:236-253
+            const T f1p = T(0.25) * ((M + 1) * (M + 1)) * rho * c * ss;
+            const T f1m = -T(0.25) * ((M - 1) * (M - 1)) * rho * c * ss;
+            const T Pp = pr * (T(0.25) * ((M + 1) * (M + 1)) * (2 - M));
+            const T Pm = pr * (T(0.25) * ((M - 1) * (M - 1)) * (2 + M));
+            fp[0] = f1p;
+            fp[1] = f1p * u + A1 * Pp;
+            fp[2] = f1p * v + A2 * Pp;
+            fp[3] = f1p * w + A3 * Pp;
+            fp[4] = f1p * h;
+            fm[0] = f1m;
+            fm[1] = f1m * u + A1 * Pm;
+            fm[2] = f1m * v + A2 * Pm;
+            fm[3] = f1m * w + A3 * Pm;
+            fm[4] = f1m * h;
+          }
+        }
+        for (int q = 0; q < 5; ++q) {
+          Fp[g.cell.at(i, j, k, q + 1)] = fp[q];
+          Fm[g.cell.at(i, j, k, q + 1)] = fm[q];
+        }
+      }
+}
+
+// advect_x / advect_y / advect_z  schemes.jl:86-290 / 291-495 / 496-698: the component-wise form of
+// advect_*c (the same sensor, UP7, WENO7 with the Float64 epsilon, WENO5 and NND2 stencils on each of
+// the first NV split-flux components, no characteristic projection).
+template <typename T>
+void advect_comp(const f3d_params* p, int d, T* F, const T* phi, const T* S, const T* Fp, const T* Fm, int NV) {
+  Grid g(p);
+  Dir D = make_dir(d);
+  const Ext& fe = d == 0 ? g.fx : (d == 1 ? g.fy : g.fz);
+  T c[7];
+  linear_coeffs<T>(p, c);
+  const T tmp1 = T(1) / T(12), tmp2 = T(1) / T(6);
+  const T eps7 = (T)p->weno7_eps, eps5 = (T)p->weno5_eps;
+  const size_t ncell = (size_t)g.Nxt * g.Nyt * g.Nzt;
+#pragma omp parallel for collapse(2)
+  for (long k = g.NG + 1; k <= g.Nzp + g.NG + D.dk; ++k)
+    for (long j = g.NG + 1; j <= g.Nyp + g.NG + D.dj; ++j)
+      for (long i = g.NG + 1; i <= g.Nxp + g.NG + D.di; ++i) {
+        auto C = [&](long o) { return g.cell.at(i + o * D.di, j + o * D.dj, k + o * D.dk); };
+        const int sch = face_scheme(p, g, phi, i, j, k, D, (T*)nullptr);
+        const T ss = 2 / (S[C(-1)] + S[C(0)]);  // schemes.jl:109
+        for (long n = 1; n <= NV; ++n) {
+          const size_t nn = ncell * (n - 1);
+          T f;
+          if (sch == F3D_SCHEME_UP7) {  // schemes.jl:119-143
+            T fp = 0, fm = 0;
+            for (int q = 0; q < 7; ++q) fp += c[q] * Fp[C(q - 4) + nn];
+            for (int q = 0; q < 7; ++q) fm += c[q] * Fm[C(3 - q) + nn];
+            f = fp + fm;
+          } else if (sch == F3D_SCHEME_WENO7) {  // schemes.jl:144-224
+            T Vp[7], Vm[7];
+            for (int q = 0; q < 7; ++q) {
+              Vp[q] = Fp[C(q - 4) + nn];
+              Vm[q] = Fm[C(3 - q) + nn];
+            }
+            f = (weno7(Vp, ss, eps7) + weno7(Vm, ss, eps7)) * tmp1;
+          } else if (sch == F3D_SCHEME_WENO5) {  // schemes.jl:225-276
+            T Vp[5], Vm[5];
+            for (int q = 0; q < 5; ++q) {
+              Vp[q] = Fp[C(q - 3) + nn];
+              Vm[q] = Fm[C(2 - q) + nn];
+            }
+            f = (weno5(Vp, ss, eps5) + weno5(Vm, ss, eps5)) * tmp2;
+          } else {  // schemes.jl:277-286
+            const T fp = Fp[C(-1) + nn] + T(0.5) * minmod(Fp[C(0) + nn] - Fp[C(-1) + nn], Fp[C(-1) + nn] - Fp[C(-2) + nn]);
+            const T fm = Fm[C(0) + nn] - T(0.5) * minmod(Fm[C(1) + nn] - Fm[C(0) + nn], Fm[C(0) + nn] - Fm[C(-1) + nn]);
+            f = fp + fm;
+          }
+          F[fe.at(i - g.NG, j - g.NG, k - g.NG, n)] = f;
+        }
+      }
+}
+
 // ---------------------------------------------------------------- viscous.jl
 // viscousFlux  viscous.jl:1-65, 108-133 (laminar; the LES branches 66-106 are off), domain
 // [3, N_tot-2]^3, stored at (i-2, j-2, k-2).  J is double in both forms (cast to T here).
@@ -763,16 +909,24 @@ void fill_ghost(const f3d_params* p, T* Q, double* U, const T* inflow, const P* 
 
 // ---------------------------------------------------------------- solver.jl
 // flowAdvance, finite-difference branch  solver.jl:133-188 (SW splitting, characteristic advect)
+// (solver.jl:134-184: per direction the split selected by splitMethod, then advect_*c when
+// `character`, else the component-wise advect_*)
 template <typename T>
 void flow_advance(const f3d_params* p, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, T* Fvx, T* Fvy,
                   T* Fvz, const T* s1, const T* s2, const T* s3, const T* const m[9],
                   const double* J, T* phi) {
-  flux_split_sw<T>(p, Q, Fp, Fm, s1, m[0], m[1], m[2]);
-  advect_c<T>(p, 0, Fx, phi, s1, Fp, Fm, Q, m[0], m[1], m[2]);
-  flux_split_sw<T>(p, Q, Fp, Fm, s2, m[3], m[4], m[5]);
-  advect_c<T>(p, 1, Fy, phi, s2, Fp, Fm, Q, m[3], m[4], m[5]);
-  flux_split_sw<T>(p, Q, Fp, Fm, s3, m[6], m[7], m[8]);
-  advect_c<T>(p, 2, Fz, phi, s3, Fp, Fm, Q, m[6], m[7], m[8]);
+  const T* S[3] = {s1, s2, s3};
+  T* F[3] = {Fx, Fy, Fz};
+  for (int d = 0; d < 3; ++d) {
+    if (p->split == F3D_SPLIT_SW)
+      flux_split_sw<T>(p, Q, Fp, Fm, S[d], m[3 * d], m[3 * d + 1], m[3 * d + 2]);
+    else
+      flux_split_other<T>(p, p->split, Q, Fp, Fm, S[d], m[3 * d], m[3 * d + 1], m[3 * d + 2]);
+    if (p->character)
+      advect_c<T>(p, d, F[d], phi, S[d], Fp, Fm, Q, m[3 * d], m[3 * d + 1], m[3 * d + 2]);
+    else
+      advect_comp<T>(p, d, F[d], phi, S[d], Fp, Fm, 5);
+  }
   if (p->viscous) viscous_flux<T>(p, Fvx, Fvy, Fvz, Q, m, J);
 }
 

@@ -71,6 +71,14 @@ class Oracle:
     def flux_split(self, Q, Fp, Fm, S, Ax, Ay, Az):
         self._c(f"oracle_flux_split_sw_{suf(Q)}", P(Q), P(Fp), P(Fm), P(S), P(Ax), P(Ay), P(Az))
 
+    def flux_split_method(self, method, Q, Fp, Fm, S, Ax, Ay, Az):
+        """fluxSplit_SW / _LF / _VL / _AUSM by method index (F3D_SPLIT_*)."""
+        self._c(f"oracle_flux_split_{suf(Q)}", c_int(method), P(Q), P(Fp), P(Fm), P(S), P(Ax), P(Ay), P(Az))
+
+    def advect_comp(self, d, F, phi, S, Fp, Fm, NV=5):
+        """Component-wise advect_x / _y / _z (schemes.jl:86-698)."""
+        self._c(f"oracle_advect_{suf(Fp)}", c_int(d), P(F), P(phi), P(S), P(Fp), P(Fm), c_int(NV))
+
     def advect(self, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
         self._c(f"oracle_advect_c_{suf(Q)}", c_int(d), P(F), P(phi), P(S), P(Fp), P(Fm), P(Q), P(Ax), P(Ay), P(Az))
 

@@ -35,9 +35,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layout_matches_header():
-    # 12 int32 then 19 doubles: the ctypes mirror must have the C size
-    assert ctypes.sizeof(f3d.F3DParams) == 12 * 4 + 19 * 8
+    # 12 int32, 19 doubles, 8 int32, 3 doubles: the ctypes mirror must have the C size
+    assert ctypes.sizeof(f3d.F3DParams) == 12 * 4 + 19 * 8 + 8 * 4 + 3 * 8
     assert f3d.F3DParams.gamma.offset == 48
+    assert f3d.F3DParams.split.offset == 200 and f3d.F3DParams.lts_cfl.offset == 232
 
 
 def test_params_defaults_are_the_reference_constants():
