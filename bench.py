@@ -225,6 +225,43 @@ def fvm_bench(f3d, torch, lib, st, U0, Q0, steps, warmup, f32b=True):
     return out
 
 
+ALTERNATIVES = [
+    ("les_smagorinsky", dict(les=1)), ("les_wale", dict(les=2)),
+    ("split_lax_friedrichs", dict(split=1)), ("split_van_leer", dict(split=2)), ("split_ausm", dict(split=3)),
+    ("component_wise", dict(character=0)),
+]
+
+
+def alternatives_bench(f3d, torch, lib, st, U0, Q0, steps, warmup):
+    """The flag-selected alternatives of the finite-difference step (LES models, other splittings,
+    component-wise reconstruction) on the same grid, initial state and dt, FP64: ms/step, the
+    Mcell-updates/s rate and the cost relative to the default step measured in the same way."""
+    base = st["p"]
+    out = {}
+    for name, kw in [("default", {})] + ALTERNATIVES:
+        p = type(base).from_buffer_copy(base)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        ctx = f3d.Flame3D(p)
+        U, Un, Q = U0.clone(), torch.zeros_like(U0), Q0.clone()
+        ms, _, _, _ = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], st["dt"]), steps, warmup,
+                                lib=None)
+        try:
+            ctx.nan_check(U)
+            ok = True
+        except f3d.F3DNaNError:
+            ok = False
+        ncells = p.Nxp * p.Nyp * p.Nzp
+        out[name] = dict(ms_per_step=round(ms, 4), value=round(ncells / (ms * 1e-3) / 1e6, 3), finite=ok)
+        del ctx
+        torch.cuda.empty_cache()
+    d = out["default"]["ms_per_step"]
+    for v in out.values():
+        v["cost_vs_default"] = round(v["ms_per_step"] / d, 3)
+    progress("alternatives: " + ", ".join(f"{k} {v['ms_per_step']:.3f}" for k, v in out.items()))
+    return out
+
+
 def cpu_baseline(f3d, cases, nx, ny, nz, steps=3):
     """CPU oracle (C++ FP64 restatement, OpenMP) on the same grid for `steps` RK3 steps."""
     import numpy as np
@@ -387,6 +424,7 @@ def main():
     ap.add_argument("--adjoint-sweep", type=float, default=0.0)
     ap.add_argument("--no-fvm", action="store_true", help="skip the finite-volume leg")
     ap.add_argument("--fvm-only", action="store_true", help="only the finite-volume leg (profiling)")
+    ap.add_argument("--no-alternatives", action="store_true", help="skip the flag-selected alternatives leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -479,6 +517,9 @@ def main():
     fvm = None
     if not args.no_fvm:
         fvm = fvm_bench(f3d, torch, lib, st, U0, Q0, args.steps, args.warmup, f32b=not args.no_f32b)
+    alt = None
+    if not args.no_alternatives:
+        alt = alternatives_bench(f3d, torch, lib, st, U0, Q0, max(3, args.steps // 4), 2)
     adj = None
     if not args.no_adjoint:
         ax, ay, az = map(int, args.adjoint_grid.split("x"))
@@ -510,6 +551,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32_bulk": out32 or None,
         "finite_volume": fvm,
+        "alternatives": alt,
         "adjoint": adj,
         "kernels": kshare,
         "host_wall_ms_per_step": round(wall64, 4),

@@ -320,7 +320,7 @@ class Flame3D:
 
     def viscousFlux(self, Fv_x, Fv_y, Fv_z, Q, dxidx, dxidy, dxidz, detadx, detady, detadz,
                     dzetadx, dzetady, dzetadz, J):
-        """viscous.jl:1-134 (laminar)"""
+        """viscous.jl:1-134 (laminar, plus the eddy viscosity of params.les: viscous.jl:66-106)"""
         self._call(f"f3d_viscous_flux_{self._suffix(Q)}",
                    *map(_ptr, (Fv_x, Fv_y, Fv_z, Q, dxidx, dxidy, dxidz, detadx, detady, detadz,
                                dzetadx, dzetady, dzetadz, J)))

@@ -73,6 +73,12 @@ F3D_HD T jmax(T a, T b) {
   return (a != a) ? a : ((b != b) ? b : ((b > a) ? b : a));
 }
 
+// NaN-propagating min (Julia Base.min on floats).
+template <typename T>
+F3D_HD T jmin(T a, T b) {
+  return (a != a) ? a : ((b != b) ? b : ((b < a) ? b : a));
+}
+
 // Scheme index from the 6-cell sensor maximum (schemes.jl:731, 813, 931, else NND2 at 1000).
 template <typename T>
 F3D_HD int scheme_of(const KP& kp, T phix) {
@@ -437,14 +443,68 @@ F3D_HD void face_flux(const KP& kp, int sch, const FPL& FP, const FML& FM,
 }
 
 // ------------------------------------------------------------------ viscous pieces
-// Laminar viscous fluxes of one cell in the three directions (viscous.jl:20-65, 108-133).
+// Eddy viscosity of the LES models (viscous.jl:66-106) from the Cartesian velocity gradient
+// (dx[q], dy[q], dz[q]: d/dx, d/dy, d/dz of u, v, w for q = 0, 1, 2), added to mu and lam
+// (lam_t = Cp mu_t / 0.9), mu_t capped at twice the laminar mu, filter width J^(-1/3).
+//   Smagorinsky: mu_t = rho (0.1 Delta)^2 sqrt(2 S_ij S_ij).
+//   WALE: mu_t = rho (0.325 Delta)^2 Sd^3 / (S^5 + Sd^(5/2)) with Sd from the traceless symmetric part
+//   of the squared gradient; 0 where S = Sd = 0 (the reference's 0/0).  The float form scales the
+//   gradient by its largest component first (S^5 leaves float's range beyond ~5e7 1/s).
+template <typename T, int LES>
+F3D_HD void les_eddy(const KP& kp, T rho, T Jac, const T* dx, const T* dy, const T* dz, T& mu, T& lam) {
+  if constexpr (LES != F3D_LES_NONE) {
+    const T half = T(0.5);
+    T mut;
+    if constexpr (LES == F3D_LES_SMAG) {
+      const T s12 = half * (dy[0] + dx[1]), s13 = half * (dz[0] + dx[2]), s23 = half * (dz[1] + dy[2]);
+      const T SS = dx[0] * dx[0] + dy[1] * dy[1] + dz[2] * dz[2] + 2 * (s12 * s12 + s13 * s13 + s23 * s23);
+      const T dl = T(0.1) / cbrt(Jac);
+      mut = jmin<T>(rho * (dl * dl) * sqrt(2 * SS), 2 * mu);
+    } else {
+      T sc = T(1);
+      if (sizeof(T) == 4) {
+        T gm = T(0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) gm = jmax<T>(gm, jmax<T>(fabs(dx[q]), jmax<T>(fabs(dy[q]), fabs(dz[q]))));
+        sc = gm > T(0) ? gm : T(1);
+      }
+      // sc = 1 in double: the divisions below are exact there
+      const T ux = dx[0] / sc, uy = dy[0] / sc, uz = dz[0] / sc;
+      const T vx = dx[1] / sc, vy = dy[1] / sc, vz = dz[1] / sc;
+      const T wx = dx[2] / sc, wy = dy[2] / sc, wz = dz[2] / sc;
+      const T t12 = half * (uy + vx), t13 = half * (uz + wx), t23 = half * (vz + wy);
+      const T S = sqrt(ux * ux + vy * vy + wz * wz + 2 * (t12 * t12 + t13 * t13 + t23 * t23));
+      T Sd11 = ux * ux + uy * vx + uz * wx;
+      T Sd22 = vx * uy + vy * vy + vz * wy;
+      T Sd33 = wx * uz + wy * vz + wz * wz;
+      const T tr = (Sd11 + Sd22 + Sd33) / T(3);
+      Sd11 -= tr;
+      Sd22 -= tr;
+      Sd33 -= tr;
+      const T Sd12 = half * (ux * vx + vx * vy + wx * vz + uy * ux + vy * uy + wy * uz);
+      const T Sd13 = half * (ux * wx + vx * wy + wx * wz + uz * ux + vz * uy + wz * uz);
+      const T Sd23 = half * (uy * wx + vy * wy + wy * wz + uz * vx + vz * vy + wz * vz);
+      const T Sd =
+          sqrt(Sd11 * Sd11 + Sd22 * Sd22 + Sd33 * Sd33 + 2 * (Sd12 * Sd12 + Sd13 * Sd13 + Sd23 * Sd23));
+      const T den = S * S * S * S * S + Sd * Sd * sqrt(Sd);
+      const T D = den > T(0) ? sc * (Sd * Sd * Sd / den) : T(0);
+      const T dl = T(0.325) / cbrt(Jac);
+      mut = jmin<T>(rho * (dl * dl) * D, 2 * mu);
+    }
+    mu += mut;
+    lam += (T)kp.Cp * mut / T(0.9);
+  }
+}
+
+// Viscous fluxes of one cell in the three directions (viscous.jl:20-65, 108-133), with the eddy
+// viscosity of LES (F3D_LES_*; rho is read only then).
 //   g[v][d]: 4th-order central derivative of (u,v,w,T)[v] along xi/eta/zeta (not yet divided).
 //   m[9]: metric row-major (xi_x, xi_y, xi_z, eta_x, ...); out[d][4].
-template <typename T>
+template <typename T, int LES = F3D_LES_NONE>
 F3D_HD void viscous_cell(const KP& kp, T u, T v, T w, T Tt, const T g[4][3],
-                                             const T m[9], T Jac, T out[3][4]) {
-  const T mu = (T)kp.Cs * Tt * sqrt(Tt) / (Tt + (T)kp.Ts);
-  const T lam = (T)kp.Cp * mu / (T)kp.Pr;
+                                             const T m[9], T Jac, T out[3][4], T rho = T(0)) {
+  T mu = (T)kp.Cs * Tt * sqrt(Tt) / (Tt + (T)kp.Ts);
+  T lam = (T)kp.Cp * mu / (T)kp.Pr;
   T dx[4], dy[4], dz[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -452,6 +512,7 @@ F3D_HD void viscous_cell(const KP& kp, T u, T v, T w, T Tt, const T g[4][3],
     dy[q] = (g[q][0] * m[1] + g[q][1] * m[4] + g[q][2] * m[7]) * Jac;
     dz[q] = (g[q][0] * m[2] + g[q][1] * m[5] + g[q][2] * m[8]) * Jac;
   }
+  les_eddy<T, LES>(kp, rho, Jac, dx, dy, dz, mu, lam);
   const T c23 = T(2) / T(3);
   const T dv = dx[0] + dy[1] + dz[2];
   const T t11 = mu * (2 * dx[0] - c23 * dv);

@@ -296,8 +296,8 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a) {
 // viscousFlux (viscous.jl:1-134) on interior +-2 columns, one lane per (i, j) column marching k:
 // u, v, w, T of the column are kept in a 5-plane register window for the zeta derivatives, so each
 // plane is read from HBM once; the xi/eta neighbours come from L1/L2.  Output in the reference Fv
-// layout (viscous.jl:119-132).
-template <typename T>
+// layout (viscous.jl:119-132).  LES: the eddy viscosity model (F3D_LES_*).
+template <typename T, int LES>
 __global__ __launch_bounds__(256) void k_viscous_march(KP kp, FArgs<T> a, T* __restrict__ Fvx, T* __restrict__ Fvy,
                                                        T* __restrict__ Fvz, int KC, int remap) {
   const Blk B = xcd_block(remap);
@@ -350,7 +350,8 @@ __global__ __launch_bounds__(256) void k_viscous_march(KP kp, FArgs<T> a, T* __r
 #pragma unroll
     for (int q = 0; q < 9; ++q) m[q] = a.m.m[q][o];
     T out[3][4];
-    viscous_cell<T>(kp, w[0][2], w[1][2], w[2][2], w[3][2], g, m, (T)a.J[o], out);
+    viscous_cell<T, LES>(kp, w[0][2], w[1][2], w[2][2], w[3][2], g, m, (T)a.J[o], out,
+                         LES != F3D_LES_NONE ? a.Q[o] : T(0));
     if (act) {
       const long long vo = (i - 2) + (long long)NX * ((j - 2) + (long long)NY * (kabs - 2));
 #pragma unroll
@@ -878,14 +879,22 @@ int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double*
   }
   FArgs<T> a{Qin, m, J, phi, ws.R, ws.heavy, ws.hcount};
   const int rm = xcd_remap();
-  if (kp.viscous && !visc_fused()) {
+  // LES configurations take the two-kernel form (viscous fluxes through HBM): the in-place viscous
+  // update is sized to the laminar cell's registers
+  const bool inplace = visc_fused() && kp.les == F3D_LES_NONE;
+  if (kp.viscous && !inplace) {
     TimedScope ts("fused_viscous", s);
     const int KC = seg_len(kp.Nzp + 4, 32);
-    hipLaunchKernelGGL(k_viscous_march<T>, dim3((kp.Nxp + 4 + 63) / 64, (kp.Nyp + 4 + 3) / 4, (kp.Nzp + 4 + KC - 1) / KC),
-                       dim3(64, 4, 1), 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC, rm);
+    const dim3 g((kp.Nxp + 4 + 63) / 64, (kp.Nyp + 4 + 3) / 4, (kp.Nzp + 4 + KC - 1) / KC), b(64, 4, 1);
+    if (kp.les == F3D_LES_SMAG)
+      hipLaunchKernelGGL((k_viscous_march<T, F3D_LES_SMAG>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC, rm);
+    else if (kp.les == F3D_LES_WALE)
+      hipLaunchKernelGGL((k_viscous_march<T, F3D_LES_WALE>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC, rm);
+    else
+      hipLaunchKernelGGL((k_viscous_march<T, F3D_LES_NONE>), g, b, 0, s, kp, a, ws.Fvx, ws.Fvy, ws.Fvz, KC, rm);
   }
   TimedScope ts("fused_update", s);
-  if (kp.viscous && visc_fused()) {
+  if (kp.viscous && inplace) {
     const int KV = visc_kc(kp, VYI);
     if (mode == 0)
       launch_visc_update<T, 0>(kp, a, nullptr, nullptr, nullptr, Rout, 0.0, KV, rm, s);
@@ -920,7 +929,9 @@ int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>
   return fused_update<T>(kp, ws, 0, nullptr, nullptr, Q, nullptr, m, J, phi, R, 0.0, s);
 }
 
-bool stage_needs_separate_q(const KP& kp) { return use_fused() && kp.viscous && visc_fused(); }
+bool stage_needs_separate_q(const KP& kp) {
+  return use_fused() && kp.viscous && visc_fused() && kp.les == F3D_LES_NONE;
+}
 
 template <typename T>
 int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,

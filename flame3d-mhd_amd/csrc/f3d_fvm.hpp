@@ -12,11 +12,6 @@ namespace f3d {
 constexpr double kFvmEps7 = 1e-10;
 constexpr double kFvmEps5 = 1e-8;
 
-template <typename T>
-F3D_HD T jmin(T a, T b) {
-  return (a != a) ? a : ((b != b) ? b : ((b < a) ? b : a));
-}
-
 // Transverse strides of direction d in the order FVM.jl uses them: i faces (j, k), j faces (i, k),
 // k faces (i, j); `flip` = 1 on j faces, whose normal FVM.jl:258-260 negates (that makes every
 // normal point along +d).

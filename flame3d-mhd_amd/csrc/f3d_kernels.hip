@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_advect(KP kp, Range3 r, T* __restrict__
 }
 
 // ------------------------------------------------------------------ viscousFlux  viscous.jl:1-134
-template <typename T>
+template <typename T, int LES>
 __global__ __launch_bounds__(256) void k_viscous(KP kp, Range3 r, T* __restrict__ Fvx, T* __restrict__ Fvy,
                                                  T* __restrict__ Fvz, const T* __restrict__ Q, MetricPtrs<T> mp,
                                                  const double* __restrict__ J) {
@@ -125,7 +125,8 @@ __global__ __launch_bounds__(256) void k_viscous(KP kp, Range3 r, T* __restrict_
 #pragma unroll
   for (int q = 0; q < 9; ++q) m[q] = mp.m[q][o];
   T out[3][4];
-  viscous_cell<T>(kp, Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], g, m, (T)J[o], out);
+  viscous_cell<T, LES>(kp, Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], g, m, (T)J[o], out,
+                       LES != F3D_LES_NONE ? Q[o] : T(0));
   const int Nv1 = kp.Nxp + kp.NG, Nv2 = kp.Nyp + kp.NG;
   const long long vn = (long long)Nv1 * Nv2 * (kp.Nzp + kp.NG);
   const long long vo = (i - 2) + (long long)Nv1 * ((j - 2) + (long long)Nv2 * (k - 2));
@@ -387,7 +388,12 @@ int launch_viscous(const KP& kp, T* Fvx, T* Fvy, T* Fvz, const T* Q, const Metri
                    hipStream_t s) {
   TimedScope ts_("viscous_flux", s);
   Range3 r{2, 2, 2, kp.Nxt - 4, kp.Nyt - 4, kp.Nzt - 4};
-  hipLaunchKernelGGL(k_viscous<T>, range_grid(r), kBlock, 0, s, kp, r, Fvx, Fvy, Fvz, Q, m, J);
+  if (kp.les == F3D_LES_SMAG)
+    hipLaunchKernelGGL((k_viscous<T, F3D_LES_SMAG>), range_grid(r), kBlock, 0, s, kp, r, Fvx, Fvy, Fvz, Q, m, J);
+  else if (kp.les == F3D_LES_WALE)
+    hipLaunchKernelGGL((k_viscous<T, F3D_LES_WALE>), range_grid(r), kBlock, 0, s, kp, r, Fvx, Fvy, Fvz, Q, m, J);
+  else
+    hipLaunchKernelGGL((k_viscous<T, F3D_LES_NONE>), range_grid(r), kBlock, 0, s, kp, r, Fvx, Fvy, Fvz, Q, m, J);
   return check_launch("viscous_flux");
 }
 

@@ -354,9 +354,8 @@ int check_branch(const f3d_params* p, bool fvm_entry, const double* U, const T* 
     set_error("%s: params select the finite-volume branch (fvm = %d); use the _fvm entry", who, p->fvm);
     return F3D_E_ARG;
   }
-  if (p->les != F3D_LES_NONE || p->lts) {
-    set_error("%s: LES (les = %d) and local time stepping (lts = %d) are not available in this entry", who,
-              p->les, p->lts);
+  if (p->lts) {
+    set_error("%s: local time stepping (lts = %d) is not available in this entry", who, p->lts);
     return F3D_E_ARG;
   }
   if (fvm_entry) {

@@ -58,6 +58,11 @@ inline T jmax(T a, T b) {
   if (b != b) return b;
   return (b > a) ? b : a;
 }
+// NaN-propagating min (Julia Base.min on floats).
+template <typename T>
+inline T jmin(T a, T b) {
+  return (a != a) ? a : ((b != b) ? b : ((b < a) ? b : a));
+}
 
 // Linear = UP7*Linear_phi + CD6*(1-Linear_phi)   run.jl:75-77
 template <typename T>
@@ -707,7 +712,66 @@ void advect_comp(const f3d_params* p, int d, T* F, const T* phi, const T* S, con
 }
 
 // ---------------------------------------------------------------- viscous.jl
-// viscousFlux  viscous.jl:1-65, 108-133 (laminar; the LES branches 66-106 are off), domain
+// Eddy viscosity of the LES branches  viscous.jl:66-106, added to mu and lam (lam_t = Cp mu_t / Prt,
+// Prt = 0.9), mu_t capped at twice the laminar mu; filter width Delta = J^(-1/3).
+//   Smagorinsky (66-79): mu_t = rho (0.1 Delta)^2 sqrt(2 S_ij S_ij).
+//   WALE (80-106): mu_t = rho (0.325 Delta)^2 Sd^3 / (S^5 + Sd^(5/2)), S = sqrt(S_ij S_ij),
+//   Sd = sqrt(Sd_ij Sd_ij) of the traceless symmetric part of g^2 (g_ij = du_i/dx_j), in the
+//   reference's expanded products.  Where S = Sd = 0 the reference's 0/0 is replaced by D = 0 (the
+//   model's value without a velocity gradient).  The float form evaluates D from the gradient scaled by
+//   its largest component (S^5 overflows float beyond gradients of ~5e7 1/s); FP64 keeps the literal form.
+// g[3][3]: rows u, v, w; columns x, y, z.
+template <typename T>
+void les_eddy(const f3d_params* p, T rho, T Jac, const T g[3][3], T& mu, T& lam) {
+  if (p->les == F3D_LES_NONE) return;
+  const T Prt = T(0.9);
+  const T half = T(0.5);
+  const T dudx = g[0][0], dudy = g[0][1], dudz = g[0][2];
+  const T dvdx = g[1][0], dvdy = g[1][1], dvdz = g[1][2];
+  const T dwdx = g[2][0], dwdy = g[2][1], dwdz = g[2][2];
+  const T s12 = half * (dudy + dvdx), s13 = half * (dudz + dwdx), s23 = half * (dvdz + dwdy);
+  const T SS = dudx * dudx + dvdy * dvdy + dwdz * dwdz + 2 * (s12 * s12 + s13 * s13 + s23 * s23);
+  const T cb = std::cbrt(Jac);
+  T mut;
+  if (p->les == F3D_LES_SMAG) {
+    const T Sijmag = std::sqrt(2 * SS);
+    const T dl = T(0.1) / cb;
+    mut = jmin(rho * (dl * dl) * Sijmag, 2 * mu);
+  } else {
+    T sc = T(1), G[3][3];
+    if (sizeof(T) == 4) {
+      T gm = T(0);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) gm = std::max(gm, (T)std::fabs(g[a][b]));
+      sc = gm > T(0) ? gm : T(1);
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) G[a][b] = g[a][b] / sc;
+    const T ux = G[0][0], uy = G[0][1], uz = G[0][2], vx = G[1][0], vy = G[1][1], vz = G[1][2];
+    const T wx = G[2][0], wy = G[2][1], wz = G[2][2];
+    const T t12 = half * (uy + vx), t13 = half * (uz + wx), t23 = half * (vz + wy);
+    const T S = std::sqrt(ux * ux + vy * vy + wz * wz + 2 * (t12 * t12 + t13 * t13 + t23 * t23));
+    T Sd11 = ux * ux + uy * vx + uz * wx;
+    T Sd22 = vx * uy + vy * vy + vz * wy;
+    T Sd33 = wx * uz + wy * vz + wz * wz;
+    const T tr = (Sd11 + Sd22 + Sd33) / T(3);
+    Sd11 -= tr;
+    Sd22 -= tr;
+    Sd33 -= tr;
+    const T Sd12 = half * (ux * vx + vx * vy + wx * vz + uy * ux + vy * uy + wy * uz);
+    const T Sd13 = half * (ux * wx + vx * wy + wx * wz + uz * ux + vz * uy + wz * uz);
+    const T Sd23 = half * (uy * wx + vy * wy + wy * wz + uz * vx + vz * vy + wz * vz);
+    const T Sd = std::sqrt(Sd11 * Sd11 + Sd22 * Sd22 + Sd33 * Sd33 + 2 * (Sd12 * Sd12 + Sd13 * Sd13 + Sd23 * Sd23));
+    const T den = S * S * S * S * S + Sd * Sd * std::sqrt(Sd);
+    const T D = den > T(0) ? sc * (Sd * Sd * Sd / den) : T(0);
+    const T dl = T(0.325) / cb;
+    mut = jmin(rho * (dl * dl) * D, 2 * mu);
+  }
+  mu += mut;
+  lam += (T)p->Cp * mut / Prt;
+}
+
+// viscousFlux  viscous.jl:1-134 (laminar + the LES eddy viscosity selected by p->les), domain
 // [3, N_tot-2]^3, stored at (i-2, j-2, k-2).  J is double in both forms (cast to T here).
 template <typename T>
 void viscous_flux(const f3d_params* p, T* Fvx, T* Fvy, T* Fvz, const T* Q, const T* const m[9],
@@ -752,6 +816,12 @@ void viscous_flux(const f3d_params* p, T* Fvx, T* Fvy, T* Fvz, const T* Q, const
         T dTdy = (Txi * xy + Teta * ey + Tzeta * zy) * Jac;
         T dTdz = (Txi * xz + Teta * ez + Tzeta * zz) * Jac;
         T dv = dudx + dvdy + dwdz;
+        if constexpr (std::is_floating_point<T>::value) {  // (the dual-number forms are laminar only)
+          if (p->les != F3D_LES_NONE) {
+            const T gr[3][3] = {{dudx, dudy, dudz}, {dvdx, dvdy, dvdz}, {dwdx, dwdy, dwdz}};
+            les_eddy<T>(p, Q[o], Jac, gr, mu, lam);
+          }
+        }
         T t11 = mu * (2 * dudx - c23 * dv);
         T t12 = mu * (dudy + dvdx);
         T t13 = mu * (dudz + dwdx);

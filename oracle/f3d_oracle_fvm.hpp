@@ -28,13 +28,6 @@ struct GeoT<long double> {
   using type = long double;
 };
 
-// Julia's min for floats propagates NaN (Base.min).
-template <typename T>
-inline T jmin(T a, T b) {
-  if (a != a) return a;
-  if (b != b) return b;
-  return (b < a) ? b : a;
-}
 
 // The two transverse dimensions of direction d in the order FVM.jl uses them: i faces (j, k)
 // (FVM.jl:16-21), j faces (i, k) (FVM.jl:250-255), k faces (i, j) (FVM.jl:465-470).

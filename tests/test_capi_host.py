@@ -141,3 +141,29 @@ def test_fvm_selector_and_entry_validation():
     h = ctypes.c_void_p()
     assert L.f3d_adjoint_workspace_create(ctypes.byref(p), ctypes.byref(h)) == -4
     assert b"finite-difference" in L.f3d_last_error()
+
+
+def test_alternative_flags_validation():
+    """split / character / les / lts are checked by f3d_params_check; the adjoint covers the default
+    Steger-Warming + characteristic + laminar + global-dt path only (rejected before any allocation)."""
+    L = f3d.lib()
+    q = f3d.F3DParams()
+    for field, bad in (("split", 4), ("split", -1), ("character", 2), ("les", 3), ("lts", 2)):
+        L.f3d_params_init(ctypes.byref(q), 16, 16, 16)
+        setattr(q, field, bad)
+        assert L.f3d_params_check(ctypes.byref(q)) == -4, field
+    for kw in ({"split": f3d.SPLIT_LF}, {"split": f3d.SPLIT_AUSM}, {"character": 0}, {"les": f3d.LES_WALE}):
+        p = f3d.make_params(16, 12, 8, **kw)
+        h = ctypes.c_void_p()
+        assert L.f3d_adjoint_workspace_create(ctypes.byref(p), ctypes.byref(h)) == -4, kw
+        assert b"adjoint covers" in L.f3d_last_error()
+    # the other splittings and the component-wise advect validate their pointers before any launch
+    p = f3d.make_params(16, 12, 8)
+    for name in ("flux_split_lf", "flux_split_vl", "flux_split_ausm"):
+        for suf in ("f64", "f32b"):
+            assert getattr(L, f"f3d_{name}_{suf}")(ctypes.byref(p), None, None, None, None, None, None, None,
+                                                   None) == -1
+    for d in "xyz":
+        for suf in ("f64", "f32b"):
+            assert getattr(L, f"f3d_advect_{d}_{suf}")(ctypes.byref(p), None, None, None, None, None,
+                                                       ctypes.c_int32(5), None) == -1
