@@ -3,7 +3,7 @@
 The reference (rygan-code/Flame3D-MHD, Julia + CUDA.jl) launches per-stage kernels from
 `time_step` (solver.jl:191-680).  This module exposes the same entry points with the same names
 and argument order -- `shockSensor`, `fluxSplit_SW/LF/VL/AUSM`, `advect_xc/yc/zc`, the
-component-wise `advect_x/y/z`, `viscousFlux`, `div`,
+component-wise `advect_x/y/z`, `viscousFlux`, `div`, `div_LTS`, `compute_dt`,
 `linComb`, `c2Prim`, `prim2c`, `exchange_ghost`, `fillGhost`, `flowAdvance`, `time_step`, and the
 finite-volume branch's `Eigen_reconstruct_{i,j,k}` / `Conser_reconstruct_{i,j,k}` (FVM.jl) -- as
 methods of a `Flame3D` context that carries what the reference keeps in `const` globals
@@ -329,6 +329,14 @@ class Flame3D:
         """div.jl:1-50"""
         self._call(f"f3d_div_{self._suffix(Fx)}", *map(_ptr, (U, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z)),
                    c_double(dt), _ptr(J))
+
+    def div_LTS(self, U, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, dt, J):
+        """div.jl:52-101 (dt: per-cell float64 array of the cell shape)"""
+        self._call(f"f3d_div_lts_{self._suffix(Fx)}", *map(_ptr, (U, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, dt, J)))
+
+    def compute_dt(self, dt, Q, J, S1, S2, S3):
+        """utils.jl:73-100 (dt: float64 array of the cell shape; interior cells written)"""
+        self._call(f"f3d_compute_dt_{self._suffix(Q)}", *map(_ptr, (dt, Q, J, S1, S2, S3)))
 
     def linComb(self, U, Un, NV, a, b):
         """utils.jl:57-70"""

@@ -56,6 +56,7 @@ KP make_kp(const f3d_params* p) {
   k.eps7 = p->weno7_eps;
   k.eps5 = p->weno5_eps;
   k.Twall = p->T_wall;
+  k.lts_cfl = p->lts_cfl;
   // Linear = UP7*Linear_phi + CD6*(1-Linear_phi)  run.jl:75-77
   const double up7[7] = {-3.0 / 420, 25.0 / 420, -101.0 / 420, 319.0 / 420, 214.0 / 420, -38.0 / 420, 4.0 / 420};
   const double cd6[7] = {0.0, 1.0 / 60, -2.0 / 15, 37.0 / 60, 37.0 / 60, -2.0 / 15, 1.0 / 60};
@@ -366,6 +367,37 @@ VISC_IMPL(f32b, float)
   }
 DIV_IMPL(f64, double)
 DIV_IMPL(f32b, float)
+
+// div_LTS  div.jl:52-101 (per-cell dt) and compute_dt  utils.jl:73-100
+#define LTS_IMPL(SUF, T)                                                                                     \
+  F3D_API int f3d_div_lts_##SUF(const f3d_params* p, double* U, const T* Fx, const T* Fy, const T* Fz,       \
+                                const T* Fvx, const T* Fvy, const T* Fvz, const double* dt, const double* J, \
+                                f3d_stream_t s) {                                                            \
+    CHECK_P();                                                                                               \
+    NEED(U);                                                                                                 \
+    NEED(Fx);                                                                                                \
+    NEED(Fy);                                                                                                \
+    NEED(Fz);                                                                                                \
+    NEED(Fvx);                                                                                               \
+    NEED(Fvy);                                                                                               \
+    NEED(Fvz);                                                                                               \
+    NEED(dt);                                                                                                \
+    NEED(J);                                                                                                 \
+    return launch_div<T>(make_kp(p), U, Fx, Fy, Fz, Fvx, Fvy, Fvz, 0.0, J, (hipStream_t)s, dt);              \
+  }                                                                                                          \
+  F3D_API int f3d_compute_dt_##SUF(const f3d_params* p, double* dt, const T* Q, const double* J, const T* S1, \
+                                   const T* S2, const T* S3, f3d_stream_t s) {                               \
+    CHECK_P();                                                                                               \
+    NEED(dt);                                                                                                \
+    NEED(Q);                                                                                                 \
+    NEED(J);                                                                                                 \
+    NEED(S1);                                                                                                \
+    NEED(S2);                                                                                                \
+    NEED(S3);                                                                                                \
+    return launch_compute_dt<T>(make_kp(p), dt, Q, J, S1, S2, S3, nullptr, (hipStream_t)s);                  \
+  }
+LTS_IMPL(f64, double)
+LTS_IMPL(f32b, float)
 
 F3D_API int f3d_lin_comb(const f3d_params* p, double* U, const double* Un, int32_t NV, double a, double b,
                          f3d_stream_t s) {

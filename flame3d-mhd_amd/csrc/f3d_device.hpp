@@ -23,6 +23,7 @@ struct KP {
   long long nc;  // cells per variable, Nxt*Nyt*Nzt
   double gamma, Rg, Cp, Cs, Ts, Pr;
   double phi1, phi2, phi3, eps7, eps5, Twall;
+  double lts_cfl;  // LTS_CFL of compute_dt
   double lin[7];
   double qinf[6];
   // differenced UP7 weights: F(i) - F(i+1) = sum_r da[r+4] Fp[i+r] (r=-4..3) + sum_r db[r+3] Fm[i+r] (r=-3..4)

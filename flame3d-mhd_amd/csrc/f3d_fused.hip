@@ -60,6 +60,7 @@ struct FArgs {
   T* R;
   unsigned long long* heavy;
   unsigned int* hcount;
+  const double* ldt = nullptr;  // per-cell dt of local time stepping (div_LTS), else the scalar dt
 };
 
 __device__ __forceinline__ long long ridx(const KP& kp, int i, int j, int k) {
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
       for (int n = 0; n < 5; ++n) Rout[o + n * nc] = (double)Rt[n] * Jc;
       continue;
     }
-    const double Jact = Jc * dt;
+    const double Jact = Jc * (a.ldt ? a.ldt[o] : dt);
     double un[5];
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
@@ -505,14 +506,14 @@ __device__ __forceinline__ void visc_eval(const KP& kp, const FArgs<T>& a, long 
 template <typename T, int MODE>
 __device__ __forceinline__ void update_cell(const KP& kp, const T* Rt, long long oc, double* __restrict__ U,
                                             double* __restrict__ Un, T* __restrict__ Q, double* __restrict__ Rout,
-                                            double Jc, double dt) {
+                                            double Jc, double dt, const double* __restrict__ ldt) {
   const long long nc = kp.nc;
   if (MODE == 0) {
 #pragma unroll
     for (int n = 0; n < 5; ++n) Rout[oc + n * nc] = (double)Rt[n] * Jc;
     return;
   }
-  const double Jact = Jc * dt;
+  const double Jact = Jc * (ldt ? ldt[oc] : dt);
   double un[5];
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
@@ -655,7 +656,7 @@ __global__ __launch_bounds__(VX * ROWS) void k_visc_update(KP kp, FArgs<T> a, do
       sdiv += c1 * (fz[n][0] - fz[n][4]) + c2 * (fz[n][1] - fz[n][3]);
       Rt[n + 1] += sdiv;
     }
-    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt);
+    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt, a.ldt);
   }
 }
 
@@ -761,7 +762,7 @@ __global__ __launch_bounds__(512) void k_visc_update8(KP kp, FArgs<T> a, double*
       sdiv += c1 * (fz[n][0] - fz[n][4]) + c2 * (fz[n][1] - fz[n][3]);
       Rt[n + 1] += sdiv;
     }
-    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt);
+    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt, a.ldt);
   }
 }
 
@@ -878,6 +879,7 @@ int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double*
     return F3D_E_ARG;
   }
   FArgs<T> a{Qin, m, J, phi, ws.R, ws.heavy, ws.hcount};
+  if (mode != 0 && kp.lts) a.ldt = ws.ldt;
   const int rm = xcd_remap();
   // LES configurations take the two-kernel form (viscous fluxes through HBM): the in-place viscous
   // update is sized to the laminar cell's registers

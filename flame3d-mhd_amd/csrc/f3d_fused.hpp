@@ -15,6 +15,7 @@ struct WsView {
   unsigned long long* heavy; // compacted non-UP7 faces: (right cell << 4) | (scheme << 2) | dir
   unsigned int* hcount;      // number of entries in `heavy`
   unsigned long long hcap;   // capacity of `heavy` (total face count of the three directions)
+  double* ldt;               // per-cell dt of local time stepping, cell-shaped (written at stage 1)
 };
 
 // Block coordinates with an XCD-aware order.  Workgroups are dispatched round-robin over the 8 XCDs

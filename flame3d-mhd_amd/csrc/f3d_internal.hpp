@@ -59,7 +59,12 @@ int launch_viscous(const KP&, T* Fvx, T* Fvy, T* Fvz, const T* Q, const MetricPt
                    hipStream_t);
 template <typename T>
 int launch_div(const KP&, double* U, const T* Fx, const T* Fy, const T* Fz, const T* Fvx, const T* Fvy,
-               const T* Fvz, double dt, const double* J, hipStream_t);
+               const T* Fvz, double dt, const double* J, hipStream_t, const double* ldt = nullptr);
+// compute_dt (utils.jl:73-100) into the interior cells of dt; m != nullptr: the sizes from the metric
+// rows instead of S1..S3.
+template <typename T>
+int launch_compute_dt(const KP&, double* dt, const T* Q, const double* J, const T* S1, const T* S2, const T* S3,
+                      const MetricPtrs<T>* m, hipStream_t);
 int launch_lin_comb(const KP&, double* U, const double* Un, int NV, double a, double b, hipStream_t);
 template <typename T> int launch_c2prim(const KP&, const double* U, T* Q, hipStream_t);
 template <typename T> int launch_prim2c(const KP&, double* U, const T* Q, hipStream_t);

@@ -143,11 +143,12 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_div(KP kp, Range3 r, double* __restrict__ U, const T* __restrict__ Fx,
                                              const T* __restrict__ Fy, const T* __restrict__ Fz,
                                              const T* __restrict__ Fvx, const T* __restrict__ Fvy,
-                                             const T* __restrict__ Fvz, double dt, const double* __restrict__ J) {
+                                             const T* __restrict__ Fvz, double dt, const double* __restrict__ J,
+                                             const double* __restrict__ ldt) {
   RANGE_IDX(r);
   const int NG = kp.NG, Nxp = kp.Nxp, Nyp = kp.Nyp, Nzp = kp.Nzp;
   const long long o = cidx(kp, i + NG, j + NG, k + NG), nc = kp.nc;
-  const double Jact = J[o] * dt;
+  const double Jact = J[o] * (ldt ? ldt[o] : dt);  // div_LTS (div.jl:63) reads the cell's own dt
   const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
   const int Nv1 = Nxp + NG, Nv2 = Nyp + NG;
   const long long vn = (long long)Nv1 * Nv2 * (Nzp + NG);
@@ -176,6 +177,36 @@ __global__ __launch_bounds__(256) void k_div(KP kp, Range3 r, double* __restrict
     }
     U[o + (n + 1) * nc] += (double)s * Jact;
   }
+}
+
+// ------------------------------------------------------------------ compute_dt  utils.jl:73-100
+// Local time step of every interior cell: lts_cfl * min(dx/(u+c), dy/(v+c), dz/(w+c)) with
+// dx = (1/J)/S1 etc. (u+c as written, not |u|+c), in double from the bulk-type inputs.  SFROM_A: the
+// sizes from the metric rows (|A_d|, the fused stage) instead of the S arrays.
+template <typename T, bool SFROM_A>
+__global__ __launch_bounds__(256) void k_compute_dt(KP kp, Range3 r, double* __restrict__ dt, const T* __restrict__ Q,
+                                                    const double* __restrict__ J, const T* __restrict__ S1,
+                                                    const T* __restrict__ S2, const T* __restrict__ S3,
+                                                    MetricPtrs<T> mp) {
+  RANGE_IDX(r);
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+  const double u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc], Tt = Q[o + 5 * nc];
+  const double V = 1.0 / J[o];
+  const double c = sqrt(kp.gamma * kp.Rg * Tt);
+  double S[3];
+  if (SFROM_A) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T a = mp.m[3 * d][o], b = mp.m[3 * d + 1][o], e = mp.m[3 * d + 2][o];
+      S[d] = (double)sqrt(a * a + b * b + e * e);
+    }
+  } else {
+    S[0] = S1[o];
+    S[1] = S2[o];
+    S[2] = S3[o];
+  }
+  const double dtx = (V / S[0]) / (u + c), dty = (V / S[1]) / (v + c), dtz = (V / S[2]) / (w + c);
+  dt[o] = jmin<double>(jmin<double>(dtx, dty), dtz) * kp.lts_cfl;
 }
 
 // ------------------------------------------------------------------ linComb, c2Prim, prim2c
@@ -399,11 +430,25 @@ int launch_viscous(const KP& kp, T* Fvx, T* Fvy, T* Fvz, const T* Q, const Metri
 
 template <typename T>
 int launch_div(const KP& kp, double* U, const T* Fx, const T* Fy, const T* Fz, const T* Fvx, const T* Fvy,
-               const T* Fvz, double dt, const double* J, hipStream_t s) {
+               const T* Fvz, double dt, const double* J, hipStream_t s, const double* ldt) {
   TimedScope ts_("div", s);
   Range3 r{0, 0, 0, kp.Nxp, kp.Nyp, kp.Nzp};
-  hipLaunchKernelGGL(k_div<T>, range_grid(r), kBlock, 0, s, kp, r, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, dt, J);
+  hipLaunchKernelGGL(k_div<T>, range_grid(r), kBlock, 0, s, kp, r, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, dt, J, ldt);
   return check_launch("div");
+}
+
+template <typename T>
+int launch_compute_dt(const KP& kp, double* dt, const T* Q, const double* J, const T* S1, const T* S2, const T* S3,
+                      const MetricPtrs<T>* m, hipStream_t s) {
+  TimedScope ts_("compute_dt", s);
+  Range3 r{kp.NG, kp.NG, kp.NG, kp.Nxp, kp.Nyp, kp.Nzp};
+  MetricPtrs<T> mp{};
+  if (m) mp = *m;
+  if (m)
+    hipLaunchKernelGGL((k_compute_dt<T, true>), range_grid(r), kBlock, 0, s, kp, r, dt, Q, J, S1, S2, S3, mp);
+  else
+    hipLaunchKernelGGL((k_compute_dt<T, false>), range_grid(r), kBlock, 0, s, kp, r, dt, Q, J, S1, S2, S3, mp);
+  return check_launch("compute_dt");
 }
 
 int launch_lin_comb(const KP& kp, double* U, const double* Un, int NV, double a, double b, hipStream_t s) {
@@ -468,7 +513,9 @@ int launch_fill(const KP& kp, T* Q, double* U, const T* inflow, hipStream_t s) {
   template int launch_viscous<T>(const KP&, T*, T*, T*, const T*, const MetricPtrs<T>&, const double*,          \
                                  hipStream_t);                                                                  \
   template int launch_div<T>(const KP&, double*, const T*, const T*, const T*, const T*, const T*, const T*,    \
-                             double, const double*, hipStream_t);                                               \
+                             double, const double*, hipStream_t, const double*);                                \
+  template int launch_compute_dt<T>(const KP&, double*, const T*, const double*, const T*, const T*, const T*,  \
+                                    const MetricPtrs<T>*, hipStream_t);                                         \
   template int launch_c2prim<T>(const KP&, const double*, T*, hipStream_t);                                     \
   template int launch_prim2c<T>(const KP&, double*, const T*, hipStream_t);                                     \
   template int launch_exchange<T>(const KP&, T*, int, hipStream_t);                                             \

@@ -23,6 +23,7 @@ struct f3d_workspace {
   unsigned int* hcount;
   unsigned long long hcap;
   void *Qa, *Qb;  // stage outputs of the fused path (primitive state, 6 vars, bulk type)
+  double* ldt;    // per-cell dt of local time stepping (cell-shaped; compute_dt at stage 1)
 };
 
 namespace {
@@ -248,6 +249,9 @@ int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double*
   MetricPtrs<T> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
   int rc = F3D_OK;
+  // local time stepping: the per-cell dt of the step from its first stage's state (solver.jl:419-421)
+  if (kp.lts && KRK == 1) rc = launch_compute_dt<T>(kp, ws->ldt, Qin, J, nullptr, nullptr, nullptr, &m, s);
+  if (rc) return rc;
   if (use_fused()) {
     rc = geo ? fvm_fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, geo, dt, s)
                 : fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, dt, s);
@@ -257,7 +261,8 @@ int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double*
     rc = geo ? rhs_chain_fvm<T>(ws, kp, Qout, U, m, J, geo, (T*)ws->phi, s)
                 : rhs_chain<T>(ws, kp, Qout, m, J, (T*)ws->phi, s);
     if (!rc)
-      rc = launch_div<T>(kp, U, (T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, dt, J, s);
+      rc = launch_div<T>(kp, U, (T*)ws->Fx, (T*)ws->Fy, (T*)ws->Fz, (T*)ws->Fvx, (T*)ws->Fvy, (T*)ws->Fvz, dt, J, s,
+                         kp.lts ? ws->ldt : nullptr);
     if (!rc && KRK == 2) rc = launch_lin_comb(kp, U, Un, 5, 0.25, 0.75, s);
     if (!rc && KRK == 3) rc = launch_lin_comb(kp, U, Un, 5, 2.0 / 3.0, 1.0 / 3.0, s);
     if (!rc) rc = launch_c2prim<T>(kp, U, Qout, s);
@@ -354,10 +359,6 @@ int check_branch(const f3d_params* p, bool fvm_entry, const double* U, const T* 
     set_error("%s: params select the finite-volume branch (fvm = %d); use the _fvm entry", who, p->fvm);
     return F3D_E_ARG;
   }
-  if (p->lts) {
-    set_error("%s: local time stepping (lts = %d) is not available in this entry", who, p->lts);
-    return F3D_E_ARG;
-  }
   if (fvm_entry) {
     if (!U || !geo || !geo[0] || !geo[1] || !geo[2]) {
       set_error("%s: the finite-volume branch needs U and the face geometry arrays geo[3]", who);
@@ -387,7 +388,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
   const size_t nc = f3d_cell_count(p);
   const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
-  size_t sz[16];
+  size_t sz[17];
   sz[0] = nc * es;                               // phi
   sz[1] = nc * 5 * es;                           // Fp
   sz[2] = nc * 5 * es;                           // Fm
@@ -403,6 +404,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[12] = nfaces * sizeof(unsigned long long);  // heavy-face list
   sz[13] = 256;                                  // heavy-face counter
   sz[14] = sz[15] = nc * 6 * es;                 // Qa, Qb
+  sz[16] = nc * sizeof(double);                  // ldt
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -413,10 +415,10 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   ws->bytes = total;
   ws->base = base;
   char* c = (char*)base;
-  void** slots[16] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+  void** slots[17] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
                       &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount,
-                      &ws->Qa, &ws->Qb};
-  for (int i = 0; i < 16; ++i) {
+                      &ws->Qa, &ws->Qb, (void**)&ws->ldt};
+  for (int i = 0; i < 17; ++i) {
     *slots[i] = c;
     c += align_up(sz[i]);
   }
@@ -460,6 +462,7 @@ WsView<T> ws_view(f3d_workspace* ws) {
   v.heavy = ws->heavy;
   v.hcount = ws->hcount;
   v.hcap = ws->hcap;
+  v.ldt = ws->ldt;
   return v;
 }
 template WsView<double> ws_view<double>(f3d_workspace*);

@@ -104,7 +104,7 @@ typedef struct f3d_params {
   int32_t split;     /* F3D_SPLIT_* (run.jl:69 splitMethod) */
   int32_t character; /* 1: characteristic reconstruction advect_*c, 0: component-wise advect_* (run.jl:68) */
   int32_t les;       /* F3D_LES_NONE / F3D_LES_SMAG / F3D_LES_WALE (run.jl:4-5) */
-  int32_t lts;       /* local time stepping (run.jl:21) */
+  int32_t lts;       /* local time stepping (run.jl:21): per-cell dt of compute_dt, div_LTS */
   int32_t filtering, filtering_nonlinear, filtering_interval; /* run.jl:50-52 */
   int32_t pad_;
   double lts_cfl;                    /* run.jl:22 */
@@ -218,6 +218,25 @@ int f3d_div_f64(const f3d_params* p, double* U, const double* Fx, const double* 
 int f3d_div_f32b(const f3d_params* p, double* U, const float* Fx, const float* Fy,
                  const float* Fz, const float* Fv_x, const float* Fv_y, const float* Fv_z,
                  double dt, const double* J, f3d_stream_t s);
+
+/* div_LTS(U, Fx, Fy, Fz, Fv_x, Fv_y, Fv_z, dt, J)  div.jl:52-101 : div with the per-cell dt
+ * (cell-shaped, double in both forms; the interior cells are read) in place of the scalar dt */
+int f3d_div_lts_f64(const f3d_params* p, double* U, const double* Fx, const double* Fy,
+                    const double* Fz, const double* Fv_x, const double* Fv_y, const double* Fv_z,
+                    const double* dt, const double* J, f3d_stream_t s);
+int f3d_div_lts_f32b(const f3d_params* p, double* U, const float* Fx, const float* Fy,
+                     const float* Fz, const float* Fv_x, const float* Fv_y, const float* Fv_z,
+                     const double* dt, const double* J, f3d_stream_t s);
+
+/* compute_dt(dt, Q, J, S1, S2, S3)  utils.jl:73-100 : the local time step of every interior cell,
+ * dt = lts_cfl * min(dx/(u+c), dy/(v+c), dz/(w+c)), dx = (1/J)/S1, dy = (1/J)/S2, dz = (1/J)/S3
+ * (u+c as the reference writes it, not |u|+c).  dt is double in both forms (the reference's
+ * LTS_dt is Float32); ghost cells of dt are not written.  The fused stage entries compute it
+ * themselves at KRK = 1 when p->lts (sizes from the metric rows) and then ignore their scalar dt. */
+int f3d_compute_dt_f64(const f3d_params* p, double* dt, const double* Q, const double* J,
+                       const double* S1, const double* S2, const double* S3, f3d_stream_t s);
+int f3d_compute_dt_f32b(const f3d_params* p, double* dt, const float* Q, const double* J,
+                        const float* S1, const float* S2, const float* S3, f3d_stream_t s);
 
 /* linComb(U, Un, NV, a, b)  utils.jl:57-70 : U = a*U + b*Un on the interior (both forms: double) */
 int f3d_lin_comb(const f3d_params* p, double* U, const double* Un, int32_t NV, double a,
@@ -337,7 +356,10 @@ int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, const float*
                  const float* const* metrics, const double* J, float* phi, f3d_stream_t s);
 
 /* One SSP-RK3 stage KRK in {1,2,3} of the loop body solver.jl:416-444: (KRK==1: Un <- U), sensor,
- * RHS, U += dt*R, (KRK==2: U = U/4 + 3Un/4, KRK==3: U = 2U/3 + Un/3), c2Prim, ghost fill of Q. */
+ * RHS, U += dt*R, (KRK==2: U = U/4 + 3Un/4, KRK==3: U = 2U/3 + Un/3), c2Prim, ghost fill of Q.
+ * p->lts: KRK==1 also computes the per-cell dt (compute_dt) from Q into the workspace, and every
+ * stage updates with it (div_LTS) instead of `dt`, which is then ignored; stages 2-3 use the per-cell
+ * dt of the workspace's last stage 1. */
 int f3d_rk3_stage_f64(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U,
                       double* Un, double* Q, const double* const* metrics, const double* J,
                       double dt, const double* inflow, f3d_stream_t s);
@@ -345,7 +367,8 @@ int f3d_rk3_stage_f32b(const f3d_params* p, f3d_workspace* ws, int32_t KRK, doub
                        double* Un, float* Q, const float* const* metrics, const double* J,
                        double dt, const float* inflow, f3d_stream_t s);
 
-/* `nsteps` full RK3 steps with a fixed dt (time_step's loop, solver.jl:410-444, without I/O). */
+/* `nsteps` full RK3 steps with a fixed dt (time_step's loop, solver.jl:410-444, without I/O);
+ * p->lts: a per-cell dt recomputed at the start of every step, as above. */
 int f3d_time_step_f64(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
                       double* Un, double* Q, const double* const* metrics, const double* J,
                       double dt, const double* inflow, f3d_stream_t s);

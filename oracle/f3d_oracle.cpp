@@ -174,6 +174,27 @@ void time_step_ref(const f3d_params* p, int nsteps, double* U, double* Un, T* Q,
 }
 }  // namespace
 
+// compute_dt (utils.jl:73-100) into dt (cell-shaped, double; interior cells written) and div_LTS
+// (div.jl:52-101) with such a per-cell dt.
+OR_API void oracle_compute_dt_f64(const f3d_params* p, double* dt, const double* Q, const double* J, const double* S1,
+                                  const double* S2, const double* S3) {
+  compute_dt<double>(p, dt, Q, J, S1, S2, S3);
+}
+OR_API void oracle_compute_dt_f32b(const f3d_params* p, double* dt, const float* Q, const double* J, const float* S1,
+                                   const float* S2, const float* S3) {
+  compute_dt<float>(p, dt, Q, J, S1, S2, S3);
+}
+OR_API void oracle_div_lts_f64(const f3d_params* p, double* U, const double* Fx, const double* Fy, const double* Fz,
+                               const double* Fvx, const double* Fvy, const double* Fvz, const double* dt,
+                               const double* J) {
+  div<double>(p, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, 0.0, J, dt);
+}
+OR_API void oracle_div_lts_f32b(const f3d_params* p, double* U, const float* Fx, const float* Fy, const float* Fz,
+                                const float* Fvx, const float* Fvy, const float* Fvz, const double* dt,
+                                const double* J) {
+  div<float>(p, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, 0.0, J, dt);
+}
+
 OR_API void oracle_rhs_f64(const f3d_params* p, double* R, const double* Q, const void* const* m, const double* J,
                            double* phi, uint8_t* mx, uint8_t* my, uint8_t* mz) {
   rhs_ref<double>(p, R, Q, m, J, phi, mx, my, mz);

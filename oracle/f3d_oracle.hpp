@@ -845,9 +845,11 @@ void viscous_flux(const f3d_params* p, T* Fvx, T* Fvy, T* Fvz, const T* Q, const
 // ---------------------------------------------------------------- div.jl
 // div  div.jl:1-50 : interior (i,j,k) in [1, Np]^3, U index +NG.  The flux differences accumulate
 // in T; the conservative update is in double.
+// ldt != nullptr: div_LTS  div.jl:52-101, the same update with the per-cell dt ldt[cell] (interior cells
+// of a cell-shaped array) in place of the scalar dt.
 template <typename T>
 void div(const f3d_params* p, double* U, const T* Fx, const T* Fy, const T* Fz, const T* Fvx,
-         const T* Fvy, const T* Fvz, double dt, const double* J) {
+         const T* Fvy, const T* Fvz, double dt, const double* J, const double* ldt = nullptr) {
   Grid g(p);
   const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
   const long NG = g.NG;
@@ -855,7 +857,8 @@ void div(const f3d_params* p, double* U, const T* Fx, const T* Fy, const T* Fz, 
   for (long k = 1; k <= g.Nzp; ++k)
     for (long j = 1; j <= g.Nyp; ++j)
       for (long i = 1; i <= g.Nxp; ++i) {
-        double Jact = J[g.cell.at(i + NG, j + NG, k + NG)] * dt;
+        const size_t oc = g.cell.at(i + NG, j + NG, k + NG);
+        double Jact = J[oc] * (ldt ? ldt[oc] : dt);
         T dV[3][4];
         for (int n = 1; n <= 4; ++n) {
           dV[0][n - 1] = c1 * (Fvx[g.fv.at(i, j + 2, k + 2, n)] - Fvx[g.fv.at(i + 4, j + 2, k + 2, n)]) +
@@ -1000,14 +1003,17 @@ void flow_advance(const f3d_params* p, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, 
   if (p->viscous) viscous_flux<T>(p, Fvx, Fvy, Fvz, Q, m, J);
 }
 
-// Scratch arrays of the reference's time_step (solver.jl:294-308), sized for one params.
+// Scratch arrays of the reference's time_step (solver.jl:294-308), sized for one params; ldt is the
+// per-cell dt of local time stepping (LTS_dt, solver.jl:304-306), double here.
 template <typename T>
 struct Scratch {
   std::vector<T> phi, Fp, Fm, Fx, Fy, Fz, Fvx, Fvy, Fvz;
+  std::vector<double> ldt;
   explicit Scratch(const f3d_params* p) {
     Grid g(p);
     size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
     phi.assign(nc, 0);
+    if (p->lts) ldt.assign(nc, 0.0);
     Fp.assign(nc * 5, 0);
     Fm.assign(nc * 5, 0);
     Fx.assign((size_t)(g.Nxp + 1) * g.Nyp * g.Nzp * 5, 0);
@@ -1033,7 +1039,32 @@ void metric_norms(const f3d_params* p, const T* const m[9], T* s1, T* s2, T* s3)
     }
 }
 
-// One RK3 stage of the time loop body  solver.jl:416-444 (LTS, filtering and I/O off).
+// compute_dt  utils.jl:73-100: the local time step of every interior cell,
+//   dt = LTS_CFL * min(dx/(u+c), dy/(v+c), dz/(w+c)),  dx = (1/J)/S1, dy = (1/J)/S2, dz = (1/J)/S3,
+// with the Cartesian velocity components u, v, w against the xi/eta/zeta sizes and u+c (not |u|+c)
+// as written (a cell whose u < -c gets a negative dt there).  In double from the bulk-type inputs;
+// Julia's NaN-propagating min.
+template <typename T>
+void compute_dt(const f3d_params* p, double* dt, const T* Q, const double* J, const T* S1, const T* S2,
+                const T* S3) {
+  Grid g(p);
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  for (long k = g.NG + 1; k <= g.Nzp + g.NG; ++k)
+    for (long j = g.NG + 1; j <= g.Nyp + g.NG; ++j)
+      for (long i = g.NG + 1; i <= g.Nxp + g.NG; ++i) {
+        const size_t o = g.cell.at(i, j, k);
+        const double u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc], Tt = Q[o + 5 * nc];
+        const double V = 1.0 / J[o];
+        const double c = std::sqrt(p->gamma * p->Rg * Tt);
+        const double dx = V / (double)S1[o], dy = V / (double)S2[o], dz = V / (double)S3[o];
+        const double dtx = dx / (u + c), dty = dy / (v + c), dtz = dz / (w + c);
+        dt[o] = jmin(jmin(dtx, dty), dtz) * p->lts_cfl;
+      }
+}
+
+// One RK3 stage of the time loop body  solver.jl:416-444 (filtering and I/O off).  Local time
+// stepping (p->lts): compute_dt from the stage-1 state at KRK == 1 (solver.jl:419-421), div_LTS
+// with it in every stage (426-428); the scalar dt is then unused.
 template <typename T>
 void rk3_stage(const f3d_params* p, Scratch<T>& sc, int KRK, double* U, double* Un, T* Q,
                const T* s1, const T* s2, const T* s3, const T* const m[9], const double* J,
@@ -1041,11 +1072,12 @@ void rk3_stage(const f3d_params* p, Scratch<T>& sc, int KRK, double* U, double* 
   Grid g(p);
   size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
   if (KRK == 1) std::memcpy(Un, U, nc * 5 * sizeof(double));  // solver.jl:418
+  if (KRK == 1 && p->lts) compute_dt<T>(p, sc.ldt.data(), Q, J, s1, s2, s3);  // solver.jl:419-421
   shock_sensor<T>(p, sc.phi.data(), Q);                        // solver.jl:424
   flow_advance<T>(p, Q, sc.Fp.data(), sc.Fm.data(), sc.Fx.data(), sc.Fy.data(), sc.Fz.data(),
                   sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), s1, s2, s3, m, J, sc.phi.data());
   div<T>(p, U, sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(),
-         sc.Fvz.data(), dt, J);                                // solver.jl:429
+         sc.Fvz.data(), dt, J, p->lts ? sc.ldt.data() : nullptr);  // solver.jl:426-430
   if (KRK == 2) lin_comb(p, U, Un, 5, 0.25, 0.75);             // solver.jl:433
   if (KRK == 3) lin_comb(p, U, Un, 5, 2.0 / 3.0, 1.0 / 3.0);   // solver.jl:435
   c2prim<T>(p, U, Q);                                          // solver.jl:438

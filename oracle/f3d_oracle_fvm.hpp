@@ -388,8 +388,8 @@ void flow_advance_fvm(const f3d_params* p, const double* U, const T* Q, T* Fx, T
   if (p->viscous) viscous_flux<T>(p, Fvx, Fvy, Fvz, Q, m, J);
 }
 
-// One RK3 stage with the finite-volume branch (solver.jl:416-444); fillGhost also refills U's
-// ghost cells, which the next stage's reconstruction reads.
+// One RK3 stage with the finite-volume branch (solver.jl:416-444), local time stepping as in
+// rk3_stage; fillGhost also refills U's ghost cells, which the next stage's reconstruction reads.
 template <typename T>
 void rk3_stage_fvm(const f3d_params* p, Scratch<T>& sc, int KRK, double* U, double* Un, T* Q, const T* s1,
                    const T* s2, const T* s3, const T* const m[9], const double* J, const double* x, const double* y,
@@ -397,10 +397,12 @@ void rk3_stage_fvm(const f3d_params* p, Scratch<T>& sc, int KRK, double* U, doub
   Grid g(p);
   size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
   if (KRK == 1) std::memcpy(Un, U, nc * 5 * sizeof(double));
+  if (KRK == 1 && p->lts) compute_dt<T>(p, sc.ldt.data(), Q, J, s1, s2, s3);
   shock_sensor<T>(p, sc.phi.data(), Q);
   flow_advance_fvm<T>(p, U, Q, sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(),
                       s1, s2, s3, m, J, x, y, z, sc.phi.data());
-  div<T>(p, U, sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), dt, J);
+  div<T>(p, U, sc.Fx.data(), sc.Fy.data(), sc.Fz.data(), sc.Fvx.data(), sc.Fvy.data(), sc.Fvz.data(), dt, J,
+         p->lts ? sc.ldt.data() : nullptr);
   if (KRK == 2) lin_comb(p, U, Un, 5, 0.25, 0.75);
   if (KRK == 3) lin_comb(p, U, Un, 5, 2.0 / 3.0, 1.0 / 3.0);
   c2prim<T>(p, U, Q);

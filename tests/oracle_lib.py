@@ -82,6 +82,14 @@ class Oracle:
     def advect(self, d, F, phi, S, Fp, Fm, Q, Ax, Ay, Az):
         self._c(f"oracle_advect_c_{suf(Q)}", c_int(d), P(F), P(phi), P(S), P(Fp), P(Fm), P(Q), P(Ax), P(Ay), P(Az))
 
+    def compute_dt(self, dt, Q, J, S1, S2, S3):
+        """compute_dt (utils.jl:73-100): per-cell local time step of the interior (dt double)."""
+        self._c(f"oracle_compute_dt_{suf(Q)}", P(dt), P(Q), P(J), P(S1), P(S2), P(S3))
+
+    def div_lts(self, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, dt, J):
+        """div_LTS (div.jl:52-101) with the per-cell dt array."""
+        self._c(f"oracle_div_lts_{suf(Fx)}", P(U), P(Fx), P(Fy), P(Fz), P(Fvx), P(Fvy), P(Fvz), P(dt), P(J))
+
     def viscous(self, Fvx, Fvy, Fvz, Q, m, J):
         self._c(f"oracle_viscous_flux_{suf(Q)}", P(Fvx), P(Fvy), P(Fvz), P(Q), PP(m), P(J))
 

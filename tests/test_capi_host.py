@@ -152,7 +152,8 @@ def test_alternative_flags_validation():
         L.f3d_params_init(ctypes.byref(q), 16, 16, 16)
         setattr(q, field, bad)
         assert L.f3d_params_check(ctypes.byref(q)) == -4, field
-    for kw in ({"split": f3d.SPLIT_LF}, {"split": f3d.SPLIT_AUSM}, {"character": 0}, {"les": f3d.LES_WALE}):
+    for kw in ({"split": f3d.SPLIT_LF}, {"split": f3d.SPLIT_AUSM}, {"character": 0}, {"les": f3d.LES_WALE},
+               {"lts": 1}):
         p = f3d.make_params(16, 12, 8, **kw)
         h = ctypes.c_void_p()
         assert L.f3d_adjoint_workspace_create(ctypes.byref(p), ctypes.byref(h)) == -4, kw
@@ -167,3 +168,11 @@ def test_alternative_flags_validation():
         for suf in ("f64", "f32b"):
             assert getattr(L, f"f3d_advect_{d}_{suf}")(ctypes.byref(p), None, None, None, None, None,
                                                        ctypes.c_int32(5), None) == -1
+    for suf in ("f64", "f32b"):
+        assert getattr(L, f"f3d_compute_dt_{suf}")(ctypes.byref(p), None, None, None, None, None, None, None) == -1
+        assert getattr(L, f"f3d_div_lts_{suf}")(ctypes.byref(p), None, None, None, None, None, None, None, None,
+                                                None, None) == -1
+    q = f3d.F3DParams()
+    L.f3d_params_init(ctypes.byref(q), 16, 16, 16)
+    q.lts, q.lts_cfl = 1, 0.0
+    assert L.f3d_params_check(ctypes.byref(q)) == -4 and b"lts_cfl" in L.f3d_last_error()
