@@ -229,12 +229,16 @@ ALTERNATIVES = [
     ("les_smagorinsky", dict(les=1)), ("les_wale", dict(les=2)),
     ("split_lax_friedrichs", dict(split=1)), ("split_van_leer", dict(split=2)), ("split_ausm", dict(split=3)),
     ("component_wise", dict(character=0)), ("local_time_stepping", dict(lts=1)),
+    # the filtering block after every step (the reference filters every filtering_interval = 100 steps)
+    ("filter_nonlinear_every_step", dict(filtering=1, filtering_nonlinear=1, filtering_interval=1)),
+    ("filter_linear_every_step", dict(filtering=1, filtering_nonlinear=0, filtering_interval=1, filtering_s0=0.05)),
 ]
 
 
 def alternatives_bench(f3d, torch, lib, st, U0, Q0, steps, warmup):
     """The flag-selected alternatives of the finite-difference step (LES models, other splittings,
-    component-wise reconstruction, local time stepping) on the same grid, initial state and dt, FP64:
+    component-wise reconstruction, local time stepping, the filters) on the same grid, initial state and
+    dt, FP64:
     ms/step, the
     Mcell-updates/s rate and the cost relative to the default step measured in the same way."""
     base = st["p"]

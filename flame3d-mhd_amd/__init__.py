@@ -3,7 +3,8 @@
 The reference (rygan-code/Flame3D-MHD, Julia + CUDA.jl) launches per-stage kernels from
 `time_step` (solver.jl:191-680).  This module exposes the same entry points with the same names
 and argument order -- `shockSensor`, `fluxSplit_SW/LF/VL/AUSM`, `advect_xc/yc/zc`, the
-component-wise `advect_x/y/z`, `viscousFlux`, `div`, `div_LTS`, `compute_dt`,
+component-wise `advect_x/y/z`, `viscousFlux`, `div`, `div_LTS`, `compute_dt`, the filters
+`pre_x/y/z`, `filter_x/y/z`, `linearFilter_x/y/z`,
 `linComb`, `c2Prim`, `prim2c`, `exchange_ghost`, `fillGhost`, `flowAdvance`, `time_step`, and the
 finite-volume branch's `Eigen_reconstruct_{i,j,k}` / `Conser_reconstruct_{i,j,k}` (FVM.jl) -- as
 methods of a `Flame3D` context that carries what the reference keeps in `const` globals
@@ -338,6 +339,50 @@ class Flame3D:
         """utils.jl:73-100 (dt: float64 array of the cell shape; interior cells written)"""
         self._call(f"f3d_compute_dt_{self._suffix(Q)}", *map(_ptr, (dt, Q, J, S1, S2, S3)))
 
+    def _pre(self, d, Q, sc, rth):
+        self._call(f"f3d_pre_{'xyz'[d]}_{self._suffix(Q)}", _ptr(sc), _ptr(Q), c_double(rth))
+
+    def pre_x(self, Q, sc, rth):
+        """utils.jl:102-124 (sc: float64 array of the cell shape)"""
+        self._pre(0, Q, sc, rth)
+
+    def pre_y(self, Q, sc, rth):
+        """utils.jl:126-147"""
+        self._pre(1, Q, sc, rth)
+
+    def pre_z(self, Q, sc, rth):
+        """utils.jl:149-169"""
+        self._pre(2, Q, sc, rth)
+
+    def filter_x(self, U, Un, sc, s0):
+        """utils.jl:171-193"""
+        self._call("f3d_filter_x", _ptr(U), _ptr(Un), _ptr(sc), c_double(s0))
+
+    def filter_y(self, U, Un, sc, s0):
+        """utils.jl:195-217"""
+        self._call("f3d_filter_y", _ptr(U), _ptr(Un), _ptr(sc), c_double(s0))
+
+    def filter_z(self, U, Un, sc, s0):
+        """utils.jl:219-241"""
+        self._call("f3d_filter_z", _ptr(U), _ptr(Un), _ptr(sc), c_double(s0))
+
+    def linearFilter_x(self, U, Un, s0):
+        """utils.jl:243-266"""
+        self._call("f3d_linear_filter_x", _ptr(U), _ptr(Un), c_double(s0))
+
+    def linearFilter_y(self, U, Un, s0):
+        """utils.jl:268-291"""
+        self._call("f3d_linear_filter_y", _ptr(U), _ptr(Un), c_double(s0))
+
+    def linearFilter_z(self, U, Un, s0):
+        """utils.jl:293-316"""
+        self._call("f3d_linear_filter_z", _ptr(U), _ptr(Un), c_double(s0))
+
+    def filter_step(self, U, Un, Q):
+        """The filtering block of the time loop (solver.jl:446-470) with the params' settings."""
+        prec = self._prec(Q)
+        self._call(f"f3d_filter_step_{self._suffix(Q)}", self.workspace(prec).handle, _ptr(U), _ptr(Un), _ptr(Q))
+
     def linComb(self, U, Un, NV, a, b):
         """utils.jl:57-70"""
         self._call("f3d_lin_comb", _ptr(U), _ptr(Un), c_int32(NV), c_double(a), c_double(b))
@@ -415,7 +460,9 @@ class Flame3D:
                   _ptr(J), c_double(dt), _ptr(inflow), self.stream))
 
     def time_step(self, nsteps, U, Un, Q, metrics, J, dt, inflow=None):
-        """`nsteps` SSP-RK3 steps (solver.jl:410-444 without I/O, filtering or LTS)."""
+        """`nsteps` SSP-RK3 steps (solver.jl:410-470 without I/O): local time stepping and filtering per
+        params (the filtering block after the steps of this call whose index is a multiple of
+        filtering_interval)."""
         prec = self._prec(Q)
         ws = self.workspace(prec)
         fn = getattr(lib(), f"f3d_time_step_{self._suffix(Q)}")

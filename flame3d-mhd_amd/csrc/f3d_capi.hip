@@ -399,6 +399,40 @@ DIV_IMPL(f32b, float)
 LTS_IMPL(f64, double)
 LTS_IMPL(f32b, float)
 
+// pre_x/_y/_z  utils.jl:102-169 (sc cell-shaped double), filter_x/_y/_z 171-241, linearFilter_x/_y/_z 243-316
+#define PRE_IMPL(X, D, SUF, T)                                                                                \
+  F3D_API int f3d_pre_##X##_##SUF(const f3d_params* p, double* sc, const T* Q, double rth, f3d_stream_t s) {  \
+    CHECK_P();                                                                                                \
+    NEED(sc);                                                                                                 \
+    NEED(Q);                                                                                                  \
+    return launch_pre_sensor<T>(make_kp(p), D, sc, Q, rth, (hipStream_t)s);                                   \
+  }
+PRE_IMPL(x, 0, f64, double)
+PRE_IMPL(y, 1, f64, double)
+PRE_IMPL(z, 2, f64, double)
+PRE_IMPL(x, 0, f32b, float)
+PRE_IMPL(y, 1, f32b, float)
+PRE_IMPL(z, 2, f32b, float)
+#define FILTER_IMPL(X, D)                                                                                     \
+  F3D_API int f3d_filter_##X(const f3d_params* p, double* U, const double* Un, const double* sc, double s0,   \
+                             f3d_stream_t s) {                                                                \
+    CHECK_P();                                                                                                \
+    NEED(U);                                                                                                  \
+    NEED(Un);                                                                                                 \
+    NEED(sc);                                                                                                 \
+    return launch_filter(make_kp(p), D, U, Un, sc, s0, (hipStream_t)s);                                       \
+  }                                                                                                           \
+  F3D_API int f3d_linear_filter_##X(const f3d_params* p, double* U, const double* Un, double s0,              \
+                                    f3d_stream_t s) {                                                         \
+    CHECK_P();                                                                                                \
+    NEED(U);                                                                                                  \
+    NEED(Un);                                                                                                 \
+    return launch_filter(make_kp(p), D, U, Un, nullptr, s0, (hipStream_t)s);                                  \
+  }
+FILTER_IMPL(x, 0)
+FILTER_IMPL(y, 1)
+FILTER_IMPL(z, 2)
+
 F3D_API int f3d_lin_comb(const f3d_params* p, double* U, const double* Un, int32_t NV, double a, double b,
                          f3d_stream_t s) {
   CHECK_P();

@@ -60,6 +60,15 @@ int launch_viscous(const KP&, T* Fvx, T* Fvy, T* Fvz, const T* Q, const MetricPt
 template <typename T>
 int launch_div(const KP&, double* U, const T* Fx, const T* Fy, const T* Fz, const T* Fvx, const T* Fvy,
                const T* Fvz, double dt, const double* J, hipStream_t, const double* ldt = nullptr);
+// Filters (utils.jl:102-316): the sensor along d (interior + one ghost layer each side along d), one
+// direction's filter of the interior (sc != nullptr: the nonlinear filter, else the linear one), and
+// the filtering block of the time loop (solver.jl:446-470: per direction Un <- U, sensor, filter).
+template <typename T>
+int launch_pre_sensor(const KP&, int d, double* sc, const T* Q, double rth, hipStream_t);
+int launch_filter(const KP&, int d, double* U, const double* Un, const double* sc, double s0, hipStream_t);
+template <typename T>
+int launch_filter_block(const KP&, double* U, double* Un, const T* Q, double* sc, int nonlinear, double rth,
+                        double s0, hipStream_t);
 // compute_dt (utils.jl:73-100) into the interior cells of dt; m != nullptr: the sizes from the metric
 // rows instead of S1..S3.
 template <typename T>

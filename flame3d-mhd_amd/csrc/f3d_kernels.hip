@@ -209,6 +209,61 @@ __global__ __launch_bounds__(256) void k_compute_dt(KP kp, Range3 r, double* __r
   dt[o] = jmin<double>(jmin<double>(dtx, dty), dtz) * kp.lts_cfl;
 }
 
+// ------------------------------------------------------------------ filters  utils.jl:102-316
+// pre_x/_y/_z (102-169): the nonlinear filter's sensor along D from the pressure, written on the
+// interior and the first ghost layer each side along D (the filter's boundary faces read those), in
+// double from the bulk-type pressure.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_pre_sensor(KP kp, Range3 r, double* __restrict__ sc,
+                                                    const T* __restrict__ Q, double rth) {
+  RANGE_IDX(r);
+  const long long st = dstride(kp, D), o = cidx(kp, i, j, k);
+  const T* pq = Q + 4 * kp.nc + o;
+  const double p1 = pq[-2 * st], p2 = pq[-st], p3 = pq[0], p4 = pq[st], p5 = pq[2 * st];
+  const double dp0 = 0.25 * (-p4 + 2 * p3 - p2);
+  const double dp1 = 0.25 * (-p5 + 2 * p4 - p3);
+  const double dp2 = 0.25 * (-p3 + 2 * p2 - p1);
+  const double ri = 0.5 * ((dp0 - dp1) * (dp0 - dp1) + (dp0 - dp2) * (dp0 - dp2)) / (p3 * p3) + 1e-16;
+  const double t = 1.0 - rth / ri;
+  sc[o] = 0.5 * (t + fabs(t));
+}
+
+// filter_x/_y/_z (171-241): U = Un - s0 (sc+ D+ - sc- D-) on the interior, 5 conserved variables.
+template <int D>
+__global__ __launch_bounds__(256) void k_filter_nl(KP kp, Range3 r, double* __restrict__ U,
+                                                   const double* __restrict__ Un, const double* __restrict__ sc,
+                                                   double s0) {
+  RANGE_IDX(r);
+  const long long st = dstride(kp, D), o = cidx(kp, i, j, k), nc = kp.nc;
+  const double c1 = -0.210383, c2 = 0.039617;
+  const double sc1 = 0.5 * (sc[o] + sc[o + st]);
+  const double sc2 = 0.5 * (sc[o] + sc[o - st]);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const double* v = Un + n * nc + o;
+    const double fp = c1 * (v[st] - v[0]) + c2 * (v[2 * st] - v[-st]);
+    const double fm = c1 * (v[0] - v[-st]) + c2 * (v[st] - v[-2 * st]);
+    U[o + n * nc] = v[0] - s0 * (sc1 * fp - sc2 * fm);
+  }
+}
+
+// linearFilter_x/_y/_z (243-316): 9-point linear filter of the interior.
+template <int D>
+__global__ __launch_bounds__(256) void k_filter_lin(KP kp, Range3 r, double* __restrict__ U,
+                                                    const double* __restrict__ Un, double s0) {
+  RANGE_IDX(r);
+  const long long st = dstride(kp, D), o = cidx(kp, i, j, k), nc = kp.nc;
+  const double d0 = 0.243527493120, d1 = -0.204788880640, d2 = 0.120007591680, d3 = -0.045211119360,
+               d4 = 0.008228661760;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const double* v = Un + n * nc + o;
+    const double f = d0 * v[0] + d1 * (v[-st] + v[st]) + d2 * (v[-2 * st] + v[2 * st]) +
+                     d3 * (v[-3 * st] + v[3 * st]) + d4 * (v[-4 * st] + v[4 * st]);
+    U[o + n * nc] = v[0] - s0 * f;
+  }
+}
+
 // ------------------------------------------------------------------ linComb, c2Prim, prim2c
 __global__ __launch_bounds__(256) void k_lin_comb(KP kp, Range3 r, double* __restrict__ U,
                                                   const double* __restrict__ Un, int NV, double a, double b) {
@@ -451,6 +506,55 @@ int launch_compute_dt(const KP& kp, double* dt, const T* Q, const double* J, con
   return check_launch("compute_dt");
 }
 
+static Range3 interior_range(const KP& kp) { return Range3{kp.NG, kp.NG, kp.NG, kp.Nxp, kp.Nyp, kp.Nzp}; }
+
+template <typename T>
+int launch_pre_sensor(const KP& kp, int d, double* sc, const T* Q, double rth, hipStream_t s) {
+  TimedScope ts_("filter_sensor", s);
+  Range3 r{kp.NG - (d == 0), kp.NG - (d == 1), kp.NG - (d == 2), kp.Nxp + 2 * (d == 0), kp.Nyp + 2 * (d == 1),
+           kp.Nzp + 2 * (d == 2)};
+  if (d == 0)
+    hipLaunchKernelGGL((k_pre_sensor<T, 0>), range_grid(r), kBlock, 0, s, kp, r, sc, Q, rth);
+  else if (d == 1)
+    hipLaunchKernelGGL((k_pre_sensor<T, 1>), range_grid(r), kBlock, 0, s, kp, r, sc, Q, rth);
+  else
+    hipLaunchKernelGGL((k_pre_sensor<T, 2>), range_grid(r), kBlock, 0, s, kp, r, sc, Q, rth);
+  return check_launch("filter sensor");
+}
+
+int launch_filter(const KP& kp, int d, double* U, const double* Un, const double* sc, double s0, hipStream_t s) {
+  TimedScope ts_("filter", s);
+  const Range3 r = interior_range(kp);
+  if (sc) {
+    if (d == 0)
+      hipLaunchKernelGGL((k_filter_nl<0>), range_grid(r), kBlock, 0, s, kp, r, U, Un, sc, s0);
+    else if (d == 1)
+      hipLaunchKernelGGL((k_filter_nl<1>), range_grid(r), kBlock, 0, s, kp, r, U, Un, sc, s0);
+    else
+      hipLaunchKernelGGL((k_filter_nl<2>), range_grid(r), kBlock, 0, s, kp, r, U, Un, sc, s0);
+  } else {
+    if (d == 0)
+      hipLaunchKernelGGL((k_filter_lin<0>), range_grid(r), kBlock, 0, s, kp, r, U, Un, s0);
+    else if (d == 1)
+      hipLaunchKernelGGL((k_filter_lin<1>), range_grid(r), kBlock, 0, s, kp, r, U, Un, s0);
+    else
+      hipLaunchKernelGGL((k_filter_lin<2>), range_grid(r), kBlock, 0, s, kp, r, U, Un, s0);
+  }
+  return check_launch("filter");
+}
+
+template <typename T>
+int launch_filter_block(const KP& kp, double* U, double* Un, const T* Q, double* sc, int nonlinear, double rth,
+                        double s0, hipStream_t s) {
+  for (int d = 0; d < 3; ++d) {
+    F3D_HIP(hipMemcpyAsync(Un, U, sizeof(double) * 5 * kp.nc, hipMemcpyDeviceToDevice, s));  // solver.jl:447
+    int rc = nonlinear ? launch_pre_sensor<T>(kp, d, sc, Q, rth, s) : F3D_OK;
+    if (!rc) rc = launch_filter(kp, d, U, Un, nonlinear ? sc : nullptr, s0, s);
+    if (rc) return rc;
+  }
+  return F3D_OK;
+}
+
 int launch_lin_comb(const KP& kp, double* U, const double* Un, int NV, double a, double b, hipStream_t s) {
   TimedScope ts_("lin_comb", s);
   Range3 r{kp.NG, kp.NG, kp.NG, kp.Nxp, kp.Nyp, kp.Nzp};
@@ -514,6 +618,9 @@ int launch_fill(const KP& kp, T* Q, double* U, const T* inflow, hipStream_t s) {
                                  hipStream_t);                                                                  \
   template int launch_div<T>(const KP&, double*, const T*, const T*, const T*, const T*, const T*, const T*,    \
                              double, const double*, hipStream_t, const double*);                                \
+  template int launch_pre_sensor<T>(const KP&, int, double*, const T*, double, hipStream_t);                     \
+  template int launch_filter_block<T>(const KP&, double*, double*, const T*, double*, int, double, double,       \
+                                      hipStream_t);                                                             \
   template int launch_compute_dt<T>(const KP&, double*, const T*, const double*, const T*, const T*, const T*,  \
                                     const MetricPtrs<T>*, hipStream_t);                                         \
   template int launch_c2prim<T>(const KP&, const double*, T*, hipStream_t);                                     \

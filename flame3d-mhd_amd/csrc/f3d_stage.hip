@@ -24,6 +24,7 @@ struct f3d_workspace {
   unsigned long long hcap;
   void *Qa, *Qb;  // stage outputs of the fused path (primitive state, 6 vars, bulk type)
   double* ldt;    // per-cell dt of local time stepping (cell-shaped; compute_dt at stage 1)
+  double* fsc;    // sensor of the nonlinear filter (cell-shaped)
 };
 
 namespace {
@@ -268,8 +269,9 @@ int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double*
     if (!rc) rc = launch_c2prim<T>(kp, U, Qout, s);
   }
   if (!rc) rc = launch_exchange<T>(kp, Qout, 6, s);
-  // U ghosts are read only by the finite-volume reconstruction; fillGhost refills them from Q's there
-  if (!rc) rc = launch_fill<T>(kp, Qout, geo ? U : nullptr, inflow, s);
+  // U ghosts are read only by the finite-volume reconstruction and the filters; fillGhost refills them
+  // from Q's there (boundary.jl fills them always)
+  if (!rc) rc = launch_fill<T>(kp, Qout, (geo || p->filtering) ? U : nullptr, inflow, s);
   return rc;
 }
 
@@ -278,21 +280,18 @@ int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, doub
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
              const T* const* geo) {
   const KP kp = make_kp(p);
-  if (!stage_needs_separate_q(kp)) {
-    for (int t = 0; t < nsteps; ++t)
-      for (int KRK = 1; KRK <= 3; ++KRK) {
-        const int rc = stage_io<T>(p, ws, KRK, U, Un, Q, Q, metrics, J, dt, inflow, s, geo);
-        if (rc) return rc;
-      }
-    return F3D_OK;
-  }
-  // stage outputs rotate Q -> Qa -> Qb -> Q: a whole step needs no copy
-  T* Qa = (T*)ws->Qa;
-  T* Qb = (T*)ws->Qb;
-  for (int t = 0; t < nsteps; ++t) {
+  // stage outputs rotate Q -> Qa -> Qb -> Q when the stage cannot write its input: no copy per step
+  const bool rot = stage_needs_separate_q(kp);
+  T* Qa = rot ? (T*)ws->Qa : Q;
+  T* Qb = rot ? (T*)ws->Qb : Q;
+  for (int t = 1; t <= nsteps; ++t) {
     int rc = stage_io<T>(p, ws, 1, U, Un, Q, Qa, metrics, J, dt, inflow, s, geo);
     if (!rc) rc = stage_io<T>(p, ws, 2, U, Un, Qa, Qb, metrics, J, dt, inflow, s, geo);
     if (!rc) rc = stage_io<T>(p, ws, 3, U, Un, Qb, Q, metrics, J, dt, inflow, s, geo);
+    // the filtering block after the steps whose index (counted from this call's first) is a multiple of
+    // filtering_interval (solver.jl:446-470); Q stays as the last stage made it
+    if (!rc && p->filtering && t % p->filtering_interval == 0)
+      rc = launch_filter_block<T>(kp, U, Un, Q, ws->fsc, p->filtering_nonlinear, p->filtering_rth, p->filtering_s0, s);
     if (rc) return rc;
   }
   return F3D_OK;
@@ -388,7 +387,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
   const size_t nc = f3d_cell_count(p);
   const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
-  size_t sz[17];
+  size_t sz[18];
   sz[0] = nc * es;                               // phi
   sz[1] = nc * 5 * es;                           // Fp
   sz[2] = nc * 5 * es;                           // Fm
@@ -405,6 +404,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[13] = 256;                                  // heavy-face counter
   sz[14] = sz[15] = nc * 6 * es;                 // Qa, Qb
   sz[16] = nc * sizeof(double);                  // ldt
+  sz[17] = nc * sizeof(double);                  // fsc
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -415,10 +415,10 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   ws->bytes = total;
   ws->base = base;
   char* c = (char*)base;
-  void** slots[17] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+  void** slots[18] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
                       &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount,
-                      &ws->Qa, &ws->Qb, (void**)&ws->ldt};
-  for (int i = 0; i < 17; ++i) {
+                      &ws->Qa, &ws->Qb, (void**)&ws->ldt, (void**)&ws->fsc};
+  for (int i = 0; i < 18; ++i) {
     *slots[i] = c;
     c += align_up(sz[i]);
   }
@@ -611,6 +611,33 @@ F3D_API int f3d_cfl_dt_f64(const f3d_params* p, f3d_workspace* ws, double* dt_ou
 F3D_API int f3d_cfl_dt_f32b(const f3d_params* p, f3d_workspace* ws, double* dt_out, const float* Q,
                             const float* const* metrics, const double* J, double cfl, f3d_stream_t s) {
   return cfl_impl<float>(p, ws, F3D_PREC_F32B, dt_out, Q, metrics, J, cfl, (hipStream_t)s, "cfl_dt_f32b");
+}
+
+// The filtering block of the time loop (solver.jl:446-470) with the params' filter settings.
+template <typename T>
+static int filter_step_impl(const f3d_params* p, f3d_workspace* ws, int prec, double* U, double* Un, const T* Q,
+                            hipStream_t s, const char* who) {
+  int rc = f3d_params_check(p);
+  if (!rc) rc = check_ws(p, ws, prec, who);
+  if (rc) return rc;
+  if (!U || !Un || !Q) {
+    set_error("%s: NULL array argument", who);
+    return F3D_E_NULL;
+  }
+  if (p->filtering_interval < 1 || !(p->filtering_s0 >= 0)) {
+    set_error("%s: needs filtering_interval >= 1 and filtering_s0 >= 0", who);
+    return F3D_E_ARG;
+  }
+  return launch_filter_block<T>(make_kp(p), U, Un, Q, ws->fsc, p->filtering_nonlinear, p->filtering_rth,
+                                p->filtering_s0, s);
+}
+F3D_API int f3d_filter_step_f64(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, const double* Q,
+                                f3d_stream_t s) {
+  return filter_step_impl<double>(p, ws, F3D_PREC_F64, U, Un, Q, (hipStream_t)s, "filter_step_f64");
+}
+F3D_API int f3d_filter_step_f32b(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, const float* Q,
+                                 f3d_stream_t s) {
+  return filter_step_impl<float>(p, ws, F3D_PREC_F32B, U, Un, Q, (hipStream_t)s, "filter_step_f32b");
 }
 
 F3D_API int f3d_nan_check(const f3d_params* p, f3d_workspace* ws, int32_t* flag_out, const double* U, int32_t NV,

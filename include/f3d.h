@@ -238,6 +238,31 @@ int f3d_compute_dt_f64(const f3d_params* p, double* dt, const double* Q, const d
 int f3d_compute_dt_f32b(const f3d_params* p, double* dt, const float* Q, const double* J,
                         const float* S1, const float* S2, const float* S3, f3d_stream_t s);
 
+/* Filters  utils.jl:102-316 (U, Un double in both forms).
+ * pre_x/_y/_z(Q, sc, rth)  102-169 : the nonlinear filter's shock sensor along x/y/z from the pressure,
+ *   sc = max(0, 1 - rth/r), r = ((dp0-dp1)^2 + (dp0-dp2)^2)/(2 p^2) + 1e-16 of the second differences
+ *   dp of p at offsets -2..2.  sc is cell-shaped double and written on the interior and on the first
+ *   ghost layer each side along the direction (the reference's sc is (Nxp, Nyp, Nzp) Float32 indexed
+ *   with the ghost-offset interior range, so its boundary neighbours alias other cells or lie outside
+ *   the allocation; here the boundary faces read the ghost cell's own sensor).
+ * filter_x/_y/_z(U, Un, sc, s0)  171-241 : U = Un - s0 (sc+ D+ - sc- D-) on the interior (conservative).
+ * linearFilter_x/_y/_z(U, Un, s0)  243-316 : 9-point linear filter of the interior. */
+int f3d_pre_x_f64(const f3d_params* p, double* sc, const double* Q, double rth, f3d_stream_t s);
+int f3d_pre_y_f64(const f3d_params* p, double* sc, const double* Q, double rth, f3d_stream_t s);
+int f3d_pre_z_f64(const f3d_params* p, double* sc, const double* Q, double rth, f3d_stream_t s);
+int f3d_pre_x_f32b(const f3d_params* p, double* sc, const float* Q, double rth, f3d_stream_t s);
+int f3d_pre_y_f32b(const f3d_params* p, double* sc, const float* Q, double rth, f3d_stream_t s);
+int f3d_pre_z_f32b(const f3d_params* p, double* sc, const float* Q, double rth, f3d_stream_t s);
+int f3d_filter_x(const f3d_params* p, double* U, const double* Un, const double* sc, double s0,
+                 f3d_stream_t s);
+int f3d_filter_y(const f3d_params* p, double* U, const double* Un, const double* sc, double s0,
+                 f3d_stream_t s);
+int f3d_filter_z(const f3d_params* p, double* U, const double* Un, const double* sc, double s0,
+                 f3d_stream_t s);
+int f3d_linear_filter_x(const f3d_params* p, double* U, const double* Un, double s0, f3d_stream_t s);
+int f3d_linear_filter_y(const f3d_params* p, double* U, const double* Un, double s0, f3d_stream_t s);
+int f3d_linear_filter_z(const f3d_params* p, double* U, const double* Un, double s0, f3d_stream_t s);
+
 /* linComb(U, Un, NV, a, b)  utils.jl:57-70 : U = a*U + b*Un on the interior (both forms: double) */
 int f3d_lin_comb(const f3d_params* p, double* U, const double* Un, int32_t NV, double a,
                  double b, f3d_stream_t s);
@@ -368,13 +393,25 @@ int f3d_rk3_stage_f32b(const f3d_params* p, f3d_workspace* ws, int32_t KRK, doub
                        double dt, const float* inflow, f3d_stream_t s);
 
 /* `nsteps` full RK3 steps with a fixed dt (time_step's loop, solver.jl:410-444, without I/O);
- * p->lts: a per-cell dt recomputed at the start of every step, as above. */
+ * p->lts: a per-cell dt recomputed at the start of every step, as above.  p->filtering: after the t-th
+ * step of the call (t = 1..nsteps) with t % filtering_interval == 0, the filtering block
+ * (f3d_filter_step_*); a caller that steps one call at a time filters with f3d_filter_step_* when its
+ * own step counter reaches a multiple of the interval. */
 int f3d_time_step_f64(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
                       double* Un, double* Q, const double* const* metrics, const double* J,
                       double dt, const double* inflow, f3d_stream_t s);
 int f3d_time_step_f32b(const f3d_params* p, f3d_workspace* ws, int32_t nsteps, double* U,
                        double* Un, float* Q, const float* const* metrics, const double* J,
                        double dt, const float* inflow, f3d_stream_t s);
+
+/* The filtering block of the time loop  solver.jl:446-470 : for x, y, z in turn, Un <- U, then the
+ * nonlinear filter with its sensor from Q (p->filtering_nonlinear, threshold filtering_rth) or the
+ * linear filter, strength filtering_s0.  Q is not updated (as in the reference).  U's ghost cells
+ * must hold the boundary procedure's values: the stage entries refill them when p->filtering. */
+int f3d_filter_step_f64(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, const double* Q,
+                        f3d_stream_t s);
+int f3d_filter_step_f32b(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, const float* Q,
+                         f3d_stream_t s);
 
 /* Face geometry of the finite-volume branch, once per mesh (as f3d_metrics_f64 for the metrics):
  * geo[d] (d = 0, 1, 2) = [face extents of Fx / Fy / Fz, 4] holding (nx, ny, nz, Area) of every

@@ -168,9 +168,12 @@ void time_step_ref(const f3d_params* p, int nsteps, double* U, double* Un, T* Q,
   M9<T> m(mv);
   std::vector<T> s1(nc), s2(nc), s3(nc);
   metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
-  for (int t = 0; t < nsteps; ++t)
+  for (int t = 1; t <= nsteps; ++t) {
     for (int KRK = 1; KRK <= 3; ++KRK)
       rk3_stage<T>(p, sc, KRK, U, Un, Q, s1.data(), s2.data(), s3.data(), m.m, J, dt, inflow);
+    // solver.jl:446: filtering after the steps whose index is a multiple of filtering_interval
+    if (p->filtering && t % p->filtering_interval == 0) filter_block<T>(p, U, Un, Q, sc.fsc.data());
+  }
 }
 }  // namespace
 
@@ -193,6 +196,28 @@ OR_API void oracle_div_lts_f32b(const f3d_params* p, double* U, const float* Fx,
                                 const float* Fvx, const float* Fvy, const float* Fvz, const double* dt,
                                 const double* J) {
   div<float>(p, U, Fx, Fy, Fz, Fvx, Fvy, Fvz, 0.0, J, dt);
+}
+
+// Filters (utils.jl:102-316): the sensor along d, the nonlinear and linear filters along d, and the
+// whole filtering block of the time loop (solver.jl:446-470).
+OR_API void oracle_pre_sensor_f64(const f3d_params* p, int d, double* sc, const double* Q, double rth) {
+  pre_sensor<double>(p, d, sc, Q, rth);
+}
+OR_API void oracle_pre_sensor_f32b(const f3d_params* p, int d, double* sc, const float* Q, double rth) {
+  pre_sensor<float>(p, d, sc, Q, rth);
+}
+OR_API void oracle_filter_nonlinear(const f3d_params* p, int d, double* U, const double* Un, const double* sc,
+                                    double s0) {
+  filter_nonlinear(p, d, U, Un, sc, s0);
+}
+OR_API void oracle_filter_linear(const f3d_params* p, int d, double* U, const double* Un, double s0) {
+  filter_linear(p, d, U, Un, s0);
+}
+OR_API void oracle_filter_block_f64(const f3d_params* p, double* U, double* Un, const double* Q, double* sc) {
+  filter_block<double>(p, U, Un, Q, sc);
+}
+OR_API void oracle_filter_block_f32b(const f3d_params* p, double* U, double* Un, const float* Q, double* sc) {
+  filter_block<float>(p, U, Un, Q, sc);
 }
 
 OR_API void oracle_rhs_f64(const f3d_params* p, double* R, const double* Q, const void* const* m, const double* J,
@@ -599,9 +624,11 @@ void time_step_fvm_ref(const f3d_params* p, int nsteps, double* U, double* Un, T
   Coords c(cv);
   std::vector<T> s1(nc), s2(nc), s3(nc);
   metric_norms<T>(p, m.m, s1.data(), s2.data(), s3.data());
-  for (int t = 0; t < nsteps; ++t)
+  for (int t = 1; t <= nsteps; ++t) {
     for (int KRK = 1; KRK <= 3; ++KRK)
       rk3_stage_fvm<T>(p, sc, KRK, U, Un, Q, s1.data(), s2.data(), s3.data(), m.m, J, c.x, c.y, c.z, dt, inflow);
+    if (p->filtering && t % p->filtering_interval == 0) filter_block<T>(p, U, Un, Q, sc.fsc.data());
+  }
 }
 
 // Finite-volume RHS in long double from double inputs, the scheme mask taken from the double sensor

@@ -880,6 +880,98 @@ void div(const f3d_params* p, double* U, const T* Fx, const T* Fy, const T* Fz, 
       }
 }
 
+// ---------------------------------------------------------------- filters  utils.jl:102-316
+// pre_x/_y/_z  utils.jl:102-169: shock sensor of the nonlinear filter along d from the pressure
+// (Q index 5, 1-based) at d-offsets -2..2:
+//   dp0 = (-p4 + 2p3 - p2)/4, dp1 = (-p5 + 2p4 - p3)/4, dp2 = (-p3 + 2p2 - p1)/4,
+//   r = ((dp0 - dp1)^2 + (dp0 - dp2)^2)/2 / p3^2 + 1e-16,  sc = (1 - rth/r + |1 - rth/r|)/2.
+// sc is cell-shaped here and written on the interior and on the first ghost layer of each side along
+// d, which the filter's boundary faces read (the reference allocates sc as (Nxp, Nyp, Nzp) but indexes
+// it with the interior's ghost-offset range, so its boundary neighbours alias other cells or lie
+// outside the allocation).  Double arithmetic from the bulk-type pressure.
+template <typename T>
+void pre_sensor(const f3d_params* p, int d, double* sc, const T* Q, double rth) {
+  Grid g(p);
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  const Dir D = make_dir(d);
+  const long lo[3] = {g.NG + 1 - D.di, g.NG + 1 - D.dj, g.NG + 1 - D.dk};
+  const long hi[3] = {g.Nxp + g.NG + D.di, g.Nyp + g.NG + D.dj, g.Nzp + g.NG + D.dk};
+  for (long k = lo[2]; k <= hi[2]; ++k)
+    for (long j = lo[1]; j <= hi[1]; ++j)
+      for (long i = lo[0]; i <= hi[0]; ++i) {
+        auto P = [&](long s) { return (double)Q[g.cell.at(i + s * D.di, j + s * D.dj, k + s * D.dk) + 4 * nc]; };
+        const double p1 = P(-2), p2 = P(-1), p3 = P(0), p4 = P(1), p5 = P(2);
+        const double dp0 = 0.25 * (-p4 + 2 * p3 - p2);
+        const double dp1 = 0.25 * (-p5 + 2 * p4 - p3);
+        const double dp2 = 0.25 * (-p3 + 2 * p2 - p1);
+        const double ri = 0.5 * ((dp0 - dp1) * (dp0 - dp1) + (dp0 - dp2) * (dp0 - dp2)) / (p3 * p3) + 1e-16;
+        sc[g.cell.at(i, j, k)] = 0.5 * (1.0 - rth / ri + std::fabs(1.0 - rth / ri));
+      }
+}
+
+// filter_x/_y/_z  utils.jl:171-241: conservative nonlinear (shock-capturing) filter of the interior,
+//   U = Un - s0 (sc+ D+ - sc- D-),  sc+- = (sc_i + sc_(i+-1))/2,
+//   D+ = c1 (Un_(i+1) - Un_i) + c2 (Un_(i+2) - Un_(i-1)),  D- = c1 (Un_i - Un_(i-1)) + c2 (Un_(i+1) - Un_(i-2)),
+// c1 = -0.210383, c2 = 0.039617, for the 5 conserved variables.
+inline void filter_nonlinear(const f3d_params* p, int d, double* U, const double* Un, const double* sc, double s0) {
+  Grid g(p);
+  const Dir D = make_dir(d);
+  const double c1 = -0.210383, c2 = 0.039617;
+  for (long k = g.NG + 1; k <= g.Nzp + g.NG; ++k)
+    for (long j = g.NG + 1; j <= g.Nyp + g.NG; ++j)
+      for (long i = g.NG + 1; i <= g.Nxp + g.NG; ++i) {
+        auto C = [&](long s) { return g.cell.at(i + s * D.di, j + s * D.dj, k + s * D.dk); };
+        const double sc1 = 0.5 * (sc[C(0)] + sc[C(1)]);
+        const double sc2 = 0.5 * (sc[C(0)] + sc[C(-1)]);
+        for (long n = 1; n <= 5; ++n) {
+          const size_t nn = (size_t)(n - 1) * g.cell.n1 * g.cell.n2 * g.cell.n3;
+          auto V = [&](long s) { return Un[C(s) + nn]; };
+          U[C(0) + nn] = V(0) - s0 * (sc1 * (c1 * (V(1) - V(0)) + c2 * (V(2) - V(-1))) -
+                                      sc2 * (c1 * (V(0) - V(-1)) + c2 * (V(1) - V(-2))));
+        }
+      }
+}
+
+// linearFilter_x/_y/_z  utils.jl:243-316: 9-point linear filter of the interior,
+//   U = Un - s0 (d0 Un_i + sum_(m=1..4) d_m (Un_(i-m) + Un_(i+m))),
+// d = (0.24352749312, -0.20478888064, 0.12000759168, -0.04521111936, 0.00822866176).
+inline void filter_linear(const f3d_params* p, int d, double* U, const double* Un, double s0) {
+  Grid g(p);
+  const Dir D = make_dir(d);
+  const double dc[5] = {0.243527493120, -0.204788880640, 0.120007591680, -0.045211119360, 0.008228661760};
+  for (long k = g.NG + 1; k <= g.Nzp + g.NG; ++k)
+    for (long j = g.NG + 1; j <= g.Nyp + g.NG; ++j)
+      for (long i = g.NG + 1; i <= g.Nxp + g.NG; ++i) {
+        auto C = [&](long s) { return g.cell.at(i + s * D.di, j + s * D.dj, k + s * D.dk); };
+        for (long n = 1; n <= 5; ++n) {
+          const size_t nn = (size_t)(n - 1) * g.cell.n1 * g.cell.n2 * g.cell.n3;
+          auto V = [&](long s) { return Un[C(s) + nn]; };
+          const double f = dc[0] * V(0) + dc[1] * (V(-1) + V(1)) + dc[2] * (V(-2) + V(2)) +
+                           dc[3] * (V(-3) + V(3)) + dc[4] * (V(-4) + V(4));
+          U[C(0) + nn] = V(0) - s0 * f;
+        }
+      }
+}
+
+// The filtering block of the time loop  solver.jl:446-470: per direction x, y, z, Un <- U, then the
+// nonlinear filter with its sensor from Q (filtering_nonlinear) or the linear filter.  Q is not
+// updated afterwards (as in the reference: the next step starts from the filtered U and the
+// unfiltered Q).
+template <typename T>
+void filter_block(const f3d_params* p, double* U, double* Un, const T* Q, double* sc) {
+  Grid g(p);
+  const size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
+  for (int d = 0; d < 3; ++d) {
+    std::memcpy(Un, U, nc * 5 * sizeof(double));
+    if (p->filtering_nonlinear) {
+      pre_sensor<T>(p, d, sc, Q, p->filtering_rth);
+      filter_nonlinear(p, d, U, Un, sc, p->filtering_s0);
+    } else {
+      filter_linear(p, d, U, Un, p->filtering_s0);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- mpi.jl / boundary.jl
 // exchange_ghost with a single rank  mpi.jl:2-103 (+ pack/unpack 105-283): for each periodic
 // dimension in the order x, y, z: low ghosts <- Q[Np+g], high ghosts <- Q[NG+g], over the full
@@ -1008,12 +1100,13 @@ void flow_advance(const f3d_params* p, T* Q, T* Fp, T* Fm, T* Fx, T* Fy, T* Fz, 
 template <typename T>
 struct Scratch {
   std::vector<T> phi, Fp, Fm, Fx, Fy, Fz, Fvx, Fvy, Fvz;
-  std::vector<double> ldt;
+  std::vector<double> ldt, fsc;  // LTS dt; the nonlinear filter's sensor (solver.jl:314-316)
   explicit Scratch(const f3d_params* p) {
     Grid g(p);
     size_t nc = (size_t)g.Nxt * g.Nyt * g.Nzt;
     phi.assign(nc, 0);
     if (p->lts) ldt.assign(nc, 0.0);
+    if (p->filtering) fsc.assign(nc, 0.0);
     Fp.assign(nc * 5, 0);
     Fm.assign(nc * 5, 0);
     Fx.assign((size_t)(g.Nxp + 1) * g.Nyp * g.Nzp * 5, 0);

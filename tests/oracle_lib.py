@@ -90,6 +90,22 @@ class Oracle:
         """div_LTS (div.jl:52-101) with the per-cell dt array."""
         self._c(f"oracle_div_lts_{suf(Fx)}", P(U), P(Fx), P(Fy), P(Fz), P(Fvx), P(Fvy), P(Fvz), P(dt), P(J))
 
+    def pre_sensor(self, d, sc, Q, rth):
+        """pre_x/_y/_z (utils.jl:102-169) into the cell-shaped double sc."""
+        self._c(f"oracle_pre_sensor_{suf(Q)}", c_int(d), P(sc), P(Q), c_double(rth))
+
+    def filter_nonlinear(self, d, U, Un, sc, s0):
+        """filter_x/_y/_z (utils.jl:171-241)."""
+        self._c("oracle_filter_nonlinear", c_int(d), P(U), P(Un), P(sc), c_double(s0))
+
+    def filter_linear(self, d, U, Un, s0):
+        """linearFilter_x/_y/_z (utils.jl:243-316)."""
+        self._c("oracle_filter_linear", c_int(d), P(U), P(Un), c_double(s0))
+
+    def filter_block(self, U, Un, Q, sc):
+        """The filtering block of the time loop (solver.jl:446-470) with the params' filter settings."""
+        self._c(f"oracle_filter_block_{suf(Q)}", P(U), P(Un), P(Q), P(sc))
+
     def viscous(self, Fvx, Fvy, Fvz, Q, m, J):
         self._c(f"oracle_viscous_flux_{suf(Q)}", P(Fvx), P(Fvy), P(Fvz), P(Q), PP(m), P(J))
 
