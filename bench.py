@@ -267,6 +267,43 @@ def alternatives_bench(f3d, torch, lib, st, U0, Q0, steps, warmup):
     return out
 
 
+def io_bench(st, Q0):
+    """The reference's on-disk formats on the bench grid (host side, libf3d_h5.so): a chk-*.h5 checkpoint
+    of Q as IO.jl:152-179 writes it (Float32, shuffle + deflate level 1, the reference's default) and
+    uncompressed, and metrics.h5 (Float32, deflate level 3 as parse_mesh.jl), written and read back."""
+    import tempfile
+    import numpy as np
+    h5 = importlib.import_module("flame3d-mhd_amd.h5io")
+    p = st["p"]
+    Q = Q0.cpu().numpy()
+    m = [a.cpu().numpy() for a in st["m"]]
+    J = st["J"].cpu().numpy()
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name, compress in (("checkpoint_deflate1", 1), ("checkpoint_raw", 0)):
+            path = os.path.join(d, f"chk-{compress}.h5")
+            t0 = time.perf_counter()
+            h5.write_checkpoint(path, p, Q, compress=compress)
+            t1 = time.perf_counter()
+            Qr = h5.read_checkpoint(path, p, dtype=np.float64)
+            t2 = time.perf_counter()
+            mb = Q.size * 4 / 1e6  # Float32 payload
+            out[name] = dict(payload_MB=round(mb, 1), file_MB=round(os.path.getsize(path) / 1e6, 1),
+                             write_MB_s=round(mb / (t1 - t0), 1), read_MB_s=round(mb / (t2 - t1), 1),
+                             exact_float32_round_trip=bool(np.array_equal(Qr, Q.astype(np.float32).astype(np.float64))))
+        path = os.path.join(d, "metrics.h5")
+        t0 = time.perf_counter()
+        h5.write_metrics(path, p, m, J, compress=3)
+        t1 = time.perf_counter()
+        h5.read_metrics(path, p)
+        t2 = time.perf_counter()
+        mb = 10 * J.size * 4 / 1e6
+        out["metrics_deflate3"] = dict(payload_MB=round(mb, 1), file_MB=round(os.path.getsize(path) / 1e6, 1),
+                                       write_MB_s=round(mb / (t1 - t0), 1), read_MB_s=round(mb / (t2 - t1), 1))
+    progress("io: " + ", ".join(f"{k} w {v['write_MB_s']} r {v['read_MB_s']} MB/s" for k, v in out.items()))
+    return out
+
+
 def cpu_baseline(f3d, cases, nx, ny, nz, steps=3):
     """CPU oracle (C++ FP64 restatement, OpenMP) on the same grid for `steps` RK3 steps."""
     import numpy as np
@@ -430,6 +467,7 @@ def main():
     ap.add_argument("--no-fvm", action="store_true", help="skip the finite-volume leg")
     ap.add_argument("--fvm-only", action="store_true", help="only the finite-volume leg (profiling)")
     ap.add_argument("--no-alternatives", action="store_true", help="skip the flag-selected alternatives leg")
+    ap.add_argument("--no-io", action="store_true", help="skip the on-disk formats leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -525,6 +563,12 @@ def main():
     alt = None
     if not args.no_alternatives:
         alt = alternatives_bench(f3d, torch, lib, st, U0, Q0, max(3, args.steps // 4), 2)
+    io = None
+    if not args.no_io:
+        try:
+            io = io_bench(st, Q0)
+        except ImportError as e:  # libf3d_h5.so not built (no HDF5 headers where it was built)
+            io = {"skipped": str(e)}
     adj = None
     if not args.no_adjoint:
         ax, ay, az = map(int, args.adjoint_grid.split("x"))
@@ -557,6 +601,7 @@ def main():
         "fp32_bulk": out32 or None,
         "finite_volume": fvm,
         "alternatives": alt,
+        "io": io,
         "adjoint": adj,
         "kernels": kshare,
         "host_wall_ms_per_step": round(wall64, 4),

@@ -53,5 +53,30 @@ def build(verbose=False, jobs=4):
     return LIB
 
 
+H5_LIB = os.path.join(HERE, "libf3d_h5.so")
+H5_PREFIX = os.environ.get("F3D_HDF5_PREFIX", "/opt/conda")
+
+
+def build_h5(verbose=False):
+    """libf3d_h5.so: the on-disk formats (HDF5 C library, host code, g++).  Returns None when the HDF5
+    headers are absent (the GPU library does not depend on it)."""
+    inc, libdir = os.path.join(H5_PREFIX, "include"), os.path.join(H5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        sys.stderr.write(f"[build] no hdf5.h under {inc}: libf3d_h5.so not built\n")
+        return None
+    tmp = H5_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall", "-Wextra",
+           f"-I{inc}", f"-I{os.path.join(os.path.dirname(HERE), 'include')}", os.path.join(CSRC, "f3d_h5io.cpp"),
+           "-o", tmp, f"-L{libdir}", "-lhdf5", f"-Wl,-rpath,{libdir}", "-static-libstdc++", "-static-libgcc"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if verbose or r.returncode:
+        sys.stderr.write(f"[build] {' '.join(cmd)}\n{r.stdout}{r.stderr}")
+    if r.returncode:
+        raise RuntimeError("build of libf3d_h5.so failed")
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_h5(verbose="-v" in sys.argv))

@@ -49,6 +49,7 @@ typedef struct ihipStream_t* f3d_stream_t; /* == hipStream_t */
 #define F3D_E_ARG (-4)      /* other invalid argument (BC kind, NV, KRK, precision ...) */
 #define F3D_E_NAN (-5)      /* NaN/Inf detected by f3d_nan_check_* (solver.jl:481-486) */
 #define F3D_E_WS (-6)       /* workspace does not match params / precision */
+#define F3D_E_IO (-7)       /* file I/O failed (f3d_h5.h) */
 
 /* boundary-condition kinds, one per domain face (xlo,xhi,ylo,yhi,zlo,zhi) */
 #define F3D_BC_PERIODIC 0      /* single-rank exchange_ghost semantics (mpi.jl:2-103) + fill_z (boundary.jl:123-148) */
