@@ -179,6 +179,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
 #pragma unroll
   for (int s = 0; s < 6; ++s) ring[s] = 0;
   T Fprev[5] = {0, 0, 0, 0, 0};
+  T Rn[5] = {0, 0, 0, 0, 0};  // RHS of the row the next step closes, loaded a step ahead
   // prefetched inputs of the next cell
   T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph;
   {
@@ -195,6 +196,14 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
       nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
       nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
       na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
+    }
+    T Rc[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Rc[n] = Rn[n];
+    if (step >= 7 && step + 1 < nstep && act) {  // row f0 + step - 7, closed at step + 1
+      const long long ro = D == 1 ? ridx(kp, ii, f0 + step - 7, tt) : ridx(kp, ii, tt, f0 + step - 7);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) Rn[n] = a.R[ro + n * rn];
     }
     T fp[5], fm[5];
     split_cell<T, SPLIT>(kp, q0, q1, q2, q3, q5, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp, fm);
@@ -218,7 +227,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
       if (fi >= 1 && act) {
         const long long ro = D == 1 ? ridx(kp, ii, f - 1, tt) : ridx(kp, ii, tt, f - 1);
 #pragma unroll
-        for (int n = 0; n < 5; ++n) a.R[ro + n * rn] += Fprev[n] - acc[0][n];
+        for (int n = 0; n < 5; ++n) a.R[ro + n * rn] = Rc[n] + (Fprev[n] - acc[0][n]);
       }
 #pragma unroll
       for (int n = 0; n < 5; ++n) Fprev[n] = acc[0][n];
@@ -549,8 +558,10 @@ template <typename T, int MODE, int ROWS>
 __global__ __launch_bounds__(VX * ROWS) void k_visc_update(KP kp, FArgs<T> a, double* __restrict__ U,
                                                            double* __restrict__ Un, T* __restrict__ Q,
                                                            double* __restrict__ Rout, double dt, int KC, int remap) {
-  __shared__ T sFx[4][ROWS][VX];       // Fv_x of the interior rows, all 64 lanes
-  __shared__ T sFy[4][ROWS + 4][VXI];  // Fv_y of rows -2 .. ROWS+1, interior columns
+  // two buffers, alternating per plane: a wave writing plane p+1 never overwrites what a slower wave
+  // still reads of plane p, so one barrier per plane suffices
+  __shared__ T sFx2[2][4][ROWS][VX];       // Fv_x of the interior rows, all 64 lanes
+  __shared__ T sFy2[2][4][ROWS + 4][VXI];  // Fv_y of rows -2 .. ROWS+1, interior columns
   const Blk B = xcd_block(remap);
   const int lane = threadIdx.x, row = threadIdx.y;
   const int NG = kp.NG;
@@ -593,6 +604,8 @@ __global__ __launch_bounds__(VX * ROWS) void k_visc_update(KP kp, FArgs<T> a, do
 #pragma unroll 1
   for (int s = -4; s < nk; ++s) {
     const int p = k0 + s + 2;
+    T(*sFx)[ROWS][VX] = sFx2[s & 1];
+    T(*sFy)[ROWS + 4][VXI] = sFy2[s & 1];
     {
       const long long o = col + (long long)(p + 2) * sz;
 #pragma unroll
@@ -642,7 +655,6 @@ __global__ __launch_bounds__(VX * ROWS) void k_visc_update(KP kp, FArgs<T> a, do
         xy[2][n] = d;
       }
     }
-    __syncthreads();
     if (s < 0 || !inner) continue;
     // ---- update of plane k = p - 2
     const int k = k0 + s;
@@ -660,27 +672,23 @@ __global__ __launch_bounds__(VX * ROWS) void k_visc_update(KP kp, FArgs<T> a, do
   }
 }
 
-// Same computation with the y-halo rows as lanes of their own: 64 x 8 lanes over a 60 x 4 tile
-// (rows 0, 1, 6, 7 are halo rows, Fv_y only), each lane with its own u,v,w,T window.  Used for the
-// FP32-bulk form, whose smaller registers leave room for two 8-wave blocks per CU; the FP64 form
-// uses k_visc_update (halo evaluated by the interior lanes, every wave updating).
-constexpr int VY8 = 8;
-
-template <typename T, int MODE>
-__global__ __launch_bounds__(512) void k_visc_update8(KP kp, FArgs<T> a, double* __restrict__ U,
+// Same computation with the y-halo rows as lanes of their own: 64 x (VI + 4) lanes over a 60 x VI
+// tile (rows 0, 1, VI + 2, VI + 3 are halo rows, Fv_y only), each lane with its own u,v,w,T window.
+template <typename T, int MODE, int VI>
+__global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update8(KP kp, FArgs<T> a, double* __restrict__ U,
                                                       double* __restrict__ Un, T* __restrict__ Q,
                                                       double* __restrict__ Rout, double dt, int KC, int remap) {
-  __shared__ T sFx[4][VYI][VX];
-  __shared__ T sFy[4][VY8][VXI];
+  __shared__ T sFx2[2][4][VI][VX];  // double-buffered per plane as in k_visc_update
+  __shared__ T sFy2[2][4][VI + 4][VXI];
   const Blk B = xcd_block(remap);
   const int lane = threadIdx.x, row = threadIdx.y;
   const int NG = kp.NG;
-  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * VYI - 2 + row;
+  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * VI - 2 + row;
   const int k0 = NG + B.z * KC;
   const int nk = min(KC, NG + kp.Nzp - k0);
   const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;
   const bool xin = lane >= 2 && lane < 2 + VXI && x < NG + kp.Nxp;
-  const bool yin = row >= 2 && row < 2 + VYI && y < NG + kp.Nyp;
+  const bool yin = row >= 2 && row < 2 + VI && y < NG + kp.Nyp;
   const bool needFx = yin && x <= xmax;
   const bool needFy = xin && y <= ymax;
   const bool inner = xin && yin;
@@ -708,6 +716,8 @@ __global__ __launch_bounds__(512) void k_visc_update8(KP kp, FArgs<T> a, double*
 #pragma unroll 1
   for (int s = -4; s < nk; ++s) {
     const int p = k0 + s + 2;
+    T(*sFx)[VI][VX] = sFx2[s & 1];
+    T(*sFy)[VI + 4][VXI] = sFy2[s & 1];
     {
       const long long o = col + (long long)(p + 2) * sz;
 #pragma unroll
@@ -749,7 +759,6 @@ __global__ __launch_bounds__(512) void k_visc_update8(KP kp, FArgs<T> a, double*
         xy[2][n] = d;
       }
     }
-    __syncthreads();
     if (s < 0 || !inner) continue;
     const int k = k0 + s;
     const long long oc = col + (long long)k * sz, ro = ridx(kp, x - NG, y - NG, k - NG);
@@ -805,18 +814,41 @@ static dim3 visc_grid(const KP& kp, int KC, int rows) {
   return dim3((kp.Nxp + VXI - 1) / VXI, (kp.Nyp + rows - 1) / rows, (kp.Nzp + KC - 1) / KC);
 }
 
-// The viscous-update march, per precision the layout measured fastest on the 512x192x64 ramp:
-// FP32-bulk k_visc_update8 (halo rows as lanes, two 8-wave blocks per CU), FP64 k_visc_update with
-// 4 interior rows (237 VGPRs: two 4-wave blocks per CU; taller blocks stall more waves per barrier).
+// Form of the viscous-update march: 0 = k_visc_update (4 interior rows, the y-halo cells evaluated
+// by the lanes of the first and last two rows); VI = 4, 8, 12 = k_visc_update8 with VI interior rows
+// and the halo rows as lanes of their own.  Defaults measured on the 512x192x64 ramp (one MI355X);
+// F3D_VISC_FORM overrides.
+static int visc_form(int es) {
+  static int v[2] = {-2, -2};
+  int& r = v[es == 8];
+  if (r == -2) {
+    const char* e = std::getenv("F3D_VISC_FORM");
+    const int def = es == 8 ? 0 : 4;
+    r = e ? std::atoi(e) : def;
+    if (r != 0 && r != 4 && r != 8 && r != 12) r = def;
+    if (es == 8 && r == 12) r = 8;  // 16 FP64 waves of ~160 VGPRs do not fit one CU
+  }
+  return r;
+}
+
+static int visc_rows(int es) {
+  const int f = visc_form(es);
+  return f == 0 ? VYI : f;
+}
+
 template <typename T, int MODE>
 static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt,
                                int KC, int rm, hipStream_t s) {
-  if (sizeof(T) == 4)
-    hipLaunchKernelGGL((k_visc_update8<T, MODE>), visc_grid(kp, KC, VYI), dim3(VX, VY8, 1), 0, s, kp, a, U, Un, Q,
-                       Rout, dt, KC, rm);
-  else
-    hipLaunchKernelGGL((k_visc_update<T, MODE, VYI>), visc_grid(kp, KC, VYI), dim3(VX, VYI, 1), 0, s, kp, a, U, Un, Q,
-                       Rout, dt, KC, rm);
+  const int f = visc_form((int)sizeof(T));
+  const dim3 g = visc_grid(kp, KC, visc_rows((int)sizeof(T)));
+  if (f == 0)
+    hipLaunchKernelGGL((k_visc_update<T, MODE, VYI>), g, dim3(VX, VYI, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
+  else if (f == 4)
+    hipLaunchKernelGGL((k_visc_update8<T, MODE, 4>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
+  else if (f == 8)
+    hipLaunchKernelGGL((k_visc_update8<T, MODE, 8>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
+  else if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL((k_visc_update8<T, MODE, 12>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
 }
 
 template <typename T>
@@ -897,7 +929,7 @@ int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double*
   }
   TimedScope ts("fused_update", s);
   if (kp.viscous && inplace) {
-    const int KV = visc_kc(kp, VYI);
+    const int KV = visc_kc(kp, visc_rows((int)sizeof(T)));
     if (mode == 0)
       launch_visc_update<T, 0>(kp, a, nullptr, nullptr, nullptr, Rout, 0.0, KV, rm, s);
     else if (mode == 1)
