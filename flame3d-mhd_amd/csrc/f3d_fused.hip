@@ -775,6 +775,151 @@ __global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update8(KP kp, FArgs<T> 
   }
 }
 
+// k_visc_update8 with the xi and eta neighbours of u,v,w,T taken from a plane tile in LDS instead of
+// per-lane loads: every lane stores its column's plane-p values (from its register window) into the
+// tile, the lanes at the x edges add the two cells beyond (cells x0-4, x0-3 and x0+62, x0+63) and the
+// halo rows the two rows beyond (y0-4, y0-3 and y0+VI+2, y0+VI+3); after a barrier the 4th-order
+// xi/eta differences read the tile (same formulas and summation order as visc_eval).  Per lane and
+// plane that replaces 32 neighbour loads by 4 (window) + 4 (edges / extra rows, some lanes only).
+// The tile is written before barrier A and read before barrier B (the viscous-flux exchange), so one
+// buffer suffices; likewise the flux buffers, written after A and read after B.
+template <typename T, int MODE, int VI>
+__global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T> a, double* __restrict__ U,
+                                                        double* __restrict__ Un, T* __restrict__ Q,
+                                                        double* __restrict__ Rout, double dt, int KC, int remap) {
+  __shared__ T sQ[4][VI + 8][VX + 4];  // u,v,w,T of plane p: rows y0-4 .. y0+VI+3, columns x0-4 .. x0+63
+  __shared__ T sFx[4][VI][VX];
+  __shared__ T sFy[4][VI + 4][VXI];
+  const Blk B = xcd_block(remap);
+  const int lane = threadIdx.x, row = threadIdx.y;
+  const int NG = kp.NG;
+  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * VI - 2 + row;
+  const int k0 = NG + B.z * KC;
+  const int nk = min(KC, NG + kp.Nzp - k0);
+  const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;
+  const bool xin = lane >= 2 && lane < 2 + VXI && x < NG + kp.Nxp;
+  const bool yin = row >= 2 && row < 2 + VI && y < NG + kp.Nyp;
+  const bool needFx = yin && x <= xmax;
+  const bool needFy = xin && y <= ymax;
+  const bool inner = xin && yin;
+  // tile columns are loaded up to the array bound (the +1/+2 neighbours of the box's last cell)
+  const int xl = min(x, kp.Nxt - 1), yl = min(y, kp.Nyt - 1);
+  const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  const long long col = cidx(kp, xl, yl, 0);
+  // extra cell of the edge lanes (x -+ 2) and of the halo rows (y -+ 2), clamped into the array
+  const bool xe = lane < 2 || lane >= VX - 2, ye = row < 2 || row >= VI + 2;
+  const int xe_x = min(max(lane < 2 ? x - 2 : x + 2, 0), kp.Nxt - 1);
+  const int ye_y = min(max(row < 2 ? y - 2 : y + 2, 0), kp.Nyt - 1);
+  const long long colx = cidx(kp, xe_x, yl, 0), coly = cidx(kp, xl, ye_y, 0);
+  const int cxe = lane < 2 ? lane : lane + 4;   // tile column of the edge cell
+  const int rye = row < 2 ? row : row + 4;      // tile row of the extra row
+  const int tc = lane + 2, tr = row + 2;        // own tile column / row
+  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
+  const int var[4] = {1, 2, 3, 5};
+  T w[4][5], fz[4][5], xy[3][4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) w[v][q] = fz[v][q] = T(0);
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) xy[q][n] = T(0);
+#pragma unroll
+  for (int q = 1; q < 5; ++q) {
+    const long long o = col + (long long)(k0 - 5 + q) * sz;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) w[v][q] = a.Q[o + var[v] * nc];
+  }
+#pragma unroll 1
+  for (int s = -4; s < nk; ++s) {
+    const int p = k0 + s + 2;
+    T ex[4], ey[4];
+    if (xe) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ex[v] = a.Q[colx + (long long)p * sz + var[v] * nc];
+    }
+    if (ye) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ey[v] = a.Q[coly + (long long)p * sz + var[v] * nc];
+    }
+    {
+      const long long o = col + (long long)(p + 2) * sz;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[v][q] = w[v][q + 1];
+        w[v][4] = a.Q[o + var[v] * nc];
+      }
+    }
+    const long long op = col + (long long)p * sz;
+    T m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = a.m.m[q][op];
+    const T Jc = (T)a.J[op];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      sQ[v][tr][tc] = w[v][2];
+      if (xe) sQ[v][tr][cxe] = ex[v];
+      if (ye) sQ[v][rye][tc] = ey[v];
+    }
+    __syncthreads();
+    T out[3][4];
+    if (needFx || needFy) {
+      T g[4][3];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        g[v][0] = c1 * (sQ[v][tr][tc - 2] - sQ[v][tr][tc + 2]) + c2 * (sQ[v][tr][tc - 1] - sQ[v][tr][tc + 1]);
+        g[v][1] = c1 * (sQ[v][tr - 2][tc] - sQ[v][tr + 2][tc]) + c2 * (sQ[v][tr - 1][tc] - sQ[v][tr + 1][tc]);
+        g[v][2] = c1 * (w[v][0] - w[v][4]) + c2 * (w[v][1] - w[v][3]);
+      }
+      viscous_cell<T>(kp, w[0][2], w[1][2], w[2][2], w[3][2], g, m, Jc, out);
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) out[d][n] = T(0);
+    }
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) fz[n][q] = fz[n][q + 1];
+      fz[n][4] = out[2][n];
+      if (needFx) sFx[n][row - 2][lane] = out[0][n];
+      if (needFy) sFy[n][row][lane - 2] = out[1][n];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      xy[0][n] = xy[1][n];
+      xy[1][n] = xy[2][n];
+    }
+    if (inner) {
+      const int r = row - 2, l = lane - 2;
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        T d = c1 * (sFx[n][r][lane - 2] - sFx[n][r][lane + 2]) + c2 * (sFx[n][r][lane - 1] - sFx[n][r][lane + 1]);
+        d += c1 * (sFy[n][row - 2][l] - sFy[n][row + 2][l]) + c2 * (sFy[n][row - 1][l] - sFy[n][row + 1][l]);
+        xy[2][n] = d;
+      }
+    }
+    if (s < 0 || !inner) continue;
+    const int k = k0 + s;
+    const long long oc = col + (long long)k * sz, ro = ridx(kp, x - NG, y - NG, k - NG);
+    T Rt[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Rt[n] = a.R[ro + n * rn];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      T sdiv = xy[0][n];
+      sdiv += c1 * (fz[n][0] - fz[n][4]) + c2 * (fz[n][1] - fz[n][3]);
+      Rt[n + 1] += sdiv;
+    }
+    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt, a.ldt);
+  }
+}
+
 // ------------------------------------------------------------------ orchestration
 // Segment length of the y/z marches; F3D_MARCH_SEG_Y / F3D_MARCH_SEG_Z override the defaults.
 static int march_seg(int D, int es) {
@@ -795,8 +940,8 @@ static int seg_len(int Np, int max_len) {
   return (Np + nseg - 1) / nseg;
 }
 
-// Planes per segment of the viscous-update march: 16 where that gives >= ~1000 blocks, shorter on
-// small grids (each segment re-evaluates 4 planes of viscous fluxes); F3D_VISC_KC overrides.
+// Planes per segment of the viscous-update march: 16 where that gives >= 512 blocks (two per CU),
+// shorter on small grids (each segment re-evaluates 4 planes of viscous fluxes); F3D_VISC_KC overrides.
 static int visc_kc(const KP& kp, int rows) {
   static int env = -2;
   if (env == -2) {
@@ -806,7 +951,7 @@ static int visc_kc(const KP& kp, int rows) {
   if (env >= 4) return seg_len(kp.Nzp, env);
   const long long cols = (long long)((kp.Nxp + VXI - 1) / VXI) * ((kp.Nyp + rows - 1) / rows);
   int kc = 16;
-  while (kc > 8 && cols * ((kp.Nzp + kc - 1) / kc) < 1000) kc -= 4;
+  while (kc > 8 && cols * ((kp.Nzp + kc - 1) / kc) < 512) kc -= 4;
   return seg_len(kp.Nzp, kc);
 }
 
@@ -815,15 +960,16 @@ static dim3 visc_grid(const KP& kp, int KC, int rows) {
 }
 
 // Form of the viscous-update march: 0 = k_visc_update (4 interior rows, the y-halo cells evaluated
-// by the lanes of the first and last two rows); VI = 4, 8, 12 = k_visc_update8 with VI interior rows
-// and the halo rows as lanes of their own.  Defaults measured on the 512x192x64 ramp (one MI355X);
-// F3D_VISC_FORM overrides.
+// by the lanes of the first and last two rows); VI = 4, 8, 12 = k_visc_update_qt (or, with
+// F3D_VISC_QT=0, k_visc_update8) with VI interior rows and the halo rows as lanes of their own.
+// Defaults measured on the 512x192x64 ramp (one MI355X): FP64 4 (8-wave blocks, two per CU), FP32 12
+// (one 16-wave block per CU); F3D_VISC_FORM overrides.
 static int visc_form(int es) {
   static int v[2] = {-2, -2};
   int& r = v[es == 8];
   if (r == -2) {
     const char* e = std::getenv("F3D_VISC_FORM");
-    const int def = es == 8 ? 0 : 4;
+    const int def = es == 8 ? 4 : 12;
     r = e ? std::atoi(e) : def;
     if (r != 0 && r != 4 && r != 8 && r != 12) r = def;
     if (es == 8 && r == 12) r = 8;  // 16 FP64 waves of ~160 VGPRs do not fit one CU
@@ -836,11 +982,34 @@ static int visc_rows(int es) {
   return f == 0 ? VYI : f;
 }
 
+// 1 (default) = the LDS-tile variant k_visc_update_qt for the forms VI = 4, 8, 12; F3D_VISC_QT=0 selects
+// k_visc_update8 (per-lane neighbour loads) for A/B measurement.
+static int visc_qt(int es) {
+  static int v[2] = {-2, -2};
+  int& r = v[es == 8];
+  if (r == -2) {
+    const char* e = std::getenv("F3D_VISC_QT");
+    r = e ? (std::atoi(e) != 0) : 1;
+  }
+  return r;
+}
+
 template <typename T, int MODE>
 static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt,
                                int KC, int rm, hipStream_t s) {
   const int f = visc_form((int)sizeof(T));
   const dim3 g = visc_grid(kp, KC, visc_rows((int)sizeof(T)));
+  if (f != 0 && visc_qt((int)sizeof(T))) {
+    if (f == 4)
+      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
+    else if (f == 8)
+      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
+                         rm);
+    else if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 12>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
+                         rm);
+    return;
+  }
   if (f == 0)
     hipLaunchKernelGGL((k_visc_update<T, MODE, VYI>), g, dim3(VX, VYI, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
   else if (f == 4)

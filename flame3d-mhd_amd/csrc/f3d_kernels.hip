@@ -20,6 +20,11 @@ static inline dim3 range_grid(const Range3& r) {
   return dim3((unsigned)((r.ni + 63) / 64), (unsigned)((r.nj + 3) / 4), (unsigned)r.nk);
 }
 static const dim3 kBlock(64, 4, 1);
+// x ghost slabs are 2*NG = 8 cells wide along i: 8 x 32 blocks keep every lane busy
+static inline dim3 slab_grid_x(const Range3& r) {
+  return dim3((unsigned)((r.ni + 7) / 8), (unsigned)((r.nj + 31) / 32), (unsigned)r.nk);
+}
+static const dim3 kBlockX(8, 32, 1);
 
 // ------------------------------------------------------------------ shockSensor  schemes.jl:1-24
 template <typename T>
@@ -588,7 +593,7 @@ int launch_exchange(const KP& kp, T* Q, int NV, hipStream_t s) {
   for (int d = 0; d < 3; ++d) {
     if (kp.bc[2 * d] != F3D_BC_PERIODIC || kp.bc[2 * d + 1] != F3D_BC_PERIODIC) continue;
     Range3 r = slab_range(kp, d);
-    if (d == 0) hipLaunchKernelGGL((k_exchange<T, 0>), range_grid(r), kBlock, 0, s, kp, r, Q, NV);
+    if (d == 0) hipLaunchKernelGGL((k_exchange<T, 0>), slab_grid_x(r), kBlockX, 0, s, kp, r, Q, NV);
     if (d == 1) hipLaunchKernelGGL((k_exchange<T, 1>), range_grid(r), kBlock, 0, s, kp, r, Q, NV);
     if (d == 2) hipLaunchKernelGGL((k_exchange<T, 2>), range_grid(r), kBlock, 0, s, kp, r, Q, NV);
   }
@@ -600,7 +605,7 @@ int launch_fill(const KP& kp, T* Q, double* U, const T* inflow, hipStream_t s) {
   TimedScope ts_("fill_ghost", s);
   for (int d = 0; d < 3; ++d) {
     Range3 r = slab_range(kp, d);
-    if (d == 0) hipLaunchKernelGGL((k_fill<T, 0>), range_grid(r), kBlock, 0, s, kp, r, Q, U, inflow);
+    if (d == 0) hipLaunchKernelGGL((k_fill<T, 0>), slab_grid_x(r), kBlockX, 0, s, kp, r, Q, U, inflow);
     if (d == 1) hipLaunchKernelGGL((k_fill<T, 1>), range_grid(r), kBlock, 0, s, kp, r, Q, U, inflow);
     if (d == 2) hipLaunchKernelGGL((k_fill<T, 2>), range_grid(r), kBlock, 0, s, kp, r, Q, U, inflow);
   }
