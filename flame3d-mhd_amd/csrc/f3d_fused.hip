@@ -554,6 +554,75 @@ __device__ __forceinline__ void update_cell(const KP& kp, const T* Rt, long long
   Q[oc + 5 * nc] = (T)(pr / (rho * kp.Rg));
 }
 
+// Operands of one cell's conservative update (RHS increment, U, Un, J, dt), loaded one march
+// iteration before they are used so that their memory latency overlaps the viscous evaluation.
+template <typename T>
+struct UpdOps {
+  T R[5];
+  double U[5], Un[5], J, dtc;
+};
+
+template <typename T, int MODE>
+__device__ __forceinline__ void load_ops(const KP& kp, const FArgs<T>& a, const double* __restrict__ U,
+                                         const double* __restrict__ Un, long long oc, long long ro, double dt,
+                                         UpdOps<T>& op) {
+  const long long nc = kp.nc, rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) op.R[n] = a.R[ro + n * rn];
+  op.J = a.J[oc];
+  if (MODE != 0) {
+    op.dtc = a.ldt ? a.ldt[oc] : dt;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      op.U[n] = U[oc + n * nc];
+      if (MODE >= 2) op.Un[n] = Un[oc + n * nc];
+    }
+  }
+}
+
+// update_cell from preloaded operands (same arithmetic).
+template <typename T, int MODE>
+__device__ __forceinline__ void update_cell_ops(const KP& kp, const T* Rt, long long oc, const UpdOps<T>& op,
+                                                double* __restrict__ U, double* __restrict__ Un, T* __restrict__ Q,
+                                                double* __restrict__ Rout) {
+  const long long nc = kp.nc;
+  if (MODE == 0) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Rout[oc + n * nc] = (double)Rt[n] * op.J;
+    return;
+  }
+  const double Jact = op.J * op.dtc;
+  double un[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const double u0 = op.U[n];
+    const double u1 = u0 + (double)Rt[n] * Jact;
+    double r;
+    if (MODE == 1) {
+      Un[oc + n * nc] = u0;
+      r = u1;
+    } else if (MODE == 2) {
+      r = u1 * 0.25 + op.Un[n] * 0.75;
+    } else {
+      r = u1 * (2.0 / 3.0) + op.Un[n] * (1.0 / 3.0);
+    }
+    un[n] = r;
+    U[oc + n * nc] = r;
+  }
+  const double eps = 2.220446049250313e-16;
+  const double rho = jmax(un[0], eps);
+  const double ri = 1.0 / rho;
+  const double u = un[1] * ri, v = un[2] * ri, ww = un[3] * ri;
+  const double ei = jmax(un[4] - 0.5 * rho * (u * u + v * v + ww * ww), eps);
+  const double pr = (kp.gamma - 1) * ei;
+  Q[oc] = (T)rho;
+  Q[oc + nc] = (T)u;
+  Q[oc + 2 * nc] = (T)v;
+  Q[oc + 3 * nc] = (T)ww;
+  Q[oc + 4 * nc] = (T)pr;
+  Q[oc + 5 * nc] = (T)(pr / (rho * kp.Rg));
+}
+
 template <typename T, int MODE, int ROWS>
 __global__ __launch_bounds__(VX * ROWS) void k_visc_update(KP kp, FArgs<T> a, double* __restrict__ U,
                                                            double* __restrict__ Un, T* __restrict__ Q,
@@ -783,7 +852,7 @@ __global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update8(KP kp, FArgs<T> 
 // plane that replaces 32 neighbour loads by 4 (window) + 4 (edges / extra rows, some lanes only).
 // The tile is written before barrier A and read before barrier B (the viscous-flux exchange), so one
 // buffer suffices; likewise the flux buffers, written after A and read after B.
-template <typename T, int MODE, int VI>
+template <typename T, int MODE, int VI, bool PF>
 __global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T> a, double* __restrict__ U,
                                                         double* __restrict__ Un, T* __restrict__ Q,
                                                         double* __restrict__ Rout, double dt, int KC, int remap) {
@@ -832,17 +901,57 @@ __global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T
 #pragma unroll
     for (int v = 0; v < 4; ++v) w[v][q] = a.Q[o + var[v] * nc];
   }
+  // PF: the metrics and J of the next plane and the operands of the next update are loaded one
+  // iteration ahead, so that their latency overlaps this plane's evaluation
+  // (FP32-bulk, whose registers allow it: the edge columns and extra rows too)
+  constexpr bool PFE = PF && sizeof(T) == 4;
+  T mn[9], exn[4], eyn[4];
+  double Jn = 0;
+  UpdOps<T> op;
+  if constexpr (PF) {
+    const long long o0 = col + (long long)(k0 - 2) * sz;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) mn[q] = a.m.m[q][o0];
+    Jn = a.J[o0];
+    if constexpr (PFE) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        exn[v] = xe ? a.Q[colx + (long long)(k0 - 2) * sz + var[v] * nc] : T(0);
+        eyn[v] = ye ? a.Q[coly + (long long)(k0 - 2) * sz + var[v] * nc] : T(0);
+      }
+    }
+    if (inner && nk > 0)
+      load_ops<T, MODE>(kp, a, U, Un, col + (long long)k0 * sz, ridx(kp, x - NG, y - NG, k0 - NG), dt, op);
+  }
 #pragma unroll 1
   for (int s = -4; s < nk; ++s) {
     const int p = k0 + s + 2;
     T ex[4], ey[4];
-    if (xe) {
+    if constexpr (PFE) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) ex[v] = a.Q[colx + (long long)p * sz + var[v] * nc];
-    }
-    if (ye) {
+      for (int v = 0; v < 4; ++v) {
+        ex[v] = exn[v];
+        ey[v] = eyn[v];
+      }
+      if (s + 1 < nk) {
+        if (xe) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) ey[v] = a.Q[coly + (long long)p * sz + var[v] * nc];
+          for (int v = 0; v < 4; ++v) exn[v] = a.Q[colx + (long long)(p + 1) * sz + var[v] * nc];
+        }
+        if (ye) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) eyn[v] = a.Q[coly + (long long)(p + 1) * sz + var[v] * nc];
+        }
+      }
+    } else {
+      if (xe) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ex[v] = a.Q[colx + (long long)p * sz + var[v] * nc];
+      }
+      if (ye) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ey[v] = a.Q[coly + (long long)p * sz + var[v] * nc];
+      }
     }
     {
       const long long o = col + (long long)(p + 2) * sz;
@@ -853,11 +962,23 @@ __global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T
         w[v][4] = a.Q[o + var[v] * nc];
       }
     }
-    const long long op = col + (long long)p * sz;
+    const long long opl = col + (long long)p * sz;
     T m[9];
+    T Jc;
+    if constexpr (PF) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) m[q] = a.m.m[q][op];
-    const T Jc = (T)a.J[op];
+      for (int q = 0; q < 9; ++q) m[q] = mn[q];
+      Jc = (T)Jn;
+      if (s + 1 < nk) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) mn[q] = a.m.m[q][opl + sz];
+        Jn = a.J[opl + sz];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) m[q] = a.m.m[q][opl];
+      Jc = (T)a.J[opl];
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       sQ[v][tr][tc] = w[v][2];
@@ -909,14 +1030,19 @@ __global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T
     const long long oc = col + (long long)k * sz, ro = ridx(kp, x - NG, y - NG, k - NG);
     T Rt[5];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) Rt[n] = a.R[ro + n * rn];
+    for (int n = 0; n < 5; ++n) Rt[n] = PF ? op.R[n] : a.R[ro + n * rn];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       T sdiv = xy[0][n];
       sdiv += c1 * (fz[n][0] - fz[n][4]) + c2 * (fz[n][1] - fz[n][3]);
       Rt[n + 1] += sdiv;
     }
-    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt, a.ldt);
+    if constexpr (PF) {
+      update_cell_ops<T, MODE>(kp, Rt, oc, op, U, Un, Q, Rout);
+      if (s + 1 < nk) load_ops<T, MODE>(kp, a, U, Un, oc + sz, ro + (long long)kp.Nxp * kp.Nyp, dt, op);
+    } else {
+      update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt, a.ldt);
+    }
   }
 }
 
@@ -940,9 +1066,10 @@ static int seg_len(int Np, int max_len) {
   return (Np + nseg - 1) / nseg;
 }
 
-// Planes per segment of the viscous-update march: 16 where that gives >= 512 blocks (two per CU),
-// shorter on small grids (each segment re-evaluates 4 planes of viscous fluxes); F3D_VISC_KC overrides.
-static int visc_kc(const KP& kp, int rows) {
+// Planes per segment of the viscous-update march: 16 (FP64) or 32 (FP32-bulk, one 12-wave block per
+// CU) where that gives >= 384 blocks, shorter on small grids (each segment re-evaluates 4 planes of
+// viscous fluxes); F3D_VISC_KC overrides.
+static int visc_kc(const KP& kp, int rows, int es) {
   static int env = -2;
   if (env == -2) {
     const char* e = std::getenv("F3D_VISC_KC");
@@ -950,8 +1077,8 @@ static int visc_kc(const KP& kp, int rows) {
   }
   if (env >= 4) return seg_len(kp.Nzp, env);
   const long long cols = (long long)((kp.Nxp + VXI - 1) / VXI) * ((kp.Nyp + rows - 1) / rows);
-  int kc = 16;
-  while (kc > 8 && cols * ((kp.Nzp + kc - 1) / kc) < 512) kc -= 4;
+  int kc = es == 4 ? 32 : 16;
+  while (kc > 8 && cols * ((kp.Nzp + kc - 1) / kc) < 384) kc -= 4;
   return seg_len(kp.Nzp, kc);
 }
 
@@ -962,14 +1089,15 @@ static dim3 visc_grid(const KP& kp, int KC, int rows) {
 // Form of the viscous-update march: 0 = k_visc_update (4 interior rows, the y-halo cells evaluated
 // by the lanes of the first and last two rows); VI = 4, 8, 12 = k_visc_update_qt (or, with
 // F3D_VISC_QT=0, k_visc_update8) with VI interior rows and the halo rows as lanes of their own.
-// Defaults measured on the 512x192x64 ramp (one MI355X): FP64 4 (8-wave blocks, two per CU), FP32 12
-// (one 16-wave block per CU); F3D_VISC_FORM overrides.
+// Defaults measured on the 512x192x64 ramp (one MI355X): FP64 4 and FP32-bulk 8, both with the
+// operand prefetch (F3D_VISC_QT=2: 242 / 151 VGPRs, one 8- / 12-wave block per CU); F3D_VISC_FORM
+// overrides.
 static int visc_form(int es) {
   static int v[2] = {-2, -2};
   int& r = v[es == 8];
   if (r == -2) {
     const char* e = std::getenv("F3D_VISC_FORM");
-    const int def = es == 8 ? 4 : 12;
+    const int def = es == 8 ? 4 : 8;
     r = e ? std::atoi(e) : def;
     if (r != 0 && r != 4 && r != 8 && r != 12) r = def;
     if (es == 8 && r == 12) r = 8;  // 16 FP64 waves of ~160 VGPRs do not fit one CU
@@ -982,14 +1110,16 @@ static int visc_rows(int es) {
   return f == 0 ? VYI : f;
 }
 
-// 1 (default) = the LDS-tile variant k_visc_update_qt for the forms VI = 4, 8, 12; F3D_VISC_QT=0 selects
-// k_visc_update8 (per-lane neighbour loads) for A/B measurement.
+// LDS-tile variant k_visc_update_qt for the forms VI = 4, 8, 12: 2 (default) with the next plane's
+// metrics and update operands prefetched (VI = 4, 8), 1 without; 0 selects k_visc_update8 (per-lane
+// neighbour loads) for A/B measurement.  F3D_VISC_QT overrides.
 static int visc_qt(int es) {
   static int v[2] = {-2, -2};
   int& r = v[es == 8];
   if (r == -2) {
     const char* e = std::getenv("F3D_VISC_QT");
-    r = e ? (std::atoi(e) != 0) : 1;
+    r = e ? std::atoi(e) : 2;
+    if (r < 0 || r > 2) r = 2;
   }
   return r;
 }
@@ -999,15 +1129,27 @@ static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, doubl
                                int KC, int rm, hipStream_t s) {
   const int f = visc_form((int)sizeof(T));
   const dim3 g = visc_grid(kp, KC, visc_rows((int)sizeof(T)));
-  if (f != 0 && visc_qt((int)sizeof(T))) {
+  const int qt = visc_qt((int)sizeof(T));
+  // (the FP64 prefetch variant of the 12-wave form spills registers: not used)
+  if ((f == 4 || (f == 8 && sizeof(T) == 4)) && qt == 2) {
     if (f == 4)
-      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
-    else if (f == 8)
-      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
+      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, true>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
                          rm);
     else if constexpr (sizeof(T) == 4)
-      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 12>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
+      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, true>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                         KC, rm);
+    return;
+  }
+  if (f != 0 && qt) {
+    if (f == 4)
+      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, false>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
                          rm);
+    else if (f == 8)
+      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, false>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                         KC, rm);
+    else if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 12, false>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                         KC, rm);
     return;
   }
   if (f == 0)
@@ -1098,7 +1240,7 @@ int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double*
   }
   TimedScope ts("fused_update", s);
   if (kp.viscous && inplace) {
-    const int KV = visc_kc(kp, visc_rows((int)sizeof(T)));
+    const int KV = visc_kc(kp, visc_rows((int)sizeof(T)), (int)sizeof(T));
     if (mode == 0)
       launch_visc_update<T, 0>(kp, a, nullptr, nullptr, nullptr, Rout, 0.0, KV, rm, s);
     else if (mode == 1)
