@@ -268,7 +268,11 @@ int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double*
     if (!rc && KRK == 3) rc = launch_lin_comb(kp, U, Un, 5, 2.0 / 3.0, 1.0 / 3.0, s);
     if (!rc) rc = launch_c2prim<T>(kp, U, Qout, s);
   }
-  if (!rc) rc = launch_exchange<T>(kp, Qout, 6, s);
+  // exchange_ghost (mpi.jl:2-103) is not launched here: on one rank it copies the ghost slabs of the
+  // periodic dimensions, and fill_d of a periodic dimension performs the same copy over its full slab
+  // afterwards.  Every fill reads only cells of its own line (same (j,k) for x, (i,k) for y, (i,j) for
+  // z), and each later fill rewrites the whole slab of its dimension, so the ghost values the exchange
+  // would have set first are all overwritten with the same final values.
   // U ghosts are read only by the finite-volume reconstruction and the filters; fillGhost refills them
   // from Q's there (boundary.jl fills them always)
   if (!rc) rc = launch_fill<T>(kp, Qout, (geo || p->filtering) ? U : nullptr, inflow, s);
