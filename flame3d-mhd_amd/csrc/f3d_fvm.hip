@@ -203,6 +203,7 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
     for (int n = 0; n < 5; ++n) W[s][n] = 0;
   }
   T Fprev[5] = {0, 0, 0, 0, 0};
+  T Rn[5] = {0, 0, 0, 0, 0};  // RHS of the row the next step closes, loaded a step ahead
   T gn[4] = {0, 0, 0, 0};  // geometry of the next face to complete
   double nu[5];
   T nph;
@@ -225,6 +226,14 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
 #pragma unroll
       for (int n = 0; n < 5; ++n) nu[n] = a.U[o + n * nc];
       nph = a.phi[o];
+    }
+    T Rc[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Rc[n] = Rn[n];
+    if (step >= 7 && step + 1 < nstep && act) {  // row f0 + step - 7, closed at step + 1
+      const long long ro = D == 1 ? rix(kp, ii, f0 + step - 7, tt) : rix(kp, ii, tt, f0 + step - 7);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) Rn[n] = a.R[ro + n * rn];
     }
     if (step >= 6) {
       const T g0 = gn[0], g1 = gn[1], g2 = gn[2], g3 = gn[3];  // face f0 + step - 7 (step >= 7)
@@ -254,7 +263,7 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
         if (fi >= 1 && act) {
           const long long ro = D == 1 ? rix(kp, ii, f - 1, tt) : rix(kp, ii, tt, f - 1);
 #pragma unroll
-          for (int n = 0; n < 5; ++n) a.R[ro + n * rn] += Fprev[n] - Fc[n];
+          for (int n = 0; n < 5; ++n) a.R[ro + n * rn] = Rc[n] + (Fprev[n] - Fc[n]);
         }
 #pragma unroll
         for (int n = 0; n < 5; ++n) Fprev[n] = Fc[n];
