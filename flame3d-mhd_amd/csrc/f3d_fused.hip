@@ -85,6 +85,30 @@ __global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, co
   phi[o] = sensor_cell<T>(P[o - 1], P[o], P[o + 1], P[o - sy], P[o + sy], P[o - sz], P[o + sz]);
 }
 
+// Same sensor, one lane per (i, j) column of [1, N_tot-2] marching a segment of KS planes: p of the
+// column at k-1, k, k+1 stays in registers, so each cell costs one new plane load plus the four xi/eta
+// neighbours (L1/L2) instead of six neighbour loads.  Identical arithmetic (sensor_cell).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi, const T* __restrict__ Q,
+                                                      unsigned int* __restrict__ hcount, int KS) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k0 = 1 + blockIdx.z * KS;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hcount = 0;
+  if (i > kp.Nxt - 2 || j > kp.Nyt - 2) return;
+  const int k1 = min(k0 + KS, kp.Nzt - 1);  // planes k0 .. k1-1
+  const T* P = Q + 4 * kp.nc;
+  const long long sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
+  long long o = cidx(kp, i, j, k0);
+  T pm = P[o - sz], pc = P[o];
+#pragma unroll 1
+  for (int k = k0; k < k1; ++k, o += sz) {
+    const T pp = P[o + sz];
+    phi[o] = sensor_cell<T>(P[o - 1], pc, P[o + 1], P[o - sy], P[o + sy], pm, pp);
+    pm = pc;
+    pc = pp;
+  }
+}
+
 // ------------------------------------------------------------------ x sweep (LDS row segments)
 constexpr int XW = 128;  // lanes per row segment (cells incl. the 4+4 halo)
 
@@ -1162,11 +1186,27 @@ static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, doubl
     hipLaunchKernelGGL((k_visc_update8<T, MODE, 12>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
 }
 
+// Planes per block of the marching sensor (F3D_SENSOR_KS; 0 = one plane per block, k_sensor_f).
+static int sensor_ks() {
+  static int v = -2;
+  if (v == -2) {
+    const char* e = std::getenv("F3D_SENSOR_KS");
+    v = e ? std::atoi(e) : 16;
+    if (v < 0) v = 16;
+  }
+  return v;
+}
+
 template <typename T>
 void fused_sensor(const KP& kp, const WsView<T>& ws, const T* Q, T* phi, hipStream_t s) {
   TimedScope ts("fused_sensor", s);
-  hipLaunchKernelGGL(k_sensor_f<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, kp.Nzt - 2), dim3(64, 4, 1), 0,
-                     s, kp, phi, Q, ws.hcount);
+  const int KS = sensor_ks();
+  if (KS > 0)
+    hipLaunchKernelGGL(k_sensor_march<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, (kp.Nzt - 2 + KS - 1) / KS),
+                       dim3(64, 4, 1), 0, s, kp, phi, Q, ws.hcount, KS);
+  else
+    hipLaunchKernelGGL(k_sensor_f<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, kp.Nzt - 2), dim3(64, 4, 1),
+                       0, s, kp, phi, Q, ws.hcount);
 }
 
 // Inviscid increments of the finite-difference branch into ws.R: x, y, z sweeps and the heavy-face
