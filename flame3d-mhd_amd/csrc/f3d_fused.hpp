@@ -109,4 +109,9 @@ template <typename T>
 int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, double* Un, T* Q,
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
              const T* const* geo = nullptr);
+// One FP64 RK3 step of the finite-difference branch as steps_io, copying the conserved state after
+// stage 1 into U1 and after stage 2 into U2 (the adjoint's stage states).  No filtering block.
+int step_keep_stages(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, double* Q,
+                     const double* const* metrics, const double* J, double dt, const double* inflow, double* U1,
+                     double* U2, hipStream_t s);
 }  // namespace f3d

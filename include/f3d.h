@@ -515,14 +515,17 @@ int f3d_rk3_stage_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t K
                           double dt, double* lam_inflow, double* lam_bc, f3d_stream_t s);
 
 /* Device bytes of the checkpoint buffer f3d_adjoint_f64 needs for `nsteps` with a checkpoint
- * every `every` steps: (ceil(nsteps/every) + every) conserved states.  Host-only. */
+ * every `every` steps: ceil(nsteps/every) + every + 2*(every == 1 ? nsteps : every) conserved
+ * states (checkpoints, the recomputed states of a segment, the final state, and the two stage
+ * states of each step of a segment).  Host-only. */
 size_t f3d_adjoint_ckpt_bytes(const f3d_params* p, int32_t nsteps, int32_t every);
 
 /* Checkpointed multi-step adjoint.  Forward: Q0 = ghosts(c2Prim(U0)), `nsteps` RK3 steps of
  * f3d_rk3_stage_f64 with a fixed dt, obj[n] = dt * sum over interior cells of W . Q^{n+1}
  * (W [Nxt,Nyt,Nzt,6], e.g. wall-face area on p of the first cell row above a wall).  Reverse:
- * U^n checkpointed every `every` steps, states inside a segment recomputed, the three stage
- * states of each step recomputed from U^n, stage VJPs 3,2,1.  Outputs: lamU0 = d(sum obj)/dU0
+ * U^n checkpointed every `every` steps; each segment recomputed once from its checkpoint, keeping
+ * the states and the stage states U^1, U^2 of its steps (every == 1: kept by the forward sweep);
+ * stage VJPs 3,2,1 at (U_s, c2Prim + ghosts of U_s).  Outputs: lamU0 = d(sum obj)/dU0
  * (interior; ghosts 0), lam_inflow / lam_bc accumulate d(sum obj)/d(inflow, q_inf, T_wall). */
 int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t nsteps, int32_t every,
                     const double* U0, const double* const* metrics, const double* J, double dt,

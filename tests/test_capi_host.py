@@ -114,10 +114,11 @@ def test_adjoint_entry_points_validate_before_any_launch():
     assert L.f3d_ghost_vjp_f64(ctypes.byref(p), None, None, None, None, None) == -1
     assert L.f3d_c2prim_vjp_f64(ctypes.byref(p), None, None, None, None) == -1
     assert b"NULL" in L.f3d_last_error()
-    # checkpoint buffer size: (ceil(n/every) + every) states of 5 variables
+    # checkpoint buffer size: ceil(n/every) + every states of 5 variables, plus two stage states per
+    # step of a segment (of every step when every == 1)
     nc = L.f3d_cell_count(ctypes.byref(p))
-    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 1) == (50 + 1) * 5 * nc * 8
-    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 10) == (5 + 10) * 5 * nc * 8
+    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 1) == (50 + 1 + 2 * 50) * 5 * nc * 8
+    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 10) == (5 + 10 + 2 * 10) * 5 * nc * 8
     assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 0, 1) == 0
 
 
@@ -145,7 +146,8 @@ def test_fvm_selector_and_entry_validation():
 
 def test_alternative_flags_validation():
     """split / character / les / lts are checked by f3d_params_check; the adjoint covers the default
-    Steger-Warming + characteristic + laminar + global-dt path only (rejected before any allocation)."""
+    Steger-Warming + characteristic + laminar + global-dt + unfiltered path only (rejected before any
+    allocation)."""
     L = f3d.lib()
     q = f3d.F3DParams()
     for field, bad in (("split", 4), ("split", -1), ("character", 2), ("les", 3), ("lts", 2)):
@@ -153,7 +155,7 @@ def test_alternative_flags_validation():
         setattr(q, field, bad)
         assert L.f3d_params_check(ctypes.byref(q)) == -4, field
     for kw in ({"split": f3d.SPLIT_LF}, {"split": f3d.SPLIT_AUSM}, {"character": 0}, {"les": f3d.LES_WALE},
-               {"lts": 1}):
+               {"lts": 1}, {"filtering": 1}):
         p = f3d.make_params(16, 12, 8, **kw)
         h = ctypes.c_void_p()
         assert L.f3d_adjoint_workspace_create(ctypes.byref(p), ctypes.byref(h)) == -4, kw
