@@ -488,10 +488,10 @@ int check_fdm(const f3d_params* p, const char* who) {
     set_error("%s: the adjoint covers the finite-difference branch only (params fvm = %d)", who, p->fvm);
     return F3D_E_ARG;
   }
-  if (p->split != F3D_SPLIT_SW || p->character != 1 || p->les != F3D_LES_NONE || p->lts) {
-    set_error("%s: the adjoint covers the Steger-Warming, characteristic, constant-viscosity, global-dt "
-              "path only (params split = %d, character = %d, les = %d, lts = %d)",
-              who, p->split, p->character, p->les, p->lts);
+  if (p->split != F3D_SPLIT_SW || p->character != 1 || p->les != F3D_LES_NONE || p->lts || p->filtering) {
+    set_error("%s: the adjoint covers the Steger-Warming, characteristic, constant-viscosity, global-dt, "
+              "unfiltered path only (params split = %d, character = %d, les = %d, lts = %d, filtering = %d)",
+              who, p->split, p->character, p->les, p->lts, p->filtering);
     return F3D_E_ARG;
   }
   return F3D_OK;
@@ -613,7 +613,13 @@ int stage_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, int KRK, const d
   return check_launch("stage vjp finalize");
 }
 
-size_t ckpt_slots(int nsteps, int every) { return (size_t)((nsteps + every - 1) / every) + (size_t)(every - 1) + 1; }
+// Conserved states in the checkpoint buffer: the checkpoints, the every-1 recomputed states of a segment,
+// the final state, and the two stage states (U^1, U^2) of each step of a segment (of every step when
+// every == 1, kept by the forward sweep itself).
+size_t stage_steps(int nsteps, int every) { return (size_t)(every == 1 ? nsteps : every); }
+size_t ckpt_slots(int nsteps, int every) {
+  return (size_t)((nsteps + every - 1) / every) + (size_t)(every - 1) + 1 + 2 * stage_steps(nsteps, every);
+}
 
 }  // namespace
 
@@ -783,26 +789,45 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
   auto slot = [&](size_t q) { return ckpt + q * 5 * nc; };
   double* seg0 = slot(nck);                 // every-1 recomputed states of a segment
   double* Ufin = slot(nck + every - 1);     // U^nsteps
+  double* stg0 = slot(nck + every);         // stage states U^1, U^2 per step (of the segment or of all steps)
   auto seg = [&](int q) { return q == 0 ? nullptr : seg0 + (size_t)(q - 1) * 5 * nc; };
+  auto stg = [&](int n, int n0, int q) {
+    const size_t idx = every == 1 ? (size_t)n : (size_t)(n - n0);
+    return stg0 + (2 * idx + (size_t)(q - 1)) * 5 * nc;
+  };
   auto obj_term = [&](const double* Qn, double* outp) {
     hipLaunchKernelGGL(k_obj_partial, dim3(aws->nred), dim3(256), 0, s, kp, Qn, W, aws->red);
     hipLaunchKernelGGL(k_obj_final, dim3(1), dim3(256), 0, s, aws->red, aws->nred, dt, outp);
   };
+  // c2Prim + ghost fill (the fills of periodic dimensions rewrite what exchange_ghost would copy)
   auto prim = [&](const double* Usrc, double* Qd) {
     int r = f3d_c2prim_f64(p, Usrc, Qd, s);
-    if (!r) r = f3d_exchange_ghost_f64(p, Qd, 6, s);
     if (!r) r = f3d_fill_ghost_f64(p, Qd, nullptr, inflow, s);
     return r;
   };
   auto step = [&](double* U, double* Q) {
     return steps_io<double>(p, aws->fws, 1, U, aws->Un, Q, metrics, J, dt, inflow, s);
   };
+  auto step_keep = [&](double* U, double* Q, double* U1, double* U2, bool third) {
+    return step_keep_stages(p, aws->fws, U, aws->Un, Q, metrics, J, dt, inflow, U1, U2, s, third);
+  };
+  const int nlast = (nck - 1) * every;  // first step of the last segment
   // ---- forward sweep: checkpoints every `every` steps, objective per step
   F3D_HIP(hipMemcpyAsync(aws->Ucur, U0, vb, hipMemcpyDeviceToDevice, s));
   rc = prim(aws->Ucur, aws->Qcur);
+  // (keeping the stage states of every step when every == 1, else the states and stage states of the
+  // last segment, whose reversal comes first)
   for (int n = 0; n < nsteps && !rc; ++n) {
     if (n % every == 0) F3D_HIP(hipMemcpyAsync(slot(n / every), aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
-    rc = step(aws->Ucur, aws->Qcur);
+    if (every == 1) {
+      rc = step_keep(aws->Ucur, aws->Qcur, stg(n, 0, 1), stg(n, 0, 2), true);
+    } else if (n >= nlast) {
+      rc = step_keep(aws->Ucur, aws->Qcur, stg(n, nlast, 1), stg(n, nlast, 2), true);
+      if (!rc && n + 1 < nsteps)
+        F3D_HIP(hipMemcpyAsync(seg(n + 1 - nlast), aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
+    } else {
+      rc = step(aws->Ucur, aws->Qcur);
+    }
     if (!rc) obj_term(aws->Qcur, obj + n);
   }
   if (rc) return rc;
@@ -811,13 +836,15 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
   hipLaunchKernelGGL(k_adj_zero, dim3(1024), dim3(256), 0, s, aws->lam, (long long)(5 * nc));
   for (int c = nck - 1; c >= 0 && !rc; --c) {
     const int n0 = c * every, n1 = (n0 + every < nsteps) ? n0 + every : nsteps;
-    // recompute U^{n0+1} .. U^{n1-1} of the segment
-    if (n1 - n0 > 1) {
+    // recompute the steps of the segment from its checkpoint, keeping U^{n0+1} .. U^{n1-1} and the
+    // stage states of every step; its last step needs stages 1 and 2 only (U^{n1} is checkpointed).
+    // The forward sweep kept all of this for every == 1 and for the last segment.
+    if (every > 1 && c < nck - 1) {
       F3D_HIP(hipMemcpyAsync(aws->Ucur, slot(c), vb, hipMemcpyDeviceToDevice, s));
       rc = prim(aws->Ucur, aws->Qcur);
-      for (int n = n0; n < n1 - 1 && !rc; ++n) {
-        rc = step(aws->Ucur, aws->Qcur);
-        if (!rc) F3D_HIP(hipMemcpyAsync(seg(n + 1 - n0), aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
+      for (int n = n0; n < n1 && !rc; ++n) {
+        rc = step_keep(aws->Ucur, aws->Qcur, stg(n, n0, 1), stg(n, n0, 2), n + 1 < n1);
+        if (!rc && n + 1 < n1) F3D_HIP(hipMemcpyAsync(seg(n + 1 - n0), aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
       }
     }
     for (int n = n1 - 1; n >= n0 && !rc; --n) {
@@ -827,13 +854,13 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
         TimedScope ts("adj_seed", s);
         hipLaunchKernelGGL(k_adj_seed, grid_interior(kp), kB, 0, s, kp, Unext, W, dt, aws->lam);
       }
-      // stage states of step n: (U^n, Q^n), (U^1, Q^1), (U^2, Q^2)
+      // stage states of step n: (U^n, Q^n), (U^1, Q^1), (U^2, Q^2); the stages' Q is c2Prim + ghost fill
+      // of their U, as the stage update itself forms it
       F3D_HIP(hipMemcpyAsync(aws->Us[0], Ustart, vb, hipMemcpyDeviceToDevice, s));
       rc = prim(aws->Us[0], aws->Qs[0]);
       for (int q = 1; q <= 2 && !rc; ++q) {
-        F3D_HIP(hipMemcpyAsync(aws->Us[q], aws->Us[q - 1], vb, hipMemcpyDeviceToDevice, s));
-        rc = stage_io<double>(p, aws->fws, q, aws->Us[q], aws->Un, aws->Qs[q - 1], aws->Qs[q], metrics, J, dt, inflow,
-                              s);
+        F3D_HIP(hipMemcpyAsync(aws->Us[q], stg(n, n0, q), vb, hipMemcpyDeviceToDevice, s));
+        rc = prim(aws->Us[q], aws->Qs[q]);
       }
       for (int KRK = 3; KRK >= 1 && !rc; --KRK)
         rc = stage_vjp_impl(p, aws, KRK, aws->lam, aws->lam, aws->lamUn, aws->Us[KRK - 1], aws->Qs[KRK - 1], metrics,

@@ -110,8 +110,9 @@ int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, doub
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
              const T* const* geo = nullptr);
 // One FP64 RK3 step of the finite-difference branch as steps_io, copying the conserved state after
-// stage 1 into U1 and after stage 2 into U2 (the adjoint's stage states).  No filtering block.
+// stage 1 into U1 and after stage 2 into U2 (the adjoint's stage states); third = false stops after
+// stage 2 (U, Q then hold the second stage's state).  No filtering block.
 int step_keep_stages(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, double* Q,
                      const double* const* metrics, const double* J, double dt, const double* inflow, double* U1,
-                     double* U2, hipStream_t s);
+                     double* U2, hipStream_t s, bool third = true);
 }  // namespace f3d

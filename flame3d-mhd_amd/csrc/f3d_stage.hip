@@ -303,7 +303,7 @@ int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, doub
 
 int step_keep_stages(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, double* Q,
                      const double* const* metrics, const double* J, double dt, const double* inflow, double* U1,
-                     double* U2, hipStream_t s) {
+                     double* U2, hipStream_t s, bool third) {
   const KP kp = make_kp(p);
   const bool rot = stage_needs_separate_q(kp);
   double* Qa = rot ? (double*)ws->Qa : Q;
@@ -313,7 +313,7 @@ int step_keep_stages(const f3d_params* p, f3d_workspace* ws, double* U, double* 
   if (!rc) F3D_HIP(hipMemcpyAsync(U1, U, vb, hipMemcpyDeviceToDevice, s));
   if (!rc) rc = stage_io<double>(p, ws, 2, U, Un, Qa, Qb, metrics, J, dt, inflow, s);
   if (!rc) F3D_HIP(hipMemcpyAsync(U2, U, vb, hipMemcpyDeviceToDevice, s));
-  if (!rc) rc = stage_io<double>(p, ws, 3, U, Un, Qb, Q, metrics, J, dt, inflow, s);
+  if (!rc && third) rc = stage_io<double>(p, ws, 3, U, Un, Qb, Q, metrics, J, dt, inflow, s);
   return rc;
 }
 
