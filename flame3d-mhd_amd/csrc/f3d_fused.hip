@@ -1077,8 +1077,8 @@ static int march_seg(int D, int es) {
   int& r = v[D - 1][es == 8];
   if (r < 0) {
     const char* e = std::getenv(D == 1 ? "F3D_MARCH_SEG_Y" : "F3D_MARCH_SEG_Z");
-    // defaults measured on the 512x192x64 ramp (one MI355X): FP64 y 48, z 16; FP32 y 24, z 32
-    const int def = es == 8 ? (D == 1 ? 48 : 16) : (D == 1 ? 24 : 32);
+    // defaults measured on the 512x192x64 ramp (one MI355X): FP64 y 48, z 32; FP32 y 24, z 32
+    const int def = es == 8 ? (D == 1 ? 48 : 32) : (D == 1 ? 24 : 32);
     r = e ? std::atoi(e) : def;
     if (r < 8) r = def;
   }
@@ -1090,9 +1090,8 @@ static int seg_len(int Np, int max_len) {
   return (Np + nseg - 1) / nseg;
 }
 
-// Planes per segment of the viscous-update march: 16 (FP64) or 32 (FP32-bulk, one 12-wave block per
-// CU) where that gives >= 384 blocks, shorter on small grids (each segment re-evaluates 4 planes of
-// viscous fluxes); F3D_VISC_KC overrides.
+// Planes per segment of the viscous-update march (each segment re-evaluates 4 planes of viscous fluxes),
+// shorter on small grids; F3D_VISC_KC overrides.
 static int visc_kc(const KP& kp, int rows, int es) {
   static int env = -2;
   if (env == -2) {
@@ -1101,8 +1100,11 @@ static int visc_kc(const KP& kp, int rows, int es) {
   }
   if (env >= 4) return seg_len(kp.Nzp, env);
   const long long cols = (long long)((kp.Nxp + VXI - 1) / VXI) * ((kp.Nyp + rows - 1) / rows);
-  int kc = es == 4 ? 32 : 16;
-  while (kc > 8 && cols * ((kp.Nzp + kc - 1) / kc) < 384) kc -= 4;
+  // FP32-bulk: 64 planes while that leaves >= 192 one-per-CU blocks (the 4 re-evaluated planes cost less
+  // than a few idle CUs on the 512x192x64 ramp: 424 vs 438 us at 32 planes); FP64: 16 planes, >= 384 blocks
+  int kc = es == 4 ? 64 : 16;
+  const long long minb = es == 4 ? 192 : 384;
+  while (kc > 8 && cols * ((kp.Nzp + kc - 1) / kc) < minb) kc = kc > 16 ? kc / 2 : kc - 4;
   return seg_len(kp.Nzp, kc);
 }
 
