@@ -1,18 +1,19 @@
 // Fused RHS / RK3 stage for gfx950.
 //
 // Per stage (solver.jl:416-444 with flowAdvance's finite-difference branch, solver.jl:133-188):
-//   k_sensor_f    shock sensor phi (schemes.jl:1-24); clears the heavy-face counter
+//   k_sensor_march shock sensor phi (schemes.jl:1-24), marching k; clears the heavy-face counter
 //   k_sweep_x     x faces: SW split of a 128-cell row segment into LDS, UP7 face fluxes, R = dF_x
 //   k_sweep_march y and z faces: one lane per (i, line), marching along the direction; the split
 //                 fluxes of each new cell are pushed into an 8-face shift register of UP7 sums, so
-//                 no split flux ever goes to HBM; R += dF_y, dF_z
+//                 no split flux ever goes to HBM; R += dF_y, dF_z (the R row loaded a step ahead)
 //   k_heavy       faces whose sensor maximum selects WENO7/WENO5/NND (appended to a compacted list
 //                 by the sweeps): split fluxes of the 8-cell stencil recomputed, characteristic
 //                 reconstruction (schemes.jl:756-1009), R += (F_scheme - F_UP7) on both cells
-//   k_viscous_march viscous fluxes (viscous.jl:1-134), reference layout, into the workspace; lanes
-//                 march k with the column's u,v,w,T in a 5-plane register window
-//   k_update      viscous divergence + conservative update + RK3 blend + c2Prim (div.jl:1-50,
-//                 utils.jl:2-70), marching k with the column's Fv_z in a 5-plane window
+//   k_visc_update_qt viscous fluxes (viscous.jl:1-134), their divergence (div.jl:14-48), the
+//                 conservative update, RK3 blend and c2Prim (utils.jl:2-70) in one k-march; the
+//                 viscous fluxes stay in LDS and registers
+//   k_viscous_march + k_update  the same as two kernels (viscous fluxes through the workspace), for
+//                 the LES models and the inviscid case
 // Splitting every face as UP7 first and correcting the rare non-UP7 faces from a compacted list
 // keeps the sweep waves free of the characteristic branch's divergence and registers.
 #include <cstdlib>
@@ -31,7 +32,7 @@ int xcd_remap() {
   return v;
 }
 
-// 1 unless F3D_VISC_FUSED=0: the viscous divergence computed inside the update march (k_visc_update)
+// 1 unless F3D_VISC_FUSED=0: the viscous divergence computed inside the update march (k_visc_update_qt)
 // instead of k_viscous_march + k_update (A/B measurement).
 static int visc_fused() {
   static int v = -1;
@@ -495,44 +496,9 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
 }
 
 // ------------------------------------------------------------------ viscous divergence + update
-// k_update with the viscous divergence computed in place: the viscous fluxes never go to HBM.
-// Block = 64 x ROWS lanes: ROWS interior rows of a 60-column tile plus 2 halo lanes each side,
-// marching k.  Each iteration, every lane evaluates the velocity/temperature gradient and
-// viscous_cell at plane p (2 ahead of the updated plane k) for its own cell; the lanes of the
-// first and last two rows also evaluate one y-halo cell (rows -2, -1 and ROWS, ROWS+1; Fv_y only).  Fv_x of the interior rows and
-// Fv_y of the interior columns are exchanged through LDS for the x/y divergence of plane p, kept
-// two iterations in a register ring; Fv_z enters a 5-plane register window for the z divergence
-// of plane k; u,v,w,T of the own column sit in a 5-plane window for the zeta derivatives (the halo
-// cell's come through L2).  Every wave updates cells, so the update's HBM streams keep all waves.
-// Summation order of the divergence as k_update.
-constexpr int VX = 64, VXI = 60, VYI = 4;
-
-template <typename T>
-__device__ __forceinline__ void visc_eval(const KP& kp, const FArgs<T>& a, long long o, const T* wz, int wzs,
-                                          long long sy, long long sz, const T (*wcol)[5], T out[3][4]) {
-  // wcol != nullptr: zeta derivatives from the register window; else from memory around o
-  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
-  const long long nc = kp.nc;
-  const int var[4] = {1, 2, 3, 5};
-  T g[4][3], val[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const T* f = a.Q + var[v] * nc + o;
-    g[v][0] = c1 * (f[-2] - f[2]) + c2 * (f[-1] - f[1]);
-    g[v][1] = c1 * (f[-2 * sy] - f[2 * sy]) + c2 * (f[-sy] - f[sy]);
-    if (wcol) {
-      g[v][2] = c1 * (wcol[v][0] - wcol[v][4]) + c2 * (wcol[v][1] - wcol[v][3]);
-      val[v] = wcol[v][2];
-    } else {
-      g[v][2] = c1 * (f[-2 * sz] - f[2 * sz]) + c2 * (f[-sz] - f[sz]);
-      val[v] = f[0];
-    }
-  }
-  T m[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) m[q] = a.m.m[q][o];
-  viscous_cell<T>(kp, val[0], val[1], val[2], val[3], g, m, (T)a.J[o], out);
-}
+// k_update with the viscous divergence computed in place (k_visc_update_qt below): the viscous fluxes
+// never go to HBM.  Tile: VX = 64 lanes over VXI = 60 interior columns plus 2 halo lanes each side.
+constexpr int VX = 64, VXI = 60;
 
 // Conservative update + RK3 blend + c2Prim of one interior cell (utils.jl:2-70, div.jl:40-48);
 // MODE 0 writes the RHS instead.
@@ -647,233 +613,17 @@ __device__ __forceinline__ void update_cell_ops(const KP& kp, const T* Rt, long 
   Q[oc + 5 * nc] = (T)(pr / (rho * kp.Rg));
 }
 
-template <typename T, int MODE, int ROWS>
-__global__ __launch_bounds__(VX * ROWS) void k_visc_update(KP kp, FArgs<T> a, double* __restrict__ U,
-                                                           double* __restrict__ Un, T* __restrict__ Q,
-                                                           double* __restrict__ Rout, double dt, int KC, int remap) {
-  // two buffers, alternating per plane: a wave writing plane p+1 never overwrites what a slower wave
-  // still reads of plane p, so one barrier per plane suffices
-  __shared__ T sFx2[2][4][ROWS][VX];       // Fv_x of the interior rows, all 64 lanes
-  __shared__ T sFy2[2][4][ROWS + 4][VXI];  // Fv_y of rows -2 .. ROWS+1, interior columns
-  const Blk B = xcd_block(remap);
-  const int lane = threadIdx.x, row = threadIdx.y;
-  const int NG = kp.NG;
-  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * ROWS + row;  // absolute cell of this lane
-  const bool hasH = row < 2 || row >= ROWS - 2;                          // lanes that also evaluate a y-halo cell
-  const int yh = row < 2 ? y - 2 : y + 2;
-  const int k0 = NG + B.z * KC;
-  const int nk = min(KC, NG + kp.Nzp - k0);
-  const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;  // last cell of the +-2 box
-  const bool xin = lane >= 2 && lane < 2 + VXI && x < NG + kp.Nxp;
-  const bool yin = y < NG + kp.Nyp;
-  const bool needFx = yin && x <= xmax;
-  const bool needFy = xin && y <= ymax;
-  const bool needH = hasH && xin && yh <= ymax;
-  const bool inner = xin && yin;
-  const int xc = min(x, xmax), yc = min(y, ymax), yhc = min(yh, ymax);  // in-bounds loads for idle lanes
-  const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
-  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
-  const long long col = cidx(kp, xc, yc, 0), colh = cidx(kp, xc, yhc, 0);
-  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
-  const int var[4] = {1, 2, 3, 5};
-  const int hrow = row < 2 ? row : row + 4;  // sFy row index of the halo cell
-  T w[4][5];   // u,v,w,T of the own column, planes p-2 .. p+2
-  T fz[4][5];  // Fv_z of the own column, planes p-4 .. p
-  T xy[3][4];  // x+y viscous divergence of planes p-2, p-1, p
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int q = 0; q < 5; ++q) w[v][q] = fz[v][q] = T(0);
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-#pragma unroll
-    for (int n = 0; n < 4; ++n) xy[q][n] = T(0);
-#pragma unroll
-  for (int q = 1; q < 5; ++q) {
-    const long long o = col + (long long)(k0 - 5 + q) * sz;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) w[v][q] = a.Q[o + var[v] * nc];
-  }
-#pragma unroll 1
-  for (int s = -4; s < nk; ++s) {
-    const int p = k0 + s + 2;
-    T(*sFx)[ROWS][VX] = sFx2[s & 1];
-    T(*sFy)[ROWS + 4][VXI] = sFy2[s & 1];
-    {
-      const long long o = col + (long long)(p + 2) * sz;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) w[v][q] = w[v][q + 1];
-        w[v][4] = a.Q[o + var[v] * nc];
-      }
-    }
-    {
-      T out[3][4];
-      if (needFx || needFy) {
-        visc_eval<T>(kp, a, col + (long long)p * sz, nullptr, 0, sy, sz, w, out);
-      } else {
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-          for (int n = 0; n < 4; ++n) out[d][n] = T(0);
-      }
-#pragma unroll
-      for (int n = 0; n < 4; ++n) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) fz[n][q] = fz[n][q + 1];
-        fz[n][4] = out[2][n];
-        if (needFx) sFx[n][row][lane] = out[0][n];
-        if (needFy) sFy[n][row + 2][lane - 2] = out[1][n];
-      }
-    }
-    if (needH) {
-      T out[3][4];
-      visc_eval<T>(kp, a, colh + (long long)p * sz, nullptr, 0, sy, sz, nullptr, out);
-#pragma unroll
-      for (int n = 0; n < 4; ++n) sFy[n][hrow][lane - 2] = out[1][n];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      xy[0][n] = xy[1][n];
-      xy[1][n] = xy[2][n];
-    }
-    if (inner) {
-      const int l = lane - 2, r = row + 2;
-#pragma unroll
-      for (int n = 0; n < 4; ++n) {
-        T d = c1 * (sFx[n][row][lane - 2] - sFx[n][row][lane + 2]) + c2 * (sFx[n][row][lane - 1] - sFx[n][row][lane + 1]);
-        d += c1 * (sFy[n][r - 2][l] - sFy[n][r + 2][l]) + c2 * (sFy[n][r - 1][l] - sFy[n][r + 1][l]);
-        xy[2][n] = d;
-      }
-    }
-    if (s < 0 || !inner) continue;
-    // ---- update of plane k = p - 2
-    const int k = k0 + s;
-    const long long oc = col + (long long)k * sz, ro = ridx(kp, x - NG, y - NG, k - NG);
-    T Rt[5];
-#pragma unroll
-    for (int n = 0; n < 5; ++n) Rt[n] = a.R[ro + n * rn];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      T sdiv = xy[0][n];
-      sdiv += c1 * (fz[n][0] - fz[n][4]) + c2 * (fz[n][1] - fz[n][3]);
-      Rt[n + 1] += sdiv;
-    }
-    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt, a.ldt);
-  }
-}
-
-// Same computation with the y-halo rows as lanes of their own: 64 x (VI + 4) lanes over a 60 x VI
-// tile (rows 0, 1, VI + 2, VI + 3 are halo rows, Fv_y only), each lane with its own u,v,w,T window.
-template <typename T, int MODE, int VI>
-__global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update8(KP kp, FArgs<T> a, double* __restrict__ U,
-                                                      double* __restrict__ Un, T* __restrict__ Q,
-                                                      double* __restrict__ Rout, double dt, int KC, int remap) {
-  __shared__ T sFx2[2][4][VI][VX];  // double-buffered per plane as in k_visc_update
-  __shared__ T sFy2[2][4][VI + 4][VXI];
-  const Blk B = xcd_block(remap);
-  const int lane = threadIdx.x, row = threadIdx.y;
-  const int NG = kp.NG;
-  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * VI - 2 + row;
-  const int k0 = NG + B.z * KC;
-  const int nk = min(KC, NG + kp.Nzp - k0);
-  const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;
-  const bool xin = lane >= 2 && lane < 2 + VXI && x < NG + kp.Nxp;
-  const bool yin = row >= 2 && row < 2 + VI && y < NG + kp.Nyp;
-  const bool needFx = yin && x <= xmax;
-  const bool needFy = xin && y <= ymax;
-  const bool inner = xin && yin;
-  const int xc = min(x, xmax), yc = min(y, ymax);
-  const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
-  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
-  const long long col = cidx(kp, xc, yc, 0);
-  const T c1 = T(1) / T(12), c2 = T(-2) / T(3);
-  const int var[4] = {1, 2, 3, 5};
-  T w[4][5], fz[4][5], xy[3][4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int q = 0; q < 5; ++q) w[v][q] = fz[v][q] = T(0);
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-#pragma unroll
-    for (int n = 0; n < 4; ++n) xy[q][n] = T(0);
-#pragma unroll
-  for (int q = 1; q < 5; ++q) {
-    const long long o = col + (long long)(k0 - 5 + q) * sz;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) w[v][q] = a.Q[o + var[v] * nc];
-  }
-#pragma unroll 1
-  for (int s = -4; s < nk; ++s) {
-    const int p = k0 + s + 2;
-    T(*sFx)[VI][VX] = sFx2[s & 1];
-    T(*sFy)[VI + 4][VXI] = sFy2[s & 1];
-    {
-      const long long o = col + (long long)(p + 2) * sz;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) w[v][q] = w[v][q + 1];
-        w[v][4] = a.Q[o + var[v] * nc];
-      }
-    }
-    T out[3][4];
-    if (needFx || needFy) {
-      visc_eval<T>(kp, a, col + (long long)p * sz, nullptr, 0, sy, sz, w, out);
-    } else {
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int n = 0; n < 4; ++n) out[d][n] = T(0);
-    }
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) fz[n][q] = fz[n][q + 1];
-      fz[n][4] = out[2][n];
-      if (needFx) sFx[n][row - 2][lane] = out[0][n];
-      if (needFy) sFy[n][row][lane - 2] = out[1][n];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      xy[0][n] = xy[1][n];
-      xy[1][n] = xy[2][n];
-    }
-    if (inner) {
-      const int r = row - 2, l = lane - 2;
-#pragma unroll
-      for (int n = 0; n < 4; ++n) {
-        T d = c1 * (sFx[n][r][lane - 2] - sFx[n][r][lane + 2]) + c2 * (sFx[n][r][lane - 1] - sFx[n][r][lane + 1]);
-        d += c1 * (sFy[n][row - 2][l] - sFy[n][row + 2][l]) + c2 * (sFy[n][row - 1][l] - sFy[n][row + 1][l]);
-        xy[2][n] = d;
-      }
-    }
-    if (s < 0 || !inner) continue;
-    const int k = k0 + s;
-    const long long oc = col + (long long)k * sz, ro = ridx(kp, x - NG, y - NG, k - NG);
-    T Rt[5];
-#pragma unroll
-    for (int n = 0; n < 5; ++n) Rt[n] = a.R[ro + n * rn];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      T sdiv = xy[0][n];
-      sdiv += c1 * (fz[n][0] - fz[n][4]) + c2 * (fz[n][1] - fz[n][3]);
-      Rt[n + 1] += sdiv;
-    }
-    update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt, a.ldt);
-  }
-}
-
-// k_visc_update8 with the xi and eta neighbours of u,v,w,T taken from a plane tile in LDS instead of
-// per-lane loads: every lane stores its column's plane-p values (from its register window) into the
-// tile, the lanes at the x edges add the two cells beyond (cells x0-4, x0-3 and x0+62, x0+63) and the
-// halo rows the two rows beyond (y0-4, y0-3 and y0+VI+2, y0+VI+3); after a barrier the 4th-order
-// xi/eta differences read the tile (same formulas and summation order as visc_eval).  Per lane and
-// plane that replaces 32 neighbour loads by 4 (window) + 4 (edges / extra rows, some lanes only).
+// Block = 64 x (VI + 4) lanes over a 60 x VI tile of interior cells; rows 0, 1, VI + 2, VI + 3 are
+// y-halo rows (waves of their own, Fv_y only).  The block marches k: at iteration p (2 ahead of the
+// updated plane k) every lane evaluates the velocity/temperature gradient and viscous_cell for its cell;
+// Fv_x of the interior rows and Fv_y of the interior columns are exchanged through LDS for the x/y
+// divergence of plane p, kept two iterations in a register ring; Fv_z enters a 5-plane register window for
+// the z divergence of plane k; u,v,w,T of the column sit in a 5-plane window for the zeta derivatives.
+// The xi and eta neighbours of u,v,w,T come from a plane tile in LDS: every lane stores its column's
+// plane-p values (from its window) into the tile, the lanes at the x edges add the two cells beyond
+// (x0-4, x0-3 and x0+62, x0+63) and the halo rows the two rows beyond (y0-4, y0-3 and y0+VI+2,
+// y0+VI+3); after a barrier the 4th-order xi/eta differences read the tile.  Summation order of the
+// divergence as k_update.
 // The tile is written before barrier A and read before barrier B (the viscous-flux exchange), so one
 // buffer suffices; likewise the flux buffers, written after A and read after B.
 template <typename T, int MODE, int VI, bool PF>
@@ -1112,12 +862,9 @@ static dim3 visc_grid(const KP& kp, int KC, int rows) {
   return dim3((kp.Nxp + VXI - 1) / VXI, (kp.Nyp + rows - 1) / rows, (kp.Nzp + KC - 1) / KC);
 }
 
-// Form of the viscous-update march: 0 = k_visc_update (4 interior rows, the y-halo cells evaluated
-// by the lanes of the first and last two rows); VI = 4, 8, 12 = k_visc_update_qt (or, with
-// F3D_VISC_QT=0, k_visc_update8) with VI interior rows and the halo rows as lanes of their own.
-// Defaults measured on the 512x192x64 ramp (one MI355X): FP64 4 and FP32-bulk 8, both with the
-// operand prefetch (F3D_VISC_QT=2: 242 / 151 VGPRs, one 8- / 12-wave block per CU); F3D_VISC_FORM
-// overrides.
+// Form of the viscous-update march: VI = 4, 8, 12 interior rows per block (plus 4 halo rows).  Defaults
+// measured on the 512x192x64 ramp (one MI355X): FP64 4 and FP32-bulk 8, both with the operand prefetch
+// (242 / 151 VGPRs: one 8- / 12-wave block per CU); F3D_VISC_FORM overrides.
 static int visc_form(int es) {
   static int v[2] = {-2, -2};
   int& r = v[es == 8];
@@ -1125,27 +872,22 @@ static int visc_form(int es) {
     const char* e = std::getenv("F3D_VISC_FORM");
     const int def = es == 8 ? 4 : 8;
     r = e ? std::atoi(e) : def;
-    if (r != 0 && r != 4 && r != 8 && r != 12) r = def;
+    if (r != 4 && r != 8 && r != 12) r = def;
     if (es == 8 && r == 12) r = 8;  // 16 FP64 waves of ~160 VGPRs do not fit one CU
   }
   return r;
 }
 
-static int visc_rows(int es) {
-  const int f = visc_form(es);
-  return f == 0 ? VYI : f;
-}
+static int visc_rows(int es) { return visc_form(es); }
 
-// LDS-tile variant k_visc_update_qt for the forms VI = 4, 8, 12: 2 (default) with the next plane's
-// metrics and update operands prefetched (VI = 4, 8), 1 without; 0 selects k_visc_update8 (per-lane
-// neighbour loads) for A/B measurement.  F3D_VISC_QT overrides.
-static int visc_qt(int es) {
+// 1 (default): the next plane's metrics and update operands prefetched an iteration ahead (forms 4 and 8
+// in FP32-bulk, 4 in FP64, whose 12-wave prefetch variant would spill); 0 without.  F3D_VISC_PF overrides.
+static int visc_pf(int es) {
   static int v[2] = {-2, -2};
   int& r = v[es == 8];
   if (r == -2) {
-    const char* e = std::getenv("F3D_VISC_QT");
-    r = e ? std::atoi(e) : 2;
-    if (r < 0 || r > 2) r = 2;
+    const char* e = std::getenv("F3D_VISC_PF");
+    r = e ? (std::atoi(e) != 0) : 1;
   }
   return r;
 }
@@ -1155,9 +897,7 @@ static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, doubl
                                int KC, int rm, hipStream_t s) {
   const int f = visc_form((int)sizeof(T));
   const dim3 g = visc_grid(kp, KC, visc_rows((int)sizeof(T)));
-  const int qt = visc_qt((int)sizeof(T));
-  // (the FP64 prefetch variant of the 12-wave form spills registers: not used)
-  if ((f == 4 || (f == 8 && sizeof(T) == 4)) && qt == 2) {
+  if (visc_pf((int)sizeof(T)) && (f == 4 || (f == 8 && sizeof(T) == 4))) {
     if (f == 4)
       hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, true>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
                          rm);
@@ -1166,26 +906,15 @@ static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, doubl
                          KC, rm);
     return;
   }
-  if (f != 0 && qt) {
-    if (f == 4)
-      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, false>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
-                         rm);
-    else if (f == 8)
-      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, false>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                         KC, rm);
-    else if constexpr (sizeof(T) == 4)
-      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 12, false>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                         KC, rm);
-    return;
-  }
-  if (f == 0)
-    hipLaunchKernelGGL((k_visc_update<T, MODE, VYI>), g, dim3(VX, VYI, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
-  else if (f == 4)
-    hipLaunchKernelGGL((k_visc_update8<T, MODE, 4>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
+  if (f == 4)
+    hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, false>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
+                       rm);
   else if (f == 8)
-    hipLaunchKernelGGL((k_visc_update8<T, MODE, 8>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
+    hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, false>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                       KC, rm);
   else if constexpr (sizeof(T) == 4)
-    hipLaunchKernelGGL((k_visc_update8<T, MODE, 12>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC, rm);
+    hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 12, false>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                       KC, rm);
 }
 
 // Planes per block of the marching sensor (F3D_SENSOR_KS; 0 = one plane per block, k_sensor_f).
