@@ -385,6 +385,93 @@ __global__ __launch_bounds__(256) void k_adj_ghost(KP kp, int side, int bc, doub
   }
 }
 
+// k_adj_ghost<0> with the four ghost cells of a line on four neighbouring lanes (a wave covers 16 lines:
+// 32-byte runs instead of one cache line per lane).  Each lane clears its ghost cell; where the four
+// ghosts fold into distinct sources (periodic, walls) each lane adds its own, where they share one
+// target (zero gradient, inflow, Dirichlet) lane 0 of the line adds them in the order of k_adj_ghost.
+__global__ __launch_bounds__(256) void k_adj_ghost_x(KP kp, int side, int bc, double* __restrict__ lamQ,
+                                                     const double* __restrict__ Q, double* __restrict__ lam_inflow,
+                                                     double* __restrict__ lam_bc) {
+  const int l = threadIdx.x & 3, a = blockIdx.x * 16 + (threadIdx.x >> 2), b = blockIdx.y * 4 + threadIdx.y;
+  const bool act = a < kp.Nyt && b < kp.Nzt;
+  const int NG = kp.NG, Np = kp.Nxp;
+  const long long nc = kp.nc;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int g = side ? Np + NG + l : l;
+  const long long o = act ? cidx(kp, g, a, b) : 0;
+  double lg[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    lg[n] = act ? lamQ[o + n * nc] : 0.0;
+    if (act) lamQ[o + n * nc] = 0.0;
+  }
+  // the four ghosts of the line, for lane 0 (shuffles by every lane: the line's lanes are one quad)
+  double all[4][6];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int n = 0; n < 6; ++n) all[q][n] = __shfl(lg[n], (threadIdx.x & ~3) | q, 64);
+  if (act) {
+    if (bc == F3D_BC_PERIODIC) {
+      const long long sidx = cidx(kp, side ? g - Np : g + Np, a, b);
+#pragma unroll
+      for (int n = 0; n < 6; ++n) lamQ[sidx + n * nc] += lg[n];
+    } else if (bc == F3D_BC_ZEROGRAD || bc == F3D_BC_INFLOW || bc == F3D_BC_DIRICHLET) {
+      if (l == 0) {
+        if (bc == F3D_BC_ZEROGRAD) {
+          const long long sidx = cidx(kp, side ? Np + NG - 1 : NG, a, b);
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int n = 0; n < 6; ++n) lamQ[sidx + n * nc] += all[q][n];
+        } else if (bc == F3D_BC_DIRICHLET) {
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int n = 0; n < 6; ++n) acc[n] += all[q][n];
+        } else if (lam_inflow) {
+          const long long io = a + (long long)kp.Nyt * b, in = (long long)kp.Nyt * kp.Nzt;
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int n = 0; n < 6; ++n) lam_inflow[io + n * in] += all[q][n];
+        }
+      }
+    } else {
+      const int mi = side ? 2 * (Np + NG) - 1 - g : 2 * NG - 1 - g;
+      const long long sidx = cidx(kp, mi, a, b);
+      double gr = lg[0], gpv = lg[4], gT = lg[5];
+      if (bc == F3D_BC_WALL_ISOTHERMAL) {
+        const double q0 = Q[o], q5 = Q[o + 5 * nc];
+        gpv += lg[0] / (kp.Rg * q5);
+        const double gq5 = lg[5] - lg[0] * q0 / q5;
+        gT = -gq5;
+        acc[6] += 2 * gq5;
+        gr = 0.0;
+      }
+      lamQ[sidx] += gr;
+      lamQ[sidx + nc] -= lg[1];
+      lamQ[sidx + 2 * nc] -= lg[2];
+      lamQ[sidx + 3 * nc] -= lg[3];
+      lamQ[sidx + 4 * nc] += gpv;
+      lamQ[sidx + 5 * nc] += gT;
+    }
+  }
+  if (bc == F3D_BC_DIRICHLET || bc == F3D_BC_WALL_ISOTHERMAL) {
+    __shared__ double sm[7][4];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+#pragma unroll
+    for (int n = 0; n < 7; ++n) {
+      double v = acc[n];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      if ((tid & 63) == 0) sm[n][tid >> 6] = v;
+    }
+    __syncthreads();
+    if (tid < 7 && lam_bc) {
+      const double v = sm[tid][0] + sm[tid][1] + sm[tid][2] + sm[tid][3];
+      if (v != 0.0) atomicAdd(&lam_bc[tid], v);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ c2Prim + RK3 blend, reverse
 // lam_s = a * lam_out + c2Prim^T(lamQ) at U_s (+ lamUn for KRK 1);  lamUn = b*lam_out (KRK 3),
 // += b*lam_out (KRK 2).  lam_s may alias lam_out.
@@ -577,7 +664,7 @@ int ghost_vjp_impl(const f3d_params* p, double* lamQ, const double* Q, double* l
   TimedScope ts("adj_ghost", s);
   auto launch = [&](int d, int side, int bc) {
     if (d == 0)
-      hipLaunchKernelGGL(k_adj_ghost<0>, dim3((kp.Nyt + 63) / 64, (kp.Nzt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
+      hipLaunchKernelGGL(k_adj_ghost_x, dim3((kp.Nyt + 15) / 16, (kp.Nzt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
                          lam_inflow, lam_bc);
     else if (d == 1)
       hipLaunchKernelGGL(k_adj_ghost<1>, dim3((kp.Nxt + 63) / 64, (kp.Nzt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
@@ -586,11 +673,11 @@ int ghost_vjp_impl(const f3d_params* p, double* lamQ, const double* Q, double* l
       hipLaunchKernelGGL(k_adj_ghost<2>, dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
                          lam_inflow, lam_bc);
   };
+  // reverse of fill_z, fill_y, fill_x.  The reverse of exchange_ghost (mpi.jl, one rank) is not launched:
+  // each reverse fill clears its whole slab, so every ghost adjoint is zero by then and the periodic
+  // folds of the exchange would add zeros.
   for (int d = 2; d >= 0; --d)
     for (int side = 1; side >= 0; --side) launch(d, side, p->bc[2 * d + side]);
-  for (int d = 2; d >= 0; --d)
-    if (p->bc[2 * d] == F3D_BC_PERIODIC && p->bc[2 * d + 1] == F3D_BC_PERIODIC)
-      for (int side = 1; side >= 0; --side) launch(d, side, F3D_BC_PERIODIC);
   return check_launch("ghost vjp");
 }
 
