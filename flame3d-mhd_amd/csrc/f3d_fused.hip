@@ -840,70 +840,98 @@ static int seg_len(int Np, int max_len) {
   return (Np + nseg - 1) / nseg;
 }
 
-// Planes per segment of the viscous-update march (each segment re-evaluates 4 planes of viscous fluxes),
-// shorter on small grids; F3D_VISC_KC overrides.
-static int visc_kc(const KP& kp, int rows, int es) {
-  static int env = -2;
-  if (env == -2) {
-    const char* e = std::getenv("F3D_VISC_KC");
-    env = e ? std::atoi(e) : -1;
-  }
-  if (env >= 4) return seg_len(kp.Nzp, env);
-  const long long cols = (long long)((kp.Nxp + VXI - 1) / VXI) * ((kp.Nyp + rows - 1) / rows);
-  // FP32-bulk: 64 planes while that leaves >= 192 one-per-CU blocks (the 4 re-evaluated planes cost less
-  // than a few idle CUs on the 512x192x64 ramp: 424 vs 438 us at 32 planes); FP64: 16 planes, >= 384 blocks
-  int kc = es == 4 ? 64 : 16;
-  const long long minb = es == 4 ? 192 : 384;
-  while (kc > 8 && cols * ((kp.Nzp + kc - 1) / kc) < minb) kc = kc > 16 ? kc / 2 : kc - 4;
-  return seg_len(kp.Nzp, kc);
-}
-
 static dim3 visc_grid(const KP& kp, int KC, int rows) {
   return dim3((kp.Nxp + VXI - 1) / VXI, (kp.Nyp + rows - 1) / rows, (kp.Nzp + KC - 1) / KC);
 }
 
-// Form of the viscous-update march: VI = 4, 8, 12 interior rows per block (plus 4 halo rows).  Defaults
-// measured on the 512x192x64 ramp (one MI355X): FP64 4 and FP32-bulk 8, both with the operand prefetch
-// (242 / 151 VGPRs: one 8- / 12-wave block per CU); F3D_VISC_FORM overrides.
-static int visc_form(int es) {
-  static int v[2] = {-2, -2};
-  int& r = v[es == 8];
-  if (r == -2) {
-    const char* e = std::getenv("F3D_VISC_FORM");
-    const int def = es == 8 ? 4 : 8;
-    r = e ? std::atoi(e) : def;
-    if (r != 4 && r != 8 && r != 12) r = def;
-    if (es == 8 && r == 12) r = 8;  // 16 FP64 waves of ~160 VGPRs do not fit one CU
-  }
-  return r;
+static int env_int(const char* name, int def) {
+  const char* e = std::getenv(name);
+  return e ? std::atoi(e) : def;
 }
 
-static int visc_rows(int es) { return visc_form(es); }
+// Compute units of the current device (one viscous-update block per CU).
+static int cu_count() {
+  static int n[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (n[dev] == 0) {
+    int v = 0;
+    n[dev] = (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
+  }
+  return n[dev];
+}
 
-// 1 (default): the next plane's metrics and update operands prefetched an iteration ahead (forms 4 and 8
-// in FP32-bulk, 4 in FP64, whose 12-wave prefetch variant would spill); 0 without.  F3D_VISC_PF overrides.
+// 1 (default): the next plane's metrics and update operands prefetched an iteration ahead; 0 without
+// (A/B measurement).  F3D_VISC_PF overrides.
 static int visc_pf(int es) {
   static int v[2] = {-2, -2};
   int& r = v[es == 8];
-  if (r == -2) {
-    const char* e = std::getenv("F3D_VISC_PF");
-    r = e ? (std::atoi(e) != 0) : 1;
-  }
+  if (r == -2) r = env_int("F3D_VISC_PF", 1) != 0;
   return r;
+}
+
+// Tile of the viscous-update march: VI interior rows per block (plus 4 halo rows, every row a wave) and
+// KC planes per segment (each segment re-evaluates 4 planes of viscous fluxes).  One block runs per CU
+// (FP64 VI = 4 with the prefetch: 242 VGPRs, two waves per SIMD; FP32-bulk VI = 6-8: 152 VGPRs, three),
+// so the march's duration is about rounds x (VI + 4) waves x (KC + 4) iterations, rounds = ceil(blocks /
+// CUs).  FP32-bulk picks VI in 6..8 and KC in 16/32/64 minimising that (on the 512x192x64 ramp VI = 7,
+// KC = 64: 252 blocks for 256 CUs, 407 us against 428 for VI = 8); FP64 keeps VI = 4 and KC = 16 where
+// that leaves >= 384 blocks (759 us against 796 at 32 planes), shorter on small grids.  F3D_VISC_FORM
+// (4, 6, 7, 8, 12) and F3D_VISC_KC override.
+struct ViscTile {
+  int VI, KC;
+};
+static ViscTile visc_tile(const KP& kp, int es) {
+  const int fenv = env_int("F3D_VISC_FORM", 0), kenv = env_int("F3D_VISC_KC", 0);
+  auto cols = [&](int vi) { return (long long)((kp.Nxp + VXI - 1) / VXI) * ((kp.Nyp + vi - 1) / vi); };
+  if (es == 8) {
+    int vi = (fenv == 4 || fenv == 8) ? fenv : 4;
+    int kc = 16;
+    while (kc > 8 && cols(vi) * ((kp.Nzp + kc - 1) / kc) < 384) kc -= 4;
+    return ViscTile{vi, seg_len(kp.Nzp, kenv >= 4 ? kenv : kc)};
+  }
+  const int ncu = cu_count();
+  ViscTile best{8, 16};
+  double bestc = -1;
+  const int vis[3] = {8, 7, 6};
+  const int kcs[3] = {64, 32, 16};
+  for (int a = 0; a < 3; ++a) {
+    const int vi = fenv ? fenv : vis[a];
+    for (int b = 0; b < 3; ++b) {
+      const int kc = seg_len(kp.Nzp, kenv >= 4 ? kenv : kcs[b]);
+      const long long blocks = cols(vi) * ((kp.Nzp + kc - 1) / kc);
+      const double c = (double)((blocks + ncu - 1) / ncu) * (vi + 4) * (kc + 4);
+      if (bestc < 0 || c < bestc * 0.999) {
+        bestc = c;
+        best = ViscTile{vi, kc};
+      }
+    }
+    if (fenv) break;
+  }
+  if (best.VI != 4 && best.VI != 6 && best.VI != 7 && best.VI != 8 && best.VI != 12) best.VI = 8;
+  return best;
 }
 
 template <typename T, int MODE>
 static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt,
-                               int KC, int rm, hipStream_t s) {
-  const int f = visc_form((int)sizeof(T));
-  const dim3 g = visc_grid(kp, KC, visc_rows((int)sizeof(T)));
-  if (visc_pf((int)sizeof(T)) && (f == 4 || (f == 8 && sizeof(T) == 4))) {
+                               const ViscTile& vt, int rm, hipStream_t s) {
+  const int f = vt.VI, KC = vt.KC;
+  const dim3 g = visc_grid(kp, KC, f);
+  if (visc_pf((int)sizeof(T)) && (f == 4 || (f != 12 && sizeof(T) == 4))) {
     if (f == 4)
       hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, true>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
                          rm);
-    else if constexpr (sizeof(T) == 4)
-      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, true>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                         KC, rm);
+    else if constexpr (sizeof(T) == 4) {
+      if (f == 6)
+        hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 6, true>), g, dim3(VX, 10, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                           KC, rm);
+      else if (f == 7)
+        hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 7, true>), g, dim3(VX, 11, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                           KC, rm);
+      else
+        hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, true>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
+                           KC, rm);
+    }
     return;
   }
   if (f == 4)
@@ -1011,7 +1039,7 @@ int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double*
   }
   TimedScope ts("fused_update", s);
   if (kp.viscous && inplace) {
-    const int KV = visc_kc(kp, visc_rows((int)sizeof(T)), (int)sizeof(T));
+    const ViscTile KV = visc_tile(kp, (int)sizeof(T));
     if (mode == 0)
       launch_visc_update<T, 0>(kp, a, nullptr, nullptr, nullptr, Rout, 0.0, KV, rm, s);
     else if (mode == 1)
