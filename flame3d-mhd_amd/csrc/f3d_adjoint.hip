@@ -56,7 +56,6 @@ __global__ __launch_bounds__(256) void k_adj_zero(double* __restrict__ a, long l
     a[t] = 0.0;
 }
 
-__global__ void k_adj_zero_count(unsigned int* c) { *c = 0; }
 
 __global__ __launch_bounds__(256) void k_adj_lambda(KP kp, const double* __restrict__ lam, const double* __restrict__ J,
                                                     double scale, double* __restrict__ Lam) {
@@ -72,7 +71,7 @@ __global__ __launch_bounds__(256) void k_adj_lambda(KP kp, const double* __restr
 template <int D>
 __global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __restrict__ phi,
                                                         unsigned long long* __restrict__ list,
-                                                        unsigned int* __restrict__ count) {
+                                                        unsigned int* __restrict__ count, HSeg hs) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   const int nx = kp.Nxp + (D == 0), ny = kp.Nyp + (D == 1);
   const bool act = i < nx && j < ny;
@@ -87,7 +86,11 @@ __global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __r
     heavy = sch != F3D_SCHEME_UP7;
     entry = ((unsigned long long)o << 4) | ((unsigned long long)sch << 2) | (unsigned long long)D;
   }
-  append_heavy(list, count, heavy, entry);
+  // the wave's run of the list (64 x 4 blocks, one face per lane)
+  const unsigned int w = launch_wave(4, threadIdx.y);
+  unsigned int cnt = 0;
+  run_append(list, hs.base + (unsigned long long)w * hs.cap, cnt, heavy, entry);
+  if (threadIdx.x == 0) count[hs.first + w] = cnt;
 }
 
 // ------------------------------------------------------------------ inviscid, dense part
@@ -145,11 +148,10 @@ __global__ __launch_bounds__(256) void k_adj_inviscid(KP kp, const double* __res
 __global__ __launch_bounds__(64) void k_adj_heavy(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
                                                   const double* __restrict__ Lam,
                                                   const unsigned long long* __restrict__ list,
-                                                  const unsigned int* __restrict__ count, double* __restrict__ lamQ) {
-  const unsigned int n = *count;
+                                                  const unsigned int* __restrict__ count, double* __restrict__ lamQ,
+                                                  HList hl) {
   const long long nc = kp.nc;
-  for (unsigned int e = blockIdx.x * 64 + threadIdx.x; e < n; e += gridDim.x * 64) {
-    const unsigned long long v = list[e];
+  for_each_heavy(hl, count, list, [&](const unsigned long long v) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4), st = dstride(kp, D);
     double gF[5];
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy(KP kp, const double* __restric
       gF[q] = Lam[o + q * nc] - Lam[o - st + q * nc];  // face is the left face of o, the right one of o-st
       any |= gF[q] != 0.0;
     }
-    if (!any) continue;
+    if (!any) return;
     const double* A1p = mp.m[3 * D];
     const double* A2p = mp.m[3 * D + 1];
     const double* A3p = mp.m[3 * D + 2];
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy(KP kp, const double* __restric
       atomicAdd(&lamQ[oL + var[q] * nc], gqL[q]);
       atomicAdd(&lamQ[o + var[q] * nc], gqR[q]);
     }
-  }
+  });
 }
 
 // ------------------------------------------------------------------ viscous
@@ -613,23 +615,40 @@ int check_metrics(const double* const* metrics, const double* J, const char* who
 }
 
 // lamQ = (dR/dQ)^T (scale * lam) over all cells of Q (RHS per unit dt, R = J * sum of differences).
+// Grid of the list kernel of direction D: one lane per face of the interior face range.
+dim3 adj_list_grid(const KP& kp, int D) {
+  return dim3((kp.Nxp + (D == 0) + 63) / 64, (kp.Nyp + (D == 1) + 3) / 4, kp.Nzp + (D == 2));
+}
+
+}  // namespace
+
+HList f3d::adj_heavy_layout(const KP& kp) {
+  unsigned long long nw[3], cap[3];
+  for (int d = 0; d < 3; ++d) {
+    const dim3 g = adj_list_grid(kp, d);
+    nw[d] = 4ULL * g.x * g.y * g.z;
+    cap[d] = 64;
+  }
+  return make_hlist(nw, cap);
+}
+
+namespace {
+
 int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam, double scale, const double* Q,
                  const double* const* metrics, const double* J, double* lamQ, hipStream_t s) {
   const KP kp = make_kp(p);
   MetricPtrs<double> mp;
   for (int q = 0; q < 9; ++q) mp.m[q] = metrics[q];
   const WsView<double> fv = ws_view<double>(aws->fws);
-  int rc = launch_shock_sensor<double>(kp, fv.phi, Q, s);
+  const HList hl = f3d::adj_heavy_layout(kp);
+  int rc = check_hlist(fv, hl, "rhs vjp");
+  if (!rc) rc = launch_shock_sensor<double>(kp, fv.phi, Q, s);
   if (rc) return rc;
   {
     TimedScope ts("adj_heavy_list", s);
-    hipLaunchKernelGGL(k_adj_zero_count, dim3(1), dim3(1), 0, s, fv.hcount);
-    hipLaunchKernelGGL(k_adj_heavy_list<0>, dim3((kp.Nxp + 1 + 63) / 64, (kp.Nyp + 3) / 4, kp.Nzp), kB, 0, s, kp,
-                       fv.phi, fv.heavy, fv.hcount);
-    hipLaunchKernelGGL(k_adj_heavy_list<1>, dim3((kp.Nxp + 63) / 64, (kp.Nyp + 1 + 3) / 4, kp.Nzp), kB, 0, s, kp,
-                       fv.phi, fv.heavy, fv.hcount);
-    hipLaunchKernelGGL(k_adj_heavy_list<2>, dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, kp.Nzp + 1), kB, 0, s, kp,
-                       fv.phi, fv.heavy, fv.hcount);
+    hipLaunchKernelGGL(k_adj_heavy_list<0>, adj_list_grid(kp, 0), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl.d[0]);
+    hipLaunchKernelGGL(k_adj_heavy_list<1>, adj_list_grid(kp, 1), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl.d[1]);
+    hipLaunchKernelGGL(k_adj_heavy_list<2>, adj_list_grid(kp, 2), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl.d[2]);
   }
   {
     TimedScope ts("adj_lambda", s);
@@ -641,7 +660,8 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
   }
   {
     TimedScope ts("adj_heavy", s);
-    hipLaunchKernelGGL(k_adj_heavy, dim3(1024), dim3(64), 0, s, kp, Q, mp, aws->Lam, fv.heavy, fv.hcount, lamQ);
+    const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
+    hipLaunchKernelGGL(k_adj_heavy, dim3(nb), dim3(64), 0, s, kp, Q, mp, aws->Lam, fv.heavy, fv.hcount, lamQ, hl);
   }
   if (kp.viscous) {
     {
@@ -763,7 +783,7 @@ F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace*
   hipLaunchKernelGGL(k_adj_zero, dim3(1024), dim3(256), 0, 0, (double*)base, (long long)(total / 8));
   e = hipDeviceSynchronize();
   if (e != hipSuccess) {
-    hipFree(base);
+    (void)hipFree(base);
     f3d_workspace_destroy(fws);
     delete a;
     set_error("adjoint_workspace_create: zero fill failed: %s", hipGetErrorString(e));

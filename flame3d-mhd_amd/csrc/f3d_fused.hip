@@ -76,10 +76,8 @@ __device__ __forceinline__ T up7c(const KP& kp, int m) {
 
 // ------------------------------------------------------------------ sensor (+ counter reset)
 template <typename T>
-__global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, const T* __restrict__ Q,
-                                                  unsigned int* __restrict__ hcount) {
+__global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, const T* __restrict__ Q) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y, k = 1 + blockIdx.z;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hcount = 0;
   if (i > kp.Nxt - 2 || j > kp.Nyt - 2 || k > kp.Nzt - 2) return;
   const T* P = Q + 4 * kp.nc;
   const long long o = cidx(kp, i, j, k), sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
@@ -90,11 +88,9 @@ __global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, co
 // column at k-1, k, k+1 stays in registers, so each cell costs one new plane load plus the four xi/eta
 // neighbours (L1/L2) instead of six neighbour loads.  Identical arithmetic (sensor_cell).
 template <typename T>
-__global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi, const T* __restrict__ Q,
-                                                      unsigned int* __restrict__ hcount, int KS) {
+__global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi, const T* __restrict__ Q, int KS) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k0 = 1 + blockIdx.z * KS;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hcount = 0;
   if (i > kp.Nxt - 2 || j > kp.Nyt - 2) return;
   const int k1 = min(k0 + KS, kp.Nzt - 1);  // planes k0 .. k1-1
   const T* P = Q + 4 * kp.nc;
@@ -114,7 +110,7 @@ __global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi
 constexpr int XW = 128;  // lanes per row segment (cells incl. the 4+4 halo)
 
 template <typename T, int SPLIT>
-__global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L) {
+__global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L, HSeg hs) {
   __shared__ T sFp[5][2][XW];
   __shared__ T sFm[5][2][XW];
   __shared__ T sF[5][2][XW];
@@ -162,8 +158,13 @@ __global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L) {
     }
   }
   const bool owned = rowok && fact && (f < f0 + L || f == kp.Nxp);
-  append_heavy(a.heavy, a.hcount, owned && sch != F3D_SCHEME_UP7,
+  {  // the wave's run of the heavy list (2 waves per row, 4 per block; at most one face per lane)
+    const unsigned int w = launch_wave(4, r * 2 + (l >> 6));
+    unsigned int cnt = 0;
+    run_append(a.heavy, hs.base + (unsigned long long)w * hs.cap, cnt, owned && sch != F3D_SCHEME_UP7,
                ((unsigned long long)cidx(kp, kp.NG + f, j, k) << 4) | ((unsigned long long)sch << 2) | 0ULL);
+    if ((l & 63) == 0) a.hcount[hs.first + w] = cnt;
+  }
   __syncthreads();
   if (rowok && l >= 4 && l < 4 + ncell) {
     const long long ro = ridx(kp, f, jj, kk), rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L) {
 // in a shift register acc[0..7] (acc[m] holds face f0+step-7+m); acc[0] is complete once cell
 // NG+f+3 has streamed in.  The next cell's loads are issued before the current cell is split.
 template <typename T, int D, int SPLIT>
-__global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
+__global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, HSeg hs) {
   const int ii = blockIdx.x * 64 + threadIdx.x;
   const int tt = blockIdx.y * 4 + threadIdx.y;
   const int Nt2 = D == 1 ? kp.Nzp : kp.Nyp;  // extent of the transverse line index
@@ -205,6 +206,10 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
   for (int s = 0; s < 6; ++s) ring[s] = 0;
   T Fprev[5] = {0, 0, 0, 0, 0};
   T Rn[5] = {0, 0, 0, 0, 0};  // RHS of the row the next step closes, loaded a step ahead
+  // the wave's run of the heavy list: at most one face per lane and completed face (<= 64 (L + 1))
+  const unsigned int hw = launch_wave(4, threadIdx.y);
+  const unsigned long long hrun = hs.base + (unsigned long long)hw * hs.cap;
+  unsigned int hcnt = 0;
   // prefetched inputs of the next cell
   T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph;
   {
@@ -246,9 +251,9 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
       for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
       const int sch = scheme_of<T>(kp, m);
       const int f = f0 + fi;
-      append_heavy(a.heavy, a.hcount, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
-                   ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) |
-                       (unsigned long long)D);
+      run_append(a.heavy, hrun, hcnt, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
+                 ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) |
+                     (unsigned long long)D);
       if (fi >= 1 && act) {
         const long long ro = D == 1 ? ridx(kp, ii, f - 1, tt) : ridx(kp, ii, tt, f - 1);
 #pragma unroll
@@ -267,15 +272,14 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L) {
     for (int s = 0; s < 5; ++s) ring[s] = ring[s + 1];
     ring[5] = ph;
   }
+  if (threadIdx.x == 0) a.hcount[hs.first + hw] = hcnt;
 }
 
 // ------------------------------------------------------------------ heavy faces
 template <typename T, int SPLIT, bool CHAR>
-__global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a) {
-  const unsigned int n = *a.hcount;
+__global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl) {
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp, nc = kp.nc;
-  for (unsigned int e = blockIdx.x * 64 + threadIdx.x; e < n; e += gridDim.x * 64) {
-    const unsigned long long v = a.heavy[e];
+  for_each_heavy(hl, a.hcount, a.heavy, [&](const unsigned long long v) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);
     const long long st = dstride(kp, D);
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a) {
       if (f <= Np - 1) atomicAdd(&a.R[rr + q * rn], dF);   // face is the left face of cell f
       if (f >= 1) atomicAdd(&a.R[rl + q * rn], -dF);       // and the right face of cell f-1
     }
-  }
+  });
 }
 
 // ------------------------------------------------------------------ viscous flux (k march)
@@ -962,11 +966,41 @@ void fused_sensor(const KP& kp, const WsView<T>& ws, const T* Q, T* phi, hipStre
   const int KS = sensor_ks();
   if (KS > 0)
     hipLaunchKernelGGL(k_sensor_march<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, (kp.Nzt - 2 + KS - 1) / KS),
-                       dim3(64, 4, 1), 0, s, kp, phi, Q, ws.hcount, KS);
+                       dim3(64, 4, 1), 0, s, kp, phi, Q, KS);
   else
     hipLaunchKernelGGL(k_sensor_f<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, kp.Nzt - 2), dim3(64, 4, 1),
-                       0, s, kp, phi, Q, ws.hcount);
+                       0, s, kp, phi, Q);
 }
+
+// Launch shapes of the finite-difference sweeps (shared by the launches and their heavy-list layout).
+static dim3 grid_sweep_x(const KP& kp, int L) { return dim3((kp.Nxp + L - 1) / L, (kp.Nyp + 1) / 2, kp.Nzp); }
+static dim3 grid_march(const KP& kp, int D, int L) {
+  const int Nt2 = D == 1 ? kp.Nzp : kp.Nyp, Np = D == 1 ? kp.Nyp : kp.Nzp;
+  return dim3((kp.Nxp + 63) / 64, (Nt2 + 3) / 4, (Np + L - 1) / L);
+}
+static unsigned long long nblocks(const dim3& g) { return (unsigned long long)g.x * g.y * g.z; }
+
+HList fdm_heavy_layout(const KP& kp, int es) {
+  const int Lx = seg_len(kp.Nxp, XW - 8);
+  const int Ly = seg_len(kp.Nyp, march_seg(1, es)), Lz = seg_len(kp.Nzp, march_seg(2, es));
+  // 4 waves per block; an x wave lists at most one face per lane, a march wave one per lane and face
+  const unsigned long long nw[3] = {4 * nblocks(grid_sweep_x(kp, Lx)), 4 * nblocks(grid_march(kp, 1, Ly)),
+                                    4 * nblocks(grid_march(kp, 2, Lz))};
+  const unsigned long long cap[3] = {64, 64ULL * (Ly + 1), 64ULL * (Lz + 1)};
+  return make_hlist(nw, cap);
+}
+
+template <typename T>
+int check_hlist(const WsView<T>& ws, const HList& h, const char* who) {
+  if (h.slots() > ws.hcap || h.nruns() > ws.hncap) {
+    set_error("%s: heavy-face list of %llu slots / %u runs exceeds the workspace's %llu / %llu (workspace created "
+              "with other segment lengths)", who, h.slots(), h.nruns(), ws.hcap, ws.hncap);
+    return F3D_E_ARG;
+  }
+  return F3D_OK;
+}
+template int check_hlist<double>(const WsView<double>&, const HList&, const char*);
+template int check_hlist<float>(const WsView<float>&, const HList&, const char*);
 
 // Inviscid increments of the finite-difference branch into ws.R: x, y, z sweeps and the heavy-face
 // corrections (after fused_sensor), for the splitting SPLIT and the reconstruction kp.character.
@@ -974,30 +1008,30 @@ template <typename T, int SPLIT>
 static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J,
                             T* phi, hipStream_t s) {
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
+  const HList hl = fdm_heavy_layout(kp, (int)sizeof(T));
+  if (int rc = check_hlist(ws, hl, "fused sweeps")) return rc;
   {
     TimedScope ts("fused_sweep_x", s);
     const int L = seg_len(kp.Nxp, XW - 8);
-    hipLaunchKernelGGL((k_sweep_x<T, SPLIT>), dim3((kp.Nxp + L - 1) / L, (kp.Nyp + 1) / 2, kp.Nzp), dim3(XW, 2, 1), 0,
-                       s, kp, a, L);
+    hipLaunchKernelGGL((k_sweep_x<T, SPLIT>), grid_sweep_x(kp, L), dim3(XW, 2, 1), 0, s, kp, a, L, hl.d[0]);
   }
   {
     TimedScope ts("fused_sweep_y", s);
     const int L = seg_len(kp.Nyp, march_seg(1, (int)sizeof(T)));
-    hipLaunchKernelGGL((k_sweep_march<T, 1, SPLIT>), dim3((kp.Nxp + 63) / 64, (kp.Nzp + 3) / 4, (kp.Nyp + L - 1) / L),
-                       dim3(64, 4, 1), 0, s, kp, a, L);
+    hipLaunchKernelGGL((k_sweep_march<T, 1, SPLIT>), grid_march(kp, 1, L), dim3(64, 4, 1), 0, s, kp, a, L, hl.d[1]);
   }
   {
     TimedScope ts("fused_sweep_z", s);
     const int L = seg_len(kp.Nzp, march_seg(2, (int)sizeof(T)));
-    hipLaunchKernelGGL((k_sweep_march<T, 2, SPLIT>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + L - 1) / L),
-                       dim3(64, 4, 1), 0, s, kp, a, L);
+    hipLaunchKernelGGL((k_sweep_march<T, 2, SPLIT>), grid_march(kp, 2, L), dim3(64, 4, 1), 0, s, kp, a, L, hl.d[2]);
   }
   {
     TimedScope ts("fused_heavy", s);
+    const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
     if (kp.character)
-      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(2048), dim3(64), 0, s, kp, a);
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl);
     else
-      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(2048), dim3(64), 0, s, kp, a);
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl);
   }
   return check_launch("fused sweeps");
 }

@@ -12,9 +12,10 @@ struct WsView {
   T* Fvy;
   T* Fvz;
   T* phi;                    // shock sensor [Nzt][Nyt][Nxt]
-  unsigned long long* heavy; // compacted non-UP7 faces: (right cell << 4) | (scheme << 2) | dir
-  unsigned int* hcount;      // number of entries in `heavy`
-  unsigned long long hcap;   // capacity of `heavy` (total face count of the three directions)
+  unsigned long long* heavy; // non-UP7 faces: (right cell << 4) | (scheme << 2) | dir, in per-wave runs (HList)
+  unsigned int* hcount;      // entries used in each wave's run
+  unsigned long long hcap;   // capacity of `heavy` (slots)
+  unsigned long long hncap;  // capacity of `hcount` (runs)
   double* ldt;               // per-cell dt of local time stepping, cell-shaped (written at stage 1)
 };
 
@@ -37,20 +38,111 @@ __device__ __forceinline__ Blk xcd_block(int remap) {
 // 1 unless F3D_XCD_REMAP=0 (A/B measurement of the block order).
 int xcd_remap();
 
-// Wave-aggregated append to the compacted heavy-face list: one atomic per wave.
-__device__ __forceinline__ void append_heavy(unsigned long long* list, unsigned int* count, bool pred,
-                                             unsigned long long entry) {
+// Heavy-face list (faces the sensor sends to WENO7/WENO5/NND) in per-wave runs: every wave of a sweep
+// owns a fixed run of `cap` slots and counts its entries in a register, so no global atomic is needed
+// (a single shared counter serialises the sweeps once a few percent of the faces are heavy: one atomic
+// per wave and step).  Run w of a launch starts at slot base + w * cap; hcount[first + w] = entries used.
+struct HSeg {
+  unsigned long long base;
+  unsigned int cap;
+  unsigned int first;
+  unsigned int nw;  // waves (runs) of the launch
+};
+// The runs of the three sweeps of one stage, for the kernel that processes the listed faces.
+struct HList {
+  HSeg d[3];
+  unsigned int nruns() const { return d[0].nw + d[1].nw + d[2].nw; }
+  // work items of for_each_heavy (a block per item; the grid needs no more blocks than items)
+  unsigned int items() const {
+    unsigned int n = 0;
+    for (int q = 0; q < 3; ++q) n += d[q].cap > 64 ? d[q].nw : (d[q].nw + 63) / 64;
+    return n;
+  }
+  unsigned long long slots() const {
+    unsigned long long n = 0;
+    for (int q = 0; q < 3; ++q) n = n > d[q].base + (unsigned long long)d[q].nw * d[q].cap ? n : d[q].base + (unsigned long long)d[q].nw * d[q].cap;
+    return n;
+  }
+};
+// Runs laid out back to back, direction after direction (nw[q] waves of cap[q] slots).
+inline HList make_hlist(const unsigned long long nw[3], const unsigned long long cap[3]) {
+  HList h{};
+  unsigned long long base = 0, first = 0;
+  for (int q = 0; q < 3; ++q) {
+    h.d[q] = HSeg{base, (unsigned int)cap[q], (unsigned int)first, (unsigned int)nw[q]};
+    base += nw[q] * cap[q];
+    first += nw[q];
+  }
+  return h;
+}
+
+// Wave-aggregated append into the wave's run: lanes with pred write their entry after the ones already
+// there (lane order); cnt (wave-uniform) advances by the number written.
+__device__ __forceinline__ void run_append(unsigned long long* list, unsigned long long run0, unsigned int& cnt,
+                                           bool pred, unsigned long long entry) {
   const unsigned long long mask = __ballot(pred);
   if (mask == 0) return;
   const int lane = __lane_id();
-  unsigned int base = 0;
-  const int leader = __ffsll((long long)mask) - 1;
-  if (lane == leader) base = atomicAdd(count, (unsigned int)__popcll(mask));
-  base = __shfl(base, leader, 64);
   if (pred) {
     const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
-    list[base + __popcll(below)] = entry;
+    list[run0 + cnt + __popcll(below)] = entry;
   }
+  cnt += (unsigned int)__popcll(mask);
+}
+
+// Calls f(entry) for every listed face, one 64-lane block per work item: a run of more than 64 slots is an
+// item of its own (its entries 64 at a time), runs of at most 64 slots go 64 runs to an item (their counts
+// read in one coalesced load, their entries packed onto the lanes).  Items are spread over the grid.
+// Blocks must be one wave of 64 lanes.
+template <typename F>
+__device__ __forceinline__ void for_each_heavy(const HList& hl, const unsigned int* __restrict__ counts,
+                                               const unsigned long long* __restrict__ list, F&& f) {
+  unsigned int items[3], total = 0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    items[q] = hl.d[q].cap > 64 ? hl.d[q].nw : (hl.d[q].nw + 63) / 64;
+    total += items[q];
+  }
+  const int lane = threadIdx.x;
+  for (unsigned int it = blockIdx.x; it < total; it += gridDim.x) {
+    const int dq = it < items[0] ? 0 : (it < items[0] + items[1] ? 1 : 2);
+    const unsigned int loc = it - (dq == 0 ? 0u : (dq == 1 ? items[0] : items[0] + items[1]));
+    const HSeg hs = hl.d[dq];
+    if (hs.cap > 64) {
+      const unsigned int n = counts[hs.first + loc];
+      const unsigned long long run0 = hs.base + (unsigned long long)loc * hs.cap;
+      for (unsigned int e = lane; e < n; e += 64) f(list[run0 + e]);
+    } else {
+      // the 64 runs' entries packed onto the lanes: exclusive prefix of their counts, then entry e goes
+      // to the last run whose prefix is <= e (empty runs share their successor's prefix)
+      __shared__ unsigned int pre[64];
+      const unsigned int w0 = loc * 64, mine = w0 + lane;
+      const unsigned int myn = mine < hs.nw ? counts[hs.first + mine] : 0u;
+      unsigned int incl = myn;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const unsigned int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      const unsigned int total = __shfl(incl, 63, 64);
+      pre[lane] = incl - myn;
+      __syncthreads();  // (a one-wave block)
+      for (unsigned int e = lane; e < total; e += 64) {
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (pre[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        f(list[hs.base + (unsigned long long)(w0 + lo) * hs.cap + (e - pre[lo])]);
+      }
+      __syncthreads();  // pre is rewritten by the next item
+    }
+  }
+}
+
+// Linear id of the calling wave in a launch of blocks of `wpb` waves (wave index wib in its block).
+__device__ __forceinline__ unsigned int launch_wave(int wpb, int wib) {
+  return ((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * (unsigned int)wpb + (unsigned int)wib;
 }
 
 }  // namespace f3d
@@ -64,6 +156,15 @@ WsView<T> ws_view(f3d_workspace* ws);
 // true when the fused sweeps serve f3d_rhs_* / f3d_rk3_stage_* (F3D_RHS_ENGINE=chain selects the
 // kernel-by-kernel chain instead, for A/B measurement).
 bool use_fused();
+
+// Heavy-face list layouts: the finite-difference sweeps of bulk element size es, the finite-volume
+// sweeps, and the adjoint's list kernels.  The workspace holds the largest of the three.
+HList fdm_heavy_layout(const KP& kp, int es);
+HList fvm_heavy_layout(const KP& kp);
+HList adj_heavy_layout(const KP& kp);
+// F3D_OK when the workspace view holds layout h, else F3D_E_ARG with a message.
+template <typename T>
+int check_hlist(const WsView<T>& ws, const HList& h, const char* who);
 
 // RHS into R (U layout, interior, double): R = J*(-dF + dFv).
 template <typename T>

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_fvm_geometry(KP kp, const double* __res
 
 // ------------------------------------------------------------------ fused x sweep (LDS rows)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L) {
+__global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L, HSeg hs) {
   __shared__ T sU[5][2][kXW];
   __shared__ T sphi[2][kXW];
   __shared__ T sF[5][2][kXW];
@@ -161,8 +161,13 @@ __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L)
     for (int n = 0; n < 5; ++n) sF[n][r][l] = Fl[n];
   }
   const bool owned = rowok && fact && (f < f0 + L || f == kp.Nxp);
-  append_heavy(a.heavy, a.hcount, owned && sch != F3D_SCHEME_UP7,
+  {  // the wave's run of the heavy list (2 waves per row, 4 per block; at most one face per lane)
+    const unsigned int w = launch_wave(4, r * 2 + (l >> 6));
+    unsigned int cnt = 0;
+    run_append(a.heavy, hs.base + (unsigned long long)w * hs.cap, cnt, owned && sch != F3D_SCHEME_UP7,
                ((unsigned long long)cidx(kp, kp.NG + f, j, k) << 4) | ((unsigned long long)sch << 2) | 0ULL);
+    if ((l & 63) == 0) a.hcount[hs.first + w] = cnt;
+  }
   __syncthreads();
   if (rowok && l >= 4 && l < 4 + ncell) {
     const long long ro = rix(kp, f, jj, kk), rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
@@ -177,7 +182,7 @@ __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L)
 // c-3..c+4 of the face between W[3] and W[4] (face f = f0 + step - 7).  The next cell's U and the
 // next face's geometry are requested a step ahead.
 template <typename T, int D>
-__global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
+__global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, HSeg hs) {
   const int ii = blockIdx.x * 64 + threadIdx.x;
   const int tt = blockIdx.y * 4 + threadIdx.y;
   const int Nt2 = D == 1 ? kp.Nzp : kp.Nyp;
@@ -204,6 +209,10 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
   }
   T Fprev[5] = {0, 0, 0, 0, 0};
   T Rn[5] = {0, 0, 0, 0, 0};  // RHS of the row the next step closes, loaded a step ahead
+  // the wave's run of the heavy list: at most one face per lane and completed face (<= 64 (L + 1))
+  const unsigned int hw = launch_wave(4, threadIdx.y);
+  const unsigned long long hrun = hs.base + (unsigned long long)hw * hs.cap;
+  unsigned int hcnt = 0;
   T gn[4] = {0, 0, 0, 0};  // geometry of the next face to complete
   double nu[5];
   T nph;
@@ -257,9 +266,9 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
 #pragma unroll
           for (int n = 0; n < 5; ++n) Fc[n] *= g3;
         }
-        append_heavy(a.heavy, a.hcount, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
-                     ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) |
-                         ((unsigned long long)sch << 2) | (unsigned long long)D);
+        run_append(a.heavy, hrun, hcnt, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
+                   ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) |
+                       ((unsigned long long)sch << 2) | (unsigned long long)D);
         if (fi >= 1 && act) {
           const long long ro = D == 1 ? rix(kp, ii, f - 1, tt) : rix(kp, ii, tt, f - 1);
 #pragma unroll
@@ -270,15 +279,15 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L) {
       }
     }
   }
+  if (threadIdx.x == 0) a.hcount[hs.first + hw] = hcnt;
 }
 
 // ------------------------------------------------------------------ heavy faces
 template <typename T, bool EIGEN>
-__global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* __restrict__ Q, MetricPtrs<T> m) {
-  const unsigned int n = *a.hcount;
+__global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* __restrict__ Q, MetricPtrs<T> m,
+                                                  HList hl) {
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
-  for (unsigned int e = blockIdx.x * 64 + threadIdx.x; e < n; e += gridDim.x * 64) {
-    const unsigned long long v = a.heavy[e];
+  for_each_heavy(hl, a.hcount, a.heavy, [&](const unsigned long long v) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);  // right cell of the face
     const FvmDir dd = fvm_dir(kp, D);
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* 
       if (f <= Np - 1) atomicAdd(&a.R[rr + q * rn], F[q]);  // left face of cell f
       if (f >= 1) atomicAdd(&a.R[rl + q * rn], -F[q]);      // right face of cell f-1
     }
-  }
+  });
 }
 
 int seg_len(int Np, int max_len) {
@@ -322,34 +331,53 @@ int fvm_seg(int D) {
   return r;
 }
 
+static dim3 fvm_grid_x(const KP& kp, int L) { return dim3((kp.Nxp + L - 1) / L, (kp.Nyp + 1) / 2, kp.Nzp); }
+static dim3 fvm_grid_march(const KP& kp, int D, int L) {
+  const int Nt2 = D == 1 ? kp.Nzp : kp.Nyp, Np = D == 1 ? kp.Nyp : kp.Nzp;
+  return dim3((kp.Nxp + 63) / 64, (Nt2 + 3) / 4, (Np + L - 1) / L);
+}
+
+}  // namespace
+
+HList fvm_heavy_layout(const KP& kp) {
+  const int Lx = seg_len(kp.Nxp, kXW - 8), Ly = seg_len(kp.Nyp, fvm_seg(1)), Lz = seg_len(kp.Nzp, fvm_seg(2));
+  auto nb = [](const dim3& g) { return (unsigned long long)g.x * g.y * g.z; };
+  const unsigned long long nw[3] = {4 * nb(fvm_grid_x(kp, Lx)), 4 * nb(fvm_grid_march(kp, 1, Ly)),
+                                    4 * nb(fvm_grid_march(kp, 2, Lz))};
+  const unsigned long long cap[3] = {64, 64ULL * (Ly + 1), 64ULL * (Lz + 1)};
+  return make_hlist(nw, cap);
+}
+
+namespace {
+
 template <typename T>
 int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
              const T* const* geo, const T* phi, hipStream_t s) {
   FvmArgs<T> a{U, {geo[0], geo[1], geo[2]}, phi, ws.R, ws.heavy, ws.hcount};
+  const HList hl = fvm_heavy_layout(kp);
+  if (int rc = check_hlist(ws, hl, "fvm sweeps")) return rc;
   {
     TimedScope ts("fvm_sweep_x", s);
     const int L = seg_len(kp.Nxp, kXW - 8);
-    hipLaunchKernelGGL(k_fvm_sweep_x<T>, dim3((kp.Nxp + L - 1) / L, (kp.Nyp + 1) / 2, kp.Nzp), dim3(kXW, 2, 1), 0, s,
-                       kp, a, L);
+    hipLaunchKernelGGL(k_fvm_sweep_x<T>, fvm_grid_x(kp, L), dim3(kXW, 2, 1), 0, s, kp, a, L, hl.d[0]);
   }
   {
     TimedScope ts("fvm_sweep_y", s);
     const int L = seg_len(kp.Nyp, fvm_seg(1));
-    hipLaunchKernelGGL((k_fvm_march<T, 1>), dim3((kp.Nxp + 63) / 64, (kp.Nzp + 3) / 4, (kp.Nyp + L - 1) / L),
-                       dim3(64, 4, 1), 0, s, kp, a, L);
+    hipLaunchKernelGGL((k_fvm_march<T, 1>), fvm_grid_march(kp, 1, L), dim3(64, 4, 1), 0, s, kp, a, L, hl.d[1]);
   }
   {
     TimedScope ts("fvm_sweep_z", s);
     const int L = seg_len(kp.Nzp, fvm_seg(2));
-    hipLaunchKernelGGL((k_fvm_march<T, 2>), dim3((kp.Nxp + 63) / 64, (kp.Nyp + 3) / 4, (kp.Nzp + L - 1) / L),
-                       dim3(64, 4, 1), 0, s, kp, a, L);
+    hipLaunchKernelGGL((k_fvm_march<T, 2>), fvm_grid_march(kp, 2, L), dim3(64, 4, 1), 0, s, kp, a, L, hl.d[2]);
   }
   {
     TimedScope ts("fvm_heavy", s);
+    const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
     if (kp.fvm == F3D_FVM_EIGEN)
-      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(2048), dim3(64), 0, s, kp, a, Q, m);
+      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl);
     else
-      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(2048), dim3(64), 0, s, kp, a, Q, m);
+      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl);
   }
   return check_launch("fvm sweeps");
 }

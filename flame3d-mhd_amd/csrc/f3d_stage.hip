@@ -21,7 +21,7 @@ struct f3d_workspace {
   int nred;      // number of partial slots
   unsigned long long* heavy;  // compacted non-UP7 face list (fused path)
   unsigned int* hcount;
-  unsigned long long hcap;
+  unsigned long long hcap, hncap;
   void *Qa, *Qb;  // stage outputs of the fused path (primitive state, 6 vars, bulk type)
   double* ldt;    // per-cell dt of local time stepping (cell-shaped; compute_dt at stage 1)
   double* fsc;    // sensor of the nonlinear filter (cell-shaped)
@@ -419,9 +419,17 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[9] = Nx * Ny * Nz * 5 * es;                 // R
   sz[10] = kRedBlocks * sizeof(double);          // red
   sz[11] = kRedBlocks * sizeof(int);             // flag
-  const size_t nfaces = (Nx + 1) * Ny * Nz + Nx * (Ny + 1) * Nz + Nx * Ny * (Nz + 1);
-  sz[12] = nfaces * sizeof(unsigned long long);  // heavy-face list
-  sz[13] = 256;                                  // heavy-face counter
+  // heavy-face list in per-wave runs: room for the largest layout of the finite-difference sweeps (this
+  // precision), the finite-volume sweeps and the adjoint's list kernels
+  const KP kp0 = make_kp(p);
+  const HList hls[3] = {fdm_heavy_layout(kp0, (int)es), fvm_heavy_layout(kp0), adj_heavy_layout(kp0)};
+  unsigned long long hslots = 0, hruns = 0;
+  for (const HList& h : hls) {
+    hslots = h.slots() > hslots ? h.slots() : hslots;
+    hruns = h.nruns() > hruns ? h.nruns() : hruns;
+  }
+  sz[12] = hslots * sizeof(unsigned long long);  // heavy-face list
+  sz[13] = hruns * sizeof(unsigned int);         // entries per run
   sz[14] = sz[15] = nc * 6 * es;                 // Qa, Qb
   sz[16] = nc * sizeof(double);                  // ldt
   sz[17] = nc * sizeof(double);                  // fsc
@@ -443,7 +451,8 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
     c += align_up(sz[i]);
   }
   ws->nred = kRedBlocks;
-  ws->hcap = nfaces;
+  ws->hcap = hslots;
+  ws->hncap = hruns;
   // zeroed once: inviscid runs read zero viscous fluxes in div (div.jl:45-48 adds them always)
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, 0, (double*)base, (long long)(total / 8));
   hipError_t e = hipDeviceSynchronize();
@@ -482,6 +491,7 @@ WsView<T> ws_view(f3d_workspace* ws) {
   v.heavy = ws->heavy;
   v.hcount = ws->hcount;
   v.hcap = ws->hcap;
+  v.hncap = ws->hncap;
   v.ldt = ws->ldt;
   return v;
 }
