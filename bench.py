@@ -267,6 +267,49 @@ def alternatives_bench(f3d, torch, lib, st, U0, Q0, steps, warmup):
     return out
 
 
+def shocked_bench(f3d, torch, lib, st, Q0, steps, warmup, amp=0.05):
+    """The step on a state with planted pressure noise (relative amplitude `amp`, seeded), so that a few
+    per cent of the faces leave the UP7 branch and take the WENO7/WENO5/NND path: the heavy-face lists and
+    the characteristic reconstruction of those faces are timed.  Same grid and metrics as the main leg, dt
+    scaled by 0.01 so that the noise survives the timed steps; starts from the main leg's initial state.  Reports the share of faces off UP7 in the
+    initial state, ms/step FP64 and FP32-bulk and the per-kernel table."""
+    p, ctx = st["p"], st["ctx"]
+    Q = Q0.clone()
+    g = torch.Generator(device="cuda:0").manual_seed(7)
+    noise = torch.rand(Q[4].shape, device="cuda:0", generator=g, dtype=torch.float64)
+    Q[4] *= 1.0 + amp * (noise - 0.5)
+    Q[5] = Q[4] / (Q[0] * p.Rg)
+    U = torch.zeros_like(st["U"])
+    ctx.fillGhost(Q, U, st["inflow"])
+    ctx.prim2c(U, Q)
+    phi = torch.zeros(p.cell_shape(), dtype=torch.float64, device="cuda:0")
+    ctx.shockSensor(phi, Q)
+    masks = [torch.zeros(p.mask_shape(d), dtype=torch.uint8, device="cuda:0") for d in range(3)]
+    ctx.schemeMask(*masks, phi)
+    torch.cuda.synchronize()
+    nf = sum(m.numel() for m in masks)
+    nh = sum(int((m != 0).sum()) for m in masks)
+    del phi, masks
+    out = dict(noise_amplitude=amp, faces=nf, faces_off_up7=nh, frac_off_up7=round(nh / nf, 4))
+    ncells = p.Nxp * p.Nyp * p.Nzp
+    for name in ("f64", "f32b"):
+        if name == "f64":
+            state = (U.clone(), torch.zeros_like(U), Q.clone(), st["m"], st["J"], st["inflow"], st["dt"] * 0.01)
+        else:
+            state = (U.clone(), torch.zeros_like(U), Q.float().contiguous(), [a.float().contiguous() for a in st["m"]],
+                     st["J"], st["inflow"].float().contiguous(), st["dt"] * 0.01)
+        ms, _, tab, _ = run_steps(torch, ctx, state, steps, warmup, lib=lib, table_steps=2)
+        ctx.nan_check(state[0])
+        out[name] = dict(ms_per_step=round(ms, 4), value=round(ncells / (ms * 1e-3) / 1e6, 3),
+                         kernels={k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1])
+                                  for k, v in sorted(tab.items())})
+        del state
+    torch.cuda.empty_cache()
+    progress(f"shocked state ({100 * nh / nf:.1f}% faces off UP7): FP64 {out['f64']['ms_per_step']:.3f}, "
+             f"FP32-bulk {out['f32b']['ms_per_step']:.3f} ms/step")
+    return out
+
+
 def io_bench(st, Q0):
     """The reference's on-disk formats on the bench grid (host side, libf3d_h5.so): a chk-*.h5 checkpoint
     of Q as IO.jl:152-179 writes it (Float32, shuffle + deflate level 1, the reference's default) and
@@ -468,6 +511,8 @@ def main():
     ap.add_argument("--fvm-only", action="store_true", help="only the finite-volume leg (profiling)")
     ap.add_argument("--no-alternatives", action="store_true", help="skip the flag-selected alternatives leg")
     ap.add_argument("--no-io", action="store_true", help="skip the on-disk formats leg")
+    ap.add_argument("--no-shocked", action="store_true",
+                    help="skip the leg on a noisy state (faces off UP7 take the WENO/NND path)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -563,6 +608,9 @@ def main():
     alt = None
     if not args.no_alternatives:
         alt = alternatives_bench(f3d, torch, lib, st, U0, Q0, max(3, args.steps // 4), 2)
+    shocked = None
+    if not args.no_shocked:
+        shocked = shocked_bench(f3d, torch, lib, st, Q0, max(3, args.steps // 4), 2)
     io = None
     if not args.no_io:
         try:
@@ -601,6 +649,7 @@ def main():
         "fp32_bulk": out32 or None,
         "finite_volume": fvm,
         "alternatives": alt,
+        "shocked_state": shocked,
         "io": io,
         "adjoint": adj,
         "kernels": kshare,
