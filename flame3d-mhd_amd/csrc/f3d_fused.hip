@@ -920,16 +920,19 @@ template <typename T, int MODE>
 static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt,
                                const ViscTile& vt, int rm, hipStream_t s) {
   const int f = vt.VI, KC = vt.KC;
-  const dim3 g = visc_grid(kp, KC, f);
-  if (visc_pf((int)sizeof(T)) && (f == 4 || (f != 12 && sizeof(T) == 4))) {
-    if (f == 4)
+  const bool pf = visc_pf((int)sizeof(T)) && (f == 4 || (f != 12 && sizeof(T) == 4));
+  // the forms built without the prefetch: 4, 8 or (FP32-bulk) 12 interior rows
+  const int rows = pf ? f : (f <= 4 ? 4 : (f <= 8 || sizeof(T) == 8) ? 8 : 12);
+  const dim3 g = visc_grid(kp, KC, rows);
+  if (pf) {
+    if (rows == 4)
       hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, true>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
                          rm);
     else if constexpr (sizeof(T) == 4) {
-      if (f == 6)
+      if (rows == 6)
         hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 6, true>), g, dim3(VX, 10, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
                            KC, rm);
-      else if (f == 7)
+      else if (rows == 7)
         hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 7, true>), g, dim3(VX, 11, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
                            KC, rm);
       else
@@ -938,10 +941,10 @@ static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, doubl
     }
     return;
   }
-  if (f == 4)
+  if (rows == 4)
     hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, false>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
                        rm);
-  else if (f == 8)
+  else if (rows == 8)
     hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, false>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
                        KC, rm);
   else if constexpr (sizeof(T) == 4)
