@@ -501,8 +501,8 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
 
 // ------------------------------------------------------------------ viscous divergence + update
 // k_update with the viscous divergence computed in place (k_visc_update_qt below): the viscous fluxes
-// never go to HBM.  Tile: VX = 64 lanes over VXI = 60 interior columns plus 2 halo lanes each side.
-constexpr int VX = 64, VXI = 60;
+// never go to HBM.  Tile rows of W lanes (64 or 32) over W - 4 interior columns plus 2 halo lanes each
+// side; with W = 32 a wave holds two rows.
 
 // Conservative update + RK3 blend + c2Prim of one interior cell (utils.jl:2-70, div.jl:40-48);
 // MODE 0 writes the RHS instead.
@@ -617,34 +617,37 @@ __device__ __forceinline__ void update_cell_ops(const KP& kp, const T* Rt, long 
   Q[oc + 5 * nc] = (T)(pr / (rho * kp.Rg));
 }
 
-// Block = 64 x (VI + 4) lanes over a 60 x VI tile of interior cells; rows 0, 1, VI + 2, VI + 3 are
-// y-halo rows (waves of their own, Fv_y only).  The block marches k: at iteration p (2 ahead of the
+// Block = W x (VI + 4) lanes over a (W - 4) x VI tile of interior cells; rows 0, 1, VI + 2, VI + 3 are
+// y-halo rows (Fv_y only; waves of their own: with W = 32, VI is even so that a wave's two rows are both
+// halo or both interior).  The block marches k: at iteration p (2 ahead of the
 // updated plane k) every lane evaluates the velocity/temperature gradient and viscous_cell for its cell;
 // Fv_x of the interior rows and Fv_y of the interior columns are exchanged through LDS for the x/y
 // divergence of plane p, kept two iterations in a register ring; Fv_z enters a 5-plane register window for
 // the z divergence of plane k; u,v,w,T of the column sit in a 5-plane window for the zeta derivatives.
 // The xi and eta neighbours of u,v,w,T come from a plane tile in LDS: every lane stores its column's
 // plane-p values (from its window) into the tile, the lanes at the x edges add the two cells beyond
-// (x0-4, x0-3 and x0+62, x0+63) and the halo rows the two rows beyond (y0-4, y0-3 and y0+VI+2,
+// (x0-4, x0-3 and x0+W-2, x0+W-1) and the halo rows the two rows beyond (y0-4, y0-3 and y0+VI+2,
 // y0+VI+3); after a barrier the 4th-order xi/eta differences read the tile.  Summation order of the
 // divergence as k_update.
 // The tile is written before barrier A and read before barrier B (the viscous-flux exchange), so one
 // buffer suffices; likewise the flux buffers, written after A and read after B.
-template <typename T, int MODE, int VI, bool PF>
-__global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T> a, double* __restrict__ U,
+template <typename T, int MODE, int VI, bool PF, int W>
+__global__ __launch_bounds__(W * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T> a, double* __restrict__ U,
                                                         double* __restrict__ Un, T* __restrict__ Q,
                                                         double* __restrict__ Rout, double dt, int KC, int remap) {
-  __shared__ T sQ[4][VI + 8][VX + 4];  // u,v,w,T of plane p: rows y0-4 .. y0+VI+3, columns x0-4 .. x0+63
-  __shared__ T sFx[4][VI][VX];
-  __shared__ T sFy[4][VI + 4][VXI];
+  static_assert(W == 64 || (W == 32 && VI % 2 == 0), "tile rows of 64 lanes, or two rows of 32 per wave");
+  constexpr int WI = W - 4;              // interior columns of the tile
+  __shared__ T sQ[4][VI + 8][W + 4];  // u,v,w,T of plane p: rows y0-4 .. y0+VI+3, columns x0-4 .. x0+W-1
+  __shared__ T sFx[4][VI][W];
+  __shared__ T sFy[4][VI + 4][WI];
   const Blk B = xcd_block(remap);
   const int lane = threadIdx.x, row = threadIdx.y;
   const int NG = kp.NG;
-  const int x = NG + B.x * VXI - 2 + lane, y = NG + B.y * VI - 2 + row;
+  const int x = NG + B.x * WI - 2 + lane, y = NG + B.y * VI - 2 + row;
   const int k0 = NG + B.z * KC;
   const int nk = min(KC, NG + kp.Nzp - k0);
   const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;
-  const bool xin = lane >= 2 && lane < 2 + VXI && x < NG + kp.Nxp;
+  const bool xin = lane >= 2 && lane < 2 + WI && x < NG + kp.Nxp;
   const bool yin = row >= 2 && row < 2 + VI && y < NG + kp.Nyp;
   const bool needFx = yin && x <= xmax;
   const bool needFy = xin && y <= ymax;
@@ -655,7 +658,7 @@ __global__ __launch_bounds__(VX * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
   const long long col = cidx(kp, xl, yl, 0);
   // extra cell of the edge lanes (x -+ 2) and of the halo rows (y -+ 2), clamped into the array
-  const bool xe = lane < 2 || lane >= VX - 2, ye = row < 2 || row >= VI + 2;
+  const bool xe = lane < 2 || lane >= W - 2, ye = row < 2 || row >= VI + 2;
   const int xe_x = min(max(lane < 2 ? x - 2 : x + 2, 0), kp.Nxt - 1);
   const int ye_y = min(max(row < 2 ? y - 2 : y + 2, 0), kp.Nyt - 1);
   const long long colx = cidx(kp, xe_x, yl, 0), coly = cidx(kp, xl, ye_y, 0);
@@ -844,8 +847,8 @@ static int seg_len(int Np, int max_len) {
   return (Np + nseg - 1) / nseg;
 }
 
-static dim3 visc_grid(const KP& kp, int KC, int rows) {
-  return dim3((kp.Nxp + VXI - 1) / VXI, (kp.Nyp + rows - 1) / rows, (kp.Nzp + KC - 1) / KC);
+static dim3 visc_grid(const KP& kp, int KC, int rows, int W) {
+  return dim3((kp.Nxp + W - 5) / (W - 4), (kp.Nyp + rows - 1) / rows, (kp.Nzp + KC - 1) / KC);
 }
 
 static int env_int(const char* name, int def) {
@@ -874,82 +877,106 @@ static int visc_pf(int es) {
   return r;
 }
 
-// Tile of the viscous-update march: VI interior rows per block (plus 4 halo rows, every row a wave) and
-// KC planes per segment (each segment re-evaluates 4 planes of viscous fluxes).  One block runs per CU
-// (FP64 VI = 4 with the prefetch: 242 VGPRs, two waves per SIMD; FP32-bulk VI = 6-8: 152 VGPRs, three),
-// so the march's duration is about rounds x (VI + 4) waves x (KC + 4) iterations, rounds = ceil(blocks /
-// CUs).  FP32-bulk picks VI in 6..8 and KC in 16/32/64 minimising that (on the 512x192x64 ramp VI = 7,
-// KC = 64: 252 blocks for 256 CUs, 407 us against 428 for VI = 8); FP64 keeps VI = 4 and KC = 16 where
-// that leaves >= 384 blocks (759 us against 796 at 32 planes), shorter on small grids.  F3D_VISC_FORM
-// (4, 6, 7, 8, 12) and F3D_VISC_KC override.
+// Tile of the viscous-update march: VI interior rows of W lanes per block (plus 4 halo rows; a wave is
+// one row of 64 or two rows of 32) and KC planes per segment (each segment re-evaluates 4 planes of
+// viscous fluxes).  One block runs per CU (FP64 with the prefetch: 242 VGPRs, two waves per SIMD, 8-wave
+// blocks; FP32-bulk: 152 VGPRs, three, up to 12-wave blocks), so the march's duration is about rounds x
+// waves x (KC + 4) iterations, rounds = ceil(blocks / CUs); the chooser minimises that over the forms
+// built.  On the 512x192x64 ramp: FP64 VI = 12 rows of 32 (28 x 12 interior cells per 8 waves instead of
+// 60 x 4: half the y-halo loads and evaluations), KC = 16: 679 us against 762 for 4 rows of 64 (KC = 8,
+// 11, 13, 22, 32: 741, 752, 681, 698, 721 us, the order the estimate gives); FP32-bulk VI = 7 rows of 64,
+// KC = 64: 252 blocks for 256 CUs, 405 us (8 rows: 428; rows of 32 lanes: 441-490).  F3D_VISC_FORM (VI),
+// F3D_VISC_W (32 or 64) and F3D_VISC_KC override; forms that are not built fall back to the default.
 struct ViscTile {
-  int VI, KC;
+  int VI, KC, W;
 };
+
+// (VI, W) forms built: with the prefetch FP64 (4, 64), (8, 32), (12, 32) and FP32-bulk (4|6|7|8, 64),
+// (12|16|20, 32); without it (4|8, 64) and FP32-bulk (12, 64).
+static bool visc_form_built(int es, bool pf, int vi, int w) {
+  if (pf) {
+    if (es == 8) return (vi == 4 && w == 64) || ((vi == 8 || vi == 12) && w == 32);
+    return (w == 64 && (vi == 4 || vi == 6 || vi == 7 || vi == 8)) || (w == 32 && (vi == 12 || vi == 16 || vi == 20));
+  }
+  return w == 64 && (vi == 4 || vi == 8 || (vi == 12 && es == 4));
+}
+
 static ViscTile visc_tile(const KP& kp, int es) {
   const int fenv = env_int("F3D_VISC_FORM", 0), kenv = env_int("F3D_VISC_KC", 0);
-  auto cols = [&](int vi) { return (long long)((kp.Nxp + VXI - 1) / VXI) * ((kp.Nyp + vi - 1) / vi); };
-  if (es == 8) {
-    int vi = (fenv == 4 || fenv == 8) ? fenv : 4;
-    int kc = 16;
-    while (kc > 8 && cols(vi) * ((kp.Nzp + kc - 1) / kc) < 384) kc -= 4;
-    return ViscTile{vi, seg_len(kp.Nzp, kenv >= 4 ? kenv : kc)};
-  }
-  const int ncu = cu_count();
-  ViscTile best{8, 16};
+  const int wenv = env_int("F3D_VISC_W", 0) == 32 ? 32 : 64;
+  const bool pf = visc_pf(es) != 0;
+  struct Form {
+    int vi, w;
+  };
+  static const Form f64pf[2] = {{12, 32}, {4, 64}}, f64[1] = {{4, 64}};
+  static const Form f32pf[3] = {{8, 64}, {7, 64}, {6, 64}}, f32[1] = {{8, 64}};
+  static const int kc64[6] = {64, 32, 22, 16, 13, 8}, kc32[3] = {64, 32, 16};
+  const Form forced{fenv, wenv};
+  const Form* forms = es == 8 ? (pf ? f64pf : f64) : (pf ? f32pf : f32);
+  int nf = es == 8 ? (pf ? 2 : 1) : (pf ? 3 : 1);
+  if (fenv && visc_form_built(es, pf, fenv, wenv)) forms = &forced, nf = 1;
+  const int* kcs = es == 8 ? kc64 : kc32;
+  const int nkc = kenv >= 4 ? 1 : (es == 8 ? 6 : 3);
+  const long long ncu = cu_count();
+  ViscTile best{forms[0].vi, 16, forms[0].w};
   double bestc = -1;
-  const int vis[3] = {8, 7, 6};
-  const int kcs[3] = {64, 32, 16};
-  for (int a = 0; a < 3; ++a) {
-    const int vi = fenv ? fenv : vis[a];
-    for (int b = 0; b < 3; ++b) {
+  for (int a = 0; a < nf; ++a) {
+    const int vi = forms[a].vi, w = forms[a].w;
+    const long long cols = (long long)((kp.Nxp + w - 5) / (w - 4)) * ((kp.Nyp + vi - 1) / vi);
+    for (int b = 0; b < nkc; ++b) {
       const int kc = seg_len(kp.Nzp, kenv >= 4 ? kenv : kcs[b]);
-      const long long blocks = cols(vi) * ((kp.Nzp + kc - 1) / kc);
-      const double c = (double)((blocks + ncu - 1) / ncu) * (vi + 4) * (kc + 4);
+      const long long blocks = cols * ((kp.Nzp + kc - 1) / kc);
+      const double c = (double)((blocks + ncu - 1) / ncu) * ((vi + 4) * w / 64.0) * (kc + 4);
       if (bestc < 0 || c < bestc * 0.999) {
         bestc = c;
-        best = ViscTile{vi, kc};
+        best = ViscTile{vi, kc, w};
       }
     }
-    if (fenv) break;
   }
-  if (best.VI != 4 && best.VI != 6 && best.VI != 7 && best.VI != 8 && best.VI != 12) best.VI = 8;
   return best;
+}
+
+template <typename T, int MODE, int VI, bool PF, int W>
+static void launch_qt(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt, int KC,
+                      int rm, hipStream_t s) {
+  hipLaunchKernelGGL((k_visc_update_qt<T, MODE, VI, PF, W>), visc_grid(kp, KC, VI, W), dim3(W, VI + 4, 1), 0, s, kp,
+                     a, U, Un, Q, Rout, dt, KC, rm);
 }
 
 template <typename T, int MODE>
 static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt,
                                const ViscTile& vt, int rm, hipStream_t s) {
-  const int f = vt.VI, KC = vt.KC;
-  const bool pf = visc_pf((int)sizeof(T)) && (f == 4 || (f != 12 && sizeof(T) == 4));
-  // the forms built without the prefetch: 4, 8 or (FP32-bulk) 12 interior rows
-  const int rows = pf ? f : (f <= 4 ? 4 : (f <= 8 || sizeof(T) == 8) ? 8 : 12);
-  const dim3 g = visc_grid(kp, KC, rows);
+  constexpr int es = (int)sizeof(T);
+  const bool pf = visc_pf(es) != 0;
+  int vi = vt.VI, w = vt.W;
+  if (!visc_form_built(es, pf, vi, w)) {
+    // the forms built without the prefetch: 4, 8 or (FP32-bulk) 12 rows of 64
+    w = 64;
+    vi = pf ? (es == 8 ? 4 : 8) : (vi <= 4 ? 4 : (vi <= 8 || es == 8) ? 8 : 12);
+  }
+  const int KC = vt.KC;
+#define F3D_QT(V, P, WW) launch_qt<T, MODE, V, P, WW>(kp, a, U, Un, Q, Rout, dt, KC, rm, s)
   if (pf) {
-    if (rows == 4)
-      hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, true>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
-                         rm);
-    else if constexpr (sizeof(T) == 4) {
-      if (rows == 6)
-        hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 6, true>), g, dim3(VX, 10, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                           KC, rm);
-      else if (rows == 7)
-        hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 7, true>), g, dim3(VX, 11, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                           KC, rm);
-      else
-        hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, true>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                           KC, rm);
+    if (w == 64 && vi == 4) F3D_QT(4, true, 64);
+    if constexpr (es == 8) {
+      if (w == 32 && vi == 8) F3D_QT(8, true, 32);
+      if (w == 32 && vi == 12) F3D_QT(12, true, 32);
+    } else {
+      if (w == 64 && vi == 6) F3D_QT(6, true, 64);
+      if (w == 64 && vi == 7) F3D_QT(7, true, 64);
+      if (w == 64 && vi == 8) F3D_QT(8, true, 64);
+      if (w == 32 && vi == 12) F3D_QT(12, true, 32);
+      if (w == 32 && vi == 16) F3D_QT(16, true, 32);
+      if (w == 32 && vi == 20) F3D_QT(20, true, 32);
     }
     return;
   }
-  if (rows == 4)
-    hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 4, false>), g, dim3(VX, 8, 1), 0, s, kp, a, U, Un, Q, Rout, dt, KC,
-                       rm);
-  else if (rows == 8)
-    hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 8, false>), g, dim3(VX, 12, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                       KC, rm);
-  else if constexpr (sizeof(T) == 4)
-    hipLaunchKernelGGL((k_visc_update_qt<T, MODE, 12, false>), g, dim3(VX, 16, 1), 0, s, kp, a, U, Un, Q, Rout, dt,
-                       KC, rm);
+  if (vi == 4) F3D_QT(4, false, 64);
+  if (vi == 8) F3D_QT(8, false, 64);
+  if constexpr (es == 4) {
+    if (vi == 12) F3D_QT(12, false, 64);
+  }
+#undef F3D_QT
 }
 
 // Planes per block of the marching sensor (F3D_SENSOR_KS; 0 = one plane per block, k_sensor_f).

@@ -1,6 +1,7 @@
 """The measurement switches of the library (environment variables read once per process: the
-kernel-by-kernel chain, the separate viscous kernels, the update tile forms and march lengths, the
-prefetch, the march segments, the sensor planes per block, the block order) against the oracle.
+kernel-by-kernel chain, the separate viscous kernels, the update tile forms (rows of 64 or two rows of
+32 lanes per wave) and march lengths, the prefetch, the march segments, the sensor planes per block, the
+block order) against the oracle.
 
 Each variant runs in a child process (tests/switch_child.py) on a ramp case with a planted pressure
 jump and spikes (faces on WENO7, WENO5 and NND), large enough for several update tiles in x and y and
@@ -28,6 +29,10 @@ VARIANTS = [
     ("form8", {"F3D_VISC_FORM": "8"}),
     ("form12", {"F3D_VISC_FORM": "12"}),
     ("form6_kc16", {"F3D_VISC_FORM": "6", "F3D_VISC_KC": "16"}),
+    ("w32_form12", {"F3D_VISC_W": "32", "F3D_VISC_FORM": "12"}),
+    ("w32_form8_kc8", {"F3D_VISC_W": "32", "F3D_VISC_FORM": "8", "F3D_VISC_KC": "8"}),
+    ("w32_form20", {"F3D_VISC_W": "32", "F3D_VISC_FORM": "20"}),
+    ("w32_form16", {"F3D_VISC_W": "32", "F3D_VISC_FORM": "16"}),
     ("no_prefetch", {"F3D_VISC_PF": "0"}),
     ("segments", {"F3D_MARCH_SEG_Y": "16", "F3D_MARCH_SEG_Z": "8"}),
     ("sensor_plane", {"F3D_SENSOR_KS": "0"}),
