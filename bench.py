@@ -91,7 +91,7 @@ ALG_BYTES_FVM = {"f64": 3 * (40 + 96 + 72 + 8 + 40) + 80.0, "f32b": 3 * (40 + 48
 # HBM bytes per launch actually moved, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # default workload (ramp 512x192x64, FP64), each scaled by what the counter reads for a 1 GiB device
 # copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); average over the three stage variants of the kernel.
-MEASURED_TRAFFIC = {("fused_update", "512x192x64", "f64"): (2246.4e6 + 2243.0e6 + 1995.6e6 + 559.5e6 + 559.3e6 + 814.4e6) / 3}
+MEASURED_TRAFFIC = {("fused_update", "512x192x64", "f64"): (2328.3e6 + 2324.3e6 + 2059.6e6 + 558.0e6 + 558.2e6 + 811.1e6) / 3}
 
 
 def build_case(f3d, cases, torch, nx, ny, nz, seed=2, sweep=0.0):
