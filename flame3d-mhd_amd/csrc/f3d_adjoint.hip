@@ -31,8 +31,7 @@ struct f3d_adj_workspace {
   double* Lam;         // scale * J * lambda (5 vars, cell layout, ghosts zero)
   double* lamQ;        // adjoint of Q (6 vars, cell layout)
   double* G;           // viscous partial adjoints (16 vars, cell layout, interior+-2 box)
-  double* Us[3];       // stage inputs U^n, U^1, U^2 of one step (5 vars each)
-  double* Qs[3];       // their primitive states with ghosts (6 vars each)
+  double* Qs[3];       // primitive states with ghosts of the stage inputs U^n, U^1, U^2 of one step
   double* Un;          // Un of the step being recomputed
   double* lam;         // adjoint of the state (5 vars)
   double* lamUn;       // adjoint of Un (5 vars)
@@ -504,6 +503,42 @@ __global__ __launch_bounds__(256) void k_adj_finalize(KP kp, const double* __res
   }
 }
 
+// k_adj_finalize inside the multi-step adjoint, with the two passes that follow it folded in: (SEED, KRK 1)
+// the objective seed of the previous step, lam += c2Prim^T(wscale * W) at the same U (c2Prim^T is linear in
+// its argument, so it is taken once of lamQ + wscale * W), and Lam = nscale * J * lam_s of the next stage
+// VJP (k_adj_lambda's product on the value just formed).
+template <int KRK, bool SEED>
+__global__ __launch_bounds__(256) void k_adj_finalize_next(KP kp, const double* __restrict__ lamQ, const double* Us,
+                                                           const double* lam_out, double* lam_s, double* lamUn,
+                                                           const double* __restrict__ W, double wscale,
+                                                           const double* __restrict__ J, double nscale,
+                                                           double* __restrict__ Lam) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= kp.Nxp || j >= kp.Nyp) return;
+  const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), nc = kp.nc;
+  const double a = KRK == 1 ? 1.0 : (KRK == 2 ? 0.25 : 2.0 / 3.0);
+  const double b = KRK == 1 ? 0.0 : (KRK == 2 ? 0.75 : 1.0 / 3.0);
+  double U[5], gQ[6], gU[5], lo[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    U[n] = Us[o + n * nc];
+    lo[n] = lam_out[o + n * nc];
+  }
+#pragma unroll
+  for (int n = 0; n < 6; ++n) gQ[n] = SEED ? lamQ[o + n * nc] + wscale * W[o + n * nc] : lamQ[o + n * nc];
+  c2prim_vjp(kp, U, gQ, gU);
+  const double sj = nscale * J[o];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    double r = a * lo[n] + gU[n];
+    if (KRK == 1) r += lamUn[o + n * nc];
+    if (KRK == 2) lamUn[o + n * nc] += b * lo[n];
+    if (KRK == 3) lamUn[o + n * nc] = b * lo[n];
+    lam_s[o + n * nc] = r;
+    Lam[o + n * nc] = sj * r;
+  }
+}
+
 // c2Prim^T of one cell into a U-shaped adjoint: lam += scale * c2Prim^T(W) at U (objective seed).
 __global__ __launch_bounds__(256) void k_adj_seed(KP kp, const double* __restrict__ U, const double* __restrict__ W,
                                                   double scale, double* __restrict__ lam) {
@@ -634,8 +669,9 @@ HList f3d::adj_heavy_layout(const KP& kp) {
 
 namespace {
 
+// lam_ready: aws->Lam already holds scale * J * lam (formed by the previous finalize pass).
 int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam, double scale, const double* Q,
-                 const double* const* metrics, const double* J, double* lamQ, hipStream_t s) {
+                 const double* const* metrics, const double* J, double* lamQ, hipStream_t s, bool lam_ready = false) {
   const KP kp = make_kp(p);
   MetricPtrs<double> mp;
   for (int q = 0; q < 9; ++q) mp.m[q] = metrics[q];
@@ -650,7 +686,7 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
     hipLaunchKernelGGL(k_adj_heavy_list<1>, adj_list_grid(kp, 1), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl.d[1]);
     hipLaunchKernelGGL(k_adj_heavy_list<2>, adj_list_grid(kp, 2), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl.d[2]);
   }
-  {
+  if (!lam_ready) {
     TimedScope ts("adj_lambda", s);
     hipLaunchKernelGGL(k_adj_lambda, grid_interior(kp), kB, 0, s, kp, lam, J, scale, aws->Lam);
   }
@@ -701,22 +737,47 @@ int ghost_vjp_impl(const f3d_params* p, double* lamQ, const double* Q, double* l
   return check_launch("ghost vjp");
 }
 
+// Next: the multi-step driver's fold of the following passes into the finalize (k_adj_finalize_next);
+// nscale == 0 keeps the plain finalize.
+struct FinNext {
+  bool lam_ready = false;       // aws->Lam already formed for this stage
+  double nscale = 0.0;          // scale of the next stage VJP's Lam (0: not formed here)
+  const double* W = nullptr;    // objective weights of the seed folded into KRK 1 (nullptr: none)
+  double wscale = 0.0;
+};
+
 int stage_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, int KRK, const double* lam_out, double* lam_s,
                    double* lamUn, const double* Us, const double* Qs, const double* const* metrics, const double* J,
-                   double dt, double* lam_inflow, double* lam_bc, hipStream_t s) {
+                   double dt, double* lam_inflow, double* lam_bc, hipStream_t s, const FinNext& nx = FinNext()) {
   const KP kp = make_kp(p);
   const double a = KRK == 1 ? 1.0 : (KRK == 2 ? 0.25 : 2.0 / 3.0);
-  int rc = rhs_vjp_impl(p, aws, lam_out, a * dt, Qs, metrics, J, aws->lamQ, s);
+  int rc = rhs_vjp_impl(p, aws, lam_out, a * dt, Qs, metrics, J, aws->lamQ, s, nx.lam_ready);
   if (rc) return rc;
   rc = ghost_vjp_impl(p, aws->lamQ, Qs, lam_inflow, lam_bc, s);
   if (rc) return rc;
   TimedScope ts("adj_finalize", s);
+  const dim3 g = grid_interior(kp);
+  if (nx.nscale != 0.0) {
+    if (KRK == 1 && nx.W)
+      hipLaunchKernelGGL((k_adj_finalize_next<1, true>), g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn, nx.W,
+                         nx.wscale, J, nx.nscale, aws->Lam);
+    else if (KRK == 1)
+      hipLaunchKernelGGL((k_adj_finalize_next<1, false>), g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
+                         nx.W, nx.wscale, J, nx.nscale, aws->Lam);
+    else if (KRK == 2)
+      hipLaunchKernelGGL((k_adj_finalize_next<2, false>), g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
+                         nx.W, nx.wscale, J, nx.nscale, aws->Lam);
+    else
+      hipLaunchKernelGGL((k_adj_finalize_next<3, false>), g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
+                         nx.W, nx.wscale, J, nx.nscale, aws->Lam);
+    return check_launch("stage vjp finalize");
+  }
   if (KRK == 1)
-    hipLaunchKernelGGL(k_adj_finalize<1>, grid_interior(kp), kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
+    hipLaunchKernelGGL(k_adj_finalize<1>, g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
   else if (KRK == 2)
-    hipLaunchKernelGGL(k_adj_finalize<2>, grid_interior(kp), kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
+    hipLaunchKernelGGL(k_adj_finalize<2>, g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
   else
-    hipLaunchKernelGGL(k_adj_finalize<3>, grid_interior(kp), kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
+    hipLaunchKernelGGL(k_adj_finalize<3>, g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn);
   return check_launch("stage vjp finalize");
 }
 
@@ -747,7 +808,7 @@ F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace*
   if (rc) return rc;
   const size_t nc = f3d_cell_count(p), v = nc * sizeof(double);
   // Lam 5, lamQ 6, G 16, Us 3x5, Qs 3x6, Un 5, lam 5, lamUn 5, Ucur 5, Qcur 6, red
-  const size_t nvars = 5 + 6 + 16 + 15 + 18 + 5 + 5 + 5 + 5 + 6;
+  const size_t nvars = 5 + 6 + 16 + 18 + 5 + 5 + 5 + 5 + 6;
   const size_t total = nvars * align_up(v) + align_up(kObjBlocks * sizeof(double));
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, total);
@@ -770,7 +831,6 @@ F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace*
   a->Lam = take(5);
   a->lamQ = take(6);
   a->G = take(16);
-  for (int q = 0; q < 3; ++q) a->Us[q] = take(5);
   for (int q = 0; q < 3; ++q) a->Qs[q] = take(6);
   a->Un = take(5);
   a->lam = take(5);
@@ -941,6 +1001,7 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
   F3D_HIP(hipMemcpyAsync(Ufin, aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
   // ---- reverse sweep
   hipLaunchKernelGGL(k_adj_zero, dim3(1024), dim3(256), 0, s, aws->lam, (long long)(5 * nc));
+  bool seeded = false;  // the seed of the step being reversed was folded into the previous finalize
   for (int c = nck - 1; c >= 0 && !rc; --c) {
     const int n0 = c * every, n1 = (n0 + every < nsteps) ? n0 + every : nsteps;
     // recompute the steps of the segment from its checkpoint, keeping U^{n0+1} .. U^{n1-1} and the
@@ -955,23 +1016,31 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
       }
     }
     for (int n = n1 - 1; n >= n0 && !rc; --n) {
-      const double* Unext = (n + 1 == nsteps) ? Ufin : (n + 1 == n1 ? slot(c + 1) : seg(n + 1 - n0));
       const double* Ustart = (n == n0) ? slot(c) : seg(n - n0);
-      {
+      if (!seeded) {  // the last step: seed at U^nsteps (the others are folded into the finalize of step n + 1)
         TimedScope ts("adj_seed", s);
-        hipLaunchKernelGGL(k_adj_seed, grid_interior(kp), kB, 0, s, kp, Unext, W, dt, aws->lam);
+        hipLaunchKernelGGL(k_adj_seed, grid_interior(kp), kB, 0, s, kp, Ufin, W, dt, aws->lam);
       }
-      // stage states of step n: (U^n, Q^n), (U^1, Q^1), (U^2, Q^2); the stages' Q is c2Prim + ghost fill
-      // of their U, as the stage update itself forms it
-      F3D_HIP(hipMemcpyAsync(aws->Us[0], Ustart, vb, hipMemcpyDeviceToDevice, s));
-      rc = prim(aws->Us[0], aws->Qs[0]);
-      for (int q = 1; q <= 2 && !rc; ++q) {
-        F3D_HIP(hipMemcpyAsync(aws->Us[q], stg(n, n0, q), vb, hipMemcpyDeviceToDevice, s));
-        rc = prim(aws->Us[q], aws->Qs[q]);
+      // stage states of step n: (U^n, Q^n), (U^1, Q^1), (U^2, Q^2), U read where it is kept (prim writes
+      // only Q); the stages' Q is c2Prim + ghost fill of their U, as the stage update itself forms it
+      const double* Ust[3] = {Ustart, stg(n, n0, 1), stg(n, n0, 2)};
+      for (int q = 0; q < 3 && !rc; ++q) rc = prim(Ust[q], aws->Qs[q]);
+      // each finalize forms the next stage VJP's Lam; that of stage 1 also adds the seed of step n - 1
+      // (at U^n, its stage-1 state) unless n == 0
+      for (int KRK = 3; KRK >= 1 && !rc; --KRK) {
+        FinNext nx;
+        nx.lam_ready = KRK < 3 || seeded;
+        if (KRK > 1) {
+          nx.nscale = (KRK == 3 ? 0.25 : 1.0) * dt;
+        } else if (n > 0) {
+          nx.nscale = (2.0 / 3.0) * dt;
+          nx.W = W;
+          nx.wscale = dt;
+        }
+        rc = stage_vjp_impl(p, aws, KRK, aws->lam, aws->lam, aws->lamUn, Ust[KRK - 1], aws->Qs[KRK - 1], metrics,
+                            J, dt, lam_inflow, lam_bc, s, nx);
       }
-      for (int KRK = 3; KRK >= 1 && !rc; --KRK)
-        rc = stage_vjp_impl(p, aws, KRK, aws->lam, aws->lam, aws->lamUn, aws->Us[KRK - 1], aws->Qs[KRK - 1], metrics,
-                            J, dt, lam_inflow, lam_bc, s);
+      seeded = true;
     }
   }
   if (rc) return rc;
