@@ -1007,7 +1007,15 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
     // recompute the steps of the segment from its checkpoint, keeping U^{n0+1} .. U^{n1-1} and the
     // stage states of every step; its last step needs stages 1 and 2 only (U^{n1} is checkpointed).
     // The forward sweep kept all of this for every == 1 and for the last segment.
-    if (every > 1 && c < nck - 1) {
+    if (every > 1 && c < nck - 1 && use_fused()) {
+      // the stages read the kept states where they are and write the next ones there: no copies
+      rc = prim(slot(c), aws->Qcur);
+      for (int n = n0; n < n1 && !rc; ++n) {
+        const bool third = n + 1 < n1;
+        rc = step_keep_stages(p, aws->fws, n == n0 ? slot(c) : seg(n - n0), aws->Un, aws->Qcur, metrics, J, dt,
+                              inflow, stg(n, n0, 1), stg(n, n0, 2), s, third, third ? seg(n + 1 - n0) : nullptr);
+      }
+    } else if (every > 1 && c < nck - 1) {
       F3D_HIP(hipMemcpyAsync(aws->Ucur, slot(c), vb, hipMemcpyDeviceToDevice, s));
       rc = prim(aws->Ucur, aws->Qcur);
       for (int n = n0; n < n1 && !rc; ++n) {

@@ -62,6 +62,7 @@ struct FArgs {
   unsigned long long* heavy;
   unsigned int* hcount;
   const double* ldt = nullptr;  // per-cell dt of local time stepping (div_LTS), else the scalar dt
+  double* Uo = nullptr;         // conserved state written by the update (nullptr: U itself, in place)
 };
 
 __device__ __forceinline__ long long ridx(const KP& kp, int i, int j, int k) {
@@ -410,9 +411,10 @@ __global__ __launch_bounds__(256) void k_viscous_march(KP kp, FArgs<T> a, T* __r
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __restrict__ Fvx,
                                                 const T* __restrict__ Fvy, const T* __restrict__ Fvz,
-                                                double* __restrict__ U, double* __restrict__ Un, T* __restrict__ Q,
+                                                double* U, double* __restrict__ Un, T* __restrict__ Q,
                                                 double* __restrict__ Rout, double dt, int KC, int remap) {
   const Blk B = xcd_block(remap);
+  double* const Uw = a.Uo ? a.Uo : U;
   const int i = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y;
   const bool act = i < kp.Nxp && j < kp.Nyp;
   const int ic = min(i, kp.Nxp - 1), jc = min(j, kp.Nyp - 1);
@@ -481,7 +483,7 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
         r = u1 * (2.0 / 3.0) + Un[o + n * nc] * (1.0 / 3.0);
       }
       un[n] = r;
-      U[o + n * nc] = r;
+      Uw[o + n * nc] = r;
     }
     // c2Prim  utils.jl:2-30
     const double eps = 2.220446049250313e-16;
@@ -507,7 +509,7 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
 // Conservative update + RK3 blend + c2Prim of one interior cell (utils.jl:2-70, div.jl:40-48);
 // MODE 0 writes the RHS instead.
 template <typename T, int MODE>
-__device__ __forceinline__ void update_cell(const KP& kp, const T* Rt, long long oc, double* __restrict__ U,
+__device__ __forceinline__ void update_cell(const KP& kp, const T* Rt, long long oc, const double* U, double* Uw,
                                             double* __restrict__ Un, T* __restrict__ Q, double* __restrict__ Rout,
                                             double Jc, double dt, const double* __restrict__ ldt) {
   const long long nc = kp.nc;
@@ -532,7 +534,7 @@ __device__ __forceinline__ void update_cell(const KP& kp, const T* Rt, long long
       r = u1 * (2.0 / 3.0) + Un[oc + n * nc] * (1.0 / 3.0);
     }
     un[n] = r;
-    U[oc + n * nc] = r;
+    Uw[oc + n * nc] = r;
   }
   const double eps = 2.220446049250313e-16;
   const double rho = jmax(un[0], eps);
@@ -819,10 +821,10 @@ __global__ __launch_bounds__(W * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T>
       Rt[n + 1] += sdiv;
     }
     if constexpr (PF) {
-      update_cell_ops<T, MODE>(kp, Rt, oc, op, U, Un, Q, Rout);
+      update_cell_ops<T, MODE>(kp, Rt, oc, op, a.Uo ? a.Uo : U, Un, Q, Rout);
       if (s + 1 < nk) load_ops<T, MODE>(kp, a, U, Un, oc + sz, ro + (long long)kp.Nxp * kp.Nyp, dt, op);
     } else {
-      update_cell<T, MODE>(kp, Rt, oc, U, Un, Q, Rout, a.J[oc], dt, a.ldt);
+      update_cell<T, MODE>(kp, Rt, oc, U, a.Uo ? a.Uo : U, Un, Q, Rout, a.J[oc], dt, a.ldt);
     }
   }
 }
@@ -1079,13 +1081,15 @@ static int fused_flux(const KP& kp, const WsView<T>& ws, const T* Q, const Metri
 
 template <typename T>
 int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double* Un, const T* Qin, T* Qout,
-                 const MetricPtrs<T>& m, const double* J, const T* phi, double* Rout, double dt, hipStream_t s) {
+                 const MetricPtrs<T>& m, const double* J, const T* phi, double* Rout, double dt, hipStream_t s,
+                 double* Uo) {
   if (mode != 0 && Qin == Qout && stage_needs_separate_q(kp)) {
     set_error("fused stage: the output state must not alias the input state");
     return F3D_E_ARG;
   }
   FArgs<T> a{Qin, m, J, phi, ws.R, ws.heavy, ws.hcount};
   if (mode != 0 && kp.lts) a.ldt = ws.ldt;
+  if (mode != 0) a.Uo = Uo;
   const int rm = xcd_remap();
   // LES configurations take the two-kernel form (viscous fluxes through HBM): the in-place viscous
   // update is sized to the laminar cell's registers
@@ -1143,7 +1147,7 @@ bool stage_needs_separate_q(const KP& kp) {
 
 template <typename T>
 int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
-                const MetricPtrs<T>& m, const double* J, double dt, hipStream_t s) {
+                const MetricPtrs<T>& m, const double* J, double dt, hipStream_t s, double* Uo) {
   if (Qin == Qout && stage_needs_separate_q(kp)) {
     set_error("fused stage: the output state must not alias the input state");
     return F3D_E_ARG;
@@ -1151,7 +1155,7 @@ int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* U
   fused_sensor<T>(kp, ws, Qin, ws.phi, s);
   int rc = fused_flux<T>(kp, ws, Qin, m, J, ws.phi, s);
   if (rc) return rc;
-  return fused_update<T>(kp, ws, KRK, U, Un, Qin, Qout, m, J, ws.phi, nullptr, dt, s);
+  return fused_update<T>(kp, ws, KRK, U, Un, Qin, Qout, m, J, ws.phi, nullptr, dt, s, Uo);
 }
 
 template int fused_rhs<double>(const KP&, const WsView<double>&, const double*, const MetricPtrs<double>&,
@@ -1159,14 +1163,14 @@ template int fused_rhs<double>(const KP&, const WsView<double>&, const double*, 
 template int fused_rhs<float>(const KP&, const WsView<float>&, const float*, const MetricPtrs<float>&, const double*,
                               float*, double*, hipStream_t);
 template int fused_stage<double>(const KP&, const WsView<double>&, int, double*, double*, const double*, double*,
-                                 const MetricPtrs<double>&, const double*, double, hipStream_t);
+                                 const MetricPtrs<double>&, const double*, double, hipStream_t, double*);
 template int fused_stage<float>(const KP&, const WsView<float>&, int, double*, double*, const float*, float*,
-                                const MetricPtrs<float>&, const double*, double, hipStream_t);
+                                const MetricPtrs<float>&, const double*, double, hipStream_t, double*);
 template void fused_sensor<double>(const KP&, const WsView<double>&, const double*, double*, hipStream_t);
 template void fused_sensor<float>(const KP&, const WsView<float>&, const float*, float*, hipStream_t);
 template int fused_update<double>(const KP&, const WsView<double>&, int, double*, double*, const double*, double*,
                                   const MetricPtrs<double>&, const double*, const double*, double*, double,
-                                  hipStream_t);
+                                  hipStream_t, double*);
 template int fused_update<float>(const KP&, const WsView<float>&, int, double*, double*, const float*, float*,
-                                 const MetricPtrs<float>&, const double*, const float*, double*, double, hipStream_t);
+                                 const MetricPtrs<float>&, const double*, const float*, double*, double, hipStream_t, double*);
 }  // namespace f3d

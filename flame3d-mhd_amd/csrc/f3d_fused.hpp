@@ -177,7 +177,7 @@ int fused_rhs(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>
 // still reads Qin of neighbouring columns while other blocks write Qout).
 template <typename T>
 int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
-                const MetricPtrs<T>& m, const double* J, double dt, hipStream_t s);
+                const MetricPtrs<T>& m, const double* J, double dt, hipStream_t s, double* Uo = nullptr);
 bool stage_needs_separate_q(const KP& kp);
 
 // Building blocks shared by the two inviscid branches.  fused_sensor: shock sensor into phi (and the
@@ -187,8 +187,10 @@ bool stage_needs_separate_q(const KP& kp);
 template <typename T>
 void fused_sensor(const KP& kp, const WsView<T>& ws, const T* Q, T* phi, hipStream_t s);
 template <typename T>
+// Uo != nullptr (stages): the updated conserved state goes to Uo and U is only read.
 int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double* Un, const T* Qin, T* Qout,
-                 const MetricPtrs<T>& m, const double* J, const T* phi, double* Rout, double dt, hipStream_t s);
+                 const MetricPtrs<T>& m, const double* J, const T* phi, double* Rout, double dt, hipStream_t s,
+                 double* Uo = nullptr);
 
 // Finite-volume branch (f3d_fvm.hip): RHS and stage as fused_rhs / fused_stage, reconstructing the
 // conserved state U (ghost cells included); geo[3]: face geometry arrays (f3d_fvm_geometry_*).
@@ -203,17 +205,22 @@ int fvm_fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, doubl
 // geo == nullptr: finite-difference branch; otherwise finite-volume (p->fvm) with these face geometry
 // arrays, and the ghost procedure refills U's ghost cells as well.
 template <typename T>
+// Uo != nullptr (fused finite-difference path only): the stage writes its conserved state to Uo and
+// leaves U as it was.
 int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
-             const T* const* geo = nullptr);
+             const T* const* geo = nullptr, double* Uo = nullptr);
 template <typename T>
 int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, double* Un, T* Q,
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
              const T* const* geo = nullptr);
-// One FP64 RK3 step of the finite-difference branch as steps_io, copying the conserved state after
-// stage 1 into U1 and after stage 2 into U2 (the adjoint's stage states); third = false stops after
-// stage 2 (U, Q then hold the second stage's state).  No filtering block.
+// One FP64 RK3 step of the finite-difference branch as steps_io that also leaves the conserved state
+// after stage 1 in U1 and after stage 2 in U2 (the adjoint's stage states); third = false stops after
+// stage 2.  Fused path: the stages write U1, U2 and then Uout (U when Uout is null) directly, U is only
+// read.  Other paths: in place on U with copies into U1, U2 (Uout must be null; after third = false U
+// holds the second stage's state).  Q: the primitive state of U on entry, of the step's result on return
+// (of the second stage when third = false).  No filtering block.
 int step_keep_stages(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, double* Q,
                      const double* const* metrics, const double* J, double dt, const double* inflow, double* U1,
-                     double* U2, hipStream_t s, bool third = true);
+                     double* U2, hipStream_t s, bool third = true, double* Uout = nullptr);
 }  // namespace f3d

@@ -245,8 +245,12 @@ namespace f3d {
 template <typename T>
 int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, const T* Qin, T* Qout,
              const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,
-             const T* const* geo) {
+             const T* const* geo, double* Uo) {
   KP kp = make_kp(p);
+  if (Uo && (geo || !use_fused())) {
+    set_error("stage: a separate output state is supported by the fused finite-difference stage only");
+    return F3D_E_ARG;
+  }
   MetricPtrs<T> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
   int rc = F3D_OK;
@@ -255,7 +259,7 @@ int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double*
   if (rc) return rc;
   if (use_fused()) {
     rc = geo ? fvm_fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, geo, dt, s)
-                : fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, dt, s);
+                : fused_stage<T>(kp, ws_view<T>(ws), KRK, U, Un, Qin, Qout, m, J, dt, s, Uo);
   } else {
     if (Qout != Qin) F3D_HIP(hipMemcpyAsync(Qout, Qin, sizeof(T) * 6 * kp.nc, hipMemcpyDeviceToDevice, s));
     if (KRK == 1) F3D_HIP(hipMemcpyAsync(Un, U, sizeof(double) * 5 * kp.nc, hipMemcpyDeviceToDevice, s));  // solver.jl:418
@@ -275,7 +279,7 @@ int stage_io(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double*
   // would have set first are all overwritten with the same final values.
   // U ghosts are read only by the finite-volume reconstruction and the filters; fillGhost refills them
   // from Q's there (boundary.jl fills them always)
-  if (!rc) rc = launch_fill<T>(kp, Qout, (geo || p->filtering) ? U : nullptr, inflow, s);
+  if (!rc) rc = launch_fill<T>(kp, Qout, (geo || p->filtering) ? (Uo ? Uo : U) : nullptr, inflow, s);
   return rc;
 }
 
@@ -303,12 +307,22 @@ int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, doub
 
 int step_keep_stages(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, double* Q,
                      const double* const* metrics, const double* J, double dt, const double* inflow, double* U1,
-                     double* U2, hipStream_t s, bool third) {
+                     double* U2, hipStream_t s, bool third, double* Uout) {
   const KP kp = make_kp(p);
   const bool rot = stage_needs_separate_q(kp);
   double* Qa = rot ? (double*)ws->Qa : Q;
   double* Qb = rot ? (double*)ws->Qb : Q;
   const size_t vb = sizeof(double) * 5 * kp.nc;
+  if (use_fused()) {  // each stage reads its input state and writes the next one: no copies
+    int rc = stage_io<double>(p, ws, 1, U, Un, Q, Qa, metrics, J, dt, inflow, s, nullptr, U1);
+    if (!rc) rc = stage_io<double>(p, ws, 2, U1, Un, Qa, Qb, metrics, J, dt, inflow, s, nullptr, U2);
+    if (!rc && third) rc = stage_io<double>(p, ws, 3, U2, Un, Qb, Q, metrics, J, dt, inflow, s, nullptr, Uout ? Uout : U);
+    return rc;
+  }
+  if (Uout) {
+    set_error("step_keep_stages: a separate output state needs the fused stage");
+    return F3D_E_ARG;
+  }
   int rc = stage_io<double>(p, ws, 1, U, Un, Q, Qa, metrics, J, dt, inflow, s);
   if (!rc) F3D_HIP(hipMemcpyAsync(U1, U, vb, hipMemcpyDeviceToDevice, s));
   if (!rc) rc = stage_io<double>(p, ws, 2, U, Un, Qa, Qb, metrics, J, dt, inflow, s);
@@ -332,10 +346,10 @@ int stage_inplace(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, do
 
 template int stage_io<double>(const f3d_params*, f3d_workspace*, int, double*, double*, const double*, double*,
                               const double* const*, const double*, double, const double*, hipStream_t,
-                              const double* const*);
+                              const double* const*, double*);
 template int stage_io<float>(const f3d_params*, f3d_workspace*, int, double*, double*, const float*, float*,
                              const float* const*, const double*, double, const float*, hipStream_t,
-                             const float* const*);
+                             const float* const*, double*);
 template int steps_io<double>(const f3d_params*, f3d_workspace*, int, double*, double*, double*,
                               const double* const*, const double*, double, const double*, hipStream_t,
                               const double* const*);
