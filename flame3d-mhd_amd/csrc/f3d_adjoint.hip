@@ -513,7 +513,8 @@ __global__ __launch_bounds__(256) void k_adj_finalize_next(KP kp, const double* 
                                                            const double* __restrict__ W, double wscale,
                                                            const double* __restrict__ J, double nscale,
                                                            double* __restrict__ Lam) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  // blocks of 256 cells along one row: 2 KB runs of each of the ~30 arrays streamed
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
   if (i >= kp.Nxp || j >= kp.Nyp) return;
   const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), nc = kp.nc;
   const double a = KRK == 1 ? 1.0 : (KRK == 2 ? 0.25 : 2.0 / 3.0);
@@ -757,18 +758,19 @@ int stage_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, int KRK, const d
   if (rc) return rc;
   TimedScope ts("adj_finalize", s);
   const dim3 g = grid_interior(kp);
+  const dim3 gr((kp.Nxp + 255) / 256, kp.Nyp, kp.Nzp), br(256, 1, 1);
   if (nx.nscale != 0.0) {
     if (KRK == 1 && nx.W)
-      hipLaunchKernelGGL((k_adj_finalize_next<1, true>), g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn, nx.W,
+      hipLaunchKernelGGL((k_adj_finalize_next<1, true>), gr, br, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn, nx.W,
                          nx.wscale, J, nx.nscale, aws->Lam);
     else if (KRK == 1)
-      hipLaunchKernelGGL((k_adj_finalize_next<1, false>), g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
+      hipLaunchKernelGGL((k_adj_finalize_next<1, false>), gr, br, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
                          nx.W, nx.wscale, J, nx.nscale, aws->Lam);
     else if (KRK == 2)
-      hipLaunchKernelGGL((k_adj_finalize_next<2, false>), g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
+      hipLaunchKernelGGL((k_adj_finalize_next<2, false>), gr, br, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
                          nx.W, nx.wscale, J, nx.nscale, aws->Lam);
     else
-      hipLaunchKernelGGL((k_adj_finalize_next<3, false>), g, kB, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
+      hipLaunchKernelGGL((k_adj_finalize_next<3, false>), gr, br, 0, s, kp, aws->lamQ, Us, lam_out, lam_s, lamUn,
                          nx.W, nx.wscale, J, nx.nscale, aws->Lam);
     return check_launch("stage vjp finalize");
   }
