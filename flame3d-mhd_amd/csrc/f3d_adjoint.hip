@@ -925,6 +925,7 @@ F3D_API int f3d_rk3_stage_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, i
 
 F3D_API size_t f3d_adjoint_ckpt_bytes(const f3d_params* p, int32_t nsteps, int32_t every) {
   if (!p || nsteps < 1 || every < 1) return 0;
+  if (every > nsteps) every = nsteps;  // one segment of nsteps steps
   return ckpt_slots(nsteps, every) * 5 * f3d_cell_count(p) * sizeof(double);
 }
 
@@ -939,6 +940,7 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
     set_error("adjoint: nsteps (%d) and every (%d) must be >= 1", nsteps, every);
     return F3D_E_ARG;
   }
+  if (every > nsteps) every = nsteps;  // one segment of nsteps steps (the slot layout below assumes it)
   if (!U0 || !W || !ckpt || !obj || !lamU0) {
     set_error("adjoint: NULL array argument");
     return F3D_E_NULL;

@@ -517,7 +517,8 @@ int f3d_rk3_stage_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t K
 /* Device bytes of the checkpoint buffer f3d_adjoint_f64 needs for `nsteps` with a checkpoint
  * every `every` steps: ceil(nsteps/every) + every + 2*(every == 1 ? nsteps : every) conserved
  * states (checkpoints, the recomputed states of a segment, the final state, and the two stage
- * states of each step of a segment).  Host-only. */
+ * states of each step of a segment); an `every` above nsteps counts as nsteps (one segment).
+ * Host-only. */
 size_t f3d_adjoint_ckpt_bytes(const f3d_params* p, int32_t nsteps, int32_t every);
 
 /* Checkpointed multi-step adjoint.  Forward: Q0 = ghosts(c2Prim(U0)), `nsteps` RK3 steps of

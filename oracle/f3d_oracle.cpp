@@ -2,30 +2,48 @@
 // __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  TEST INFRASTRUCTURE ONLY: the
 // product path (flame3d-mhd_amd/, libf3d_hip.so) never links, loads or calls this file.
 //
-// What pins each oracle function, apart from GPU parity (tests/test_oracle_*.py):
-//   c2prim / prim2c (utils.jl:2-54)   round trip; eps floors; hand-evaluated cell.
-//   shock_sensor (schemes.jl:1-24)    constant p -> 0; closed form on a planted jump; brute numpy.
-//   scheme_mask (schemes.jl:724-1009) isolated jump -> the known band pattern at distance <= 3.
-//   flux_split_sw (split.jl:2-68)     Fp+Fm = Euler flux projected on A (exact algebra);
-//                                     Fp has no negative eigen-part (supersonic: Fm = 0).
-//   advect_c UP7 (schemes.jl:731-754) constant fluxes -> F = Fp+Fm; 7th-order convergence.
-//   advect_c WENO7/WENO5/NND          constant state -> exact; equal smoothness -> UP7 / the
-//                                     5th-order linear scheme; L*R = I; minmod truth table;
-//                                     independent numpy restatement on small random cases.
-//   viscous_flux + div (viscous.jl, div.jl) linear fields -> zero viscous RHS; quadratic fields ->
-//                                     closed form; periodic telescoping sum(dU/J) = 0.
-//   exchange_ghost / fill_ghost       index identities on a labelled array (value = index).
-//   rk3 (solver.jl:416-444)           du/dt = lambda*u amplification 1+z+z^2/2+z^3/6.
-//   cfl_dt                            uniform flow closed form.
-//   metrics (parse_mesh.jl:139-248)   Cartesian spacing h -> dxi/dx = h^2, J = h^-3; affine exact.
-//   fvm geometry (FVM.jl:16-31)       affine mesh -> n*Area = the J^-1-scaled metric row of the
-//                                     direction (all three, incl. the j-face orientation flip).
-//   hllc_flux (FVM.jl:1224-1358)      UL = UR -> the Euler flux; F(UL,UR,n) = -F(UR,UL,-n);
-//                                     rotation covariance; an isolated contact is resolved exactly.
-//   fvm reconstruction (FVM.jl:33-226, 707-846) cell averages of polynomials -> the exact face
-//                                     value (UP7 degree 6, WENO7 3, WENO5 2, minmod-left 1), eigen
-//                                     and conser; the right minmod state equals the right cell on
-//                                     monotone data (as written in the reference).
+// What pins each oracle function apart from GPU parity (test file : test name, under tests/):
+//   c2prim / prim2c (utils.jl:2-54)   round trip, eps floors, hand-evaluated cell
+//                                     (test_oracle_invariants: test_c2prim_prim2c_round_trip_and_floors).
+//   shock_sensor (schemes.jl:1-24)    constant p -> 0 and a numpy evaluation (test_sensor_constant_and_numpy).
+//   scheme_mask (schemes.jl:724-1009) band pattern at distance <= 3 of a spike (test_mask_band_around_spike).
+//   flux_split_sw (split.jl:2-68)     Fp+Fm = Euler flux on the metric row; Fm = 0 when supersonic
+//                                     (test_split_sums_to_euler_flux_and_supersonic_limit).
+//   advect_c UP7 (schemes.jl:731-754) 7th-order flux difference (test_up7_seventh_order); uniform state
+//                                     (test_uniform_state_reconstructs_exactly, both tangent branches:
+//                                     test_uniform_state_affine_metrics_both_tangent_branches).
+//   WENO7 / WENO5 (schemes.jl:813-999, FVM.jl:116-201) equal to an exact rational evaluation of their
+//                                     definition (polynomial candidates, Jiang-Shu/Balsara-Shu integral
+//                                     indicators, derived linear weights; FP64, long double, dual) on
+//                                     random stencils with eps negligible, dominant and comparable
+//                                     (test_oracle_weno: test_weno_equals_definition); ENO selection at a
+//                                     step (test_eno_selection_at_a_step); order 7 / 5 on smooth data,
+//                                     scalar and through advect_c and the finite-volume reconstructions
+//                                     with the band forced (test_weno_order_on_smooth_data,
+//                                     test_advect_c_forced_band_order, test_fvm_reconstruct_forced_band_order);
+//                                     linear data -> linear schemes (test_weno_linear_data_reduces_to_linear_schemes).
+//   NND2 (schemes.jl:1000-1009)       minmod truth table (test_nnd_minmod_truth_table).
+//   viscous_flux + div (viscous.jl, div.jl) linear velocity at constant T -> zero momentum RHS and the
+//                                     dissipation in energy (test_viscous_linear_velocity_closed_form);
+//                                     affine fields incl. grad T on sheared meshes against the closed-form
+//                                     flux (test_oracle_les: test_les_general_gradient[LES_NONE]); periodic
+//                                     telescoping sum(dU/J) = 0 (test_periodic_conservation_telescopes);
+//                                     free stream -> 0 (test_free_stream_rhs_is_zero).
+//   exchange_ghost / fill_ghost       index identities on a labelled array (test_ghost_fill_index_identities,
+//                                     test_exchange_periodic_slabs).
+//   rk3 stage blends (solver.jl:416-444) third order in dt on a smooth case, for the finite-difference,
+//                                     finite-volume and forward-mode steps (test_oracle_weno:
+//                                     test_rk3_third_order_in_time).
+//   cfl_dt                            uniform flow closed form (test_cfl_dt_uniform_flow).
+//   metrics (parse_mesh.jl:139-248)   affine grid -> the cofactor matrix (test_metrics_affine_grid_exact).
+//   fvm geometry (FVM.jl:16-31)       affine mesh -> n*Area = the J^-1-scaled metric row of the direction,
+//                                     curved mesh -> second order (test_oracle_fvm).
+//   hllc_flux (FVM.jl:1224-1358)      UL = UR -> the Euler flux; mirror and rotation symmetry; contact and
+//                                     shear exact; the p*-form of the star flux (test_oracle_fvm).
+//   fvm reconstruction (FVM.jl:33-226, 707-846) cell averages of polynomials -> the exact face value
+//                                     (UP7 degree 6, WENO7 3, WENO5 2, minmod-left 1), eigen and conser;
+//                                     the right minmod state as written (test_oracle_fvm).
+//   forward mode (f3d_dual.hpp)       central finite differences (test_oracle_jvp).
 // No reference test, fixture or golden vector exists for this path (the reference has no tests);
 // the CUDA.jl fastmath Float32 outputs of the reference itself are unpinned (no Julia here).
 #include "f3d_oracle.hpp"
@@ -811,6 +829,44 @@ OR_API void oracle_weno7_is_f64(int form, const double* V, double* Is) {
           V5 * (2843 * V5 - 1642 * V6) + V6 * (267 * V6);
   Is[3] = V4 * (2107 * V4 - 9402 * V5 + 7042 * V6 - 1854 * V7) + V5 * (11003 * V5 - 17246 * V6 + 4642 * V7) +
           V6 * (7043 * V6 - 3882 * V7) + V7 * (547 * V7);
+}
+
+// One scalar WENO reconstruction, exactly as the face loops call it: kind 0 the finite-difference
+// WENO7 weno7(V, s0, eps) (schemes.jl:847-879), 1 the finite-difference WENO5 weno5(V[0..4], s0, eps)
+// (schemes.jl:958-972), 2 the finite-volume WENO7 fvm_weno7(V, ss) (FVM.jl:116-147, its own eps
+// 1e-10; `eps` unused), 3 the finite-volume WENO5 fvm_weno5(V, ss) on V[1..5] (FVM.jl:151-201, eps
+// 1e-8).  form 0 double, 1 long double, 2 dual number (value part; kinds 0-1 only), 3 float.  The
+// finite-difference kinds return the weighted sum of the unscaled candidates (the face loops multiply
+// by 1/12 or 1/6 afterwards); the finite-volume kinds return the face value.
+OR_API double oracle_weno_scalar(int kind, int form, const double* V, double s0, double eps) {
+  auto run = [&](auto tag) -> double {
+    using T = decltype(tag);
+    T v[7];
+    for (int s = 0; s < 7; ++s) v[s] = (T)V[s];
+    switch (kind) {
+      case 0: return (double)weno7<T>(v, (T)s0, (T)eps);
+      case 1: return (double)weno5<T>(v, (T)s0, (T)eps);
+      default: return std::nan("");
+    }
+  };
+  auto run_fvm = [&](auto tag) -> double {
+    using T = decltype(tag);
+    T v[7];
+    for (int s = 0; s < 7; ++s) v[s] = (T)V[s];
+    return (double)(kind == 2 ? fvm_weno7<T>(v, (T)s0) : fvm_weno5<T>(v, (T)s0));
+  };
+  if (kind < 0 || kind > 3) return std::nan("");
+  if (kind >= 2) {
+    if (form == 0) return run_fvm(0.0);
+    if (form == 1) return run_fvm((long double)0);
+    if (form == 3) return run_fvm(0.f);
+    return std::nan("");
+  }
+  if (form == 0) return run(0.0);
+  if (form == 1) return run((long double)0);
+  if (form == 2) return run(Dual(0.0));
+  if (form == 3) return run(0.f);
+  return std::nan("");
 }
 
 OR_API int oracle_max_threads(void) {

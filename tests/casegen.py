@@ -160,8 +160,12 @@ def fp64_rhs_check(name, gpu_R, case):
     o.rhs_ld(Rl, case.Q, case.metrics, case.J)
     sc = rhs_scales(case)
     e_gpu, e_or = scaled_err(gpu_R, Rl, sc), scaled_err(R, Rl, sc)
-    print(f"{name}: FP64 RHS vs long double {e_gpu:.2e} (oracle {e_or:.2e}; gpu vs oracle {scaled_err(gpu_R, R, sc):.2e})")
+    e_norm = relerr(gpu_R, R)
+    print(f"{name}: FP64 RHS vs long double {e_gpu:.2e} (oracle {e_or:.2e}; gpu vs oracle {scaled_err(gpu_R, R, sc):.2e}); "
+          f"normwise max|dR|/max|R| vs the FP64 oracle {e_norm:.2e}")
     assert e_gpu <= max(1e-12, 3 * e_or), f"{name}: {e_gpu:.3e} vs oracle floor {e_or:.3e}"
+    # the plain normwise bar, per variable: max over cells |R_gpu - R_oracle| / max over cells |R_oracle|
+    assert e_norm <= 1e-12, f"{name}: normwise {e_norm:.3e} > 1e-12"
     return e_gpu, e_or
 
 

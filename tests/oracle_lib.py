@@ -11,7 +11,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle_f3d.so")
+# F3D_ORACLE_SO: another build of the oracle (a copy with a planted mistake, to check that the pins fail)
+ORACLE_SO = os.environ.get("F3D_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle_f3d.so")
 
 _lib = None
 
@@ -212,3 +213,19 @@ class Oracle:
 
 def max_threads():
     return olib().oracle_max_threads()
+
+
+WENO_FORMS = {"f64": 0, "ld": 1, "dual": 2, "f32": 3}
+
+
+def weno_scalar(kind, form, V, s0, eps=0.0):
+    """One scalar WENO reconstruction of the oracle: kind 0 finite-difference WENO7 (returns 12x the
+    face value), 1 finite-difference WENO5 on V[0..4] (6x), 2 finite-volume WENO7 (the value; its own
+    eps), 3 finite-volume WENO5 on V[1..5] (the value; its own eps).  form: a WENO_FORMS key."""
+    L = olib()
+    f = L.oracle_weno_scalar
+    f.restype = c_double
+    f.argtypes = [c_int, c_int, c_void_p, c_double, c_double]
+    v = np.zeros(7)
+    v[:len(V)] = np.asarray(V, dtype=np.float64)
+    return f(int(kind), WENO_FORMS[form], v.ctypes.data, float(s0), float(eps))

@@ -172,6 +172,9 @@ def test_rhs_autograd_function(torch_mod):
 def _stage_case(name):
     if name == "ramp":
         return ramp_case(16, 12, 8, jump=1.0, spikes=[(5, 3, 2, 20.0)], hybrid_phi3=2.5, interior_noise=1e-3)
+    if name == "swept":
+        return ramp_case(20, 12, 10, sweep=30.0, jump=0.5, spikes=[(6, 4, 3, 20.0)], hybrid_phi3=2.5,
+                         interior_noise=1e-3)
     return tgv_case(12)
 
 
@@ -229,12 +232,17 @@ def _wall_weights(case):
     return W
 
 
-@pytest.mark.parametrize("name", ["ramp", "tgv"])
+MULTISTEP = {"ramp": (3, (2,)), "tgv": (3, (2,)), "swept": (7, (2, 3, 4, 5, 10))}
+
+
+@pytest.mark.parametrize("name", list(MULTISTEP))
 def test_multistep_adjoint(torch_mod, name):
+    """Checkpointed adjoint of `nsteps` steps: every interval (a short last segment after full ones,
+    one segment when every >= nsteps) against every = 1, then against forward mode and central FD."""
     torch = torch_mod
     case = _stage_case(name)
     p, o = case.params, Oracle(case.params)
-    nsteps = 3
+    nsteps, everies = MULTISTEP[name]
     U0 = case.U()
     dt = o.cfl_dt(case.Q, case.metrics, case.J, 0.3)
     W = _wall_weights(case)
@@ -253,10 +261,13 @@ def test_multistep_adjoint(torch_mod, name):
         return host(obj), host(lamU0), (None if lin is None else host(lin)), host(lbc)
 
     obj1, g1, gin1, gbc1 = run(U0, 1)
-    obj2, g2, gin2, gbc2 = run(U0, 2)
-    np.testing.assert_allclose(obj2, obj1, rtol=1e-12)
-    for x, y in ((g2, g1), (gbc2, gbc1)) + (((gin2, gin1),) if gin1 is not None else ()):
-        assert np.max(np.abs(x - y)) <= 1e-8 * max(np.max(np.abs(y)), 1e-300)
+    for every in everies:
+        obj2, g2, gin2, gbc2 = run(U0, every)
+        print(f"{case.name}: every={every} vs every=1: objective {np.max(np.abs(obj2 - obj1)):.1e}, "
+              f"state gradient {np.max(np.abs(g2 - g1)):.1e} (max {np.max(np.abs(g1)):.1e})")
+        np.testing.assert_allclose(obj2, obj1, rtol=1e-12)
+        for x, y in ((g2, g1), (gbc2, gbc1)) + (((gin2, gin1),) if gin1 is not None else ()):
+            assert np.max(np.abs(x - y)) <= 1e-8 * max(np.max(np.abs(y)), 1e-300), every
     ghost = np.ones(U0.shape[1:], dtype=bool)
     ghost[INT[1:]] = False
     assert not g1[:, ghost].any(), "ghost entries of the state gradient stay zero"
@@ -286,3 +297,45 @@ def test_multistep_adjoint(torch_mod, name):
     rel_fd = abs(fd - adj_u) / max(abs(adj_u), 1e-300)
     print(f"{case.name}: state direction FD {fd:.10e} adjoint {adj_u:.10e} rel {rel_fd:.2e}")
     assert rel_fd <= 1e-4
+
+
+@pytest.mark.parametrize("env", [{"F3D_RHS_ENGINE": "chain"}, {}], ids=["chain", "fused"])
+def test_adjoint_engine_child(torch_mod, tmp_path, env):
+    """The checkpointed adjoint (every = 3 over 5 steps: one full segment, then a short one, recomputed
+    through copies with the kernel-by-kernel engine, in place with the fused one) in a child process
+    under the engine switch, against the fused engine's every = 1 adjoint in this process."""
+    import os
+    import subprocess
+    import sys
+    torch = torch_mod
+    case = _stage_case("swept")
+    p, o = case.params, Oracle(case.params)
+    nsteps, every = 5, 3
+    U0 = case.U()
+    dt = o.cfl_dt(case.Q, case.metrics, case.J, 0.3)
+    W = _wall_weights(case)
+    ctx = f3d.Flame3D(p)
+    obj = torch.zeros(nsteps, dtype=torch.float64, device="cuda:0")
+    lamU0 = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
+    lin = torch.zeros(case.inflow.shape, dtype=torch.float64, device="cuda:0")
+    lbc = torch.zeros(7, dtype=torch.float64, device="cuda:0")
+    ctx.adjoint(nsteps, 1, dev(torch, U0), [dev(torch, a) for a in case.metrics], dev(torch, case.J), dt,
+                dev(torch, W), obj, lamU0, dev(torch, case.inflow), lin, lbc)
+    ref = dict(obj=host(obj), lamU0=host(lamU0), lin=host(lin), lbc=host(lbc))
+    src, dst = str(tmp_path / "in.npz"), str(tmp_path / "out.npz")
+    arrays = dict(params=np.frombuffer(bytes(p), dtype=np.uint8), nsteps=np.int64(nsteps), every=np.int64(every),
+                  U0=U0, J=case.J, dt=np.float64(dt), W=W, inflow=case.inflow)
+    arrays.update({f"m{i}": a for i, a in enumerate(case.metrics)})
+    np.savez(src, **arrays)
+    e = {k: v for k, v in os.environ.items() if not k.startswith("F3D_")}
+    e.update(env)
+    print(f"adjoint child {env}", flush=True)
+    r = subprocess.run([sys.executable, os.path.join(os.path.dirname(os.path.abspath(__file__)), "adjoint_child.py"),
+                        src, dst], env=e, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, f"child exit {r.returncode}\n{r.stderr[-2000:]}"
+    out = np.load(dst)
+    np.testing.assert_allclose(out["obj"], ref["obj"], rtol=1e-11)
+    for k in ("lamU0", "lin", "lbc"):
+        err = np.max(np.abs(out[k] - ref[k])) / max(np.max(np.abs(ref[k])), 1e-300)
+        print(f"{env or 'fused'} every={every} {k}: {err:.2e} against fused every=1")
+        assert err <= 1e-7, (k, err)
