@@ -451,6 +451,11 @@ class Flame3D:
         _check(fn(ctypes.byref(self.params), ws.handle, _ptr(R), _ptr(Q), _ptrs(metrics), _ptr(J), _ptr(phi),
                   self.stream))
 
+    def rhs_noalloc(self, R, Q, metrics, J, phi=None):
+        """The FP64 RHS without a workspace (f3d_rhs_noalloc_f64: the library's static per-GPU scratch)."""
+        _check(lib().f3d_rhs_noalloc_f64(ctypes.byref(self.params), _ptr(R), _ptr(Q), _ptrs(metrics), _ptr(J),
+                                         _ptr(phi), self.stream))
+
     def rk3_stage(self, KRK, U, Un, Q, metrics, J, dt, inflow=None):
         """One stage of the loop body solver.jl:416-444."""
         prec = self._prec(Q)

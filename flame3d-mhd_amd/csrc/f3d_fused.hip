@@ -43,6 +43,8 @@ static int visc_fused() {
   return v;
 }
 
+bool fused_needs_fv(const KP& kp) { return kp.viscous && !(visc_fused() && kp.les == F3D_LES_NONE); }
+
 bool use_fused() {
   static int v = -1;
   if (v < 0) {

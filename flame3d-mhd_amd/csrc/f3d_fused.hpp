@@ -156,6 +156,9 @@ WsView<T> ws_view(f3d_workspace* ws);
 // true when the fused sweeps serve f3d_rhs_* / f3d_rk3_stage_* (F3D_RHS_ENGINE=chain selects the
 // kernel-by-kernel chain instead, for A/B measurement).
 bool use_fused();
+// true when the fused stage routes the viscous fluxes through the workspace's Fv arrays (LES models, or
+// F3D_VISC_FUSED=0) instead of computing them inside the update march.
+bool fused_needs_fv(const KP& kp);
 
 // Heavy-face list layouts: the finite-difference sweeps of bulk element size es, the finite-volume
 // sweeps, and the adjoint's list kernels.  The workspace holds the largest of the three.

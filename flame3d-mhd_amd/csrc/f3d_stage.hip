@@ -2,6 +2,7 @@
 // metric computation.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "f3d_internal.hpp"
 #include "f3d_fused.hpp"
@@ -528,6 +529,115 @@ F3D_API int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, cons
   rc = check_branch<float>(p, false, nullptr, nullptr, "rhs_f32b");
   if (rc) return rc;
   return rhs_impl<float>(p, ws, R, Q, metrics, J, phi, (hipStream_t)s);
+}
+
+// ---- FP64 RHS without allocation: the fused RHS of f3d_rhs_f64 with its scratch (the interior RHS
+// accumulator, the heavy-face runs and their counts, and phi when the caller passes none) carved from
+// one file-scope device array instead of a workspace.  Each GPU the library runs on holds its own copy
+// of the array (kNoAllocScratchBytes) until the process ends; its address is looked up once per device.
+// Nothing in it needs clearing between calls: the x sweep writes every accumulator entry before the
+// y/z marches add to it, each sweep wave writes its run's count, and the sensor writes phi.
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct ScratchSlot {
+  std::once_flag once;
+  void* ptr = nullptr;
+  hipError_t err = hipSuccess;
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+// Device address of `scratch` on the current device (looked up with that device current, once).
+int scratch_address(void** out, const char* who) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("%s: hipGetDevice: %s", who, hipGetErrorString(e));
+    return (int)e;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("%s: device %d beyond the %d the library tracks", who, dev, kMaxDevices);
+    return F3D_E_ARG;
+  }
+  ScratchSlot& sl = g_scratch[dev];
+  std::call_once(sl.once, [&] { sl.err = hipGetSymbolAddress(&sl.ptr, HIP_SYMBOL(scratch)); });
+  if (sl.err != hipSuccess || !sl.ptr) {
+    set_error("%s: hipGetSymbolAddress(scratch): %s", who, hipGetErrorString(sl.err));
+    return sl.err != hipSuccess ? (int)sl.err : F3D_E_ARG;
+  }
+  *out = sl.ptr;
+  return F3D_OK;
+}
+
+// Scratch layout of the no-allocation RHS: offsets of R, the heavy-face runs, their counts and phi.
+struct NoAllocLayout {
+  size_t R, heavy, hcount, phi, total;
+  unsigned long long hcap, hncap;
+};
+NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
+  const KP kp = make_kp(p);
+  const HList h = fdm_heavy_layout(kp, 8);
+  NoAllocLayout L;
+  L.hcap = h.slots();
+  L.hncap = h.nruns();
+  size_t off = 0;
+  L.R = off;
+  off += align_up((size_t)p->Nxp * p->Nyp * p->Nzp * 5 * sizeof(double));
+  L.heavy = off;
+  off += align_up(L.hcap * sizeof(unsigned long long));
+  L.hcount = off;
+  off += align_up(L.hncap * sizeof(unsigned int));
+  L.phi = off;
+  if (with_phi) off += align_up(f3d_cell_count(p) * sizeof(double));
+  L.total = off;
+  return L;
+}
+}  // namespace
+
+F3D_API int f3d_rhs_noalloc_f64(const f3d_params* p, double* R, const double* Q, const double* const* metrics,
+                                const double* J, double* phi, f3d_stream_t s) {
+  const char* who = "rhs_noalloc_f64";
+  int rc = f3d_params_check(p);
+  if (rc) return rc;
+  if (!R || !Q || !metrics || !J) {
+    set_error("%s: NULL array argument", who);
+    return F3D_E_NULL;
+  }
+  for (int q = 0; q < 9; ++q)
+    if (!metrics[q]) {
+      set_error("%s: metric %d is NULL", who, q);
+      return F3D_E_NULL;
+    }
+  rc = check_branch<double>(p, false, nullptr, nullptr, who);
+  if (rc) return rc;
+  const KP kp = make_kp(p);
+  if (!use_fused() || fused_needs_fv(kp)) {
+    set_error("%s: runs the fused engine with the viscous fluxes inside the update march only (no LES model, "
+              "F3D_RHS_ENGINE and F3D_VISC_FUSED unset); use f3d_rhs_f64 with a workspace", who);
+    return F3D_E_ARG;
+  }
+  const NoAllocLayout L = noalloc_layout(p, phi == nullptr);
+  if (L.total > kNoAllocScratchBytes) {
+    set_error("%s: a %dx%dx%d grid needs %zu bytes of scratch, the no-allocation entry holds %zu "
+              "(use f3d_rhs_f64 with a workspace)", who, p->Nxp, p->Nyp, p->Nzp, L.total, kNoAllocScratchBytes);
+    return F3D_E_DIMS;
+  }
+  void* base = nullptr;
+  rc = scratch_address(&base, who);
+  if (rc) return rc;
+  char* c = (char*)base;
+  WsView<double> v{};
+  v.R = (double*)(c + L.R);
+  v.heavy = (unsigned long long*)(c + L.heavy);
+  v.hcount = (unsigned int*)(c + L.hcount);
+  v.hcap = L.hcap;
+  v.hncap = L.hncap;
+  v.phi = phi ? phi : (double*)(c + L.phi);
+  MetricPtrs<double> m;
+  for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
+  return fused_rhs<double>(kp, v, Q, m, J, v.phi, R, (hipStream_t)s);
 }
 
 F3D_API int f3d_rk3_stage_f64(const f3d_params* p, f3d_workspace* ws, int32_t KRK, double* U, double* Un,

@@ -381,6 +381,18 @@ int f3d_rhs_f64(const f3d_params* p, f3d_workspace* ws, double* R, const double*
 int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, const float* Q,
                  const float* const* metrics, const double* J, float* phi, f3d_stream_t s);
 
+/* f3d_rhs_f64 without a workspace and without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The scratch (interior RHS accumulator, heavy-face runs, and phi when `phi`
+ * is NULL) comes from one static 64 MiB device array per GPU, held until the process ends: grids whose
+ * scratch exceeds it (about 83 bytes per interior cell: some 0.8M cells, e.g. 92^3) are rejected with
+ * F3D_E_DIMS before any launch.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two streams, two threads)
+ * race.  Finite-difference branch, fused engine with the viscous fluxes inside the update march only
+ * (LES models and the F3D_RHS_ENGINE / F3D_VISC_FUSED switches give F3D_E_ARG). */
+int f3d_rhs_noalloc_f64(const f3d_params* p, double* R, const double* Q, const double* const* metrics,
+                        const double* J, double* phi, f3d_stream_t s);
+
 /* One SSP-RK3 stage KRK in {1,2,3} of the loop body solver.jl:416-444: (KRK==1: Un <- U), sensor,
  * RHS, U += dt*R, (KRK==2: U = U/4 + 3Un/4, KRK==3: U = 2U/3 + Un/3), c2Prim, ghost fill of Q.
  * p->lts: KRK==1 also computes the per-cell dt (compute_dt) from Q into the workspace, and every
