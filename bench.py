@@ -29,6 +29,8 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic (compulsory) HBM bytes per interior cell-update of a fully fused RK3 step:
 # stage 1 reads U + 9 metrics + J and writes U; stages 2-3 also read Un.
 ALG_BYTES_PER_CELL_UPDATE = {"f64": 560.0, "f32b": 452.0}
+# The same per RHS + stage (one third of a step): the unit of the dominant kernel's roofline.
+ALG_BYTES_PER_CELL_STAGE = {k: v / 3.0 for k, v in ALG_BYTES_PER_CELL_UPDATE.items()}
 
 
 def kernel_alg_bytes(name, p, es):
@@ -115,12 +117,14 @@ def build_case(f3d, cases, torch, nx, ny, nz, seed=2, sweep=0.0):
     return dict(c=c, p=p, ctx=ctx, m=m, J=J, Q=Q, U=U, inflow=inflow, dt=float(dt_t.item()))
 
 
-def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3, geo=None):
+def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3, geo=None, capture_after=None):
     """Warm up; run `table_steps` steps with every library launch bracketed by hipEvents (per-kernel
     table); then time `steps` RK3 steps between two synchronisations with events around the
     launches of the dominant kernel only (its live average launch time).  `geo` (face geometry
-    arrays): the finite-volume branch.  Returns (ms/step, host wall ms/step, kernel table,
-    (dominant name, its total ms, its launches))."""
+    arrays): the finite-volume branch.  `capture_after` = n: the timed steps run as two timed parts,
+    n steps and steps - n, with a copy of U taken between them outside the timed parts (returned as
+    a fifth value).  Returns (ms/step, host wall ms/step, kernel table,
+    (dominant name, its total ms, its launches)[, captured U])."""
     U, Un, Q, m, J, inflow, dt = state
     if geo is None:
         step = lambda: ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
@@ -144,22 +148,35 @@ def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3, geo=Non
             lib.f3d_timing_only(dom.encode())
             lib.f3d_timing_reset()
             lib.f3d_timing_enable(1)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record()
-    for _ in range(steps):
-        step()
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    parts = [steps] if capture_after is None else [capture_after, steps - capture_after]
+    ms_tot, wall, captured = 0.0, 0.0, None
+    for q, n in enumerate(parts):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record()
+        for _ in range(n):
+            step()
+        ev1.record()
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+        ms_tot += ev0.elapsed_time(ev1)
+        if q == 0 and capture_after is not None:
+            if lib is not None and dom is not None:
+                lib.f3d_timing_enable(0)
+            captured = U.clone()
+            torch.cuda.synchronize()
+            if lib is not None and dom is not None:
+                lib.f3d_timing_enable(1)
     live = None
     if dom is not None:
         lib.f3d_timing_enable(0)
         live = (dom,) + kernel_table(lib)[dom]
         lib.f3d_timing_only(None)
-    return ev0.elapsed_time(ev1) / steps, wall / steps * 1e3, table, live
+    if capture_after is not None:
+        return ms_tot / steps, wall / steps * 1e3, table, live, captured
+    return ms_tot / steps, wall / steps * 1e3, table, live
 
 
 def progress(msg):
@@ -289,8 +306,10 @@ def shocked_bench(f3d, torch, lib, st, Q0, steps, warmup, amp=0.05):
     torch.cuda.synchronize()
     nf = sum(m.numel() for m in masks)
     nh = sum(int((m != 0).sum()) for m in masks)
+    hist = {name: int(sum(int((m == k).sum()) for m in masks)) for k, name in enumerate(SCHEMES)}
     del phi, masks
-    out = dict(noise_amplitude=amp, faces=nf, faces_off_up7=nh, frac_off_up7=round(nh / nf, 4))
+    out = dict(noise_amplitude=amp, faces=nf, faces_off_up7=nh, frac_off_up7=round(nh / nf, 4),
+               mask_histogram_initial=hist)
     ncells = p.Nxp * p.Nyp * p.Nzp
     for name in ("f64", "f32b"):
         if name == "f64":
@@ -347,9 +366,36 @@ def io_bench(st, Q0):
     return out
 
 
-def cpu_baseline(f3d, cases, nx, ny, nz, steps=3):
-    """CPU oracle (C++ FP64 restatement, OpenMP) on the same grid for `steps` RK3 steps."""
+ORACLE_FLAGS = "-O3 -std=c++17 -ffp-contract=off -fno-fast-math -fopenmp -fPIC -shared"
+
+
+def start_native_oracle_build():
+    """Start compiling the CPU oracle for this host (-march=native) in the background; the CPU baseline
+    waits for it (falls back to the prebuilt -march=x86-64-v3 oracle/liboracle_f3d.so if it fails)."""
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp(prefix="f3d_oracle_native_")
+    so = os.path.join(d, "liboracle_f3d.so")
+    cmd = ["g++", *ORACLE_FLAGS.split(), "-march=native", "-o", so, os.path.join(ROOT, "oracle", "f3d_oracle.cpp")]
+    try:
+        proc = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    except OSError:
+        return None
+    return dict(proc=proc, so=so)
+
+
+def cpu_baseline(f3d, cases, nx, ny, nz, steps=20, native=None):
+    """CPU oracle (C++ FP64 restatement, OpenMP) on the same grid for `steps` RK3 steps, built for this
+    host's cores (-march=native) when that build succeeded."""
     import numpy as np
+    build = "-march=x86-64-v3 (prebuilt oracle/liboracle_f3d.so)"
+    if native is not None:
+        rc = native["proc"].wait()
+        if rc == 0:
+            os.environ["F3D_ORACLE_SO"] = native["so"]
+            build = "-march=native (compiled on this host)"
+        else:
+            build += f"; the native build failed ({rc})"
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle, max_threads
     c = cases.ramp(nx, ny, nz)
@@ -368,20 +414,30 @@ def cpu_baseline(f3d, cases, nx, ny, nz, steps=3):
     t0 = time.perf_counter()
     o.time_step(steps, U, Un, Q, m, J, dt, c["inflow"])
     el = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), cpu)
+    except OSError:
+        pass
     return dict(value=round(nx * ny * nz * steps / el / 1e6, 4), unit="Mcell-updates/s", cores=max_threads(),
-                kind="port", sample=f"oracle FP64, {steps} RK3 steps of the {nx}x{ny}x{nz} ramp ({el:.1f} s)")
+                kind="port", sample=f"oracle FP64, {steps} RK3 steps of the {nx}x{ny}x{nz} ramp ({el:.1f} s)",
+                omp_threads=max_threads(), nproc=os.cpu_count(), affinity_cpus=len(os.sched_getaffinity(0)),
+                cpu_model=cpu, build=build, flags=ORACLE_FLAGS)
 
 
-def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True, every=1, sweep=0.0):
+def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True, every=1, sweep=0.0, case=None, U_start=None,
+                  dt=None, fd_params=(0, 1, 2)):
     """Inflow-parameter adjoint of the ramp: theta = (M_inf, T_wall, 16 Fourier amplitudes),
     objective = sum over `nsteps` FP64 RK3 steps of dt * sum_wall p * |grad eta|/J (wall pressure
-    x wall-face area on the first cell row), checkpoint every RK step.  Times one forward step and
-    the adjoint (forward sweep + reverse with recomputation), chains the boundary-data adjoints to
-    d objective / d theta with complex-step derivatives of the boundary data, and checks three
-    components against GPU central finite differences."""
+    x wall-face area on the first cell row), checkpoint every `every` RK steps.  Times one forward
+    step and the adjoint (forward sweep + reverse with recomputation), chains the boundary-data adjoints
+    to d objective / d theta with complex-step derivatives of the boundary data, and checks the
+    components in `fd_params` against GPU central finite differences (the amplitudes at two steps).
+    `case` (a cases.ramp dict), `U_start` (conserved state to start from) and `dt` reuse a forward leg's."""
     import numpy as np
     NG = 4
-    c = cases.ramp(nx, ny, nz, sweep=sweep)
+    c = case if case is not None else cases.ramp(nx, ny, nz, sweep=sweep)
     st = c["state"]
     ygh = c["y"][:, :, 0:NG].mean(axis=2)
     th = cases.theta0()
@@ -398,17 +454,27 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True, every=1, 
     m = [torch.zeros(p.cell_shape(), dtype=torch.float64, device="cuda:0") for _ in range(9)]
     J = torch.zeros(p.cell_shape(), dtype=torch.float64, device="cuda:0")
     ctx.metrics(d(c["x"]), d(c["y"]), d(c["z"]), m, J)
-    Q0 = d(c["Q"])
-    U0 = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
-    ctx.exchange_ghost(Q0, 6)
-    ctx.fillGhost(Q0, U0, inflow)
-    ctx.prim2c(U0, Q0)
-    dt_t = torch.zeros(1, dtype=torch.float64, device="cuda:0")
-    ctx.cfl_dt(dt_t, Q0, m, J, 0.5)
-    dt = float(dt_t.item())
+    if U_start is not None:
+        U0 = U_start
+        Q0 = torch.zeros(p.cell_shape(6), dtype=torch.float64, device="cuda:0")
+        ctx.c2Prim(U0, Q0)
+        ctx.exchange_ghost(Q0, 6)
+        ctx.fillGhost(Q0, None, inflow)
+    else:
+        Q0 = d(c["Q"])
+        U0 = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
+        ctx.exchange_ghost(Q0, 6)
+        ctx.fillGhost(Q0, U0, inflow)
+        ctx.prim2c(U0, Q0)
+    if dt is None:
+        dt_t = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+        ctx.cfl_dt(dt_t, Q0, m, J, 0.5)
+        dt = float(dt_t.item())
+    hist = mask_histogram(torch, ctx, p, Q0)
     W = torch.zeros(p.cell_shape(6), dtype=torch.float64, device="cuda:0")
     area = torch.sqrt(m[3] ** 2 + m[4] ** 2 + m[5] ** 2)
     W[4, NG:-NG, NG, NG:-NG] = area[NG:-NG, NG, NG:-NG]
+    del area
 
     def objective(params, inflow_t):
         cx = f3d.Flame3D(params)
@@ -422,14 +488,20 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True, every=1, 
         for _ in range(nsteps):
             cx.time_step(1, U, Un, Q, m, J, dt, inflow_t)
             obj += dt * (W * Q).sum()
-        return float(obj.item())
+        val = float(obj.item())
+        del cx, U, Un, Q
+        torch.cuda.empty_cache()
+        return val
 
     # forward step time on this grid
     U, Un, Q = U0.clone(), torch.zeros_like(U0), torch.zeros(p.cell_shape(6), dtype=torch.float64, device="cuda:0")
     ctx.c2Prim(U, Q)
     ctx.exchange_ghost(Q, 6)
     ctx.fillGhost(Q, None, inflow)
-    ms_fwd, _, _, _ = run_steps(torch, ctx, (U, Un, Q, m, J, inflow, dt), nsteps, 2)
+    ms_fwd, _, _, _ = run_steps(torch, ctx, (U, Un, Q, m, J, inflow, dt), min(nsteps, 50), 2)
+    del U, Un, Q
+    ctx._ws.clear()  # the adjoint workspace holds its own forward workspace
+    torch.cuda.empty_cache()
     # adjoint (one short call first: workspace creation)
     obj = torch.zeros(nsteps, dtype=torch.float64, device="cuda:0")
     lamU0 = torch.zeros_like(U0)
@@ -446,15 +518,23 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True, every=1, 
     ev1.record()
     torch.cuda.synchronize()
     ms_adj = ev0.elapsed_time(ev1)
-    # per-kernel time of the reverse sweep (separate run, hipEvent-bracketed launches)
+    # per-kernel time of the reverse sweep (separate run, hipEvent-bracketed launches); the same
+    # gradient again (deterministic kernels give the same bits)
     lib.f3d_timing_reset()
     lib.f3d_timing_enable(1)
     obj2 = torch.zeros_like(obj)
-    ctx.adjoint(nsteps, every, U0, m, J, dt, W, obj2, torch.zeros_like(U0), inflow, torch.zeros_like(inflow),
-                torch.zeros_like(lam_bc), ckpt=ckpt)
+    lamU0b, lam_in2, lam_bc2 = torch.zeros_like(U0), torch.zeros_like(inflow), torch.zeros_like(lam_bc)
+    ctx.adjoint(nsteps, every, U0, m, J, dt, W, obj2, lamU0b, inflow, lam_in2, lam_bc2, ckpt=ckpt)
     torch.cuda.synchronize()
     lib.f3d_timing_enable(0)
     ktab = {k: v for k, v in kernel_table(lib).items() if k.startswith("adj_")}
+    repeat = dict(objective_bits_equal=bool(torch.equal(obj, obj2)), lamU0_bits_equal=bool(torch.equal(lamU0, lamU0b)),
+                  lam_inflow_bits_equal=bool(torch.equal(lam_in, lam_in2)),
+                  lam_bc_bits_equal=bool(torch.equal(lam_bc, lam_bc2)))
+    ckpt_bytes = int(ckpt.numel() * 8)
+    del ckpt, lamU0b, lam_in2, lam_bc2
+    ctx._aws = None
+    torch.cuda.empty_cache()
     # chain to theta
     jac = cases.ramp_theta_jacobian(ygh, st, th)
     li, lb = lam_in.cpu().numpy(), lam_bc.cpu().numpy()
@@ -462,32 +542,148 @@ def adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, nsteps, fd=True, every=1, 
     J0 = float(obj.sum().item())
     fd_out = []
     if fd:
-        # repeatability of the objective itself (heavy-face corrections add with atomics)
-        j_a, j_b = objective(*setup(th)), objective(*setup(th))
-        for k, name in ((0, "M_inf"), (1, "T_wall"), (2, "a_1")):
-            h = 1e-4 * abs(th[k]) if k < 2 else 1e-2  # amplitudes enter almost linearly
-            tp, tm = th.copy(), th.copy()
-            tp[k] += h
-            tm[k] -= h
-            jp = objective(*setup(tp))
-            jm = objective(*setup(tm))
-            fdv = (jp - jm) / (2 * h)
-            fd_out.append(dict(param=name, adjoint=grad[k], central_fd=fdv, step=h,
-                               rel_diff=abs(fdv - grad[k]) / max(abs(fdv), 1e-300),
-                               fd_noise_rel=abs(j_a - j_b) / max(abs(jp - jm), 1e-300)))
+        j_a, j_b = objective(*setup(th)), objective(*setup(th))  # repeatability of the objective itself
+        names = ["M_inf", "T_wall"] + [f"a_{q}" for q in range(1, len(th) - 1)]
+        for k in fd_params:
+            steps_h = [1e-4 * abs(th[k])] if k < 2 else [1e-2, 4e-2]  # the amplitudes enter almost linearly
+            for h in steps_h:
+                tp, tm = th.copy(), th.copy()
+                tp[k] += h
+                tm[k] -= h
+                jp = objective(*setup(tp))
+                jm = objective(*setup(tm))
+                fdv = (jp - jm) / (2 * h)
+                fd_out.append(dict(param=names[k], adjoint=grad[k], central_fd=fdv, step=h,
+                                   rel_diff=abs(fdv - grad[k]) / max(abs(fdv), 1e-300),
+                                   objective_change_rel=abs(jp - jm) / abs(J0),
+                                   fd_noise_rel=abs(j_a - j_b) / max(abs(jp - jm), 1e-300)))
+        progress("adjoint FD: " + ", ".join(f"{f['param']} (h {f['step']:.1e}) {f['rel_diff']:.1e}" for f in fd_out))
     rev = (ms_adj - nsteps * ms_fwd) / nsteps
     return dict(
         config=f"ramp {nx}x{ny}x{nz}{f' swept {sweep:g} deg' if sweep else ''} FP64, {nsteps} RK3 steps, checkpoint "
                f"every {every} step(s) (states inside a segment recomputed); theta = (M_inf, T_wall, 16 inflow "
                f"Fourier amplitudes); objective = sum_n dt * wall pressure x wall-face area",
-        checkpoint_bytes=int(ckpt.numel() * 8),
+        mask_histogram_initial=hist,
+        checkpoint_bytes=ckpt_bytes,
         ms_forward_step=round(ms_fwd, 4), ms_adjoint_total=round(ms_adj, 3),
         ms_reverse_step=round(rev, 4), adjoint_over_forward_step=round(rev / ms_fwd, 3),
-        objective=J0, objective_check_rel=abs(J0 - float(obj2.sum().item())) / abs(J0),
+        adjoint_total_over_forward_step=round(ms_adj / nsteps / ms_fwd, 3),
+        ratio_note="adjoint_over_forward_step: reverse sweep (with recomputation) per step / forward step; "
+                   "adjoint_total_over_forward_step: the whole call (forward sweep + reverse) per step / forward step",
+        objective=J0, objective_check_rel=abs(J0 - float(obj2.sum().item())) / abs(J0), repeat=repeat,
         grad_theta=[float(g) for g in grad], fd_check=fd_out,
         kernels={k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1], total_ms=round(v[0], 3))
                  for k, v in sorted(ktab.items())},
     )
+
+
+SCHEMES = ("UP7", "WENO7", "WENO5", "NND2")
+
+
+def mask_histogram(torch, ctx, p, Q):
+    """Faces per scheme of the face mask (schemes.jl:724-731, 813, 931) at the state Q, the three
+    directions together, and the share of faces off UP7."""
+    phi = torch.zeros(p.cell_shape(), dtype=Q.dtype, device="cuda:0")
+    ctx.shockSensor(phi, Q)
+    masks = [torch.zeros(p.mask_shape(d), dtype=torch.uint8, device="cuda:0") for d in range(3)]
+    ctx.schemeMask(*masks, phi)
+    cnt = [int(sum(int((m == k).sum()) for m in masks)) for k in range(4)]
+    del phi, masks
+    out = dict(zip(SCHEMES, cnt))
+    out["frac_off_up7"] = round(1.0 - cnt[0] / max(sum(cnt), 1), 6)
+    return out
+
+
+def forward_pair(f3d, cases, torch, lib, nx, ny, nz, steps, warmup, sweep=0.0, tables=True, capture_after=None,
+                 f32b=True):
+    """The ramp case on an nx x ny x nz grid: FP64 and FP32-bulk RK3 steps from the same initial state and
+    dt (warmup + 3 table steps + `steps` timed), with the mask histogram of the initial state and the
+    density and wall-pressure errors of FP32-bulk against FP64 after the same steps.  Returns
+    (result dict, case dict st, U0 and Q0 of the initial state, extras)."""
+    NG = 4
+    st = build_case(f3d, cases, torch, nx, ny, nz, sweep=sweep)
+    p, ctx, dt = st["p"], st["ctx"], st["dt"]
+    ncells = nx * ny * nz
+    U0, Q0 = st["U"].clone(), st["Q"].clone()
+    hist = mask_histogram(torch, ctx, p, Q0)
+    tab = lambda kt, ms: {k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1], share=round(v[0] / (ms * 3), 4))
+                          for k, v in sorted(kt.items())}
+    U, Un, Q = st["U"], torch.zeros_like(st["U"]), st["Q"]
+    ms64, wall64, ktab, live = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], dt), steps, warmup,
+                                         lib=lib if tables else None)
+    ctx.nan_check(U)
+    v64 = ncells / (ms64 * 1e-3) / 1e6
+    progress(f"{nx}x{ny}x{nz}{' swept' if sweep else ''} FP64 {steps} steps: {ms64:.3f} ms/step")
+    out = dict(value=round(v64, 3), unit="Mcell-updates/s", ms_per_step=round(ms64, 4), steps=steps,
+               hbm_frac_path=round(ALG_BYTES_PER_CELL_UPDATE["f64"] * v64 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+               mask_histogram_initial=hist)
+    if tables:
+        out["kernels"] = tab(ktab, ms64)
+    if not f32b:
+        return out, st, U0, Q0, dict(live=live, wall64=wall64, captured=None)
+    m32 = [a.float().contiguous() for a in st["m"]]
+    U32, Un32, Q32 = U0.clone(), torch.zeros_like(U0), Q0.float().contiguous()
+    in32 = st["inflow"].float().contiguous()
+    r = run_steps(torch, ctx, (U32, Un32, Q32, m32, st["J"], in32, dt), steps, warmup, lib=lib if tables else None,
+                  capture_after=capture_after)
+    ms32, ktab32 = r[0], r[2]
+    ctx.nan_check(U32)
+    v32 = ncells / (ms32 * 1e-3) / 1e6
+    progress(f"{nx}x{ny}x{nz}{' swept' if sweep else ''} FP32-bulk {steps} steps: {ms32:.3f} ms/step")
+    inner = (slice(NG, -NG),) * 3
+    r64, r32 = U[0][inner].double(), U32[0][inner].double()
+    pw64 = Q[4][NG:-NG, NG, NG:-NG].mean(dim=0)  # p at the first cell above the wall, spanwise mean
+    pw32 = Q32[4][NG:-NG, NG, NG:-NG].double().mean(dim=0)
+    f32 = dict(value=round(v32, 3), ms_per_step=round(ms32, 4), speedup_vs_f64=round(v32 / v64, 3),
+               hbm_frac_path=round(ALG_BYTES_PER_CELL_UPDATE["f32b"] * v32 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+               rho_L2_rel=float(torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)),
+               rho_Linf_rel=float((r32 - r64).abs().max() / r64.abs().max()),
+               pwall_L2_rel=float(torch.linalg.norm(pw32 - pw64) / torch.linalg.norm(pw64)),
+               pwall_Linf_rel=float((pw32 - pw64).abs().max() / pw64.abs().max()),
+               steps_compared=steps + warmup + (3 if tables else 0))
+    if tables:
+        f32["kernels"] = tab(ktab32, ms32)
+    out["fp32_bulk"] = f32
+    del m32, Un32, Q32, U32
+    extras = dict(live=live, wall64=wall64, captured=r[4] if capture_after is not None else None)
+    return out, st, U0, Q0, extras
+
+
+def large_config_leg(f3d, cases, torch, lib, grid, steps, warmup, sweep=0.0):
+    """A large configuration of the ramp (FP64 and FP32-bulk, `steps` timed steps each, errors of FP32-bulk
+    against FP64), without per-kernel tables."""
+    nx, ny, nz = grid
+    out, st, U0, Q0, _ = forward_pair(f3d, cases, torch, lib, nx, ny, nz, steps, warmup, sweep=sweep, tables=False)
+    out["workload"] = f"compression ramp 24 deg{f', swept {sweep:g} deg' if sweep else ''}, {nx}x{ny}x{nz}"
+    del st, U0, Q0
+    torch.cuda.empty_cache()
+    return out
+
+
+def swept_leg(f3d, cases, torch, lib, grid, steps, warmup, adj_steps, every, fd=True):
+    """The swept-ramp one-GPU subdomain: FP64 and FP32-bulk forward over `steps` timed steps (after warmup
+    and table steps), then the FP64 adjoint over the last `adj_steps` of the FP32-bulk trajectory
+    (from its state `adj_steps` steps before the end), checkpoint every `every` steps with segment
+    recomputation, with central finite-difference checks of d objective / d (M_inf, T_wall)."""
+    nx, ny, nz = grid
+    out, st, U0, Q0, ex = forward_pair(f3d, cases, torch, lib, nx, ny, nz, steps, warmup, sweep=30.0, tables=False,
+                                       capture_after=steps - adj_steps)
+    out["workload"] = f"compression ramp 24 deg, swept 30 deg, {nx}x{ny}x{nz}"
+    Ustart = ex["captured"]
+    dt = st["dt"]
+    case = st["c"]
+    for k in ("m", "J", "Q", "U", "inflow", "ctx"):
+        st.pop(k, None)
+    del U0, Q0
+    torch.cuda.empty_cache()
+    adj = adjoint_bench(f3d, cases, torch, lib, nx, ny, nz, adj_steps, fd=fd, every=every, sweep=30.0, case=case,
+                        U_start=Ustart, dt=dt, fd_params=(0, 1))
+    adj["start"] = (f"FP32-bulk trajectory after {steps - adj_steps} of its {steps} timed steps "
+                    f"({warmup} warmup + 3 table steps before them): the adjoint covers the last {adj_steps}")
+    out["adjoint"] = adj
+    del Ustart
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -498,7 +694,7 @@ def main():
     ap.add_argument("--grid", default="512x192x64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32b", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-grid", default="256x128x64")
     ap.add_argument("--adjoint-steps", type=int, default=50)
@@ -513,23 +709,40 @@ def main():
     ap.add_argument("--no-io", action="store_true", help="skip the on-disk formats leg")
     ap.add_argument("--no-shocked", action="store_true",
                     help="skip the leg on a noisy state (faces off UP7 take the WENO/NND path)")
+    ap.add_argument("--shocked-only", action="store_true", help="only the noisy-state leg (profiling)")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1024x384x256 ramp leg")
+    ap.add_argument("--large-grid", default="1024x384x256")
+    ap.add_argument("--large-steps", type=int, default=200)
+    ap.add_argument("--no-swept", action="store_true", help="skip the swept 1024x256x256 leg")
+    ap.add_argument("--swept-grid", default="1024x256x256")
+    ap.add_argument("--swept-steps", type=int, default=500)
+    ap.add_argument("--swept-adjoint-steps", type=int, default=100)
+    ap.add_argument("--swept-adjoint-every", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only run a 1 GiB device-to-device copy 5 times (calibrates HBM byte counters)")
+                    help="only run a 1 GiB device-to-device copy 5 times (calibrates the HBM write counter)")
+    ap.add_argument("--read-probe", action="store_true",
+                    help="only run a 1 GiB FP64 sum 5 times (calibrates the HBM read counter: coalesced 8-byte "
+                         "loads of 64 lanes, as the marches read)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the ramp's whole subdomain on one MI355X); "
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
+    native = None if (args.no_cpu or args.copy_probe or args.read_probe or args.adjoint_only or args.fvm_only
+                      or args.shocked_only) else start_native_oracle_build()
     import numpy as np
     import torch
     assert torch.cuda.is_available(), "bench needs a GPU"
-    if args.copy_probe:
+    if args.copy_probe or args.read_probe:
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")  # 1 GiB
         b = torch.empty_like(a)
         for _ in range(5):
-            b.copy_(a)
+            if args.copy_probe:
+                b.copy_(a)
+            else:
+                b[:1] = a.sum()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copies": 5}))
+        print(json.dumps({"probe": "copy" if args.copy_probe else "read", "bytes_read": a.numel() * 8, "runs": 5}))
         return
     f3d = importlib.import_module("flame3d-mhd_amd")
     cases = importlib.import_module("flame3d-mhd_amd.cases")
@@ -542,69 +755,43 @@ def main():
                                                    fd=not args.no_fd, every=args.adjoint_every,
                                                    sweep=args.adjoint_sweep)}))
         return
-
-    st = build_case(f3d, cases, torch, nx, ny, nz, sweep=args.sweep)
-    progress(f"case {nx}x{ny}x{nz} built")
-    p, ctx, dt = st["p"], st["ctx"], st["dt"]
-    U0 = st["U"].clone()
-    Q0 = st["Q"].clone()
-    if args.fvm_only:
-        print(json.dumps({"finite_volume": fvm_bench(f3d, torch, lib, st, U0, Q0, args.steps, args.warmup,
-                                                     f32b=not args.no_f32b)}))
+    if args.fvm_only or args.shocked_only:
+        st = build_case(f3d, cases, torch, nx, ny, nz, sweep=args.sweep)
+        U0, Q0 = st["U"].clone(), st["Q"].clone()
+        if args.fvm_only:
+            print(json.dumps({"finite_volume": fvm_bench(f3d, torch, lib, st, U0, Q0, args.steps, args.warmup,
+                                                         f32b=not args.no_f32b)}))
+        else:
+            print(json.dumps({"shocked_state": shocked_bench(f3d, torch, lib, st, Q0, max(3, args.steps // 4), 2)}))
         return
-    # ---- FP64
-    U, Un, Q = st["U"], torch.zeros_like(st["U"]), st["Q"]
-    ms64, wall64, ktab, live = run_steps(torch, ctx, (U, Un, Q, st["m"], st["J"], st["inflow"], dt), args.steps,
-                                         args.warmup, lib=lib)
-    ctx.nan_check(U)
-    value64 = ncells / (ms64 * 1e-3) / 1e6
-    progress(f"FP64 {args.steps} steps: {ms64:.3f} ms/step")
-    rho64 = U[0].clone()
-    # ---- FP32-bulk from the same initial state, same dt and step count
-    out32 = {}
-    if not args.no_f32b:
-        m32 = [a.float().contiguous() for a in st["m"]]
-        U32, Un32, Q32 = U0.clone(), torch.zeros_like(U0), Q0.float().contiguous()
-        in32 = st["inflow"].float().contiguous()
-        ms32, _, ktab32, _ = run_steps(torch, ctx, (U32, Un32, Q32, m32, st["J"], in32, dt), args.steps, args.warmup,
-                                       lib=lib)
-        ctx.nan_check(U32)
-        value32 = ncells / (ms32 * 1e-3) / 1e6
-        progress(f"FP32-bulk {args.steps} steps: {ms32:.3f} ms/step")
-        NG = 4
-        inner = (slice(NG, -NG), slice(NG, -NG), slice(NG, -NG))
-        r64, r32 = U[0][inner].double(), U32[0][inner].double()
-        pw64 = Q[4][NG:-NG, NG, NG:-NG].mean(dim=0)  # p at the first cell above the wall, spanwise mean
-        pw32 = Q32[4][NG:-NG, NG, NG:-NG].double().mean(dim=0)
-        out32 = dict(
-            value=round(value32, 3), ms_per_step=round(ms32, 4),
-            speedup_vs_f64=round(value32 / value64, 3),
-            hbm_frac_path=round(ALG_BYTES_PER_CELL_UPDATE["f32b"] * value32 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
-            rho_L2_rel=float(torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)),
-            rho_Linf_rel=float((r32 - r64).abs().max() / r64.abs().max()),
-            pwall_L2_rel=float(torch.linalg.norm(pw32 - pw64) / torch.linalg.norm(pw64)),
-            pwall_Linf_rel=float((pw32 - pw64).abs().max() / pw64.abs().max()),
-            steps_compared=args.steps + args.warmup + 3,
-            kernels={k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1],
-                             share=round(v[0] / (ms32 * 3), 4)) for k, v in sorted(ktab32.items())},
-        )
+    # ---- headline: FP64 (and FP32-bulk) on the ramp grid
+    main_leg, st, U0, Q0, ex = forward_pair(f3d, cases, torch, lib, nx, ny, nz, args.steps, args.warmup,
+                                            sweep=args.sweep, f32b=not args.no_f32b)
+    p, dt = st["p"], st["dt"]
+    value64, ms64 = main_leg["value"], main_leg["ms_per_step"]
+    live = ex["live"]
     # ---- roofline of the dominant kernel (hipEvent-timed launches inside the FP64 timed region)
     roof = None
     if live:
         name, tot_ms, n = live
         avg_s = tot_ms / n * 1e-3
-        alg = kernel_alg_bytes(name, p, 8)
+        alg = ALG_BYTES_PER_CELL_STAGE["f64"] * ncells  # algorithmic bytes of one cell-stage x interior cells
+        kbytes = kernel_alg_bytes(name, p, 8)
         achieved = alg / avg_s / 1e9
         traffic = MEASURED_TRAFFIC.get((name, args.grid, "f64"))
         roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
                     avg_launch_us=round(avg_s * 1e6, 2), alg_bytes_per_launch=alg,
+                    alg_bytes_basis="186.7 B per interior cell per RHS + stage (FP64), x interior cells",
+                    kernel_bytes_per_launch=kbytes,
+                    achieved_kernel_bytes=round(kbytes / avg_s / 1e9, 2),
+                    frac_kernel_bytes=round(kbytes / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+                    traffic_over_alg=round(traffic / alg, 3) if traffic else None,
                     share_of_step=round(tot_ms / (ms64 * args.steps), 4))
-    kshare = {k: dict(avg_us=round(v[0] / v[1] * 1e3, 2), launches=v[1],
-                      share=round(v[0] / (ms64 * 3), 4)) for k, v in sorted(ktab.items())}
     fvm = None
     if not args.no_fvm:
         fvm = fvm_bench(f3d, torch, lib, st, U0, Q0, args.steps, args.warmup, f32b=not args.no_f32b)
+        fvm["mask_histogram_initial"] = main_leg["mask_histogram_initial"]
     alt = None
     if not args.no_alternatives:
         alt = alternatives_bench(f3d, torch, lib, st, U0, Q0, max(3, args.steps // 4), 2)
@@ -617,16 +804,28 @@ def main():
             io = io_bench(st, Q0)
         except ImportError as e:  # libf3d_h5.so not built (no HDF5 headers where it was built)
             io = {"skipped": str(e)}
+    del st, U0, Q0
+    torch.cuda.empty_cache()
     adj = None
     if not args.no_adjoint:
         ax, ay, az = map(int, args.adjoint_grid.split("x"))
         adj = adjoint_bench(f3d, cases, torch, lib, ax, ay, az, args.adjoint_steps, fd=not args.no_fd,
                             every=args.adjoint_every, sweep=args.adjoint_sweep)
-    if adj:
-        progress(f"adjoint: {adj['ms_reverse_step']:.3f} ms per reverse step")
+        progress(f"adjoint: {adj['ms_reverse_step']:.3f} ms per reverse step, "
+                 f"{adj['adjoint_total_over_forward_step']:.3f}x a forward step with the forward sweep")
+    large = None
+    if not args.no_large:
+        large = large_config_leg(f3d, cases, torch, lib, tuple(map(int, args.large_grid.split("x"))),
+                                 args.large_steps, args.warmup)
+    swept = None
+    if not args.no_swept:
+        swept = swept_leg(f3d, cases, torch, lib, tuple(map(int, args.swept_grid.split("x"))), args.swept_steps,
+                          args.warmup, args.swept_adjoint_steps, args.swept_adjoint_every, fd=not args.no_fd)
+        progress(f"swept adjoint: {swept['adjoint']['adjoint_total_over_forward_step']:.3f}x a forward step with "
+                 f"the forward sweep")
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(f3d, cases, nx, ny, nz, steps=args.cpu_steps)
+        cpu = cpu_baseline(f3d, cases, nx, ny, nz, steps=args.cpu_steps, native=native)
     line = {
         "metric": "Mcell-updates/s (FP64 RHS + SSP-RK3 step)",
         "value": round(value64, 3),
@@ -643,17 +842,20 @@ def main():
         "config": {"workload": f"compression ramp 24 deg{f', swept {args.sweep:g} deg' if args.sweep else ''}, "
                                f"{nx}x{ny}x{nz}, FP64, M2.9, Re_delta 6e4, isothermal wall, periodic span",
                    "cells": ncells, "dt": dt},
-        "hbm_frac_path": round(ALG_BYTES_PER_CELL_UPDATE["f64"] * value64 * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+        "hbm_frac_path": main_leg["hbm_frac_path"],
+        "mask_histogram_initial": main_leg["mask_histogram_initial"],
         "roofline": roof,
         "cpu_baseline": cpu,
-        "fp32_bulk": out32 or None,
+        "fp32_bulk": main_leg.get("fp32_bulk"),
+        "large_1024x384x256": large,
+        "swept_1024x256x256": swept,
         "finite_volume": fvm,
         "alternatives": alt,
         "shocked_state": shocked,
         "io": io,
         "adjoint": adj,
-        "kernels": kshare,
-        "host_wall_ms_per_step": round(wall64, 4),
+        "kernels": main_leg.get("kernels"),
+        "host_wall_ms_per_step": round(ex["wall64"], 4),
     }
     print(json.dumps(line))
 

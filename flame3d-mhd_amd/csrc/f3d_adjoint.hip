@@ -305,6 +305,42 @@ __global__ __launch_bounds__(256) void k_adj_visc2(KP kp, const double* __restri
 }
 
 // ------------------------------------------------------------------ ghost procedure, reverse
+// Cell of the lamQ ghost slab of face (D, side) that holds partial n (0..6) of the block whose first line
+// is (a0, b0): variables 0..5 of the line's first ghost cell, then variable 0 of its second.
+template <int D>
+__device__ __forceinline__ long long bc_slot(const KP& kp, int side, int a0, int b0, int n) {
+  const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+  const int g = (side ? Np + kp.NG : 0) + (n < 6 ? 0 : 1);
+  const long long o = D == 0 ? cidx(kp, g, a0, b0) : (D == 1 ? cidx(kp, a0, g, b0) : cidx(kp, a0, b0, g));
+  return o + (n < 6 ? (long long)n * kp.nc : 0);
+}
+
+// lam_bc[n] += the partials of the nbx x nby blocks of one reverse fill, summed in a fixed order (a
+// strided sum per thread, then a fixed tree): the boundary-data gradient is the same bit for bit on every
+// run.  The partial slots are cleared again (every ghost adjoint is zero after a reverse fill).
+template <int D>
+__global__ __launch_bounds__(256) void k_adj_bc_final(KP kp, int side, int lw, int nbx, int nby,
+                                                      double* __restrict__ lamQ, double* __restrict__ lam_bc) {
+  __shared__ double sm[256];
+  const int nb = nbx * nby;
+  for (int n = 0; n < 7; ++n) {
+    double v = 0.0;
+    for (int q = threadIdx.x; q < nb; q += 256) {
+      const long long o = bc_slot<D>(kp, side, (q % nbx) * lw, (q / nbx) * 4, n);
+      v += lamQ[o];
+      lamQ[o] = 0.0;
+    }
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+      if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) lam_bc[n] += sm[0];
+    __syncthreads();
+  }
+}
+
 // Reverse of one side of one dimension's ghost fill (boundary.jl:1-156 with the kinds of f3d.h):
 // each lane owns one line of NG ghost cells, folds their adjoints into the source cells and
 // clears them.  lam_bc[0..5] += adjoint of q_inf, lam_bc[6] += adjoint of T_wall.
@@ -379,9 +415,11 @@ __global__ __launch_bounds__(256) void k_adj_ghost(KP kp, int side, int bc, doub
       if ((tid & 63) == 0) sm[n][tid >> 6] = v;
     }
     __syncthreads();
+    // the block's partial sums go into its own first ghost line (cleared above, read by no other block);
+    // k_adj_bc_final adds them up in block order and clears them again
     if (tid < 7 && lam_bc) {
       const double v = sm[tid][0] + sm[tid][1] + sm[tid][2] + sm[tid][3];
-      if (v != 0.0) atomicAdd(&lam_bc[tid], v);
+      lamQ[bc_slot<D>(kp, side, blockIdx.x * 64, blockIdx.y * 4, tid)] = v;
     }
   }
 }
@@ -466,9 +504,11 @@ __global__ __launch_bounds__(256) void k_adj_ghost_x(KP kp, int side, int bc, do
       if ((tid & 63) == 0) sm[n][tid >> 6] = v;
     }
     __syncthreads();
+    // the block's partial sums go into its own first ghost line (cleared above, read by no other block);
+    // k_adj_bc_final adds them up in block order and clears them again
     if (tid < 7 && lam_bc) {
       const double v = sm[tid][0] + sm[tid][1] + sm[tid][2] + sm[tid][3];
-      if (v != 0.0) atomicAdd(&lam_bc[tid], v);
+      lamQ[bc_slot<0>(kp, side, blockIdx.x * 16, blockIdx.y * 4, tid)] = v;
     }
   }
 }
@@ -720,15 +760,23 @@ int ghost_vjp_impl(const f3d_params* p, double* lamQ, const double* Q, double* l
   const KP kp = make_kp(p);
   TimedScope ts("adj_ghost", s);
   auto launch = [&](int d, int side, int bc) {
-    if (d == 0)
-      hipLaunchKernelGGL(k_adj_ghost_x, dim3((kp.Nyt + 15) / 16, (kp.Nzt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
-                         lam_inflow, lam_bc);
-    else if (d == 1)
-      hipLaunchKernelGGL(k_adj_ghost<1>, dim3((kp.Nxt + 63) / 64, (kp.Nzt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
-                         lam_inflow, lam_bc);
-    else
-      hipLaunchKernelGGL(k_adj_ghost<2>, dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4), kB, 0, s, kp, side, bc, lamQ, Q,
-                         lam_inflow, lam_bc);
+    const bool red = lam_bc && (bc == F3D_BC_DIRICHLET || bc == F3D_BC_WALL_ISOTHERMAL);
+    if (d == 0) {
+      const dim3 g((kp.Nyt + 15) / 16, (kp.Nzt + 3) / 4);
+      hipLaunchKernelGGL(k_adj_ghost_x, g, kB, 0, s, kp, side, bc, lamQ, Q, lam_inflow, lam_bc);
+      if (red) hipLaunchKernelGGL(k_adj_bc_final<0>, dim3(1), dim3(256), 0, s, kp, side, 16, (int)g.x, (int)g.y, lamQ,
+                                  lam_bc);
+    } else if (d == 1) {
+      const dim3 g((kp.Nxt + 63) / 64, (kp.Nzt + 3) / 4);
+      hipLaunchKernelGGL(k_adj_ghost<1>, g, kB, 0, s, kp, side, bc, lamQ, Q, lam_inflow, lam_bc);
+      if (red) hipLaunchKernelGGL(k_adj_bc_final<1>, dim3(1), dim3(256), 0, s, kp, side, 64, (int)g.x, (int)g.y, lamQ,
+                                  lam_bc);
+    } else {
+      const dim3 g((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4);
+      hipLaunchKernelGGL(k_adj_ghost<2>, g, kB, 0, s, kp, side, bc, lamQ, Q, lam_inflow, lam_bc);
+      if (red) hipLaunchKernelGGL(k_adj_bc_final<2>, dim3(1), dim3(256), 0, s, kp, side, 64, (int)g.x, (int)g.y, lamQ,
+                                  lam_bc);
+    }
   };
   // reverse of fill_z, fill_y, fill_x.  The reverse of exchange_ghost (mpi.jl, one rank) is not launched:
   // each reverse fill clears its whole slab, so every ghost adjoint is zero by then and the periodic
