@@ -279,10 +279,15 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
 }
 
 // ------------------------------------------------------------------ heavy faces
+// Every listed face (all three directions, one launch): its characteristic WENO7/WENO5/NND flux
+// (schemes.jl:756-1009) minus the UP7 flux the sweep accumulated, kept in the face's list slot; the x faces
+// also add it to the cell right of them here (distinct x faces, distinct cells, and no other RHS write in
+// this launch).  The other five corrections follow in ordered passes (k_heavy_apply), so no floating-point
+// atomics: the corrections are summed in the same order on every evaluation.
 template <typename T, int SPLIT, bool CHAR>
-__global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl) {
+__global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux) {
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp, nc = kp.nc;
-  for_each_heavy(hl, a.hcount, a.heavy, [&](const unsigned long long v) {
+  for_each_heavy_slot(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);
     const long long st = dstride(kp, D);
@@ -296,7 +301,7 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl) {
       const T A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
       split_cell<T, SPLIT>(kp, a.Q[oc], a.Q[oc + nc], a.Q[oc + 2 * nc], a.Q[oc + 3 * nc],
                            a.Q[oc + split_qvar<SPLIT>() * nc], A1, A2, A3,
-                  sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp[s], fm[s]);
+                           sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp[s], fm[s]);
     }
     auto FP = [&](int s, int q) { return fp[s + 4][q]; };
     auto FM = [&](int s, int q) { return fm[s + 4][q]; };
@@ -318,18 +323,15 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl) {
       for (int s = 0; s < 7; ++s) acc += up7c<T>(kp, s) * fm[7 - s][q];
       Fu[q] = acc;
     }
-    // decode the right cell's interior index along D
+    // the x face is the left face of cell I (its right cell)
     const int ci = (int)(o % kp.Nxt), cj = (int)((o / kp.Nxt) % kp.Nyt), ck = (int)(o / ((long long)kp.Nxt * kp.Nyt));
-    const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
-    const int f = D == 0 ? I : (D == 1 ? Jj : K);
-    const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
-    const long long rr = ridx(kp, I, Jj, K);
-    const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
+    const int I = ci - kp.NG;
+    const long long rr = ridx(kp, I, cj - kp.NG, ck - kp.NG);
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       const T dF = Fs[q] - Fu[q];
-      if (f <= Np - 1) atomicAdd(&a.R[rr + q * rn], dF);   // face is the left face of cell f
-      if (f >= 1) atomicAdd(&a.R[rl + q * rn], -dF);       // and the right face of cell f-1
+      hflux[slot * 5 + q] = dF;
+      if (D == 0 && I <= kp.Nxp - 1) a.R[rr + q * rn] += dF;
     }
   });
 }
@@ -1063,9 +1065,10 @@ static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const
     TimedScope ts("fused_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
     if (kp.character)
-      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl);
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux);
     else
-      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl);
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux);
+    launch_heavy_apply<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, s);
   }
   return check_launch("fused sweeps");
 }

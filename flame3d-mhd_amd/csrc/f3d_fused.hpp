@@ -17,6 +17,7 @@ struct WsView {
   unsigned long long hcap;   // capacity of `heavy` (slots)
   unsigned long long hncap;  // capacity of `hcount` (runs)
   double* ldt;               // per-cell dt of local time stepping, cell-shaped (written at stage 1)
+  T* hflux;                  // per list slot: the 5 flux values a heavy face adds to its cells (hcap x 5)
 };
 
 // Block coordinates with an XCD-aware order.  Workgroups are dispatched round-robin over the 8 XCDs
@@ -90,53 +91,117 @@ __device__ __forceinline__ void run_append(unsigned long long* list, unsigned lo
   cnt += (unsigned int)__popcll(mask);
 }
 
-// Calls f(entry) for every listed face, one 64-lane block per work item: a run of more than 64 slots is an
-// item of its own (its entries 64 at a time), runs of at most 64 slots go 64 runs to an item (their counts
-// read in one coalesced load, their entries packed onto the lanes).  Items are spread over the grid.
-// Blocks must be one wave of 64 lanes.
+// Work items of the heavy-face walkers (a block per item): a run of more than 64 slots is an item of its
+// own (its entries 64 at a time), runs of at most 64 slots go 64 runs to an item (their counts read in one
+// coalesced load, their entries packed onto the lanes).
+__host__ __device__ inline unsigned int dir_items(const HSeg& hs) { return hs.cap > 64 ? hs.nw : (hs.nw + 63) / 64; }
+
+// Calls f(entry, slot) for the listed faces of item `loc` of one direction (slot: the entry's index in the
+// list, where a later pass finds what an earlier one kept for the same face).  Blocks of one 64-lane wave.
+template <typename F>
+__device__ __forceinline__ void heavy_item(const HSeg& hs, unsigned int loc, const unsigned int* __restrict__ counts,
+                                           const unsigned long long* __restrict__ list, F&& f) {
+  const int lane = threadIdx.x;
+  if (hs.cap > 64) {
+    const unsigned int n = counts[hs.first + loc];
+    const unsigned long long run0 = hs.base + (unsigned long long)loc * hs.cap;
+    for (unsigned int e = lane; e < n; e += 64) f(list[run0 + e], run0 + e);
+  } else {
+    // the 64 runs' entries packed onto the lanes: exclusive prefix of their counts, then entry e goes
+    // to the last run whose prefix is <= e (empty runs share their successor's prefix)
+    __shared__ unsigned int pre[64];
+    const unsigned int w0 = loc * 64, mine = w0 + lane;
+    const unsigned int myn = mine < hs.nw ? counts[hs.first + mine] : 0u;
+    unsigned int incl = myn;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned int t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const unsigned int tot = __shfl(incl, 63, 64);
+    pre[lane] = incl - myn;
+    __syncthreads();  // (a one-wave block)
+    for (unsigned int e = lane; e < tot; e += 64) {
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= e) lo = mid; else hi = mid - 1;
+      }
+      const unsigned long long slot = hs.base + (unsigned long long)(w0 + lo) * hs.cap + (e - pre[lo]);
+      f(list[slot], slot);
+    }
+    __syncthreads();  // pre is rewritten by the next item
+  }
+}
+
+// f(entry, slot) for every listed face of all three directions, items spread over the grid.
+template <typename F>
+__device__ __forceinline__ void for_each_heavy_slot(const HList& hl, const unsigned int* __restrict__ counts,
+                                                    const unsigned long long* __restrict__ list, F&& f) {
+  const unsigned int i0 = dir_items(hl.d[0]), i1 = dir_items(hl.d[1]), total = i0 + i1 + dir_items(hl.d[2]);
+  for (unsigned int it = blockIdx.x; it < total; it += gridDim.x) {
+    const int dq = it < i0 ? 0 : (it < i0 + i1 ? 1 : 2);
+    heavy_item(hl.d[dq], it - (dq == 0 ? 0u : (dq == 1 ? i0 : i0 + i1)), counts, list, f);
+  }
+}
+
+// f(entry) for every listed face of all three directions.
 template <typename F>
 __device__ __forceinline__ void for_each_heavy(const HList& hl, const unsigned int* __restrict__ counts,
                                                const unsigned long long* __restrict__ list, F&& f) {
-  unsigned int items[3], total = 0;
+  for_each_heavy_slot(hl, counts, list, [&](const unsigned long long v, unsigned long long) { f(v); });
+}
+
+// f(entry, slot) for the listed faces of one direction.
+template <typename F>
+__device__ __forceinline__ void for_each_heavy_dir(const HSeg& hs, const unsigned int* __restrict__ counts,
+                                                   const unsigned long long* __restrict__ list, F&& f) {
+  const unsigned int total = dir_items(hs);
+  for (unsigned int loc = blockIdx.x; loc < total; loc += gridDim.x) heavy_item(hs, loc, counts, list, f);
+}
+
+// One ordered application pass of the heavy-face fluxes kept per list slot (the first pass, which forms
+// them, already added the x faces to the cells right of them): SIDE 0 adds the kept flux of each listed
+// face of the direction to the cell right of the face (its left face), SIDE 1 subtracts it from the cell
+// left of the face.  Distinct faces of one direction and side touch distinct cells, so no two lanes update
+// one cell, and the passes run in a fixed order: every evaluation sums the corrections the same way.
+template <typename T, int SIDE>
+__global__ __launch_bounds__(64) void k_heavy_apply(KP kp, T* __restrict__ R, HSeg hs,
+                                                    const unsigned int* __restrict__ counts,
+                                                    const unsigned long long* __restrict__ list,
+                                                    const T* __restrict__ hflux) {
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  for_each_heavy_dir(hs, counts, list, [&](const unsigned long long v, const unsigned long long slot) {
+    const int D = (int)(v & 3);
+    const long long o = (long long)(v >> 4);  // right cell of the face
+    const int ci = (int)(o % kp.Nxt), cj = (int)((o / kp.Nxt) % kp.Nyt), ck = (int)(o / ((long long)kp.Nxt * kp.Nyt));
+    const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
+    const int f = D == 0 ? I : (D == 1 ? Jj : K);
+    const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+    const long long rr = (long long)I + (long long)kp.Nxp * ((long long)Jj + (long long)kp.Nyp * K);
+    if (SIDE == 0 && f <= Np - 1) {
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    items[q] = hl.d[q].cap > 64 ? hl.d[q].nw : (hl.d[q].nw + 63) / 64;
-    total += items[q];
-  }
-  const int lane = threadIdx.x;
-  for (unsigned int it = blockIdx.x; it < total; it += gridDim.x) {
-    const int dq = it < items[0] ? 0 : (it < items[0] + items[1] ? 1 : 2);
-    const unsigned int loc = it - (dq == 0 ? 0u : (dq == 1 ? items[0] : items[0] + items[1]));
-    const HSeg hs = hl.d[dq];
-    if (hs.cap > 64) {
-      const unsigned int n = counts[hs.first + loc];
-      const unsigned long long run0 = hs.base + (unsigned long long)loc * hs.cap;
-      for (unsigned int e = lane; e < n; e += 64) f(list[run0 + e]);
-    } else {
-      // the 64 runs' entries packed onto the lanes: exclusive prefix of their counts, then entry e goes
-      // to the last run whose prefix is <= e (empty runs share their successor's prefix)
-      __shared__ unsigned int pre[64];
-      const unsigned int w0 = loc * 64, mine = w0 + lane;
-      const unsigned int myn = mine < hs.nw ? counts[hs.first + mine] : 0u;
-      unsigned int incl = myn;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const unsigned int t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-      }
-      const unsigned int total = __shfl(incl, 63, 64);
-      pre[lane] = incl - myn;
-      __syncthreads();  // (a one-wave block)
-      for (unsigned int e = lane; e < total; e += 64) {
-        int lo = 0, hi = 63;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (pre[mid] <= e) lo = mid; else hi = mid - 1;
-        }
-        f(list[hs.base + (unsigned long long)(w0 + lo) * hs.cap + (e - pre[lo])]);
-      }
-      __syncthreads();  // pre is rewritten by the next item
+      for (int q = 0; q < 5; ++q) R[rr + q * rn] += hflux[slot * 5 + q];
     }
+    if (SIDE == 1 && f >= 1) {
+      const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
+#pragma unroll
+      for (int q = 0; q < 5; ++q) R[rl + q * rn] -= hflux[slot * 5 + q];
+    }
+  });
+}
+
+// Host side: the five ordered application passes after the first (x right, y left, y right, z left,
+// z right), each over its direction's runs.
+template <typename T>
+inline void launch_heavy_apply(const KP& kp, T* R, const HList& hl, const unsigned int* counts,
+                               const unsigned long long* list, const T* hflux, hipStream_t s) {
+  for (int d = 0; d < 3; ++d) {
+    const unsigned int it = dir_items(hl.d[d]);
+    const unsigned int nb = it < 8192u ? (it ? it : 1u) : 8192u;
+    if (d > 0)
+      hipLaunchKernelGGL((k_heavy_apply<T, 0>), dim3(nb), dim3(64), 0, s, kp, R, hl.d[d], counts, list, hflux);
+    hipLaunchKernelGGL((k_heavy_apply<T, 1>), dim3(nb), dim3(64), 0, s, kp, R, hl.d[d], counts, list, hflux);
   }
 }
 

@@ -283,11 +283,15 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
 }
 
 // ------------------------------------------------------------------ heavy faces
+// Every listed face (one launch, all directions): the full characteristic / component-wise
+// reconstruction + HLLC flux (the sweep gave these faces zero), kept in the face's list slot; the x faces
+// also add it to the cell right of them here.  The other five contributions follow in the ordered passes of
+// k_heavy_apply (no floating-point atomics: the same summation order on every evaluation).
 template <typename T, bool EIGEN>
 __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* __restrict__ Q, MetricPtrs<T> m,
-                                                  HList hl) {
+                                                  HList hl, T* __restrict__ hflux) {
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
-  for_each_heavy(hl, a.hcount, a.heavy, [&](const unsigned long long v) {
+  for_each_heavy_slot(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);  // right cell of the face
     const FvmDir dd = fvm_dir(kp, D);
@@ -301,14 +305,11 @@ __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* 
     const T n3[3] = {g[go], g[go + gs], g[go + 2 * gs]};
     T F[5];
     fvm_face_flux_n<T, EIGEN>(kp, sch, Q, a.U, oc, dd.st, n3, g[go + 3 * gs], Sc, Sc1, F);
-    const int f = D == 0 ? I : (D == 1 ? Jj : K);
-    const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
     const long long rr = rix(kp, I, Jj, K);
-    const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      if (f <= Np - 1) atomicAdd(&a.R[rr + q * rn], F[q]);  // left face of cell f
-      if (f >= 1) atomicAdd(&a.R[rl + q * rn], -F[q]);      // right face of cell f-1
+      hflux[slot * 5 + q] = F[q];
+      if (D == 0 && I <= kp.Nxp - 1) a.R[rr + q * rn] += F[q];  // left face of cell I
     }
   });
 }
@@ -375,9 +376,10 @@ int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, con
     TimedScope ts("fvm_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
     if (kp.fvm == F3D_FVM_EIGEN)
-      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl);
+      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux);
     else
-      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl);
+      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux);
+    launch_heavy_apply<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, s);
   }
   return check_launch("fvm sweeps");
 }

@@ -26,6 +26,7 @@ struct f3d_workspace {
   void *Qa, *Qb;  // stage outputs of the fused path (primitive state, 6 vars, bulk type)
   double* ldt;    // per-cell dt of local time stepping (cell-shaped; compute_dt at stage 1)
   double* fsc;    // sensor of the nonlinear filter (cell-shaped)
+  void* hflux;    // heavy-face flux per list slot (bulk type, 5 per slot)
 };
 
 namespace {
@@ -422,7 +423,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
   const size_t nc = f3d_cell_count(p);
   const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
-  size_t sz[18];
+  size_t sz[19];
   sz[0] = nc * es;                               // phi
   sz[1] = nc * 5 * es;                           // Fp
   sz[2] = nc * 5 * es;                           // Fm
@@ -448,6 +449,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[14] = sz[15] = nc * 6 * es;                 // Qa, Qb
   sz[16] = nc * sizeof(double);                  // ldt
   sz[17] = nc * sizeof(double);                  // fsc
+  sz[18] = hslots * 5 * es;                      // heavy-face flux per slot
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -458,10 +460,10 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   ws->bytes = total;
   ws->base = base;
   char* c = (char*)base;
-  void** slots[18] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+  void** slots[19] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
                       &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount,
-                      &ws->Qa, &ws->Qb, (void**)&ws->ldt, (void**)&ws->fsc};
-  for (int i = 0; i < 18; ++i) {
+                      &ws->Qa, &ws->Qb, (void**)&ws->ldt, (void**)&ws->fsc, &ws->hflux};
+  for (int i = 0; i < 19; ++i) {
     *slots[i] = c;
     c += align_up(sz[i]);
   }
@@ -508,6 +510,7 @@ WsView<T> ws_view(f3d_workspace* ws) {
   v.hcap = ws->hcap;
   v.hncap = ws->hncap;
   v.ldt = ws->ldt;
+  v.hflux = (T*)ws->hflux;
   return v;
 }
 template WsView<double> ws_view<double>(f3d_workspace*);
@@ -536,7 +539,8 @@ F3D_API int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, cons
 // one file-scope device array instead of a workspace.  Each GPU the library runs on holds its own copy
 // of the array (kNoAllocScratchBytes) until the process ends; its address is looked up once per device.
 // Nothing in it needs clearing between calls: the x sweep writes every accumulator entry before the
-// y/z marches add to it, each sweep wave writes its run's count, and the sensor writes phi.
+// y/z marches add to it, each sweep wave writes its run's count, the heavy faces' first pass writes the
+// flux slots their second pass reads, and the sensor writes phi.
 namespace {
 constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
 constexpr int kMaxDevices = 64;
@@ -573,7 +577,7 @@ int scratch_address(void** out, const char* who) {
 
 // Scratch layout of the no-allocation RHS: offsets of R, the heavy-face runs, their counts and phi.
 struct NoAllocLayout {
-  size_t R, heavy, hcount, phi, total;
+  size_t R, heavy, hcount, hflux, phi, total;
   unsigned long long hcap, hncap;
 };
 NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
@@ -589,6 +593,8 @@ NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   off += align_up(L.hcap * sizeof(unsigned long long));
   L.hcount = off;
   off += align_up(L.hncap * sizeof(unsigned int));
+  L.hflux = off;
+  off += align_up(L.hcap * 5 * sizeof(double));
   L.phi = off;
   if (with_phi) off += align_up(f3d_cell_count(p) * sizeof(double));
   L.total = off;
@@ -634,6 +640,7 @@ F3D_API int f3d_rhs_noalloc_f64(const f3d_params* p, double* R, const double* Q,
   v.hcount = (unsigned int*)(c + L.hcount);
   v.hcap = L.hcap;
   v.hncap = L.hncap;
+  v.hflux = (double*)(c + L.hflux);
   v.phi = phi ? phi : (double*)(c + L.phi);
   MetricPtrs<double> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
