@@ -15,6 +15,7 @@
 // Stage VJP: RHS VJP with scale a*dt, reverse of the ghost procedure (fill z,y,x then exchange
 // z,y,x; ghost adjoints fold into their sources, Dirichlet / inflow / isothermal-wall values
 // into parameter gradients) and k_adj_finalize (reverse of c2Prim + the RK3 blend).
+#include <cstdlib>
 #include <cstring>
 
 #include "f3d_adjoint.hpp"
@@ -66,30 +67,32 @@ __global__ __launch_bounds__(256) void k_adj_lambda(KP kp, const double* __restr
   for (int n = 0; n < 5; ++n) Lam[o + n * kp.nc] = s * lam[o + n * kp.nc];
 }
 
-// Faces of direction D whose sensor maximum over cells c-3..c+2 (c = right cell) is not UP7.
-template <int D>
+// Faces whose sensor maximum over cells c-3..c+2 (c = right cell) is not UP7, the three directions in one
+// launch: a lane takes the x, y and z faces whose right cell is its cell and appends them to its wave's run
+// of each direction.
 __global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __restrict__ phi,
                                                         unsigned long long* __restrict__ list,
-                                                        unsigned int* __restrict__ count, HSeg hs) {
+                                                        unsigned int* __restrict__ count, HList hl) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  const int nx = kp.Nxp + (D == 0), ny = kp.Nyp + (D == 1);
-  const bool act = i < nx && j < ny;
-  bool heavy = false;
-  unsigned long long entry = 0;
-  if (act) {
-    const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), st = dstride(kp, D);
-    double m = phi[o - 3 * st];
-#pragma unroll
-    for (int s = -2; s <= 2; ++s) m = jmax(m, phi[o + s * st]);
-    const int sch = scheme_of<double>(kp, m);
-    heavy = sch != F3D_SCHEME_UP7;
-    entry = ((unsigned long long)o << 4) | ((unsigned long long)sch << 2) | (unsigned long long)D;
-  }
-  // the wave's run of the list (64 x 4 blocks, one face per lane)
   const unsigned int w = launch_wave(4, threadIdx.y);
-  unsigned int cnt = 0;
-  run_append(list, hs.base + (unsigned long long)w * hs.cap, cnt, heavy, entry);
-  if (threadIdx.x == 0) count[hs.first + w] = cnt;
+#pragma unroll
+  for (int D = 0; D < 3; ++D) {
+    const bool act = i < kp.Nxp + (D == 0) && j < kp.Nyp + (D == 1) && k < kp.Nzp + (D == 2);
+    bool heavy = false;
+    unsigned long long entry = 0;
+    if (act) {
+      const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), st = dstride(kp, D);
+      double m = phi[o - 3 * st];
+#pragma unroll
+      for (int s = -2; s <= 2; ++s) m = jmax(m, phi[o + s * st]);
+      const int sch = scheme_of<double>(kp, m);
+      heavy = sch != F3D_SCHEME_UP7;
+      entry = ((unsigned long long)o << 4) | ((unsigned long long)sch << 2) | (unsigned long long)D;
+    }
+    unsigned int cnt = 0;
+    run_append(list, hl.d[D].base + (unsigned long long)w * hl.d[D].cap, cnt, heavy, entry);
+    if (threadIdx.x == 0) count[hl.d[D].first + w] = cnt;
+  }
 }
 
 // ------------------------------------------------------------------ inviscid, dense part
@@ -304,6 +307,161 @@ __global__ __launch_bounds__(256) void k_adj_visc2(KP kp, const double* __restri
   for (int v = 0; v < 4; ++v) lamQ[o + var[v] * nc] += acc[v];
 }
 
+// k_adj_visc1 + k_adj_visc2 in one k-march: the 16 partial adjoints G of a cell never reach HBM.  A block
+// holds a tile of 16 rows of 32 lanes (two rows per wave), the outer 2 lanes and rows a halo, and marches
+// the planes of its segment: per plane p every lane forms G at its cell (u, v, w, T and Lam of the column
+// in 5-plane register rings for the zeta differences, the xi/eta neighbours from L1/L2), the xi and eta
+// parts go through LDS to the tile's inner cells, which add the transposed xi/eta stencils and their own
+// G[12+v] to lamQ at plane p; the zeta parts stay in a 5-plane ring and reach lamQ at plane p-2.  G is
+// zero outside the interior+-2 box, as k_adj_visc1 leaves it, so every cell of lamQ gets what
+// k_adj_visc2 gathers.
+constexpr int kAVW = 32, kAVR = 16;  // row width, rows (inner: kAVW-4 columns, kAVR-4 rows)
+__global__ __launch_bounds__(512) void k_adj_visc_march(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
+                                                        const double* __restrict__ J, const double* __restrict__ Lam,
+                                                        double* __restrict__ lamQ, int KC) {
+  __shared__ double sG[8][kAVR][kAVW];            // G[3v] (xi) then G[3v+1] (eta) of plane p
+  __shared__ double sLQ[8][kAVR + 4][kAVW + 4];   // Lam (vars 1..4) then u,v,w,T of plane p, tile +-2
+  const int lane = threadIdx.x, col = lane & (kAVW - 1), row = 2 * threadIdx.y + (lane >> 5);
+  const int x = blockIdx.x * (kAVW - 4) - 2 + col, y = blockIdx.y * (kAVR - 4) - 2 + row;
+  const int kz0 = blockIdx.z * KC, kz1 = min(kz0 + KC, kp.Nzt);  // lamQ planes of this block: [kz0, kz1)
+  const bool act = x >= 0 && x < kp.Nxt && y >= 0 && y < kp.Nyt;
+  const bool outl = act && col >= 2 && col < kAVW - 2 && row >= 2 && row < kAVR - 2;
+  const bool xybox = x >= 2 && x <= kp.Nxt - 3 && y >= 2 && y <= kp.Nyt - 3;
+  const int NG = kp.NG;
+  const bool inx = interior(x, kp.Nxp, NG), iny = interior(y, kp.Nyp, NG);
+  const long long nc = kp.nc, sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
+  auto cl = [](int a, int n) { return min(max(a, 0), n - 1); };
+  const long long colo = (long long)cl(x, kp.Nxt) + sy * cl(y, kp.Nyt);
+  // the extra cell an edge lane (x -+ 2) or a halo row (y -+ 2) stages for the tile's +-2 neighbours
+  const bool xe = col < 2 || col >= kAVW - 2, ye = row < 2 || row >= kAVR - 2;
+  const long long colx = (long long)cl(col < 2 ? x - 2 : x + 2, kp.Nxt) + sy * cl(y, kp.Nyt);
+  const long long coly = (long long)cl(x, kp.Nxt) + sy * cl(row < 2 ? y - 2 : y + 2, kp.Nyt);
+  const int tcx = col < 2 ? col : col + 4, trx = row + 2;  // tile position of the x-extra cell
+  const int tcy = col + 2, try_ = row < 2 ? row : row + 4;  // and of the y-extra cell
+  const int tc = col + 2, tr = row + 2;
+  const double c1 = 1.0 / 12.0, c2 = -2.0 / 3.0;
+  const int var[4] = {1, 2, 3, 5};
+  // rings: slot s holds plane p-2+s of Lam (vars 1..4) and of u,v,w,T (slot 5: plane p+3, requested a
+  // step ahead); gz slot s the zeta part of G at plane p-4+s
+  double L[4][6], F[4][6], gz[4][5];
+  auto load = [&](int pl, int sl) {
+    const long long o = colo + (long long)cl(pl, kp.Nzt) * sz;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      L[v][sl] = Lam[o + (v + 1) * nc];
+      F[v][sl] = Q[o + var[v] * nc];
+    }
+  };
+  const int p0 = kz0 - 2;
+#pragma unroll
+  for (int sl = 0; sl < 5; ++sl) load(p0 - 2 + sl, sl);
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int sl = 0; sl < 5; ++sl) gz[v][sl] = 0.0;
+#pragma unroll 1
+  for (int p = p0; p <= kz1 + 1; ++p) {
+    load(p + 3, 5);
+    const long long o = colo + (long long)cl(p, kp.Nzt) * sz;
+    // stage plane p of Lam and u,v,w,T in the tile (own cell, and the edge / halo extra cells)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      sLQ[v][tr][tc] = L[v][2];
+      sLQ[4 + v][tr][tc] = F[v][2];
+    }
+    if (xe) {
+      const long long ox = colx + (long long)cl(p, kp.Nzt) * sz;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        sLQ[v][trx][tcx] = Lam[ox + (v + 1) * nc];
+        sLQ[4 + v][trx][tcx] = Q[ox + var[v] * nc];
+      }
+    }
+    if (ye) {
+      const long long oy = coly + (long long)cl(p, kp.Nzt) * sz;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        sLQ[v][try_][tcy] = Lam[oy + (v + 1) * nc];
+        sLQ[4 + v][try_][tcy] = Q[oy + var[v] * nc];
+      }
+    }
+    __syncthreads();
+    double Gx[4] = {0, 0, 0, 0}, Gy[4] = {0, 0, 0, 0}, Gs[4] = {0, 0, 0, 0};
+    gz[0][4] = gz[1][4] = gz[2][4] = gz[3][4] = 0.0;
+    const bool inz = interior(p, kp.Nzp, NG);
+    const bool slab[3] = {iny && inz, inx && inz, inx && iny};
+    if (act && xybox && p >= 2 && p <= kp.Nzt - 3 && (slab[0] || slab[1] || slab[2])) {
+      double go[3][4], g[4][3];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        go[0][n] = slab[0] ? c1 * (sLQ[n][tr][tc + 2] - sLQ[n][tr][tc - 2]) +
+                                 c2 * (sLQ[n][tr][tc + 1] - sLQ[n][tr][tc - 1])
+                           : 0.0;
+        go[1][n] = slab[1] ? c1 * (sLQ[n][tr + 2][tc] - sLQ[n][tr - 2][tc]) +
+                                 c2 * (sLQ[n][tr + 1][tc] - sLQ[n][tr - 1][tc])
+                           : 0.0;
+        go[2][n] = slab[2] ? c1 * (L[n][4] - L[n][0]) + c2 * (L[n][3] - L[n][1]) : 0.0;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        g[v][0] = c1 * (sLQ[4 + v][tr][tc - 2] - sLQ[4 + v][tr][tc + 2]) +
+                  c2 * (sLQ[4 + v][tr][tc - 1] - sLQ[4 + v][tr][tc + 1]);
+        g[v][1] = c1 * (sLQ[4 + v][tr - 2][tc] - sLQ[4 + v][tr + 2][tc]) +
+                  c2 * (sLQ[4 + v][tr - 1][tc] - sLQ[4 + v][tr + 1][tc]);
+        g[v][2] = c1 * (F[v][0] - F[v][4]) + c2 * (F[v][1] - F[v][3]);
+      }
+      double m[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) m[q] = mp.m[q][o];
+      double gg[4][3];
+      viscous_cell_vjp(kp, F[0][2], F[1][2], F[2][2], F[3][2], g, m, J[o], go, gg, Gs);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        Gx[v] = gg[v][0];
+        Gy[v] = gg[v][1];
+        gz[v][4] = gg[v][2];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      sG[v][row][col] = Gx[v];
+      sG[4 + v][row][col] = Gy[v];
+    }
+    __syncthreads();
+    // plane p: own G[12+v] + transposed xi / eta stencils (d g[c] / d f[a] for c = a + r: r = +2 -> c1,
+    // -2 -> -c1, +1 -> c2, -1 -> -c2)
+    if (outl && p >= kz0 && p < kz1) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double acc = Gs[v];
+        acc += c1 * (sG[v][row][col + 2] - sG[v][row][col - 2]) + c2 * (sG[v][row][col + 1] - sG[v][row][col - 1]);
+        acc += c1 * (sG[4 + v][row + 2][col] - sG[4 + v][row - 2][col]) +
+               c2 * (sG[4 + v][row + 1][col] - sG[4 + v][row - 1][col]);
+        lamQ[o + var[v] * nc] += acc;
+      }
+    }
+    // plane p-2: the transposed zeta stencil of the ring
+    const int q = p - 2;
+    if (outl && q >= kz0 && q < kz1) {
+      const long long oq = colo + (long long)q * sz;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        lamQ[oq + var[v] * nc] += c1 * (gz[v][4] - gz[v][0]) + c2 * (gz[v][3] - gz[v][1]);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+#pragma unroll
+      for (int sl = 0; sl < 5; ++sl) {
+        L[v][sl] = L[v][sl + 1];
+        F[v][sl] = F[v][sl + 1];
+      }
+#pragma unroll
+      for (int sl = 0; sl < 4; ++sl) gz[v][sl] = gz[v][sl + 1];
+    }
+    __syncthreads();  // sG and sLQ are rewritten for the next plane
+  }
+}
+
 // ------------------------------------------------------------------ ghost procedure, reverse
 // Cell of the lamQ ghost slab of face (D, side) that holds partial n (0..6) of the block whose first line
 // is (a0, b0): variables 0..5 of the line's first ghost cell, then variable 0 of its second.
@@ -315,20 +473,24 @@ __device__ __forceinline__ long long bc_slot(const KP& kp, int side, int a0, int
   return o + (n < 6 ? (long long)n * kp.nc : 0);
 }
 
-// lam_bc[n] += the partials of the nbx x nby blocks of one reverse fill, summed in a fixed order (a
-// strided sum per thread, then a fixed tree): the boundary-data gradient is the same bit for bit on every
-// run.  The partial slots are cleared again (every ghost adjoint is zero after a reverse fill).
+// lam_bc[n] += the partials of the nbx x nby blocks of the reverse fills of dimension D (the sides with
+// red[side] set), summed in a fixed order (a strided sum per thread, then a fixed tree): the boundary-data
+// gradient is the same bit for bit on every run.  The partial slots are cleared again (every ghost adjoint
+// is zero after a reverse fill).
 template <int D>
-__global__ __launch_bounds__(256) void k_adj_bc_final(KP kp, int side, int lw, int nbx, int nby,
+__global__ __launch_bounds__(256) void k_adj_bc_final(KP kp, int red0, int red1, int lw, int nbx, int nby,
                                                       double* __restrict__ lamQ, double* __restrict__ lam_bc) {
   __shared__ double sm[256];
   const int nb = nbx * nby;
   for (int n = 0; n < 7; ++n) {
     double v = 0.0;
-    for (int q = threadIdx.x; q < nb; q += 256) {
-      const long long o = bc_slot<D>(kp, side, (q % nbx) * lw, (q / nbx) * 4, n);
-      v += lamQ[o];
-      lamQ[o] = 0.0;
+    for (int side = 1; side >= 0; --side) {
+      if (!(side ? red1 : red0)) continue;
+      for (int q = threadIdx.x; q < nb; q += 256) {
+        const long long o = bc_slot<D>(kp, side, (q % nbx) * lw, (q / nbx) * 4, n);
+        v += lamQ[o];
+        lamQ[o] = 0.0;
+      }
     }
     sm[threadIdx.x] = v;
     __syncthreads();
@@ -344,10 +506,12 @@ __global__ __launch_bounds__(256) void k_adj_bc_final(KP kp, int side, int lw, i
 // Reverse of one side of one dimension's ghost fill (boundary.jl:1-156 with the kinds of f3d.h):
 // each lane owns one line of NG ghost cells, folds their adjoints into the source cells and
 // clears them.  lam_bc[0..5] += adjoint of q_inf, lam_bc[6] += adjoint of T_wall.
+// Both sides in one launch when their sources cannot overlap (side = side0 + blockIdx.z, kind bc0 / bc1).
 template <int D>
-__global__ __launch_bounds__(256) void k_adj_ghost(KP kp, int side, int bc, double* __restrict__ lamQ,
+__global__ __launch_bounds__(256) void k_adj_ghost(KP kp, int side0, int bc0, int bc1, double* __restrict__ lamQ,
                                                    const double* __restrict__ Q, double* __restrict__ lam_inflow,
                                                    double* __restrict__ lam_bc) {
+  const int side = side0 + (int)blockIdx.z, bc = side ? bc1 : bc0;
   const int a = blockIdx.x * 64 + threadIdx.x, b = blockIdx.y * 4 + threadIdx.y;
   // the other two dimensions, full extents
   const int na = D == 0 ? kp.Nyt : kp.Nxt, nb = D == 2 ? kp.Nyt : kp.Nzt;
@@ -428,9 +592,10 @@ __global__ __launch_bounds__(256) void k_adj_ghost(KP kp, int side, int bc, doub
 // 32-byte runs instead of one cache line per lane).  Each lane clears its ghost cell; where the four
 // ghosts fold into distinct sources (periodic, walls) each lane adds its own, where they share one
 // target (zero gradient, inflow, Dirichlet) lane 0 of the line adds them in the order of k_adj_ghost.
-__global__ __launch_bounds__(256) void k_adj_ghost_x(KP kp, int side, int bc, double* __restrict__ lamQ,
+__global__ __launch_bounds__(256) void k_adj_ghost_x(KP kp, int side0, int bc0, int bc1, double* __restrict__ lamQ,
                                                      const double* __restrict__ Q, double* __restrict__ lam_inflow,
                                                      double* __restrict__ lam_bc) {
+  const int side = side0 + (int)blockIdx.z, bc = side ? bc1 : bc0;
   const int l = threadIdx.x & 3, a = blockIdx.x * 16 + (threadIdx.x >> 2), b = blockIdx.y * 4 + threadIdx.y;
   const bool act = a < kp.Nyt && b < kp.Nzt;
   const int NG = kp.NG, Np = kp.Nxp;
@@ -692,16 +857,54 @@ int check_metrics(const double* const* metrics, const double* J, const char* who
 
 // lamQ = (dR/dQ)^T (scale * lam) over all cells of Q (RHS per unit dt, R = J * sum of differences).
 // Grid of the list kernel of direction D: one lane per face of the interior face range.
-dim3 adj_list_grid(const KP& kp, int D) {
-  return dim3((kp.Nxp + (D == 0) + 63) / 64, (kp.Nyp + (D == 1) + 3) / 4, kp.Nzp + (D == 2));
+// 1 unless F3D_ADJ_VISC=split: the viscous reverse as one march (k_adj_visc_march) instead of
+// k_adj_visc1 + k_adj_visc2 (A/B measurement).
+bool adj_visc_march() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("F3D_ADJ_VISC");
+    v = (e && std::strcmp(e, "split") == 0) ? 0 : 1;
+  }
+  return v == 1;
 }
+
+// Planes per block of k_adj_visc_march: one block per CU at a time, so the march takes about
+// rounds x (KC + 4) plane steps with rounds = ceil(blocks / CUs); the smallest such KC among a few.
+// F3D_ADJ_VISC_KC overrides.
+int adj_visc_kc(const KP& kp) {
+  static int env = -2, ncu = 0;
+  if (env == -2) {
+    const char* e = std::getenv("F3D_ADJ_VISC_KC");
+    env = e ? std::atoi(e) : -1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                hipSuccess || ncu < 1)
+      ncu = 256;
+  }
+  if (env > 0) return env;
+  const long long tiles = (long long)((kp.Nxt + kAVW - 5) / (kAVW - 4)) * ((kp.Nyt + kAVR - 5) / (kAVR - 4));
+  int best = 16;
+  long long cost = -1;
+  for (int kc : {8, 12, 16, 20, 24, 32, 48, 64}) {
+    const long long nb = tiles * ((kp.Nzt + kc - 1) / kc);
+    const long long c = ((nb + ncu - 1) / ncu) * (kc + 4);
+    if (cost < 0 || c < cost) {
+      cost = c;
+      best = kc;
+    }
+  }
+  return best;
+}
+
+// One lane per cell of the (Nxp+1) x (Nyp+1) x (Nzp+1) face-index box (its x, y and z faces).
+dim3 adj_list_grid(const KP& kp) { return dim3((kp.Nxp + 1 + 63) / 64, (kp.Nyp + 1 + 3) / 4, kp.Nzp + 1); }
 
 }  // namespace
 
 HList f3d::adj_heavy_layout(const KP& kp) {
   unsigned long long nw[3], cap[3];
+  const dim3 g = adj_list_grid(kp);
   for (int d = 0; d < 3; ++d) {
-    const dim3 g = adj_list_grid(kp, d);
     nw[d] = 4ULL * g.x * g.y * g.z;
     cap[d] = 64;
   }
@@ -723,9 +926,7 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
   if (rc) return rc;
   {
     TimedScope ts("adj_heavy_list", s);
-    hipLaunchKernelGGL(k_adj_heavy_list<0>, adj_list_grid(kp, 0), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl.d[0]);
-    hipLaunchKernelGGL(k_adj_heavy_list<1>, adj_list_grid(kp, 1), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl.d[1]);
-    hipLaunchKernelGGL(k_adj_heavy_list<2>, adj_list_grid(kp, 2), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl.d[2]);
+    hipLaunchKernelGGL(k_adj_heavy_list, adj_list_grid(kp), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl);
   }
   if (!lam_ready) {
     TimedScope ts("adj_lambda", s);
@@ -740,7 +941,13 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
     hipLaunchKernelGGL(k_adj_heavy, dim3(nb), dim3(64), 0, s, kp, Q, mp, aws->Lam, fv.heavy, fv.hcount, lamQ, hl);
   }
-  if (kp.viscous) {
+  if (kp.viscous && adj_visc_march()) {
+    TimedScope ts("adj_visc", s);
+    const int KC = adj_visc_kc(kp);
+    hipLaunchKernelGGL(k_adj_visc_march, dim3((kp.Nxt + kAVW - 5) / (kAVW - 4), (kp.Nyt + kAVR - 5) / (kAVR - 4),
+                                              (kp.Nzt + KC - 1) / KC), dim3(64, kAVR / 2, 1), 0, s, kp, Q, mp, J,
+                       aws->Lam, lamQ, KC);
+  } else if (kp.viscous) {
     {
       TimedScope ts("adj_visc1", s);
       hipLaunchKernelGGL(k_adj_visc1, dim3((kp.Nxt - 4 + 63) / 64, (kp.Nyt - 4 + 3) / 4, kp.Nzt - 4), kB, 0, s, kp,
@@ -759,30 +966,50 @@ int ghost_vjp_impl(const f3d_params* p, double* lamQ, const double* Q, double* l
                    hipStream_t s) {
   const KP kp = make_kp(p);
   TimedScope ts("adj_ghost", s);
-  auto launch = [&](int d, int side, int bc) {
-    const bool red = lam_bc && (bc == F3D_BC_DIRICHLET || bc == F3D_BC_WALL_ISOTHERMAL);
-    if (d == 0) {
-      const dim3 g((kp.Nyt + 15) / 16, (kp.Nzt + 3) / 4);
-      hipLaunchKernelGGL(k_adj_ghost_x, g, kB, 0, s, kp, side, bc, lamQ, Q, lam_inflow, lam_bc);
-      if (red) hipLaunchKernelGGL(k_adj_bc_final<0>, dim3(1), dim3(256), 0, s, kp, side, 16, (int)g.x, (int)g.y, lamQ,
-                                  lam_bc);
-    } else if (d == 1) {
-      const dim3 g((kp.Nxt + 63) / 64, (kp.Nzt + 3) / 4);
-      hipLaunchKernelGGL(k_adj_ghost<1>, g, kB, 0, s, kp, side, bc, lamQ, Q, lam_inflow, lam_bc);
-      if (red) hipLaunchKernelGGL(k_adj_bc_final<1>, dim3(1), dim3(256), 0, s, kp, side, 64, (int)g.x, (int)g.y, lamQ,
-                                  lam_bc);
-    } else {
-      const dim3 g((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4);
-      hipLaunchKernelGGL(k_adj_ghost<2>, g, kB, 0, s, kp, side, bc, lamQ, Q, lam_inflow, lam_bc);
-      if (red) hipLaunchKernelGGL(k_adj_bc_final<2>, dim3(1), dim3(256), 0, s, kp, side, 64, (int)g.x, (int)g.y, lamQ,
-                                  lam_bc);
+  // the two sides of a dimension in one launch when their source cells cannot coincide (Np >= 2 NG)
+  auto launch = [&](int d) {
+    const int bc0 = p->bc[2 * d], bc1 = p->bc[2 * d + 1];
+    const int Np = d == 0 ? kp.Nxp : (d == 1 ? kp.Nyp : kp.Nzp);
+    const bool both = Np >= 2 * kp.NG;
+    auto reduces = [&](int bc) { return lam_bc && (bc == F3D_BC_DIRICHLET || bc == F3D_BC_WALL_ISOTHERMAL); };
+    const dim3 g = d == 0 ? dim3((kp.Nyt + 15) / 16, (kp.Nzt + 3) / 4)
+                          : (d == 1 ? dim3((kp.Nxt + 63) / 64, (kp.Nzt + 3) / 4) : dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4));
+    auto ghost = [&](int side0, int nside) {
+      const dim3 gz(g.x, g.y, nside);
+      if (d == 0)
+        hipLaunchKernelGGL(k_adj_ghost_x, gz, kB, 0, s, kp, side0, bc0, bc1, lamQ, Q, lam_inflow, lam_bc);
+      else if (d == 1)
+        hipLaunchKernelGGL(k_adj_ghost<1>, gz, kB, 0, s, kp, side0, bc0, bc1, lamQ, Q, lam_inflow, lam_bc);
+      else
+        hipLaunchKernelGGL(k_adj_ghost<2>, gz, kB, 0, s, kp, side0, bc0, bc1, lamQ, Q, lam_inflow, lam_bc);
+    };
+    auto final_ = [&](int red0, int red1) {
+      if (!red0 && !red1) return;
+      const int lw = d == 0 ? 16 : 64;
+      if (d == 0)
+        hipLaunchKernelGGL(k_adj_bc_final<0>, dim3(1), dim3(256), 0, s, kp, red0, red1, lw, (int)g.x, (int)g.y, lamQ,
+                           lam_bc);
+      else if (d == 1)
+        hipLaunchKernelGGL(k_adj_bc_final<1>, dim3(1), dim3(256), 0, s, kp, red0, red1, lw, (int)g.x, (int)g.y, lamQ,
+                           lam_bc);
+      else
+        hipLaunchKernelGGL(k_adj_bc_final<2>, dim3(1), dim3(256), 0, s, kp, red0, red1, lw, (int)g.x, (int)g.y, lamQ,
+                           lam_bc);
+    };
+    if (both) {
+      ghost(0, 2);
+      final_(reduces(bc0), reduces(bc1));
+    } else {  // the high side, then the low side (the reverse of the fill's order)
+      ghost(1, 1);
+      final_(0, reduces(bc1));
+      ghost(0, 1);
+      final_(reduces(bc0), 0);
     }
   };
   // reverse of fill_z, fill_y, fill_x.  The reverse of exchange_ghost (mpi.jl, one rank) is not launched:
   // each reverse fill clears its whole slab, so every ghost adjoint is zero by then and the periodic
   // folds of the exchange would add zeros.
-  for (int d = 2; d >= 0; --d)
-    for (int side = 1; side >= 0; --side) launch(d, side, p->bc[2 * d + side]);
+  for (int d = 2; d >= 0; --d) launch(d);
   return check_launch("ghost vjp");
 }
 
@@ -838,6 +1065,10 @@ size_t stage_steps(int nsteps, int every) { return (size_t)(every == 1 ? nsteps 
 size_t ckpt_slots(int nsteps, int every) {
   return (size_t)((nsteps + every - 1) / every) + (size_t)(every - 1) + 1 + 2 * stage_steps(nsteps, every);
 }
+// With a checkpoint every step (and the fused stage), the forward sweep also keeps the primitive states
+// (ghosts filled) of the three stage inputs of every step, as the stages form them, so the reverse sweep
+// needs no c2Prim + ghost fill of its own: 3 x 6 values per cell and step after the conserved states.
+bool keep_q(int every) { return every == 1 && use_fused(); }
 
 }  // namespace
 
@@ -857,8 +1088,8 @@ F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace*
   rc = f3d_workspace_create(p, F3D_PREC_F64, &fws);
   if (rc) return rc;
   const size_t nc = f3d_cell_count(p), v = nc * sizeof(double);
-  // Lam 5, lamQ 6, G 16, Us 3x5, Qs 3x6, Un 5, lam 5, lamUn 5, Ucur 5, Qcur 6, red
-  const size_t nvars = 5 + 6 + 16 + 18 + 5 + 5 + 5 + 5 + 6;
+  // Lam 5, lamQ 6, G 16 (the split viscous reverse only), Qs 3x6, Un 5, lam 5, lamUn 5, Ucur 5, Qcur 6, red
+  const size_t nvars = 5 + 6 + (adj_visc_march() ? 0 : 16) + 18 + 5 + 5 + 5 + 5 + 6;
   const size_t total = nvars * align_up(v) + align_up(kObjBlocks * sizeof(double));
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, total);
@@ -880,7 +1111,7 @@ F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace*
   };
   a->Lam = take(5);
   a->lamQ = take(6);
-  a->G = take(16);
+  a->G = adj_visc_march() ? nullptr : take(16);
   for (int q = 0; q < 3; ++q) a->Qs[q] = take(6);
   a->Un = take(5);
   a->lam = take(5);
@@ -974,7 +1205,8 @@ F3D_API int f3d_rk3_stage_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, i
 F3D_API size_t f3d_adjoint_ckpt_bytes(const f3d_params* p, int32_t nsteps, int32_t every) {
   if (!p || nsteps < 1 || every < 1) return 0;
   if (every > nsteps) every = nsteps;  // one segment of nsteps steps
-  return ckpt_slots(nsteps, every) * 5 * f3d_cell_count(p) * sizeof(double);
+  const size_t nc = f3d_cell_count(p);
+  return (ckpt_slots(nsteps, every) * 5 + (keep_q(every) ? (size_t)nsteps * 18 : 0)) * nc * sizeof(double);
 }
 
 F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t nsteps, int32_t every,
@@ -1010,6 +1242,9 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
   double* Ufin = slot(nck + every - 1);     // U^nsteps
   double* stg0 = slot(nck + every);         // stage states U^1, U^2 per step (of the segment or of all steps)
   auto seg = [&](int q) { return q == 0 ? nullptr : seg0 + (size_t)(q - 1) * 5 * nc; };
+  const bool kq = keep_q(every);
+  double* const q0 = ckpt + ckpt_slots(nsteps, every) * 5 * nc;
+  auto qk = [&](int n, int q) { return q0 + ((size_t)3 * n + q) * 6 * nc; };  // Q of stage q of step n
   auto stg = [&](int n, int n0, int q) {
     const size_t idx = every == 1 ? (size_t)n : (size_t)(n - n0);
     return stg0 + (2 * idx + (size_t)(q - 1)) * 5 * nc;
@@ -1033,11 +1268,18 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
   const int nlast = (nck - 1) * every;  // first step of the last segment
   // ---- forward sweep: checkpoints every `every` steps, objective per step
   F3D_HIP(hipMemcpyAsync(aws->Ucur, U0, vb, hipMemcpyDeviceToDevice, s));
-  rc = prim(aws->Ucur, aws->Qcur);
+  rc = prim(aws->Ucur, kq ? qk(0, 0) : aws->Qcur);
   // (keeping the stage states of every step when every == 1, else the states and stage states of the
   // last segment, whose reversal comes first)
   for (int n = 0; n < nsteps && !rc; ++n) {
     if (n % every == 0) F3D_HIP(hipMemcpyAsync(slot(n / every), aws->Ucur, vb, hipMemcpyDeviceToDevice, s));
+    if (kq) {
+      double* Qn1 = n + 1 < nsteps ? qk(n + 1, 0) : aws->Qcur;
+      rc = step_keep_q(p, aws->fws, aws->Ucur, aws->Un, qk(n, 0), metrics, J, dt, inflow, stg(n, 0, 1), stg(n, 0, 2),
+                       qk(n, 1), qk(n, 2), Qn1, s);
+      if (!rc) obj_term(Qn1, obj + n);
+      continue;
+    }
     if (every == 1) {
       rc = step_keep(aws->Ucur, aws->Qcur, stg(n, 0, 1), stg(n, 0, 2), true);
     } else if (n >= nlast) {
@@ -1084,7 +1326,15 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
       // stage states of step n: (U^n, Q^n), (U^1, Q^1), (U^2, Q^2), U read where it is kept (prim writes
       // only Q); the stages' Q is c2Prim + ghost fill of their U, as the stage update itself forms it
       const double* Ust[3] = {Ustart, stg(n, n0, 1), stg(n, n0, 2)};
-      for (int q = 0; q < 3 && !rc; ++q) rc = prim(Ust[q], aws->Qs[q]);
+      const double* Qst[3];
+      for (int q = 0; q < 3 && !rc; ++q) {
+        if (kq) {
+          Qst[q] = qk(n, q);  // formed by the forward sweep's stages
+        } else {
+          rc = prim(Ust[q], aws->Qs[q]);
+          Qst[q] = aws->Qs[q];
+        }
+      }
       // each finalize forms the next stage VJP's Lam; that of stage 1 also adds the seed of step n - 1
       // (at U^n, its stage-1 state) unless n == 0
       for (int KRK = 3; KRK >= 1 && !rc; --KRK) {
@@ -1097,8 +1347,8 @@ F3D_API int f3d_adjoint_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t
           nx.W = W;
           nx.wscale = dt;
         }
-        rc = stage_vjp_impl(p, aws, KRK, aws->lam, aws->lam, aws->lamUn, Ust[KRK - 1], aws->Qs[KRK - 1], metrics,
-                            J, dt, lam_inflow, lam_bc, s, nx);
+        rc = stage_vjp_impl(p, aws, KRK, aws->lam, aws->lam, aws->lamUn, Ust[KRK - 1], Qst[KRK - 1], metrics, J, dt,
+                            lam_inflow, lam_bc, s, nx);
       }
       seeded = true;
     }

@@ -291,4 +291,11 @@ int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, doub
 int step_keep_stages(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, double* Q,
                      const double* const* metrics, const double* J, double dt, const double* inflow, double* U1,
                      double* U2, hipStream_t s, bool third = true, double* Uout = nullptr);
+// One fused FP64 step that keeps the stage states where the caller wants them: stage 1 reads Qin and
+// writes U^1 to U1 and its primitive state (ghosts filled) to Q1, stage 2 reads (U1, Q1) and writes
+// (U2, Q2), stage 3 reads (U2, Q2) and writes U^{n+1} to U and its primitive state to Qnext.  The Q's are
+// the states the stages themselves form (c2Prim in the update, then the ghost fill).  Fused engine only.
+int step_keep_q(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, const double* Qin,
+                const double* const* metrics, const double* J, double dt, const double* inflow, double* U1,
+                double* U2, double* Q1, double* Q2, double* Qnext, hipStream_t s);
 }  // namespace f3d

@@ -333,6 +333,19 @@ int step_keep_stages(const f3d_params* p, f3d_workspace* ws, double* U, double* 
   return rc;
 }
 
+int step_keep_q(const f3d_params* p, f3d_workspace* ws, double* U, double* Un, const double* Qin,
+                const double* const* metrics, const double* J, double dt, const double* inflow, double* U1,
+                double* U2, double* Q1, double* Q2, double* Qnext, hipStream_t s) {
+  if (!use_fused()) {
+    set_error("step_keep_q: needs the fused stage");
+    return F3D_E_ARG;
+  }
+  int rc = stage_io<double>(p, ws, 1, U, Un, Qin, Q1, metrics, J, dt, inflow, s, nullptr, U1);
+  if (!rc) rc = stage_io<double>(p, ws, 2, U1, Un, Q1, Q2, metrics, J, dt, inflow, s, nullptr, U2);
+  if (!rc) rc = stage_io<double>(p, ws, 3, U2, Un, Q2, Qnext, metrics, J, dt, inflow, s, nullptr, U);
+  return rc;
+}
+
 template <typename T>
 int stage_inplace(const f3d_params* p, f3d_workspace* ws, int KRK, double* U, double* Un, T* Q,
                   const T* const* metrics, const double* J, double dt, const T* inflow, hipStream_t s,

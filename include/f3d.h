@@ -529,7 +529,9 @@ int f3d_rk3_stage_vjp_f64(const f3d_params* p, f3d_adj_workspace* aws, int32_t K
 /* Device bytes of the checkpoint buffer f3d_adjoint_f64 needs for `nsteps` with a checkpoint
  * every `every` steps: ceil(nsteps/every) + every + 2*(every == 1 ? nsteps : every) conserved
  * states (checkpoints, the recomputed states of a segment, the final state, and the two stage
- * states of each step of a segment); an `every` above nsteps counts as nsteps (one segment).
+ * states of each step of a segment); with every == 1 (and the fused engine) also the primitive states
+ * (6 values, ghosts filled) of the three stage inputs of every step, which the forward sweep keeps so
+ * that the reverse sweep forms none; an `every` above nsteps counts as nsteps (one segment).
  * Host-only. */
 size_t f3d_adjoint_ckpt_bytes(const f3d_params* p, int32_t nsteps, int32_t every);
 

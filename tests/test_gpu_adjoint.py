@@ -299,7 +299,8 @@ def test_multistep_adjoint(torch_mod, name):
     assert rel_fd <= 1e-4
 
 
-@pytest.mark.parametrize("env", [{"F3D_RHS_ENGINE": "chain"}, {}], ids=["chain", "fused"])
+@pytest.mark.parametrize("env", [{"F3D_RHS_ENGINE": "chain"}, {}, {"F3D_ADJ_VISC": "split"}],
+                         ids=["chain", "fused", "visc_split"])
 def test_adjoint_engine_child(torch_mod, tmp_path, env):
     """The checkpointed adjoint (every = 3 over 5 steps: one full segment, then a short one, recomputed
     through copies with the kernel-by-kernel engine, in place with the fused one) in a child process
