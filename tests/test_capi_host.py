@@ -115,10 +115,12 @@ def test_adjoint_entry_points_validate_before_any_launch():
     assert L.f3d_c2prim_vjp_f64(ctypes.byref(p), None, None, None, None) == -1
     assert b"NULL" in L.f3d_last_error()
     # checkpoint buffer size: ceil(n/every) + every states of 5 variables, plus two stage states per
-    # step of a segment (of every step when every == 1)
+    # step of a segment (of every step when every == 1, which also keeps three primitive states of 6
+    # variables per step); an interval above nsteps counts as nsteps
     nc = L.f3d_cell_count(ctypes.byref(p))
-    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 1) == (50 + 1 + 2 * 50) * 5 * nc * 8
+    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 1) == ((50 + 1 + 2 * 50) * 5 + 50 * 18) * nc * 8
     assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 50, 10) == (5 + 10 + 2 * 10) * 5 * nc * 8
+    assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 7, 10) == (1 + 7 + 2 * 7) * 5 * nc * 8
     assert L.f3d_adjoint_ckpt_bytes(ctypes.byref(p), 0, 1) == 0
 
 
