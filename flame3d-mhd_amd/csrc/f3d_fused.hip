@@ -323,16 +323,20 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __
       for (int s = 0; s < 7; ++s) acc += up7c<T>(kp, s) * fm[7 - s][q];
       Fu[q] = acc;
     }
-    // the x face is the left face of cell I (its right cell)
+    // the face is the left face of its right cell (interior index I, Jj, K) and the right face of the cell
+    // before it along D
     const int ci = (int)(o % kp.Nxt), cj = (int)((o / kp.Nxt) % kp.Nyt), ck = (int)(o / ((long long)kp.Nxt * kp.Nyt));
-    const int I = ci - kp.NG;
-    const long long rr = ridx(kp, I, cj - kp.NG, ck - kp.NG);
+    const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
+    const int f = D == 0 ? I : (D == 1 ? Jj : K);
+    const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+    const long long rr = ridx(kp, I, Jj, K);
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       const T dF = Fs[q] - Fu[q];
       hflux[slot * 5 + q] = dF;
-      if (D == 0 && I <= kp.Nxp - 1) a.R[rr + q * rn] += dF;
+      if (D == 0 && f <= Np - 1) a.R[rr + q * rn] += dF;
     }
+    a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
   });
 }
 

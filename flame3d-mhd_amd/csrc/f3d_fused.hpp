@@ -161,10 +161,11 @@ __device__ __forceinline__ void for_each_heavy_dir(const HSeg& hs, const unsigne
 }
 
 // One ordered application pass of the heavy-face fluxes kept per list slot (the first pass, which forms
-// them, already added the x faces to the cells right of them): SIDE 0 adds the kept flux of each listed
-// face of the direction to the cell right of the face (its left face), SIDE 1 subtracts it from the cell
-// left of the face.  Distinct faces of one direction and side touch distinct cells, so no two lanes update
-// one cell, and the passes run in a fixed order: every evaluation sums the corrections the same way.
+// them, already added the x faces to the cells right of them, and rewrote each entry as
+// heavy_apply_entry below): SIDE 0 adds the kept flux of each listed face of the direction to the cell right
+// of the face (its left face), SIDE 1 subtracts it from the cell left of the face.  Distinct faces of one
+// direction and side touch distinct cells, so no two lanes update one cell, and the passes run in a fixed
+// order: every evaluation sums the corrections the same way.
 template <typename T, int SIDE>
 __global__ __launch_bounds__(64) void k_heavy_apply(KP kp, T* __restrict__ R, HSeg hs,
                                                     const unsigned int* __restrict__ counts,
@@ -173,22 +174,24 @@ __global__ __launch_bounds__(64) void k_heavy_apply(KP kp, T* __restrict__ R, HS
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
   for_each_heavy_dir(hs, counts, list, [&](const unsigned long long v, const unsigned long long slot) {
     const int D = (int)(v & 3);
-    const long long o = (long long)(v >> 4);  // right cell of the face
-    const int ci = (int)(o % kp.Nxt), cj = (int)((o / kp.Nxt) % kp.Nyt), ck = (int)(o / ((long long)kp.Nxt * kp.Nyt));
-    const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
-    const int f = D == 0 ? I : (D == 1 ? Jj : K);
-    const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
-    const long long rr = (long long)I + (long long)kp.Nxp * ((long long)Jj + (long long)kp.Nyp * K);
-    if (SIDE == 0 && f <= Np - 1) {
+    const long long rr = (long long)(v >> 4);  // interior index of the cell right of the face
+    if (SIDE == 0 && (v & 4)) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) R[rr + q * rn] += hflux[slot * 5 + q];
     }
-    if (SIDE == 1 && f >= 1) {
+    if (SIDE == 1 && (v & 8)) {
       const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
 #pragma unroll
       for (int q = 0; q < 5; ++q) R[rl + q * rn] -= hflux[slot * 5 + q];
     }
   });
+}
+
+// The list entry the forming pass leaves for the application passes: (interior index of the right cell << 4)
+// | (the face has a cell on its left << 3) | (a cell on its right << 2) | direction.
+__device__ __forceinline__ unsigned long long heavy_apply_entry(long long rr, bool left, bool right, int D) {
+  return ((unsigned long long)rr << 4) | ((unsigned long long)left << 3) | ((unsigned long long)right << 2) |
+         (unsigned long long)D;
 }
 
 // Host side: the five ordered application passes after the first (x right, y left, y right, z left,

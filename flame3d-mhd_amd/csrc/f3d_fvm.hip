@@ -305,12 +305,15 @@ __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* 
     const T n3[3] = {g[go], g[go + gs], g[go + 2 * gs]};
     T F[5];
     fvm_face_flux_n<T, EIGEN>(kp, sch, Q, a.U, oc, dd.st, n3, g[go + 3 * gs], Sc, Sc1, F);
+    const int f = D == 0 ? I : (D == 1 ? Jj : K);
+    const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
     const long long rr = rix(kp, I, Jj, K);
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       hflux[slot * 5 + q] = F[q];
-      if (D == 0 && I <= kp.Nxp - 1) a.R[rr + q * rn] += F[q];  // left face of cell I
+      if (D == 0 && f <= Np - 1) a.R[rr + q * rn] += F[q];  // left face of cell f
     }
+    a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
   });
 }
 
