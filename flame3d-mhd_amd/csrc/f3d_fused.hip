@@ -16,6 +16,7 @@
 //                 the LES models and the inviscid case
 // Splitting every face as UP7 first and correcting the rare non-UP7 faces from a compacted list
 // keeps the sweep waves free of the characteristic branch's divergence and registers.
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -278,6 +279,168 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
   if (threadIdx.x == 0) a.hcount[hs.first + hw] = hcnt;
 }
 
+// ------------------------------------------------------------------ x and y faces in one march
+// Lanes = a window of XYW cells along x (4 halo cells each side, XYW - 8 interior columns), threadIdx.y =
+// z line; the block marches a segment of L rows along y.  Per step the new row's cells are split along x
+// into LDS and the UP7 x-face fluxes formed (k_sweep_x's sums, same order), kept for four steps in an LDS
+// ring; the y part is k_sweep_march's shift register.  When a y face closes row r,
+//   R[r] = (F_x,left - F_x,right) + (F_y,prev - F_y,new),
+// the same sums and roundings as the x sweep's store followed by the y march's update (the compiler
+// contracts some products into FMAs differently once both splits of a cell share a kernel, so the two
+// forms agree to rounding), while R is written once instead of written, read and written again, and Q and
+// phi are read once.  The x faces go to one run of the heavy list per wave, the y faces to another.
+constexpr int XYW = 64;        // lanes of a row window (one wave)
+constexpr int XYI = XYW - 8;   // interior columns per window
+constexpr int XYL = 2;         // z lines (waves) per block
+
+template <typename T, int SPLIT>
+__global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L, HSeg hx, HSeg hy) {
+  __shared__ T sFp[5][XYL][XYW];
+  __shared__ T sFm[5][XYL][XYW];
+  __shared__ T sphi[XYL][XYW];
+  __shared__ T sF[4][5][XYL][XYW];  // x-face fluxes of the last four rows (slot = step & 3)
+  const int l = threadIdx.x, r = threadIdx.y;
+  const int fx0 = blockIdx.x * XYI;
+  const int ncx = min(XYI, kp.Nxp - fx0);  // interior columns of the window
+  const int nfx = ncx + 1;                 // x faces formed, fx0 .. fx0 + ncx (lanes 4 .. 4 + ncx)
+  const int tt = blockIdx.y * XYL + r;
+  const bool lineok = tt < kp.Nzp;
+  const int k = kp.NG + min(tt, kp.Nzp - 1);
+  const int x = min(kp.NG + fx0 - 4 + l, kp.Nxt - 1);  // the lane's cell (lanes past the stencil clamped)
+  const bool yin = l >= 4 && l < 4 + ncx;               // interior column: its y line is updated
+  const bool act = lineok && yin;
+  const int ii = fx0 + l - 4;
+  const int f0 = blockIdx.z * L;
+  const int ncell = min(L, kp.Nyp - f0);
+  const int nstep = ncell + 8;
+  const long long st = kp.Nxt, nc = kp.nc;
+  const long long obase = cidx(kp, x, kp.NG + f0 - 4, k);
+  const T* __restrict__ A1p = a.m.m[3];
+  const T* __restrict__ A2p = a.m.m[4];
+  const T* __restrict__ A3p = a.m.m[5];
+  const T* __restrict__ X1p = a.m.m[0];
+  const T* __restrict__ X2p = a.m.m[1];
+  const T* __restrict__ X3p = a.m.m[2];
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  T acc[8][5];
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int n = 0; n < 5; ++n) acc[s][n] = 0;
+  T ring[6];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) ring[s] = 0;
+  T Fprev[5] = {0, 0, 0, 0, 0};
+  const unsigned int hw = launch_wave(XYL, r);
+  const unsigned long long xrun = hx.base + (unsigned long long)hw * hx.cap;
+  const unsigned long long yrun = hy.base + (unsigned long long)hw * hy.cap;
+  unsigned int xcnt = 0, ycnt = 0;
+  // prefetched inputs of the next cell (x metrics only on the steps that form x faces: rows f0 ..)
+  T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph, nb1 = 0, nb2 = 0, nb3 = 0;
+  {
+    const long long o = obase;
+    nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
+    nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
+    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
+  }
+#pragma unroll 1
+  for (int step = 0; step < nstep; ++step) {
+    const T q0 = nq0, q1 = nq1, q2 = nq2, q3 = nq3, q5 = nq5, A1 = na1, A2 = na2, A3 = na3, ph = nph;
+    const T B1 = nb1, B2 = nb2, B3 = nb3;
+    const bool xstep = step >= 4 && step < 4 + ncell;  // this step's row is interior: form its x faces
+    if (step + 1 < nstep) {
+      const long long o = obase + (long long)(step + 1) * st;
+      nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
+      nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
+      na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
+      if (step + 1 >= 4 && step + 1 < 4 + ncell) {
+        nb1 = X1p[o]; nb2 = X2p[o]; nb3 = X3p[o];
+      }
+    }
+    if (xstep) {
+      T fp[5], fm[5];
+      split_cell<T, SPLIT>(kp, q0, q1, q2, q3, q5, B1, B2, B3, sqrt(B1 * B1 + B2 * B2 + B3 * B3), fp, fm);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        sFp[n][r][l] = fp[n];
+        sFm[n][r][l] = fm[n];
+      }
+      sphi[r][l] = ph;
+    }
+    {
+      T fp[5], fm[5];
+      split_cell<T, SPLIT>(kp, q0, q1, q2, q3, q5, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp, fm);
+#pragma unroll
+      for (int q = 0; q < 7; ++q)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          acc[7 - q][n] += up7c<T>(kp, q) * fp[n];
+          acc[q][n] += up7c<T>(kp, q) * fm[n];
+        }
+    }
+    if (step >= 7) {  // y face f0 + step - 7 complete
+      const int fi = step - 7;
+      T m = ring[0];
+#pragma unroll
+      for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
+      const int sch = scheme_of<T>(kp, m);
+      const int f = f0 + fi;
+      run_append(a.heavy, yrun, ycnt, act && (fi < L || f == kp.Nyp) && sch != F3D_SCHEME_UP7,
+                 ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) | 1ULL);
+      if (fi >= 1 && act) {  // row f - 1: its x faces were formed four steps ago (slot step & 3)
+        const long long ro = ridx(kp, ii, f - 1, tt);
+        const int sl = step & 3;
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          const T Rc = sF[sl][n][r][l] - sF[sl][n][r][l + 1];
+          a.R[ro + n * rn] = Rc + (Fprev[n] - acc[0][n]);
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < 5; ++n) Fprev[n] = acc[0][n];
+    }
+#pragma unroll
+    for (int s = 0; s < 7; ++s)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) acc[s][n] = acc[s + 1][n];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) acc[7][n] = 0;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) ring[s] = ring[s + 1];
+    ring[5] = ph;
+    __syncthreads();  // split fluxes of this row written; the ring slot read above is free
+    if (xstep) {
+      const bool fact = l >= 4 && l < 4 + nfx;
+      const int f = fx0 + l - 4;
+      const int sl = step & 3;
+      int sch = F3D_SCHEME_UP7;
+      if (fact) {
+        T m = sphi[r][l - 3];
+#pragma unroll
+        for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
+        sch = scheme_of<T>(kp, m);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          T acx = 0;
+#pragma unroll
+          for (int q = 0; q < 7; ++q) acx += up7c<T>(kp, q) * sFp[n][r][l - 4 + q];
+#pragma unroll
+          for (int q = 0; q < 7; ++q) acx += up7c<T>(kp, q) * sFm[n][r][l + 3 - q];
+          sF[sl][n][r][l] = acx;
+        }
+      }
+      const int j = kp.NG + f0 + step - 4;
+      run_append(a.heavy, xrun, xcnt, lineok && fact && (f < fx0 + XYI || f == kp.Nxp) && sch != F3D_SCHEME_UP7,
+                 ((unsigned long long)cidx(kp, kp.NG + f, j, k) << 4) | ((unsigned long long)sch << 2) | 0ULL);
+    }
+    __syncthreads();  // the split fluxes are read before the next row overwrites them
+  }
+  if (l == 0) {
+    a.hcount[hx.first + hw] = xcnt;
+    a.hcount[hy.first + hw] = ycnt;
+  }
+}
+
 // ------------------------------------------------------------------ heavy faces
 // Every listed face (all three directions, one launch): its characteristic WENO7/WENO5/NND flux
 // (schemes.jl:756-1009) minus the UP7 flux the sweep accumulated, kept in the face's list slot; the x faces
@@ -483,7 +646,7 @@ __global__ __launch_bounds__(256) void k_update(KP kp, FArgs<T> a, const T* __re
       const double u1 = u0 + (double)Rt[n] * Jact;
       double r;
       if (MODE == 1) {
-        Un[o + n * nc] = u0;
+        if (Un) Un[o + n * nc] = u0;  // (null: the caller keeps U^n in U, Uo takes the stage)
         r = u1;
       } else if (MODE == 2) {
         r = u1 * 0.25 + Un[o + n * nc] * 0.75;
@@ -534,7 +697,7 @@ __device__ __forceinline__ void update_cell(const KP& kp, const T* Rt, long long
     const double u1 = u0 + (double)Rt[n] * Jact;
     double r;
     if (MODE == 1) {
-      Un[oc + n * nc] = u0;
+      if (Un) Un[oc + n * nc] = u0;
       r = u1;
     } else if (MODE == 2) {
       r = u1 * 0.25 + Un[oc + n * nc] * 0.75;
@@ -603,7 +766,7 @@ __device__ __forceinline__ void update_cell_ops(const KP& kp, const T* Rt, long 
     const double u1 = u0 + (double)Rt[n] * Jact;
     double r;
     if (MODE == 1) {
-      Un[oc + n * nc] = u0;
+      if (Un) Un[oc + n * nc] = u0;
       r = u1;
     } else if (MODE == 2) {
       r = u1 * 0.25 + op.Un[n] * 0.75;
@@ -1018,11 +1181,76 @@ static dim3 grid_march(const KP& kp, int D, int L) {
   const int Nt2 = D == 1 ? kp.Nzp : kp.Nyp, Np = D == 1 ? kp.Nyp : kp.Nzp;
   return dim3((kp.Nxp + 63) / 64, (Nt2 + 3) / 4, (Np + L - 1) / L);
 }
+static dim3 grid_xy(const KP& kp, int L) {
+  return dim3((kp.Nxp + XYI - 1) / XYI, (kp.Nzp + XYL - 1) / XYL, (kp.Nyp + L - 1) / L);
+}
 static unsigned long long nblocks(const dim3& g) { return (unsigned long long)g.x * g.y * g.z; }
+
+// 1 (default): x and y faces in one march (k_sweep_xy); F3D_SWEEP_XY=0 runs k_sweep_x and the y march
+// (A/B measurement; the two forms agree to rounding: the same sums in the same order, contracted into
+// FMAs differently by the compiler).
+static int sweep_xy() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("F3D_SWEEP_XY");
+    v = (e && std::strcmp(e, "0") == 0) ? 0 : 1;
+  }
+  return v;
+}
+
+// Resident waves per CU of a kernel launched in blocks of `threads` lanes (the occupancy query;
+// `fallback` when it fails, e.g. without a device).
+static int waves_per_cu(const void* kern, int threads, int fallback) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, 0) != hipSuccess || nb <= 0) {
+    (void)hipGetLastError();
+    return fallback;
+  }
+  return nb * (threads / 64);
+}
+
+// Row-segment length of the x+y march.  Its launch has few enough waves that their count against the
+// chip's resident-wave slots sets the duration: w waves over s slots take about (L + 8) * ceil(w / s)
+// marching steps when they overfill the chip, and (L + 8) / sqrt(w / s) when they fill only part of it
+// (fewer waves keep fewer bytes in flight); the chooser minimises that over L = 16 .. 64.  Measured on
+// the 512x192x64 ramp, FP64 L = 16, 24, 32, 48, 64, 96: 281, 285, 271, 297, 264, 344 us; FP32-bulk 164,
+// 166, 187, 161, 199, 282 us; the estimate ranks them the same way.  F3D_MARCH_SEG_Y overrides.
+static int xy_seg(const KP& kp, int es) {
+  const char* e = std::getenv("F3D_MARCH_SEG_Y");
+  if (e && std::atoi(e) >= 8) return seg_len(kp.Nyp, std::atoi(e));
+  static int wpc[2] = {0, 0};
+  int& w = wpc[es == 8];
+  if (w == 0)
+    w = es == 8 ? waves_per_cu((const void*)&k_sweep_xy<double, F3D_SPLIT_SW>, XYW * XYL, 8)
+                : waves_per_cu((const void*)&k_sweep_xy<float, F3D_SPLIT_SW>, XYW * XYL, 12);
+  const double slots = (double)cu_count() * w;
+  const double per_seg = (double)XYL * ((kp.Nxp + XYI - 1) / XYI) * ((kp.Nzp + XYL - 1) / XYL);
+  static const int cand[5] = {16, 24, 32, 48, 64};
+  int best = seg_len(kp.Nyp, 64);
+  double bc = -1;
+  for (int c : cand) {
+    const int L = seg_len(kp.Nyp, c);
+    const double f = per_seg * ((kp.Nyp + L - 1) / L) / slots;
+    const double cost = (L + 8) * (f > 1 ? std::ceil(f) : 1.0 / std::sqrt(f));
+    if (bc < 0 || cost < bc * 0.999) {
+      bc = cost;
+      best = L;
+    }
+  }
+  return best;
+}
 
 HList fdm_heavy_layout(const KP& kp, int es) {
   const int Lx = seg_len(kp.Nxp, XW - 8);
   const int Ly = seg_len(kp.Nyp, march_seg(1, es)), Lz = seg_len(kp.Nzp, march_seg(2, es));
+  if (sweep_xy()) {
+    // one wave per z line of a block; per row at most one x face per lane, per closed y face one per lane
+    const int Lxy = xy_seg(kp, es);
+    const unsigned long long w = XYL * nblocks(grid_xy(kp, Lxy));
+    const unsigned long long nw[3] = {w, w, 4 * nblocks(grid_march(kp, 2, Lz))};
+    const unsigned long long cap[3] = {64ULL * Lxy, 64ULL * (Lxy + 1), 64ULL * (Lz + 1)};
+    return make_hlist(nw, cap);
+  }
   // 4 waves per block; an x wave lists at most one face per lane, a march wave one per lane and face
   const unsigned long long nw[3] = {4 * nblocks(grid_sweep_x(kp, Lx)), 4 * nblocks(grid_march(kp, 1, Ly)),
                                     4 * nblocks(grid_march(kp, 2, Lz))};
@@ -1050,15 +1278,21 @@ static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
   const HList hl = fdm_heavy_layout(kp, (int)sizeof(T));
   if (int rc = check_hlist(ws, hl, "fused sweeps")) return rc;
-  {
-    TimedScope ts("fused_sweep_x", s);
-    const int L = seg_len(kp.Nxp, XW - 8);
-    hipLaunchKernelGGL((k_sweep_x<T, SPLIT>), grid_sweep_x(kp, L), dim3(XW, 2, 1), 0, s, kp, a, L, hl.d[0]);
-  }
-  {
-    TimedScope ts("fused_sweep_y", s);
-    const int L = seg_len(kp.Nyp, march_seg(1, (int)sizeof(T)));
-    hipLaunchKernelGGL((k_sweep_march<T, 1, SPLIT>), grid_march(kp, 1, L), dim3(64, 4, 1), 0, s, kp, a, L, hl.d[1]);
+  if (sweep_xy()) {
+    TimedScope ts("fused_sweep_xy", s);
+    const int L = xy_seg(kp, (int)sizeof(T));
+    hipLaunchKernelGGL((k_sweep_xy<T, SPLIT>), grid_xy(kp, L), dim3(XYW, XYL, 1), 0, s, kp, a, L, hl.d[0], hl.d[1]);
+  } else {
+    {
+      TimedScope ts("fused_sweep_x", s);
+      const int L = seg_len(kp.Nxp, XW - 8);
+      hipLaunchKernelGGL((k_sweep_x<T, SPLIT>), grid_sweep_x(kp, L), dim3(XW, 2, 1), 0, s, kp, a, L, hl.d[0]);
+    }
+    {
+      TimedScope ts("fused_sweep_y", s);
+      const int L = seg_len(kp.Nyp, march_seg(1, (int)sizeof(T)));
+      hipLaunchKernelGGL((k_sweep_march<T, 1, SPLIT>), grid_march(kp, 1, L), dim3(64, 4, 1), 0, s, kp, a, L, hl.d[1]);
+    }
   }
   {
     TimedScope ts("fused_sweep_z", s);

@@ -16,7 +16,8 @@
 //   k_fvm_march      y and z faces: lanes march along the direction with the last 8 cells' U in a
 //                    register window, next cell and next face geometry prefetched; R += dF_y, dF_z
 //   k_fvm_heavy      the listed faces: full characteristic / component-wise reconstruction + HLLC,
-//                    added to the two cells with atomics
+//                    kept in the face's list slot, then added to the two cells by the ordered
+//                    k_heavy_apply passes (no floating-point atomics)
 //   fused_update     viscous divergence + update + RK3 blend + c2Prim (f3d_fused.hip)
 #include <cstdlib>
 #include <cstring>

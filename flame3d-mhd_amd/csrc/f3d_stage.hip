@@ -294,7 +294,21 @@ int steps_io(const f3d_params* p, f3d_workspace* ws, int nsteps, double* U, doub
   const bool rot = stage_needs_separate_q(kp);
   T* Qa = rot ? (T*)ws->Qa : Q;
   T* Qb = rot ? (T*)ws->Qb : Q;
+  // Fused finite-difference stages, every step but the call's last: U keeps U^n while the stages write
+  // U^1 and U^2 into Un, and the third stage blends Un's U^2 with U's U^n into U, so stage 1 copies
+  // nothing (solver.jl:418's Un <- U is implicit).  The last step takes the reference's sequence, so
+  // that Un holds that step's U^n on return.
+  const bool swap = use_fused() && !geo;
   for (int t = 1; t <= nsteps; ++t) {
+    if (swap && t < nsteps) {
+      int rc = stage_io<T>(p, ws, 1, U, nullptr, Q, Qa, metrics, J, dt, inflow, s, geo, Un);
+      if (!rc) rc = stage_io<T>(p, ws, 2, Un, U, Qa, Qb, metrics, J, dt, inflow, s, geo, Un);
+      if (!rc) rc = stage_io<T>(p, ws, 3, Un, U, Qb, Q, metrics, J, dt, inflow, s, geo, U);
+      if (!rc && p->filtering && t % p->filtering_interval == 0)
+        rc = launch_filter_block<T>(kp, U, Un, Q, ws->fsc, p->filtering_nonlinear, p->filtering_rth, p->filtering_s0, s);
+      if (rc) return rc;
+      continue;
+    }
     int rc = stage_io<T>(p, ws, 1, U, Un, Q, Qa, metrics, J, dt, inflow, s, geo);
     if (!rc) rc = stage_io<T>(p, ws, 2, U, Un, Qa, Qb, metrics, J, dt, inflow, s, geo);
     if (!rc) rc = stage_io<T>(p, ws, 3, U, Un, Qb, Q, metrics, J, dt, inflow, s, geo);

@@ -1,0 +1,66 @@
+"""The fused stage's two forms of the x and y faces agree to rounding: the one-march form (x faces formed
+in the y march, R written once; the default) against k_sweep_x followed by the y march
+(F3D_SWEEP_XY=0, in a child process: the library reads its switches once per process).  The sums are the
+same, in the same order; the compiler contracts products into FMAs differently once both splits of a cell
+sit in one kernel, so the bar is rounding-level: RHS within 1e-14 (FP64) / 2e-6 (FP32-bulk) of the
+per-variable scale of the terms it is summed from, and the state after three time steps (whose last leaves
+Un = U^n) within 1e-13 / 1e-6 normwise.  Inputs with faces on WENO7, WENO5 and NND, window and segment
+edges that do not divide the grid, both numeric forms."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from casegen import f3d, ramp_case, relerr, rhs_scales, scaled_err, tgv_case
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+
+def _case(name, dtype):
+    if name == "ramp70x30x9":  # two x windows (56 + 14 columns), y segments that do not divide 30
+        return ramp_case(70, 30, 9, dtype=dtype, jump=1.0, spikes=[(5, 3, 2, 20.0), (60, 27, 7, 3.0)],
+                         hybrid_phi3=2.5, interior_noise=0.02)
+    if name == "swept20x12x10":  # one narrow window
+        return ramp_case(20, 12, 10, sweep=30, dtype=dtype, jump=0.5, spikes=[(6, 4, 3, 20.0)], hybrid_phi3=2.5,
+                         interior_noise=1e-3)
+    return tgv_case(16, dtype=dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32b"])
+@pytest.mark.parametrize("name", ["ramp70x30x9", "swept20x12x10", "tgv16"])
+def test_sweep_forms_agree(tmp_path, name, dtype):
+    import torch
+    import sweep_child
+    assert torch.cuda.is_available(), "GPU test"
+    case = _case(name, dtype)
+    inflow = case.inflow if case.inflow is not None else np.zeros(0, dtype=dtype)
+    arrays = dict(params=np.frombuffer(bytes(case.params), dtype=np.uint8), Q=case.Q, U=case.U(), J=case.J,
+                  inflow=inflow, nsteps=np.int64(3),
+                  dt=np.float64(0.3 * Oracle(case.params).cfl_dt(case.Q, case.metrics, case.J, 1.0)))
+    arrays.update({f"m{i}": a for i, a in enumerate(case.metrics)})
+    src, dst = str(tmp_path / "in.npz"), str(tmp_path / "out.npz")
+    np.savez(src, **arrays)
+    z = np.load(src)
+    mine = sweep_child.run(f3d, torch, z)
+    env = {k: v for k, v in os.environ.items() if not k.startswith("F3D_")}
+    env["F3D_SWEEP_XY"] = "0"
+    print(f"{name} {np.dtype(dtype).name}: child with F3D_SWEEP_XY=0", flush=True)
+    r = subprocess.run([sys.executable, os.path.join(HERE, "sweep_child.py"), src, dst], env=env,
+                       capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, f"child exit {r.returncode}\n{r.stderr[-2000:]}"
+    ref = np.load(dst)
+    f64 = dtype == np.float64
+    eR = scaled_err(mine["R"], ref["R"], rhs_scales(case))
+    print(f"  R: term-scaled |one march - two kernels| = {eR:.2e}")
+    assert eR <= (1e-14 if f64 else 2e-6), eR
+    for k in ("U", "Un", "Q"):
+        e = relerr(mine[k], ref[k])
+        print(f"  {k}: normwise {e:.2e}")
+        assert e <= (1e-13 if f64 else 1e-6), (k, e)
+    assert np.any(ref["R"] != 0)
