@@ -44,6 +44,16 @@ static int visc_fused() {
   return v;
 }
 
+__global__ __launch_bounds__(256) void k_clear_hidx(unsigned long long* __restrict__ a, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    a[t] = 0ULL;
+}
+
+void clear_hidx(unsigned long long* idx, long long n, hipStream_t s) {
+  const long long nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_clear_hidx, dim3((unsigned int)(nb < 4096 ? (nb ? nb : 1) : 4096)), dim3(256), 0, s, idx, n);
+}
+
 bool fused_needs_fv(const KP& kp) { return kp.viscous && !(visc_fused() && kp.les == F3D_LES_NONE); }
 
 bool use_fused() {
@@ -443,13 +453,12 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
 
 // ------------------------------------------------------------------ heavy faces
 // Every listed face (all three directions, one launch): its characteristic WENO7/WENO5/NND flux
-// (schemes.jl:756-1009) minus the UP7 flux the sweep accumulated, kept in the face's list slot; the x faces
-// also add it to the cell right of them here (distinct x faces, distinct cells, and no other RHS write in
-// this launch).  The other five corrections follow in ordered passes (k_heavy_apply), so no floating-point
-// atomics: the corrections are summed in the same order on every evaluation.
+// (schemes.jl:756-1009) minus the UP7 flux the sweep accumulated, kept in the face's list slot and recorded
+// in the per-face index; k_heavy_gather then adds each cell's corrections in a fixed order, so no
+// floating-point atomics: the corrections are summed the same way on every evaluation.
 template <typename T, int SPLIT, bool CHAR>
-__global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux) {
-  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp, nc = kp.nc;
+__global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux, HIdx hi) {
+  const long long nc = kp.nc;
   for_each_heavy_slot(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);
@@ -494,11 +503,8 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __
     const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
     const long long rr = ridx(kp, I, Jj, K);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      const T dF = Fs[q] - Fu[q];
-      hflux[slot * 5 + q] = dF;
-      if (D == 0 && f <= Np - 1) a.R[rr + q * rn] += dF;
-    }
+    for (int q = 0; q < 5; ++q) hflux[slot * 5 + q] = Fs[q] - Fu[q];
+    heavy_record(kp, hi, D, I, Jj, K, slot);
     a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
   });
 }
@@ -1302,11 +1308,12 @@ static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const
   {
     TimedScope ts("fused_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
+    const HIdx hi = next_hidx(kp, ws.hidx, ws.hstamp, s);
     if (kp.character)
-      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux);
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
     else
-      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux);
-    launch_heavy_apply<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, s);
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
+    launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s);
   }
   return check_launch("fused sweeps");
 }

@@ -18,6 +18,8 @@ struct WsView {
   unsigned long long hncap;  // capacity of `hcount` (runs)
   double* ldt;               // per-cell dt of local time stepping, cell-shaped (written at stage 1)
   T* hflux;                  // per list slot: the 5 flux values a heavy face adds to its cells (hcap x 5)
+  unsigned long long* hidx;  // per-face records of the listed faces (3 x hbox, see HIdx)
+  unsigned int* hstamp;      // host counter of the records' stamps
 };
 
 // Block coordinates with an XCD-aware order.  Workgroups are dispatched round-robin over the 8 XCDs
@@ -152,60 +154,103 @@ __device__ __forceinline__ void for_each_heavy(const HList& hl, const unsigned i
   for_each_heavy_slot(hl, counts, list, [&](const unsigned long long v, unsigned long long) { f(v); });
 }
 
-// f(entry, slot) for the listed faces of one direction.
-template <typename F>
-__device__ __forceinline__ void for_each_heavy_dir(const HSeg& hs, const unsigned int* __restrict__ counts,
-                                                   const unsigned long long* __restrict__ list, F&& f) {
-  const unsigned int total = dir_items(hs);
-  for (unsigned int loc = blockIdx.x; loc < total; loc += gridDim.x) heavy_item(hs, loc, counts, list, f);
-}
-
-// One ordered application pass of the heavy-face fluxes kept per list slot (the first pass, which forms
-// them, already added the x faces to the cells right of them, and rewrote each entry as
-// heavy_apply_entry below): SIDE 0 adds the kept flux of each listed face of the direction to the cell right
-// of the face (its left face), SIDE 1 subtracts it from the cell left of the face.  Distinct faces of one
-// direction and side touch distinct cells, so no two lanes update one cell, and the passes run in a fixed
-// order: every evaluation sums the corrections the same way.
-template <typename T, int SIDE>
-__global__ __launch_bounds__(64) void k_heavy_apply(KP kp, T* __restrict__ R, HSeg hs,
-                                                    const unsigned int* __restrict__ counts,
-                                                    const unsigned long long* __restrict__ list,
-                                                    const T* __restrict__ hflux) {
-  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
-  for_each_heavy_dir(hs, counts, list, [&](const unsigned long long v, const unsigned long long slot) {
-    const int D = (int)(v & 3);
-    const long long rr = (long long)(v >> 4);  // interior index of the cell right of the face
-    if (SIDE == 0 && (v & 4)) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) R[rr + q * rn] += hflux[slot * 5 + q];
-    }
-    if (SIDE == 1 && (v & 8)) {
-      const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
-#pragma unroll
-      for (int q = 0; q < 5; ++q) R[rl + q * rn] -= hflux[slot * 5 + q];
-    }
-  });
-}
-
-// The list entry the forming pass leaves for the application passes: (interior index of the right cell << 4)
+// The list entry the forming pass leaves for the gather pass: (interior index of the right cell << 4)
 // | (the face has a cell on its left << 3) | (a cell on its right << 2) | direction.
 __device__ __forceinline__ unsigned long long heavy_apply_entry(long long rr, bool left, bool right, int D) {
   return ((unsigned long long)rr << 4) | ((unsigned long long)left << 3) | ((unsigned long long)right << 2) |
          (unsigned long long)D;
 }
 
-// Host side: the five ordered application passes after the first (x right, y left, y right, z left,
-// z right), each over its direction's runs.
+// Per-face record of the listed faces, for the gather pass: hidx[d * hbox + I + (Nxp+1) * (J + (Nyp+1) * K)]
+// holds (stamp << 36) | list slot for the face left of interior cell (I, J, K) along d (the last face of a
+// line at index N_d).  Every heavy-face evaluation takes a new stamp, so the records of earlier ones need
+// no clearing (the array is cleared when the 28-bit stamp wraps).
+constexpr int kHStampBits = 28;
+constexpr unsigned long long kHSlotMask = (1ULL << (64 - kHStampBits)) - 1;
+struct HIdx {
+  unsigned long long* idx;
+  unsigned long long stamp;  // stamp << (64 - kHStampBits)
+};
+__host__ __device__ inline long long hbox(const KP& kp) {
+  return (long long)(kp.Nxp + 1) * (kp.Nyp + 1) * (kp.Nzp + 1);
+}
+__device__ __forceinline__ long long hface(const KP& kp, int I, int J, int K) {
+  return I + (long long)(kp.Nxp + 1) * (J + (long long)(kp.Nyp + 1) * K);
+}
+// Records the listed face of direction D left of interior cell (I, J, K) as held in `slot`.
+__device__ __forceinline__ void heavy_record(const KP& kp, const HIdx& hi, int D, int I, int J, int K,
+                                             unsigned long long slot) {
+  hi.idx[D * hbox(kp) + hface(kp, I, J, K)] = hi.stamp | slot;
+}
+
+// The correction kept per list slot added to the RHS, one cell at a time: each listed face looks at its
+// two cells; a cell is taken by the first of its listed faces in the order (x left, x right, y left,
+// y right, z left, z right), whose lane adds all of the cell's corrections in that order (a face's flux
+// enters the cell right of it with +, the cell left of it with -).  One lane per cell and a fixed
+// summation order: no floating-point atomics, the same bits on every evaluation.
 template <typename T>
-inline void launch_heavy_apply(const KP& kp, T* R, const HList& hl, const unsigned int* counts,
-                               const unsigned long long* list, const T* hflux, hipStream_t s) {
-  for (int d = 0; d < 3; ++d) {
-    const unsigned int it = dir_items(hl.d[d]);
-    const unsigned int nb = it < 8192u ? (it ? it : 1u) : 8192u;
-    if (d > 0)
-      hipLaunchKernelGGL((k_heavy_apply<T, 0>), dim3(nb), dim3(64), 0, s, kp, R, hl.d[d], counts, list, hflux);
-    hipLaunchKernelGGL((k_heavy_apply<T, 1>), dim3(nb), dim3(64), 0, s, kp, R, hl.d[d], counts, list, hflux);
+__global__ __launch_bounds__(64) void k_heavy_gather(KP kp, T* __restrict__ R, HList hl,
+                                                     const unsigned int* __restrict__ counts,
+                                                     const unsigned long long* __restrict__ list,
+                                                     const T* __restrict__ hflux, HIdx hi) {
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp, nb = hbox(kp);
+  for_each_heavy_slot(hl, counts, list, [&](const unsigned long long v, unsigned long long) {
+    const int D = (int)(v & 3);
+    const long long rr = (long long)(v >> 4);  // interior index of the cell right of the face
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      if (!(v & (side == 0 ? 4ULL : 8ULL))) continue;
+      const long long c = side == 0 ? rr : rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
+      const int I = (int)(c % kp.Nxp), J = (int)((c / kp.Nxp) % kp.Nyp), K = (int)(c / ((long long)kp.Nxp * kp.Nyp));
+      long long sl[6];
+#pragma unroll
+      for (int o = 0; o < 6; ++o) {
+        const int d = o >> 1, r = o & 1;
+        const unsigned long long e =
+            hi.idx[d * nb + hface(kp, I + (d == 0 ? r : 0), J + (d == 1 ? r : 0), K + (d == 2 ? r : 0))];
+        sl[o] = (e & ~kHSlotMask) == hi.stamp ? (long long)(e & kHSlotMask) : -1;
+      }
+      const int mine = 2 * D + side;  // this face's place in the cell's order
+      bool owner = true;
+#pragma unroll
+      for (int o = 0; o < 6; ++o)
+        if (o < mine && sl[o] >= 0) owner = false;
+      if (!owner) continue;
+      T acc[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc[q] = R[c + q * rn];
+#pragma unroll
+      for (int o = 0; o < 6; ++o) {
+        if (sl[o] < 0) continue;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc[q] = (o & 1) ? acc[q] - hflux[sl[o] * 5 + q] : acc[q] + hflux[sl[o] * 5 + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 5; ++q) R[c + q * rn] = acc[q];
+    }
+  });
+}
+
+// Clears n records (stamp wrap; the no-allocation entry's scratch).
+void clear_hidx(unsigned long long* idx, long long n, hipStream_t s);
+
+// A new stamp for one heavy-face evaluation (the records array of hbox(kp) x 3 entries is cleared when the
+// stamp wraps).
+inline HIdx next_hidx(const KP& kp, unsigned long long* idx, unsigned int* stamp, hipStream_t s) {
+  if (++*stamp >= (1u << kHStampBits)) {
+    clear_hidx(idx, 3 * hbox(kp), s);
+    *stamp = 1;
   }
+  return HIdx{idx, (unsigned long long)*stamp << (64 - kHStampBits)};
+}
+
+// Host side: the gather pass over the runs of all three directions.
+template <typename T>
+inline void launch_heavy_gather(const KP& kp, T* R, const HList& hl, const unsigned int* counts,
+                                const unsigned long long* list, const T* hflux, const HIdx& hi, hipStream_t s) {
+  const unsigned int it = hl.items();
+  const unsigned int nb = it < 8192u ? (it ? it : 1u) : 8192u;
+  hipLaunchKernelGGL((k_heavy_gather<T>), dim3(nb), dim3(64), 0, s, kp, R, hl, counts, list, hflux, hi);
 }
 
 // Linear id of the calling wave in a launch of blocks of `wpb` waves (wave index wib in its block).

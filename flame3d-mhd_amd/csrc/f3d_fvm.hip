@@ -16,8 +16,8 @@
 //   k_fvm_march      y and z faces: lanes march along the direction with the last 8 cells' U in a
 //                    register window, next cell and next face geometry prefetched; R += dF_y, dF_z
 //   k_fvm_heavy      the listed faces: full characteristic / component-wise reconstruction + HLLC,
-//                    kept in the face's list slot, then added to the two cells by the ordered
-//                    k_heavy_apply passes (no floating-point atomics)
+//                    kept in the face's list slot, then added to the two cells by k_heavy_gather
+//                    (fixed order per cell, no floating-point atomics)
 //   fused_update     viscous divergence + update + RK3 blend + c2Prim (f3d_fused.hip)
 #include <cstdlib>
 #include <cstring>
@@ -285,13 +285,12 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
 
 // ------------------------------------------------------------------ heavy faces
 // Every listed face (one launch, all directions): the full characteristic / component-wise
-// reconstruction + HLLC flux (the sweep gave these faces zero), kept in the face's list slot; the x faces
-// also add it to the cell right of them here.  The other five contributions follow in the ordered passes of
-// k_heavy_apply (no floating-point atomics: the same summation order on every evaluation).
+// reconstruction + HLLC flux (the sweep gave these faces zero), kept in the face's list slot and recorded
+// in the per-face index; k_heavy_gather adds each cell's fluxes in a fixed order (no floating-point
+// atomics: the same summation order on every evaluation).
 template <typename T, bool EIGEN>
 __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* __restrict__ Q, MetricPtrs<T> m,
-                                                  HList hl, T* __restrict__ hflux) {
-  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+                                                  HList hl, T* __restrict__ hflux, HIdx hi) {
   for_each_heavy_slot(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);  // right cell of the face
@@ -310,10 +309,8 @@ __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* 
     const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
     const long long rr = rix(kp, I, Jj, K);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      hflux[slot * 5 + q] = F[q];
-      if (D == 0 && f <= Np - 1) a.R[rr + q * rn] += F[q];  // left face of cell f
-    }
+    for (int q = 0; q < 5; ++q) hflux[slot * 5 + q] = F[q];
+    heavy_record(kp, hi, D, I, Jj, K, slot);
     a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
   });
 }
@@ -379,11 +376,12 @@ int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, con
   {
     TimedScope ts("fvm_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
+    const HIdx hi = next_hidx(kp, ws.hidx, ws.hstamp, s);
     if (kp.fvm == F3D_FVM_EIGEN)
-      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux);
+      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hi);
     else
-      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux);
-    launch_heavy_apply<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, s);
+      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hi);
+    launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s);
   }
   return check_launch("fvm sweeps");
 }

@@ -27,6 +27,8 @@ struct f3d_workspace {
   double* ldt;    // per-cell dt of local time stepping (cell-shaped; compute_dt at stage 1)
   double* fsc;    // sensor of the nonlinear filter (cell-shaped)
   void* hflux;    // heavy-face flux per list slot (bulk type, 5 per slot)
+  unsigned long long* hidx;  // per-face records of the listed faces (3 x hbox)
+  unsigned int hstamp;       // stamp of the last heavy-face evaluation
 };
 
 namespace {
@@ -450,7 +452,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
   const size_t nc = f3d_cell_count(p);
   const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
-  size_t sz[19];
+  size_t sz[20];
   sz[0] = nc * es;                               // phi
   sz[1] = nc * 5 * es;                           // Fp
   sz[2] = nc * 5 * es;                           // Fm
@@ -477,6 +479,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[16] = nc * sizeof(double);                  // ldt
   sz[17] = nc * sizeof(double);                  // fsc
   sz[18] = hslots * 5 * es;                      // heavy-face flux per slot
+  sz[19] = 3 * (size_t)hbox(kp0) * sizeof(unsigned long long);  // per-face records of the listed faces
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -487,14 +490,15 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   ws->bytes = total;
   ws->base = base;
   char* c = (char*)base;
-  void** slots[19] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+  void** slots[20] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
                       &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount,
-                      &ws->Qa, &ws->Qb, (void**)&ws->ldt, (void**)&ws->fsc, &ws->hflux};
-  for (int i = 0; i < 19; ++i) {
+                      &ws->Qa, &ws->Qb, (void**)&ws->ldt, (void**)&ws->fsc, &ws->hflux, (void**)&ws->hidx};
+  for (int i = 0; i < 20; ++i) {
     *slots[i] = c;
     c += align_up(sz[i]);
   }
   ws->nred = kRedBlocks;
+  ws->hstamp = 0;  // the records start cleared (zero fill below): stamps count from 1
   ws->hcap = hslots;
   ws->hncap = hruns;
   // zeroed once: inviscid runs read zero viscous fluxes in div (div.jl:45-48 adds them always)
@@ -538,6 +542,8 @@ WsView<T> ws_view(f3d_workspace* ws) {
   v.hncap = ws->hncap;
   v.ldt = ws->ldt;
   v.hflux = (T*)ws->hflux;
+  v.hidx = ws->hidx;
+  v.hstamp = &ws->hstamp;
   return v;
 }
 template WsView<double> ws_view<double>(f3d_workspace*);
@@ -565,11 +571,13 @@ F3D_API int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, cons
 // accumulator, the heavy-face runs and their counts, and phi when the caller passes none) carved from
 // one file-scope device array instead of a workspace.  Each GPU the library runs on holds its own copy
 // of the array (kNoAllocScratchBytes) until the process ends; its address is looked up once per device.
-// Nothing in it needs clearing between calls: the x sweep writes every accumulator entry before the
-// y/z marches add to it, each sweep wave writes its run's count, the heavy faces' first pass writes the
-// flux slots their second pass reads, and the sensor writes phi.
+// Only the per-face records of the heavy faces are cleared on every call (by a fill kernel, since the
+// previous call may have laid the array out for another grid); everything else is written before it is
+// read: the x+y march (or the x sweep) writes every accumulator entry before the z march adds to it, each
+// sweep wave writes its run's count, the heavy faces' forming pass writes the flux slots the gather pass
+// reads, and the sensor writes phi.
 namespace {
-constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
 constexpr int kMaxDevices = 64;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -604,8 +612,9 @@ int scratch_address(void** out, const char* who) {
 
 // Scratch layout of the no-allocation RHS: offsets of R, the heavy-face runs, their counts and phi.
 struct NoAllocLayout {
-  size_t R, heavy, hcount, hflux, phi, total;
+  size_t R, heavy, hcount, hflux, hidx, phi, total;
   unsigned long long hcap, hncap;
+  long long nidx;  // records of the listed faces
 };
 NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   const KP kp = make_kp(p);
@@ -622,6 +631,9 @@ NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   off += align_up(L.hncap * sizeof(unsigned int));
   L.hflux = off;
   off += align_up(L.hcap * 5 * sizeof(double));
+  L.hidx = off;
+  L.nidx = 3 * hbox(kp);
+  off += align_up((size_t)L.nidx * sizeof(unsigned long long));
   L.phi = off;
   if (with_phi) off += align_up(f3d_cell_count(p) * sizeof(double));
   L.total = off;
@@ -668,7 +680,11 @@ F3D_API int f3d_rhs_noalloc_f64(const f3d_params* p, double* R, const double* Q,
   v.hcap = L.hcap;
   v.hncap = L.hncap;
   v.hflux = (double*)(c + L.hflux);
+  v.hidx = (unsigned long long*)(c + L.hidx);
+  unsigned int stamp = 0;  // records cleared below: this call's faces take stamp 1
+  v.hstamp = &stamp;
   v.phi = phi ? phi : (double*)(c + L.phi);
+  clear_hidx(v.hidx, L.nidx, (hipStream_t)s);
   MetricPtrs<double> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
   return fused_rhs<double>(kp, v, Q, m, J, v.phi, R, (hipStream_t)s);
