@@ -454,7 +454,7 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
 // ------------------------------------------------------------------ heavy faces
 // Every listed face (all three directions, one launch): its characteristic WENO7/WENO5/NND flux
 // (schemes.jl:756-1009) minus the UP7 flux the sweep accumulated, kept in the face's list slot and recorded
-// in the per-face index; k_heavy_gather then adds each cell's corrections in a fixed order, so no
+// in its cells' records; k_heavy_gather then adds each cell's corrections in a fixed order, so no
 // floating-point atomics: the corrections are summed the same way on every evaluation.
 template <typename T, int SPLIT, bool CHAR>
 __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux, HIdx hi) {
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __
     const long long rr = ridx(kp, I, Jj, K);
 #pragma unroll
     for (int q = 0; q < 5; ++q) hflux[slot * 5 + q] = Fs[q] - Fu[q];
-    heavy_record(kp, hi, D, I, Jj, K, slot);
+    heavy_record(kp, hi, D, rr, f >= 1, f <= Np - 1, slot);
     a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
   });
 }

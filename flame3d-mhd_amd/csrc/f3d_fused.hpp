@@ -18,7 +18,7 @@ struct WsView {
   unsigned long long hncap;  // capacity of `hcount` (runs)
   double* ldt;               // per-cell dt of local time stepping, cell-shaped (written at stage 1)
   T* hflux;                  // per list slot: the 5 flux values a heavy face adds to its cells (hcap x 5)
-  unsigned long long* hidx;  // per-face records of the listed faces (3 x hbox, see HIdx)
+  unsigned long long* hidx;  // per-cell records of the listed faces (hrec_count, see HIdx)
   unsigned int* hstamp;      // host counter of the records' stamps
 };
 
@@ -161,26 +161,28 @@ __device__ __forceinline__ unsigned long long heavy_apply_entry(long long rr, bo
          (unsigned long long)D;
 }
 
-// Per-face record of the listed faces, for the gather pass: hidx[d * hbox + I + (Nxp+1) * (J + (Nyp+1) * K)]
-// holds (stamp << 36) | list slot for the face left of interior cell (I, J, K) along d (the last face of a
-// line at index N_d).  Every heavy-face evaluation takes a new stamp, so the records of earlier ones need
-// no clearing (the array is cleared when the 28-bit stamp wraps).
+// Per-cell record of the listed faces, for the gather pass: hrec[6 c + o] (c = interior index of the cell,
+// o = 2 d + side: its face along d on the left (side 0) or right (side 1)) holds (stamp << 36) | list slot.
+// A face writes the records of its two cells (distinct faces, distinct records: no conflicts).  Every
+// heavy-face evaluation takes a new stamp, so the records of earlier ones need no clearing (the array is
+// cleared when the 28-bit stamp wraps).
 constexpr int kHStampBits = 28;
 constexpr unsigned long long kHSlotMask = (1ULL << (64 - kHStampBits)) - 1;
 struct HIdx {
-  unsigned long long* idx;
+  unsigned long long* rec;
   unsigned long long stamp;  // stamp << (64 - kHStampBits)
 };
-__host__ __device__ inline long long hbox(const KP& kp) {
-  return (long long)(kp.Nxp + 1) * (kp.Nyp + 1) * (kp.Nzp + 1);
-}
-__device__ __forceinline__ long long hface(const KP& kp, int I, int J, int K) {
-  return I + (long long)(kp.Nxp + 1) * (J + (long long)(kp.Nyp + 1) * K);
-}
-// Records the listed face of direction D left of interior cell (I, J, K) as held in `slot`.
-__device__ __forceinline__ void heavy_record(const KP& kp, const HIdx& hi, int D, int I, int J, int K,
+__host__ __device__ inline long long hrec_count(const KP& kp) { return 6LL * kp.Nxp * kp.Nyp * kp.Nzp; }
+// Records the listed face of direction D held in `slot`: the left face of interior cell rr (when the face has a
+// cell on its right) and the right face of the cell before it along D (when it has one on its left).
+__device__ __forceinline__ void heavy_record(const KP& kp, const HIdx& hi, int D, long long rr, bool left, bool right,
                                              unsigned long long slot) {
-  hi.idx[D * hbox(kp) + hface(kp, I, J, K)] = hi.stamp | slot;
+  const unsigned long long e = hi.stamp | slot;
+  if (right) hi.rec[6 * rr + 2 * D] = e;
+  if (left) {
+    const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
+    hi.rec[6 * rl + 2 * D + 1] = e;
+  }
 }
 
 // The correction kept per list slot added to the RHS, one cell at a time: each listed face looks at its
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(64) void k_heavy_gather(KP kp, T* __restrict__ R, H
                                                      const unsigned int* __restrict__ counts,
                                                      const unsigned long long* __restrict__ list,
                                                      const T* __restrict__ hflux, HIdx hi) {
-  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp, nb = hbox(kp);
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
   for_each_heavy_slot(hl, counts, list, [&](const unsigned long long v, unsigned long long) {
     const int D = (int)(v & 3);
     const long long rr = (long long)(v >> 4);  // interior index of the cell right of the face
@@ -201,15 +203,12 @@ __global__ __launch_bounds__(64) void k_heavy_gather(KP kp, T* __restrict__ R, H
     for (int side = 0; side < 2; ++side) {
       if (!(v & (side == 0 ? 4ULL : 8ULL))) continue;
       const long long c = side == 0 ? rr : rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
-      const int I = (int)(c % kp.Nxp), J = (int)((c / kp.Nxp) % kp.Nyp), K = (int)(c / ((long long)kp.Nxp * kp.Nyp));
+      const ulonglong2* r2 = reinterpret_cast<const ulonglong2*>(hi.rec + 6 * c);  // 48 B per cell, 16-B aligned
+      const ulonglong2 e01 = r2[0], e23 = r2[1], e45 = r2[2];
+      const unsigned long long e[6] = {e01.x, e01.y, e23.x, e23.y, e45.x, e45.y};
       long long sl[6];
 #pragma unroll
-      for (int o = 0; o < 6; ++o) {
-        const int d = o >> 1, r = o & 1;
-        const unsigned long long e =
-            hi.idx[d * nb + hface(kp, I + (d == 0 ? r : 0), J + (d == 1 ? r : 0), K + (d == 2 ? r : 0))];
-        sl[o] = (e & ~kHSlotMask) == hi.stamp ? (long long)(e & kHSlotMask) : -1;
-      }
+      for (int o = 0; o < 6; ++o) sl[o] = (e[o] & ~kHSlotMask) == hi.stamp ? (long long)(e[o] & kHSlotMask) : -1;
       const int mine = 2 * D + side;  // this face's place in the cell's order
       bool owner = true;
 #pragma unroll
@@ -234,14 +233,13 @@ __global__ __launch_bounds__(64) void k_heavy_gather(KP kp, T* __restrict__ R, H
 // Clears n records (stamp wrap; the no-allocation entry's scratch).
 void clear_hidx(unsigned long long* idx, long long n, hipStream_t s);
 
-// A new stamp for one heavy-face evaluation (the records array of hbox(kp) x 3 entries is cleared when the
-// stamp wraps).
-inline HIdx next_hidx(const KP& kp, unsigned long long* idx, unsigned int* stamp, hipStream_t s) {
+// A new stamp for one heavy-face evaluation (the hrec_count(kp) records are cleared when the stamp wraps).
+inline HIdx next_hidx(const KP& kp, unsigned long long* rec, unsigned int* stamp, hipStream_t s) {
   if (++*stamp >= (1u << kHStampBits)) {
-    clear_hidx(idx, 3 * hbox(kp), s);
+    clear_hidx(rec, hrec_count(kp), s);
     *stamp = 1;
   }
-  return HIdx{idx, (unsigned long long)*stamp << (64 - kHStampBits)};
+  return HIdx{rec, (unsigned long long)*stamp << (64 - kHStampBits)};
 }
 
 // Host side: the gather pass over the runs of all three directions.

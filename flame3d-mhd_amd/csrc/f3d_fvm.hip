@@ -286,7 +286,7 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
 // ------------------------------------------------------------------ heavy faces
 // Every listed face (one launch, all directions): the full characteristic / component-wise
 // reconstruction + HLLC flux (the sweep gave these faces zero), kept in the face's list slot and recorded
-// in the per-face index; k_heavy_gather adds each cell's fluxes in a fixed order (no floating-point
+// in its cells' records; k_heavy_gather adds each cell's fluxes in a fixed order (no floating-point
 // atomics: the same summation order on every evaluation).
 template <typename T, bool EIGEN>
 __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* __restrict__ Q, MetricPtrs<T> m,
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* 
     const long long rr = rix(kp, I, Jj, K);
 #pragma unroll
     for (int q = 0; q < 5; ++q) hflux[slot * 5 + q] = F[q];
-    heavy_record(kp, hi, D, I, Jj, K, slot);
+    heavy_record(kp, hi, D, rr, f >= 1, f <= Np - 1, slot);
     a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
   });
 }

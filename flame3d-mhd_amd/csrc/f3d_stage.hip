@@ -27,7 +27,7 @@ struct f3d_workspace {
   double* ldt;    // per-cell dt of local time stepping (cell-shaped; compute_dt at stage 1)
   double* fsc;    // sensor of the nonlinear filter (cell-shaped)
   void* hflux;    // heavy-face flux per list slot (bulk type, 5 per slot)
-  unsigned long long* hidx;  // per-face records of the listed faces (3 x hbox)
+  unsigned long long* hidx;  // per-cell records of the listed faces (6 per interior cell)
   unsigned int hstamp;       // stamp of the last heavy-face evaluation
 };
 
@@ -479,7 +479,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[16] = nc * sizeof(double);                  // ldt
   sz[17] = nc * sizeof(double);                  // fsc
   sz[18] = hslots * 5 * es;                      // heavy-face flux per slot
-  sz[19] = 3 * (size_t)hbox(kp0) * sizeof(unsigned long long);  // per-face records of the listed faces
+  sz[19] = (size_t)hrec_count(kp0) * sizeof(unsigned long long);  // per-cell records of the listed faces
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -571,7 +571,7 @@ F3D_API int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, cons
 // accumulator, the heavy-face runs and their counts, and phi when the caller passes none) carved from
 // one file-scope device array instead of a workspace.  Each GPU the library runs on holds its own copy
 // of the array (kNoAllocScratchBytes) until the process ends; its address is looked up once per device.
-// Only the per-face records of the heavy faces are cleared on every call (by a fill kernel, since the
+// Only the per-cell records of the heavy faces are cleared on every call (by a fill kernel, since the
 // previous call may have laid the array out for another grid); everything else is written before it is
 // read: the x+y march (or the x sweep) writes every accumulator entry before the z march adds to it, each
 // sweep wave writes its run's count, the heavy faces' forming pass writes the flux slots the gather pass
@@ -632,7 +632,7 @@ NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   L.hflux = off;
   off += align_up(L.hcap * 5 * sizeof(double));
   L.hidx = off;
-  L.nidx = 3 * hbox(kp);
+  L.nidx = hrec_count(kp);
   off += align_up((size_t)L.nidx * sizeof(unsigned long long));
   L.phi = off;
   if (with_phi) off += align_up(f3d_cell_count(p) * sizeof(double));
