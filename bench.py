@@ -127,20 +127,20 @@ def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3, geo=Non
     a fifth value).  Returns (ms/step, host wall ms/step, kernel table,
     (dominant name, its total ms, its launches)[, captured U])."""
     U, Un, Q, m, J, inflow, dt = state
+    # n steps in one call, as the reference's time loop runs them (solver.jl:410-444)
     if geo is None:
-        step = lambda: ctx.time_step(1, U, Un, Q, m, J, dt, inflow)
+        steps_call = lambda n: ctx.time_step(n, U, Un, Q, m, J, dt, inflow)
     else:
-        step = lambda: ctx.time_step_fvm(1, U, Un, Q, m, J, geo, dt, inflow)
-    for _ in range(warmup):
-        step()
+        steps_call = lambda n: ctx.time_step_fvm(n, U, Un, Q, m, J, geo, dt, inflow)
+    if warmup:
+        steps_call(warmup)
     torch.cuda.synchronize()
     table, dom = {}, None
     if lib is not None and table_steps:
         lib.f3d_timing_only(None)
         lib.f3d_timing_reset()
         lib.f3d_timing_enable(1)
-        for _ in range(table_steps):
-            step()
+        steps_call(table_steps)
         torch.cuda.synchronize()
         lib.f3d_timing_enable(0)
         table = kernel_table(lib)
@@ -157,8 +157,8 @@ def run_steps(torch, ctx, state, steps, warmup, lib=None, table_steps=3, geo=Non
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0.record()
-        for _ in range(n):
-            step()
+        if n:
+            steps_call(n)
         ev1.record()
         torch.cuda.synchronize()
         wall += time.perf_counter() - t0

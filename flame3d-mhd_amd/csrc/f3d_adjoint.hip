@@ -4,9 +4,11 @@
 // RHS VJP, lambda (interior, U layout) -> lamQ = (dR/dQ)^T lambda over every cell of Q:
 //   k_adj_lambda     Lam = scale * J * lambda on interior cells (ghost entries stay zero)
 //   k_adj_heavy_list faces whose 6-cell sensor maximum selects WENO7/WENO5/NND, compacted
-//   k_adj_inviscid   every face taken as UP7: the transposed difference-of-UP7 stencil gathers
-//                    dFp, dFm per cell and direction from Lam, then the reverse of the
-//                    Steger-Warming split gives (rho,u,v,w,T) adjoints; lamQ written for all cells
+//   k_adj_inv_z, k_adj_inv_xy  every face taken as UP7: the transposed difference-of-UP7 stencil
+//                    gathers dFp, dFm per cell and direction from Lam (register windows along the
+//                    marches, an LDS row for x), then the reverse of the Steger-Warming split gives
+//                    (rho,u,v,w,T) adjoints; lamQ written for all cells (k_adj_inviscid: the same in
+//                    one kernel gathering from the cache, F3D_ADJ_INV=gather)
 //   k_adj_heavy      per listed face: (scheme - UP7) reverse through the characteristic WENO /
 //                    NND reconstruction, the frame and the eight splits, atomically added
 //   k_adj_visc1      per cell of the interior+-2 box: transposed viscous divergence -> adjoint of
@@ -144,6 +146,143 @@ __global__ __launch_bounds__(256) void k_adj_inviscid(KP kp, const double* __res
   lamQ[o + 3 * nc] = acc[3];
   lamQ[o + 4 * nc] = 0.0;
   lamQ[o + 5 * nc] = acc[4];
+}
+
+// The same VJP as two marches (a direction's share of a cell needs Lam at the 8 cells around it along that
+// direction, which k_adj_inviscid gathers from the cache per cell and direction):
+//   k_adj_inv_z  lane = (i, j) column, marching k with Lam of the column in a 9-plane register window;
+//                writes lamQ of every cell: the z share where i and j are interior, else zero;
+//   k_adj_inv_xy lane = i in a 64-cell window (4 halo cells each side, 56 cells written), threadIdx.y = an
+//                interior k line, marching j with Lam of the column in a 9-row register window; the row's Lam
+//                is exchanged through LDS for the x share; adds the x share (interior j) and the y share
+//                (interior i) to lamQ of the cells with interior k.
+// Each Lam value is read from HBM once per march.  Same sums, in the order z, x, y per cell.
+constexpr int kAIL = 2;   // k lines per block of the x+y march
+constexpr int kAIW = 64;  // lanes of a row window
+constexpr int kAII = kAIW - 8;
+
+// dFp, dFm of a cell from Lam at the nine cells around it along one direction: LW(r, n) = Lam of variable n
+// at offset r - 4 (the sums of k_adj_inviscid, in its order).
+template <class F>
+__device__ __forceinline__ void adj_gather(const KP& kp, const F& LW, double* gp, double* gm) {
+#pragma unroll
+  for (int n = 0; n < 5; ++n) gp[n] = gm[n] = 0;
+#pragma unroll
+  for (int r = -4; r <= 4; ++r) {
+    const double ca = r >= -3 ? kp.da[4 - r] : 0.0;
+    const double cb = r <= 3 ? kp.db[3 - r] : 0.0;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      const double l = LW(r + 4, n);
+      gp[n] += ca * l;
+      gm[n] += cb * l;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_inv_z(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
+                                                   const double* __restrict__ Lam, double* __restrict__ lamQ, int L) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= kp.Nxt || j >= kp.Nyt) return;
+  const int NG = kp.NG, k0 = blockIdx.z * L, k1 = min(k0 + L, kp.Nzt);
+  const long long nc = kp.nc, sz = (long long)kp.Nxt * kp.Nyt, col = cidx(kp, i, j, 0);
+  const bool zcol = interior(i, kp.Nxp, NG) && interior(j, kp.Nyp, NG);
+  double Lw[9][5];
+  auto load = [&](int k, double* d) {
+    const bool in = zcol && k >= 0 && k < kp.Nzt;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) d[n] = in ? Lam[col + (long long)k * sz + n * nc] : 0.0;
+  };
+#pragma unroll
+  for (int r = 0; r < 8; ++r) load(k0 - 4 + r, Lw[r]);
+#pragma unroll 1
+  for (int k = k0; k < k1; ++k) {
+    load(k + 4, Lw[8]);
+    const long long o = col + (long long)k * sz;
+    double g[5] = {0, 0, 0, 0, 0};
+    if (zcol) {
+      double gp[5], gm[5];
+      adj_gather(kp, [&](int r, int n) { return Lw[r][n]; }, gp, gm);
+      const double A1 = mp.m[6][o], A2 = mp.m[7][o], A3 = mp.m[8][o];
+      sw_split_vjp(kp, Q[o], Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], A1, A2, A3,
+                   sqrt(A1 * A1 + A2 * A2 + A3 * A3), gp, gm, g);
+    }
+    lamQ[o] = g[0];
+    lamQ[o + nc] = g[1];
+    lamQ[o + 2 * nc] = g[2];
+    lamQ[o + 3 * nc] = g[3];
+    lamQ[o + 4 * nc] = 0.0;
+    lamQ[o + 5 * nc] = g[4];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) Lw[r][n] = Lw[r + 1][n];
+  }
+}
+
+__global__ __launch_bounds__(kAIW * kAIL) void k_adj_inv_xy(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
+                                                            const double* __restrict__ Lam, double* __restrict__ lamQ,
+                                                            int L) {
+  __shared__ double sL[5][kAIL][kAIW];
+  const int t = threadIdx.x, ln = threadIdx.y, NG = kp.NG;
+  const int i = blockIdx.x * kAII - 4 + t;            // the lane's cell (may lie outside [0, Nxt))
+  const bool iok = i >= 0 && i < kp.Nxt;
+  const bool out = t >= 4 && t < 4 + kAII && iok;     // lamQ of this lane's cells is updated
+  const bool iin = interior(i, kp.Nxp, NG);
+  const int kk = blockIdx.y * kAIL + ln;
+  const bool kok = kk < kp.Nzp;
+  const int k = NG + (kok ? kk : kp.Nzp - 1);
+  const int j0 = blockIdx.z * L, j1 = min(j0 + L, kp.Nyt);
+  const long long nc = kp.nc, sy = kp.Nxt;
+  const long long row0 = cidx(kp, iok ? i : 0, 0, k);
+  double Lw[9][5];
+  // Lam of row j of this lane's column (zero outside the array: Lam is zero on ghost cells anyway)
+  auto load = [&](int j, double* d) {
+    const bool in = iok && kok && j >= 0 && j < kp.Nyt;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) d[n] = in ? Lam[row0 + (long long)j * sy + n * nc] : 0.0;
+  };
+#pragma unroll
+  for (int r = 0; r < 8; ++r) load(j0 - 4 + r, Lw[r]);
+#pragma unroll 1
+  for (int j = j0; j < j1; ++j) {
+    load(j + 4, Lw[8]);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) sL[n][ln][t] = Lw[4][n];
+    __syncthreads();
+    const bool jin = interior(j, kp.Nyp, NG);
+    if (out && kok && (jin || iin)) {
+      const long long o = row0 + (long long)j * sy;
+      const double rho = Q[o], u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc], Tt = Q[o + 5 * nc];
+      double acc[5] = {lamQ[o], lamQ[o + nc], lamQ[o + 2 * nc], lamQ[o + 3 * nc], lamQ[o + 5 * nc]};
+      if (jin) {  // x share: Lam of the row from the neighbouring lanes
+        double gp[5], gm[5], g[5];
+        adj_gather(kp, [&](int r, int n) { return sL[n][ln][t - 4 + r]; }, gp, gm);
+        const double A1 = mp.m[0][o], A2 = mp.m[1][o], A3 = mp.m[2][o];
+        sw_split_vjp(kp, rho, u, v, w, Tt, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), gp, gm, g);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) acc[n] += g[n];
+      }
+      if (iin) {  // y share from the column window
+        double gp[5], gm[5], g[5];
+        adj_gather(kp, [&](int r, int n) { return Lw[r][n]; }, gp, gm);
+        const double A1 = mp.m[3][o], A2 = mp.m[4][o], A3 = mp.m[5][o];
+        sw_split_vjp(kp, rho, u, v, w, Tt, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), gp, gm, g);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) acc[n] += g[n];
+      }
+      lamQ[o] = acc[0];
+      lamQ[o + nc] = acc[1];
+      lamQ[o + 2 * nc] = acc[2];
+      lamQ[o + 3 * nc] = acc[3];
+      lamQ[o + 5 * nc] = acc[4];
+    }
+    __syncthreads();  // the row in LDS is read before the next row overwrites it
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) Lw[r][n] = Lw[r + 1][n];
+  }
 }
 
 // ------------------------------------------------------------------ inviscid, listed faces
@@ -897,6 +1036,21 @@ int adj_visc_kc(const KP& kp) {
 }
 
 // One lane per cell of the (Nxp+1) x (Nyp+1) x (Nzp+1) face-index box (its x, y and z faces).
+// 1 (default): the inviscid VJP as the z and x+y marches; F3D_ADJ_INV=gather: k_adj_inviscid (A/B).
+int adj_inv_march() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("F3D_ADJ_INV");
+    v = (e && std::strcmp(e, "gather") == 0) ? 0 : 1;
+  }
+  return v;
+}
+
+int seg_len_adj(int N, int max_len) {
+  const int nseg = (N + max_len - 1) / max_len;
+  return (N + nseg - 1) / nseg;
+}
+
 dim3 adj_list_grid(const KP& kp) { return dim3((kp.Nxp + 1 + 63) / 64, (kp.Nyp + 1 + 3) / 4, kp.Nzp + 1); }
 
 }  // namespace
@@ -932,7 +1086,19 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
     TimedScope ts("adj_lambda", s);
     hipLaunchKernelGGL(k_adj_lambda, grid_interior(kp), kB, 0, s, kp, lam, J, scale, aws->Lam);
   }
-  {
+  if (adj_inv_march()) {
+    TimedScope ts("adj_inviscid", s);
+    // segments of at most 24 cells (measured on the 256x128x64 ramp: z / y segments of 24 / 24 take 228 us,
+    // 32 / 32 246 us, 72 / 68 463 us: more, shorter columns fill the chip); F3D_ADJ_INV_LZ / _LY override
+    const char* ez = std::getenv("F3D_ADJ_INV_LZ");
+    const char* ey = std::getenv("F3D_ADJ_INV_LY");
+    const int Lz = seg_len_adj(kp.Nzt, ez && std::atoi(ez) >= 8 ? std::atoi(ez) : 24);
+    const int Ly = seg_len_adj(kp.Nyt, ey && std::atoi(ey) >= 8 ? std::atoi(ey) : 24);
+    hipLaunchKernelGGL(k_adj_inv_z, dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4, (kp.Nzt + Lz - 1) / Lz), kB, 0, s, kp,
+                       Q, mp, aws->Lam, lamQ, Lz);
+    hipLaunchKernelGGL(k_adj_inv_xy, dim3((kp.Nxt + kAII - 1) / kAII, (kp.Nzp + kAIL - 1) / kAIL, (kp.Nyt + Ly - 1) / Ly),
+                       dim3(kAIW, kAIL, 1), 0, s, kp, Q, mp, aws->Lam, lamQ, Ly);
+  } else {
     TimedScope ts("adj_inviscid", s);
     hipLaunchKernelGGL(k_adj_inviscid, grid_all(kp), kB, 0, s, kp, Q, mp, aws->Lam, lamQ);
   }
