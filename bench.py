@@ -70,6 +70,8 @@ FUSED_BYTES = {
     "fused_sensor": lambda p, es: p.tot[0] * p.tot[1] * p.tot[2] * 2 * es,
     # rho,u,v,w,T + 3 metrics + phi in, 5 RHS values out, per interior cell
     "fused_sweep_x": lambda p, es: _nint(p) * (9 * es + 5 * es),
+    # x and y faces in one march: rho,u,v,w,T + 6 metrics + phi in, 5 RHS values out, per interior cell
+    "fused_sweep_xy": lambda p, es: _nint(p) * (12 * es + 5 * es),
     # as x, but the RHS is read and written
     "fused_sweep_y": lambda p, es: _nint(p) * (9 * es + 10 * es),
     "fused_sweep_z": lambda p, es: _nint(p) * (9 * es + 10 * es),
