@@ -253,8 +253,10 @@ def test_flow_advance_and_rhs(torch_mod, case):
     check_f32b(f"{case.name} rhs", host(Rg), R, R64, sc)
 
 
-@pytest.mark.parametrize("nsteps", [1, 2])
+@pytest.mark.parametrize("nsteps", [1, 2, 3])
 def test_time_step(torch_mod, case, nsteps):
+    """nsteps RK3 steps in one call against the oracle's time_step: U and Q, and Un, which holds the last
+    step's U^n on return as in the reference's loop (the library stages the other steps through Un)."""
     torch = torch_mod
     p, o, ctx = case.params, Oracle(case.params), f3d.Flame3D(case.params)
     U = case.U()
@@ -285,10 +287,13 @@ def test_time_step(torch_mod, case, nsteps):
     sens_q = relerr(Qp, Qo)
     e = relerr(host(Ug)[inner], Uo[inner])
     eq = relerr(host(Qg), Qo)
+    eun = relerr(host(Ung)[inner], Uno[inner])
+    sens_un = relerr(Unp[inner], Uno[inner])
     print(f"{case.name} {case.bulk} {nsteps} steps: U err {e:.2e} (oracle sensitivity {sens_u:.2e}), "
-          f"Q err {eq:.2e} (sensitivity {sens_q:.2e})")
+          f"Q err {eq:.2e} (sensitivity {sens_q:.2e}), Un err {eun:.2e} (sensitivity {sens_un:.2e})")
     assert e <= 10 * sens_u + floor, f"U after {nsteps} steps: {e}"
     assert eq <= 10 * sens_q + floor, f"Q after {nsteps} steps: {eq}"
+    assert eun <= 10 * sens_un + floor, f"Un after {nsteps} steps: {eun}"
 
 
 def test_cfl_and_nan(torch_mod, case):
