@@ -9,14 +9,18 @@
 //                    marches, an LDS row for x), then the reverse of the Steger-Warming split gives
 //                    (rho,u,v,w,T) adjoints; lamQ written for all cells (k_adj_inviscid: the same in
 //                    one kernel gathering from the cache, F3D_ADJ_INV=gather)
-//   k_adj_heavy      per listed face: (scheme - UP7) reverse through the characteristic WENO /
-//                    NND reconstruction, the frame and the eight splits, atomically added
+//   k_adj_heavy_form per listed face: (scheme - UP7) reverse through the characteristic WENO /
+//                    NND reconstruction, the frame and the eight splits, kept per face (its stencil
+//                    cells flagged); the dense inviscid kernels add each flagged cell's kept
+//                    contributions in a fixed order (no floating-point atomics); k_adj_heavy_rest:
+//                    faces beyond the buffer, atomically
 //   k_adj_visc1      per cell of the interior+-2 box: transposed viscous divergence -> adjoint of
 //                    the cell's viscous fluxes -> adjoint of its 12 derivatives and (u,v,w,T)
 //   k_adj_visc2      transposed 4th-order derivative stencils gather those into lamQ
 // Stage VJP: RHS VJP with scale a*dt, reverse of the ghost procedure (fill z,y,x then exchange
 // z,y,x; ghost adjoints fold into their sources, Dirichlet / inflow / isothermal-wall values
 // into parameter gradients) and k_adj_finalize (reverse of c2Prim + the RK3 blend).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -42,6 +46,17 @@ struct f3d_adj_workspace {
   double* Qcur;        // forward sweep primitive state (6 vars)
   double* red;         // reduction partials
   int nred;
+  // listed faces' reverse contributions, gathered by the dense inviscid kernels in a fixed order per cell
+  // (no floating-point atomics): hbuf[40 c + 5 s + q] = contribution of the face with compact index c to
+  // variable q (rho, u, v, w, T) of its stencil cell s; hrec[d][o] = (stamp << 24) | (c + 1) for the face
+  // of direction d whose right cell is o; hctr counts the compact indices taken (faces beyond hcap keep
+  // the atomic path).
+  double* hbuf;
+  unsigned int* hrec;
+  unsigned char* hflag;
+  unsigned int* hctr;
+  unsigned int hcap;
+  unsigned int hstamp;
 };
 
 namespace {
@@ -74,8 +89,13 @@ __global__ __launch_bounds__(256) void k_adj_lambda(KP kp, const double* __restr
 // of each direction.
 __global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __restrict__ phi,
                                                         unsigned long long* __restrict__ list,
-                                                        unsigned int* __restrict__ count, HList hl) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+                                                        unsigned int* __restrict__ count, HList hl,
+                                                        unsigned int* __restrict__ hctr) {
+  // lanes run over the (Nxp+1) x (Nyp+1) face-index positions of plane k in order (a wave's 64 positions
+  // may span two rows), so that no wave is left mostly idle at the end of a row
+  const long long t = (long long)(blockIdx.x * 4 + threadIdx.y) * 64 + threadIdx.x;
+  const int i = (int)(t % (kp.Nxp + 1)), j = (int)(t / (kp.Nxp + 1)), k = blockIdx.z;
+  if (blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hctr = 0u;
   const unsigned int w = launch_wave(4, threadIdx.y);
 #pragma unroll
   for (int D = 0; D < 3; ++D) {
@@ -97,12 +117,44 @@ __global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __r
   }
 }
 
+// HRec: the listed faces' contributions of one RHS VJP (workspace arrays, this evaluation's stamp).
+struct HRec {
+  double* buf;
+  unsigned int* rec;     // 3 x nc, direction-major
+  unsigned char* flag;   // per cell: the stamp when some kept face's stencil covers the cell
+  unsigned int* ctr;
+  unsigned int cap, stamp;  // stamp in bits 24..31
+  __device__ bool covered(long long o) const { return flag[o] == (unsigned char)(stamp >> 24); }
+};
+
+// Compact index of the record of face (D, right cell o), or -1 when this evaluation kept none for it.
+__device__ __forceinline__ int hrec_slot(const HRec& hr, unsigned int e) {
+  return (e & 0xFF000000u) == hr.stamp ? (int)(e & 0x00FFFFFFu) - 1 : -1;
+}
+
+// Adds to g[0..4] the kept contributions of the listed faces of one direction around a cell, in the order of
+// their right cells' offsets m = -3..4 along the direction (the cell is stencil cell 4 - m of each):
+// RW(m) = the record of the face whose right cell is at offset m.
+template <class RW>
+__device__ __forceinline__ void adj_heavy_add(const HRec& hr, const RW& rw, double* g) {
+#pragma unroll 1
+  for (int m = -3; m <= 4; ++m) {
+    const int cs = hrec_slot(hr, rw(m));
+    if (cs >= 0) {
+      const double* c = hr.buf + 40ULL * (unsigned int)cs + 5 * (4 - m);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) g[q] += c[q];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ inviscid, dense part
 // lamQ(rho,u,v,w,T) of every cell from the UP7 form of every face: for direction d the RHS is
 //   R[c] = J_c * (sum_r da[r+4] Fp[c+r] + sum_r db[r+3] Fm[c+r]),
 // so dFp[e] = sum_{r=-3..4} da[4-r] Lam[e+r] and dFm[e] = sum_{r=-4..3} db[3-r] Lam[e+r].
 __global__ __launch_bounds__(256) void k_adj_inviscid(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
-                                                      const double* __restrict__ Lam, double* __restrict__ lamQ) {
+                                                      const double* __restrict__ Lam, double* __restrict__ lamQ,
+                                                      HRec hr) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= kp.Nxt || j >= kp.Nyt) return;
   const long long o = cidx(kp, i, j, k), nc = kp.nc;
@@ -138,6 +190,10 @@ __global__ __launch_bounds__(256) void k_adj_inviscid(KP kp, const double* __res
       sw_split_vjp(kp, rho, u, v, w, Tt, A1, A2, A3, S, gp, gm, g);
 #pragma unroll
       for (int n = 0; n < 5; ++n) acc[n] += g[n];
+      if (hr.covered(o)) {
+        const unsigned int* rd = hr.rec + (long long)d * nc + o;
+        adj_heavy_add(hr, [&](int m) { return a + m >= 0 && a + m < Nt ? rd[m * st] : 0u; }, acc);
+      }
     }
   }
   lamQ[o] = acc[0];
@@ -181,7 +237,8 @@ __device__ __forceinline__ void adj_gather(const KP& kp, const F& LW, double* gp
 }
 
 __global__ __launch_bounds__(256) void k_adj_inv_z(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
-                                                   const double* __restrict__ Lam, double* __restrict__ lamQ, int L) {
+                                                   const double* __restrict__ Lam, double* __restrict__ lamQ, int L,
+                                                   HRec hr) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
   if (i >= kp.Nxt || j >= kp.Nyt) return;
   const int NG = kp.NG, k0 = blockIdx.z * L, k1 = min(k0 + L, kp.Nzt);
@@ -193,12 +250,20 @@ __global__ __launch_bounds__(256) void k_adj_inv_z(KP kp, const double* __restri
 #pragma unroll
     for (int n = 0; n < 5; ++n) d[n] = in ? Lam[col + (long long)k * sz + n * nc] : 0.0;
   };
+  // records of the listed faces (read where a kept face's stencil covers the cell)
+  const unsigned int* rz = hr.rec + 2 * nc + col;
+  const unsigned int* ry = hr.rec + nc + col;
+  const unsigned int* rx = hr.rec + col;
+  const unsigned char f8 = (unsigned char)(hr.stamp >> 24);
+  unsigned char fnext = k0 < k1 ? hr.flag[col + (long long)k0 * sz] : 0;  // coverage flag, a step ahead
 #pragma unroll
   for (int r = 0; r < 8; ++r) load(k0 - 4 + r, Lw[r]);
 #pragma unroll 1
   for (int k = k0; k < k1; ++k) {
     load(k + 4, Lw[8]);
     const long long o = col + (long long)k * sz;
+    const bool covered = fnext == f8;
+    if (k + 1 < k1) fnext = hr.flag[o + sz];
     double g[5] = {0, 0, 0, 0, 0};
     if (zcol) {
       double gp[5], gm[5];
@@ -206,6 +271,14 @@ __global__ __launch_bounds__(256) void k_adj_inv_z(KP kp, const double* __restri
       const double A1 = mp.m[6][o], A2 = mp.m[7][o], A3 = mp.m[8][o];
       sw_split_vjp(kp, Q[o], Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], A1, A2, A3,
                    sqrt(A1 * A1 + A2 * A2 + A3 * A3), gp, gm, g);
+    }
+    // the listed faces' kept contributions to this cell, in the order z, x, y (k_adj_inv_xy adds the dense
+    // x and y shares afterwards)
+    if (covered) {
+      const long long ok = (long long)k * sz;
+      adj_heavy_add(hr, [&](int m) { return k + m >= 0 && k + m < kp.Nzt ? rz[(long long)(k + m) * sz] : 0u; }, g);
+      adj_heavy_add(hr, [&](int m) { return i + m >= 0 && i + m < kp.Nxt ? rx[ok + m] : 0u; }, g);
+      adj_heavy_add(hr, [&](int m) { return j + m >= 0 && j + m < kp.Nyt ? ry[ok + (long long)m * kp.Nxt] : 0u; }, g);
     }
     lamQ[o] = g[0];
     lamQ[o + nc] = g[1];
@@ -247,10 +320,10 @@ __global__ __launch_bounds__(kAIW * kAIL) void k_adj_inv_xy(KP kp, const double*
 #pragma unroll 1
   for (int j = j0; j < j1; ++j) {
     load(j + 4, Lw[8]);
+    const bool jin = interior(j, kp.Nyp, NG);
 #pragma unroll
     for (int n = 0; n < 5; ++n) sL[n][ln][t] = Lw[4][n];
     __syncthreads();
-    const bool jin = interior(j, kp.Nyp, NG);
     if (out && kok && (jin || iin)) {
       const long long o = row0 + (long long)j * sy;
       const double rho = Q[o], u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc], Tt = Q[o + 5 * nc];
@@ -286,72 +359,130 @@ __global__ __launch_bounds__(kAIW * kAIL) void k_adj_inv_xy(KP kp, const double*
 }
 
 // ------------------------------------------------------------------ inviscid, listed faces
-__global__ __launch_bounds__(64) void k_adj_heavy(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
-                                                  const double* __restrict__ Lam,
-                                                  const unsigned long long* __restrict__ list,
-                                                  const unsigned int* __restrict__ count, double* __restrict__ lamQ,
-                                                  HList hl) {
-  const long long nc = kp.nc;
+// Adjoint of a listed face's flux, gF = Lam[o] - Lam[o - st] (the face is the left face of o, the right one
+// of o - st); false when it vanishes.
+__device__ __forceinline__ bool adj_face_seed(const KP& kp, const double* __restrict__ Lam, long long o, long long st,
+                                              double* gF) {
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    gF[q] = Lam[o + q * kp.nc] - Lam[o - st + q * kp.nc];
+    any |= gF[q] != 0.0;
+  }
+  return any;
+}
+
+// Reverse of one listed face's (scheme - UP7) flux from its adjoint gF, cell by cell: emit(s, g) receives the
+// contribution g[0..4] to (rho, u, v, w, T) of stencil cell s (offset s - 4 along D from the right cell o), the
+// frame's dependence on (u, v, w, T) of cells 3 and 4 included in theirs.
+template <class Emit>
+__device__ __forceinline__ void adj_face_reverse(const KP& kp, const double* __restrict__ Q, const MetricPtrs<double>& mp,
+                                                 int D, int sch, long long o, const double* gF, Emit&& emit) {
+  const long long nc = kp.nc, st = dstride(kp, D);
+  const double* A1p = mp.m[3 * D];
+  const double* A2p = mp.m[3 * D + 1];
+  const double* A3p = mp.m[3 * D + 2];
+  double fp[8][5], fm[8][5], gfp[8][5], gfm[8][5];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const long long oc = o + (s - 4) * st;
+    const double A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
+    sw_split<double>(kp, Q[oc], Q[oc + nc], Q[oc + 2 * nc], Q[oc + 3 * nc], Q[oc + 5 * nc], A1, A2, A3,
+                     sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp[s], fm[s]);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) gfp[s][q] = gfm[s][q] = 0.0;
+  }
+  const long long oL = o - st;
+  const double qL[4] = {Q[oL + nc], Q[oL + 2 * nc], Q[oL + 3 * nc], Q[oL + 5 * nc]};
+  const double qR[4] = {Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc]};
+  const double AL[3] = {A1p[oL], A2p[oL], A3p[oL]};
+  const double AR[3] = {A1p[o], A2p[o], A3p[o]};
+  const double SL = sqrt(AL[0] * AL[0] + AL[1] * AL[1] + AL[2] * AL[2]);
+  const double SR = sqrt(AR[0] * AR[0] + AR[1] * AR[1] + AR[2] * AR[2]);
+  double gqL[4] = {0, 0, 0, 0}, gqR[4] = {0, 0, 0, 0};
+  face_flux_vjp(kp, sch, fp, fm, qL, qR, AL, AR, SL, SR, gF, gfp, gfm, gqL, gqR);
+  // the dense kernels already took this face as UP7
+#pragma unroll
+  for (int q = 0; q < 7; ++q)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      gfp[q][m] -= kp.lin[q] * gF[m];
+      gfm[7 - q][m] -= kp.lin[q] * gF[m];
+    }
+#pragma unroll 1
+  for (int s = 0; s < 8; ++s) {
+    const long long oc = o + (s - 4) * st;
+    const double A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
+    double g[5];
+    sw_split_vjp(kp, Q[oc], Q[oc + nc], Q[oc + 2 * nc], Q[oc + 3 * nc], Q[oc + 5 * nc], A1, A2, A3,
+                 sqrt(A1 * A1 + A2 * A2 + A3 * A3), gfp[s], gfm[s], g);
+    if (s == 3) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[q + 1] += gqL[q];
+    } else if (s == 4) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[q + 1] += gqR[q];
+    }
+    emit(s, g);
+  }
+}
+
+// Per listed face: the reverse contributions to its eight stencil cells kept in the face's compact slot (a
+// wave takes its slots with one counter update) and the slot recorded under the face's right cell; the dense
+// inviscid kernels then add them cell by cell in a fixed order.  Faces beyond the buffer's capacity are left
+// to k_adj_heavy_rest.
+__global__ __launch_bounds__(64) void k_adj_heavy_form(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
+                                                       const double* __restrict__ Lam,
+                                                       const unsigned long long* __restrict__ list,
+                                                       const unsigned int* __restrict__ count, HList hl, HRec hr) {
   for_each_heavy(hl, count, list, [&](const unsigned long long v) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4), st = dstride(kp, D);
     double gF[5];
-    bool any = false;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      gF[q] = Lam[o + q * nc] - Lam[o - st + q * nc];  // face is the left face of o, the right one of o-st
-      any |= gF[q] != 0.0;
-    }
+    const bool any = adj_face_seed(kp, Lam, o, st, gF);
+    const unsigned long long m = __ballot(any);
+    if (!m) return;
+    const int lane = __lane_id(), leader = __ffsll((unsigned long long)m) - 1;
+    unsigned int b = 0;
+    if (lane == leader) b = atomicAdd(hr.ctr, (unsigned int)__popcll(m));
+    b = __shfl(b, leader);
     if (!any) return;
-    const double* A1p = mp.m[3 * D];
-    const double* A2p = mp.m[3 * D + 1];
-    const double* A3p = mp.m[3 * D + 2];
-    double fp[8][5], fm[8][5], gfp[8][5], gfm[8][5];
+    const unsigned int cs = b + (unsigned int)__popcll(m & ((1ULL << lane) - 1ULL));
+    if (cs >= hr.cap) return;
+    double* dst = hr.buf + 40ULL * cs;
+    const unsigned char f8 = (unsigned char)(hr.stamp >> 24);
+    adj_face_reverse(kp, Q, mp, D, sch, o, gF, [&](int s, const double* g) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
+      for (int q = 0; q < 5; ++q) dst[5 * s + q] = g[q];
+      hr.flag[o + (s - 4) * st] = f8;
+    });
+    hr.rec[(long long)D * kp.nc + o] = hr.stamp | (cs + 1);
+  });
+}
+
+// Faces the buffer could not keep (only when more than `cap` listed faces have an adjoint): their
+// contributions added to lamQ with atomics after the dense kernels (summation order not fixed).
+__global__ __launch_bounds__(64) void k_adj_heavy_rest(KP kp, const double* __restrict__ Q, MetricPtrs<double> mp,
+                                                       const double* __restrict__ Lam,
+                                                       const unsigned long long* __restrict__ list,
+                                                       const unsigned int* __restrict__ count, double* __restrict__ lamQ,
+                                                       HList hl, HRec hr) {
+  if (*hr.ctr <= hr.cap) return;
+  const long long nc = kp.nc;
+  for_each_heavy(hl, count, list, [&](const unsigned long long v) {
+    const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
+    const long long o = (long long)(v >> 4), st = dstride(kp, D);
+    if (hrec_slot(hr, hr.rec[(long long)D * nc + o]) >= 0) return;
+    double gF[5];
+    if (!adj_face_seed(kp, Lam, o, st, gF)) return;
+    adj_face_reverse(kp, Q, mp, D, sch, o, gF, [&](int s, const double* g) {
       const long long oc = o + (s - 4) * st;
-      const double A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
-      sw_split<double>(kp, Q[oc], Q[oc + nc], Q[oc + 2 * nc], Q[oc + 3 * nc], Q[oc + 5 * nc], A1, A2, A3,
-                       sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp[s], fm[s]);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) gfp[s][q] = gfm[s][q] = 0.0;
-    }
-    const long long oL = o - st;
-    const double qL[4] = {Q[oL + nc], Q[oL + 2 * nc], Q[oL + 3 * nc], Q[oL + 5 * nc]};
-    const double qR[4] = {Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc]};
-    const double AL[3] = {A1p[oL], A2p[oL], A3p[oL]};
-    const double AR[3] = {A1p[o], A2p[o], A3p[o]};
-    const double SL = sqrt(AL[0] * AL[0] + AL[1] * AL[1] + AL[2] * AL[2]);
-    const double SR = sqrt(AR[0] * AR[0] + AR[1] * AR[1] + AR[2] * AR[2]);
-    double gqL[4] = {0, 0, 0, 0}, gqR[4] = {0, 0, 0, 0};
-    face_flux_vjp(kp, sch, fp, fm, qL, qR, AL, AR, SL, SR, gF, gfp, gfm, gqL, gqR);
-    // the dense kernel already took this face as UP7
-#pragma unroll
-    for (int q = 0; q < 7; ++q)
-#pragma unroll
-      for (int m = 0; m < 5; ++m) {
-        gfp[q][m] -= kp.lin[q] * gF[m];
-        gfm[7 - q][m] -= kp.lin[q] * gF[m];
-      }
-#pragma unroll 1
-    for (int s = 0; s < 8; ++s) {
-      const long long oc = o + (s - 4) * st;
-      const double A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
-      double g[5];
-      sw_split_vjp(kp, Q[oc], Q[oc + nc], Q[oc + 2 * nc], Q[oc + 3 * nc], Q[oc + 5 * nc], A1, A2, A3,
-                   sqrt(A1 * A1 + A2 * A2 + A3 * A3), gfp[s], gfm[s], g);
       atomicAdd(&lamQ[oc], g[0]);
       atomicAdd(&lamQ[oc + nc], g[1]);
       atomicAdd(&lamQ[oc + 2 * nc], g[2]);
       atomicAdd(&lamQ[oc + 3 * nc], g[3]);
       atomicAdd(&lamQ[oc + 5 * nc], g[4]);
-    }
-    const int var[4] = {1, 2, 3, 5};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      atomicAdd(&lamQ[oL + var[q] * nc], gqL[q]);
-      atomicAdd(&lamQ[o + var[q] * nc], gqR[q]);
-    }
+    });
   });
 }
 
@@ -1046,12 +1177,35 @@ int adj_inv_march() {
   return v;
 }
 
+// Faces of the grid (x, y and z faces of the interior).
+long long face_count(const f3d_params* p) {
+  const long long nx = p->Nxp, ny = p->Nyp, nz = p->Nzp;
+  return (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1);
+}
+
+// Listed faces whose reverse contributions the workspace keeps for the ordered gather: every face of the
+// grid up to 2^23 faces (2.5 GiB), else an eighth of the cell count (4.2 % of the faces), at most 2^24 - 2
+// (24-bit record slots); F3D_ADJ_HEAVY_CAP overrides (testing the path beyond it).  When it covers every
+// face, the atomic path for the faces beyond it is never launched.
+unsigned int adj_heavy_cap(const f3d_params* p, size_t nc) {
+  const long long faces = face_count(p);
+  const char* e = std::getenv("F3D_ADJ_HEAVY_CAP");
+  long long c = std::min(faces, std::max((long long)(nc / 8), 1LL << 23));
+  if (e) c = std::atoll(e);
+  if (c < 1) c = 1;
+  if (c > (1LL << 24) - 2) c = (1LL << 24) - 2;
+  return (unsigned int)c;
+}
+
 int seg_len_adj(int N, int max_len) {
   const int nseg = (N + max_len - 1) / max_len;
   return (N + nseg - 1) / nseg;
 }
 
-dim3 adj_list_grid(const KP& kp) { return dim3((kp.Nxp + 1 + 63) / 64, (kp.Nyp + 1 + 3) / 4, kp.Nzp + 1); }
+dim3 adj_list_grid(const KP& kp) {
+  const long long npos = (long long)(kp.Nxp + 1) * (kp.Nyp + 1);
+  return dim3((unsigned int)((npos + 255) / 256), 1, kp.Nzp + 1);
+}
 
 }  // namespace
 
@@ -1078,13 +1232,27 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
   int rc = check_hlist(fv, hl, "rhs vjp");
   if (!rc) rc = launch_shock_sensor<double>(kp, fv.phi, Q, s);
   if (rc) return rc;
+  // a new record stamp per evaluation (8 bits; the records are cleared when it wraps)
+  if (++aws->hstamp > 255u) {
+    // the record and flag arrays lie back to back
+    hipLaunchKernelGGL(k_adj_zero, dim3(1024), dim3(256), 0, s, (double*)aws->hrec,
+                       (long long)((align_up(3 * (size_t)kp.nc * sizeof(unsigned int)) + (size_t)kp.nc + sizeof(double) - 1) /
+                                   sizeof(double)));
+    aws->hstamp = 1;
+  }
+  const HRec hr{aws->hbuf, aws->hrec, aws->hflag, aws->hctr, aws->hcap, aws->hstamp << 24};
+  const unsigned int nbh = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
   {
     TimedScope ts("adj_heavy_list", s);
-    hipLaunchKernelGGL(k_adj_heavy_list, adj_list_grid(kp), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl);
+    hipLaunchKernelGGL(k_adj_heavy_list, adj_list_grid(kp), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl, aws->hctr);
   }
   if (!lam_ready) {
     TimedScope ts("adj_lambda", s);
     hipLaunchKernelGGL(k_adj_lambda, grid_interior(kp), kB, 0, s, kp, lam, J, scale, aws->Lam);
+  }
+  {
+    TimedScope ts("adj_heavy", s);
+    hipLaunchKernelGGL(k_adj_heavy_form, dim3(nbh), dim3(64), 0, s, kp, Q, mp, aws->Lam, fv.heavy, fv.hcount, hl, hr);
   }
   if (adj_inv_march()) {
     TimedScope ts("adj_inviscid", s);
@@ -1095,17 +1263,18 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
     const int Lz = seg_len_adj(kp.Nzt, ez && std::atoi(ez) >= 8 ? std::atoi(ez) : 24);
     const int Ly = seg_len_adj(kp.Nyt, ey && std::atoi(ey) >= 8 ? std::atoi(ey) : 24);
     hipLaunchKernelGGL(k_adj_inv_z, dim3((kp.Nxt + 63) / 64, (kp.Nyt + 3) / 4, (kp.Nzt + Lz - 1) / Lz), kB, 0, s, kp,
-                       Q, mp, aws->Lam, lamQ, Lz);
+                       Q, mp, aws->Lam, lamQ, Lz, hr);
     hipLaunchKernelGGL(k_adj_inv_xy, dim3((kp.Nxt + kAII - 1) / kAII, (kp.Nzp + kAIL - 1) / kAIL, (kp.Nyt + Ly - 1) / Ly),
                        dim3(kAIW, kAIL, 1), 0, s, kp, Q, mp, aws->Lam, lamQ, Ly);
   } else {
     TimedScope ts("adj_inviscid", s);
-    hipLaunchKernelGGL(k_adj_inviscid, grid_all(kp), kB, 0, s, kp, Q, mp, aws->Lam, lamQ);
+    hipLaunchKernelGGL(k_adj_inviscid, grid_all(kp), kB, 0, s, kp, Q, mp, aws->Lam, lamQ, hr);
   }
-  {
-    TimedScope ts("adj_heavy", s);
-    const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
-    hipLaunchKernelGGL(k_adj_heavy, dim3(nb), dim3(64), 0, s, kp, Q, mp, aws->Lam, fv.heavy, fv.hcount, lamQ, hl);
+  if ((long long)aws->hcap < face_count(p)) {
+    TimedScope ts("adj_heavy_rest", s);
+    // a small grid: it exits at once unless the buffer overflowed (the walk strides over the items)
+    hipLaunchKernelGGL(k_adj_heavy_rest, dim3(nbh < 256u ? nbh : 256u), dim3(64), 0, s, kp, Q, mp, aws->Lam, fv.heavy,
+                       fv.hcount, lamQ, hl, hr);
   }
   if (kp.viscous && adj_visc_march()) {
     TimedScope ts("adj_visc", s);
@@ -1256,7 +1425,10 @@ F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace*
   const size_t nc = f3d_cell_count(p), v = nc * sizeof(double);
   // Lam 5, lamQ 6, G 16 (the split viscous reverse only), Qs 3x6, Un 5, lam 5, lamUn 5, Ucur 5, Qcur 6, red
   const size_t nvars = 5 + 6 + (adj_visc_march() ? 0 : 16) + 18 + 5 + 5 + 5 + 5 + 6;
-  const size_t total = nvars * align_up(v) + align_up(kObjBlocks * sizeof(double));
+  const unsigned int hcap = adj_heavy_cap(p, nc);
+  const size_t hbytes = align_up((size_t)hcap * 40 * sizeof(double)) + align_up(3 * nc * sizeof(unsigned int)) +
+                        align_up(nc) + 256;
+  const size_t total = nvars * align_up(v) + align_up(kObjBlocks * sizeof(double)) + hbytes;
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, total);
   if (e != hipSuccess) {
@@ -1286,6 +1458,16 @@ F3D_API int f3d_adjoint_workspace_create(const f3d_params* p, f3d_adj_workspace*
   a->Qcur = take(6);
   a->red = (double*)c;
   a->nred = kObjBlocks;
+  c += align_up(kObjBlocks * sizeof(double));
+  a->hbuf = (double*)c;
+  c += align_up((size_t)hcap * 40 * sizeof(double));
+  a->hrec = (unsigned int*)c;
+  c += align_up(3 * nc * sizeof(unsigned int));
+  a->hflag = (unsigned char*)c;
+  c += align_up(nc);
+  a->hctr = (unsigned int*)c;
+  a->hcap = hcap;
+  a->hstamp = 0;
   // zeroed once: ghost entries of Lam are never written and must read as zero
   hipLaunchKernelGGL(k_adj_zero, dim3(1024), dim3(256), 0, 0, (double*)base, (long long)(total / 8));
   e = hipDeviceSynchronize();
