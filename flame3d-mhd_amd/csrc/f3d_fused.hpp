@@ -274,7 +274,7 @@ bool fused_needs_fv(const KP& kp);
 // Heavy-face list layouts: the finite-difference sweeps of bulk element size es, the finite-volume
 // sweeps, and the adjoint's list kernels.  The workspace holds the largest of the three.
 HList fdm_heavy_layout(const KP& kp, int es);
-HList fvm_heavy_layout(const KP& kp);
+HList fvm_heavy_layout(const KP& kp, int es);
 HList adj_heavy_layout(const KP& kp);
 // F3D_OK when the workspace view holds layout h, else F3D_E_ARG with a message.
 template <typename T>

@@ -19,6 +19,7 @@
 //                    kept in the face's list slot, then added to the two cells by k_heavy_gather
 //                    (fixed order per cell, no floating-point atomics)
 //   fused_update     viscous divergence + update + RK3 blend + c2Prim (f3d_fused.hip)
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -283,6 +284,165 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
   if (threadIdx.x == 0) a.hcount[hs.first + hw] = hcnt;
 }
 
+// ------------------------------------------------------------------ fused x + y march
+// Lanes = a window of FXW cells along x (4 halo cells each side, FXW - 8 interior columns), threadIdx.y = a
+// z line; the block marches a segment of L rows along y.  Per step the new row's U and sensor go to LDS and
+// the x faces of the row are formed there (k_fvm_sweep_x's states and HLLC), kept four steps in an LDS
+// ring; the y faces come from the column's 8-cell U window (k_fvm_march).  When a y face closes row r,
+//   R[r] = (F_x,left - F_x,right) + (F_y,prev - F_y,new),
+// the x sweep's store and the y march's update in one expression: R is written once (not written, read and
+// written again) and U, the sensor and the row's geometry are read once.  The x faces go to one run of the
+// heavy list per wave, the y faces to another.
+constexpr int FXW = 64;         // lanes of a row window (one wave)
+constexpr int FXI = FXW - 8;    // interior columns per window
+constexpr int FXL = 2;          // z lines (waves) per block
+
+template <typename T>
+__global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T> a, int L, HSeg hx, HSeg hy) {
+  __shared__ T sU[5][FXL][FXW];
+  __shared__ T sphi[FXL][FXW];
+  __shared__ T sF[4][5][FXL][FXW];  // x-face fluxes of the last four rows (slot = step & 3)
+  const int l = threadIdx.x, r = threadIdx.y;
+  const int fx0 = blockIdx.x * FXI;
+  const int ncx = min(FXI, kp.Nxp - fx0);  // interior columns of the window
+  const int nfx = ncx + 1;                 // x faces formed, fx0 .. fx0 + ncx (lanes 4 .. 4 + ncx)
+  const int tt = blockIdx.y * FXL + r;
+  const bool lineok = tt < kp.Nzp;
+  const int tc = min(tt, kp.Nzp - 1);
+  const int k = kp.NG + tc;
+  const int x = min(kp.NG + fx0 - 4 + l, kp.Nxt - 1);  // the lane's cell (lanes past the stencil clamped)
+  const int ii = fx0 + l - 4;                           // interior column index of the lane
+  const bool yin = l >= 4 && l < 4 + ncx;               // interior column: its y line is updated
+  const bool act = lineok && yin;
+  const bool fact = l >= 4 && l < 4 + nfx;              // the lane forms the x face between cells l-1, l
+  const int f0 = blockIdx.z * L;
+  const int ncell = min(L, kp.Nyp - f0);
+  const int nstep = ncell + 8;
+  const long long st = kp.Nxt, nc = kp.nc;
+  const long long obase = cidx(kp, x, kp.NG + f0 - 4, k);
+  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
+  const long long gs0 = fcount(kp, 0), gs1 = fcount(kp, 1);
+  const long long gy0 = fidx(kp, 1, yin ? ii : 0, f0, tc);  // geometry of this column's y face f0
+  T W[8][5];
+  T ph[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    ph[q] = 0;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) W[q][n] = 0;
+  }
+  T Fprev[5] = {0, 0, 0, 0, 0};
+  T gn[4] = {0, 0, 0, 0};  // geometry of the next y face to complete
+  T gx[4] = {0, 0, 0, 0};  // geometry of the x face of the next interior row
+  const unsigned int hw = launch_wave(FXL, r);
+  const unsigned long long xrun = hx.base + (unsigned long long)hw * hx.cap;
+  const unsigned long long yrun = hy.base + (unsigned long long)hw * hy.cap;
+  unsigned int xcnt = 0, ycnt = 0;
+  double nu[5];
+  T nph;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) nu[n] = a.U[obase + n * nc];
+  nph = a.phi[obase];
+#pragma unroll 1
+  for (int step = 0; step < nstep; ++step) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      ph[q] = ph[q + 1];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) W[q][n] = W[q + 1][n];
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) W[7][n] = (T)nu[n];
+    ph[7] = nph;
+    const bool xstep = step >= 4 && step < 4 + ncell;  // this step's row is interior: form its x faces
+    const T gx0 = gx[0], gx1 = gx[1], gx2 = gx[2], gx3 = gx[3];
+    if (step + 1 < nstep) {
+      const long long o = obase + (long long)(step + 1) * st;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) nu[n] = a.U[o + n * nc];
+      nph = a.phi[o];
+      if (fact && lineok && step + 1 >= 4 && step + 1 < 4 + ncell) {
+        const long long go = fidx(kp, 0, ii, f0 + step + 1 - 4, tc);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) gx[q] = a.geo[0][go + q * gs0];
+      }
+    }
+    if (xstep) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) sU[n][r][l] = W[7][n];
+      sphi[r][l] = ph[7];
+    }
+    if (step >= 6) {
+      const T g0 = gn[0], g1 = gn[1], g2 = gn[2], g3 = gn[3];  // face f0 + step - 7 (step >= 7)
+      if (act && step - 6 <= ncell) {  // request face f0 + step - 6
+        const long long go = gy0 + (long long)(step - 6) * kp.Nxp;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) gn[q] = a.geo[1][go + q * gs1];
+      }
+      if (step >= 7) {  // y face f0 + step - 7 complete: it closes row f - 1
+        const int fi = step - 7;
+        const int f = f0 + fi;
+        T m = ph[1];
+#pragma unroll
+        for (int q = 2; q <= 6; ++q) m = jmax(m, ph[q]);
+        const int sch = scheme_of<T>(kp, m);
+        T Fc[5] = {0, 0, 0, 0, 0};
+        if (sch == F3D_SCHEME_UP7) {
+          T UL[5], UR[5];
+          fvm_up7_states<T>(kp, W, UL, UR);
+          fvm_hllc<T>(kp, UL, UR, g0, g1, g2, Fc);
+#pragma unroll
+          for (int n = 0; n < 5; ++n) Fc[n] *= g3;
+        }
+        run_append(a.heavy, yrun, ycnt, act && (fi < L || f == kp.Nyp) && sch != F3D_SCHEME_UP7,
+                   ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) | 1ULL);
+        if (fi >= 1 && act) {  // row f - 1: its x faces were formed four steps ago (slot step & 3)
+          const long long ro = rix(kp, ii, f - 1, tc);
+          const int sl = step & 3;
+#pragma unroll
+          for (int n = 0; n < 5; ++n) a.R[ro + n * rn] = (sF[sl][n][r][l] - sF[sl][n][r][l + 1]) + (Fprev[n] - Fc[n]);
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) Fprev[n] = Fc[n];
+      }
+    }
+    __syncthreads();  // this row's U and sensor are in LDS; the ring slot read above is free
+    if (xstep) {
+      const int sl = step & 3;
+      int sch = F3D_SCHEME_UP7;
+      if (fact) {
+        T m = sphi[r][l - 3];
+#pragma unroll
+        for (int q = -2; q <= 2; ++q) m = jmax(m, sphi[r][l + q]);
+        sch = scheme_of<T>(kp, m);
+        T Fl[5] = {0, 0, 0, 0, 0};
+        if (sch == F3D_SCHEME_UP7) {
+          T Wx[8][5];
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int n = 0; n < 5; ++n) Wx[q][n] = sU[n][r][l - 4 + q];
+          T UL[5], UR[5];
+          fvm_up7_states<T>(kp, Wx, UL, UR);
+          fvm_hllc<T>(kp, UL, UR, gx0, gx1, gx2, Fl);
+#pragma unroll
+          for (int n = 0; n < 5; ++n) Fl[n] *= gx3;
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) sF[sl][n][r][l] = Fl[n];
+      }
+      const int j = kp.NG + f0 + step - 4;
+      run_append(a.heavy, xrun, xcnt, lineok && fact && (ii < fx0 + FXI || ii == kp.Nxp) && sch != F3D_SCHEME_UP7,
+                 ((unsigned long long)cidx(kp, kp.NG + ii, j, k) << 4) | ((unsigned long long)sch << 2) | 0ULL);
+    }
+    __syncthreads();  // the row's U is read before the next row overwrites it
+  }
+  if (l == 0) {
+    a.hcount[hx.first + hw] = xcnt;
+    a.hcount[hy.first + hw] = ycnt;
+  }
+}
+
 // ------------------------------------------------------------------ heavy faces
 // Every listed face (one launch, all directions): the full characteristic / component-wise
 // reconstruction + HLLC flux (the sweep gave these faces zero), kept in the face's list slot and recorded
@@ -334,6 +494,62 @@ int fvm_seg(int D) {
 }
 
 static dim3 fvm_grid_x(const KP& kp, int L) { return dim3((kp.Nxp + L - 1) / L, (kp.Nyp + 1) / 2, kp.Nzp); }
+static dim3 fvm_grid_xy(const KP& kp, int L) {
+  return dim3((kp.Nxp + FXI - 1) / FXI, (kp.Nzp + FXL - 1) / FXL, (kp.Nyp + L - 1) / L);
+}
+
+// 1 (default): x and y faces in one march (k_fvm_sweep_xy); F3D_FVM_SWEEP_XY=0 runs k_fvm_sweep_x and the y
+// march (A/B measurement; the two forms give the same sums in the same order).
+int fvm_sweep_xy() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("F3D_FVM_SWEEP_XY");
+    v = (e && std::strcmp(e, "0") == 0) ? 0 : 1;
+  }
+  return v;
+}
+
+int cu_count_fvm() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      v <= 0)
+    return 256;
+  return v;
+}
+
+// Row-segment length of the x+y march (as for the finite-difference x+y march): w waves over s resident-wave
+// slots take about (L + 8) x ceil(w / s) steps when they overfill the chip and (L + 8) / sqrt(w / s) when they
+// fill part of it; minimised over L = 16 .. 64.  F3D_FVM_SEG_Y overrides.
+int fvm_xy_seg(const KP& kp, int es) {
+  const char* e = std::getenv("F3D_FVM_SEG_Y");
+  if (e && std::atoi(e) >= 8) return seg_len(kp.Nyp, std::atoi(e));
+  static int wpc[2] = {0, 0};
+  int& w = wpc[es == 8];
+  if (w == 0) {
+    int nb = 0;
+    const void* kern = es == 8 ? (const void*)&k_fvm_sweep_xy<double> : (const void*)&k_fvm_sweep_xy<float>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, FXW * FXL, 0) != hipSuccess || nb <= 0) {
+      (void)hipGetLastError();
+      nb = es == 8 ? 4 : 6;
+    }
+    w = nb * FXL;
+  }
+  const double slots = (double)cu_count_fvm() * w;
+  const double per_seg = (double)FXL * ((kp.Nxp + FXI - 1) / FXI) * ((kp.Nzp + FXL - 1) / FXL);
+  static const int cand[5] = {16, 24, 32, 48, 64};
+  int best = seg_len(kp.Nyp, 64);
+  double bc = -1;
+  for (int c : cand) {
+    const int L = seg_len(kp.Nyp, c);
+    const double f = per_seg * ((kp.Nyp + L - 1) / L) / slots;
+    const double cost = (L + 8) * (f > 1 ? std::ceil(f) : 1.0 / std::sqrt(f));
+    if (bc < 0 || cost < bc * 0.999) {
+      bc = cost;
+      best = L;
+    }
+  }
+  return best;
+}
 static dim3 fvm_grid_march(const KP& kp, int D, int L) {
   const int Nt2 = D == 1 ? kp.Nzp : kp.Nyp, Np = D == 1 ? kp.Nyp : kp.Nzp;
   return dim3((kp.Nxp + 63) / 64, (Nt2 + 3) / 4, (Np + L - 1) / L);
@@ -341,9 +557,17 @@ static dim3 fvm_grid_march(const KP& kp, int D, int L) {
 
 }  // namespace
 
-HList fvm_heavy_layout(const KP& kp) {
+HList fvm_heavy_layout(const KP& kp, int es) {
   const int Lx = seg_len(kp.Nxp, kXW - 8), Ly = seg_len(kp.Nyp, fvm_seg(1)), Lz = seg_len(kp.Nzp, fvm_seg(2));
   auto nb = [](const dim3& g) { return (unsigned long long)g.x * g.y * g.z; };
+  if (fvm_sweep_xy()) {
+    // one wave per z line of a block; per row at most one x face per lane, per closed y face one per lane
+    const int Lxy = fvm_xy_seg(kp, es);
+    const unsigned long long w = FXL * nb(fvm_grid_xy(kp, Lxy));
+    const unsigned long long nw[3] = {w, w, 4 * nb(fvm_grid_march(kp, 2, Lz))};
+    const unsigned long long cap[3] = {64ULL * Lxy, 64ULL * (Lxy + 1), 64ULL * (Lz + 1)};
+    return make_hlist(nw, cap);
+  }
   const unsigned long long nw[3] = {4 * nb(fvm_grid_x(kp, Lx)), 4 * nb(fvm_grid_march(kp, 1, Ly)),
                                     4 * nb(fvm_grid_march(kp, 2, Lz))};
   const unsigned long long cap[3] = {64, 64ULL * (Ly + 1), 64ULL * (Lz + 1)};
@@ -356,17 +580,23 @@ template <typename T>
 int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
              const T* const* geo, const T* phi, hipStream_t s) {
   FvmArgs<T> a{U, {geo[0], geo[1], geo[2]}, phi, ws.R, ws.heavy, ws.hcount};
-  const HList hl = fvm_heavy_layout(kp);
+  const HList hl = fvm_heavy_layout(kp, (int)sizeof(T));
   if (int rc = check_hlist(ws, hl, "fvm sweeps")) return rc;
-  {
-    TimedScope ts("fvm_sweep_x", s);
-    const int L = seg_len(kp.Nxp, kXW - 8);
-    hipLaunchKernelGGL(k_fvm_sweep_x<T>, fvm_grid_x(kp, L), dim3(kXW, 2, 1), 0, s, kp, a, L, hl.d[0]);
-  }
-  {
-    TimedScope ts("fvm_sweep_y", s);
-    const int L = seg_len(kp.Nyp, fvm_seg(1));
-    hipLaunchKernelGGL((k_fvm_march<T, 1>), fvm_grid_march(kp, 1, L), dim3(64, 4, 1), 0, s, kp, a, L, hl.d[1]);
+  if (fvm_sweep_xy()) {
+    TimedScope ts("fvm_sweep_xy", s);
+    const int L = fvm_xy_seg(kp, (int)sizeof(T));
+    hipLaunchKernelGGL(k_fvm_sweep_xy<T>, fvm_grid_xy(kp, L), dim3(FXW, FXL, 1), 0, s, kp, a, L, hl.d[0], hl.d[1]);
+  } else {
+    {
+      TimedScope ts("fvm_sweep_x", s);
+      const int L = seg_len(kp.Nxp, kXW - 8);
+      hipLaunchKernelGGL(k_fvm_sweep_x<T>, fvm_grid_x(kp, L), dim3(kXW, 2, 1), 0, s, kp, a, L, hl.d[0]);
+    }
+    {
+      TimedScope ts("fvm_sweep_y", s);
+      const int L = seg_len(kp.Nyp, fvm_seg(1));
+      hipLaunchKernelGGL((k_fvm_march<T, 1>), fvm_grid_march(kp, 1, L), dim3(64, 4, 1), 0, s, kp, a, L, hl.d[1]);
+    }
   }
   {
     TimedScope ts("fvm_sweep_z", s);

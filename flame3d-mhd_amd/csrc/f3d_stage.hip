@@ -467,7 +467,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   // heavy-face list in per-wave runs: room for the largest layout of the finite-difference sweeps (this
   // precision), the finite-volume sweeps and the adjoint's list kernels
   const KP kp0 = make_kp(p);
-  const HList hls[3] = {fdm_heavy_layout(kp0, (int)es), fvm_heavy_layout(kp0), adj_heavy_layout(kp0)};
+  const HList hls[3] = {fdm_heavy_layout(kp0, (int)es), fvm_heavy_layout(kp0, (int)es), adj_heavy_layout(kp0)};
   unsigned long long hslots = 0, hruns = 0;
   for (const HList& h : hls) {
     hslots = h.slots() > hslots ? h.slots() : hslots;

@@ -1,5 +1,5 @@
-"""Child process of test_gpu_sweep_forms.py: evaluates the RHS and `nsteps` time steps of the inputs held
-in one .npz under the environment switches its parent set (the library reads them once per process) and
+"""Child process of test_gpu_sweep_forms.py: evaluates the RHS and `nsteps` time steps (finite-difference,
+or finite-volume when the parameters select it) of the inputs held in one .npz under the environment switches its parent set (the library reads them once per process) and
 writes R, U, Un and Q to another .npz.  Usage: sweep_child.py IN.npz OUT.npz"""
 import importlib
 import os
@@ -22,9 +22,15 @@ def run(f3d, torch, z):
     J, Q, U = d(z["J"]), d(z["Q"]), d(z["U"])
     inflow = d(z["inflow"]) if z["inflow"].size else None
     R = torch.zeros(p.cell_shape(5), dtype=torch.float64, device="cuda:0")
-    ctx.rhs(R, Q, m, J)
     Un = torch.zeros_like(U)
-    ctx.time_step(int(z["nsteps"]), U, Un, Q, m, J, float(z["dt"]), inflow)
+    if p.fvm:  # finite-volume branch: face geometry from the mesh points
+        geo = [torch.zeros((4,) + p.face_shape(q)[1:], dtype=Q.dtype, device="cuda:0") for q in range(3)]
+        ctx.fvm_geometry(d(z["x"]), d(z["y"]), d(z["z"]), geo)
+        ctx.rhs_fvm(R, Q, U, m, J, geo)
+        ctx.time_step_fvm(int(z["nsteps"]), U, Un, Q, m, J, geo, float(z["dt"]), inflow)
+    else:
+        ctx.rhs(R, Q, m, J)
+        ctx.time_step(int(z["nsteps"]), U, Un, Q, m, J, float(z["dt"]), inflow)
     torch.cuda.synchronize()
     return dict(R=R.cpu().numpy(), U=U.cpu().numpy(), Un=Un.cpu().numpy(), Q=Q.cpu().numpy())
 
