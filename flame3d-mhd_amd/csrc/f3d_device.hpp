@@ -443,6 +443,61 @@ F3D_HD void face_flux(const KP& kp, int sch, const FPL& FP, const FML& FM,
   out[4] = flux[0] * (H - qn * c) + flux[1] * q2 + flux[2] * (H + qn * c) + flux[3] * ql + flux[4] * qm;
 }
 
+// Characteristic frame of one face (schemes.jl:756-811): arithmetic means of (u,v,w,T) of the two cells,
+// the averaged metric direction n, the tangents l and m, and the quantities the left and right eigenvectors
+// are formed from; face_flux's arithmetic, for kernels that form the rows of L one at a time.
+template <typename T>
+struct CharFrame {
+  T u, v, w, nx, ny, nz, lx, ly, lz, mx, my, mz, qn, ql, qm, q2, c, ic2, K, K2;
+  F3D_HD CharFrame(const KP& kp, const T* qL, const T* qR, const T* AL, const T* AR, T SL, T SR) {
+    const T s0 = T(2) / (SL + SR);
+    u = T(0.5) * (qR[0] + qL[0]);
+    v = T(0.5) * (qR[1] + qL[1]);
+    w = T(0.5) * (qR[2] + qL[2]);
+    const T Tt = T(0.5) * (qR[3] + qL[3]);
+    nx = (AR[0] + AL[0]) * T(0.5) * s0;
+    ny = (AR[1] + AL[1]) * T(0.5) * s0;
+    nz = (AR[2] + AL[2]) * T(0.5) * s0;
+    if (fabs(nz) <= fabs(ny)) {
+      const T ss = T(1) / sqrt(nx * nx + ny * ny);
+      lx = -ny * ss; ly = nx * ss; lz = 0;
+    } else {
+      const T ss = T(1) / sqrt(nx * nx + nz * nz);
+      lx = -nz * ss; ly = 0; lz = nx * ss;
+    }
+    mx = ny * lz - nz * ly; my = nz * lx - nx * lz; mz = nx * ly - ny * lx;
+    qn = u * nx + v * ny + w * nz; ql = u * lx + v * ly + w * lz; qm = u * mx + v * my + w * mz;
+    q2 = T(0.5) * (u * u + v * v + w * w);
+    c = sqrt((T)kp.gamma * (T)kp.Rg * Tt);
+    ic2 = T(1) / (2 * c);
+    K = ((T)kp.gamma - 1) / (c * c);
+    K2 = K * T(0.5);
+  }
+  // row n of the left eigenvector matrix (schemes.jl:801-806)
+  F3D_HD void lrow(int n, T* L) const {
+    if (n == 0) {
+      L[0] = K2 * q2 + qn * ic2; L[1] = -(K2 * u + nx * ic2); L[2] = -(K2 * v + ny * ic2); L[3] = -(K2 * w + nz * ic2); L[4] = K2;
+    } else if (n == 1) {
+      L[0] = 1 - K * q2; L[1] = K * u; L[2] = K * v; L[3] = K * w; L[4] = -K;
+    } else if (n == 2) {
+      L[0] = K2 * q2 - qn * ic2; L[1] = -(K2 * u - nx * ic2); L[2] = -(K2 * v - ny * ic2); L[3] = -(K2 * w - nz * ic2); L[4] = K2;
+    } else if (n == 3) {
+      L[0] = -ql; L[1] = lx; L[2] = ly; L[3] = lz; L[4] = T(0);
+    } else {
+      L[0] = -qm; L[1] = mx; L[2] = my; L[3] = mz; L[4] = T(0);
+    }
+  }
+  // back-projection with the right eigenvectors (schemes.jl:807-811, 926-930)
+  F3D_HD void project_back(const T* flux, T* out) const {
+    const T H = T(1) / K + q2;
+    out[0] = flux[0] + flux[1] + flux[2];
+    out[1] = flux[0] * (u - c * nx) + flux[1] * u + flux[2] * (u + c * nx) + flux[3] * lx + flux[4] * mx;
+    out[2] = flux[0] * (v - c * ny) + flux[1] * v + flux[2] * (v + c * ny) + flux[3] * ly + flux[4] * my;
+    out[3] = flux[0] * (w - c * nz) + flux[1] * w + flux[2] * (w + c * nz) + flux[3] * lz + flux[4] * mz;
+    out[4] = flux[0] * (H - qn * c) + flux[1] * q2 + flux[2] * (H + qn * c) + flux[3] * ql + flux[4] * qm;
+  }
+};
+
 // ------------------------------------------------------------------ viscous pieces
 // Eddy viscosity of the LES models (viscous.jl:66-106) from the Cartesian velocity gradient
 // (dx[q], dy[q], dz[q]: d/dx, d/dy, d/dz of u, v, w for q = 0, 1, 2), added to mu and lam

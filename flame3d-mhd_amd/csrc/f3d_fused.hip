@@ -509,6 +509,146 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __
   });
 }
 
+// The value `v` of the other lane of this lane's pair (lanes 2p, 2p + 1): a DPP quad permutation (1, 0, 3, 2),
+// no LDS.  Both lanes of a pair must be active.
+__device__ __forceinline__ float pair_swap(float v) {
+  const int b = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double pair_swap(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)(unsigned int)(unsigned long long)b, hi = (int)(b >> 32);
+  const int slo = __builtin_amdgcn_update_dpp(lo, lo, 0xB1, 0xF, 0xF, false);
+  const int shi = __builtin_amdgcn_update_dpp(hi, hi, 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned int)shi << 32) | (unsigned int)slo));
+}
+
+// k_heavy with a pair of lanes per listed face: the even lane splits stencil cells 0..3 and reconstructs
+// the F+ half (cells 0..6), the odd lane splits cells 4..7 and reconstructs the F- half (cells 7..1); the
+// split fluxes each half needs from the other lane's cells (three cells, five components), the frame's
+// two cells and the two halves' results cross with DPP moves.  The same loads and splits as one lane per
+// face (each cell once), half the registers per lane (two or more waves per SIMD instead of one in FP64),
+// the same arithmetic per field (F+ part + F- part); the UP7 flux subtracted is the F+ sum plus the F- sum.
+template <typename T, int SPLIT, bool CHAR>
+__global__ __launch_bounds__(64) void k_heavy_pair(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux, HIdx hi) {
+  const long long nc = kp.nc;
+  const bool odd = (threadIdx.x & 1) != 0;
+  for_each_heavy_pairs(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
+    const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
+    const long long o = (long long)(v >> 4);
+    const long long st = dstride(kp, D);
+    const T* A1p = a.m.m[3 * D];
+    const T* A2p = a.m.m[3 * D + 1];
+    const T* A3p = a.m.m[3 * D + 2];
+    // this lane's cells (even: 0..3, odd: 4..7) split; its frame cell (even: 3 = the face's left cell, odd: 4)
+    T fp[4][5], fm[4][5], cq[4], ca[3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long long oc = o + ((odd ? 4 : 0) + q - 4) * st;
+      const T A1 = A1p[oc], A2 = A2p[oc], A3 = A3p[oc];
+      const T r = a.Q[oc], u = a.Q[oc + nc], w2 = a.Q[oc + 2 * nc], w3 = a.Q[oc + 3 * nc];
+      split_cell<T, SPLIT>(kp, r, u, w2, w3, a.Q[oc + split_qvar<SPLIT>() * nc], A1, A2, A3,
+                           sqrt(A1 * A1 + A2 * A2 + A3 * A3), fp[q], fm[q]);
+      if (q == (odd ? 0 : 3)) {
+        cq[0] = u; cq[1] = w2; cq[2] = w3; cq[3] = a.Q[oc + 5 * nc];
+        ca[0] = A1; ca[1] = A2; ca[2] = A3;
+      }
+    }
+    T qL[4], qR[4], AL[3], AR[3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const T x = pair_swap(cq[q]);
+      qL[q] = odd ? x : cq[q];
+      qR[q] = odd ? cq[q] : x;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T x = pair_swap(ca[q]);
+      AL[q] = odd ? x : ca[q];
+      AR[q] = odd ? ca[q] : x;
+    }
+    const T SL = sqrt(AL[0] * AL[0] + AL[1] * AL[1] + AL[2] * AL[2]);
+    const T SR = sqrt(AR[0] * AR[0] + AR[1] * AR[1] + AR[2] * AR[2]);
+    // V[s]: even lane F+ of cell s, odd lane F- of cell 7 - s (s = 0..6); three of them come from the
+    // other lane (even: F+ of cells 4..6, odd: F- of cells 3..1)
+    T V[7][5];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        const T x = pair_swap(odd ? fp[q][n] : fm[q + 1][n]);
+        if (odd) V[6 - q][n] = x; else V[4 + q][n] = x;
+      }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) V[q][n] = odd ? fm[3 - q][n] : fp[q][n];
+    T Fu[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {  // the UP7 flux: F+ sum (even lane) + F- sum (odd lane)
+      T acc = 0;
+#pragma unroll
+      for (int q = 0; q < 7; ++q) acc += up7c<T>(kp, q) * V[q][n];
+      const T other = pair_swap(acc);
+      Fu[n] = odd ? other + acc : acc + other;
+    }
+    T Fs[5];
+    if (sch == F3D_SCHEME_NND2) {  // schemes.jl:1000-1009, component-wise in both forms
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        const T half = odd ? V[3][n] - T(0.5) * minmod(V[2][n] - V[3][n], V[3][n] - V[4][n])
+                           : V[3][n] + T(0.5) * minmod(V[4][n] - V[3][n], V[3][n] - V[2][n]);
+        const T other = pair_swap(half);
+        Fs[n] = odd ? other + half : half + other;
+      }
+    } else {
+      const T s0 = T(2) / (SL + SR);
+      T flux[5];
+      const CharFrame<T> fr(kp, qL, qR, AL, AR, SL, SR);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        T L[5] = {0, 0, 0, 0, 0};
+        if (CHAR) fr.lrow(n, L);
+        auto proj = [&](int q) {
+          return CHAR ? V[q][0] * L[0] + V[q][1] * L[1] + V[q][2] * L[2] + V[q][3] * L[3] + V[q][4] * L[4] : V[q][n];
+        };
+        T half;
+        if (sch == F3D_SCHEME_WENO7) {
+          T val[7];
+#pragma unroll
+          for (int q = 0; q < 7; ++q) val[q] = proj(q);
+          half = weno7(val, s0, (T)kp.eps7);
+        } else {
+          T val[5];
+#pragma unroll
+          for (int q = 0; q < 5; ++q) val[q] = proj(q + 1);
+          half = weno5(val, s0, (T)kp.eps5);
+        }
+        const T other = pair_swap(half);
+        const T sum = odd ? other + half : half + other;
+        flux[n] = sch == F3D_SCHEME_WENO7 ? sum * (T(1) / T(12)) : sum * (T(1) / T(6));
+      }
+      if (CHAR) {
+        fr.project_back(flux, Fs);
+      } else {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) Fs[n] = flux[n];
+      }
+    }
+    if (!odd) {
+      const int ci = (int)(o % kp.Nxt), cj = (int)((o / kp.Nxt) % kp.Nyt), ck = (int)(o / ((long long)kp.Nxt * kp.Nyt));
+      const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
+      const int f = D == 0 ? I : (D == 1 ? Jj : K);
+      const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+      const long long rr = ridx(kp, I, Jj, K);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) hflux[slot * 5 + q] = Fs[q] - Fu[q];
+      heavy_record(kp, hi, D, rr, f >= 1, f <= Np - 1, slot);
+      a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
+    }
+  });
+}
+
 // ------------------------------------------------------------------ viscous flux (k march)
 // viscousFlux (viscous.jl:1-134) on interior +-2 columns, one lane per (i, j) column marching k:
 // u, v, w, T of the column are kept in a 5-plane register window for the zeta derivatives, so each
@@ -1030,6 +1170,19 @@ static dim3 visc_grid(const KP& kp, int KC, int rows, int W) {
   return dim3((kp.Nxp + W - 5) / (W - 4), (kp.Nyp + rows - 1) / rows, (kp.Nzp + KC - 1) / KC);
 }
 
+// Listed faces formed by a pair of lanes each (k_heavy_pair) or one lane each (k_heavy).  Default: pairs in
+// FP64 (one wave per SIMD becomes two: forming + gather 503 -> 480 us per stage with 4.6 % of the faces off
+// UP7 on the 512x192x64 ramp), one lane in FP32-bulk (two waves per SIMD already; pairs 405 against 395
+// us).  F3D_HEAVY_PAIR=0 / 1 forces either.
+static int heavy_pairs(int es) {
+  static int v = -2;
+  if (v == -2) {
+    const char* e = std::getenv("F3D_HEAVY_PAIR");
+    v = e ? (std::strcmp(e, "0") == 0 ? 0 : 1) : -1;
+  }
+  return v >= 0 ? v : (es == 8 ? 1 : 0);
+}
+
 static int env_int(const char* name, int def) {
   const char* e = std::getenv(name);
   return e ? std::atoi(e) : def;
@@ -1309,10 +1462,16 @@ static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const
     TimedScope ts("fused_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
     const HIdx hi = next_hidx(kp, ws.hidx, ws.hstamp, s);
-    if (kp.character)
+    if (heavy_pairs((int)sizeof(T))) {
+      if (kp.character)
+        hipLaunchKernelGGL((k_heavy_pair<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
+      else
+        hipLaunchKernelGGL((k_heavy_pair<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
+    } else if (kp.character) {
       hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
-    else
+    } else {
       hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
+    }
     launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s);
   }
   return check_launch("fused sweeps");

@@ -136,6 +136,54 @@ __device__ __forceinline__ void heavy_item(const HSeg& hs, unsigned int loc, con
   }
 }
 
+// heavy_item with a pair of lanes per entry: lanes 2p and 2p + 1 both receive entry p, p + 32, ... of the
+// item (32 entries per pass of the wave), so that the two halves of one face's work can run side by side
+// and exchange values with cross-lane moves (the two lanes of a pair are always active together).
+template <typename F>
+__device__ __forceinline__ void heavy_item_pairs(const HSeg& hs, unsigned int loc, const unsigned int* __restrict__ counts,
+                                                 const unsigned long long* __restrict__ list, F&& f) {
+  const int lane = threadIdx.x, pr = lane >> 1;
+  if (hs.cap > 64) {
+    const unsigned int n = counts[hs.first + loc];
+    const unsigned long long run0 = hs.base + (unsigned long long)loc * hs.cap;
+    for (unsigned int e = pr; e < n; e += 32) f(list[run0 + e], run0 + e);
+  } else {
+    __shared__ unsigned int pre[64];
+    const unsigned int w0 = loc * 64, mine = w0 + lane;
+    const unsigned int myn = mine < hs.nw ? counts[hs.first + mine] : 0u;
+    unsigned int incl = myn;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned int t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const unsigned int tot = __shfl(incl, 63, 64);
+    pre[lane] = incl - myn;
+    __syncthreads();  // (a one-wave block)
+    for (unsigned int e = pr; e < tot; e += 32) {
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= e) lo = mid; else hi = mid - 1;
+      }
+      const unsigned long long slot = hs.base + (unsigned long long)(w0 + lo) * hs.cap + (e - pre[lo]);
+      f(list[slot], slot);
+    }
+    __syncthreads();  // pre is rewritten by the next item
+  }
+}
+
+// f(entry, slot) for every listed face, a pair of lanes per face (heavy_item_pairs).
+template <typename F>
+__device__ __forceinline__ void for_each_heavy_pairs(const HList& hl, const unsigned int* __restrict__ counts,
+                                                     const unsigned long long* __restrict__ list, F&& f) {
+  const unsigned int i0 = dir_items(hl.d[0]), i1 = dir_items(hl.d[1]), total = i0 + i1 + dir_items(hl.d[2]);
+  for (unsigned int it = blockIdx.x; it < total; it += gridDim.x) {
+    const int dq = it < i0 ? 0 : (it < i0 + i1 ? 1 : 2);
+    heavy_item_pairs(hl.d[dq], it - (dq == 0 ? 0u : (dq == 1 ? i0 : i0 + i1)), counts, list, f);
+  }
+}
+
 // f(entry, slot) for every listed face of all three directions, items spread over the grid.
 template <typename F>
 __device__ __forceinline__ void for_each_heavy_slot(const HList& hl, const unsigned int* __restrict__ counts,

@@ -1,7 +1,7 @@
 """The measurement switches of the library (environment variables read once per process: the
 kernel-by-kernel chain, the separate viscous kernels, the update tile forms (rows of 64 or two rows of
 32 lanes per wave) and march lengths, the prefetch, the march segments, the sensor planes per block, the
-block order) against the oracle.
+block order, one lane or a pair of lanes per listed face) against the oracle.
 
 Each variant runs in a child process (tests/switch_child.py) on a ramp case with a planted pressure
 jump and spikes (faces on WENO7, WENO5 and NND), large enough for several update tiles in x and y and
@@ -37,6 +37,8 @@ VARIANTS = [
     ("segments", {"F3D_MARCH_SEG_Y": "16", "F3D_MARCH_SEG_Z": "8"}),
     ("sensor_plane", {"F3D_SENSOR_KS": "0"}),
     ("plain_order", {"F3D_XCD_REMAP": "0"}),
+    ("heavy_one_lane", {"F3D_HEAVY_PAIR": "0"}),
+    ("heavy_lane_pairs", {"F3D_HEAVY_PAIR": "1"}),
 ]
 
 
