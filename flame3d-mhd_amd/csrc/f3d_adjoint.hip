@@ -90,13 +90,17 @@ __global__ __launch_bounds__(256) void k_adj_lambda(KP kp, const double* __restr
 __global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __restrict__ phi,
                                                         unsigned long long* __restrict__ list,
                                                         unsigned int* __restrict__ count, HList hl,
-                                                        unsigned int* __restrict__ hctr) {
+                                                        unsigned int* __restrict__ hctr, const unsigned int* pflag,
+                                                        unsigned int pgen) {
   // lanes run over the (Nxp+1) x (Nyp+1) face-index positions of plane k in order (a wave's 64 positions
   // may span two rows), so that no wave is left mostly idle at the end of a row
   const long long t = (long long)(blockIdx.x * 4 + threadIdx.y) * 64 + threadIdx.x;
   const int i = (int)(t % (kp.Nxp + 1)), j = (int)(t / (kp.Nxp + 1)), k = blockIdx.z;
-  if (blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hctr = 0u;
   const unsigned int w = launch_wave(4, threadIdx.y);
+  // no sensor value of this state left the UP7 band: no face is listed, and the kernels that walk the list
+  // (k_adj_heavy_form, k_adj_heavy_rest) check the same flag and return at once
+  if (!heavy_possible(pflag, pgen)) return;
+  if (blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hctr = 0u;
 #pragma unroll
   for (int D = 0; D < 3; ++D) {
     const bool act = i < kp.Nxp + (D == 0) && j < kp.Nyp + (D == 1) && k < kp.Nzp + (D == 2);
@@ -119,6 +123,8 @@ __global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __r
 
 // HRec: the listed faces' contributions of one RHS VJP (workspace arrays, this evaluation's stamp).
 struct HRec {
+  const unsigned int* pflag;  // sensor band flag of the state and its generation (heavy_possible)
+  unsigned int pgen;
   double* buf;
   unsigned int* rec;     // 3 x nc, direction-major
   unsigned char* flag;   // per cell: the stamp when some kept face's stencil covers the cell
@@ -435,6 +441,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy_form(KP kp, const double* __re
                                                        const double* __restrict__ Lam,
                                                        const unsigned long long* __restrict__ list,
                                                        const unsigned int* __restrict__ count, HList hl, HRec hr) {
+  if (!heavy_possible(hr.pflag, hr.pgen)) return;  // nothing listed
   for_each_heavy(hl, count, list, [&](const unsigned long long v) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4), st = dstride(kp, D);
@@ -467,7 +474,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy_rest(KP kp, const double* __re
                                                        const unsigned long long* __restrict__ list,
                                                        const unsigned int* __restrict__ count, double* __restrict__ lamQ,
                                                        HList hl, HRec hr) {
-  if (*hr.ctr <= hr.cap) return;
+  if (!heavy_possible(hr.pflag, hr.pgen) || *hr.ctr <= hr.cap) return;
   const long long nc = kp.nc;
   for_each_heavy(hl, count, list, [&](const unsigned long long v) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
@@ -750,27 +757,32 @@ __device__ __forceinline__ long long bc_slot(const KP& kp, int side, int a0, int
 template <int D>
 __global__ __launch_bounds__(256) void k_adj_bc_final(KP kp, int red0, int red1, int lw, int nbx, int nby,
                                                       double* __restrict__ lamQ, double* __restrict__ lam_bc) {
-  __shared__ double sm[256];
+  // the seven components side by side; each one's partials are added in the same order as one at a time
+  // (per thread: side 1's blocks, then side 0's, q ascending; then the same halving tree)
+  __shared__ double sm[7][256];
   const int nb = nbx * nby;
-  for (int n = 0; n < 7; ++n) {
-    double v = 0.0;
-    for (int side = 1; side >= 0; --side) {
-      if (!(side ? red1 : red0)) continue;
-      for (int q = threadIdx.x; q < nb; q += 256) {
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int side = 1; side >= 0; --side) {
+    if (!(side ? red1 : red0)) continue;
+    for (int q = threadIdx.x; q < nb; q += 256) {
+#pragma unroll
+      for (int n = 0; n < 7; ++n) {
         const long long o = bc_slot<D>(kp, side, (q % nbx) * lw, (q / nbx) * 4, n);
-        v += lamQ[o];
+        v[n] += lamQ[o];
         lamQ[o] = 0.0;
       }
     }
-    sm[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
-      if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) lam_bc[n] += sm[0];
+  }
+#pragma unroll
+  for (int n = 0; n < 7; ++n) sm[n][threadIdx.x] = v[n];
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int n = 0; n < 7; ++n) sm[n][threadIdx.x] += sm[n][threadIdx.x + w];
     __syncthreads();
   }
+  if (threadIdx.x < 7) lam_bc[threadIdx.x] += sm[threadIdx.x][0];
 }
 
 // Reverse of one side of one dimension's ghost fill (boundary.jl:1-156 with the kinds of f3d.h):
@@ -1230,8 +1242,8 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
   const WsView<double> fv = ws_view<double>(aws->fws);
   const HList hl = f3d::adj_heavy_layout(kp);
   int rc = check_hlist(fv, hl, "rhs vjp");
-  if (!rc) rc = launch_shock_sensor<double>(kp, fv.phi, Q, s);
   if (rc) return rc;
+  fused_sensor<double>(kp, fv, Q, fv.phi, s);  // the same sensor values; marks whether any left the UP7 band
   // a new record stamp per evaluation (8 bits; the records are cleared when it wraps)
   if (++aws->hstamp > 255u) {
     // the record and flag arrays lie back to back
@@ -1240,11 +1252,13 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
                                    sizeof(double)));
     aws->hstamp = 1;
   }
-  const HRec hr{aws->hbuf, aws->hrec, aws->hflag, aws->hctr, aws->hcap, aws->hstamp << 24};
+  const HRec hr{fv.pgen ? fv.pflag : nullptr, fv.pgen ? *fv.pgen : 0u, aws->hbuf, aws->hrec, aws->hflag, aws->hctr,
+                aws->hcap, aws->hstamp << 24};
   const unsigned int nbh = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
   {
     TimedScope ts("adj_heavy_list", s);
-    hipLaunchKernelGGL(k_adj_heavy_list, adj_list_grid(kp), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl, aws->hctr);
+    hipLaunchKernelGGL(k_adj_heavy_list, adj_list_grid(kp), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl, aws->hctr,
+                       fv.pgen ? fv.pflag : nullptr, fv.pgen ? *fv.pgen : 0u);
   }
   if (!lam_ready) {
     TimedScope ts("adj_lambda", s);

@@ -76,6 +76,8 @@ struct FArgs {
   unsigned int* hcount;
   const double* ldt = nullptr;  // per-cell dt of local time stepping (div_LTS), else the scalar dt
   double* Uo = nullptr;         // conserved state written by the update (nullptr: U itself, in place)
+  const unsigned int* pflag = nullptr;  // sensor band flag of this stage (heavy_possible), and its generation
+  unsigned int pgen = 0;
 };
 
 __device__ __forceinline__ long long ridx(const KP& kp, int i, int j, int k) {
@@ -89,20 +91,35 @@ __device__ __forceinline__ T up7c(const KP& kp, int m) {
 }
 
 // ------------------------------------------------------------------ sensor (+ counter reset)
+// A wave with a sensor value not below phi1 (NaN included) records the stage's generation in the flag
+// (every such wave stores the same value).
 template <typename T>
-__global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, const T* __restrict__ Q) {
+__device__ __forceinline__ void mark_band(const KP& kp, bool out, unsigned int* pflag, unsigned int gen) {
+  if (pflag && __ballot(out) != 0ULL && __lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1) *pflag = gen;
+}
+template <typename T>
+__device__ __forceinline__ bool out_of_band(const KP& kp, T ph) {
+  return !(ph < (T)kp.phi1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, const T* __restrict__ Q,
+                                                  unsigned int* pflag, unsigned int gen) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y, k = 1 + blockIdx.z;
   if (i > kp.Nxt - 2 || j > kp.Nyt - 2 || k > kp.Nzt - 2) return;
   const T* P = Q + 4 * kp.nc;
   const long long o = cidx(kp, i, j, k), sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
-  phi[o] = sensor_cell<T>(P[o - 1], P[o], P[o + 1], P[o - sy], P[o + sy], P[o - sz], P[o + sz]);
+  const T ph = sensor_cell<T>(P[o - 1], P[o], P[o + 1], P[o - sy], P[o + sy], P[o - sz], P[o + sz]);
+  phi[o] = ph;
+  mark_band<T>(kp, out_of_band(kp, ph), pflag, gen);
 }
 
 // Same sensor, one lane per (i, j) column of [1, N_tot-2] marching a segment of KS planes: p of the
 // column at k-1, k, k+1 stays in registers, so each cell costs one new plane load plus the four xi/eta
 // neighbours (L1/L2) instead of six neighbour loads.  Identical arithmetic (sensor_cell).
 template <typename T>
-__global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi, const T* __restrict__ Q, int KS) {
+__global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi, const T* __restrict__ Q, int KS,
+                                                      unsigned int* pflag, unsigned int gen) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k0 = 1 + blockIdx.z * KS;
   if (i > kp.Nxt - 2 || j > kp.Nyt - 2) return;
@@ -111,13 +128,17 @@ __global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi
   const long long sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
   long long o = cidx(kp, i, j, k0);
   T pm = P[o - sz], pc = P[o];
+  bool out = false;
 #pragma unroll 1
   for (int k = k0; k < k1; ++k, o += sz) {
     const T pp = P[o + sz];
-    phi[o] = sensor_cell<T>(P[o - 1], pc, P[o + 1], P[o - sy], P[o + sy], pm, pp);
+    const T ph = sensor_cell<T>(P[o - 1], pc, P[o + 1], P[o - sy], P[o + sy], pm, pp);
+    phi[o] = ph;
+    out |= out_of_band(kp, ph);
     pm = pc;
     pc = pp;
   }
+  mark_band<T>(kp, out, pflag, gen);
 }
 
 // ------------------------------------------------------------------ x sweep (LDS row segments)
@@ -224,13 +245,15 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
   const unsigned int hw = launch_wave(4, threadIdx.y);
   const unsigned long long hrun = hs.base + (unsigned long long)hw * hs.cap;
   unsigned int hcnt = 0;
+  // no face can leave UP7 when no sensor value of the stage left the band: no sensor reads, no masks
+  const bool hp = heavy_possible(a.pflag, a.pgen);
   // prefetched inputs of the next cell
   T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph;
   {
     const long long o = obase;
     nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
     nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
-    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
+    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = hp ? a.phi[o] : T(0);
   }
 #pragma unroll 1
   for (int step = 0; step < nstep; ++step) {
@@ -239,7 +262,8 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
       const long long o = obase + (long long)(step + 1) * st;
       nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
       nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
-      na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
+      na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o];
+      if (hp) nph = a.phi[o];
     }
     T Rc[5];
 #pragma unroll
@@ -260,14 +284,16 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
       }
     if (step >= 7) {  // face f0 + step - 7 complete
       const int fi = step - 7;
-      T m = ring[0];
-#pragma unroll
-      for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
-      const int sch = scheme_of<T>(kp, m);
       const int f = f0 + fi;
-      run_append(a.heavy, hrun, hcnt, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
-                 ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) |
-                     (unsigned long long)D);
+      if (hp) {
+        T m = ring[0];
+#pragma unroll
+        for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
+        const int sch = scheme_of<T>(kp, m);
+        run_append(a.heavy, hrun, hcnt, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
+                   ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) |
+                       (unsigned long long)D);
+      }
       if (fi >= 1 && act) {
         const long long ro = D == 1 ? ridx(kp, ii, f - 1, tt) : ridx(kp, ii, tt, f - 1);
 #pragma unroll
@@ -345,13 +371,15 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
   const unsigned long long xrun = hx.base + (unsigned long long)hw * hx.cap;
   const unsigned long long yrun = hy.base + (unsigned long long)hw * hy.cap;
   unsigned int xcnt = 0, ycnt = 0;
+  // no face can leave UP7 when no sensor value of the stage left the band: no sensor reads, no masks
+  const bool hp = heavy_possible(a.pflag, a.pgen);
   // prefetched inputs of the next cell (x metrics only on the steps that form x faces: rows f0 ..)
   T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph, nb1 = 0, nb2 = 0, nb3 = 0;
   {
     const long long o = obase;
     nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
     nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
-    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
+    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = hp ? a.phi[o] : T(0);
   }
 #pragma unroll 1
   for (int step = 0; step < nstep; ++step) {
@@ -362,7 +390,8 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
       const long long o = obase + (long long)(step + 1) * st;
       nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
       nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
-      na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = a.phi[o];
+      na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o];
+      if (hp) nph = a.phi[o];
       if (step + 1 >= 4 && step + 1 < 4 + ncell) {
         nb1 = X1p[o]; nb2 = X2p[o]; nb3 = X3p[o];
       }
@@ -390,13 +419,15 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
     }
     if (step >= 7) {  // y face f0 + step - 7 complete
       const int fi = step - 7;
-      T m = ring[0];
-#pragma unroll
-      for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
-      const int sch = scheme_of<T>(kp, m);
       const int f = f0 + fi;
-      run_append(a.heavy, yrun, ycnt, act && (fi < L || f == kp.Nyp) && sch != F3D_SCHEME_UP7,
-                 ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) | 1ULL);
+      if (hp) {
+        T m = ring[0];
+#pragma unroll
+        for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
+        const int sch = scheme_of<T>(kp, m);
+        run_append(a.heavy, yrun, ycnt, act && (fi < L || f == kp.Nyp) && sch != F3D_SCHEME_UP7,
+                   ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) | 1ULL);
+      }
       if (fi >= 1 && act) {  // row f - 1: its x faces were formed four steps ago (slot step & 3)
         const long long ro = ridx(kp, ii, f - 1, tt);
         const int sl = step & 3;
@@ -425,10 +456,12 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
       const int sl = step & 3;
       int sch = F3D_SCHEME_UP7;
       if (fact) {
-        T m = sphi[r][l - 3];
+        if (hp) {
+          T m = sphi[r][l - 3];
 #pragma unroll
-        for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
-        sch = scheme_of<T>(kp, m);
+          for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
+          sch = scheme_of<T>(kp, m);
+        }
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
           T acx = 0;
@@ -440,8 +473,9 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
         }
       }
       const int j = kp.NG + f0 + step - 4;
-      run_append(a.heavy, xrun, xcnt, lineok && fact && (f < fx0 + XYI || f == kp.Nxp) && sch != F3D_SCHEME_UP7,
-                 ((unsigned long long)cidx(kp, kp.NG + f, j, k) << 4) | ((unsigned long long)sch << 2) | 0ULL);
+      if (hp)
+        run_append(a.heavy, xrun, xcnt, lineok && fact && (f < fx0 + XYI || f == kp.Nxp) && sch != F3D_SCHEME_UP7,
+                   ((unsigned long long)cidx(kp, kp.NG + f, j, k) << 4) | ((unsigned long long)sch << 2) | 0ULL);
     }
     __syncthreads();  // the split fluxes are read before the next row overwrites them
   }
@@ -458,6 +492,7 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
 // floating-point atomics: the corrections are summed the same way on every evaluation.
 template <typename T, int SPLIT, bool CHAR>
 __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux, HIdx hi) {
+  if (!heavy_possible(a.pflag, a.pgen)) return;  // nothing listed this stage
   const long long nc = kp.nc;
   for_each_heavy_slot(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
@@ -531,6 +566,7 @@ __device__ __forceinline__ double pair_swap(double v) {
 // the same arithmetic per field (F+ part + F- part); the UP7 flux subtracted is the F+ sum plus the F- sum.
 template <typename T, int SPLIT, bool CHAR>
 __global__ __launch_bounds__(64) void k_heavy_pair(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux, HIdx hi) {
+  if (!heavy_possible(a.pflag, a.pgen)) return;  // nothing listed this stage
   const long long nc = kp.nc;
   const bool odd = (threadIdx.x & 1) != 0;
   for_each_heavy_pairs(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
@@ -1326,12 +1362,16 @@ template <typename T>
 void fused_sensor(const KP& kp, const WsView<T>& ws, const T* Q, T* phi, hipStream_t s) {
   TimedScope ts("fused_sensor", s);
   const int KS = sensor_ks();
+  // a new generation for the band flag (generation 0 is never current: the flag starts cleared)
+  if (ws.pgen && ++*ws.pgen == 0) *ws.pgen = 1;
+  const unsigned int gen = ws.pgen ? *ws.pgen : 0;
+  unsigned int* pf = ws.pgen ? ws.pflag : nullptr;
   if (KS > 0)
     hipLaunchKernelGGL(k_sensor_march<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, (kp.Nzt - 2 + KS - 1) / KS),
-                       dim3(64, 4, 1), 0, s, kp, phi, Q, KS);
+                       dim3(64, 4, 1), 0, s, kp, phi, Q, KS, pf, gen);
   else
     hipLaunchKernelGGL(k_sensor_f<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, kp.Nzt - 2), dim3(64, 4, 1),
-                       0, s, kp, phi, Q);
+                       0, s, kp, phi, Q, pf, gen);
 }
 
 // Launch shapes of the finite-difference sweeps (shared by the launches and their heavy-list layout).
@@ -1435,6 +1475,8 @@ template <typename T, int SPLIT>
 static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J,
                             T* phi, hipStream_t s) {
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
+  a.pflag = ws.pgen ? ws.pflag : nullptr;
+  a.pgen = ws.pgen ? *ws.pgen : 0u;
   const HList hl = fdm_heavy_layout(kp, (int)sizeof(T));
   if (int rc = check_hlist(ws, hl, "fused sweeps")) return rc;
   if (sweep_xy()) {
@@ -1472,7 +1514,7 @@ static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const
     } else {
       hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
     }
-    launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s);
+    launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s, a.pflag, a.pgen);
   }
   return check_launch("fused sweeps");
 }

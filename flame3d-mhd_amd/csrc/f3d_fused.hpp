@@ -20,7 +20,17 @@ struct WsView {
   T* hflux;                  // per list slot: the 5 flux values a heavy face adds to its cells (hcap x 5)
   unsigned long long* hidx;  // per-cell records of the listed faces (hrec_count, see HIdx)
   unsigned int* hstamp;      // host counter of the records' stamps
+  unsigned int* pflag;       // device word: the sensor generation of the last stage in which some cell's
+                             // sensor value left the UP7 band (no face can be off UP7 otherwise)
+  unsigned int* pgen;        // host counter of the sensor generations
 };
+
+// Whether any face of this stage can be off UP7: the sensor marks its generation when some cell's value is
+// not below phi1 (NaN included), and every face scheme is chosen from the maximum of six cells' values.  A
+// NULL flag (callers without the fused sensor) means "possibly".
+__device__ __forceinline__ bool heavy_possible(const unsigned int* pflag, unsigned int gen) {
+  return pflag == nullptr || *pflag == gen;
+}
 
 // Block coordinates with an XCD-aware order.  Workgroups are dispatched round-robin over the 8 XCDs
 // (XCD = linear workgroup id mod 8) and each XCD has its own L2; giving every XCD one contiguous
@@ -242,7 +252,9 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_heavy_gather(KP kp, T* __restrict__ R, HList hl,
                                                      const unsigned int* __restrict__ counts,
                                                      const unsigned long long* __restrict__ list,
-                                                     const T* __restrict__ hflux, HIdx hi) {
+                                                     const T* __restrict__ hflux, HIdx hi, const unsigned int* pflag,
+                                                     unsigned int pgen) {
+  if (!heavy_possible(pflag, pgen)) return;  // nothing listed this stage
   const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
   for_each_heavy_slot(hl, counts, list, [&](const unsigned long long v, unsigned long long) {
     const int D = (int)(v & 3);
@@ -293,10 +305,11 @@ inline HIdx next_hidx(const KP& kp, unsigned long long* rec, unsigned int* stamp
 // Host side: the gather pass over the runs of all three directions.
 template <typename T>
 inline void launch_heavy_gather(const KP& kp, T* R, const HList& hl, const unsigned int* counts,
-                                const unsigned long long* list, const T* hflux, const HIdx& hi, hipStream_t s) {
+                                const unsigned long long* list, const T* hflux, const HIdx& hi, hipStream_t s,
+                                const unsigned int* pflag = nullptr, unsigned int pgen = 0) {
   const unsigned int it = hl.items();
   const unsigned int nb = it < 8192u ? (it ? it : 1u) : 8192u;
-  hipLaunchKernelGGL((k_heavy_gather<T>), dim3(nb), dim3(64), 0, s, kp, R, hl, counts, list, hflux, hi);
+  hipLaunchKernelGGL((k_heavy_gather<T>), dim3(nb), dim3(64), 0, s, kp, R, hl, counts, list, hflux, hi, pflag, pgen);
 }
 
 // Linear id of the calling wave in a launch of blocks of `wpb` waves (wave index wib in its block).

@@ -40,6 +40,8 @@ struct FvmArgs {
   T* R;
   unsigned long long* heavy;
   unsigned int* hcount;
+  const unsigned int* pflag = nullptr;  // sensor band flag of this stage (heavy_possible), and its generation
+  unsigned int pgen = 0;
 };
 
 __device__ __forceinline__ long long rix(const KP& kp, int i, int j, int k) {
@@ -126,6 +128,7 @@ __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L,
   // lane l handles the face between cells l-1 and l of the segment (face index f); its geometry is
   // requested before the row is staged
   const bool fact = l >= 4 && l < 4 + nface;
+  const bool hp = heavy_possible(a.pflag, a.pgen);  // some face of the stage may be off UP7
   const int f = f0 + l - 4;
   T gn[4] = {0, 0, 0, 0};
   if (fact) {
@@ -137,15 +140,17 @@ __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L,
     const long long o = cidx(kp, xc, j, k);
 #pragma unroll
     for (int n = 0; n < 5; ++n) sU[n][r][l] = (T)a.U[o + n * nc];
-    sphi[r][l] = a.phi[o];
+    sphi[r][l] = hp ? a.phi[o] : T(0);
   }
   __syncthreads();
   int sch = F3D_SCHEME_UP7;
   if (fact) {
-    T m = sphi[r][l - 3];
+    if (hp) {
+      T m = sphi[r][l - 3];
 #pragma unroll
-    for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
-    sch = scheme_of<T>(kp, m);
+      for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
+      sch = scheme_of<T>(kp, m);
+    }
     T Fl[5] = {0, 0, 0, 0, 0};
     if (sch == F3D_SCHEME_UP7) {
       T W[8][5];
@@ -218,9 +223,10 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
   T gn[4] = {0, 0, 0, 0};  // geometry of the next face to complete
   double nu[5];
   T nph;
+  const bool hp = heavy_possible(a.pflag, a.pgen);  // some face of the stage may be off UP7
 #pragma unroll
   for (int n = 0; n < 5; ++n) nu[n] = a.U[obase + n * nc];
-  nph = a.phi[obase];
+  nph = hp ? a.phi[obase] : T(0);
 #pragma unroll 1
   for (int step = 0; step < nstep; ++step) {
 #pragma unroll
@@ -236,7 +242,7 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
       const long long o = obase + (long long)(step + 1) * st;
 #pragma unroll
       for (int n = 0; n < 5; ++n) nu[n] = a.U[o + n * nc];
-      nph = a.phi[o];
+      if (hp) nph = a.phi[o];
     }
     T Rc[5];
 #pragma unroll
@@ -256,10 +262,13 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
       if (step >= 7) {  // face f0 + step - 7 complete
         const int fi = step - 7;
         const int f = f0 + fi;
-        T m = ph[1];
+        int sch = F3D_SCHEME_UP7;
+        if (hp) {
+          T m = ph[1];
 #pragma unroll
-        for (int s = 2; s <= 6; ++s) m = jmax(m, ph[s]);
-        const int sch = scheme_of<T>(kp, m);
+          for (int s = 2; s <= 6; ++s) m = jmax(m, ph[s]);
+          sch = scheme_of<T>(kp, m);
+        }
         T Fc[5] = {0, 0, 0, 0, 0};
         if (sch == F3D_SCHEME_UP7) {
           T UL[5], UR[5];
@@ -268,7 +277,8 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
 #pragma unroll
           for (int n = 0; n < 5; ++n) Fc[n] *= g3;
         }
-        run_append(a.heavy, hrun, hcnt, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
+        if (hp)
+          run_append(a.heavy, hrun, hcnt, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
                    ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) |
                        ((unsigned long long)sch << 2) | (unsigned long long)D);
         if (fi >= 1 && act) {
@@ -340,9 +350,10 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
   unsigned int xcnt = 0, ycnt = 0;
   double nu[5];
   T nph;
+  const bool hp = heavy_possible(a.pflag, a.pgen);  // some face of the stage may be off UP7
 #pragma unroll
   for (int n = 0; n < 5; ++n) nu[n] = a.U[obase + n * nc];
-  nph = a.phi[obase];
+  nph = hp ? a.phi[obase] : T(0);
 #pragma unroll 1
   for (int step = 0; step < nstep; ++step) {
 #pragma unroll
@@ -360,7 +371,7 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
       const long long o = obase + (long long)(step + 1) * st;
 #pragma unroll
       for (int n = 0; n < 5; ++n) nu[n] = a.U[o + n * nc];
-      nph = a.phi[o];
+      if (hp) nph = a.phi[o];
       if (fact && lineok && step + 1 >= 4 && step + 1 < 4 + ncell) {
         const long long go = fidx(kp, 0, ii, f0 + step + 1 - 4, tc);
 #pragma unroll
@@ -382,10 +393,13 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
       if (step >= 7) {  // y face f0 + step - 7 complete: it closes row f - 1
         const int fi = step - 7;
         const int f = f0 + fi;
-        T m = ph[1];
+        int sch = F3D_SCHEME_UP7;
+        if (hp) {
+          T m = ph[1];
 #pragma unroll
-        for (int q = 2; q <= 6; ++q) m = jmax(m, ph[q]);
-        const int sch = scheme_of<T>(kp, m);
+          for (int q = 2; q <= 6; ++q) m = jmax(m, ph[q]);
+          sch = scheme_of<T>(kp, m);
+        }
         T Fc[5] = {0, 0, 0, 0, 0};
         if (sch == F3D_SCHEME_UP7) {
           T UL[5], UR[5];
@@ -394,7 +408,8 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
 #pragma unroll
           for (int n = 0; n < 5; ++n) Fc[n] *= g3;
         }
-        run_append(a.heavy, yrun, ycnt, act && (fi < L || f == kp.Nyp) && sch != F3D_SCHEME_UP7,
+        if (hp)
+          run_append(a.heavy, yrun, ycnt, act && (fi < L || f == kp.Nyp) && sch != F3D_SCHEME_UP7,
                    ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) | 1ULL);
         if (fi >= 1 && act) {  // row f - 1: its x faces were formed four steps ago (slot step & 3)
           const long long ro = rix(kp, ii, f - 1, tc);
@@ -411,10 +426,12 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
       const int sl = step & 3;
       int sch = F3D_SCHEME_UP7;
       if (fact) {
-        T m = sphi[r][l - 3];
+        if (hp) {
+          T m = sphi[r][l - 3];
 #pragma unroll
-        for (int q = -2; q <= 2; ++q) m = jmax(m, sphi[r][l + q]);
-        sch = scheme_of<T>(kp, m);
+          for (int q = -2; q <= 2; ++q) m = jmax(m, sphi[r][l + q]);
+          sch = scheme_of<T>(kp, m);
+        }
         T Fl[5] = {0, 0, 0, 0, 0};
         if (sch == F3D_SCHEME_UP7) {
           T Wx[8][5];
@@ -432,7 +449,8 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
         for (int n = 0; n < 5; ++n) sF[sl][n][r][l] = Fl[n];
       }
       const int j = kp.NG + f0 + step - 4;
-      run_append(a.heavy, xrun, xcnt, lineok && fact && (ii < fx0 + FXI || ii == kp.Nxp) && sch != F3D_SCHEME_UP7,
+      if (hp)
+        run_append(a.heavy, xrun, xcnt, lineok && fact && (ii < fx0 + FXI || ii == kp.Nxp) && sch != F3D_SCHEME_UP7,
                  ((unsigned long long)cidx(kp, kp.NG + ii, j, k) << 4) | ((unsigned long long)sch << 2) | 0ULL);
     }
     __syncthreads();  // the row's U is read before the next row overwrites it
@@ -451,6 +469,7 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
 template <typename T, bool EIGEN>
 __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* __restrict__ Q, MetricPtrs<T> m,
                                                   HList hl, T* __restrict__ hflux, HIdx hi) {
+  if (!heavy_possible(a.pflag, a.pgen)) return;  // nothing listed this stage
   for_each_heavy_slot(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);  // right cell of the face
@@ -580,6 +599,8 @@ template <typename T>
 int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
              const T* const* geo, const T* phi, hipStream_t s) {
   FvmArgs<T> a{U, {geo[0], geo[1], geo[2]}, phi, ws.R, ws.heavy, ws.hcount};
+  a.pflag = ws.pgen ? ws.pflag : nullptr;
+  a.pgen = ws.pgen ? *ws.pgen : 0u;
   const HList hl = fvm_heavy_layout(kp, (int)sizeof(T));
   if (int rc = check_hlist(ws, hl, "fvm sweeps")) return rc;
   if (fvm_sweep_xy()) {
@@ -611,7 +632,7 @@ int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, con
       hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hi);
     else
       hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hi);
-    launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s);
+    launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s, a.pflag, a.pgen);
   }
   return check_launch("fvm sweeps");
 }

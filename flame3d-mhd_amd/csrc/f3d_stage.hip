@@ -29,6 +29,8 @@ struct f3d_workspace {
   void* hflux;    // heavy-face flux per list slot (bulk type, 5 per slot)
   unsigned long long* hidx;  // per-cell records of the listed faces (6 per interior cell)
   unsigned int hstamp;       // stamp of the last heavy-face evaluation
+  unsigned int* pflag;       // sensor generation of the last stage with a value outside the UP7 band
+  unsigned int pgen;         // sensor generations launched
 };
 
 namespace {
@@ -452,7 +454,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
   const size_t nc = f3d_cell_count(p);
   const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
-  size_t sz[20];
+  size_t sz[21];
   sz[0] = nc * es;                               // phi
   sz[1] = nc * 5 * es;                           // Fp
   sz[2] = nc * 5 * es;                           // Fm
@@ -480,6 +482,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[17] = nc * sizeof(double);                  // fsc
   sz[18] = hslots * 5 * es;                      // heavy-face flux per slot
   sz[19] = (size_t)hrec_count(kp0) * sizeof(unsigned long long);  // per-cell records of the listed faces
+  sz[20] = sizeof(unsigned int);                 // sensor band flag
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -490,15 +493,17 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   ws->bytes = total;
   ws->base = base;
   char* c = (char*)base;
-  void** slots[20] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+  void** slots[21] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
                       &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount,
-                      &ws->Qa, &ws->Qb, (void**)&ws->ldt, (void**)&ws->fsc, &ws->hflux, (void**)&ws->hidx};
-  for (int i = 0; i < 20; ++i) {
+                      &ws->Qa, &ws->Qb, (void**)&ws->ldt, (void**)&ws->fsc, &ws->hflux, (void**)&ws->hidx,
+                      (void**)&ws->pflag};
+  for (int i = 0; i < 21; ++i) {
     *slots[i] = c;
     c += align_up(sz[i]);
   }
   ws->nred = kRedBlocks;
   ws->hstamp = 0;  // the records start cleared (zero fill below): stamps count from 1
+  ws->pgen = 0;    // the flag starts cleared: generations count from 1
   ws->hcap = hslots;
   ws->hncap = hruns;
   // zeroed once: inviscid runs read zero viscous fluxes in div (div.jl:45-48 adds them always)
@@ -544,6 +549,8 @@ WsView<T> ws_view(f3d_workspace* ws) {
   v.hflux = (T*)ws->hflux;
   v.hidx = ws->hidx;
   v.hstamp = &ws->hstamp;
+  v.pflag = ws->pflag;
+  v.pgen = &ws->pgen;
   return v;
 }
 template WsView<double> ws_view<double>(f3d_workspace*);
@@ -633,7 +640,7 @@ NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   off += align_up(L.hcap * 5 * sizeof(double));
   L.hidx = off;
   L.nidx = hrec_count(kp);
-  off += align_up((size_t)L.nidx * sizeof(unsigned long long));
+  off += align_up((size_t)(L.nidx + 1) * sizeof(unsigned long long));  // the records, then the sensor band flag
   L.phi = off;
   if (with_phi) off += align_up(f3d_cell_count(p) * sizeof(double));
   L.total = off;
@@ -683,8 +690,11 @@ F3D_API int f3d_rhs_noalloc_f64(const f3d_params* p, double* R, const double* Q,
   v.hidx = (unsigned long long*)(c + L.hidx);
   unsigned int stamp = 0;  // records cleared below: this call's faces take stamp 1
   v.hstamp = &stamp;
+  unsigned int gen = 0;  // the flag word after the records, cleared with them: this call's sensor is generation 1
+  v.pflag = (unsigned int*)(v.hidx + L.nidx);
+  v.pgen = &gen;
   v.phi = phi ? phi : (double*)(c + L.phi);
-  clear_hidx(v.hidx, L.nidx, (hipStream_t)s);
+  clear_hidx(v.hidx, L.nidx + 1, (hipStream_t)s);
   MetricPtrs<double> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
   return fused_rhs<double>(kp, v, Q, m, J, v.phi, R, (hipStream_t)s);
