@@ -621,6 +621,9 @@ __global__ __launch_bounds__(512) void k_adj_visc_march(KP kp, const double* __r
   // rings: slot s holds plane p-2+s of Lam (vars 1..4) and of u,v,w,T (slot 5: plane p+3, requested a
   // step ahead); gz slot s the zeta part of G at plane p-4+s
   double L[4][6], F[4][6], gz[4][5];
+  // the xi / eta part (with the own G[12+v]) of planes p-2 and p-1, added to lamQ together with the zeta
+  // part of the same plane two iterations after it is formed: one read-modify-write of lamQ per cell
+  double pend[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   auto load = [&](int pl, int sl) {
     const long long o = colo + (long long)cl(pl, kp.Nzt) * sz;
 #pragma unroll
@@ -706,7 +709,8 @@ __global__ __launch_bounds__(512) void k_adj_visc_march(KP kp, const double* __r
     }
     __syncthreads();
     // plane p: own G[12+v] + transposed xi / eta stencils (d g[c] / d f[a] for c = a + r: r = +2 -> c1,
-    // -2 -> -c1, +1 -> c2, -1 -> -c2)
+    // -2 -> -c1, +1 -> c2, -1 -> -c2), kept two iterations
+    double accp[4] = {0, 0, 0, 0};
     if (outl && p >= kz0 && p < kz1) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -714,16 +718,21 @@ __global__ __launch_bounds__(512) void k_adj_visc_march(KP kp, const double* __r
         acc += c1 * (sG[v][row][col + 2] - sG[v][row][col - 2]) + c2 * (sG[v][row][col + 1] - sG[v][row][col - 1]);
         acc += c1 * (sG[4 + v][row + 2][col] - sG[4 + v][row - 2][col]) +
                c2 * (sG[4 + v][row + 1][col] - sG[4 + v][row - 1][col]);
-        lamQ[o + var[v] * nc] += acc;
+        accp[v] = acc;
       }
     }
-    // plane p-2: the transposed zeta stencil of the ring
+    // plane p-2: its xi / eta part plus the transposed zeta stencil of the ring
     const int q = p - 2;
     if (outl && q >= kz0 && q < kz1) {
       const long long oq = colo + (long long)q * sz;
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        lamQ[oq + var[v] * nc] += c1 * (gz[v][4] - gz[v][0]) + c2 * (gz[v][3] - gz[v][1]);
+        lamQ[oq + var[v] * nc] += pend[v][0] + (c1 * (gz[v][4] - gz[v][0]) + c2 * (gz[v][3] - gz[v][1]));
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      pend[v][0] = pend[v][1];
+      pend[v][1] = accp[v];
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -1151,7 +1160,7 @@ bool adj_visc_march() {
 }
 
 // Planes per block of k_adj_visc_march: one block per CU at a time, so the march takes about
-// rounds x (KC + 4) plane steps with rounds = ceil(blocks / CUs); the smallest such KC among a few.
+// rounds x (KC + 4) plane steps with rounds = ceil(blocks / CUs); the KC of equal segments minimising it.
 // F3D_ADJ_VISC_KC overrides.
 int adj_visc_kc(const KP& kp) {
   static int env = -2, ncu = 0;
@@ -1164,10 +1173,13 @@ int adj_visc_kc(const KP& kp) {
       ncu = 256;
   }
   if (env > 0) return env;
+  // the k range in nseg equal segments, nseg = 1 .. 32 (on the 256x128x64 ramp: 2 segments of 36 planes, one
+  // round of 240 blocks, 301 us per launch against 343 for 6 of 12 in three rounds)
   const long long tiles = (long long)((kp.Nxt + kAVW - 5) / (kAVW - 4)) * ((kp.Nyt + kAVR - 5) / (kAVR - 4));
-  int best = 16;
+  int best = kp.Nzt;
   long long cost = -1;
-  for (int kc : {8, 12, 16, 20, 24, 32, 48, 64}) {
+  for (int nseg = 1; nseg <= 32; ++nseg) {
+    const int kc = (kp.Nzt + nseg - 1) / nseg;
     const long long nb = tiles * ((kp.Nzt + kc - 1) / kc);
     const long long c = ((nb + ncu - 1) / ncu) * (kc + 4);
     if (cost < 0 || c < cost) {
