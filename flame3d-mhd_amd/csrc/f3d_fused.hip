@@ -8,7 +8,8 @@
 //                 no split flux ever goes to HBM; R += dF_y, dF_z (the R row loaded a step ahead)
 //   k_heavy       faces whose sensor maximum selects WENO7/WENO5/NND (appended to a compacted list
 //                 by the sweeps): split fluxes of the 8-cell stencil recomputed, characteristic
-//                 reconstruction (schemes.jl:756-1009), R += (F_scheme - F_UP7) on both cells
+//                 reconstruction (schemes.jl:756-1009), F_scheme - F_UP7 kept per face; k_heavy_apply
+//                 adds it to both cells (R += on the right cell, -= on the left, fixed order per cell)
 //   k_visc_update_qt viscous fluxes (viscous.jl:1-134), their divergence (div.jl:14-48), the
 //                 conservative update, RK3 blend and c2Prim (utils.jl:2-70) in one k-march; the
 //                 viscous fluxes stay in LDS and registers
@@ -487,14 +488,14 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
 
 // ------------------------------------------------------------------ heavy faces
 // Every listed face (all three directions, one launch): its characteristic WENO7/WENO5/NND flux
-// (schemes.jl:756-1009) minus the UP7 flux the sweep accumulated, kept in the face's list slot and recorded
-// in its cells' records; k_heavy_gather then adds each cell's corrections in a fixed order, so no
+// (schemes.jl:756-1009) minus the UP7 flux the sweep accumulated, kept at the face's position and marked in
+// its two cells (heavy_keep); k_heavy_apply then adds each cell's corrections in a fixed order, so no
 // floating-point atomics: the corrections are summed the same way on every evaluation.
 template <typename T, int SPLIT, bool CHAR>
-__global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux, HIdx hi) {
+__global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __restrict__ hcorr, HMark hm) {
   if (!heavy_possible(a.pflag, a.pgen)) return;  // nothing listed this stage
   const long long nc = kp.nc;
-  for_each_heavy_slot(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
+  for_each_heavy(hl, a.hcount, a.heavy, [&](const unsigned long long v) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);
     const long long st = dstride(kp, D);
@@ -536,11 +537,10 @@ __global__ __launch_bounds__(64) void k_heavy(KP kp, FArgs<T> a, HList hl, T* __
     const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
     const int f = D == 0 ? I : (D == 1 ? Jj : K);
     const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
-    const long long rr = ridx(kp, I, Jj, K);
+    T dF[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) hflux[slot * 5 + q] = Fs[q] - Fu[q];
-    heavy_record(kp, hi, D, rr, f >= 1, f <= Np - 1, slot);
-    a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
+    for (int q = 0; q < 5; ++q) dF[q] = Fs[q] - Fu[q];
+    heavy_keep(kp, hm, hcorr, D, I, Jj, K, f, Np, dF);
   });
 }
 
@@ -565,11 +565,11 @@ __device__ __forceinline__ double pair_swap(double v) {
 // face (each cell once), half the registers per lane (two or more waves per SIMD instead of one in FP64),
 // the same arithmetic per field (F+ part + F- part); the UP7 flux subtracted is the F+ sum plus the F- sum.
 template <typename T, int SPLIT, bool CHAR>
-__global__ __launch_bounds__(64) void k_heavy_pair(KP kp, FArgs<T> a, HList hl, T* __restrict__ hflux, HIdx hi) {
+__global__ __launch_bounds__(64) void k_heavy_pair(KP kp, FArgs<T> a, HList hl, T* __restrict__ hcorr, HMark hm) {
   if (!heavy_possible(a.pflag, a.pgen)) return;  // nothing listed this stage
   const long long nc = kp.nc;
   const bool odd = (threadIdx.x & 1) != 0;
-  for_each_heavy_pairs(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
+  for_each_heavy_pairs(hl, a.hcount, a.heavy, [&](const unsigned long long v, unsigned long long) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);
     const long long st = dstride(kp, D);
@@ -676,11 +676,10 @@ __global__ __launch_bounds__(64) void k_heavy_pair(KP kp, FArgs<T> a, HList hl, 
       const int I = ci - kp.NG, Jj = cj - kp.NG, K = ck - kp.NG;
       const int f = D == 0 ? I : (D == 1 ? Jj : K);
       const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
-      const long long rr = ridx(kp, I, Jj, K);
+      T dF[5];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) hflux[slot * 5 + q] = Fs[q] - Fu[q];
-      heavy_record(kp, hi, D, rr, f >= 1, f <= Np - 1, slot);
-      a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
+      for (int q = 0; q < 5; ++q) dF[q] = Fs[q] - Fu[q];
+      heavy_keep(kp, hm, hcorr, D, I, Jj, K, f, Np, dF);
     }
   });
 }
@@ -1503,18 +1502,18 @@ static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const
   {
     TimedScope ts("fused_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
-    const HIdx hi = next_hidx(kp, ws.hidx, ws.hstamp, s);
+    const HMark hm = next_hmark(kp, ws.hidx, ws.hstamp, s);
     if (heavy_pairs((int)sizeof(T))) {
       if (kp.character)
-        hipLaunchKernelGGL((k_heavy_pair<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
+        hipLaunchKernelGGL((k_heavy_pair<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hm);
       else
-        hipLaunchKernelGGL((k_heavy_pair<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
+        hipLaunchKernelGGL((k_heavy_pair<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hm);
     } else if (kp.character) {
-      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hm);
     } else {
-      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hi);
+      hipLaunchKernelGGL((k_heavy<T, SPLIT, false>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hm);
     }
-    launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s, a.pflag, a.pgen);
+    launch_heavy_apply<T>(kp, ws.R, hm, ws.hflux, s, a.pflag, a.pgen);
   }
   return check_launch("fused sweeps");
 }

@@ -17,9 +17,9 @@ struct WsView {
   unsigned long long hcap;   // capacity of `heavy` (slots)
   unsigned long long hncap;  // capacity of `hcount` (runs)
   double* ldt;               // per-cell dt of local time stepping, cell-shaped (written at stage 1)
-  T* hflux;                  // per list slot: the 5 flux values a heavy face adds to its cells (hcap x 5)
-  unsigned long long* hidx;  // per-cell records of the listed faces (hrec_count, see HIdx)
-  unsigned int* hstamp;      // host counter of the records' stamps
+  T* hflux;                  // per face position: the 5 values a listed face adds to its cells (heavy_face_count x 5)
+  unsigned long long* hidx;  // per-cell marks of the listed faces (hmark_count, see HMark)
+  unsigned int* hstamp;      // host counter of the marks' stamps
   unsigned int* pflag;       // device word: the sensor generation of the last stage in which some cell's
                              // sensor value left the UP7 band (no face can be off UP7 otherwise)
   unsigned int* pgen;        // host counter of the sensor generations
@@ -212,104 +212,108 @@ __device__ __forceinline__ void for_each_heavy(const HList& hl, const unsigned i
   for_each_heavy_slot(hl, counts, list, [&](const unsigned long long v, unsigned long long) { f(v); });
 }
 
-// The list entry the forming pass leaves for the gather pass: (interior index of the right cell << 4)
-// | (the face has a cell on its left << 3) | (a cell on its right << 2) | direction.
-__device__ __forceinline__ unsigned long long heavy_apply_entry(long long rr, bool left, bool right, int D) {
-  return ((unsigned long long)rr << 4) | ((unsigned long long)left << 3) | ((unsigned long long)right << 2) |
-         (unsigned long long)D;
-}
-
-// Per-cell record of the listed faces, for the gather pass: hrec[6 c + o] (c = interior index of the cell,
-// o = 2 d + side: its face along d on the left (side 0) or right (side 1)) holds (stamp << 36) | list slot.
-// A face writes the records of its two cells (distinct faces, distinct records: no conflicts).  Every
-// heavy-face evaluation takes a new stamp, so the records of earlier ones need no clearing (the array is
-// cleared when the 28-bit stamp wraps).
-constexpr int kHStampBits = 28;
-constexpr unsigned long long kHSlotMask = (1ULL << (64 - kHStampBits)) - 1;
-struct HIdx {
-  unsigned long long* rec;
-  unsigned long long stamp;  // stamp << (64 - kHStampBits)
+// Heavy-face corrections and the marks of the cells they enter.  The forming pass keeps each listed face's
+// F_scheme - F_UP7 at its face position (hcorr[5 f + q], f = face_id) and marks the face in its two cells:
+// byte o = 2 d + side of hmark[c] (c = interior index of the cell, side 0 = its left face along d, 1 = its
+// right face) takes the evaluation's 8-bit stamp (distinct faces write distinct bytes, no conflicts).
+// k_heavy_apply then walks the cells in order (coalesced mark reads; the RHS read and written only where a
+// mark holds the stamp) and adds a cell's corrections in the order o = 0..5 (+ for a left face, - for a
+// right face): one lane per cell and a fixed order, so no floating-point atomics and the same bits on every
+// evaluation.  Every evaluation takes a new stamp, so marks of earlier ones need no clearing (the array is
+// cleared when the stamp wraps); bytes 6 and 7 stay zero.
+constexpr unsigned int kHStampMax = 255;
+struct HMark {
+  unsigned long long* mark;
+  unsigned int stamp;  // 1 .. kHStampMax
 };
-__host__ __device__ inline long long hrec_count(const KP& kp) { return 6LL * kp.Nxp * kp.Nyp * kp.Nzp; }
-// Records the listed face of direction D held in `slot`: the left face of interior cell rr (when the face has a
-// cell on its right) and the right face of the cell before it along D (when it has one on its left).
-__device__ __forceinline__ void heavy_record(const KP& kp, const HIdx& hi, int D, long long rr, bool left, bool right,
-                                             unsigned long long slot) {
-  const unsigned long long e = hi.stamp | slot;
-  if (right) hi.rec[6 * rr + 2 * D] = e;
-  if (left) {
+__host__ __device__ inline long long hmark_count(const KP& kp) { return (long long)kp.Nxp * kp.Nyp * kp.Nzp; }
+// Faces of the three directions: (Nxp + 1) Nyp Nzp along x, then Nxp (Nyp + 1) Nzp along y, Nxp Nyp (Nzp + 1)
+// along z.
+__host__ __device__ inline long long heavy_face_count(const KP& kp) {
+  const long long X = kp.Nxp, Y = kp.Nyp, Z = kp.Nzp;
+  return (X + 1) * Y * Z + X * (Y + 1) * Z + X * Y * (Z + 1);
+}
+// Position of the face of direction D left of interior cell (I, J, K); the coordinate along D runs to Np (the
+// right boundary face).
+__host__ __device__ inline long long face_id(const KP& kp, int D, int I, int J, int K) {
+  const long long X = kp.Nxp, Y = kp.Nyp, Z = kp.Nzp;
+  if (D == 0) return I + (X + 1) * (J + Y * (long long)K);
+  if (D == 1) return (X + 1) * Y * Z + I + X * (J + (Y + 1) * (long long)K);
+  return (X + 1) * Y * Z + X * (Y + 1) * Z + I + X * (J + Y * (long long)K);
+}
+// Keeps the correction of the listed face of direction D left of interior position (I, J, K) (f = its
+// coordinate along D, Np the cells along D) and marks it in the cell right of it (f < Np) and the cell left of
+// it (f > 0).
+template <typename T>
+__device__ __forceinline__ void heavy_keep(const KP& kp, const HMark& hm, T* __restrict__ corr, int D, int I, int J,
+                                           int K, int f, int Np, const T* dF) {
+  const long long fid = face_id(kp, D, I, J, K);
+#pragma unroll
+  for (int q = 0; q < 5; ++q) corr[fid * 5 + q] = dF[q];
+  unsigned char* mb = reinterpret_cast<unsigned char*>(hm.mark);
+  const long long rr = (long long)I + (long long)kp.Nxp * ((long long)J + (long long)kp.Nyp * K);
+  if (f < Np) mb[8 * rr + 2 * D] = (unsigned char)hm.stamp;
+  if (f > 0) {
     const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
-    hi.rec[6 * rl + 2 * D + 1] = e;
+    mb[8 * rl + 2 * D + 1] = (unsigned char)hm.stamp;
+  }
+}
+// Adds the corrections of interior cell (I, J, K), whose mark word is mw, to its RHS Rt, in the order x left,
+// x right, y left, y right, z left, z right.
+template <typename T>
+__device__ __forceinline__ void heavy_add_mark(const KP& kp, unsigned long long mw, const T* __restrict__ corr,
+                                               unsigned int stamp, int I, int J, int K, T* Rt) {
+  const unsigned long long x = mw ^ (0x0101010101010101ULL * stamp);
+  if ((((x - 0x0101010101010101ULL) & ~x) & 0x8080808080808080ULL) == 0) return;  // no byte holds the stamp
+#pragma unroll
+  for (int o = 0; o < 6; ++o) {
+    if (((mw >> (8 * o)) & 0xffULL) != stamp) continue;
+    const int D = o >> 1, sd = o & 1;
+    const long long f = face_id(kp, D, I + (D == 0 ? sd : 0), J + (D == 1 ? sd : 0), K + (D == 2 ? sd : 0));
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Rt[q] = sd ? Rt[q] - corr[f * 5 + q] : Rt[q] + corr[f * 5 + q];
   }
 }
 
-// The correction kept per list slot added to the RHS, one cell at a time: each listed face looks at its
-// two cells; a cell is taken by the first of its listed faces in the order (x left, x right, y left,
-// y right, z left, z right), whose lane adds all of the cell's corrections in that order (a face's flux
-// enters the cell right of it with +, the cell left of it with -).  One lane per cell and a fixed
-// summation order: no floating-point atomics, the same bits on every evaluation.
+// The corrections of every marked cell added to the RHS R (interior layout [5][Nzp][Nyp][Nxp]), one lane per
+// cell in index order.
 template <typename T>
-__global__ __launch_bounds__(64) void k_heavy_gather(KP kp, T* __restrict__ R, HList hl,
-                                                     const unsigned int* __restrict__ counts,
-                                                     const unsigned long long* __restrict__ list,
-                                                     const T* __restrict__ hflux, HIdx hi, const unsigned int* pflag,
-                                                     unsigned int pgen) {
+__global__ __launch_bounds__(256) void k_heavy_apply(KP kp, T* __restrict__ R, const unsigned long long* __restrict__ mark,
+                                                     const T* __restrict__ corr, unsigned int stamp,
+                                                     const unsigned int* pflag, unsigned int pgen) {
   if (!heavy_possible(pflag, pgen)) return;  // nothing listed this stage
-  const long long rn = (long long)kp.Nxp * kp.Nyp * kp.Nzp;
-  for_each_heavy_slot(hl, counts, list, [&](const unsigned long long v, unsigned long long) {
-    const int D = (int)(v & 3);
-    const long long rr = (long long)(v >> 4);  // interior index of the cell right of the face
+  const long long rn = hmark_count(kp), sxy = (long long)kp.Nxp * kp.Nyp;
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < rn; c += (long long)gridDim.x * 256) {
+    const unsigned long long mw = mark[c];
+    const unsigned long long x = mw ^ (0x0101010101010101ULL * stamp);
+    if ((((x - 0x0101010101010101ULL) & ~x) & 0x8080808080808080ULL) == 0) continue;
+    const int I = (int)(c % kp.Nxp), J = (int)((c / kp.Nxp) % kp.Nyp), K = (int)(c / sxy);
+    T Rt[5];
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      if (!(v & (side == 0 ? 4ULL : 8ULL))) continue;
-      const long long c = side == 0 ? rr : rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
-      const ulonglong2* r2 = reinterpret_cast<const ulonglong2*>(hi.rec + 6 * c);  // 48 B per cell, 16-B aligned
-      const ulonglong2 e01 = r2[0], e23 = r2[1], e45 = r2[2];
-      const unsigned long long e[6] = {e01.x, e01.y, e23.x, e23.y, e45.x, e45.y};
-      long long sl[6];
+    for (int q = 0; q < 5; ++q) Rt[q] = R[c + q * rn];
+    heavy_add_mark(kp, mw, corr, stamp, I, J, K, Rt);
 #pragma unroll
-      for (int o = 0; o < 6; ++o) sl[o] = (e[o] & ~kHSlotMask) == hi.stamp ? (long long)(e[o] & kHSlotMask) : -1;
-      const int mine = 2 * D + side;  // this face's place in the cell's order
-      bool owner = true;
-#pragma unroll
-      for (int o = 0; o < 6; ++o)
-        if (o < mine && sl[o] >= 0) owner = false;
-      if (!owner) continue;
-      T acc[5];
-#pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] = R[c + q * rn];
-#pragma unroll
-      for (int o = 0; o < 6; ++o) {
-        if (sl[o] < 0) continue;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) acc[q] = (o & 1) ? acc[q] - hflux[sl[o] * 5 + q] : acc[q] + hflux[sl[o] * 5 + q];
-      }
-#pragma unroll
-      for (int q = 0; q < 5; ++q) R[c + q * rn] = acc[q];
-    }
-  });
+    for (int q = 0; q < 5; ++q) R[c + q * rn] = Rt[q];
+  }
+}
+template <typename T>
+inline void launch_heavy_apply(const KP& kp, T* R, const HMark& hm, const T* corr, hipStream_t s,
+                               const unsigned int* pflag, unsigned int pgen) {
+  const long long nb = (hmark_count(kp) + 255) / 256;
+  hipLaunchKernelGGL((k_heavy_apply<T>), dim3((unsigned int)(nb < 8192 ? (nb ? nb : 1) : 8192)), dim3(256), 0, s, kp,
+                     R, hm.mark, corr, hm.stamp, pflag, pgen);
 }
 
-// Clears n records (stamp wrap; the no-allocation entry's scratch).
+// Clears n 8-byte words (stamp wrap; the no-allocation entry's scratch).
 void clear_hidx(unsigned long long* idx, long long n, hipStream_t s);
 
-// A new stamp for one heavy-face evaluation (the hrec_count(kp) records are cleared when the stamp wraps).
-inline HIdx next_hidx(const KP& kp, unsigned long long* rec, unsigned int* stamp, hipStream_t s) {
-  if (++*stamp >= (1u << kHStampBits)) {
-    clear_hidx(rec, hrec_count(kp), s);
+// A new stamp for one heavy-face evaluation (the hmark_count(kp) marks are cleared when the stamp wraps).
+inline HMark next_hmark(const KP& kp, unsigned long long* mark, unsigned int* stamp, hipStream_t s) {
+  if (++*stamp > kHStampMax) {
+    clear_hidx(mark, hmark_count(kp), s);
     *stamp = 1;
   }
-  return HIdx{rec, (unsigned long long)*stamp << (64 - kHStampBits)};
-}
-
-// Host side: the gather pass over the runs of all three directions.
-template <typename T>
-inline void launch_heavy_gather(const KP& kp, T* R, const HList& hl, const unsigned int* counts,
-                                const unsigned long long* list, const T* hflux, const HIdx& hi, hipStream_t s,
-                                const unsigned int* pflag = nullptr, unsigned int pgen = 0) {
-  const unsigned int it = hl.items();
-  const unsigned int nb = it < 8192u ? (it ? it : 1u) : 8192u;
-  hipLaunchKernelGGL((k_heavy_gather<T>), dim3(nb), dim3(64), 0, s, kp, R, hl, counts, list, hflux, hi, pflag, pgen);
+  return HMark{mark, *stamp};
 }
 
 // Linear id of the calling wave in a launch of blocks of `wpb` waves (wave index wib in its block).

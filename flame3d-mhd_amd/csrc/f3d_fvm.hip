@@ -16,7 +16,7 @@
 //   k_fvm_march      y and z faces: lanes march along the direction with the last 8 cells' U in a
 //                    register window, next cell and next face geometry prefetched; R += dF_y, dF_z
 //   k_fvm_heavy      the listed faces: full characteristic / component-wise reconstruction + HLLC,
-//                    kept in the face's list slot, then added to the two cells by k_heavy_gather
+//                    kept at the face's position, then added to the two cells by k_heavy_apply
 //                    (fixed order per cell, no floating-point atomics)
 //   fused_update     viscous divergence + update + RK3 blend + c2Prim (f3d_fused.hip)
 #include <cmath>
@@ -463,14 +463,14 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
 
 // ------------------------------------------------------------------ heavy faces
 // Every listed face (one launch, all directions): the full characteristic / component-wise
-// reconstruction + HLLC flux (the sweep gave these faces zero), kept in the face's list slot and recorded
-// in its cells' records; k_heavy_gather adds each cell's fluxes in a fixed order (no floating-point
-// atomics: the same summation order on every evaluation).
+// reconstruction + HLLC flux (the sweep gave these faces zero), kept at the face's position and marked in its
+// two cells (heavy_keep); k_heavy_apply adds each cell's fluxes in a fixed order (no floating-point atomics:
+// the same summation order on every evaluation).
 template <typename T, bool EIGEN>
 __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* __restrict__ Q, MetricPtrs<T> m,
-                                                  HList hl, T* __restrict__ hflux, HIdx hi) {
+                                                  HList hl, T* __restrict__ hcorr, HMark hm) {
   if (!heavy_possible(a.pflag, a.pgen)) return;  // nothing listed this stage
-  for_each_heavy_slot(hl, a.hcount, a.heavy, [&](const unsigned long long v, const unsigned long long slot) {
+  for_each_heavy(hl, a.hcount, a.heavy, [&](const unsigned long long v) {
     const int D = (int)(v & 3), sch = (int)((v >> 2) & 3);
     const long long o = (long long)(v >> 4);  // right cell of the face
     const FvmDir dd = fvm_dir(kp, D);
@@ -486,11 +486,7 @@ __global__ __launch_bounds__(64) void k_fvm_heavy(KP kp, FvmArgs<T> a, const T* 
     fvm_face_flux_n<T, EIGEN>(kp, sch, Q, a.U, oc, dd.st, n3, g[go + 3 * gs], Sc, Sc1, F);
     const int f = D == 0 ? I : (D == 1 ? Jj : K);
     const int Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
-    const long long rr = rix(kp, I, Jj, K);
-#pragma unroll
-    for (int q = 0; q < 5; ++q) hflux[slot * 5 + q] = F[q];
-    heavy_record(kp, hi, D, rr, f >= 1, f <= Np - 1, slot);
-    a.heavy[slot] = heavy_apply_entry(rr, f >= 1, f <= Np - 1, D);
+    heavy_keep(kp, hm, hcorr, D, I, Jj, K, f, Np, F);
   });
 }
 
@@ -627,12 +623,12 @@ int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, con
   {
     TimedScope ts("fvm_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
-    const HIdx hi = next_hidx(kp, ws.hidx, ws.hstamp, s);
+    const HMark hm = next_hmark(kp, ws.hidx, ws.hstamp, s);
     if (kp.fvm == F3D_FVM_EIGEN)
-      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hi);
+      hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hm);
     else
-      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hi);
-    launch_heavy_gather<T>(kp, ws.R, hl, ws.hcount, ws.heavy, ws.hflux, hi, s, a.pflag, a.pgen);
+      hipLaunchKernelGGL((k_fvm_heavy<T, false>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hm);
+    launch_heavy_apply<T>(kp, ws.R, hm, ws.hflux, s, a.pflag, a.pgen);
   }
   return check_launch("fvm sweeps");
 }

@@ -26,8 +26,8 @@ struct f3d_workspace {
   void *Qa, *Qb;  // stage outputs of the fused path (primitive state, 6 vars, bulk type)
   double* ldt;    // per-cell dt of local time stepping (cell-shaped; compute_dt at stage 1)
   double* fsc;    // sensor of the nonlinear filter (cell-shaped)
-  void* hflux;    // heavy-face flux per list slot (bulk type, 5 per slot)
-  unsigned long long* hidx;  // per-cell records of the listed faces (6 per interior cell)
+  void* hflux;    // heavy-face corrections per face position (bulk type, 5 per face)
+  unsigned long long* hidx;  // per-cell marks of the listed faces (one word per interior cell)
   unsigned int hstamp;       // stamp of the last heavy-face evaluation
   unsigned int* pflag;       // sensor generation of the last stage with a value outside the UP7 band
   unsigned int pgen;         // sensor generations launched
@@ -480,8 +480,8 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[14] = sz[15] = nc * 6 * es;                 // Qa, Qb
   sz[16] = nc * sizeof(double);                  // ldt
   sz[17] = nc * sizeof(double);                  // fsc
-  sz[18] = hslots * 5 * es;                      // heavy-face flux per slot
-  sz[19] = (size_t)hrec_count(kp0) * sizeof(unsigned long long);  // per-cell records of the listed faces
+  sz[18] = (size_t)heavy_face_count(kp0) * 5 * es;                 // heavy-face corrections per face
+  sz[19] = (size_t)hmark_count(kp0) * sizeof(unsigned long long);  // per-cell marks of the listed faces
   sz[20] = sizeof(unsigned int);                 // sensor band flag
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
@@ -502,7 +502,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
     c += align_up(sz[i]);
   }
   ws->nred = kRedBlocks;
-  ws->hstamp = 0;  // the records start cleared (zero fill below): stamps count from 1
+  ws->hstamp = 0;  // the marks start cleared (zero fill below): stamps count from 1
   ws->pgen = 0;    // the flag starts cleared: generations count from 1
   ws->hcap = hslots;
   ws->hncap = hruns;
@@ -578,11 +578,11 @@ F3D_API int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, cons
 // accumulator, the heavy-face runs and their counts, and phi when the caller passes none) carved from
 // one file-scope device array instead of a workspace.  Each GPU the library runs on holds its own copy
 // of the array (kNoAllocScratchBytes) until the process ends; its address is looked up once per device.
-// Only the per-cell records of the heavy faces are cleared on every call (by a fill kernel, since the
+// Only the per-cell marks of the heavy faces are cleared on every call (by a fill kernel, since the
 // previous call may have laid the array out for another grid); everything else is written before it is
 // read: the x+y march (or the x sweep) writes every accumulator entry before the z march adds to it, each
-// sweep wave writes its run's count, the heavy faces' forming pass writes the flux slots the gather pass
-// reads, and the sensor writes phi.
+// sweep wave writes its run's count, the heavy faces' forming pass writes the corrections of the faces it
+// marks (the update reads only those), and the sensor writes phi.
 namespace {
 constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
 constexpr int kMaxDevices = 64;
@@ -621,7 +621,7 @@ int scratch_address(void** out, const char* who) {
 struct NoAllocLayout {
   size_t R, heavy, hcount, hflux, hidx, phi, total;
   unsigned long long hcap, hncap;
-  long long nidx;  // records of the listed faces
+  long long nidx;  // marks of the listed faces (one word per interior cell)
 };
 NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   const KP kp = make_kp(p);
@@ -637,10 +637,10 @@ NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   L.hcount = off;
   off += align_up(L.hncap * sizeof(unsigned int));
   L.hflux = off;
-  off += align_up(L.hcap * 5 * sizeof(double));
+  off += align_up((size_t)heavy_face_count(kp) * 5 * sizeof(double));
   L.hidx = off;
-  L.nidx = hrec_count(kp);
-  off += align_up((size_t)(L.nidx + 1) * sizeof(unsigned long long));  // the records, then the sensor band flag
+  L.nidx = hmark_count(kp);
+  off += align_up((size_t)(L.nidx + 1) * sizeof(unsigned long long));  // the marks, then the sensor band flag
   L.phi = off;
   if (with_phi) off += align_up(f3d_cell_count(p) * sizeof(double));
   L.total = off;
@@ -688,9 +688,9 @@ F3D_API int f3d_rhs_noalloc_f64(const f3d_params* p, double* R, const double* Q,
   v.hncap = L.hncap;
   v.hflux = (double*)(c + L.hflux);
   v.hidx = (unsigned long long*)(c + L.hidx);
-  unsigned int stamp = 0;  // records cleared below: this call's faces take stamp 1
+  unsigned int stamp = 0;  // marks cleared below: this call's faces take stamp 1
   v.hstamp = &stamp;
-  unsigned int gen = 0;  // the flag word after the records, cleared with them: this call's sensor is generation 1
+  unsigned int gen = 0;  // the flag word after the marks, cleared with them: this call's sensor is generation 1
   v.pflag = (unsigned int*)(v.hidx + L.nidx);
   v.pgen = &gen;
   v.phi = phi ? phi : (double*)(c + L.phi);
