@@ -393,6 +393,149 @@ __global__ __launch_bounds__(256) void k_fill(KP kp, Range3 r, T* __restrict__ Q
   }
 }
 
+// All ghost cells in one launch.  fill_d of x, then y, then z rewrites its whole slab (the other two
+// coordinates over the full array, ghosts included), reading only cells of the same line whose coordinate
+// along d is interior (periodic and zero-gradient sources, wall mirrors) or nothing (Dirichlet, inflow).  So
+// the final value of a ghost cell is: a z ghost, the z map of the final value of its source (whose x/y ghost
+// coordinates, if any, are resolved the same way); a y ghost with interior k, the y map of its source's final
+// value; an x ghost with interior j and k, the x map of interior cells.  k_fill_all evaluates that
+// composition for the disjoint union of the z slab, the y slab at interior k and the x slab at interior j
+// and k, one cell per lane: the same values, with the same arithmetic, as the three k_fill launches, without
+// the intermediate writes the later fills overwrite.
+template <typename T>
+struct GhostVal {
+  T q[6];
+  double u[5];
+};
+
+template <typename T>
+__device__ __forceinline__ GhostVal<T> ghost_cell(const KP& kp, const T* Q, const double* U, int i, int j, int k) {
+  GhostVal<T> v;
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) v.q[n] = Q[o + n * nc];
+  if (U) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) v.u[n] = U[o + n * nc];
+  }
+  return v;
+}
+
+// fill_d of direction D (k_fill's expressions) for the cell whose coordinate along D is the ghost g, with
+// (j, k) its other coordinates as k_fill reads them for the inflow plane; src(t) = final values of the cell
+// at coordinate t along D on the same line.
+template <typename T, int D, typename Src>
+__device__ __forceinline__ GhostVal<T> ghost_map(const KP& kp, const T* __restrict__ inflow, bool withU, int g, int j,
+                                                 int k, Src&& src) {
+  const int NG = kp.NG, Np = D == 0 ? kp.Nxp : (D == 1 ? kp.Nyp : kp.Nzp);
+  const int side = g >= NG ? 1 : 0;
+  const int bc = kp.bc[2 * D + side];
+  double q[6];
+  if (bc == F3D_BC_PERIODIC || bc == F3D_BC_ZEROGRAD) {
+    return src(bc == F3D_BC_PERIODIC ? (side ? g - Np : g + Np) : (side ? Np + NG - 1 : NG));
+  } else if (bc == F3D_BC_DIRICHLET) {
+#pragma unroll
+    for (int n = 0; n < 6; ++n) q[n] = kp.qinf[n];
+  } else if (bc == F3D_BC_INFLOW) {
+    // k_fill's slab index along D stands in the inflow plane index for y and z boundaries
+    const int jj = D == 1 ? (side ? g - Np : g) : j, kk = D == 2 ? (side ? g - Np : g) : k;
+    const long long io = jj + (long long)kp.Nyt * kk, in = (long long)kp.Nyt * kp.Nzt;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) q[n] = inflow[io + n * in];
+  } else {
+    const GhostVal<T> sv = src(side ? 2 * (Np + NG) - 1 - g : 2 * NG - 1 - g);
+    q[0] = sv.q[0];
+    q[1] = -(double)sv.q[1];
+    q[2] = -(double)sv.q[2];
+    q[3] = -(double)sv.q[3];
+    q[4] = sv.q[4];
+    q[5] = sv.q[5];
+    if (bc == F3D_BC_WALL_ISOTHERMAL) {
+      q[5] = 2 * kp.Twall - q[5];
+      q[0] = q[4] / (kp.Rg * q[5]);
+    }
+  }
+  GhostVal<T> v;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) v.q[n] = (T)q[n];
+  if (withU) {
+    const double rho = v.q[0], u = v.q[1], w1 = v.q[2], w2 = v.q[3], p = v.q[4];
+    v.u[0] = rho;
+    v.u[1] = rho * u;
+    v.u[2] = rho * w1;
+    v.u[3] = rho * w2;
+    v.u[4] = p / (kp.gamma - 1) + 0.5 * rho * (u * u + w1 * w1 + w2 * w2);
+  }
+  return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_all(KP kp, T* __restrict__ Q, double* __restrict__ U,
+                                                  const T* __restrict__ inflow, long long nz, long long ny,
+                                                  long long nx) {
+  const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int NG = kp.NG, G2 = 2 * NG;
+  int i, j, k;
+  if (id < nz) {  // z slab: every (i, j)
+    const long long r = id / kp.Nxt;
+    const int kk = (int)(r / kp.Nyt);
+    i = (int)(id % kp.Nxt);
+    j = (int)(r % kp.Nyt);
+    k = kk < NG ? kk : kk - NG + kp.Nzp + NG;
+  } else if (id < nz + ny) {  // y slab at interior k
+    const long long e = id - nz, r = e / kp.Nxt;
+    const int jj = (int)(r % G2);
+    i = (int)(e % kp.Nxt);
+    j = jj < NG ? jj : jj - NG + kp.Nyp + NG;
+    k = NG + (int)(r / G2);
+  } else {  // x slab at interior j and k
+    const long long e = id - nz - ny;
+    if (e >= nx) return;
+    const long long r = e / G2;
+    const int ii = (int)(e % G2);
+    i = ii < NG ? ii : ii - NG + kp.Nxp + NG;
+    j = NG + (int)(r % kp.Nyp);
+    k = NG + (int)(r / kp.Nyp);
+  }
+  const bool withU = U != nullptr;
+  auto xg = [&](int c) { return c < NG || c >= NG + kp.Nxp; };
+  auto yg = [&](int c) { return c < NG || c >= NG + kp.Nyp; };
+  auto fx = [&](int ci, int cj, int ck) {  // x ghost ci, interior cj and ck
+    return ghost_map<T, 0>(kp, inflow, withU, ci, cj, ck, [&](int t) { return ghost_cell<T>(kp, Q, U, t, cj, ck); });
+  };
+  auto fy = [&](int ci, int cj, int ck) {  // y ghost cj, interior ck
+    return ghost_map<T, 1>(kp, inflow, withU, cj, cj, ck, [&](int t) {
+      return xg(ci) ? fx(ci, t, ck) : ghost_cell<T>(kp, Q, U, ci, t, ck);
+    });
+  };
+  GhostVal<T> v;
+  if (k < NG || k >= NG + kp.Nzp) {
+    v = ghost_map<T, 2>(kp, inflow, withU, k, j, k, [&](int t) {
+      return yg(j) ? fy(i, j, t) : (xg(i) ? fx(i, j, t) : ghost_cell<T>(kp, Q, U, i, j, t));
+    });
+  } else if (yg(j)) {
+    v = fy(i, j, k);
+  } else {
+    v = fx(i, j, k);
+  }
+  const long long o = cidx(kp, i, j, k), nc = kp.nc;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) Q[o + n * nc] = v.q[n];
+  if (withU) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) U[o + n * nc] = v.u[n];
+  }
+}
+
+// 0 when F3D_FILL_SPLIT=1: the three per-direction fills instead of k_fill_all (A/B measurement).
+static bool fill_all() {
+  static const bool v = [] {
+    const char* e = std::getenv("F3D_FILL_SPLIT");
+    return !(e && std::atoi(e) == 1);
+  }();
+  return v;
+}
+
 // ------------------------------------------------------------------ host launchers
 template <typename T>
 int launch_shock_sensor(const KP& kp, T* phi, const T* Q, hipStream_t s) {
@@ -603,6 +746,14 @@ int launch_exchange(const KP& kp, T* Q, int NV, hipStream_t s) {
 template <typename T>
 int launch_fill(const KP& kp, T* Q, double* U, const T* inflow, hipStream_t s) {
   TimedScope ts_("fill_ghost", s);
+  if (fill_all()) {
+    const long long G2 = 2 * kp.NG;
+    const long long nz = (long long)kp.Nxt * kp.Nyt * G2, ny = (long long)kp.Nxt * G2 * kp.Nzp,
+                    nx = G2 * kp.Nyp * kp.Nzp;
+    const long long nb = (nz + ny + nx + 255) / 256;
+    hipLaunchKernelGGL(k_fill_all<T>, dim3((unsigned int)nb), dim3(256), 0, s, kp, Q, U, inflow, nz, ny, nx);
+    return check_launch("fill_ghost");
+  }
   for (int d = 0; d < 3; ++d) {
     Range3 r = slab_range(kp, d);
     if (d == 0) hipLaunchKernelGGL((k_fill<T, 0>), slab_grid_x(r), kBlockX, 0, s, kp, r, Q, U, inflow);

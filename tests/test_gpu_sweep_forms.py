@@ -47,9 +47,9 @@ def _inputs(tmp_path, case, dtype):
     return src, dst
 
 
-def _child(src, dst, switch):
+def _child(src, dst, switch, value="0"):
     env = {k: v for k, v in os.environ.items() if not k.startswith("F3D_")}
-    env[switch] = "0"
+    env[switch] = value
     r = subprocess.run([sys.executable, os.path.join(HERE, "sweep_child.py"), src, dst], env=env,
                        capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, f"child exit {r.returncode}\n{r.stderr[-2000:]}"
@@ -106,3 +106,24 @@ def test_fvm_sweep_forms_agree(tmp_path, name, dtype):
         print(f"  {k}: bit-identical {same}, normwise {relerr(mine[k], ref[k]):.2e}")
         assert same, k
     assert np.any(ref["R"] != 0)
+
+
+@pytest.mark.parametrize("fvm", [0, 1], ids=["fdm", "fvm"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32b"])
+@pytest.mark.parametrize("name", ["ramp70x30x9", "swept20x12x10", "tgv16"])
+def test_fill_forms_agree(tmp_path, name, dtype, fvm):
+    """The one-launch ghost fill (k_fill_all, the default) against the x, y and z fills in turn
+    (F3D_FILL_SPLIT=1): the same values bit for bit, Q and (finite-volume branch) U, after three steps of
+    inflow, outflow, isothermal-wall and periodic boundaries (the ramps) and periodic ones (the box)."""
+    import torch
+    import sweep_child
+    assert torch.cuda.is_available(), "GPU test"
+    case = _case(name, dtype, fvm=f3d.FVM_EIGEN if fvm else 0)
+    src, dst = _inputs(tmp_path, case, dtype)
+    mine = sweep_child.run(f3d, torch, np.load(src))
+    print(f"{'fvm' if fvm else 'fdm'} {name} {np.dtype(dtype).name}: child with F3D_FILL_SPLIT=1", flush=True)
+    ref = _child(src, dst, "F3D_FILL_SPLIT", "1")
+    for k in ("R", "U", "Un", "Q"):
+        same = np.array_equal(mine[k], ref[k])
+        print(f"  {k}: bit-identical {same}")
+        assert same, k
