@@ -264,19 +264,26 @@ __global__ __launch_bounds__(256) void k_adj_inv_z(KP kp, const double* __restri
   unsigned char fnext = k0 < k1 ? hr.flag[col + (long long)k0 * sz] : 0;  // coverage flag, a step ahead
 #pragma unroll
   for (int r = 0; r < 8; ++r) load(k0 - 4 + r, Lw[r]);
+  // the cell's state and zeta metrics, requested a step ahead (their latency overlaps the step before)
+  double nq[5] = {0, 0, 0, 0, 0}, na[3] = {0, 0, 0};
+  auto load_cell = [&](long long oc) {
+    nq[0] = Q[oc]; nq[1] = Q[oc + nc]; nq[2] = Q[oc + 2 * nc]; nq[3] = Q[oc + 3 * nc]; nq[4] = Q[oc + 5 * nc];
+    na[0] = mp.m[6][oc]; na[1] = mp.m[7][oc]; na[2] = mp.m[8][oc];
+  };
+  if (zcol && k0 < k1) load_cell(col + (long long)k0 * sz);
 #pragma unroll 1
   for (int k = k0; k < k1; ++k) {
     load(k + 4, Lw[8]);
     const long long o = col + (long long)k * sz;
     const bool covered = fnext == f8;
     if (k + 1 < k1) fnext = hr.flag[o + sz];
+    const double q0 = nq[0], q1 = nq[1], q2 = nq[2], q3 = nq[3], q5 = nq[4], A1 = na[0], A2 = na[1], A3 = na[2];
+    if (zcol && k + 1 < k1) load_cell(o + sz);
     double g[5] = {0, 0, 0, 0, 0};
     if (zcol) {
       double gp[5], gm[5];
       adj_gather(kp, [&](int r, int n) { return Lw[r][n]; }, gp, gm);
-      const double A1 = mp.m[6][o], A2 = mp.m[7][o], A3 = mp.m[8][o];
-      sw_split_vjp(kp, Q[o], Q[o + nc], Q[o + 2 * nc], Q[o + 3 * nc], Q[o + 5 * nc], A1, A2, A3,
-                   sqrt(A1 * A1 + A2 * A2 + A3 * A3), gp, gm, g);
+      sw_split_vjp(kp, q0, q1, q2, q3, q5, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), gp, gm, g);
     }
     // the listed faces' kept contributions to this cell, in the order z, x, y (k_adj_inv_xy adds the dense
     // x and y shares afterwards)
@@ -329,15 +336,23 @@ __global__ __launch_bounds__(kAIW * kAIL) void k_adj_inv_xy(KP kp, const double*
     const bool jin = interior(j, kp.Nyp, NG);
 #pragma unroll
     for (int n = 0; n < 5; ++n) sL[n][ln][t] = Lw[4][n];
+    // the cell's operands are requested before the barrier, so that their latency overlaps the wait
+    const bool upd = out && kok && (jin || iin);
+    const long long o = row0 + (long long)j * sy;
+    double rho = 0, u = 0, v = 0, w = 0, Tt = 0, acc[5] = {0, 0, 0, 0, 0}, X[3] = {0, 0, 0}, Y[3] = {0, 0, 0};
+    if (upd) {
+      rho = Q[o]; u = Q[o + nc]; v = Q[o + 2 * nc]; w = Q[o + 3 * nc]; Tt = Q[o + 5 * nc];
+      acc[0] = lamQ[o]; acc[1] = lamQ[o + nc]; acc[2] = lamQ[o + 2 * nc]; acc[3] = lamQ[o + 3 * nc];
+      acc[4] = lamQ[o + 5 * nc];
+      if (jin) { X[0] = mp.m[0][o]; X[1] = mp.m[1][o]; X[2] = mp.m[2][o]; }
+      if (iin) { Y[0] = mp.m[3][o]; Y[1] = mp.m[4][o]; Y[2] = mp.m[5][o]; }
+    }
     __syncthreads();
-    if (out && kok && (jin || iin)) {
-      const long long o = row0 + (long long)j * sy;
-      const double rho = Q[o], u = Q[o + nc], v = Q[o + 2 * nc], w = Q[o + 3 * nc], Tt = Q[o + 5 * nc];
-      double acc[5] = {lamQ[o], lamQ[o + nc], lamQ[o + 2 * nc], lamQ[o + 3 * nc], lamQ[o + 5 * nc]};
+    if (upd) {
       if (jin) {  // x share: Lam of the row from the neighbouring lanes
         double gp[5], gm[5], g[5];
         adj_gather(kp, [&](int r, int n) { return sL[n][ln][t - 4 + r]; }, gp, gm);
-        const double A1 = mp.m[0][o], A2 = mp.m[1][o], A3 = mp.m[2][o];
+        const double A1 = X[0], A2 = X[1], A3 = X[2];
         sw_split_vjp(kp, rho, u, v, w, Tt, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), gp, gm, g);
 #pragma unroll
         for (int n = 0; n < 5; ++n) acc[n] += g[n];
@@ -345,7 +360,7 @@ __global__ __launch_bounds__(kAIW * kAIL) void k_adj_inv_xy(KP kp, const double*
       if (iin) {  // y share from the column window
         double gp[5], gm[5], g[5];
         adj_gather(kp, [&](int r, int n) { return Lw[r][n]; }, gp, gm);
-        const double A1 = mp.m[3][o], A2 = mp.m[4][o], A3 = mp.m[5][o];
+        const double A1 = Y[0], A2 = Y[1], A3 = Y[2];
         sw_split_vjp(kp, rho, u, v, w, Tt, A1, A2, A3, sqrt(A1 * A1 + A2 * A2 + A3 * A3), gp, gm, g);
 #pragma unroll
         for (int n = 0; n < 5; ++n) acc[n] += g[n];
