@@ -1,6 +1,7 @@
-"""Run-to-run determinism of the fused stage: the heavy-face corrections are added in two ordered passes
-per direction (no floating-point atomics), so repeated time steps from the same input give the same bits,
-in both numeric forms and both branches, on inputs with faces on WENO7, WENO5 and NND."""
+"""Run-to-run determinism of the fused stage: the heavy-face corrections are added one cell at a time in a
+fixed order (no floating-point atomics), so repeated time steps from the same input give the same bits,
+in both numeric forms and both branches, on inputs with faces on WENO7, WENO5 and NND; and across the wrap
+of the cell marks' evaluation stamp."""
 import numpy as np
 import pytest
 
@@ -43,3 +44,59 @@ def test_time_steps_bit_identical(fvm, dtype):
     U2, Q2 = _steps(torch, case, 3, geo)
     print(f"fvm={fvm} {np.dtype(dtype).name}: max |U1 - U2| = {np.max(np.abs(U1 - U2)):.1e}")
     assert np.array_equal(U1, U2) and np.array_equal(Q1, Q2)
+
+
+@pytest.mark.parametrize("fvm", [0, 1], ids=["fdm", "fvm"])
+def test_heavy_marks_across_stamp_wrap(fvm):
+    """The heavy faces' cell marks carry an 8-bit evaluation stamp and are cleared when it wraps (every 255
+    evaluations): 90 steps (270 evaluations) from one state in a fresh context, against the same 90 steps
+    in a context whose stamp was first advanced by 40 steps of a state with its non-UP7 faces elsewhere (so
+    the wrap falls at another step and stale marks would sit on other cells): the same bits.  The
+    finite-difference run is also held to the oracle's 90 steps at the time-step parity bar; the finite-volume
+    one is not, since on this input (a 20x pressure spike) the FP64 oracle's own finite-volume RHS is off
+    its long-double evaluation by up to 1.5e-8 per variable, where the GPU's is within 1e-13."""
+    import torch
+    from casegen import relerr
+    from oracle_lib import Oracle
+    assert torch.cuda.is_available(), "GPU test"
+    kw = dict(jump=1.0, hybrid_phi3=2.5, interior_noise=0.02, fvm=f3d.FVM_EIGEN if fvm else 0)
+    case = ramp_case(40, 24, 16, spikes=[(5, 3, 2, 20.0), (25, 10, 7, 3.0)], **kw)
+    other = ramp_case(40, 24, 16, spikes=[(30, 18, 12, 20.0), (12, 20, 3, 3.0)], **kw)
+    p = case.params
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    geo = None
+    if fvm:
+        geo = [torch.zeros((4,) + p.face_shape(k)[1:], dtype=torch.float64, device="cuda:0") for k in range(3)]
+        f3d.Flame3D(p).fvm_geometry(*[d(a) for a in case.coords], geo)
+    m, J, inflow, dt = [d(a) for a in case.metrics], d(case.J), d(case.inflow), 1e-9
+
+    def steps(ctx, c, n):
+        U, Q = d(c.U()), d(c.Q)
+        Un = torch.zeros_like(U)
+        if geo is None:
+            ctx.time_step(n, U, Un, Q, m, J, dt, inflow)
+        else:
+            ctx.time_step_fvm(n, U, Un, Q, m, J, geo, dt, inflow)
+        torch.cuda.synchronize()
+        return U.cpu().numpy(), Q.cpu().numpy()
+
+    print(f"{'fvm' if fvm else 'fdm'}: 90 steps in a fresh context", flush=True)
+    Ua, Qa = steps(f3d.Flame3D(p), case, 90)
+    ctx_b = f3d.Flame3D(p)
+    print("  40 steps of another state, then the same 90 steps", flush=True)
+    steps(ctx_b, other, 40)
+    Ub, Qb = steps(ctx_b, case, 90)
+    print(f"  max |Ua - Ub| = {np.max(np.abs(Ua - Ub)):.1e}")
+    assert np.array_equal(Ua, Ub) and np.array_equal(Qa, Qb)
+    if fvm:
+        return
+    o = Oracle(p)
+    Uo, Qo = case.U(), case.Q.copy()
+    Up = Uo * (1 + 1e-16 * (2 * np.random.default_rng(5).random(Uo.shape) - 1))
+    Qp = Qo.copy()
+    o.time_step(90, Uo, np.zeros_like(Uo), Qo, case.metrics, case.J, dt, case.inflow)
+    o.time_step(90, Up, np.zeros_like(Up), Qp, case.metrics, case.J, dt, case.inflow)
+    inner = (slice(None),) + (slice(4, -4),) * 3
+    e, sens = relerr(Ua[inner], Uo[inner]), relerr(Up[inner], Uo[inner])
+    print(f"  U against the oracle: {e:.2e} (oracle sensitivity {sens:.2e})")
+    assert e <= 10 * sens + 1e-12, e
