@@ -79,6 +79,7 @@ struct FArgs {
   double* Uo = nullptr;         // conserved state written by the update (nullptr: U itself, in place)
   const unsigned int* pflag = nullptr;  // sensor band flag of this stage (heavy_possible), and its generation
   unsigned int pgen = 0;
+  const unsigned char* cls = nullptr;   // per-cell sensor class (scheme_of of the cell's phi), read by the sweeps
 };
 
 __device__ __forceinline__ long long ridx(const KP& kp, int i, int j, int k) {
@@ -104,14 +105,15 @@ __device__ __forceinline__ bool out_of_band(const KP& kp, T ph) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, const T* __restrict__ Q,
-                                                  unsigned int* pflag, unsigned int gen) {
+__global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, unsigned char* __restrict__ cls,
+                                                  const T* __restrict__ Q, unsigned int* pflag, unsigned int gen) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y, k = 1 + blockIdx.z;
   if (i > kp.Nxt - 2 || j > kp.Nyt - 2 || k > kp.Nzt - 2) return;
   const T* P = Q + 4 * kp.nc;
   const long long o = cidx(kp, i, j, k), sy = kp.Nxt, sz = (long long)kp.Nxt * kp.Nyt;
   const T ph = sensor_cell<T>(P[o - 1], P[o], P[o + 1], P[o - sy], P[o + sy], P[o - sz], P[o + sz]);
-  phi[o] = ph;
+  if (phi) phi[o] = ph;
+  cls[o] = (unsigned char)scheme_of<T>(kp, ph);
   mark_band<T>(kp, out_of_band(kp, ph), pflag, gen);
 }
 
@@ -119,8 +121,9 @@ __global__ __launch_bounds__(256) void k_sensor_f(KP kp, T* __restrict__ phi, co
 // column at k-1, k, k+1 stays in registers, so each cell costs one new plane load plus the four xi/eta
 // neighbours (L1/L2) instead of six neighbour loads.  Identical arithmetic (sensor_cell).
 template <typename T>
-__global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi, const T* __restrict__ Q, int KS,
-                                                      unsigned int* pflag, unsigned int gen) {
+__global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi, unsigned char* __restrict__ cls,
+                                                      const T* __restrict__ Q, int KS, unsigned int* pflag,
+                                                      unsigned int gen) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k0 = 1 + blockIdx.z * KS;
   if (i > kp.Nxt - 2 || j > kp.Nyt - 2) return;
@@ -134,7 +137,8 @@ __global__ __launch_bounds__(256) void k_sensor_march(KP kp, T* __restrict__ phi
   for (int k = k0; k < k1; ++k, o += sz) {
     const T pp = P[o + sz];
     const T ph = sensor_cell<T>(P[o - 1], pc, P[o + 1], P[o - sy], P[o + sy], pm, pp);
-    phi[o] = ph;
+    if (phi) phi[o] = ph;
+    cls[o] = (unsigned char)scheme_of<T>(kp, ph);
     out |= out_of_band(kp, ph);
     pm = pc;
     pc = pp;
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L, HSeg 
   __shared__ T sFp[5][2][XW];
   __shared__ T sFm[5][2][XW];
   __shared__ T sF[5][2][XW];
-  __shared__ T sphi[2][XW];
+  __shared__ unsigned char scl[2][XW];
   const int l = threadIdx.x, r = threadIdx.y;
   const int jj = blockIdx.y * 2 + r, kk = blockIdx.z;
   const bool rowok = jj < kp.Nyp;
@@ -172,17 +176,18 @@ __global__ __launch_bounds__(256) void k_sweep_x(KP kp, FArgs<T> a, int L, HSeg 
       sFp[n][r][l] = fp[n];
       sFm[n][r][l] = fm[n];
     }
-    sphi[r][l] = a.phi[o];
+    scl[r][l] = a.cls[o];
   }
   __syncthreads();
   const bool fact = l >= 4 && l < 4 + nface;
   const int f = f0 + l - 4;
   int sch = F3D_SCHEME_UP7;
   if (fact) {
-    T m = sphi[r][l - 3];
+    // the face's scheme: the largest sensor class of its six cells (the class of the cells' largest phi)
+    int m = scl[r][l - 3];
 #pragma unroll
-    for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
-    sch = scheme_of<T>(kp, m);
+    for (int s = -2; s <= 2; ++s) m = max(m, (int)scl[r][l + s]);
+    sch = m;
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
       T acc = 0;
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
   for (int s = 0; s < 8; ++s)
 #pragma unroll
     for (int n = 0; n < 5; ++n) acc[s][n] = 0;
-  T ring[6];
+  int ring[6];  // sensor classes of the last six cells
 #pragma unroll
   for (int s = 0; s < 6; ++s) ring[s] = 0;
   T Fprev[5] = {0, 0, 0, 0, 0};
@@ -249,22 +254,24 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
   // no face can leave UP7 when no sensor value of the stage left the band: no sensor reads, no masks
   const bool hp = heavy_possible(a.pflag, a.pgen);
   // prefetched inputs of the next cell
-  T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph;
+  T nq0, nq1, nq2, nq3, nq5, na1, na2, na3;
+  int ncl;
   {
     const long long o = obase;
     nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
     nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
-    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = hp ? a.phi[o] : T(0);
+    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; ncl = hp ? (int)a.cls[o] : 0;
   }
 #pragma unroll 1
   for (int step = 0; step < nstep; ++step) {
-    const T q0 = nq0, q1 = nq1, q2 = nq2, q3 = nq3, q5 = nq5, A1 = na1, A2 = na2, A3 = na3, ph = nph;
+    const T q0 = nq0, q1 = nq1, q2 = nq2, q3 = nq3, q5 = nq5, A1 = na1, A2 = na2, A3 = na3;
+    const int cl = ncl;
     if (step + 1 < nstep) {
       const long long o = obase + (long long)(step + 1) * st;
       nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
       nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
       na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o];
-      if (hp) nph = a.phi[o];
+      if (hp) ncl = a.cls[o];
     }
     T Rc[5];
 #pragma unroll
@@ -287,10 +294,9 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
       const int fi = step - 7;
       const int f = f0 + fi;
       if (hp) {
-        T m = ring[0];
+        int sch = ring[0];  // the largest class of the face's six cells
 #pragma unroll
-        for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
-        const int sch = scheme_of<T>(kp, m);
+        for (int s = 1; s < 6; ++s) sch = max(sch, ring[s]);
         run_append(a.heavy, hrun, hcnt, act && (fi < L || f == Np) && sch != F3D_SCHEME_UP7,
                    ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) |
                        (unsigned long long)D);
@@ -311,7 +317,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(KP kp, FArgs<T> a, int L, H
     for (int n = 0; n < 5; ++n) acc[7][n] = 0;
 #pragma unroll
     for (int s = 0; s < 5; ++s) ring[s] = ring[s + 1];
-    ring[5] = ph;
+    ring[5] = cl;
   }
   if (threadIdx.x == 0) a.hcount[hs.first + hw] = hcnt;
 }
@@ -334,7 +340,7 @@ template <typename T, int SPLIT>
 __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L, HSeg hx, HSeg hy) {
   __shared__ T sFp[5][XYL][XYW];
   __shared__ T sFm[5][XYL][XYW];
-  __shared__ T sphi[XYL][XYW];
+  __shared__ unsigned char scl[XYL][XYW];
   __shared__ T sF[4][5][XYL][XYW];  // x-face fluxes of the last four rows (slot = step & 3)
   const int l = threadIdx.x, r = threadIdx.y;
   const int fx0 = blockIdx.x * XYI;
@@ -364,7 +370,7 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
   for (int s = 0; s < 8; ++s)
 #pragma unroll
     for (int n = 0; n < 5; ++n) acc[s][n] = 0;
-  T ring[6];
+  int ring[6];  // sensor classes of the last six cells
 #pragma unroll
   for (int s = 0; s < 6; ++s) ring[s] = 0;
   T Fprev[5] = {0, 0, 0, 0, 0};
@@ -375,16 +381,18 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
   // no face can leave UP7 when no sensor value of the stage left the band: no sensor reads, no masks
   const bool hp = heavy_possible(a.pflag, a.pgen);
   // prefetched inputs of the next cell (x metrics only on the steps that form x faces: rows f0 ..)
-  T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nph, nb1 = 0, nb2 = 0, nb3 = 0;
+  T nq0, nq1, nq2, nq3, nq5, na1, na2, na3, nb1 = 0, nb2 = 0, nb3 = 0;
+  int ncl;
   {
     const long long o = obase;
     nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
     nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
-    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; nph = hp ? a.phi[o] : T(0);
+    na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o]; ncl = hp ? (int)a.cls[o] : 0;
   }
 #pragma unroll 1
   for (int step = 0; step < nstep; ++step) {
-    const T q0 = nq0, q1 = nq1, q2 = nq2, q3 = nq3, q5 = nq5, A1 = na1, A2 = na2, A3 = na3, ph = nph;
+    const T q0 = nq0, q1 = nq1, q2 = nq2, q3 = nq3, q5 = nq5, A1 = na1, A2 = na2, A3 = na3;
+    const int cl = ncl;
     const T B1 = nb1, B2 = nb2, B3 = nb3;
     const bool xstep = step >= 4 && step < 4 + ncell;  // this step's row is interior: form its x faces
     if (step + 1 < nstep) {
@@ -392,7 +400,7 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
       nq0 = a.Q[o]; nq1 = a.Q[o + nc]; nq2 = a.Q[o + 2 * nc]; nq3 = a.Q[o + 3 * nc];
       nq5 = a.Q[o + split_qvar<SPLIT>() * nc];
       na1 = A1p[o]; na2 = A2p[o]; na3 = A3p[o];
-      if (hp) nph = a.phi[o];
+      if (hp) ncl = a.cls[o];
       if (step + 1 >= 4 && step + 1 < 4 + ncell) {
         nb1 = X1p[o]; nb2 = X2p[o]; nb3 = X3p[o];
       }
@@ -405,7 +413,7 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
         sFp[n][r][l] = fp[n];
         sFm[n][r][l] = fm[n];
       }
-      sphi[r][l] = ph;
+      scl[r][l] = (unsigned char)cl;
     }
     {
       T fp[5], fm[5];
@@ -422,10 +430,9 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
       const int fi = step - 7;
       const int f = f0 + fi;
       if (hp) {
-        T m = ring[0];
+        int sch = ring[0];  // the largest class of the face's six cells
 #pragma unroll
-        for (int s = 1; s < 6; ++s) m = jmax(m, ring[s]);
-        const int sch = scheme_of<T>(kp, m);
+        for (int s = 1; s < 6; ++s) sch = max(sch, ring[s]);
         run_append(a.heavy, yrun, ycnt, act && (fi < L || f == kp.Nyp) && sch != F3D_SCHEME_UP7,
                    ((unsigned long long)(obase + (long long)(fi + 4) * st) << 4) | ((unsigned long long)sch << 2) | 1ULL);
       }
@@ -449,7 +456,7 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
     for (int n = 0; n < 5; ++n) acc[7][n] = 0;
 #pragma unroll
     for (int s = 0; s < 5; ++s) ring[s] = ring[s + 1];
-    ring[5] = ph;
+    ring[5] = cl;
     __syncthreads();  // split fluxes of this row written; the ring slot read above is free
     if (xstep) {
       const bool fact = l >= 4 && l < 4 + nfx;
@@ -458,10 +465,9 @@ __global__ __launch_bounds__(XYW * XYL) void k_sweep_xy(KP kp, FArgs<T> a, int L
       int sch = F3D_SCHEME_UP7;
       if (fact) {
         if (hp) {
-          T m = sphi[r][l - 3];
+          sch = scl[r][l - 3];
 #pragma unroll
-          for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
-          sch = scheme_of<T>(kp, m);
+          for (int s = -2; s <= 2; ++s) sch = max(sch, (int)scl[r][l + s]);
         }
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
@@ -1401,10 +1407,10 @@ void fused_sensor(const KP& kp, const WsView<T>& ws, const T* Q, T* phi, hipStre
   unsigned int* pf = ws.pgen ? ws.pflag : nullptr;
   if (KS > 0)
     hipLaunchKernelGGL(k_sensor_march<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, (kp.Nzt - 2 + KS - 1) / KS),
-                       dim3(64, 4, 1), 0, s, kp, phi, Q, KS, pf, gen);
+                       dim3(64, 4, 1), 0, s, kp, phi, ws.cls, Q, KS, pf, gen);
   else
     hipLaunchKernelGGL(k_sensor_f<T>, dim3((kp.Nxt - 2 + 63) / 64, (kp.Nyt - 2 + 3) / 4, kp.Nzt - 2), dim3(64, 4, 1),
-                       0, s, kp, phi, Q, pf, gen);
+                       0, s, kp, phi, ws.cls, Q, pf, gen);
 }
 
 // Launch shapes of the finite-difference sweeps (shared by the launches and their heavy-list layout).
@@ -1508,6 +1514,7 @@ template <typename T, int SPLIT>
 static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J,
                             T* phi, hipStream_t s) {
   FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
+  a.cls = ws.cls;
   a.pflag = ws.pgen ? ws.pflag : nullptr;
   a.pgen = ws.pgen ? *ws.pgen : 0u;
   const HList hl = fdm_heavy_layout(kp, (int)sizeof(T));
@@ -1636,7 +1643,7 @@ int fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, double* U
     set_error("fused stage: the output state must not alias the input state");
     return F3D_E_ARG;
   }
-  fused_sensor<T>(kp, ws, Qin, ws.phi, s);
+  fused_sensor<T>(kp, ws, Qin, nullptr, s);  // the sweeps read the sensor classes; phi is not kept
   int rc = fused_flux<T>(kp, ws, Qin, m, J, ws.phi, s);
   if (rc) return rc;
   return fused_update<T>(kp, ws, KRK, U, Un, Qin, Qout, m, J, ws.phi, nullptr, dt, s, Uo);

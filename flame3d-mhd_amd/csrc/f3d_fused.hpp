@@ -23,6 +23,8 @@ struct WsView {
   unsigned int* pflag;       // device word: the sensor generation of the last stage in which some cell's
                              // sensor value left the UP7 band (no face can be off UP7 otherwise)
   unsigned int* pgen;        // host counter of the sensor generations
+  unsigned char* cls;        // per-cell sensor class: scheme_of of the cell's phi (a face's scheme is the largest
+                             // class of its six cells: the class of their largest phi, NaN the largest)
 };
 
 // Whether any face of this stage can be off UP7: the sensor marks its generation when some cell's value is

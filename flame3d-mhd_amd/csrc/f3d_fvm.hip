@@ -42,6 +42,7 @@ struct FvmArgs {
   unsigned int* hcount;
   const unsigned int* pflag = nullptr;  // sensor band flag of this stage (heavy_possible), and its generation
   unsigned int pgen = 0;
+  const unsigned char* cls = nullptr;   // per-cell sensor class (scheme_of of the cell's phi)
 };
 
 __device__ __forceinline__ long long rix(const KP& kp, int i, int j, int k) {
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256) void k_fvm_geometry(KP kp, const double* __res
 template <typename T>
 __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L, HSeg hs) {
   __shared__ T sU[5][2][kXW];
-  __shared__ T sphi[2][kXW];
+  __shared__ unsigned char scl[2][kXW];
   __shared__ T sF[5][2][kXW];
   const int l = threadIdx.x, r = threadIdx.y;
   const int jj = blockIdx.y * 2 + r, kk = blockIdx.z;
@@ -140,16 +141,15 @@ __global__ __launch_bounds__(256) void k_fvm_sweep_x(KP kp, FvmArgs<T> a, int L,
     const long long o = cidx(kp, xc, j, k);
 #pragma unroll
     for (int n = 0; n < 5; ++n) sU[n][r][l] = (T)a.U[o + n * nc];
-    sphi[r][l] = hp ? a.phi[o] : T(0);
+    scl[r][l] = hp ? a.cls[o] : (unsigned char)0;
   }
   __syncthreads();
   int sch = F3D_SCHEME_UP7;
   if (fact) {
-    if (hp) {
-      T m = sphi[r][l - 3];
+    if (hp) {  // the largest sensor class of the face's six cells
+      sch = scl[r][l - 3];
 #pragma unroll
-      for (int s = -2; s <= 2; ++s) m = jmax(m, sphi[r][l + s]);
-      sch = scheme_of<T>(kp, m);
+      for (int s = -2; s <= 2; ++s) sch = max(sch, (int)scl[r][l + s]);
     }
     T Fl[5] = {0, 0, 0, 0, 0};
     if (sch == F3D_SCHEME_UP7) {
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
   const long long gstride = D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp;  // next face along D
   const long long gbase = D == 1 ? fidx(kp, 1, ic, f0, tc) : fidx(kp, 2, ic, tc, f0);
   T W[8][5];
-  T ph[8];
+  int ph[8];  // sensor classes of the window's cells
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     ph[s] = 0;
@@ -222,11 +222,11 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
   unsigned int hcnt = 0;
   T gn[4] = {0, 0, 0, 0};  // geometry of the next face to complete
   double nu[5];
-  T nph;
+  int nph;
   const bool hp = heavy_possible(a.pflag, a.pgen);  // some face of the stage may be off UP7
 #pragma unroll
   for (int n = 0; n < 5; ++n) nu[n] = a.U[obase + n * nc];
-  nph = hp ? a.phi[obase] : T(0);
+  nph = hp ? (int)a.cls[obase] : 0;
 #pragma unroll 1
   for (int step = 0; step < nstep; ++step) {
 #pragma unroll
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
       const long long o = obase + (long long)(step + 1) * st;
 #pragma unroll
       for (int n = 0; n < 5; ++n) nu[n] = a.U[o + n * nc];
-      if (hp) nph = a.phi[o];
+      if (hp) nph = a.cls[o];
     }
     T Rc[5];
 #pragma unroll
@@ -264,10 +264,9 @@ __global__ __launch_bounds__(256) void k_fvm_march(KP kp, FvmArgs<T> a, int L, H
         const int f = f0 + fi;
         int sch = F3D_SCHEME_UP7;
         if (hp) {
-          T m = ph[1];
+          sch = ph[1];
 #pragma unroll
-          for (int s = 2; s <= 6; ++s) m = jmax(m, ph[s]);
-          sch = scheme_of<T>(kp, m);
+          for (int s = 2; s <= 6; ++s) sch = max(sch, ph[s]);
         }
         T Fc[5] = {0, 0, 0, 0, 0};
         if (sch == F3D_SCHEME_UP7) {
@@ -310,7 +309,7 @@ constexpr int FXL = 2;          // z lines (waves) per block
 template <typename T>
 __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T> a, int L, HSeg hx, HSeg hy) {
   __shared__ T sU[5][FXL][FXW];
-  __shared__ T sphi[FXL][FXW];
+  __shared__ unsigned char scl[FXL][FXW];
   __shared__ T sF[4][5][FXL][FXW];  // x-face fluxes of the last four rows (slot = step & 3)
   const int l = threadIdx.x, r = threadIdx.y;
   const int fx0 = blockIdx.x * FXI;
@@ -334,7 +333,7 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
   const long long gs0 = fcount(kp, 0), gs1 = fcount(kp, 1);
   const long long gy0 = fidx(kp, 1, yin ? ii : 0, f0, tc);  // geometry of this column's y face f0
   T W[8][5];
-  T ph[8];
+  int ph[8];  // sensor classes of the window's cells
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     ph[q] = 0;
@@ -349,11 +348,11 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
   const unsigned long long yrun = hy.base + (unsigned long long)hw * hy.cap;
   unsigned int xcnt = 0, ycnt = 0;
   double nu[5];
-  T nph;
+  int nph;
   const bool hp = heavy_possible(a.pflag, a.pgen);  // some face of the stage may be off UP7
 #pragma unroll
   for (int n = 0; n < 5; ++n) nu[n] = a.U[obase + n * nc];
-  nph = hp ? a.phi[obase] : T(0);
+  nph = hp ? (int)a.cls[obase] : 0;
 #pragma unroll 1
   for (int step = 0; step < nstep; ++step) {
 #pragma unroll
@@ -371,7 +370,7 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
       const long long o = obase + (long long)(step + 1) * st;
 #pragma unroll
       for (int n = 0; n < 5; ++n) nu[n] = a.U[o + n * nc];
-      if (hp) nph = a.phi[o];
+      if (hp) nph = a.cls[o];
       if (fact && lineok && step + 1 >= 4 && step + 1 < 4 + ncell) {
         const long long go = fidx(kp, 0, ii, f0 + step + 1 - 4, tc);
 #pragma unroll
@@ -381,7 +380,7 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
     if (xstep) {
 #pragma unroll
       for (int n = 0; n < 5; ++n) sU[n][r][l] = W[7][n];
-      sphi[r][l] = ph[7];
+      scl[r][l] = (unsigned char)ph[7];
     }
     if (step >= 6) {
       const T g0 = gn[0], g1 = gn[1], g2 = gn[2], g3 = gn[3];  // face f0 + step - 7 (step >= 7)
@@ -395,10 +394,9 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
         const int f = f0 + fi;
         int sch = F3D_SCHEME_UP7;
         if (hp) {
-          T m = ph[1];
+          sch = ph[1];
 #pragma unroll
-          for (int q = 2; q <= 6; ++q) m = jmax(m, ph[q]);
-          sch = scheme_of<T>(kp, m);
+          for (int q = 2; q <= 6; ++q) sch = max(sch, ph[q]);
         }
         T Fc[5] = {0, 0, 0, 0, 0};
         if (sch == F3D_SCHEME_UP7) {
@@ -427,10 +425,9 @@ __global__ __launch_bounds__(FXW * FXL, 2) void k_fvm_sweep_xy(KP kp, FvmArgs<T>
       int sch = F3D_SCHEME_UP7;
       if (fact) {
         if (hp) {
-          T m = sphi[r][l - 3];
+          sch = scl[r][l - 3];
 #pragma unroll
-          for (int q = -2; q <= 2; ++q) m = jmax(m, sphi[r][l + q]);
-          sch = scheme_of<T>(kp, m);
+          for (int q = -2; q <= 2; ++q) sch = max(sch, (int)scl[r][l + q]);
         }
         T Fl[5] = {0, 0, 0, 0, 0};
         if (sch == F3D_SCHEME_UP7) {
@@ -595,6 +592,7 @@ template <typename T>
 int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
              const T* const* geo, const T* phi, hipStream_t s) {
   FvmArgs<T> a{U, {geo[0], geo[1], geo[2]}, phi, ws.R, ws.heavy, ws.hcount};
+  a.cls = ws.cls;
   a.pflag = ws.pgen ? ws.pflag : nullptr;
   a.pgen = ws.pgen ? *ws.pgen : 0u;
   const HList hl = fvm_heavy_layout(kp, (int)sizeof(T));
@@ -695,7 +693,7 @@ int fvm_fused_stage(const KP& kp, const WsView<T>& ws, int KRK, double* U, doubl
     set_error("fused stage: the output state must not alias the input state");
     return F3D_E_ARG;
   }
-  fused_sensor<T>(kp, ws, Qin, ws.phi, s);
+  fused_sensor<T>(kp, ws, Qin, nullptr, s);  // the sweeps read the sensor classes; phi is not kept
   int rc = fvm_flux<T>(kp, ws, Qin, U, m, geo, ws.phi, s);
   if (rc) return rc;
   return fused_update<T>(kp, ws, KRK, U, Un, Qin, Qout, m, J, ws.phi, nullptr, dt, s);

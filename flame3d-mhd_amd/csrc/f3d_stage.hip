@@ -31,6 +31,7 @@ struct f3d_workspace {
   unsigned int hstamp;       // stamp of the last heavy-face evaluation
   unsigned int* pflag;       // sensor generation of the last stage with a value outside the UP7 band
   unsigned int pgen;         // sensor generations launched
+  unsigned char* cls;        // per-cell sensor class (the scheme its phi alone selects), read by the sweeps
 };
 
 namespace {
@@ -454,7 +455,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   const size_t es = precision == F3D_PREC_F64 ? 8 : 4;
   const size_t nc = f3d_cell_count(p);
   const size_t Nx = p->Nxp, Ny = p->Nyp, Nz = p->Nzp, NG = p->NG;
-  size_t sz[21];
+  size_t sz[22];
   sz[0] = nc * es;                               // phi
   sz[1] = nc * 5 * es;                           // Fp
   sz[2] = nc * 5 * es;                           // Fm
@@ -483,6 +484,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[18] = (size_t)heavy_face_count(kp0) * 5 * es;                 // heavy-face corrections per face
   sz[19] = (size_t)hmark_count(kp0) * sizeof(unsigned long long);  // per-cell marks of the listed faces
   sz[20] = sizeof(unsigned int);                 // sensor band flag
+  sz[21] = nc;                                   // sensor class per cell
   size_t total = 0;
   for (size_t b : sz) total += align_up(b);
   void* base = nullptr;
@@ -493,11 +495,11 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   ws->bytes = total;
   ws->base = base;
   char* c = (char*)base;
-  void** slots[21] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
+  void** slots[22] = {&ws->phi, &ws->Fp, &ws->Fm, &ws->Fx, &ws->Fy, &ws->Fz, &ws->Fvx, &ws->Fvy, &ws->Fvz,
                       &ws->R, (void**)&ws->red, (void**)&ws->flag, (void**)&ws->heavy, (void**)&ws->hcount,
                       &ws->Qa, &ws->Qb, (void**)&ws->ldt, (void**)&ws->fsc, &ws->hflux, (void**)&ws->hidx,
-                      (void**)&ws->pflag};
-  for (int i = 0; i < 21; ++i) {
+                      (void**)&ws->pflag, (void**)&ws->cls};
+  for (int i = 0; i < 22; ++i) {
     *slots[i] = c;
     c += align_up(sz[i]);
   }
@@ -551,6 +553,7 @@ WsView<T> ws_view(f3d_workspace* ws) {
   v.hstamp = &ws->hstamp;
   v.pflag = ws->pflag;
   v.pgen = &ws->pgen;
+  v.cls = ws->cls;
   return v;
 }
 template WsView<double> ws_view<double>(f3d_workspace*);
@@ -619,7 +622,7 @@ int scratch_address(void** out, const char* who) {
 
 // Scratch layout of the no-allocation RHS: offsets of R, the heavy-face runs, their counts and phi.
 struct NoAllocLayout {
-  size_t R, heavy, hcount, hflux, hidx, phi, total;
+  size_t R, heavy, hcount, hflux, hidx, cls, phi, total;
   unsigned long long hcap, hncap;
   long long nidx;  // marks of the listed faces (one word per interior cell)
 };
@@ -641,6 +644,8 @@ NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   L.hidx = off;
   L.nidx = hmark_count(kp);
   off += align_up((size_t)(L.nidx + 1) * sizeof(unsigned long long));  // the marks, then the sensor band flag
+  L.cls = off;
+  off += align_up(f3d_cell_count(p));
   L.phi = off;
   if (with_phi) off += align_up(f3d_cell_count(p) * sizeof(double));
   L.total = off;
@@ -694,6 +699,7 @@ F3D_API int f3d_rhs_noalloc_f64(const f3d_params* p, double* R, const double* Q,
   v.pflag = (unsigned int*)(v.hidx + L.nidx);
   v.pgen = &gen;
   v.phi = phi ? phi : (double*)(c + L.phi);
+  v.cls = (unsigned char*)(c + L.cls);
   clear_hidx(v.hidx, L.nidx + 1, (hipStream_t)s);
   MetricPtrs<double> m;
   for (int q = 0; q < 9; ++q) m.m[q] = metrics[q];
