@@ -1543,7 +1543,7 @@ static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const
   {
     TimedScope ts("fused_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
-    const HMark hm = next_hmark(kp, ws.hidx, ws.hstamp, s);
+    const HMark hm = hmarks(ws.hidx);
     if (heavy_pairs((int)sizeof(T))) {
       if (kp.character)
         hipLaunchKernelGGL((k_heavy_pair<T, SPLIT, true>), dim3(nb), dim3(64), 0, s, kp, a, hl, ws.hflux, hm);

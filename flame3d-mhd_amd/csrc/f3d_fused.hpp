@@ -18,8 +18,7 @@ struct WsView {
   unsigned long long hncap;  // capacity of `hcount` (runs)
   double* ldt;               // per-cell dt of local time stepping, cell-shaped (written at stage 1)
   T* hflux;                  // per face position: the 5 values a listed face adds to its cells (heavy_face_count x 5)
-  unsigned long long* hidx;  // per-cell marks of the listed faces (hmark_count, see HMark)
-  unsigned int* hstamp;      // host counter of the marks' stamps
+  unsigned long long* hidx;  // per-cell mark bytes of the listed faces (hmark_words, see HMark)
   unsigned int* pflag;       // device word: the sensor generation of the last stage in which some cell's
                              // sensor value left the UP7 band (no face can be off UP7 otherwise)
   unsigned int* pgen;        // host counter of the sensor generations
@@ -216,19 +215,18 @@ __device__ __forceinline__ void for_each_heavy(const HList& hl, const unsigned i
 
 // Heavy-face corrections and the marks of the cells they enter.  The forming pass keeps each listed face's
 // F_scheme - F_UP7 at its face position (hcorr[5 f + q], f = face_id) and marks the face in its two cells:
-// byte o = 2 d + side of hmark[c] (c = interior index of the cell, side 0 = its left face along d, 1 = its
-// right face) takes the evaluation's 8-bit stamp (distinct faces write distinct bytes, no conflicts).
-// k_heavy_apply then walks the cells in order (coalesced mark reads; the RHS read and written only where a
-// mark holds the stamp) and adds a cell's corrections in the order o = 0..5 (+ for a left face, - for a
-// right face): one lane per cell and a fixed order, so no floating-point atomics and the same bits on every
-// evaluation.  Every evaluation takes a new stamp, so marks of earlier ones need no clearing (the array is
-// cleared when the stamp wraps); bytes 6 and 7 stay zero.
-constexpr unsigned int kHStampMax = 255;
+// bit o = 2 d + side of the cell's mark byte (side 0 = its left face along d, 1 = its right face), set with
+// an integer atomic OR on the 32-bit word holding four cells' bytes (the same bits whatever the order).
+// k_heavy_apply then walks the cells in order (coalesced byte reads; the RHS read and written only where a
+// byte is non-zero), adds a cell's corrections in the order o = 0..5 (+ for a left face, - for a right
+// face) and clears the byte: one lane per cell and a fixed order, so no floating-point atomics and the same
+// bits on every evaluation, and the marks are all zero again for the next one.
 struct HMark {
-  unsigned long long* mark;
-  unsigned int stamp;  // 1 .. kHStampMax
+  unsigned int* mark;  // one byte per interior cell, four to a word
 };
 __host__ __device__ inline long long hmark_count(const KP& kp) { return (long long)kp.Nxp * kp.Nyp * kp.Nzp; }
+// 8-byte words holding the mark bytes of every interior cell.
+__host__ __device__ inline long long hmark_words(const KP& kp) { return (hmark_count(kp) + 7) / 8; }
 // Faces of the three directions: (Nxp + 1) Nyp Nzp along x, then Nxp (Nyp + 1) Nzp along y, Nxp Nyp (Nzp + 1)
 // along z.
 __host__ __device__ inline long long heavy_face_count(const KP& kp) {
@@ -252,24 +250,21 @@ __device__ __forceinline__ void heavy_keep(const KP& kp, const HMark& hm, T* __r
   const long long fid = face_id(kp, D, I, J, K);
 #pragma unroll
   for (int q = 0; q < 5; ++q) corr[fid * 5 + q] = dF[q];
-  unsigned char* mb = reinterpret_cast<unsigned char*>(hm.mark);
   const long long rr = (long long)I + (long long)kp.Nxp * ((long long)J + (long long)kp.Nyp * K);
-  if (f < Np) mb[8 * rr + 2 * D] = (unsigned char)hm.stamp;
+  if (f < Np) atomicOr(&hm.mark[rr >> 2], (1u << (2 * D)) << (8 * (int)(rr & 3)));
   if (f > 0) {
     const long long rl = rr - (D == 0 ? 1 : (D == 1 ? (long long)kp.Nxp : (long long)kp.Nxp * kp.Nyp));
-    mb[8 * rl + 2 * D + 1] = (unsigned char)hm.stamp;
+    atomicOr(&hm.mark[rl >> 2], (1u << (2 * D + 1)) << (8 * (int)(rl & 3)));
   }
 }
-// Adds the corrections of interior cell (I, J, K), whose mark word is mw, to its RHS Rt, in the order x left,
+// Adds the corrections of interior cell (I, J, K), whose mark byte is mb, to its RHS Rt, in the order x left,
 // x right, y left, y right, z left, z right.
 template <typename T>
-__device__ __forceinline__ void heavy_add_mark(const KP& kp, unsigned long long mw, const T* __restrict__ corr,
-                                               unsigned int stamp, int I, int J, int K, T* Rt) {
-  const unsigned long long x = mw ^ (0x0101010101010101ULL * stamp);
-  if ((((x - 0x0101010101010101ULL) & ~x) & 0x8080808080808080ULL) == 0) return;  // no byte holds the stamp
+__device__ __forceinline__ void heavy_add_mark(const KP& kp, unsigned int mb, const T* __restrict__ corr, int I,
+                                               int J, int K, T* Rt) {
 #pragma unroll
   for (int o = 0; o < 6; ++o) {
-    if (((mw >> (8 * o)) & 0xffULL) != stamp) continue;
+    if (!((mb >> o) & 1u)) continue;
     const int D = o >> 1, sd = o & 1;
     const long long f = face_id(kp, D, I + (D == 0 ? sd : 0), J + (D == 1 ? sd : 0), K + (D == 2 ? sd : 0));
 #pragma unroll
@@ -278,24 +273,24 @@ __device__ __forceinline__ void heavy_add_mark(const KP& kp, unsigned long long 
 }
 
 // The corrections of every marked cell added to the RHS R (interior layout [5][Nzp][Nyp][Nxp]), one lane per
-// cell in index order.
+// cell in index order; the marks are cleared as they are used.
 template <typename T>
-__global__ __launch_bounds__(256) void k_heavy_apply(KP kp, T* __restrict__ R, const unsigned long long* __restrict__ mark,
-                                                     const T* __restrict__ corr, unsigned int stamp,
-                                                     const unsigned int* pflag, unsigned int pgen) {
-  if (!heavy_possible(pflag, pgen)) return;  // nothing listed this stage
+__global__ __launch_bounds__(256) void k_heavy_apply(KP kp, T* __restrict__ R, unsigned char* __restrict__ mark,
+                                                     const T* __restrict__ corr, const unsigned int* pflag,
+                                                     unsigned int pgen) {
+  if (!heavy_possible(pflag, pgen)) return;  // nothing listed (or marked) this stage
   const long long rn = hmark_count(kp), sxy = (long long)kp.Nxp * kp.Nyp;
   for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < rn; c += (long long)gridDim.x * 256) {
-    const unsigned long long mw = mark[c];
-    const unsigned long long x = mw ^ (0x0101010101010101ULL * stamp);
-    if ((((x - 0x0101010101010101ULL) & ~x) & 0x8080808080808080ULL) == 0) continue;
+    const unsigned int mb = mark[c];
+    if (!mb) continue;
     const int I = (int)(c % kp.Nxp), J = (int)((c / kp.Nxp) % kp.Nyp), K = (int)(c / sxy);
     T Rt[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) Rt[q] = R[c + q * rn];
-    heavy_add_mark(kp, mw, corr, stamp, I, J, K, Rt);
+    heavy_add_mark(kp, mb, corr, I, J, K, Rt);
 #pragma unroll
     for (int q = 0; q < 5; ++q) R[c + q * rn] = Rt[q];
+    mark[c] = 0;
   }
 }
 template <typename T>
@@ -303,20 +298,14 @@ inline void launch_heavy_apply(const KP& kp, T* R, const HMark& hm, const T* cor
                                const unsigned int* pflag, unsigned int pgen) {
   const long long nb = (hmark_count(kp) + 255) / 256;
   hipLaunchKernelGGL((k_heavy_apply<T>), dim3((unsigned int)(nb < 8192 ? (nb ? nb : 1) : 8192)), dim3(256), 0, s, kp,
-                     R, hm.mark, corr, hm.stamp, pflag, pgen);
+                     R, reinterpret_cast<unsigned char*>(hm.mark), corr, pflag, pgen);
 }
 
-// Clears n 8-byte words (stamp wrap; the no-allocation entry's scratch).
+// Clears n 8-byte words (the no-allocation entry's scratch).
 void clear_hidx(unsigned long long* idx, long long n, hipStream_t s);
 
-// A new stamp for one heavy-face evaluation (the hmark_count(kp) marks are cleared when the stamp wraps).
-inline HMark next_hmark(const KP& kp, unsigned long long* mark, unsigned int* stamp, hipStream_t s) {
-  if (++*stamp > kHStampMax) {
-    clear_hidx(mark, hmark_count(kp), s);
-    *stamp = 1;
-  }
-  return HMark{mark, *stamp};
-}
+// The marks of one heavy-face evaluation (all zero between evaluations: the apply pass clears what it reads).
+inline HMark hmarks(unsigned long long* words) { return HMark{reinterpret_cast<unsigned int*>(words)}; }
 
 // Linear id of the calling wave in a launch of blocks of `wpb` waves (wave index wib in its block).
 __device__ __forceinline__ unsigned int launch_wave(int wpb, int wib) {

@@ -621,7 +621,7 @@ int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, con
   {
     TimedScope ts("fvm_heavy", s);
     const unsigned int nb = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
-    const HMark hm = next_hmark(kp, ws.hidx, ws.hstamp, s);
+    const HMark hm = hmarks(ws.hidx);
     if (kp.fvm == F3D_FVM_EIGEN)
       hipLaunchKernelGGL((k_fvm_heavy<T, true>), dim3(nb), dim3(64), 0, s, kp, a, Q, m, hl, ws.hflux, hm);
     else

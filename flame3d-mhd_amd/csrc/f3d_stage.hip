@@ -28,7 +28,6 @@ struct f3d_workspace {
   double* fsc;    // sensor of the nonlinear filter (cell-shaped)
   void* hflux;    // heavy-face corrections per face position (bulk type, 5 per face)
   unsigned long long* hidx;  // per-cell marks of the listed faces (one word per interior cell)
-  unsigned int hstamp;       // stamp of the last heavy-face evaluation
   unsigned int* pflag;       // sensor generation of the last stage with a value outside the UP7 band
   unsigned int pgen;         // sensor generations launched
   unsigned char* cls;        // per-cell sensor class (the scheme its phi alone selects), read by the sweeps
@@ -482,7 +481,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
   sz[16] = nc * sizeof(double);                  // ldt
   sz[17] = nc * sizeof(double);                  // fsc
   sz[18] = (size_t)heavy_face_count(kp0) * 5 * es;                 // heavy-face corrections per face
-  sz[19] = (size_t)hmark_count(kp0) * sizeof(unsigned long long);  // per-cell marks of the listed faces
+  sz[19] = (size_t)hmark_words(kp0) * sizeof(unsigned long long);  // per-cell mark bytes of the listed faces
   sz[20] = sizeof(unsigned int);                 // sensor band flag
   sz[21] = nc;                                   // sensor class per cell
   size_t total = 0;
@@ -504,7 +503,7 @@ F3D_API int f3d_workspace_create(const f3d_params* p, int32_t precision, f3d_wor
     c += align_up(sz[i]);
   }
   ws->nred = kRedBlocks;
-  ws->hstamp = 0;  // the marks start cleared (zero fill below): stamps count from 1
+  // the marks start cleared (zero fill below); the apply pass leaves them cleared
   ws->pgen = 0;    // the flag starts cleared: generations count from 1
   ws->hcap = hslots;
   ws->hncap = hruns;
@@ -550,7 +549,6 @@ WsView<T> ws_view(f3d_workspace* ws) {
   v.ldt = ws->ldt;
   v.hflux = (T*)ws->hflux;
   v.hidx = ws->hidx;
-  v.hstamp = &ws->hstamp;
   v.pflag = ws->pflag;
   v.pgen = &ws->pgen;
   v.cls = ws->cls;
@@ -624,7 +622,7 @@ int scratch_address(void** out, const char* who) {
 struct NoAllocLayout {
   size_t R, heavy, hcount, hflux, hidx, cls, phi, total;
   unsigned long long hcap, hncap;
-  long long nidx;  // marks of the listed faces (one word per interior cell)
+  long long nidx;  // 8-byte words of the listed faces' mark bytes (one byte per interior cell)
 };
 NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   const KP kp = make_kp(p);
@@ -642,7 +640,7 @@ NoAllocLayout noalloc_layout(const f3d_params* p, bool with_phi) {
   L.hflux = off;
   off += align_up((size_t)heavy_face_count(kp) * 5 * sizeof(double));
   L.hidx = off;
-  L.nidx = hmark_count(kp);
+  L.nidx = hmark_words(kp);
   off += align_up((size_t)(L.nidx + 1) * sizeof(unsigned long long));  // the marks, then the sensor band flag
   L.cls = off;
   off += align_up(f3d_cell_count(p));
@@ -693,8 +691,6 @@ F3D_API int f3d_rhs_noalloc_f64(const f3d_params* p, double* R, const double* Q,
   v.hncap = L.hncap;
   v.hflux = (double*)(c + L.hflux);
   v.hidx = (unsigned long long*)(c + L.hidx);
-  unsigned int stamp = 0;  // marks cleared below: this call's faces take stamp 1
-  v.hstamp = &stamp;
   unsigned int gen = 0;  // the flag word after the marks, cleared with them: this call's sensor is generation 1
   v.pflag = (unsigned int*)(v.hidx + L.nidx);
   v.pgen = &gen;

@@ -1,7 +1,7 @@
 """Run-to-run determinism of the fused stage: the heavy-face corrections are added one cell at a time in a
 fixed order (no floating-point atomics), so repeated time steps from the same input give the same bits,
-in both numeric forms and both branches, on inputs with faces on WENO7, WENO5 and NND; and across the wrap
-of the cell marks' evaluation stamp."""
+in both numeric forms and both branches, on inputs with faces on WENO7, WENO5 and NND, whatever the
+context ran before."""
 import numpy as np
 import pytest
 
@@ -47,11 +47,11 @@ def test_time_steps_bit_identical(fvm, dtype):
 
 
 @pytest.mark.parametrize("fvm", [0, 1], ids=["fdm", "fvm"])
-def test_heavy_marks_across_stamp_wrap(fvm):
-    """The heavy faces' cell marks carry an 8-bit evaluation stamp and are cleared when it wraps (every 255
-    evaluations): 90 steps (270 evaluations) from one state in a fresh context, against the same 90 steps
-    in a context whose stamp was first advanced by 40 steps of a state with its non-UP7 faces elsewhere (so
-    the wrap falls at another step and stale marks would sit on other cells): the same bits.  The
+def test_heavy_marks_history_independent(fvm):
+    """The heavy faces' cell marks are set by the forming pass and cleared by the apply pass of the same
+    evaluation, so no evaluation sees another's marks: 90 steps (270 evaluations) from one state in a fresh
+    context, against the same 90 steps in a context that first ran 40 steps of a state with its non-UP7 faces
+    elsewhere (where stale marks would sit on other cells): the same bits.  The
     finite-difference run is also held to the oracle's 90 steps at the time-step parity bar; the finite-volume
     one is not, since on this input (a 20x pressure spike) the FP64 oracle's own finite-volume RHS is off
     its long-double evaluation by up to 1.5e-8 per variable, where the GPU's is within 1e-13."""
