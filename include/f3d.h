@@ -384,9 +384,9 @@ int f3d_rhs_f32b(const f3d_params* p, f3d_workspace* ws, double* R, const float*
 /* f3d_rhs_f64 without a workspace and without any allocation.  No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
  * that forbid hipMalloc).  The scratch (interior RHS accumulator, heavy-face runs, their corrections and
- * per-cell marks, and phi when `phi` is NULL) comes from one static 128 MiB device array per GPU, held until
- * the process ends: grids whose scratch exceeds it (about 210 bytes per interior cell: some 0.64M cells,
- * e.g. 86^3) are rejected with F3D_E_DIMS before any launch.
+ * per-cell marks, the sensor classes, and phi when `phi` is NULL) comes from one static 128 MiB device array
+ * per GPU, held until the process ends: grids whose scratch exceeds it (about 200 bytes per interior cell:
+ * some 0.66M cells, e.g. 87^3) are rejected with F3D_E_DIMS before any launch.
  * Every call on one GPU shares that array: calls that overlap on one GPU (two streams, two threads)
  * race.  Finite-difference branch, fused engine with the viscous fluxes inside the update march only
  * (LES models and the F3D_RHS_ENGINE / F3D_VISC_FUSED switches give F3D_E_ARG). */
