@@ -84,10 +84,10 @@ __global__ __launch_bounds__(256) void k_adj_lambda(KP kp, const double* __restr
   for (int n = 0; n < 5; ++n) Lam[o + n * kp.nc] = s * lam[o + n * kp.nc];
 }
 
-// Faces whose sensor maximum over cells c-3..c+2 (c = right cell) is not UP7, the three directions in one
-// launch: a lane takes the x, y and z faces whose right cell is its cell and appends them to its wave's run
-// of each direction.
-__global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __restrict__ phi,
+// Faces whose largest sensor class over cells c-3..c+2 (c = right cell) is not UP7, the three directions in
+// one launch: a lane takes the x, y and z faces whose right cell is its cell and appends them to its wave's
+// run of each direction.
+__global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const unsigned char* __restrict__ cls,
                                                         unsigned long long* __restrict__ list,
                                                         unsigned int* __restrict__ count, HList hl,
                                                         unsigned int* __restrict__ hctr, const unsigned int* pflag,
@@ -108,10 +108,9 @@ __global__ __launch_bounds__(256) void k_adj_heavy_list(KP kp, const double* __r
     unsigned long long entry = 0;
     if (act) {
       const long long o = cidx(kp, i + kp.NG, j + kp.NG, k + kp.NG), st = dstride(kp, D);
-      double m = phi[o - 3 * st];
+      int sch = cls[o - 3 * st];
 #pragma unroll
-      for (int s = -2; s <= 2; ++s) m = jmax(m, phi[o + s * st]);
-      const int sch = scheme_of<double>(kp, m);
+      for (int s = -2; s <= 2; ++s) sch = max(sch, (int)cls[o + s * st]);
       heavy = sch != F3D_SCHEME_UP7;
       entry = ((unsigned long long)o << 4) | ((unsigned long long)sch << 2) | (unsigned long long)D;
     }
@@ -1270,7 +1269,7 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
   const HList hl = f3d::adj_heavy_layout(kp);
   int rc = check_hlist(fv, hl, "rhs vjp");
   if (rc) return rc;
-  fused_sensor<double>(kp, fv, Q, fv.phi, s);  // the same sensor values; marks whether any left the UP7 band
+  fused_sensor<double>(kp, fv, Q, nullptr, s);  // the same sensor classes; marks whether any left the UP7 band
   // a new record stamp per evaluation (8 bits; the records are cleared when it wraps)
   if (++aws->hstamp > 255u) {
     // the record and flag arrays lie back to back
@@ -1284,7 +1283,7 @@ int rhs_vjp_impl(const f3d_params* p, f3d_adj_workspace* aws, const double* lam,
   const unsigned int nbh = hl.items() < 8192u ? (hl.items() ? hl.items() : 1u) : 8192u;
   {
     TimedScope ts("adj_heavy_list", s);
-    hipLaunchKernelGGL(k_adj_heavy_list, adj_list_grid(kp), kB, 0, s, kp, fv.phi, fv.heavy, fv.hcount, hl, aws->hctr,
+    hipLaunchKernelGGL(k_adj_heavy_list, adj_list_grid(kp), kB, 0, s, kp, fv.cls, fv.heavy, fv.hcount, hl, aws->hctr,
                        fv.pgen ? fv.pflag : nullptr, fv.pgen ? *fv.pgen : 0u);
   }
   if (!lam_ready) {
