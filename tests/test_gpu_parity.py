@@ -325,3 +325,25 @@ def test_metrics_fp64(torch_mod, cases):
     for q in range(9):
         assert relerr(host(mg[q]), m[q]) < 1e-12, f"metric {q}"
     assert relerr(host(Jg), J) < 1e-12
+
+
+@pytest.mark.parametrize("bc", [[5, 2, 4, 3, 3, 4], [1, 2, 3, 1, 4, 2], [2, 1, 1, 4, 3, 3]],
+                         ids=["inflow_walls", "dirichlet_mix", "zerograd_walls"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32b"])
+def test_ghost_fill_every_kind(torch_mod, bc, dtype):
+    """The one-launch ghost fill against the oracle's x, y, z fills in turn, on boundary sets that put every
+    kind (inflow, Dirichlet, zero gradient, adiabatic and isothermal walls) on faces of all three directions,
+    so that the edge and corner ghosts compose different kinds: Q bit for bit, U (from Q in double) to
+    rounding."""
+    torch = torch_mod
+    case = ramp_case(20, 12, 10, dtype=dtype, jump=0.5, bc=bc)
+    p, o, ctx = case.params, Oracle(case.params), f3d.Flame3D(case.params)
+    rng = np.random.default_rng(13)
+    Q = (case.Q * (1 + 0.01 * rng.random(case.Q.shape))).astype(case.bulk)
+    U = rng.random(p.cell_shape(5))
+    Qo, Uo = Q.copy(), U.copy()
+    o.fill(Qo, Uo, case.inflow)
+    Qg, Ug = dev(torch, Q), dev(torch, U)
+    ctx.fillGhost(Qg, Ug, None if case.inflow is None else dev(torch, case.inflow))
+    assert np.array_equal(host(Qg), Qo), "ghost fill of Q differs"
+    assert relerr(host(Ug), Uo) < 1e-15
