@@ -71,7 +71,6 @@ struct FArgs {
   const T* Q;
   MetricPtrs<T> m;
   const double* J;
-  const T* phi;
   T* R;
   unsigned long long* heavy;
   unsigned int* hcount;
@@ -1513,7 +1512,7 @@ template int check_hlist<float>(const WsView<float>&, const HList&, const char*)
 template <typename T, int SPLIT>
 static int fused_flux_split(const KP& kp, const WsView<T>& ws, const T* Q, const MetricPtrs<T>& m, const double* J,
                             T* phi, hipStream_t s) {
-  FArgs<T> a{Q, m, J, phi, ws.R, ws.heavy, ws.hcount};
+  FArgs<T> a{Q, m, J, ws.R, ws.heavy, ws.hcount};
   a.cls = ws.cls;
   a.pflag = ws.pgen ? ws.pflag : nullptr;
   a.pgen = ws.pgen ? *ws.pgen : 0u;
@@ -1578,7 +1577,7 @@ int fused_update(const KP& kp, const WsView<T>& ws, int mode, double* U, double*
     set_error("fused stage: the output state must not alias the input state");
     return F3D_E_ARG;
   }
-  FArgs<T> a{Qin, m, J, phi, ws.R, ws.heavy, ws.hcount};
+  FArgs<T> a{Qin, m, J, ws.R, ws.heavy, ws.hcount};
   if (mode != 0 && kp.lts) a.ldt = ws.ldt;
   if (mode != 0) a.Uo = Uo;
   const int rm = xcd_remap();

@@ -36,7 +36,6 @@ template <typename T>
 struct FvmArgs {
   const double* U;
   const T* geo[3];  // face geometry (nx, ny, nz, Area) per direction, face-array layout
-  const T* phi;
   T* R;
   unsigned long long* heavy;
   unsigned int* hcount;
@@ -591,7 +590,7 @@ namespace {
 template <typename T>
 int fvm_flux(const KP& kp, const WsView<T>& ws, const T* Q, const double* U, const MetricPtrs<T>& m,
              const T* const* geo, const T* phi, hipStream_t s) {
-  FvmArgs<T> a{U, {geo[0], geo[1], geo[2]}, phi, ws.R, ws.heavy, ws.hcount};
+  FvmArgs<T> a{U, {geo[0], geo[1], geo[2]}, ws.R, ws.heavy, ws.hcount};
   a.cls = ws.cls;
   a.pflag = ws.pgen ? ws.pflag : nullptr;
   a.pgen = ws.pgen ? *ws.pgen : 0u;
