@@ -17,6 +17,7 @@
 //                 the LES models and the inviscid case
 // Splitting every face as UP7 first and correcting the rare non-UP7 faces from a compacted list
 // keeps the sweep waves free of the characteristic branch's divergence and registers.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -993,18 +994,39 @@ __device__ __forceinline__ void update_cell_ops(const KP& kp, const T* Rt, long 
 template <typename T, int MODE, int VI, bool PF, int W>
 __global__ __launch_bounds__(W * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T> a, double* __restrict__ U,
                                                         double* __restrict__ Un, T* __restrict__ Q,
-                                                        double* __restrict__ Rout, double dt, int KC, int remap) {
+                                                        double* __restrict__ Rout, double dt, int KC, int remap,
+                                                        int nfull) {
   static_assert(W == 64 || (W == 32 && VI % 2 == 0), "tile rows of 64 lanes, or two rows of 32 per wave");
   constexpr int WI = W - 4;              // interior columns of the tile
   __shared__ T sQ[4][VI + 8][W + 4];  // u,v,w,T of plane p: rows y0-4 .. y0+VI+3, columns x0-4 .. x0+W-1
   __shared__ T sFx[4][VI][W];
   __shared__ T sFy[4][VI + 4][WI];
-  const Blk B = xcd_block(remap);
+  // Work of the block (1-D grid, started in workgroup order): the first nfull workgroups (whole rounds of
+  // one per CU) march tile columns 0 .. nfull - 1 bottom to top; the others march one KC-plane segment of
+  // the remaining columns, segment by segment, so that the blocks that run at one time march
+  // neighbouring columns over the same planes.  Each part is renumbered apart so that each XCD takes
+  // neighbouring columns.
+  const unsigned id = blockIdx.x;
+  const int L = !remap ? (int)id
+                       : (id < (unsigned)nfull ? (int)xcd_rank(id, nfull)
+                                               : nfull + (int)xcd_rank(id - nfull, gridDim.x - nfull));
+  const int gx = (kp.Nxp + WI - 1) / WI, ncol = gx * ((kp.Nyp + VI - 1) / VI);
+  int c, kk, nkk;
+  if (L < nfull) {
+    c = L;
+    kk = 0;
+    nkk = kp.Nzp;
+  } else {
+    const int j = L - nfull, rest = ncol - nfull;
+    c = nfull + j % rest;
+    kk = (j / rest) * KC;
+    nkk = min(KC, kp.Nzp - kk);
+  }
   const int lane = threadIdx.x, row = threadIdx.y;
   const int NG = kp.NG;
-  const int x = NG + B.x * WI - 2 + lane, y = NG + B.y * VI - 2 + row;
-  const int k0 = NG + B.z * KC;
-  const int nk = min(KC, NG + kp.Nzp - k0);
+  const int x = NG + (c % gx) * WI - 2 + lane, y = NG + (c / gx) * VI - 2 + row;
+  const int k0 = NG + kk;
+  const int nk = nkk;
   const int xmax = NG + kp.Nxp + 1, ymax = NG + kp.Nyp + 1;
   const bool xin = lane >= 2 && lane < 2 + WI && x < NG + kp.Nxp;
   const bool yin = row >= 2 && row < 2 + VI && y < NG + kp.Nyp;
@@ -1295,6 +1317,7 @@ static int visc_pf(int es) {
 // F3D_VISC_W (32 or 64) and F3D_VISC_KC override; forms that are not built fall back to the default.
 struct ViscTile {
   int VI, KC, W;
+  int nfull;  // tile columns marched whole (the first rounds) before the KC-plane segments of the rest
 };
 
 // (VI, W) forms built: with the prefetch FP64 (4, 64), (8, 32), (12, 32) and FP32-bulk (4|6|7|8, 64),
@@ -1324,18 +1347,26 @@ static ViscTile visc_tile(const KP& kp, int es) {
   const int* kcs = es == 8 ? kc64 : kc32;
   const int nkc = kenv >= 4 ? 1 : (es == 8 ? 6 : 3);
   const long long ncu = cu_count();
-  ViscTile best{forms[0].vi, 16, forms[0].w};
+  // F3D_VISC_FULL = n: n whole columns (0: segments only; unset: whole rounds of one column per CU, as
+  // many as the estimate takes)
+  const int fenv2 = env_int("F3D_VISC_FULL", -1);
+  ViscTile best{forms[0].vi, 16, forms[0].w, 0};
   double bestc = -1;
   for (int a = 0; a < nf; ++a) {
     const int vi = forms[a].vi, w = forms[a].w;
     const long long cols = (long long)((kp.Nxp + w - 5) / (w - 4)) * ((kp.Nyp + vi - 1) / vi);
-    for (int b = 0; b < nkc; ++b) {
-      const int kc = seg_len(kp.Nzp, kenv >= 4 ? kenv : kcs[b]);
-      const long long blocks = cols * ((kp.Nzp + kc - 1) / kc);
-      const double c = (double)((blocks + ncu - 1) / ncu) * ((vi + 4) * w / 64.0) * (kc + 4);
-      if (bestc < 0 || c < bestc * 0.999) {
-        bestc = c;
-        best = ViscTile{vi, kc, w};
+    const long long maxr = fenv2 >= 0 ? 0 : cols / ncu;
+    for (long long r = 0; r <= maxr; ++r) {
+      const long long whole = fenv2 >= 0 ? std::min<long long>(fenv2, cols) : r * ncu, rest = cols - whole;
+      for (int b = 0; b < nkc; ++b) {
+        const int kc = seg_len(kp.Nzp, kenv >= 4 ? kenv : kcs[b]);
+        const long long blocks = rest * ((kp.Nzp + kc - 1) / kc);
+        const double c = ((double)((whole + ncu - 1) / ncu) * (kp.Nzp + 4) +
+                          (double)((blocks + ncu - 1) / ncu) * (kc + 4)) * ((vi + 4) * w / 64.0);
+        if (bestc < 0 || c < bestc * 0.999) {
+          bestc = c;
+          best = ViscTile{vi, kc, w, (int)whole};
+        }
       }
     }
   }
@@ -1344,9 +1375,13 @@ static ViscTile visc_tile(const KP& kp, int es) {
 
 template <typename T, int MODE, int VI, bool PF, int W>
 static void launch_qt(const KP& kp, const FArgs<T>& a, double* U, double* Un, T* Q, double* Rout, double dt, int KC,
-                      int rm, hipStream_t s) {
-  hipLaunchKernelGGL((k_visc_update_qt<T, MODE, VI, PF, W>), visc_grid(kp, KC, VI, W), dim3(W, VI + 4, 1), 0, s, kp,
-                     a, U, Un, Q, Rout, dt, KC, rm);
+                      int nfull, int rm, hipStream_t s) {
+  const dim3 g = visc_grid(kp, KC, VI, W);
+  const int cols = (int)(g.x * g.y);
+  if (nfull > cols) nfull = cols;
+  const unsigned nb = (unsigned)(nfull + (cols - nfull) * (int)g.z);
+  hipLaunchKernelGGL((k_visc_update_qt<T, MODE, VI, PF, W>), dim3(nb, 1, 1), dim3(W, VI + 4, 1), 0, s, kp, a, U, Un,
+                     Q, Rout, dt, KC, rm, nfull);
 }
 
 template <typename T, int MODE>
@@ -1361,7 +1396,9 @@ static void launch_visc_update(const KP& kp, const FArgs<T>& a, double* U, doubl
     vi = pf ? (es == 8 ? 4 : 8) : (vi <= 4 ? 4 : (vi <= 8 || es == 8) ? 8 : 12);
   }
   const int KC = vt.KC;
-#define F3D_QT(V, P, WW) launch_qt<T, MODE, V, P, WW>(kp, a, U, Un, Q, Rout, dt, KC, rm, s)
+  // whole columns counted for the form chosen; a form not built falls back to segments only
+  const int nfull = (vi == vt.VI && w == vt.W) ? vt.nfull : 0;
+#define F3D_QT(V, P, WW) launch_qt<T, MODE, V, P, WW>(kp, a, U, Un, Q, Rout, dt, KC, nfull, rm, s)
   if (pf) {
     if (w == 64 && vi == 4) F3D_QT(4, true, 64);
     if constexpr (es == 8) {

@@ -40,12 +40,15 @@ __device__ __forceinline__ bool heavy_possible(const unsigned int* pflag, unsign
 struct Blk {
   int x, y, z;
 };
+// Rank of workgroup `id` among n when each XCD (id mod 8) takes one contiguous range of ranks.
+__device__ __forceinline__ unsigned xcd_rank(unsigned id, unsigned n) {
+  const unsigned xcd = id & 7u, r = id >> 3, q = n >> 3, rem = n & 7u;
+  return xcd * q + (xcd < rem ? xcd : rem) + r;
+}
 __device__ __forceinline__ Blk xcd_block(int remap) {
   if (!remap) return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
   const unsigned gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
-  const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  const unsigned xcd = id & 7u, r = id >> 3, q = nb >> 3, rem = nb & 7u;
-  const unsigned L = xcd * q + (xcd < rem ? xcd : rem) + r;
+  const unsigned L = xcd_rank(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
   return Blk{(int)(L % gx), (int)((L / gx) % gy), (int)(L / (gx * gy))};
 }
 

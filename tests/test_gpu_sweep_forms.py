@@ -127,3 +127,27 @@ def test_fill_forms_agree(tmp_path, name, dtype, fvm):
         same = np.array_equal(mine[k], ref[k])
         print(f"  {k}: bit-identical {same}")
         assert same, k
+
+
+@pytest.mark.parametrize("fvm", [0, 1], ids=["fdm", "fvm"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32b"])
+@pytest.mark.parametrize("name", ["ramp70x30x9", "tgv16"])
+def test_update_schedules_agree(tmp_path, name, dtype, fvm):
+    """The viscous-update march's block schedules: plane segments of every tile column (what these small
+    grids get by default) against whole columns for the first blocks and segments of the rest
+    (F3D_VISC_FULL=3: three whole columns, then segments of the others) and whole columns only
+    (F3D_VISC_FULL=1000).  Every plane is marched with the same operands in the same order, so the RHS, U,
+    Un and Q agree bit for bit after three steps."""
+    import torch
+    import sweep_child
+    assert torch.cuda.is_available(), "GPU test"
+    case = _case(name, dtype, fvm=f3d.FVM_EIGEN if fvm else 0)
+    src, dst = _inputs(tmp_path, case, dtype)
+    mine = sweep_child.run(f3d, torch, np.load(src))
+    for whole in ("3", "1000"):
+        print(f"{'fvm' if fvm else 'fdm'} {name} {np.dtype(dtype).name}: child with F3D_VISC_FULL={whole}", flush=True)
+        ref = _child(src, dst, "F3D_VISC_FULL", whole)
+        for k in ("R", "U", "Un", "Q"):
+            same = np.array_equal(mine[k], ref[k])
+            print(f"  {k}: bit-identical {same}")
+            assert same, (whole, k)
