@@ -97,7 +97,7 @@ ALG_BYTES_FVM = {"f64": 3 * (40 + 96 + 72 + 8 + 40) + 80.0, "f32b": 3 * (40 + 48
 # by what the counter reads for 1 GiB moved by a plain kernel (--read-probe: an FP64 sum, and --copy-probe:
 # a device copy, both FETCH_SIZE x 2.000; WRITE_SIZE x 1.000); average over the three stage variants (the
 # stage-1 variant's dispatches mix the calls' last steps, which write Un, with the others, which do not).
-MEASURED_TRAFFIC = {("fused_update", "512x192x64", "f64"): (2060.2e6 + 686.1e6 + 2326.5e6 + 561.0e6 + 2325.5e6 + 562.0e6) / 3}
+MEASURED_TRAFFIC = {("fused_update", "512x192x64", "f64"): (1874.4e6 + 690.5e6 + 2166.9e6 + 566.7e6 + 2170.9e6 + 568.6e6) / 3}
 
 
 def build_case(f3d, cases, torch, nx, ny, nz, seed=2, sweep=0.0):
