@@ -1348,16 +1348,16 @@ static ViscTile visc_tile(const KP& kp, int es) {
   const int nkc = kenv >= 4 ? 1 : (es == 8 ? 6 : 3);
   const long long ncu = cu_count();
   // F3D_VISC_FULL = n: n whole columns (0: segments only; unset: whole rounds of one column per CU, as
-  // many as the estimate takes)
+  // many as the estimate takes, the last of them with the columns left when fewer than the CUs)
   const int fenv2 = env_int("F3D_VISC_FULL", -1);
   ViscTile best{forms[0].vi, 16, forms[0].w, 0};
   double bestc = -1;
   for (int a = 0; a < nf; ++a) {
     const int vi = forms[a].vi, w = forms[a].w;
     const long long cols = (long long)((kp.Nxp + w - 5) / (w - 4)) * ((kp.Nyp + vi - 1) / vi);
-    const long long maxr = fenv2 >= 0 ? 0 : cols / ncu;
+    const long long maxr = fenv2 >= 0 ? 0 : (cols + ncu - 1) / ncu;
     for (long long r = 0; r <= maxr; ++r) {
-      const long long whole = fenv2 >= 0 ? std::min<long long>(fenv2, cols) : r * ncu, rest = cols - whole;
+      const long long whole = std::min<long long>(fenv2 >= 0 ? fenv2 : r * ncu, cols), rest = cols - whole;
       for (int b = 0; b < nkc; ++b) {
         const int kc = seg_len(kp.Nzp, kenv >= 4 ? kenv : kcs[b]);
         const long long blocks = rest * ((kp.Nzp + kc - 1) / kc);
