@@ -1001,11 +1001,11 @@ __global__ __launch_bounds__(W * (VI + 4)) void k_visc_update_qt(KP kp, FArgs<T>
   __shared__ T sQ[4][VI + 8][W + 4];  // u,v,w,T of plane p: rows y0-4 .. y0+VI+3, columns x0-4 .. x0+W-1
   __shared__ T sFx[4][VI][W];
   __shared__ T sFy[4][VI + 4][WI];
-  // Work of the block (1-D grid, started in workgroup order): the first nfull workgroups (whole rounds of
-  // one per CU) march tile columns 0 .. nfull - 1 bottom to top; the others march one KC-plane segment of
-  // the remaining columns, segment by segment, so that the blocks that run at one time march
-  // neighbouring columns over the same planes.  Each part is renumbered apart so that each XCD takes
-  // neighbouring columns.
+  // Work of the block (1-D grid, started in workgroup order): the first nfull workgroups (normally whole
+  // rounds of one per CU) march tile columns 0 .. nfull - 1 bottom to top; the others march one KC-plane
+  // segment of the remaining columns, segment by segment, so that the blocks that run at one time march
+  // neighbouring columns over the same planes.  Each part is renumbered apart (a bijection on the part;
+  // it gives each XCD neighbouring columns when the part starts at a multiple of 8 workgroups).
   const unsigned id = blockIdx.x;
   const int L = !remap ? (int)id
                        : (id < (unsigned)nfull ? (int)xcd_rank(id, nfull)
